@@ -1,0 +1,669 @@
+/*
+ * nsx_case.c -- host-side set-up for the prismatic sigma-grid channel.
+ *
+ * Builds everything the per-step hot path consumes, in the reference's own
+ * array layout (column-major, 1-based connectivity, ghost cells appended after
+ * N_CELL0):
+ *   - a seeded synthetic mesh: nx*ny quads on [0,Lx]x[0,Ly], each split into
+ *     two counter-clockwise triangles along a diagonal drawn per quad from the
+ *     seed, optional jitter of interior vertices and an (inert) bed;
+ *   - tanh-clustered sigma levels;
+ *   - the seeded initial condition (Poiseuille profile + uniform noise);
+ *   - the geometry of src/Main/calcul_geometry.F90 (centroids, areas, normals,
+ *     periodic ghost cells, IDW weights, Kim-Choi / LSM coefficients);
+ *   - the boundary flags of src/BoundaryConditions/update_bc.F90 (adjust_tagv,
+ *     adjust_bc) for the doubly periodic box with sigma walls;
+ *   - the stored PPE matrix of src/AdvectionDiffusion/PDE.F90 (PDE_center).
+ *
+ * The geometry routines follow the reference loop by loop and operation by
+ * operation (no FMA contraction: build with -ffp-contract=off) so that every
+ * array is bit-identical to what the Fortran set-up produces for the same
+ * vertices; tests check that against the compiled reference.
+ *
+ * Only the doubly periodic horizontal box (XPB=YPB=1, the reference's channel
+ * test) is generated; a ghost cell without a periodic partner is an error.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "nsx_case.h"
+
+#define IX2(i, j, n) ((size_t)(i) + (size_t)(j) * (size_t)(n)) /* 0-based (i,j) of an (n,*) array */
+
+/* ---------------------------------------------------------------- RNG ---- */
+static uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+double nsx_rng_u01(uint64_t seed, uint32_t stream, uint64_t index) {
+  uint64_t h = mix64(mix64(seed ^ ((uint64_t)stream << 48)) ^ index);
+  return (double)(h >> 11) * 0x1.0p-53;
+}
+
+/* ------------------------------------------------------------ helpers ---- */
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) {
+    fprintf(stderr, "nsx_case: out of memory (%zu x %zu)\n", n, sz);
+    abort();
+  }
+  return p;
+}
+
+/* Fortran single-precision literals used in comparisons (1.0E-7 etc.). */
+#define F32_1EM7 ((double)1.0e-7f)
+
+typedef struct {
+  long key;
+  int tri, j;
+} edge_rec;
+
+static int edge_cmp(const void *a, const void *b) {
+  long ka = ((const edge_rec *)a)->key, kb = ((const edge_rec *)b)->key;
+  return (ka > kb) - (ka < kb);
+}
+
+/* ------------------------------------------------------------- mesh ------ */
+static int build_mesh(nsx_case *c) {
+  const int nx = c->nx, ny = c->ny;
+  const int N0 = c->N_CELL0, NV = c->N_VERT;
+  const double dx = c->Lx / nx, dy = c->Ly / ny;
+
+  for (int jy = 0; jy <= ny; ++jy)
+    for (int ix = 0; ix <= nx; ++ix) {
+      int v = jy * (nx + 1) + ix;
+      double x = (ix == 0) ? 0.0 : (ix == nx ? c->Lx : c->Lx * (double)ix / (double)nx);
+      double y = (jy == 0) ? 0.0 : (jy == ny ? c->Ly : c->Ly * (double)jy / (double)ny);
+      int bnd = (ix == 0 || ix == nx || jy == 0 || jy == ny);
+      if (!bnd && c->jitter > 0.0) {
+        x += (2.0 * nsx_rng_u01(c->seed, NSX_STREAM_JITTER_X, (uint64_t)v) - 1.0) * c->jitter * dx;
+        y += (2.0 * nsx_rng_u01(c->seed, NSX_STREAM_JITTER_Y, (uint64_t)v) - 1.0) * c->jitter * dy;
+      }
+      c->xv[v] = x;
+      c->yv[v] = y;
+      c->nbev[v] = bnd ? 1 : 0;
+      c->zbv[v] = c->bed ? 0.05 * c->H * sin(2.0 * M_PI * x / c->Lx) * cos(2.0 * M_PI * y / c->Ly) : 0.0;
+    }
+
+  for (int jy = 0; jy < ny; ++jy)
+    for (int ix = 0; ix < nx; ++ix) {
+      int q = jy * nx + ix;
+      int v00 = jy * (nx + 1) + ix, v10 = v00 + 1, v01 = v00 + (nx + 1), v11 = v01 + 1;
+      int d = nsx_rng_u01(c->seed, NSX_STREAM_DIAG, (uint64_t)q) < 0.5 ? 0 : 1;
+      int t0 = 2 * q, t1 = 2 * q + 1;
+      int a[3], b[3];
+      if (d == 0) { /* diagonal v00-v11 */
+        a[0] = v00; a[1] = v10; a[2] = v11;
+        b[0] = v00; b[1] = v11; b[2] = v01;
+      } else {      /* diagonal v10-v01 */
+        a[0] = v00; a[1] = v10; a[2] = v01;
+        b[0] = v10; b[1] = v11; b[2] = v01;
+      }
+      for (int j = 0; j < 3; ++j) {
+        c->No_vp[IX2(t0, j, N0)] = a[j] + 1;
+        c->No_vp[IX2(t1, j, N0)] = b[j] + 1;
+      }
+    }
+
+  /* neighbour across edge j = (No_vp(i,j), No_vp(i,j+1)) */
+  edge_rec *e = xcalloc((size_t)3 * N0, sizeof(edge_rec));
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int va = c->No_vp[IX2(i, j, N0)] - 1, vb = c->No_vp[IX2(i, (j + 1) % 3, N0)] - 1;
+      int lo = va < vb ? va : vb, hi = va < vb ? vb : va;
+      e[3 * i + j].key = (long)lo * NV + hi;
+      e[3 * i + j].tri = i;
+      e[3 * i + j].j = j;
+    }
+  qsort(e, (size_t)3 * N0, sizeof(edge_rec), edge_cmp);
+  for (int i = 0; i < N0; ++i) c->nbe[i] = 0;
+  for (size_t n = 0; n < (size_t)3 * N0;) {
+    size_t m = n + 1;
+    while (m < (size_t)3 * N0 && e[m].key == e[n].key) ++m;
+    if (m - n == 2) {
+      c->No_cp[IX2(e[n].tri, e[n].j, c->N_CELL)] = e[n + 1].tri + 1;
+      c->No_cp[IX2(e[n + 1].tri, e[n + 1].j, c->N_CELL)] = e[n].tri + 1;
+    } else if (m - n == 1) {
+      c->No_cp[IX2(e[n].tri, e[n].j, c->N_CELL)] = 0;
+      c->nbe[e[n].tri] = 1;
+    } else {
+      free(e);
+      snprintf(c->err, sizeof c->err, "non-manifold edge in generated mesh");
+      return -1;
+    }
+    n = m;
+  }
+  free(e);
+  return 0;
+}
+
+/* --------------------------------------------------------- sigma levels -- */
+static void build_sigma(nsx_case *c) {
+  const int NZ = c->NZ, L = c->NN;
+  double *sigv = c->sigv, *dsigv = c->dsigv, *sig = c->sig, *dsig = c->dsig;
+  /* vertex (interface) levels z_k = (H/2)(1 + tanh(g(2k/L-1))/tanh g), k=0..L */
+  for (int k = 0; k <= L; ++k) {
+    double z;
+    if (k == 0) z = 0.0;
+    else if (k == L) z = c->H;
+    else if (c->stretch > 0.0)
+      z = 0.5 * c->H * (1.0 + tanh(c->stretch * (2.0 * k / (double)L - 1.0)) / tanh(c->stretch));
+    else
+      z = c->H * (double)k / (double)L;
+    sigv[k] = z; /* sigv(k+1) */
+  }
+  /* thicknesses and cell centres with the recurrence of calcul_geometry.F90:263-283 */
+  for (int k = 0; k < NZ - 2; ++k) dsigv[k] = sigv[k + 1] - sigv[k];
+  dsigv[NZ - 2] = dsigv[NZ - 3]; /* top ghost cell mirrors the top cell */
+  dsig[0] = dsigv[0];
+  sig[0] = sigv[0] - 0.5 * dsigv[0];
+  for (int k = 1; k < NZ - 1; ++k) {
+    dsig[k] = 0.5 * (dsigv[k] + dsigv[k - 1]);
+    sig[k] = sig[k - 1] + dsig[k - 1];
+  }
+  dsig[NZ - 1] = dsigv[NZ - 2];
+  sig[NZ - 1] = sig[NZ - 2] + dsig[NZ - 2];
+}
+
+/* ------------------------------------------------------------ geometry --- */
+/* calcul_geometry.F90:290-367 (per-cell), 402-578 (ghost cells, sequential
+ * periodic branch), 851-1040 (tags, vertex pairs), 1055-1076 (IDW weights),
+ * 1343-1517 (Kim-Choi / LSM coefficients). */
+static int build_geometry(nsx_case *c) {
+  const int N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL;
+  int *No_cp = c->No_cp;
+  const int *No_vp = c->No_vp, *nbe = c->nbe, *nbev = c->nbev;
+  const double *xv = c->xv, *yv = c->yv;
+  double *xc = c->xc, *yc = c->yc, *area = c->areaCell;
+  double *dxVV = c->dxVV, *dyVV = c->dyVV, *dlVV = c->dlVV, *xnn = c->xnn, *ynn = c->ynn;
+#define VP(i, j) (No_vp[IX2(i, j, N0)] - 1)
+#define CP(i, j) No_cp[IX2(i, j, NC)]
+
+  double Volsum = 0.;
+  for (int i = 0; i < N0; ++i) {
+    int jv1 = VP(i, 0), jv2 = VP(i, 1), jv3 = VP(i, 2);
+    xc[i] = (xv[jv1] + xv[jv2] + xv[jv3]) / 3.0;
+    yc[i] = (yv[jv1] + yv[jv2] + yv[jv3]) / 3.0;
+    area[i] = 0.5 * fabs(xv[jv1] * (yv[jv2] - yv[jv3]) + xv[jv2] * (yv[jv3] - yv[jv1]) +
+                         xv[jv3] * (yv[jv1] - yv[jv2]));
+    dxVV[IX2(i, 0, N0)] = xv[jv2] - xv[jv1];
+    dxVV[IX2(i, 1, N0)] = xv[jv3] - xv[jv2];
+    dxVV[IX2(i, 2, N0)] = xv[jv1] - xv[jv3];
+    dyVV[IX2(i, 0, N0)] = yv[jv2] - yv[jv1];
+    dyVV[IX2(i, 1, N0)] = yv[jv3] - yv[jv2];
+    dyVV[IX2(i, 2, N0)] = yv[jv1] - yv[jv3];
+    for (int j = 0; j < 3; ++j) {
+      double a = dxVV[IX2(i, j, N0)], b = dyVV[IX2(i, j, N0)];
+      dlVV[IX2(i, j, N0)] = sqrt(a * a + b * b);
+    }
+    for (int j = 0; j < 3; ++j) xnn[IX2(i, j, N0)] = dyVV[IX2(i, j, N0)] / dlVV[IX2(i, j, N0)];
+    for (int j = 0; j < 3; ++j) ynn[IX2(i, j, N0)] = -dxVV[IX2(i, j, N0)] / dlVV[IX2(i, j, N0)];
+    Volsum = Volsum + area[i];
+  }
+  c->TotalVolume = Volsum;
+
+  /* ghost cells (sequential build, KeyTESTpBC) */
+  const double XDIni = 0.0, XDFin = c->Lx, YDIni = 0.0, YDFin = c->Ly;
+  int ii = N0;
+  for (int i = 0; i < N0; ++i) {
+    if (nbe[i] <= 0) continue;
+    for (int j = 0; j < 3; ++j) {
+      int nc = CP(i, j);
+      if (!(nc < 1 || nc > N0)) continue;
+      if (ii >= NC) {
+        snprintf(c->err, sizeof c->err, "more ghost cells than N_CELL allows");
+        return -1;
+      }
+      int g = ii; /* 0-based ghost row */
+      ii = ii + 1;
+      CP(i, j) = ii;
+      CP(g, 0) = i + 1;
+      CP(g, 1) = j + 1;
+      area[g] = area[i];
+      int jj = (j + 1) % 3;
+      int jv1 = VP(i, j), jv2 = VP(i, jj);
+      double xee, yee;
+      if (fabs(dyVV[IX2(i, j, N0)]) < F32_1EM7) {
+        yee = yv[jv2];
+        xee = xc[i];
+      } else {
+        double dxody = dxVV[IX2(i, j, N0)] / dyVV[IX2(i, j, N0)];
+        double dxody2 = dxody * dxody;
+        yee = (yc[i] - (xv[jv2] - xc[i]) * dxody + yv[jv2] * dxody2) / (1.0 + dxody2);
+        xee = xv[jv2] + (yee - yv[jv2]) * dxody;
+      }
+      xc[g] = 2.0 * xee - xc[i];
+      yc[g] = 2.0 * yee - yc[i];
+      CP(g, 2) = -1;
+
+      double dxl = XDFin - XDIni, dyl = YDFin - YDIni;
+      if (xc[g] > XDFin) {
+        dxl = -1.0 * dxl;
+        dyl = 0.;
+      } else if (xc[g] < XDIni) {
+        dyl = 0.;
+      }
+      if (yc[g] > YDFin) {
+        dxl = 0.;
+        dyl = -1.0 * dyl;
+      } else if (yc[g] < YDIni) {
+        dxl = 0.;
+      }
+      if (dxl * dyl > F32_1EM7) {
+        snprintf(c->err, sizeof c->err, "ghost cell of cell %d edge %d has no periodic partner", i + 1, j + 1);
+        return -1;
+      }
+      int jv1p = -1, jv2p = -1, ncount = 0;
+      xee = xv[jv1] + dxl;
+      yee = yv[jv1] + dyl;
+      for (int nv = 0; nv < NV; ++nv)
+        if (nbev[nv] != 0 && fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
+          jv1p = nv;
+          ++ncount;
+        }
+      if (ncount != 1) goto pairfail;
+      ncount = 0;
+      xee = xv[jv2] + dxl;
+      yee = yv[jv2] + dyl;
+      for (int nv = 0; nv < NV; ++nv)
+        if (nbev[nv] != 0 && fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
+          jv2p = nv;
+          ++ncount;
+        }
+      if (ncount != 1) goto pairfail;
+      {
+        /* partner cell: the boundary triangle holding edge (jv1p, jv2p); the
+         * reference tests products of 1-based vertex numbers */
+        long p1 = (long)(jv1p + 1) * (jv2p + 1);
+        int cc = -1;
+        ncount = 0;
+        for (int i0 = 0; i0 < N0; ++i0) {
+          if (nbe[i0] == 0) continue;
+          long a10 = VP(i0, 0) + 1, a20 = VP(i0, 1) + 1, a30 = VP(i0, 2) + 1;
+          if (a10 * a20 == p1) {
+            if (a10 == jv1p + 1 || a10 == jv2p + 1) { cc = (int)a30 - 1; CP(g, 2) = i0 + 1; ++ncount; }
+          }
+          if (a30 * a20 == p1) {
+            if (a30 == jv1p + 1 || a30 == jv2p + 1) { cc = (int)a10 - 1; CP(g, 2) = i0 + 1; ++ncount; }
+          }
+          if (a10 * a30 == p1) {
+            if (a10 == jv1p + 1 || a10 == jv2p + 1) { cc = (int)a20 - 1; CP(g, 2) = i0 + 1; ++ncount; }
+          }
+        }
+        if (ncount != 1) goto pairfail;
+        xee = xv[cc] - dxl;
+        yee = yv[cc] - dyl;
+        xc[g] = (xv[jv1] + xv[jv2] + xee) / 3.0;
+        yc[g] = (yv[jv1] + yv[jv2] + yee) / 3.0;
+      }
+      continue;
+    pairfail:
+      snprintf(c->err, sizeof c->err, "periodic matching failed for cell %d edge %d", i + 1, j + 1);
+      return -1;
+    }
+  }
+  c->N_CELLexact = ii;
+  c->N_CELLghost = ii - N0;
+
+  /* vertex tags (calcul_geometry.F90:895-918) and periodic pairs (:920-1025) */
+  int *tagXV = c->tagXV, *tagYV = c->tagYV;
+  for (int nv = 0; nv < NV; ++nv) {
+    tagXV[nv] = 0;
+    tagYV[nv] = 0;
+    if (nbev[nv] != 0) {
+      if (fabs(xv[nv] - XDIni) < 1.0e-7) tagXV[nv] = 1;
+      if (fabs(xv[nv] - XDFin) < 1.0e-7) tagXV[nv] = 2;
+      if (fabs(yv[nv] - YDIni) < 1.0e-7) tagYV[nv] = 1;
+      if (fabs(yv[nv] - YDFin) < 1.0e-7) tagYV[nv] = 2;
+    }
+  }
+  for (int nv = 0; nv < NV; ++nv) {
+    c->vertex_pair[nv] = -1;
+    c->vertex_edge[nv] = -1;
+  }
+  for (int nv = 0; nv < NV; ++nv) {
+    double xee = xv[nv], yee = yv[nv];
+    int ncount = (tagXV[nv] * tagYV[nv] > 0) ? 2 : ((tagXV[nv] > 0 || tagYV[nv] > 0) ? 1 : 0);
+    if (ncount == 1) {
+      int vcount = 0;
+      for (int i = 0; i < NV; ++i) {
+        if (nbev[i] == 0) continue;
+        if (fabs(xee - xv[i]) < 1.0e-7 && fabs(yee - yv[i]) > (YDFin - YDIni - 1.0e-7)) {
+          ++vcount;
+          c->vertex_pair[nv] = i + 1;
+        }
+        if (fabs(yee - yv[i]) < 1.0e-7 && fabs(xee - xv[i]) > (XDFin - XDIni - 1.0e-7)) {
+          ++vcount;
+          c->vertex_pair[nv] = i + 1;
+        }
+      }
+      if (vcount == 0) c->vertex_pair[nv] = -1;
+      else if (vcount != 1) {
+        snprintf(c->err, sizeof c->err, "vertex %d: %d periodic partners", nv + 1, vcount);
+        return -1;
+      }
+    }
+    if (ncount == 2) c->vertex_edge[nv] = 1; /* XPB*YPB = 1 */
+  }
+
+  /* inverse-distance weights, interior cells only (calcul_geometry.F90:1055-1076) */
+  for (int nv = 0; nv < NV; ++nv) {
+    c->Dimsurrounding[nv] = 0;
+    c->dlVsum[nv] = 0.0;
+  }
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int nv = VP(i, j);
+      int j0 = c->Dimsurrounding[nv];
+      if (j0 >= NSX_MAX_SURROUND) {
+        snprintf(c->err, sizeof c->err, "vertex %d touches more than %d cells", nv + 1, NSX_MAX_SURROUND);
+        return -1;
+      }
+      double dxCV = xc[i] - xv[nv], dyCV = yc[i] - yv[nv];
+      c->surrounding[IX2(nv, j0, NV)] = i + 1;
+      c->weight[IX2(nv, j0, NV)] = 1.0 / sqrt(dxCV * dxCV + dyCV * dyCV);
+      c->dlVsum[nv] = c->dlVsum[nv] + c->weight[IX2(nv, j0, NV)];
+      c->Dimsurrounding[nv] = j0 + 1;
+    }
+
+  /* edge points and Kim-Choi offsets, EdgePointOption = 1 (:1346-1387) */
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int jj = (j + 1) % 3;
+      int jv1 = VP(i, j), jv2 = VP(i, jj);
+      size_t f = IX2(i, j, N0);
+      c->xme[f] = 0.5 * (xv[jv2] + xv[jv1]);
+      c->yme[f] = 0.5 * (yv[jv2] + yv[jv1]);
+      int jc = CP(i, j) - 1;
+      double xm1 = xc[i], xm2 = xc[jc], xm3 = xv[jv1], xm4 = xv[jv2];
+      double ym1 = yc[i], ym2 = yc[jc], ym3 = yv[jv1], ym4 = yv[jv2];
+      double den = ((xm1 - xm2) * (ym3 - ym4) - (ym1 - ym2) * (xm3 - xm4));
+      c->xmo[f] = ((xm1 * ym2 - ym1 * xm2) * (xm3 - xm4) - (xm1 - xm2) * (xm3 * ym4 - ym3 * xm4)) / den;
+      c->ymo[f] = ((xm1 * ym2 - ym1 * xm2) * (ym3 - ym4) - (ym1 - ym2) * (xm3 * ym4 - ym3 * xm4)) / den;
+      xm1 = c->xme[f] - c->xmo[f];
+      ym1 = c->yme[f] - c->ymo[f];
+      c->doe[f] = sqrt(xm1 * xm1 + ym1 * ym1);
+      xm2 = xm1 * dxVV[f] + ym1 * dyVV[f];
+      if (xm2 < 0.0) c->doe[f] = -c->doe[f];
+    }
+
+  /* LSM sums and Kim-Choi decomposition (:1460-1517, KeyKimChoi) */
+  for (int i = 0; i < N0; ++i) {
+    double sumX = 0.0, sumY = 0.0, sumXY = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      size_t f = IX2(i, j, N0);
+      int jc = CP(i, j) - 1;
+      c->dxCC[f] = xc[jc] - xc[i];
+      c->dyCC[f] = yc[jc] - yc[i];
+      c->dlCC[f] = sqrt(c->dxCC[f] * c->dxCC[f] + c->dyCC[f] * c->dyCC[f]);
+      sumX = sumX + c->dxCC[f] * c->dxCC[f];
+      sumY = sumY + c->dyCC[f] * c->dyCC[f];
+      sumXY = sumXY + c->dxCC[f] * c->dyCC[f];
+      double xee = c->dxCC[f] / c->dlCC[f];
+      double yee = c->dyCC[f] / c->dlCC[f];
+      double nfn1 = xee * xnn[f] + yee * ynn[f];
+      xee = xee / nfn1;
+      yee = yee / nfn1;
+      c->dle1[f] = 1.0 / nfn1;
+      c->dle3[f] = 0.;
+      xee = xnn[f] - xee;
+      yee = ynn[f] - yee;
+      nfn1 = xee * dxVV[f] + yee * dyVV[f];
+      if (nfn1 < 0.) c->dle2[f] = -1.0 * sqrt(xee * xee + yee * yee);
+      else c->dle2[f] = sqrt(xee * xee + yee * yee);
+    }
+    c->sum_xc2[i] = sumX;
+    c->sum_yc2[i] = sumY;
+    c->sum_xcyc[i] = sumXY;
+  }
+#undef VP
+#undef CP
+  return 0;
+}
+
+/* -------------------------------------------------------- BC flags ------- */
+/* adjust_tagv / adjust_bc (update_bc.F90:4-848) for XPB=YPB=1, TopBC/BotBC walls */
+static int build_bc(nsx_case *c) {
+  const int N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL;
+  for (int i = 0; i < NC; ++i) {
+    c->bcuc[i] = c->bcvc[i] = c->bcwc[i] = c->bcpc[i] = -1;
+    c->tagsp[i] = 0;
+  }
+  for (int i = 0; i < N0; ++i) {
+    if (c->nbe[i] == 0) continue;
+    for (int j = 0; j < 3; ++j) {
+      int nc = c->No_cp[IX2(i, j, NC)];
+      if (!(nc < 1 || nc > N0)) continue;
+      int ii = c->No_cp[IX2(nc - 1, 2, NC)];
+      int fB;
+      if (ii > 0) fB = 3;
+      else if (ii == 0) fB = -1;
+      else {
+        snprintf(c->err, sizeof c->err, "wall ghost cells are not generated by this set-up");
+        return -1;
+      }
+      c->bcuc[nc - 1] = c->bcvc[nc - 1] = c->bcwc[nc - 1] = c->bcpc[nc - 1] = fB;
+    }
+  }
+  /* adjust_tagv: corners keep the y tag */
+  for (int nv = 0; nv < NV; ++nv)
+    if (c->nbev[nv] != 0 && c->tagXV[nv] * c->tagYV[nv] != 0) c->tagXV[nv] = 0;
+  for (int nv = 0; nv < NV; ++nv) {
+    c->bcuv[nv] = c->bcvv[nv] = c->bcwv[nv] = c->bcpv[nv] = -1;
+    if (c->nbev[nv] == 0) continue;
+    int flag = 0;
+    if (c->tagXV[nv] > 0) ++flag;
+    if (c->tagYV[nv] > 0) ++flag;
+    if (flag != 1) {
+      snprintf(c->err, sizeof c->err, "boundary vertex %d has %d boundary tags", nv + 1, flag);
+      return -1;
+    }
+    c->bcuv[nv] = c->bcvv[nv] = c->bcwv[nv] = c->bcpv[nv] = 3;
+    if (c->vertex_edge[nv] > 0) c->bcpv[nv] = 3;
+    if (c->vertex_pair[nv] > 0) c->bcpv[nv] = 3;
+  }
+  c->bctop = 1;
+  c->bcbot = 1;
+  return 0;
+}
+
+/* ---------------------------------------------------------- PPE matrix --- */
+/* PDE_center (PDE.F90:69-121), KeyKimChoi, ZPB = 0 */
+static void build_ppe(nsx_case *c) {
+  const int N0 = c->N_CELL0, NZ = c->NZ, NC = c->N_CELL;
+  for (size_t n = 0; n < (size_t)N0 * NZ; ++n)
+    c->AmC0[n] = c->AmC1[n] = c->AmC2[n] = c->AmC3[n] = c->AmCT[n] = c->AmCB[n] = 0.0;
+  for (int k = 1; k < NZ - 1; ++k)
+    for (int i = 0; i < N0; ++i) {
+      double AA[3];
+      for (int j = 0; j < 3; ++j) {
+        size_t f = IX2(i, j, N0);
+        int jc = c->No_cp[IX2(i, j, NC)] - 1;
+        double co1 = c->dle1[f] / c->dlCC[f];
+        if (c->bcpc[jc] == 1) AA[j] = 0.0;
+        else AA[j] = co1 * c->dlVV[f] * c->dsigv[k - 1];
+      }
+      double AAB = c->areaCell[i] / c->dsig[k - 1];
+      double AAT = c->areaCell[i] / c->dsig[k];
+      if (k == 1 && c->bcbot == 1) AAB = 0.0;
+      if (k == NZ - 2 && c->bctop == 1) AAT = 0.0;
+      double AA0 = -1.0 * (AAT + AAB + AA[0] + AA[1] + AA[2]);
+      size_t n = IX2(i, k, N0);
+      c->AmC0[n] = AA0;
+      c->AmC1[n] = AA[0];
+      c->AmC2[n] = AA[1];
+      c->AmC3[n] = AA[2];
+      c->AmCT[n] = AAT;
+      c->AmCB[n] = AAB;
+    }
+}
+
+/* ------------------------------------------------------ initial field ---- */
+/* u = 1.5 Ub (1 - (z/h-1)^2) + noise, v = w = noise, noise ~ U(-a,a) Ub, p = 0,
+ * at the centroids of interior cells and layers k = 2..NZ-1; ghost entries 0. */
+static void build_ic(nsx_case *c) {
+  const int NC = c->N_CELL, NZ = c->NZ, N0 = c->N_CELL0;
+  const double h = 0.5 * c->H, Ub = c->Ubulk, a = c->noise;
+  memset(c->u0, 0, sizeof(double) * NC * NZ);
+  memset(c->v0, 0, sizeof(double) * NC * NZ);
+  memset(c->w0, 0, sizeof(double) * NC * NZ);
+  memset(c->p0, 0, sizeof(double) * NC * NZ);
+  for (int k = 1; k < NZ - 1; ++k)
+    for (int i = 0; i < N0; ++i) {
+      uint64_t idx = (uint64_t)k * (uint64_t)N0 + (uint64_t)i;
+      double zeta = c->sig[k] / h - 1.0;
+      double up = 1.5 * Ub * (1.0 - zeta * zeta);
+      size_t n = IX2(i, k, NC);
+      c->u0[n] = up + (2.0 * nsx_rng_u01(c->seed, NSX_STREAM_NOISE_U, idx) - 1.0) * a * Ub;
+      c->v0[n] = (2.0 * nsx_rng_u01(c->seed, NSX_STREAM_NOISE_V, idx) - 1.0) * a * Ub;
+      c->w0[n] = (2.0 * nsx_rng_u01(c->seed, NSX_STREAM_NOISE_W, idx) - 1.0) * a * Ub;
+    }
+}
+
+/* -------------------------------------------------------------- API ------ */
+void nsx_case_free(nsx_case *c) {
+  if (!c) return;
+  for (int n = 0; n < c->n_arrays; ++n) free(c->arrays[n].ptr);
+  free(c);
+}
+
+static void *add_array(nsx_case *c, const char *name, int type, size_t n) {
+  size_t sz = type == NSX_T_F64 ? sizeof(double) : sizeof(int);
+  void *p = xcalloc(n, sz);
+  nsx_case_array *a = &c->arrays[c->n_arrays++];
+  snprintf(a->name, sizeof a->name, "%s", name);
+  a->type = type;
+  a->n = (int64_t)n;
+  a->ptr = p;
+  return p;
+}
+
+nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
+                          double jitter, int bed, double Ubulk, double noise, uint64_t seed) {
+  nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  c->nx = nx; c->ny = ny; c->NN = nn; c->NZ = nn + 2;
+  c->Lx = Lx; c->Ly = Ly; c->H = H; c->stretch = stretch;
+  c->jitter = jitter; c->bed = bed; c->Ubulk = Ubulk; c->noise = noise; c->seed = seed;
+  c->N_CELL0 = 2 * nx * ny;
+  c->N_VERT = (nx + 1) * (ny + 1);
+  c->N_CELL = c->N_CELL0 + 100 * nn; /* N_CELLghostMAX = 100*NN (common.mpf:113) */
+  if (nx < 2 || ny < 2 || nn < 2 || 2 * (nx + ny) > 100 * nn) {
+    snprintf(c->err, sizeof c->err, "bad sizes nx=%d ny=%d nn=%d", nx, ny, nn);
+    c->status = -1;
+    return c;
+  }
+  const size_t N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL, NZ = c->NZ;
+  c->No_vp = add_array(c, "No_vp", NSX_T_I32, N0 * 3);
+  c->No_cp = add_array(c, "No_cp", NSX_T_I32, NC * 3);
+  c->nbe = add_array(c, "nbe", NSX_T_I32, N0);
+  c->nbev = add_array(c, "nbev", NSX_T_I32, NV);
+  c->xv = add_array(c, "xv", NSX_T_F64, NV);
+  c->yv = add_array(c, "yv", NSX_T_F64, NV);
+  c->zbv = add_array(c, "zbv", NSX_T_F64, NV);
+  c->sig = add_array(c, "sig", NSX_T_F64, NZ);
+  c->dsig = add_array(c, "dsig", NSX_T_F64, NZ);
+  c->sigv = add_array(c, "sigv", NSX_T_F64, NZ - 1);
+  c->dsigv = add_array(c, "dsigv", NSX_T_F64, NZ - 1);
+  c->xc = add_array(c, "xc", NSX_T_F64, NC);
+  c->yc = add_array(c, "yc", NSX_T_F64, NC);
+  c->areaCell = add_array(c, "areaCell", NSX_T_F64, NC);
+  c->dxVV = add_array(c, "dxVV", NSX_T_F64, N0 * 3);
+  c->dyVV = add_array(c, "dyVV", NSX_T_F64, N0 * 3);
+  c->dlVV = add_array(c, "dlVV", NSX_T_F64, N0 * 3);
+  c->xnn = add_array(c, "xnn", NSX_T_F64, N0 * 3);
+  c->ynn = add_array(c, "ynn", NSX_T_F64, N0 * 3);
+  c->dxCC = add_array(c, "dxCC", NSX_T_F64, N0 * 3);
+  c->dyCC = add_array(c, "dyCC", NSX_T_F64, N0 * 3);
+  c->dlCC = add_array(c, "dlCC", NSX_T_F64, N0 * 3);
+  c->xme = add_array(c, "xme", NSX_T_F64, N0 * 3);
+  c->yme = add_array(c, "yme", NSX_T_F64, N0 * 3);
+  c->xmo = add_array(c, "xmo", NSX_T_F64, N0 * 3);
+  c->ymo = add_array(c, "ymo", NSX_T_F64, N0 * 3);
+  c->doe = add_array(c, "doe", NSX_T_F64, N0 * 3);
+  c->dle1 = add_array(c, "dle1", NSX_T_F64, N0 * 3);
+  c->dle2 = add_array(c, "dle2", NSX_T_F64, N0 * 3);
+  c->dle3 = add_array(c, "dle3", NSX_T_F64, N0 * 3);
+  c->sum_xc2 = add_array(c, "sum_xc2", NSX_T_F64, N0);
+  c->sum_yc2 = add_array(c, "sum_yc2", NSX_T_F64, N0);
+  c->sum_xcyc = add_array(c, "sum_xcyc", NSX_T_F64, N0);
+  c->surrounding = add_array(c, "surrounding", NSX_T_I32, NV * NSX_MAX_SURROUND);
+  c->weight = add_array(c, "weight", NSX_T_F64, NV * NSX_MAX_SURROUND);
+  c->Dimsurrounding = add_array(c, "Dimsurrounding", NSX_T_I32, NV);
+  c->dlVsum = add_array(c, "dlVsum", NSX_T_F64, NV);
+  c->vertex_pair = add_array(c, "vertex_pair", NSX_T_I32, NV);
+  c->vertex_edge = add_array(c, "vertex_edge", NSX_T_I32, NV);
+  c->tagXV = add_array(c, "tagXV", NSX_T_I32, NV);
+  c->tagYV = add_array(c, "tagYV", NSX_T_I32, NV);
+  c->tagsp = add_array(c, "tagsp", NSX_T_I32, NC);
+  c->bcuc = add_array(c, "bcuc", NSX_T_I32, NC);
+  c->bcvc = add_array(c, "bcvc", NSX_T_I32, NC);
+  c->bcwc = add_array(c, "bcwc", NSX_T_I32, NC);
+  c->bcpc = add_array(c, "bcpc", NSX_T_I32, NC);
+  c->bcuv = add_array(c, "bcuv", NSX_T_I32, NV);
+  c->bcvv = add_array(c, "bcvv", NSX_T_I32, NV);
+  c->bcwv = add_array(c, "bcwv", NSX_T_I32, NV);
+  c->bcpv = add_array(c, "bcpv", NSX_T_I32, NV);
+  c->AmC0 = add_array(c, "AmC0", NSX_T_F64, N0 * NZ);
+  c->AmC1 = add_array(c, "AmC1", NSX_T_F64, N0 * NZ);
+  c->AmC2 = add_array(c, "AmC2", NSX_T_F64, N0 * NZ);
+  c->AmC3 = add_array(c, "AmC3", NSX_T_F64, N0 * NZ);
+  c->AmCT = add_array(c, "AmCT", NSX_T_F64, N0 * NZ);
+  c->AmCB = add_array(c, "AmCB", NSX_T_F64, N0 * NZ);
+  c->walld = add_array(c, "walld", NSX_T_F64, NZ);
+  c->u0 = add_array(c, "u0", NSX_T_F64, NC * NZ);
+  c->v0 = add_array(c, "v0", NSX_T_F64, NC * NZ);
+  c->w0 = add_array(c, "w0", NSX_T_F64, NC * NZ);
+  c->p0 = add_array(c, "p0", NSX_T_F64, NC * NZ);
+
+  if (build_mesh(c) || (build_sigma(c), 0) || build_geometry(c) || build_bc(c)) {
+    c->status = -1;
+    return c;
+  }
+  build_ppe(c);
+  /* channel wall distance: |sig(k)| (sgm_les.F90:856-862) */
+  for (int k = 0; k < c->NZ; ++k) c->walld[k] = fabs(c->sig[k]);
+  build_ic(c);
+  c->status = 0;
+  return c;
+}
+
+int nsx_case_status(const nsx_case *c, char *msg, int len) {
+  if (msg && len > 0) snprintf(msg, (size_t)len, "%s", c->err);
+  return c->status;
+}
+
+int nsx_case_info(const nsx_case *c, int *out /* 8 ints */) {
+  out[0] = c->N_CELL0;
+  out[1] = c->N_CELL;
+  out[2] = c->N_VERT;
+  out[3] = c->NZ;
+  out[4] = c->N_CELLexact;
+  out[5] = c->N_CELLghost;
+  out[6] = c->bctop;
+  out[7] = c->bcbot;
+  return 0;
+}
+
+double nsx_case_total_volume(const nsx_case *c) { return c->TotalVolume; }
+
+int nsx_case_num_arrays(const nsx_case *c) { return c->n_arrays; }
+
+const char *nsx_case_array_name(const nsx_case *c, int n) {
+  return (n >= 0 && n < c->n_arrays) ? c->arrays[n].name : NULL;
+}
+
+void *nsx_case_get(const nsx_case *c, const char *name, int64_t *n, int *type) {
+  for (int a = 0; a < c->n_arrays; ++a)
+    if (strcmp(c->arrays[a].name, name) == 0) {
+      if (n) *n = c->arrays[a].n;
+      if (type) *type = c->arrays[a].type;
+      return c->arrays[a].ptr;
+    }
+  return NULL;
+}

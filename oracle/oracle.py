@@ -1,0 +1,144 @@
+"""Python handle on the CPU restatement (nsx_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "liboracle.so")
+SRC = os.path.join(HERE, "nsx_oracle.c")
+
+_lib = None
+
+
+def build() -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = LIB + ".tmp"
+    # -ffp-contract=off and no -march: plain SSE2 double arithmetic in the
+    # reference's operation order, the same as the Fortran build
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-Wall",
+                    "-o", tmp, SRC, "-lm"], check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, ci, cd, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ci] * 5
+        L.orc_free.argtypes = [vp]
+        L.orc_set_array.argtypes = [vp, cs, vp]
+        L.orc_state.restype = ctypes.POINTER(cd)
+        L.orc_state.argtypes = [vp, cs]
+        L.orc_set_params.argtypes = [vp, cd, cd, cd, cd, cd, cd, ci, ci, ci, ci, ci, ci, ci, cd, cd, cd]
+        L.orc_init_state.argtypes = [vp]
+        L.orc_step.argtypes = [vp]
+        L.orc_hydro.argtypes = [vp]
+        L.orc_solver_record.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(cd)]
+        for name, n in [("orc_bcglobalvc", 6), ("orc_bcglobalp", 2), ("orc_grad_lsm", 4), ("orc_calcul_face", 7),
+                        ("orc_advdiff", 7), ("orc_sgles", 3), ("orc_source3d", 4), ("orc_update_rhsp", 3),
+                        ("orc_bicgstab", 3), ("orc_newu3d", 2), ("orc_interpolate_u", 3),
+                        ("orc_bcpcenter", 1)]:
+            f = getattr(L, name)
+            f.argtypes = [vp] + [vp] * n + ([ci] if name in ("orc_bcglobalvc", "orc_bcglobalp") else [])
+        L.orc_bicgstab.restype = ci
+        L.orc_hydro.restype = ci
+        L.orc_step.restype = ci
+        _lib = L
+    return _lib
+
+
+_BORROW = ["No_cp", "No_vp", "nbe", "nbev", "areaCell", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "doe",
+           "dle1", "dle2", "sum_xc2", "sum_yc2", "sum_xcyc", "surrounding", "Dimsurrounding", "vertex_pair",
+           "vertex_edge", "weight", "dlVsum", "bcuc", "bcvc", "bcwc", "bcpc", "bcuv", "bcvv", "bcwv", "bcpv",
+           "AmC0", "AmC1", "AmC2", "AmC3", "AmCT", "AmCB", "sig", "dsig", "sigv", "dsigv", "walld"]
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Oracle:
+    """The reference's step on the CPU, holding its own state (reference layout)."""
+
+    def __init__(self, case, *, p0=None):
+        L = lib()
+        self.case = case
+        c = case
+        self.h = L.orc_create(c.N_CELL0, c.N_CELL, c.N_VERT, c.NZ, c.N_CELLexact)
+        self._keep = {}
+        for name in _BORROW:
+            a = np.asfortranarray(c.arrays[name])
+            if a.dtype.kind == "i":
+                a = a.astype(np.int32, order="F")
+            self._keep[name] = a
+            if L.orc_set_array(self.h, name.encode(), _ptr(a)) != 0:
+                raise KeyError(name)
+        p = c.params
+        L.orc_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, p.MaxIters, p.FinRK, p.Iturbu,
+                         p.TopBC, p.BotBC, c.bctop, c.bcbot, c.TotalVolume, p.sigIni, p.sigFin)
+        for nm, src in (("ufn", c.u0), ("vfn", c.v0), ("wfn", c.w0), ("pfn", c.p0 if p0 is None else p0)):
+            self.state(nm)[:] = src
+        L.orc_init_state(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_free(self.h)
+            self.h = None
+
+    def state(self, name: str) -> np.ndarray:
+        """Live view of a state array, shaped like the Fortran array."""
+        c = self.case
+        ptr = lib().orc_state(self.h, name.encode())
+        if not ptr:
+            raise KeyError(name)
+        if name in ("ufv", "vfv", "wfv", "pfv"):
+            shape = (c.N_VERT, c.NZ - 1)
+        elif name in ("U1", "U2", "U3", "UT", "UB"):
+            shape = (c.N_CELL0, c.NZ)
+        else:
+            shape = (c.N_CELL, c.NZ)
+        n = shape[0] * shape[1]
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape, order="F")
+
+    def step(self) -> int:
+        return lib().orc_step(self.h)
+
+    def hydro(self) -> int:
+        return lib().orc_hydro(self.h)
+
+    def solver_record(self) -> dict:
+        iv = (ctypes.c_int * 4)()
+        dv = (ctypes.c_double * 2)()
+        lib().orc_solver_record(self.h, iv, dv)
+        return dict(iter=iv[0], n_solves=iv[1], iter_total=iv[2], exit=iv[3], e1=dv[0], e2=dv[1])
+
+    # -- single routines on caller arrays (reference layout, modified in place)
+    def call(self, name: str, *arrays, tagdir=None):
+        args = [self.h] + [_ptr(a) for a in arrays]
+        if tagdir is not None:
+            args.append(tagdir)
+        return getattr(lib(), "orc_" + name)(*args)
+
+    def cell(self):
+        return self.case.cell_field()
+
+    def vert(self):
+        return self.case.vert_field()
+
+    def face(self):
+        c = self.case
+        return np.zeros((c.N_CELL0, c.NZ), order="F")

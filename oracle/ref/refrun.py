@@ -1,0 +1,115 @@
+"""Run the compiled Fortran reference (oracle/_ref) on an nsx case.
+
+Test infrastructure only: writes the binary case file read by
+nsx_ref_driver.F90, runs the driver, and parses its dump into numpy arrays
+shaped like the Fortran arrays.  The reference itself is built from
+/root/reference by build_ref.sh and exists only where that tree does.
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_OUT = os.path.join(os.path.dirname(HERE), "_ref")
+MAGIC = 1314084913
+
+
+def driver_path(nn: int) -> str:
+    return os.path.join(REF_OUT, f"nn{nn}", "nsx_ref_driver")
+
+
+def ensure_built(nn: int) -> str:
+    exe = driver_path(nn)
+    if not os.path.exists(exe):
+        subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(nn)], check=True,
+                       stdout=subprocess.DEVNULL)
+    return exe
+
+
+def write_case(path: str, case, *, mode: int, nsteps: int = 1, dump_every: int = 1,
+               p0=None) -> None:
+    p = case.params
+    hdr = np.zeros(16, dtype=np.int32)
+    hdr[:10] = [MAGIC, case.N_CELL0, case.N_VERT, case.nn, nsteps, p.FinRK, p.MaxIters, p.Iturbu,
+                mode, dump_every]
+    rhdr = np.zeros(16, dtype=np.float64)
+    rhdr[:6] = [p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps]
+    n0 = case.N_CELL0
+    pressure = case.p0 if p0 is None else p0
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(rhdr.tobytes())
+        f.write(np.asfortranarray(case.No_vp).astype(np.int32).tobytes(order="F"))
+        f.write(np.asfortranarray(case.No_cp[:n0]).astype(np.int32).tobytes(order="F"))
+        f.write(case.nbe.astype(np.int32).tobytes())
+        f.write(case.nbev.astype(np.int32).tobytes())
+        for a in (case.xv, case.yv, case.zbv, case.sig, case.dsig, case.sigv, case.dsigv):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        for a in (case.u0, case.v0, case.w0, pressure):
+            f.write(np.asfortranarray(a[:n0]).tobytes(order="F"))
+
+
+def read_dump(path: str, case) -> dict:
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        name = data[pos:pos + 32].decode().strip()
+        typ = int(np.frombuffer(data, np.int32, 1, pos + 32)[0])
+        n = int(np.frombuffer(data, np.int64, 1, pos + 36)[0])
+        pos += 44
+        dt = np.float64 if typ == 1 else np.int32
+        arr = np.frombuffer(data, dt, n, pos).copy()
+        pos += n * arr.itemsize
+        out[name] = _shape(arr, case)
+    return out
+
+
+def _shape(a, c):
+    n = a.size
+    table = [(c.N_CELL * c.NZ, (c.N_CELL, c.NZ)), (c.N_CELL0 * c.NZ, (c.N_CELL0, c.NZ)),
+             (c.N_VERT * (c.NZ - 1), (c.N_VERT, c.NZ - 1)), (3 * c.N_CELL, (c.N_CELL, 3)),
+             (3 * c.N_CELL0, (c.N_CELL0, 3)), (10 * c.N_VERT, (c.N_VERT, 10))]
+    for size, shape in table:
+        if n == size:
+            return a.reshape(shape, order="F")
+    return a
+
+
+_ITER = re.compile(r"iter=\s*(\d+)\s*e1norm=\s*(\S+)\s*e2norm=\s*(\S+)")
+
+
+def parse_solves(log: str) -> list:
+    """Final (iter, e1norm, e2norm) of every BiCGSTAB solve printed by the reference."""
+    solves = []
+    cur = None
+    for line in log.splitlines():
+        if "SOLVE FOR P FIELD" in line:
+            if cur is not None:
+                solves.append(cur)
+            cur = (0, None, None)
+        m = _ITER.search(line)
+        if m and cur is not None:
+            cur = (int(m.group(1)), float(m.group(2).replace("D", "E")), float(m.group(3).replace("D", "E")))
+    if cur is not None:
+        solves.append(cur)
+    return solves
+
+
+def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workdir: str | None = None):
+    """Run the reference; returns (dump dict, stdout text)."""
+    exe = ensure_built(case.nn)
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
+        cf = os.path.join(td, "case.bin")
+        of = os.path.join(td, "out.bin")
+        write_case(cf, case, mode=mode, nsteps=nsteps, dump_every=dump_every, p0=p0)
+        r = subprocess.run([exe, cf, of], cwd=td, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference driver failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
+        return read_dump(of, case), r.stdout

@@ -1,0 +1,96 @@
+"""Write the golden fixtures of tests/golden from the compiled reference.
+
+Runs the reference's own Fortran routines (oracle/_ref, built from
+/root/reference by oracle/ref/build_ref.sh) on seeded nsx cases and stores
+what they produce:
+
+  tiny_unit.npz   6x4 quads x 4 layers (jittered): set-up geometry, flags and
+                  PPE matrix, and every hot-path routine applied once to the
+                  initial field (full arrays, ghost rows included);
+  cfg0_steps.json workload 0 (32x16 quads x 32 layers, 10 RK2 steps):
+                  per-step SHA-256 of the state arrays, norms, sample values
+                  and the BiCGSTAB iteration count of every pressure solve;
+  cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
+
+Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
+adding +0.0 (so -0.0 and +0.0 hash alike).  Run in the build container:
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "3d-channel-flow-code_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle", "ref"))
+
+from nsx.case import config_case, make_case  # noqa: E402
+
+TINY = dict(nx=6, ny=4, nn=4, seed=7, jitter=0.1, bed=True)
+TINY_P_SEED = 3
+STEP_ARRAYS = ["ufn", "vfn", "wfn", "pfn", "ufv", "vfv", "wfv", "pfv", "U1FACE", "U2FACE", "U3FACE", "UTFACE",
+               "UBFACE", "rhsuf", "rhsvf", "rhswf", "eddy"]
+
+
+def tiny_case():
+    c = make_case(TINY["nx"], TINY["ny"], TINY["nn"], seed=TINY["seed"], jitter=TINY["jitter"], bed=TINY["bed"])
+    rng = np.random.default_rng(TINY_P_SEED)
+    p0 = np.zeros((c.N_CELL, c.NZ), order="F")
+    p0[:c.N_CELL0, 1:-1] = rng.uniform(-1.0, 1.0, (c.N_CELL0, c.NZ - 2))
+    return c, p0
+
+
+def rows_of(case, a):
+    return case.N_CELLexact if a.shape[0] == case.N_CELL else a.shape[0]
+
+
+def digest(case, a) -> str:
+    b = np.ascontiguousarray(np.asarray(a)[:rows_of(case, a)], dtype=np.float64) + 0.0
+    return hashlib.sha256(b.tobytes()).hexdigest()
+
+
+def main():
+    import refrun
+
+    c, p0 = tiny_case()
+    d, log = refrun.run(c, mode=1, p0=p0)
+    out = {k: (v[:rows_of(c, v)] if isinstance(v, np.ndarray) and v.ndim == 2 else v) for k, v in d.items()}
+    out["solves"] = np.array([s[0] for s in refrun.parse_solves(log)], dtype=np.int32)
+    np.savez_compressed(os.path.join(HERE, "tiny_unit.npz"), **out)
+    print("tiny_unit.npz:", len(out), "arrays")
+
+    for cfg, nsteps in ((0, 10), (3, 2)):
+        c = config_case(cfg)
+        d, log = refrun.run(c, mode=2, nsteps=nsteps, dump_every=1)
+        solves = refrun.parse_solves(log)
+        rec = {"cfg": cfg, "nsteps": nsteps, "solves": [s[0] for s in solves],
+               "solve_norms": [[s[1], s[2]] for s in solves], "steps": {}}
+        for s in range(0, nsteps + 1):
+            st = {}
+            for nm in STEP_ARRAYS:
+                key = f"{nm}_{s}"
+                if key not in d:
+                    continue
+                a = d[key]
+                r = rows_of(c, a)
+                st[nm] = {"sha256": digest(c, a), "l2": float(np.linalg.norm(a[:r])),
+                          "sample": [float(x) for x in np.asarray(a[:r]).ravel(order="F")[::max(1, a[:r].size // 7)][:8]]}
+            rec["steps"][str(s)] = st
+        stage = {}
+        for nm in ("s1_ufnp", "s1_vfnp", "s1_wfnp", "s1_pfnp", "s1_ufv", "s1_pfv", "s1_U1", "s1_UT", "s1_rhsuf",
+                   "s1_eddy"):
+            stage[nm] = {"sha256": digest(c, d[nm]), "l2": float(np.linalg.norm(d[nm][:rows_of(c, d[nm])]))}
+        rec["stage1"] = stage
+        with open(os.path.join(HERE, f"cfg{cfg}_steps.json"), "w") as f:
+            json.dump(rec, f, indent=1)
+        print(f"cfg{cfg}_steps.json:", len(rec["solves"]), "solves")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+"""The oracle (CPU restatement) and the host set-up against the reference.
+
+The fixtures in tests/golden were written by the reference's own Fortran
+routines (tests/golden/make_golden.py); every comparison here is bit-exact.
+Closed-form checks pin what a fixture could not tell apart by itself.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from common import STATE_MAP, digest, load_steps, load_tiny, tiny_case
+from nsx.case import config_case, make_case
+
+GEOM_CELL = ["xc", "yc", "areaCell", "No_cp", "bcuc", "bcvc", "bcwc", "bcpc"]
+GEOM_FULL = ["dxVV", "dyVV", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "xme", "yme", "xmo", "ymo", "doe",
+             "dle1", "dle2", "dle3", "sum_xc2", "sum_yc2", "sum_xcyc", "dlVsum", "Dimsurrounding",
+             "vertex_pair", "vertex_edge", "tagXV", "tagYV", "bcuv", "bcvv", "bcwv", "bcpv", "sig", "dsig", "sigv",
+             "dsigv"]
+
+
+def test_setup_geometry_bitwise():
+    """host/nsx_case.c == calcul_geometry + calcul_bc + PDE_center of the reference."""
+    c, _ = tiny_case()
+    g = load_tiny()
+    ne = c.N_CELLexact
+    for k in GEOM_CELL:
+        assert np.array_equal(g[k][:ne], c.arrays[k][:ne]), k
+    for k in GEOM_FULL:
+        assert np.array_equal(g[k], c.arrays[k]), k
+    for k in ["AmC0", "AmC1", "AmC2", "AmC3", "AmCT", "AmCB"]:
+        assert np.array_equal(g[k][:, 1:-1], c.arrays[k][:, 1:-1]), k
+    ds = c.Dimsurrounding
+    for v in range(c.N_VERT):
+        assert np.array_equal(g["surrounding"][v, :ds[v]], c.surrounding[v, :ds[v]])
+        assert np.array_equal(g["weight"][v, :ds[v]], c.weight[v, :ds[v]])
+    assert list(g["sizes"]) == [c.N_CELLexact, c.N_CELLghost, c.bctop, c.bcbot]
+    assert g["TotalVolume"][0] == c.TotalVolume
+
+
+def run_unit_sequence(O, c, p0):
+    """The single-routine sequence of nsx_ref_driver's unit mode, on the oracle."""
+    r = {}
+    for nm, st in (("ufn_0", "ufn"), ("vfn_0", "vfn"), ("wfn_0", "wfn"), ("pfn_0", "pfn"), ("ufv_0", "ufv"),
+                   ("vfv_0", "vfv"), ("wfv_0", "wfv"), ("pfv_0", "pfv"), ("U1FACE_0", "U1"), ("U2FACE_0", "U2"),
+                   ("U3FACE_0", "U3"), ("UTFACE_0", "UT"), ("UBFACE_0", "UB")):
+        r[nm] = O.state(st).copy()
+
+    def st(n):
+        return O.state(n).copy(order="F")
+
+    u, v, w, p = st("ufn"), st("vfn"), st("wfn"), st("pfn")
+    uv, vv, wv = O.vert(), O.vert(), O.vert()
+    O.call("bcglobalvc", u, v, w, uv, vv, wv, tagdir=2)
+    r["vc2_u"], r["vc2_v"], r["vc2_w"] = uv.copy(), vv.copy(), wv.copy()
+    pv = O.vert()
+    O.call("bcglobalp", p, pv, tagdir=2)
+    r["pv2"] = pv.copy()
+    pv3 = O.vert()
+    O.call("bcglobalp", p, pv3, tagdir=3)
+    r["pv3"] = pv3
+    gx, gy, gs = O.cell(), O.cell(), O.cell()
+    O.call("grad_lsm", gx, gy, gs, p)
+    r["lsm_p_x"], r["lsm_p_y"], r["lsm_p_s"] = gx.copy(), gy.copy(), gs.copy()
+    O.call("grad_lsm", gx, gy, gs, u)
+    r["lsm_u_x"], r["lsm_u_y"], r["lsm_u_s"] = gx.copy(), gy.copy(), gs.copy()
+    f = [O.face() for _ in range(5)]
+    O.call("calcul_face", *f, u, uv)
+    for q, s in enumerate("123TB"):
+        r["face_u_" + s] = f[q]
+    O.call("sgles", st("ufn"), st("vfn"), st("wfn"))
+    r["eddy_0"] = O.state("eddy").copy()
+    prm = c.params
+    ru, rv, rw = O.cell() + prm.bdx, O.cell() + prm.bdy, O.cell() + prm.bdz
+    O.call("advdiff", ru, rv, rw, st("ufn"), st("vfn"), st("wfn"), st("eddy"))
+    r["rhsu"], r["rhsv"], r["rhsw"] = ru, rv, rw
+    src = O.cell()
+    O.call("source3d", src, st("ufn"), st("vfn"), st("wfn"))
+    r["src"] = src.copy()
+    for s, nm in zip("123TB", ("U1", "U2", "U3", "UT", "UB")):
+        r["src_U" + s] = O.state(nm).copy()
+    src[:c.N_CELL0, 1:-1] /= prm.dt
+    O.call("update_rhsp", src, st("pfn"), O.vert())
+    r["rhsp"] = src.copy()
+    p6, pv6 = st("pfn"), O.vert()
+    O.call("bicgstab", p6, pv6, src)
+    r["solves"] = np.array([O.solver_record()["iter"]], dtype=np.int32)
+    r["psol"], r["psol_v"] = p6.copy(), pv6.copy()
+    O.call("grad_lsm", gx, gy, gs, p6)
+    r["gp_x"], r["gp_y"], r["gp_s"] = gx.copy(), gy.copy(), gs.copy()
+    O.call("newu3d", p6, pv6)
+    for s, nm in zip("123TB", ("U1", "U2", "U3", "UT", "UB")):
+        r["newu_U" + s] = O.state(nm).copy()
+    r["newu_pv"] = pv6
+    return r
+
+
+def test_oracle_routines_bitwise():
+    """Every hot-path routine of the oracle == the reference's, bit for bit."""
+    c, p0 = tiny_case()
+    g = load_tiny()
+    O = oracle.Oracle(c, p0=p0)
+    r = run_unit_sequence(O, c, p0)
+    assert r.keys() <= g.keys()
+    for k, mine in r.items():
+        ref = g[k]
+        n = ref.shape[0]
+        assert np.array_equal(ref, np.asarray(mine)[:n]), k
+
+
+@pytest.mark.parametrize("cfg", [0, 3])
+def test_oracle_trajectory_bitwise(cfg):
+    """Whole RK2 steps (hydro x2, average, update) == the reference's, hash for hash."""
+    rec = load_steps(cfg)
+    c = config_case(cfg)
+    O = oracle.Oracle(c)
+    iters = []
+    for s in range(1, rec["nsteps"] + 1):
+        for _ in range(2):
+            pass
+        before = O.solver_record()["n_solves"]
+        O.step()
+        assert O.solver_record()["n_solves"] == before + 2
+        for nm, want in rec["steps"][str(s)].items():
+            assert digest(c, O.state(STATE_MAP[nm])) == want["sha256"], (s, nm)
+    assert O.solver_record()["iter_total"] == sum(rec["solves"])
+    assert O.solver_record()["iter"] == rec["solves"][-1]
+
+
+# ---------------------------------------------------------- closed forms ----
+
+def _linear(c, a, b, e):
+    f = c.cell_field()
+    ne = c.N_CELLexact
+    f[:ne] = a * c.xc[:ne, None] + b * c.yc[:ne, None] + e * c.sig[None, :]
+    return f
+
+
+def test_lsm_gradient_exact_for_linear_fields():
+    """grandientLSM is exact for f = a x + b y (+ any function of sigma) in the interior."""
+    c = make_case(8, 6, 5, seed=3, jitter=0.1)
+    O = oracle.Oracle(c)
+    # avoid the periodic seam: cells whose stencil does not wrap
+    f = _linear(c, 0.7, -1.3, 0.0)
+    gx, gy, gs = O.cell(), O.cell(), O.cell()
+    O.call("grad_lsm", gx, gy, gs, f)
+    xc = c.xc
+    inner = [i for i in range(c.N_CELL0) if c.nbe[i] == 0 and
+             all(abs(xc[c.No_cp[i, j] - 1] - xc[i]) < 1.0 for j in range(3))]
+    assert len(inner) > 10
+    np.testing.assert_allclose(gx[inner, 1:-1], 0.7, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(gy[inner, 1:-1], -1.3, rtol=0, atol=1e-12)
+
+
+def test_vertical_gradient_exact_for_quadratic_on_uniform_levels():
+    c = make_case(4, 4, 6, seed=1, stretch=0.0)
+    O = oracle.Oracle(c)
+    f = c.cell_field()
+    f[:c.N_CELLexact] = 3.0 * c.sig[None, :] - 2.0
+    gx, gy, gs = O.cell(), O.cell(), O.cell()
+    O.call("grad_lsm", gx, gy, gs, f)
+    np.testing.assert_allclose(gs[:c.N_CELL0, 1:-1], 3.0, atol=1e-12)
+
+
+def test_idw_reproduces_constants():
+    c = make_case(6, 5, 4, seed=5, jitter=0.1)
+    O = oracle.Oracle(c)
+    p = c.cell_field() + 2.5
+    pv = O.vert()
+    O.call("bcglobalp", p, pv, tagdir=2)
+    np.testing.assert_allclose(pv, 2.5, rtol=1e-15)
+
+
+def test_ppe_rows_sum_to_zero_and_manufactured_solve():
+    """PDE_center rows sum to zero; BICGSTAB_SOLVE recovers x from b = A x."""
+    c = make_case(6, 4, 6, seed=2, jitter=0.1)
+    A = [c.AmC0, c.AmC1, c.AmC2, c.AmC3, c.AmCT, c.AmCB]
+    rows = sum(a[:, 1:-1] for a in A)
+    assert np.abs(rows).max() < 1e-12 * np.abs(c.AmC0[:, 1:-1]).max()
+    # manufactured zero-mean solution
+    rng = np.random.default_rng(0)
+    x = c.cell_field()
+    x[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
+    O = oracle.Oracle(c)
+    O.call("bcpcenter", x)
+    b = c.cell_field()
+    nz = c.NZ
+    cp = c.No_cp[:c.N_CELL0] - 1
+    part = [x[cp[:, j]] for j in range(3)]
+    for k in range(1, nz - 1):
+        b[:c.N_CELL0, k] = (A[0][:, k] * x[:c.N_CELL0, k] + A[1][:, k] * part[0][:, k] + A[2][:, k] * part[1][:, k] +
+                            A[3][:, k] * part[2][:, k] + A[4][:, k] * x[:c.N_CELL0, k + 1] +
+                            A[5][:, k] * x[:c.N_CELL0, k - 1])
+    phi = c.cell_field()
+    pv = O.vert()
+    O.call("bicgstab", phi, pv, b.copy(order="F"))
+    # the solution is fixed up to a constant; the solver removes the volume mean
+    vol = c.areaCell[:c.N_CELL0, None] * c.dsig[None, 1:-1]
+    xm = x[:c.N_CELL0, 1:-1] - (x[:c.N_CELL0, 1:-1] * vol).sum() / (c.TotalVolume * 2.0)
+    np.testing.assert_allclose(phi[:c.N_CELL0, 1:-1], xm, atol=1e-7 * np.abs(xm).max())
