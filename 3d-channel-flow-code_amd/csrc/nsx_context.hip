@@ -1,0 +1,1037 @@
+// nsx_context.hip -- the C ABI of include/nsx.h: one device context holding
+// geometry, matrix and every field resident in HBM, the stage/step drivers
+// and the single-routine entry points.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nsx.h"
+#include "nsx_kernels.h"
+
+namespace {
+
+enum FieldKind { FK_NONE = 0, FK_CELL = 1, FK_VERT = 2, FK_FACE = 3 };
+constexpr int NSLOT = 128;
+constexpr int MAXGRID = 2048;
+
+struct Timer {
+  double ms = 0.0;
+  long long n = 0;
+};
+
+}  // namespace
+
+struct nsx_ctx {
+  int N0 = 0, NC = 0, NV = 0, NZ = 0, NEX = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool committed = false;
+  // set-up arrays (host, reference layout) kept until commit
+  std::vector<int> No_cp, No_vp, nbe, nbev;
+  std::vector<double> areaCell, dlVV, dlCC, dxCC, dyCC, xnn, ynn, dle1, dle2, doe, sum_xc2, sum_yc2, sum_xcyc;
+  double TotalVolume = 0.0;
+  std::vector<double> sig, dsig, sigv, dsigv;
+  double sigIni = 0.0, sigFin = 1.0;
+  std::vector<int> surrounding, Dimsurrounding, vertex_pair, vertex_edge;
+  std::vector<double> weight, dlVsum;
+  std::vector<int> bcc[4], bcv[4];  // u,v,w,p
+  int bctop = 1, bcbot = 1, TopBC = 0, BotBC = 0;
+  std::vector<double> AmC[6];
+  std::vector<double> walld;
+  double dt = 0, Re = 1, bdx = 0, bdy = 0, bdz = 0, eps = 1e-10;
+  int MaxIters = 20000, FinRK = 2, Iturbu = 1;
+  unsigned have = 0;
+  // device
+  Geo g{};
+  std::vector<void *> allocs;
+  double *slot[NSLOT] = {};
+  int kind[NSLOT] = {};
+  double *pool_cell[3][4] = {};  // three time-level sets of (u,v,w,p)
+  int set_n = 0;                 // pool set holding time level n
+  double *ustar[3] = {}, *uv2[3] = {}, *rhsp = nullptr;
+  double *sr = nullptr, *srhat = nullptr, *sp[2] = {}, *svv[2] = {}, *ss = nullptr, *st_ = nullptr;
+  double *W = nullptr, *invD = nullptr;
+  SolverState *st_dev = nullptr, *st_host = nullptr;
+  Partials part{};
+  nsx_solve_info info{};
+  int iter_hint = 0;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  Timer t_a, t_b, t_c, t_stage;
+  std::vector<std::pair<size_t, int>> pending;  // (event index, which timer: 0 a,1 b,2 c,3 stage)
+};
+
+static int nsx_fail(nsx_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+static int nsx_fail_hip(nsx_ctx *ctx, hipError_t e, const char *what, int line) {
+  return nsx_fail(ctx, -2, "HIP error %d (%s) at nsx_context.hip:%d: %s", (int)e, hipGetErrorString(e), line, what);
+}
+
+#define LAUNCH(x)                                                                     \
+  do {                                                                                \
+    int e_ = (x);                                                                     \
+    if (e_ != 0) return nsx_fail_hip(ctx, (hipError_t)e_, #x, __LINE__);              \
+  } while (0)
+
+#define NEED_COMMIT(ctx)                                                              \
+  do {                                                                                \
+    if (!(ctx)) return -1;                                                            \
+    if (!(ctx)->committed) return nsx_fail(ctx, -1, "nsx_commit has not been called"); \
+  } while (0)
+
+template <class T>
+static int dev_alloc(nsx_ctx *ctx, T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess) return nsx_fail(ctx, -3, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  ctx->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+template <class T>
+static int dev_upload(nsx_ctx *ctx, T **p, const std::vector<T> &h) {
+  int rc = dev_alloc(ctx, p, h.size());
+  if (rc) return rc;
+  HIPCHECK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static size_t field_elems(const nsx_ctx *c, int kind) {
+  if (kind == FK_VERT) return (size_t)c->NV * (c->NZ - 1);
+  return (size_t)c->N0 * c->NZ;
+}
+
+// ------------------------------------------------------------- lifecycle --
+extern "C" int nsx_create(int n_cell0, int n_cell, int n_vert, int nz, nsx_ctx **out) {
+  if (!out) return -1;
+  *out = nullptr;
+  if (n_cell0 <= 0 || n_cell < n_cell0 || n_vert <= 0 || nz < 4) return -1;
+  nsx_ctx *ctx = new nsx_ctx();
+  ctx->N0 = n_cell0;
+  ctx->NC = n_cell;
+  ctx->NV = n_vert;
+  ctx->NZ = nz;
+  hipError_t e = hipGetDevice(&ctx->dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return -2;
+  }
+  *out = ctx;
+  return 0;
+}
+
+extern "C" int nsx_destroy(nsx_ctx *ctx) {
+  if (!ctx) return 0;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->st_host) (void)hipHostFree(ctx->st_host);
+  for (hipEvent_t ev : ctx->evpool) (void)hipEventDestroy(ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+extern "C" const char *nsx_last_error(const nsx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+extern "C" int nsx_device(const nsx_ctx *ctx) { return ctx ? ctx->dev : -1; }
+
+// ---------------------------------------------------------------- set-up --
+template <class T>
+static void take(std::vector<T> &v, const T *p, size_t n) {
+  v.assign(p, p + n);
+}
+
+extern "C" int nsx_set_mesh(nsx_ctx *ctx, const int *No_cp, const int *No_vp, const int *nbe, const int *nbev) {
+  if (!ctx || !No_cp || !No_vp || !nbe || !nbev) return -1;
+  take(ctx->No_cp, No_cp, (size_t)ctx->NC * 3);
+  take(ctx->No_vp, No_vp, (size_t)ctx->N0 * 3);
+  take(ctx->nbe, nbe, (size_t)ctx->N0);
+  take(ctx->nbev, nbev, (size_t)ctx->NV);
+  ctx->have |= 1;
+  return 0;
+}
+
+extern "C" int nsx_set_geometry(nsx_ctx *ctx, const double *areaCell, const double *dlVV, const double *dlCC,
+                                const double *dxCC, const double *dyCC, const double *xnn, const double *ynn,
+                                const double *dle1, const double *dle2, const double *doe, const double *sum_xc2,
+                                const double *sum_yc2, const double *sum_xcyc, double TotalVolume) {
+  if (!ctx) return -1;
+  const size_t n3 = (size_t)ctx->N0 * 3;
+  take(ctx->areaCell, areaCell, (size_t)ctx->NC);
+  take(ctx->dlVV, dlVV, n3);
+  take(ctx->dlCC, dlCC, n3);
+  take(ctx->dxCC, dxCC, n3);
+  take(ctx->dyCC, dyCC, n3);
+  take(ctx->xnn, xnn, n3);
+  take(ctx->ynn, ynn, n3);
+  take(ctx->dle1, dle1, n3);
+  take(ctx->dle2, dle2, n3);
+  take(ctx->doe, doe, n3);
+  take(ctx->sum_xc2, sum_xc2, (size_t)ctx->N0);
+  take(ctx->sum_yc2, sum_yc2, (size_t)ctx->N0);
+  take(ctx->sum_xcyc, sum_xcyc, (size_t)ctx->N0);
+  ctx->TotalVolume = TotalVolume;
+  ctx->have |= 2;
+  return 0;
+}
+
+extern "C" int nsx_set_sigma(nsx_ctx *ctx, const double *sig, const double *dsig, const double *sigv,
+                             const double *dsigv, double sigIni, double sigFin) {
+  if (!ctx) return -1;
+  take(ctx->sig, sig, (size_t)ctx->NZ);
+  take(ctx->dsig, dsig, (size_t)ctx->NZ);
+  take(ctx->sigv, sigv, (size_t)ctx->NZ - 1);
+  take(ctx->dsigv, dsigv, (size_t)ctx->NZ - 1);
+  ctx->sigIni = sigIni;
+  ctx->sigFin = sigFin;
+  ctx->have |= 4;
+  return 0;
+}
+
+extern "C" int nsx_set_vertex_weights(nsx_ctx *ctx, const int *surrounding, const double *weight,
+                                      const int *Dimsurrounding, const double *dlVsum, const int *vertex_pair,
+                                      const int *vertex_edge) {
+  if (!ctx) return -1;
+  take(ctx->surrounding, surrounding, (size_t)ctx->NV * 10);
+  take(ctx->weight, weight, (size_t)ctx->NV * 10);
+  take(ctx->Dimsurrounding, Dimsurrounding, (size_t)ctx->NV);
+  take(ctx->dlVsum, dlVsum, (size_t)ctx->NV);
+  take(ctx->vertex_pair, vertex_pair, (size_t)ctx->NV);
+  take(ctx->vertex_edge, vertex_edge, (size_t)ctx->NV);
+  ctx->have |= 8;
+  return 0;
+}
+
+extern "C" int nsx_set_bc(nsx_ctx *ctx, const int *bcuc, const int *bcvc, const int *bcwc, const int *bcpc,
+                          const int *bcuv, const int *bcvv, const int *bcwv, const int *bcpv, int bctop, int bcbot,
+                          int TopBC, int BotBC) {
+  if (!ctx) return -1;
+  const int *c[4] = {bcuc, bcvc, bcwc, bcpc}, *v[4] = {bcuv, bcvv, bcwv, bcpv};
+  for (int q = 0; q < 4; ++q) {
+    take(ctx->bcc[q], c[q], (size_t)ctx->NC);
+    take(ctx->bcv[q], v[q], (size_t)ctx->NV);
+  }
+  if (TopBC < 0 || TopBC > 1 || BotBC < 0 || BotBC > 1)
+    return nsx_fail(ctx, -1, "TopBC/BotBC must be 0 (no slip) or 1 (free slip), got %d/%d", TopBC, BotBC);
+  ctx->bctop = bctop;
+  ctx->bcbot = bcbot;
+  ctx->TopBC = TopBC;
+  ctx->BotBC = BotBC;
+  ctx->have |= 16;
+  return 0;
+}
+
+extern "C" int nsx_set_ppe_matrix(nsx_ctx *ctx, const double *A0, const double *A1, const double *A2,
+                                  const double *A3, const double *AT, const double *AB) {
+  if (!ctx) return -1;
+  const double *a[6] = {A0, A1, A2, A3, AT, AB};
+  for (int q = 0; q < 6; ++q) take(ctx->AmC[q], a[q], (size_t)ctx->N0 * ctx->NZ);
+  ctx->have |= 32;
+  return 0;
+}
+
+extern "C" int nsx_set_les(nsx_ctx *ctx, const double *walld_level) {
+  if (!ctx) return -1;
+  take(ctx->walld, walld_level, (size_t)ctx->NZ);
+  ctx->have |= 64;
+  return 0;
+}
+
+extern "C" int nsx_set_params(nsx_ctx *ctx, double dt, double Re, double bdx, double bdy, double bdz, double eps,
+                              int MaxIters, int FinRK, int Iturbu) {
+  if (!ctx) return -1;
+  if (FinRK != 1 && FinRK != 2) return nsx_fail(ctx, -1, "FinRK must be 1 or 2");
+  ctx->dt = dt;
+  ctx->Re = Re;
+  ctx->bdx = bdx;
+  ctx->bdy = bdy;
+  ctx->bdz = bdz;
+  ctx->eps = eps;
+  ctx->MaxIters = MaxIters;
+  ctx->FinRK = FinRK;
+  ctx->Iturbu = Iturbu;
+  ctx->g.dt = dt;
+  ctx->g.nu = 1. / Re;
+  ctx->g.bdx = bdx;
+  ctx->g.bdy = bdy;
+  ctx->g.bdz = bdz;
+  ctx->have |= 128;
+  return 0;
+}
+
+// (N0,3) column-major -> [i*3+j]
+static std::vector<double> edge_major(const std::vector<double> &a, int N0) {
+  std::vector<double> o((size_t)N0 * 3);
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) o[(size_t)i * 3 + j] = a[(size_t)i + (size_t)j * N0];
+  return o;
+}
+
+extern "C" int nsx_commit(nsx_ctx *ctx) {
+  if (!ctx) return -1;
+  if (ctx->committed) return nsx_fail(ctx, -1, "context already committed");
+  if ((ctx->have & 255) != 255)
+    return nsx_fail(ctx, -1, "set-up incomplete (mask %u): mesh, geometry, sigma, vertex weights, bc, ppe matrix, les, params",
+                    ctx->have);
+  const int N0 = ctx->N0, NC = ctx->NC, NV = ctx->NV, NZ = ctx->NZ;
+  Geo &g = ctx->g;
+  g.N0 = N0;
+  g.NV = NV;
+  g.NZ = NZ;
+  g.NZi = NZ - 2;
+  g.NL = NZ - 1;
+  g.ncell = (uint32_t)N0 * (uint32_t)(NZ - 2);
+  if ((uint64_t)N0 * NZ >= (1ull << 31) || (uint64_t)NV * (NZ - 1) >= (1ull << 31))
+    return nsx_fail(ctx, -1, "mesh too large for 32-bit cell numbering");
+  g.divNZi = make_fastdiv((uint32_t)g.NZi);
+  g.divNL = make_fastdiv((uint32_t)g.NL);
+  g.bctop = ctx->bctop;
+  g.bcbot = ctx->bcbot;
+  g.TopBC = ctx->TopBC;
+  g.BotBC = ctx->BotBC;
+  g.TotalVolume = ctx->TotalVolume;
+  g.sigIni = ctx->sigIni;
+  g.sigFin = ctx->sigFin;
+
+  // neighbours: periodic ghosts resolved to their partner columns
+  std::vector<int> nbr((size_t)N0 * 3), vtx((size_t)N0 * 3);
+  int nex = N0;
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      int jc = ctx->No_cp[(size_t)i + (size_t)j * NC] - 1;
+      if (jc < 0 || jc >= NC) return nsx_fail(ctx, -1, "No_cp(%d,%d) = %d out of range", i + 1, j + 1, jc + 1);
+      if (jc >= N0) {
+        if (jc + 1 > nex) nex = jc + 1;
+        for (int q = 0; q < 4; ++q)
+          if (ctx->bcc[q][jc] != 3)
+            return nsx_fail(ctx, -1, "ghost cell %d has boundary flag %d: only periodic horizontal ghosts are supported",
+                            jc + 1, ctx->bcc[q][jc]);
+        int partner = ctx->No_cp[(size_t)jc + 2 * (size_t)NC] - 1;
+        if (partner < 0 || partner >= N0) return nsx_fail(ctx, -1, "ghost cell %d has no periodic partner", jc + 1);
+        jc = partner;
+      }
+      nbr[(size_t)i * 3 + j] = jc;
+      int v = ctx->No_vp[(size_t)i + (size_t)j * N0] - 1;
+      if (v < 0 || v >= NV) return nsx_fail(ctx, -1, "No_vp(%d,%d) out of range", i + 1, j + 1);
+      vtx[(size_t)i * 3 + j] = v;
+    }
+  ctx->NEX = nex;
+
+  std::vector<double> ppec((size_t)N0 * 3);
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const size_t f = (size_t)i + (size_t)j * N0;
+      const double co1 = ctx->dle1[f] / ctx->dlCC[f];
+      int jc = ctx->No_cp[(size_t)i + (size_t)j * NC] - 1;
+      ppec[(size_t)i * 3 + j] = (ctx->bcc[3][jc] == 1) ? 0.0 : co1 * ctx->dlVV[f];
+    }
+  // the compact matrix must reproduce PDE_center's coefficients bit for bit
+  for (int k = 1; k < NZ - 1; ++k)
+    for (int i = 0; i < N0; ++i) {
+      const size_t n = (size_t)i + (size_t)k * N0;
+      const double sv = ctx->dsigv[k - 1];
+      const double a1 = ppec[(size_t)i * 3] * sv, a2 = ppec[(size_t)i * 3 + 1] * sv, a3 = ppec[(size_t)i * 3 + 2] * sv;
+      double AT = ctx->areaCell[i] / ctx->dsig[k], AB = ctx->areaCell[i] / ctx->dsig[k - 1];
+      if (k == 1 && ctx->bcbot == 1) AB = 0.0;
+      if (k == NZ - 2 && ctx->bctop == 1) AT = 0.0;
+      const double A0 = -1.0 * (AT + AB + a1 + a2 + a3);
+      const double ref[6] = {ctx->AmC[0][n], ctx->AmC[1][n], ctx->AmC[2][n], ctx->AmC[3][n], ctx->AmC[4][n],
+                             ctx->AmC[5][n]};
+      const double mine[6] = {A0, a1, a2, a3, AT, AB};
+      for (int q = 0; q < 6; ++q)
+        if (!(ref[q] == mine[q]))
+          return nsx_fail(ctx, -1, "PPE coefficient %d at (%d,%d) is %.17g, PDE_center's form gives %.17g", q, i + 1,
+                          k + 1, ref[q], mine[q]);
+    }
+
+  // vertex stencil (CSR) and periodic data
+  std::vector<int> vptr(NV + 1), vcell;
+  std::vector<double> vw;
+  std::vector<signed char> vkind(NV), vact[4];
+  for (int q = 0; q < 4; ++q) vact[q].assign(NV, VA_NONE);
+  vptr[0] = 0;
+  g.ncorner = 0;
+  for (int v = 0; v < NV; ++v) {
+    const int d = ctx->Dimsurrounding[v];
+    if (d < 0 || d > 10) return nsx_fail(ctx, -1, "Dimsurrounding(%d) = %d", v + 1, d);
+    for (int j = 0; j < d; ++j) {
+      int c = ctx->surrounding[(size_t)v + (size_t)j * NV] - 1;
+      if (c < 0 || c >= N0) return nsx_fail(ctx, -1, "surrounding(%d,%d) is not an interior cell", v + 1, j + 1);
+      vcell.push_back(c);
+      vw.push_back(ctx->weight[(size_t)v + (size_t)j * NV]);
+    }
+    vptr[v + 1] = (int)vcell.size();
+    if (ctx->vertex_edge[v] == 1 && g.ncorner < 4) g.corner[g.ncorner++] = v;
+    if (ctx->nbev[v] == 0) {
+      vkind[v] = VK_INTERIOR;
+      continue;
+    }
+    vkind[v] = ctx->vertex_edge[v] == 1 ? VK_CORNER : (ctx->vertex_pair[v] > 0 ? VK_PAIR : VK_OTHER);
+    for (int q = 0; q < 4; ++q) {
+      const int fl = ctx->bcv[q][v];
+      if (fl == 2) return nsx_fail(ctx, -1, "inflow vertex flags are not supported (vertex %d)", v + 1);
+      vact[q][v] = fl == 0 ? VA_ZERO : (fl == 3 ? VA_PERIODIC : VA_NONE);
+    }
+  }
+  std::vector<int> vpair(NV);
+  for (int v = 0; v < NV; ++v) {
+    vpair[v] = ctx->vertex_pair[v] > 0 ? ctx->vertex_pair[v] - 1 : -1;
+    if (vpair[v] >= NV) return nsx_fail(ctx, -1, "vertex_pair(%d) out of range", v + 1);
+    if (vkind[v] == VK_PAIR && vpair[v] < 0) return nsx_fail(ctx, -1, "vertex %d has no pair", v + 1);
+  }
+  if (g.ncorner != 0 && g.ncorner != 4) return nsx_fail(ctx, -1, "%d corner vertices (expected 0 or 4)", g.ncorner);
+
+  // upload
+  int rc = 0;
+#define UP(dst, vec)                                              \
+  do {                                                            \
+    rc = dev_upload(ctx, &dst, vec);                              \
+    if (rc) return rc;                                            \
+  } while (0)
+  int *d_nbr, *d_vtx, *d_vptr, *d_vcell, *d_vpair;
+  double *d[16];
+  UP(d_nbr, nbr);
+  UP(d_vtx, vtx);
+  const std::vector<double> *edge[9] = {&ctx->dlVV, &ctx->dlCC, &ctx->dle1, &ctx->dle2, &ctx->doe,
+                                        &ctx->xnn, &ctx->ynn, &ctx->dxCC, &ctx->dyCC};
+  for (int q = 0; q < 9; ++q) UP(d[q], edge_major(*edge[q], N0));
+  UP(d[9], ppec);
+  g.nbr = d_nbr;
+  g.vtx = d_vtx;
+  g.dlVV = d[0];
+  g.dlCC = d[1];
+  g.dle1 = d[2];
+  g.dle2 = d[3];
+  g.doe = d[4];
+  g.xnn = d[5];
+  g.ynn = d[6];
+  g.dxCC = d[7];
+  g.dyCC = d[8];
+  g.ppec = d[9];
+  double *a_area, *a_sx, *a_sy, *a_sxy, *a_sig, *a_dsig, *a_sigv, *a_dsigv, *a_walld, *a_vw, *a_dl;
+  UP(a_area, std::vector<double>(ctx->areaCell.begin(), ctx->areaCell.begin() + N0));
+  UP(a_sx, ctx->sum_xc2);
+  UP(a_sy, ctx->sum_yc2);
+  UP(a_sxy, ctx->sum_xcyc);
+  UP(a_sig, ctx->sig);
+  UP(a_dsig, ctx->dsig);
+  UP(a_sigv, ctx->sigv);
+  UP(a_dsigv, ctx->dsigv);
+  UP(a_walld, ctx->walld);
+  UP(d_vptr, vptr);
+  UP(d_vcell, vcell);
+  UP(a_vw, vw);
+  UP(a_dl, ctx->dlVsum);
+  UP(d_vpair, vpair);
+  g.area = a_area;
+  g.sum_xc2 = a_sx;
+  g.sum_yc2 = a_sy;
+  g.sum_xcyc = a_sxy;
+  g.sig = a_sig;
+  g.dsig = a_dsig;
+  g.sigv = a_sigv;
+  g.dsigv = a_dsigv;
+  g.walld = a_walld;
+  g.vptr = d_vptr;
+  g.vcell = d_vcell;
+  g.vw = a_vw;
+  g.dlVsum = a_dl;
+  g.vpair = d_vpair;
+  signed char *d_vkind, *d_act[4];
+  UP(d_vkind, vkind);
+  for (int q = 0; q < 4; ++q) UP(d_act[q], vact[q]);
+  g.vkind = d_vkind;
+  g.vact_u = d_act[0];
+  g.vact_v = d_act[1];
+  g.vact_w = d_act[2];
+  g.vact_p = d_act[3];
+#undef UP
+
+  // fields
+  const size_t nc = (size_t)N0 * NZ, nv = (size_t)NV * (NZ - 1);
+  auto cell = [&](double **p) { return dev_alloc(ctx, p, nc); };
+  for (int s = 0; s < 3; ++s)
+    for (int q = 0; q < 4; ++q)
+      if ((rc = cell(&ctx->pool_cell[s][q]))) return rc;
+  int ids_cell[] = {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY};
+  for (int id : ids_cell) {
+    if ((rc = cell(&ctx->slot[id]))) return rc;
+    ctx->kind[id] = FK_CELL;
+  }
+  for (int id = NSX_F_UFV; id <= NSX_F_PFV; ++id) {
+    if ((rc = dev_alloc(ctx, &ctx->slot[id], nv))) return rc;
+    ctx->kind[id] = FK_VERT;
+  }
+  for (int id = NSX_F_U1FACE; id <= NSX_F_UBFACE; ++id) {
+    if ((rc = cell(&ctx->slot[id]))) return rc;
+    ctx->kind[id] = FK_FACE;
+  }
+  for (int q = 0; q < NSX_N_CELL_SCRATCH; ++q) {
+    if ((rc = cell(&ctx->slot[NSX_F_CELL_SCRATCH + q]))) return rc;
+    ctx->kind[NSX_F_CELL_SCRATCH + q] = FK_CELL;
+  }
+  for (int q = 0; q < NSX_N_VERT_SCRATCH; ++q) {
+    if ((rc = dev_alloc(ctx, &ctx->slot[NSX_F_VERT_SCRATCH + q], nv))) return rc;
+    ctx->kind[NSX_F_VERT_SCRATCH + q] = FK_VERT;
+  }
+  for (int q = 0; q < NSX_N_FACE_SCRATCH; ++q) {
+    if ((rc = cell(&ctx->slot[NSX_F_FACE_SCRATCH + q]))) return rc;
+    ctx->kind[NSX_F_FACE_SCRATCH + q] = FK_FACE;
+  }
+  for (int q = 0; q < 3; ++q) {
+    if ((rc = cell(&ctx->ustar[q]))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->uv2[q], nv))) return rc;
+  }
+  if ((rc = cell(&ctx->rhsp)) || (rc = cell(&ctx->sr)) || (rc = cell(&ctx->srhat)) || (rc = cell(&ctx->sp[0])) ||
+      (rc = cell(&ctx->sp[1])) || (rc = cell(&ctx->svv[0])) || (rc = cell(&ctx->svv[1])) || (rc = cell(&ctx->ss)) ||
+      (rc = cell(&ctx->st_)) || (rc = cell(&ctx->W)) || (rc = cell(&ctx->invD)))
+    return rc;
+  // the time-level slots point into pool set 0 (n), 1 (stage), 2 (new)
+  ctx->set_n = 0;
+  for (int q = 0; q < 4; ++q) {
+    ctx->slot[NSX_F_UFN + q] = ctx->pool_cell[0][q];
+    ctx->slot[NSX_F_UF + q] = ctx->pool_cell[1][q];
+    ctx->slot[NSX_F_UFNP + q] = ctx->pool_cell[2][q];
+    ctx->kind[NSX_F_UFN + q] = ctx->kind[NSX_F_UF + q] = ctx->kind[NSX_F_UFNP + q] = FK_CELL;
+  }
+  // every field starts at zero, like the zero-filled work arrays of the reference
+  for (int st = 0; st < 3; ++st)
+    for (int q = 0; q < 4; ++q) HIPCHECK(hipMemsetAsync(ctx->pool_cell[st][q], 0, nc * sizeof(double), ctx->stream));
+  for (int id = 0; id < NSLOT; ++id)
+    if (ctx->slot[id] && ctx->kind[id] != FK_NONE && !(id >= NSX_F_UFN && id <= NSX_F_PFNP))
+      HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, field_elems(ctx, ctx->kind[id]) * sizeof(double), ctx->stream));
+  double *zs[] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2], ctx->rhsp, ctx->sr, ctx->srhat, ctx->sp[0],
+                  ctx->sp[1], ctx->svv[0], ctx->svv[1], ctx->ss, ctx->st_};
+  for (double *p : zs) HIPCHECK(hipMemsetAsync(p, 0, nc * sizeof(double), ctx->stream));
+  for (int q = 0; q < 3; ++q) HIPCHECK(hipMemsetAsync(ctx->uv2[q], 0, nv * sizeof(double), ctx->stream));
+
+  // solver state and reduction scratch
+  if ((rc = dev_alloc(ctx, &ctx->st_dev, 1))) return rc;
+  HIPCHECK(hipMemsetAsync(ctx->st_dev, 0, sizeof(SolverState), ctx->stream));
+  HIPCHECK(hipHostMalloc((void **)&ctx->st_host, sizeof(SolverState), hipHostMallocDefault));
+  if ((rc = dev_alloc(ctx, &ctx->part.buf, (size_t)8 * MAXGRID))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->part.ticket, 8))) return rc;
+  HIPCHECK(hipMemsetAsync(ctx->part.ticket, 0, 8 * sizeof(unsigned int), ctx->stream));
+
+  // compact PPE operator, and a device-side check of its diagonal
+  double *A0chk = nullptr;
+  if ((rc = cell(&A0chk))) return rc;
+  HIPCHECK(hipMemsetAsync(ctx->W, 0, nc * sizeof(double), ctx->stream));
+  HIPCHECK(hipMemsetAsync(ctx->invD, 0, nc * sizeof(double), ctx->stream));
+  LAUNCH(launch_ppe_build(g, ctx->stream, ctx->W, ctx->invD, A0chk));
+  std::vector<double> a0((size_t)nc);
+  HIPCHECK(hipMemcpyAsync(a0.data(), A0chk, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  for (int k = 1; k < NZ - 1; ++k)
+    for (int i = 0; i < N0; ++i)
+      if (!(a0[(size_t)i * NZ + k] == ctx->AmC[0][(size_t)i + (size_t)k * N0]))
+        return nsx_fail(ctx, -1, "device PPE diagonal differs from AmC0 at (%d,%d)", i + 1, k + 1);
+  ctx->committed = true;
+  // the host copies are no longer needed except the ghost map of nsx_get_field
+  for (int q = 0; q < 6; ++q) std::vector<double>().swap(ctx->AmC[q]);
+  return 0;
+}
+
+// ---------------------------------------------------------------- fields --
+static int check_field(nsx_ctx *ctx, int f) {
+  if (f < 0 || f >= NSLOT || !ctx->slot[f]) return nsx_fail(ctx, -1, "unknown field id %d", f);
+  return 0;
+}
+
+extern "C" int nsx_put_field(nsx_ctx *ctx, int f, const double *host) {
+  NEED_COMMIT(ctx);
+  if (check_field(ctx, f)) return -1;
+  const int N0 = ctx->N0, NZ = ctx->NZ, NV = ctx->NV;
+  std::vector<double> buf(field_elems(ctx, ctx->kind[f]));
+  if (ctx->kind[f] == FK_VERT) {
+    for (int l = 0; l < NZ - 1; ++l)
+      for (int v = 0; v < NV; ++v) buf[(size_t)v * (NZ - 1) + l] = host[(size_t)v + (size_t)l * NV];
+  } else {
+    const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)ctx->NC;
+    for (int k = 0; k < NZ; ++k)
+      for (int i = 0; i < N0; ++i) buf[(size_t)i * NZ + k] = host[(size_t)i + (size_t)k * ld];
+  }
+  HIPCHECK(hipMemcpyAsync(ctx->slot[f], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_get_field(nsx_ctx *ctx, int f, double *host, int fill_ghost_rows) {
+  NEED_COMMIT(ctx);
+  if (check_field(ctx, f)) return -1;
+  const int N0 = ctx->N0, NZ = ctx->NZ, NV = ctx->NV, NC = ctx->NC;
+  std::vector<double> buf(field_elems(ctx, ctx->kind[f]));
+  HIPCHECK(hipMemcpyAsync(buf.data(), ctx->slot[f], buf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (ctx->kind[f] == FK_VERT) {
+    for (int l = 0; l < NZ - 1; ++l)
+      for (int v = 0; v < NV; ++v) host[(size_t)v + (size_t)l * NV] = buf[(size_t)v * (NZ - 1) + l];
+    return 0;
+  }
+  const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)NC;
+  for (int k = 0; k < NZ; ++k)
+    for (int i = 0; i < N0; ++i) host[(size_t)i + (size_t)k * ld] = buf[(size_t)i * NZ + k];
+  if (fill_ghost_rows && ctx->kind[f] == FK_CELL)
+    for (int gh = N0; gh < ctx->NEX; ++gh) {
+      const int partner = ctx->No_cp[(size_t)gh + 2 * (size_t)NC] - 1;
+      if (partner < 0 || partner >= N0) continue;
+      for (int k = 0; k < NZ; ++k) host[(size_t)gh + (size_t)k * NC] = buf[(size_t)partner * NZ + k];
+    }
+  return 0;
+}
+
+extern "C" int nsx_copy_field(nsx_ctx *ctx, int dst, int src) {
+  NEED_COMMIT(ctx);
+  if (check_field(ctx, dst) || check_field(ctx, src)) return -1;
+  if (ctx->kind[dst] != ctx->kind[src]) return nsx_fail(ctx, -1, "field kinds differ");
+  HIPCHECK(hipMemcpyAsync(ctx->slot[dst], ctx->slot[src], field_elems(ctx, ctx->kind[src]) * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// -------------------------------------------------------------- timing ----
+static hipEvent_t *events(nsx_ctx *ctx, int n) {
+  while (ctx->evpool.size() < ctx->evused + n) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->evpool.push_back(e);
+  }
+  hipEvent_t *p = &ctx->evpool[ctx->evused];
+  ctx->evused += n;
+  return p;
+}
+
+static void collect_timing(nsx_ctx *ctx) {
+  if (!ctx->timing) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  Timer *t[4] = {&ctx->t_a, &ctx->t_b, &ctx->t_c, &ctx->t_stage};
+  for (auto &pr : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
+      t[pr.second]->ms += ms;
+      t[pr.second]->n += 1;
+    }
+  }
+  ctx->pending.clear();
+  ctx->evused = 0;
+}
+
+// ---------------------------------------------------------- the solver ----
+// bicgstab.F90:8-168: Glob_Norm(rhs,1) [sum done by the caller into
+// st->avec_rhs], BICGSTAB_SOLVE, Glob_Norm(phi,0) and the ghost layers of p.
+static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  PPEOp op{ctx->W, ctx->invD};
+  LAUNCH(launch_bicg_init(g, s, op, rhs, phi_in, x, ctx->sr, ctx->srhat, ctx->part, ctx->st_dev, ctx->eps,
+                          ctx->MaxIters));
+  int issued = 0, parity = 0;
+  const size_t pend0 = ctx->pending.size();
+  // queue about as many iterations as the previous solve needed, then top up
+  // in small chunks: few host round trips and few no-op launches
+  const int hint = ctx->iter_hint;
+  int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      hipEvent_t *ev = nullptr;
+      if (ctx->timing) {
+        ev = events(ctx, 6);
+        if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+      }
+      LAUNCH(launch_bicg_iter(g, s, op, parity, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->svv[0], ctx->sp[1],
+                              ctx->svv[1], ctx->ss, ctx->st_, ctx->part, ctx->st_dev, ev));
+      if (ev) {
+        const size_t base = (size_t)(ev - &ctx->evpool[0]);
+        ctx->pending.push_back({base + 0, 0});
+        ctx->pending.push_back({base + 2, 1});
+        ctx->pending.push_back({base + 4, 2});
+      }
+      parity ^= 1;
+      ++issued;
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (ctx->st_host->done) break;
+    if (issued > ctx->MaxIters + 4) return nsx_fail(ctx, -4, "solver did not stop after %d iterations", issued);
+    chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
+  }
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  const SolverState &h = *ctx->st_host;
+  ctx->info.iters = h.iter;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.e1;
+  ctx->info.e2norm = h.e2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += h.iter;
+  ctx->iter_hint = h.iter > 0 ? h.iter : ctx->iter_hint;
+  // time only the iterations that ran, not the queued launches that found
+  // the solver finished
+  if (ctx->timing && ctx->pending.size() > pend0 + 3 * (size_t)h.iter) ctx->pending.resize(pend0 + 3 * (size_t)h.iter);
+  collect_timing(ctx);
+  return 0;
+}
+
+// vertex values of up to three fields; kinds 0 p, 1 u, 2 v, 3 w
+static int vertex_of(nsx_ctx *ctx, int nf, const double *const f[3], double *const fv[3], const int kinds[3],
+                     int tagdir) {
+  const Geo &g = ctx->g;
+  const signed char *act[3];
+  int zb[3], zt[3];
+  for (int q = 0; q < nf; ++q) {
+    const int kq = kinds[q];
+    act[q] = kq == 0 ? g.vact_p : (kq == 1 ? g.vact_u : (kq == 2 ? g.vact_v : g.vact_w));
+    // BCvelvertex3D zeroes the wall levels for tagdir 3 (BC.F90:737-787); BCpvertex does not
+    zb[q] = kq == 0 ? 0 : (g.bcbot == 1 && (g.BotBC == 0 || (g.BotBC == 1 && kq == 3)));
+    zt[q] = kq == 0 ? 0 : (g.bctop == 1 && (g.TopBC == 0 || (g.TopBC == 1 && kq == 3)));
+  }
+  LAUNCH(launch_vertex(g, ctx->stream, nf, f, fv, act, zb, zt, tagdir));
+  return 0;
+}
+
+// ---------------------------------------------------------------- stage ---
+// hydro.F90:8-501 with inputs from pool set `in` and outputs into set `out`
+static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  const double c1 = flag_ab == 1 ? 1.55 : 1.0, c2 = flag_ab == 1 ? -0.55 : 0.0;
+  double *const *X = ctx->pool_cell[in];
+  double *const *Y = ctx->pool_cell[out];
+  hipEvent_t *ev = nullptr;
+  if (ctx->timing) {
+    ev = events(ctx, 4);
+    if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+    (void)hipEventRecord(ev[0], s);
+  }
+  const double *uvw[3] = {X[0], X[1], X[2]};
+  const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
+  // vertex values of the stage velocity (tagdir 2), shared by sgles and AdvDiffNS
+  int rc = vertex_of(ctx, 3, uvw, ctx->uv2, vk, 2);
+  if (rc) return rc;
+  if (ctx->Iturbu == 1)
+    LAUNCH(launch_les(g, s, X[0], X[1], X[2], ctx->uv2[0], ctx->uv2[1], ctx->uv2[2], ctx->slot[NSX_F_EDDY]));
+  const double *vtx[3] = {ctx->uv2[0], ctx->uv2[1], ctx->uv2[2]};
+  const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                           ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  double *hist[3] = {ctx->slot[NSX_F_RHSUF], ctx->slot[NSX_F_RHSVF], ctx->slot[NSX_F_RHSWF]};
+  LAUNCH(launch_momentum(g, s, uvw, vtx, face, ctx->slot[NSX_F_EDDY], nullptr, hist, nullptr, ctx->ustar, c1, c2));
+  // update_rhsp's BCglobalP(pf, pfv, 2)
+  const double *pin[3] = {X[3], nullptr, nullptr};
+  double *pfv[3] = {ctx->slot[NSX_F_PFV], nullptr, nullptr};
+  if ((rc = vertex_of(ctx, 1, pin, pfv, pk, 2))) return rc;
+  const double *us[3] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2]};
+  double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                      ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  LAUNCH(launch_source(g, s, us, facew, nullptr, ctx->rhsp, ctx->slot[NSX_F_PFV], ctx->part, ctx->st_dev));
+  if (ev) {
+    (void)hipEventRecord(ev[1], s);
+    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 3});
+  }
+  if ((rc = solve(ctx, ctx->rhsp, X[3], Y[3]))) return rc;
+  if (ctx->timing) {
+    ev = events(ctx, 2);
+    (void)hipEventRecord(ev[0], s);
+  }
+  double *unp[3] = {Y[0], Y[1], Y[2]};
+  LAUNCH(launch_corrector(g, s, Y[3], us, unp));
+  const double *un[3] = {Y[0], Y[1], Y[2]};
+  double *uvf[3] = {ctx->slot[NSX_F_UFV], ctx->slot[NSX_F_VFV], ctx->slot[NSX_F_WFV]};
+  if ((rc = vertex_of(ctx, 3, un, uvf, vk, 3))) return rc;
+  const double *pout[3] = {Y[3], nullptr, nullptr};
+  if ((rc = vertex_of(ctx, 1, pout, pfv, pk, 2))) return rc;
+  LAUNCH(launch_newu(g, s, Y[3], ctx->slot[NSX_F_PFV], facew));
+  if (ev) {
+    (void)hipEventRecord(ev[1], s);
+    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 3});
+    collect_timing(ctx);
+  }
+  return 0;
+}
+
+static void point_slots(nsx_ctx *ctx, int n, int stage_in, int newest) {
+  for (int q = 0; q < 4; ++q) {
+    ctx->slot[NSX_F_UFN + q] = ctx->pool_cell[n][q];
+    ctx->slot[NSX_F_UF + q] = ctx->pool_cell[stage_in][q];
+    ctx->slot[NSX_F_UFNP + q] = ctx->pool_cell[newest][q];
+  }
+}
+
+extern "C" int nsx_init_state(nsx_ctx *ctx) {
+  NEED_COMMIT(ctx);
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  // the UFN..PFN slots hold the initial field; make them pool set 0
+  int n = -1;
+  for (int st = 0; st < 3; ++st)
+    if (ctx->slot[NSX_F_UFN] == ctx->pool_cell[st][0]) n = st;
+  if (n < 0) return nsx_fail(ctx, -1, "time-level slots are inconsistent");
+  double *const *X = ctx->pool_cell[n];
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
+  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
+  const double *uvw[3] = {X[0], X[1], X[2]};
+  const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
+  double *uvf[3] = {ctx->slot[NSX_F_UFV], ctx->slot[NSX_F_VFV], ctx->slot[NSX_F_WFV]};
+  int rc = vertex_of(ctx, 3, uvw, uvf, vk, 3);
+  if (rc) return rc;
+  const double *pin[3] = {X[3], nullptr, nullptr};
+  double *pfv[3] = {ctx->slot[NSX_F_PFV], nullptr, nullptr};
+  if ((rc = vertex_of(ctx, 1, pin, pfv, pk, 3))) return rc;
+  double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                      ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  LAUNCH(launch_source(g, s, uvw, facew, nullptr, nullptr, nullptr, ctx->part, ctx->st_dev));
+  const size_t nc = (size_t)ctx->N0 * ctx->NZ;
+  for (int id : {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY})
+    HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, nc * sizeof(double), s));
+  // LoopTimeInitial: (.)np <- (.)n
+  const int a = (n + 1) % 3, b = (n + 2) % 3;
+  for (int q = 0; q < 4; ++q)
+    HIPCHECK(hipMemcpyAsync(ctx->pool_cell[b][q], X[q], nc * sizeof(double), hipMemcpyDeviceToDevice, s));
+  point_slots(ctx, n, a, b);
+  ctx->set_n = n;
+  ctx->info = nsx_solve_info{};
+  HIPCHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int nsx_hydro(nsx_ctx *ctx, int flag_ab, nsx_solve_info *info) {
+  NEED_COMMIT(ctx);
+  // UF.. -> UFNP..: find their pool sets
+  int in = -1, out = -1;
+  for (int st = 0; st < 3; ++st) {
+    if (ctx->slot[NSX_F_UF] == ctx->pool_cell[st][0]) in = st;
+    if (ctx->slot[NSX_F_UFNP] == ctx->pool_cell[st][0]) out = st;
+  }
+  if (in < 0 || out < 0 || in == out) return nsx_fail(ctx, -1, "stage input and output must be distinct fields");
+  int rc = stage(ctx, in, out, flag_ab);
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (info) *info = ctx->info;
+  return ctx->info.exit_code == NSX_SOLVE_MAX_ITERS ? NSX_SOLVE_MAX_ITERS : 0;
+}
+
+extern "C" int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info *info) {
+  NEED_COMMIT(ctx);
+  if (precision != NSX_PREC_FP64) return nsx_fail(ctx, -1, "precision %d is not available in this build", precision);
+  hipStream_t s = ctx->stream;
+  int worst = 0;
+  for (int n = 0; n < nsteps; ++n) {
+    const int N = ctx->set_n, A = (N + 1) % 3, B = (N + 2) % 3;
+    int rc;
+    if (ctx->FinRK == 2) {
+      if ((rc = stage(ctx, N, A, 0))) return rc;
+      if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
+      if ((rc = stage(ctx, A, B, 0))) return rc;
+      if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
+      const double *un[3] = {ctx->pool_cell[N][0], ctx->pool_cell[N][1], ctx->pool_cell[N][2]};
+      double *unp[3] = {ctx->pool_cell[B][0], ctx->pool_cell[B][1], ctx->pool_cell[B][2]};
+      LAUNCH(launch_rk2avg(ctx->g, s, un, unp));
+      ctx->set_n = B;
+      point_slots(ctx, B, A, B);
+    } else {
+      // Euler, then AB2 from the second step on (hydro.F90:469-478)
+      const int flag_ab = ctx->info.n_solves > 0 ? 1 : 0;
+      if ((rc = stage(ctx, N, A, flag_ab))) return rc;
+      if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
+      ctx->set_n = A;
+      point_slots(ctx, A, N, A);
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  if (info) *info = ctx->info;
+  return worst;
+}
+
+extern "C" int nsx_solve_info_get(nsx_ctx *ctx, nsx_solve_info *info) {
+  if (!ctx || !info) return -1;
+  *info = ctx->info;
+  return 0;
+}
+
+// ------------------------------------------------------ single routines ---
+static int kind_ok(nsx_ctx *ctx, int id, int kind) {
+  if (check_field(ctx, id)) return -1;
+  if (ctx->kind[id] != kind) return nsx_fail(ctx, -1, "field %d has the wrong kind for this argument", id);
+  return 0;
+}
+
+extern "C" int nsx_bcglobalvc(nsx_ctx *ctx, int u, int v, int w, int uv, int vv, int wv, int tagdir) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL) || kind_ok(ctx, uv, FK_VERT) ||
+      kind_ok(ctx, vv, FK_VERT) || kind_ok(ctx, wv, FK_VERT))
+    return -1;
+  if (tagdir != 2 && tagdir != 3) return nsx_fail(ctx, -1, "tagdir must be 2 or 3");
+  const int ids[3] = {u, v, w};
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[q]], q + 1));
+  const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
+  double *fv[3] = {ctx->slot[uv], ctx->slot[vv], ctx->slot[wv]};
+  const int vk[3] = {1, 2, 3};
+  int rc = vertex_of(ctx, 3, f, fv, vk, tagdir);
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+static int bcglobalp_(nsx_ctx *ctx, int p, int pv, int tagdir) {
+  LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[p], 0));
+  const double *f[3] = {ctx->slot[p], nullptr, nullptr};
+  double *fv[3] = {ctx->slot[pv], nullptr, nullptr};
+  const int pk[3] = {0, 0, 0};
+  return vertex_of(ctx, 1, f, fv, pk, tagdir);
+}
+
+extern "C" int nsx_bcglobalp(nsx_ctx *ctx, int p, int pv, int tagdir) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, p, FK_CELL) || kind_ok(ctx, pv, FK_VERT)) return -1;
+  if (tagdir != 2 && tagdir != 3) return nsx_fail(ctx, -1, "tagdir must be 2 or 3");
+  int rc = bcglobalp_(ctx, p, pv, tagdir);
+  if (rc) return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_grad_lsm(nsx_ctx *ctx, int dfdx, int dfdy, int dfds, int f) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, dfdx, FK_CELL) || kind_ok(ctx, dfdy, FK_CELL) || kind_ok(ctx, dfds, FK_CELL) ||
+      kind_ok(ctx, f, FK_CELL))
+    return -1;
+  LAUNCH(launch_grad_lsm(ctx->g, ctx->stream, ctx->slot[f], ctx->slot[dfdx], ctx->slot[dfdy], ctx->slot[dfds]));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_calcul_face(nsx_ctx *ctx, int f1, int f2, int f3, int fT, int fB, int fun, int funv) {
+  NEED_COMMIT(ctx);
+  const int o[5] = {f1, f2, f3, fT, fB};
+  for (int q = 0; q < 5; ++q)
+    if (kind_ok(ctx, o[q], FK_FACE)) return -1;
+  if (kind_ok(ctx, fun, FK_CELL) || kind_ok(ctx, funv, FK_VERT)) return -1;
+  double *out[5] = {ctx->slot[f1], ctx->slot[f2], ctx->slot[f3], ctx->slot[fT], ctx->slot[fB]};
+  LAUNCH(launch_calcul_face(ctx->g, ctx->stream, ctx->slot[fun], ctx->slot[funv], out));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_sgles(nsx_ctx *ctx, int u, int v, int w) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL)) return -1;
+  const int ids[3] = {u, v, w};
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[q]], q + 1));
+  const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
+  const int vk[3] = {1, 2, 3};
+  int rc = vertex_of(ctx, 3, f, ctx->uv2, vk, 2);
+  if (rc) return rc;
+  LAUNCH(launch_les(ctx->g, ctx->stream, f[0], f[1], f[2], ctx->uv2[0], ctx->uv2[1], ctx->uv2[2],
+                    ctx->slot[NSX_F_EDDY]));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_advdiff(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u, int v, int w, int eddy) {
+  NEED_COMMIT(ctx);
+  const int ids[7] = {rhsu, rhsv, rhsw, u, v, w, eddy};
+  for (int q = 0; q < 7; ++q)
+    if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[3 + q]], q + 1));
+  const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
+  const int vk[3] = {1, 2, 3};
+  int rc = vertex_of(ctx, 3, f, ctx->uv2, vk, 2);
+  if (rc) return rc;
+  const double *vtx[3] = {ctx->uv2[0], ctx->uv2[1], ctx->uv2[2]};
+  const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                           ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  const double *rin[3] = {ctx->slot[rhsu], ctx->slot[rhsv], ctx->slot[rhsw]};
+  double *rout[3] = {ctx->slot[rhsu], ctx->slot[rhsv], ctx->slot[rhsw]};
+  LAUNCH(launch_momentum(ctx->g, ctx->stream, f, vtx, face, ctx->slot[eddy], rin, nullptr, rout, nullptr, 1.0, 0.0));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w) {
+  NEED_COMMIT(ctx);
+  const int ids[4] = {src, u, v, w};
+  for (int q = 0; q < 4; ++q)
+    if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[1 + q]], q + 1));
+  const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
+  double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                      ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  HIPCHECK(hipMemsetAsync(ctx->slot[src], 0, (size_t)ctx->N0 * ctx->NZ * sizeof(double), ctx->stream));
+  LAUNCH(launch_source(ctx->g, ctx->stream, f, facew, ctx->slot[src], nullptr, nullptr, ctx->part, ctx->st_dev));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_update_rhsp(nsx_ctx *ctx, int rhs, int p, int pv) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, rhs, FK_CELL) || kind_ok(ctx, p, FK_CELL) || kind_ok(ctx, pv, FK_VERT)) return -1;
+  int rc = bcglobalp_(ctx, p, pv, 2);
+  if (rc) return rc;
+  LAUNCH(launch_update_rhsp(ctx->g, ctx->stream, ctx->slot[rhs], ctx->slot[pv]));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, phi, FK_CELL) || kind_ok(ctx, phiv, FK_VERT) || kind_ok(ctx, rhs, FK_CELL)) return -1;
+  LAUNCH(launch_gnorm_sum(ctx->g, ctx->stream, ctx->slot[rhs], ctx->part, ctx->st_dev));
+  int rc = solve(ctx, ctx->slot[rhs], ctx->slot[phi], ctx->slot[phi]);
+  if (rc) return rc;
+  if ((rc = bcglobalp_(ctx, phi, phiv, 3))) return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (info) *info = ctx->info;
+  return ctx->info.exit_code == NSX_SOLVE_CONVERGED ? 0 : ctx->info.exit_code;
+}
+
+extern "C" int nsx_newu3d(nsx_ctx *ctx, int p, int pv) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, p, FK_CELL) || kind_ok(ctx, pv, FK_VERT)) return -1;
+  int rc = bcglobalp_(ctx, p, pv, 2);
+  if (rc) return rc;
+  double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                      ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  LAUNCH(launch_newu(ctx->g, ctx->stream, ctx->slot[p], ctx->slot[pv], facew));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------- timing --
+extern "C" int nsx_timing_enable(nsx_ctx *ctx, int on) {
+  if (!ctx) return -1;
+  ctx->timing = on != 0;
+  return 0;
+}
+
+extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches) {
+  if (!ctx || !name) return -1;
+  const Timer *t = nullptr;
+  if (!strcmp(name, "bicgstab_a")) t = &ctx->t_a;
+  else if (!strcmp(name, "bicgstab_b")) t = &ctx->t_b;
+  else if (!strcmp(name, "bicgstab_c")) t = &ctx->t_c;
+  else if (!strcmp(name, "stage")) t = &ctx->t_stage;
+  if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
+  if (total_ms) *total_ms = t->ms;
+  if (launches) *launches = t->n;
+  return 0;
+}
+
+extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
+  if (!ctx) return -1;
+  ctx->t_a = ctx->t_b = ctx->t_c = ctx->t_stage = Timer{};
+  return 0;
+}

@@ -1,0 +1,179 @@
+// nsx_dev.h -- device-side data layout and helpers of the nsx HIP library.
+//
+// Layout in HBM (gfx950):
+//   cell field   double[N0][NZ]   column-contiguous: entry (i,k) at i*NZ+k; the
+//                                 ghost layers k=0 and k=NZ-1 are stored, the
+//                                 horizontal (periodic) ghost cells are not: a
+//                                 neighbour index nbr[i*3+j] already points at
+//                                 the periodic partner column;
+//   vertex field double[NV][NZ-1] entry (v,l) at v*(NZ-1)+l;
+//   face field   double[N0][NZ]   U1FACE..UBFACE, same indexing as cells.
+// A wavefront walks consecutive k of one column (then the next column), so
+// every access to the cell itself, to its vertical neighbours and to a
+// horizontal neighbour column is a contiguous run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NSX_BLOCK 256
+#define NSX_NXCD 8
+
+// n / d for 32-bit n and a run-time divisor (Granlund-Montgomery, round-up
+// variant with a 33-bit intermediate).
+struct FastDiv {
+  uint32_t d, m, l;
+};
+
+static inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  f.l = l;
+  f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+  return f;
+}
+
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t t = __umulhi(f.m, n);
+#else
+  uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+#endif
+  return (uint32_t)(((uint64_t)t + n) >> f.l);
+}
+
+// Vertex boundary action per field: what BCvelvertex3D / BCpvertex and
+// bc_periodic do to a vertex value (BC.F90:603-1408).
+enum : signed char { VA_NONE = 0, VA_ZERO = 1, VA_PERIODIC = 3 };
+// Periodic vertex kind (vertex_edge / vertex_pair of calcul_geometry.F90:920-1025)
+enum : signed char { VK_INTERIOR = 0, VK_PAIR = 1, VK_CORNER = 2, VK_OTHER = 3 };
+
+struct Geo {
+  int N0, NV, NZ, NZi, NL;  // NZi = NZ-2 interior layers, NL = NZ-1 vertex levels
+  uint32_t ncell;           // N0*NZi interior prisms
+  FastDiv divNZi, divNL;
+  // per cell-edge (N0*3, edge-fastest: index i*3+j)
+  const int *nbr;           // neighbour column across edge j (periodic partner resolved), 0-based
+  const int *vtx;           // No_vp(i,j), 0-based
+  const double *dlVV, *dlCC, *dle1, *dle2, *doe, *xnn, *ynn, *dxCC, *dyCC;
+  const double *ppec;       // (dle1/dlCC)*dlVV: PPE edge coefficient before the dsigv factor
+  // per cell (N0)
+  const double *area, *sum_xc2, *sum_yc2, *sum_xcyc;
+  // per level
+  const double *sig, *dsig;  // NZ
+  const double *sigv, *dsigv;  // NZ-1
+  const double *walld;       // NZ
+  // vertices: inverse-distance stencil in CSR form (interior cells, reference order)
+  const int *vptr, *vcell;
+  const double *vw, *dlVsum;
+  const int *vpair;          // 0-based partner or -1
+  const signed char *vkind;  // VK_*
+  const signed char *vact_u, *vact_v, *vact_w, *vact_p;  // VA_* per vertex
+  int corner[4], ncorner;
+  int bctop, bcbot, TopBC, BotBC;
+  double TotalVolume, sigIni, sigFin;
+  double dt, nu, bdx, bdy, bdz;
+};
+
+// velocity ghost layer rule of BCvelcenter3D (BC.F90:459-527) for ZPB = 0:
+// +1 symmetric copy, -1 antisymmetric copy, 0 no fill
+__host__ __device__ inline int vel_ghost_sign(int bcflag, int wallbc, int comp /*0 u,1 v,2 w*/) {
+  if (bcflag != 1) return 0;
+  if (wallbc == 0) return -1;
+  if (wallbc == 1) return comp == 2 ? -1 : 1;
+  return 0;
+}
+
+// ------------------------------------------------------------ iteration ---
+// Interior prisms are numbered c = col*NZi + (k-1).  A launch uses a fixed
+// grid; block b works on one contiguous range of c.  Blocks are dealt
+// round-robin to the 8 XCDs, so the logical block index is remapped to give
+// each XCD (and its L2) one contiguous slice of the columns.
+__device__ inline void block_range(uint32_t n, uint32_t &beg, uint32_t &end) {
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  uint32_t lb = b;
+  if ((G % NSX_NXCD) == 0) lb = (b % NSX_NXCD) * (G / NSX_NXCD) + b / NSX_NXCD;
+  uint64_t chunk = ((uint64_t)n + G - 1) / G;
+  chunk = (chunk + NSX_BLOCK - 1) / NSX_BLOCK * NSX_BLOCK;
+  uint64_t b0 = (uint64_t)lb * chunk, b1 = b0 + chunk;
+  beg = (uint32_t)(b0 < n ? b0 : n);
+  end = (uint32_t)(b1 < n ? b1 : n);
+}
+
+__device__ inline void cell_of(const Geo &g, uint32_t c, int &col, int &k) {
+  uint32_t q = fdiv(c, g.divNZi);
+  col = (int)q;
+  k = (int)(c - q * (uint32_t)g.NZi) + 1;
+}
+
+// ---------------------------------------------------------- reductions ---
+// Deterministic grid reduction: every block writes its partial sums to
+// partials[q*G + block]; the last block to finish (atomic ticket) sums the
+// partials in block order.  Same inputs -> same bits, run after run.
+template <int K>
+__device__ inline void wave_sum(double (&v)[K]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] += __shfl_down(v[q], off, 64);
+}
+
+template <int K>
+__device__ inline bool block_partials_and_ticket(double (&v)[K], double *partials, unsigned int *ticket) {
+  __shared__ double sh[K][NSX_BLOCK / 64];
+  __shared__ bool last;
+  wave_sum<K>(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < K; ++q) sh[q][wid] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[q][w];
+      partials[(size_t)q * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    unsigned int prev = atomicAdd(ticket, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  return last;
+}
+
+// Called by all threads of the last block: totals of the K partial rows.
+template <int K>
+__device__ inline void final_sums(double (&tot)[K], const double *partials, unsigned int *ticket) {
+  __threadfence();
+  double v[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += NSX_BLOCK)
+      s += partials[(size_t)q * gridDim.x + b];
+    v[q] = s;
+  }
+  __shared__ double sh[K][NSX_BLOCK / 64];
+  wave_sum<K>(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < K; ++q) sh[q][wid] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[q][w];
+    tot[q] = s;
+  }
+  if (threadIdx.x == 0) *ticket = 0u;
+}
+
+#define HIPCHECK(x)                                                     \
+  do {                                                                  \
+    hipError_t e_ = (x);                                                \
+    if (e_ != hipSuccess) return nsx_fail_hip(ctx, e_, #x, __LINE__);   \
+  } while (0)

@@ -1,0 +1,60 @@
+// nsx_kernels.h -- launchers of the stage kernels (nsx_stage.hip) and of the
+// pressure solver (nsx_bicgstab.hip), called by the context (nsx_context.hip).
+#pragma once
+#include "nsx_dev.h"
+
+struct Partials {
+  double *buf;            // [8][maxgrid]
+  unsigned int *ticket;   // one per reduction site
+};
+
+// BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals)
+struct SolverState {
+  double rho, rhokm, alpha, omega;
+  double b1, b2, e1, e2;
+  double avec_rhs;  // Glob_Norm(rhs,1) average
+  double xvol;      // sum x*Vol of the current iterate (Glob_Norm(phi,0))
+  int iter, done, exit_code, max_iters;
+  double eps;
+  int iters_cap;    // iterations the host allowed so far (host-driven chunks)
+  int pad;
+};
+
+// Compact PPE operator: AmC_j(i,k) = ppec(i,j)*dsigv(k-1), AmCT(i,k) = W(i,k),
+// AmCB(i,k) = W(i,k-1), AmC0 = -(AmCT+AmCB+AmC1+AmC2+AmC3): the coefficients of
+// PDE_center regenerated on the fly, bit for bit; invD = 1/AmC0.
+struct PPEOp {
+  const double *W;     // [N0][NZ]
+  const double *invD;  // [N0][NZ]
+};
+
+int nsx_grid_for(uint32_t n);
+
+// ---- stage kernels (stream-ordered launches; return hipError_t as int)
+int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
+                  const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir);
+int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
+               const double *vv, const double *wv, double *eddy);
+int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                    const double *const face[5], const double *eddy, const double *const rhs_in[3],
+                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2);
+int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
+                  double *rhsp_out, const double *pv, Partials part, SolverState *st);
+int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *pv);
+int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, double *dy, double *ds);
+int launch_calcul_face(const Geo &g, hipStream_t s, const double *f, const double *fv, double *const out[5]);
+int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double *const us[3],
+                     double *const unp[3]);
+int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, double *const face[5]);
+int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]);
+int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 1 u, 2 v, 3 w*/);
+int launch_ppe_build(const Geo &g, hipStream_t s, double *W, double *invD, double *A0_check);
+
+// ---- solver
+int launch_bicg_init(const Geo &g, hipStream_t s, PPEOp op, const double *rhs, const double *phi_in, double *x,
+                     double *r, double *rhat, Partials part, SolverState *st, double eps, int max_iters);
+int launch_bicg_iter(const Geo &g, hipStream_t s, PPEOp op, int parity, double *x, double *r, const double *rhat,
+                     double *p0, double *v0, double *p1, double *v1, double *sv, double *tv, Partials part,
+                     SolverState *st, hipEvent_t *ev /* nullable: 6 events, start/end of phases A, B, C */);
+int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
+int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part, SolverState *st);

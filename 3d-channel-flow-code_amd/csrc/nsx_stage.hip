@@ -1,0 +1,615 @@
+// nsx_stage.hip -- the non-solver kernels of one projection stage (hydro.F90).
+//
+// Each kernel states the reference routine(s) it replaces.  Arithmetic keeps
+// the reference's operation order and is compiled without FMA contraction, so
+// every kernel here except the Smagorinsky cube root reproduces the CPU
+// results bit for bit; the cube root uses the device pow.
+//
+// HBM roofline: all of these are streaming stencils at < 1 flop/byte, no
+// MFMA; per-kernel algorithmic bytes are listed next to each kernel.
+#include "nsx_kernels.h"
+
+int nsx_grid_for(uint32_t n) {
+  uint32_t blocks = (n + NSX_BLOCK - 1) / NSX_BLOCK;
+  uint32_t g = blocks < 2048 ? blocks : 2048;
+  g = (g + NSX_NXCD - 1) / NSX_NXCD * NSX_NXCD;
+  return g ? (int)g : NSX_NXCD;
+}
+
+#define FOR_EACH_CELL(g, c)                                                         \
+  uint32_t beg_, end_;                                                              \
+  block_range((g).ncell, beg_, end_);                                               \
+  for (uint32_t base_ = beg_; base_ < end_; base_ += NSX_BLOCK)                     \
+    for (uint32_t c = base_ + threadIdx.x, once_ = 1; once_ && c < end_; once_ = 0)
+
+// ------------------------------------------------------- cell -> vertex ----
+// interpolation2D / interpolation3D(P) + BCvelvertex3D / BCpvertex +
+// bc_periodic (interpolation2D.F90:8-150, interpolation3D.F90:8-444,
+// BC.F90:603-1408).  One thread per (vertex, level); the periodic average of
+// a boundary vertex recomputes its partners' values in-thread.
+// Bytes: NF fields x (read ~1 cell field + write 0.5 vertex field).
+struct VtxArgs {
+  const double *f[3];
+  double *fv[3];
+  const signed char *act[3];
+  int zero_bot[3], zero_top[3];
+  int tagdir;
+};
+
+__device__ inline double idw(const Geo &g, int v, int layer, const double *f) {
+  double s = 0.0;
+  const int b = g.vptr[v], e = g.vptr[v + 1];
+  for (int j = b; j < e; ++j) s = s + g.vw[j] * f[(size_t)g.vcell[j] * g.NZ + layer];
+  return s / g.dlVsum[v];
+}
+
+template <int NF>
+__global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
+  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  if (t >= (uint32_t)g.NV * g.NL) return;
+  const int v = (int)fdiv(t, g.divNL);
+  const int l = (int)(t - (uint32_t)v * g.NL);
+  double dzT = 0.0, dzB = 0.0;
+  if (a.tagdir == 3) {
+    dzT = fabs(g.sigv[l] - g.sig[l + 1]);
+    dzB = fabs(g.sigv[l] - g.sig[l]);
+  }
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const double *f = a.f[q];
+    const signed char *act = a.act[q];
+    const bool zero_wall = (a.tagdir == 3) && ((l == 0 && a.zero_bot[q]) || (l == g.NL - 1 && a.zero_top[q]));
+    auto val = [&](int vv) -> double {
+      double r;
+      if (a.tagdir == 3)
+        r = (dzB * idw(g, vv, l + 1, f) + dzT * idw(g, vv, l, f)) / (dzT + dzB);
+      else
+        r = idw(g, vv, l, f);
+      if (act[vv] == VA_ZERO) r = 0.;
+      if (zero_wall) r = 0.;
+      return r;
+    };
+    double mine = val(v);
+    if (act[v] == VA_PERIODIC) {
+      double o = 0.0;
+      const signed char kind = g.vkind[v];
+      if (kind == VK_CORNER) {
+        double sd = 0.0;
+        for (int q4 = 0; q4 < g.ncorner; ++q4) {
+          const int c = g.corner[q4];
+          o = o + val(c) * g.dlVsum[c];
+          sd = sd + g.dlVsum[c];
+        }
+        o = o / sd;
+      } else if (kind == VK_PAIR) {
+        const int p = g.vpair[v];
+        const double sd = g.dlVsum[v] + g.dlVsum[p];
+        o = g.dlVsum[v] * mine + g.dlVsum[p] * val(p);
+        o = o / sd;
+      }
+      mine = o;
+    }
+    a.fv[q][(size_t)v * g.NL + l] = mine;
+  }
+}
+
+int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
+                  const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir) {
+  VtxArgs a;
+  for (int q = 0; q < 3; ++q) {
+    a.f[q] = q < nf ? f[q] : nullptr;
+    a.fv[q] = q < nf ? fv[q] : nullptr;
+    a.act[q] = q < nf ? act[q] : nullptr;
+    a.zero_bot[q] = q < nf ? zero_bot[q] : 0;
+    a.zero_top[q] = q < nf ? zero_top[q] : 0;
+  }
+  a.tagdir = tagdir;
+  const uint32_t n = (uint32_t)g.NV * g.NL;
+  const dim3 grid((n + NSX_BLOCK - 1) / NSX_BLOCK);
+  if (nf == 3) hipLaunchKernelGGL(k_vertex<3>, grid, dim3(NSX_BLOCK), 0, s, g, a);
+  else hipLaunchKernelGGL(k_vertex<1>, grid, dim3(NSX_BLOCK), 0, s, g, a);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------ face values --
+// calcul_face, Kim-Choi branch (interpolationFace.F90:85-111):
+// phi_f = 0.5 phi_nb + 0.5 phi_i + doe (phiv(nv2) - phiv(nv1)) / dlVV
+__device__ inline void faces_of(const Geo &g, int i, int k, size_t s, const double *f, const double *fv,
+                                double (&out)[5]) {
+  const double fi = f[s];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int e = i * 3 + j;
+    const int nb = g.nbr[e];
+    const int nv1 = g.vtx[e], nv2 = g.vtx[i * 3 + (j + 1) % 3];
+    const double extra = g.doe[e] * (fv[(size_t)nv2 * g.NL + k] - fv[(size_t)nv1 * g.NL + k]) / g.dlVV[e];
+    out[j] = f[(size_t)nb * g.NZ + k] * 0.5 + fi * 0.5 + extra;
+  }
+  out[3] = 0.5 * (fi + f[s + 1]);
+  out[4] = 0.5 * (fi + f[s - 1]);
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_calcul_face(Geo g, const double *f, const double *fv, double *o0,
+                                                            double *o1, double *o2, double *o3, double *o4) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    double fc[5];
+    faces_of(g, i, k, s, f, fv, fc);
+    o0[s] = fc[0];
+    o1[s] = fc[1];
+    o2[s] = fc[2];
+    o3[s] = fc[3];
+    o4[s] = fc[4];
+  }
+}
+
+int launch_calcul_face(const Geo &g, hipStream_t s, const double *f, const double *fv, double *const out[5]) {
+  hipLaunchKernelGGL(k_calcul_face, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, f, fv, out[0], out[1],
+                     out[2], out[3], out[4]);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------ Smagorinsky --
+// sgles + fvm_gradient + sgdamp (sgm_les.F90:24-502, 1247-1281), channel
+// (utau = 1, walld = |sig(k)|), Mason power 2 evaluated as products and a
+// square root, as the compiled reference does.  dvdx and dvdy take dlVV of
+// edge 3 for edge 2 and of edge 2 for edge 3 (sgm_les.F90:444-446,453-455).
+// Bytes/cell: read u,v,w (+3 vertex fields at 0.5) = 4.5, write eddy = 1.
+__global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const double *v, const double *w,
+                                                    const double *uv, const double *vv, const double *wv,
+                                                    double *eddy) {
+  const double Cs = 0.1, karman = (double)0.415f;
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    double U[5], V[5], W[5];
+    faces_of(g, i, k, s, u, uv, U);
+    faces_of(g, i, k, s, v, vv, V);
+    faces_of(g, i, k, s, w, wv, W);
+    const int e = i * 3;
+    const double x1 = g.xnn[e], x2 = g.xnn[e + 1], x3 = g.xnn[e + 2];
+    const double y1 = g.ynn[e], y2 = g.ynn[e + 1], y3 = g.ynn[e + 2];
+    const double l1 = g.dlVV[e], l2 = g.dlVV[e + 1], l3 = g.dlVV[e + 2];
+    const double a = g.area[i];
+    double dudx = U[0] * x1 * l1 + U[1] * x2 * l2 + U[2] * x3 * l3;
+    double dvdx = V[0] * x1 * l1 + V[1] * x2 * l3 + V[2] * x3 * l2;
+    double dwdx = W[0] * x1 * l1 + W[1] * x2 * l2 + W[2] * x3 * l3;
+    double dudy = U[0] * y1 * l1 + U[1] * y2 * l2 + U[2] * y3 * l3;
+    double dvdy = V[0] * y1 * l1 + V[1] * y2 * l3 + V[2] * y3 * l2;
+    double dwdy = W[0] * y1 * l1 + W[1] * y2 * l2 + W[2] * y3 * l3;
+    dudx = dudx / a;
+    dvdx = dvdx / a;
+    dwdx = dwdx / a;
+    dudy = dudy / a;
+    dvdy = dvdy / a;
+    dwdy = dwdy / a;
+    const double dz = g.dsig[k];
+    const double dudz = (U[3] - U[4]) / dz, dvdz = (V[3] - V[4]) / dz, dwdz = (W[3] - W[4]) / dz;
+    double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
+    const double lwall = karman * g.walld[k];
+    const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
+    const double dn2 = 1.0 / dn1;
+    const double del = sqrt(dn2);
+    DD = del * del;
+    const double nut = 1.0 * DD;
+    const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
+    const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
+    const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
+    const double ed = nut * Sij;
+    eddy[s] = ed;
+    // BCpcenter(eddy): Neumann ghost layers
+    if (k == 1 && g.bcbot == 1) eddy[s - 1] = ed;
+    if (k == g.NZ - 2 && g.bctop == 1) eddy[s + 1] = ed;
+  }
+}
+
+int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
+               const double *vv, const double *wv, double *eddy) {
+  hipLaunchKernelGGL(k_les, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
+  return (int)hipGetLastError();
+}
+
+// --------------------------------------------- momentum fluxes + predictor --
+// AdvDiffNS = diffusion3DNEW (Kim-Choi, diffusion3D.F90:110-191) then
+// advection3DNEW (advection3D.F90:112-284), fused with the predictor of
+// hydro.F90:322-333 (u* = u + dt(c1 rhs + c2 rhs_old), rhs_old = rhs) and
+// with the ghost layers of BCvelcenter3D for u* (hydro.F90:342-345).
+// Bytes/cell (fused): read u,v,w, 3 vertex fields (1.5), 5 faces, eddy,
+// 3 history = 12.5; write u*,v*,w*, 3 history = 6.
+struct MomArgs {
+  const double *uvw[3], *vtx[3], *face[5], *eddy, *rhs_in[3];
+  double *hist[3], *rhs_out[3], *ustar[3];
+  double c1, c2;
+};
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
+  const double bd[3] = {g.bdx, g.bdy, g.bdz};
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const int e = i * 3;
+    const double area = g.area[i];
+    const double sv = g.dsigv[k - 1];
+    const double Vol = area * sv;
+    const double Gam = a.eddy[s] + g.nu;
+    const double UF1 = a.face[0][s] * g.dlVV[e] * sv;
+    const double UF2 = a.face[1][s] * g.dlVV[e + 1] * sv;
+    const double UF3 = a.face[2][s] * g.dlVV[e + 2] * sv;
+    const double UFT = a.face[3][s] * area;
+    const double UFB = a.face[4][s] * area;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double *phi = a.uvw[q];
+      const double *phiv = a.vtx[q];
+      const double pi = phi[s];
+      double dn[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int ej = e + j;
+        const int nb = g.nbr[ej];
+        const int jv1 = g.vtx[ej], jv2 = g.vtx[e + (j + 1) % 3];
+        const double diff1 = (phi[(size_t)nb * g.NZ + k] - pi) / g.dlCC[ej];
+        const double diff2 = (phiv[(size_t)jv2 * g.NL + k] - phiv[(size_t)jv1 * g.NL + k]) / g.dlVV[ej];
+        dn[j] = diff1 * g.dle1[ej] + diff2 * g.dle2[ej];
+      }
+      const double dT = (phi[s + 1] - pi) / g.dsig[k];
+      const double dB = (phi[s - 1] - pi) / g.dsig[k - 1];
+      const double src = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv +
+                         (dT + dB) * area;
+      double r = a.rhs_in[q] ? a.rhs_in[q][s] : bd[q];
+      r = r + Gam * src / Vol;
+      double fc[5];
+      faces_of(g, i, k, s, phi, phiv, fc);
+      const double flux = fc[0] * UF1 + fc[1] * UF2 + fc[2] * UF3 + fc[3] * UFT + fc[4] * UFB;
+      r = r - flux / Vol;
+      if (a.rhs_out[q]) a.rhs_out[q][s] = r;
+      if (a.ustar[q]) {
+        const double us = pi + g.dt * (a.c1 * r + a.c2 * a.hist[q][s]);
+        a.hist[q][s] = r;
+        a.ustar[q][s] = us;
+        if (k == 1) {
+          const int sg = vel_ghost_sign(g.bcbot, g.BotBC, q);
+          if (sg) a.ustar[q][s - 1] = sg > 0 ? us : -us;
+        }
+        if (k == g.NZ - 2) {
+          const int sg = vel_ghost_sign(g.bctop, g.TopBC, q);
+          if (sg) a.ustar[q][s + 1] = sg > 0 ? us : -us;
+        }
+      }
+    }
+  }
+}
+
+int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                    const double *const face[5], const double *eddy, const double *const rhs_in[3],
+                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2) {
+  MomArgs a;
+  for (int q = 0; q < 3; ++q) {
+    a.uvw[q] = uvw[q];
+    a.vtx[q] = vtx[q];
+    a.rhs_in[q] = rhs_in ? rhs_in[q] : nullptr;
+    a.hist[q] = hist ? hist[q] : nullptr;
+    a.rhs_out[q] = rhs_out ? rhs_out[q] : nullptr;
+    a.ustar[q] = ustar ? ustar[q] : nullptr;
+  }
+  for (int q = 0; q < 5; ++q) a.face[q] = face[q];
+  a.eddy = eddy;
+  a.c1 = c1;
+  a.c2 = c2;
+  hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, a);
+  return (int)hipGetLastError();
+}
+
+// --------------------------------------------------------- PPE right side --
+// Source3D / interpolateU (source3D.F90:92-127, interpolationU.F90:122-160):
+// face-normal velocities of u* and their divergence; fused (rhsp_out != 0)
+// with rhsp/dt (hydro.F90:365-373), update_rhsp's lagged Kim-Choi flux
+// (source3D.F90:255-288) and the partial sums of Glob_Norm(rhs,1)
+// (bicgstab.F90:635-653).
+// Bytes/cell (fused): read u*,v*,w*, p vertex (0.5) = 3.5; write 5 faces, rhsp = 6.
+__global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, const double *v, const double *w,
+                                                       double *F1, double *F2, double *F3, double *FT, double *FB,
+                                                       double *src_out, double *rhsp_out, const double *pv,
+                                                       double *partials, unsigned int *ticket, SolverState *st) {
+  double acc[1] = {0.0};
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const int e = i * 3;
+    const double ui = u[s], vi = v[s];
+    double Uf[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const size_t sn = (size_t)g.nbr[e + j] * g.NZ + k;
+      const double Ui = 0.5 * (ui + u[sn]);
+      const double Vi = 0.5 * (vi + v[sn]);
+      Uf[j] = Ui * g.xnn[e + j] + Vi * g.ynn[e + j];
+    }
+    const double UT = 0.5 * (w[s] + w[s + 1]);
+    const double UB = -1.0 * (0.5 * (w[s] + w[s - 1]));
+    F1[s] = Uf[0];
+    F2[s] = Uf[1];
+    F3[s] = Uf[2];
+    FT[s] = UT;
+    FB[s] = UB;
+    const double sv = g.dsigv[k - 1], area = g.area[i];
+    double resid = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) resid = resid + Uf[j] * g.dlVV[e + j] * sv;
+    resid = resid + UT * area;
+    resid = resid + UB * area;
+    if (src_out) src_out[s] = resid;
+    if (rhsp_out) {
+      double rp = resid / g.dt;
+      double dn[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
+        const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
+        dn[j] = diff2 * g.dle2[e + j];
+      }
+      const double corr = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv;
+      rp = rp - corr;
+      rhsp_out[s] = rp;
+      acc[0] += rp;
+    }
+  }
+  if (!rhsp_out) return;
+  if (block_partials_and_ticket<1>(acc, partials, ticket)) {
+    double tot[1];
+    final_sums<1>(tot, partials, ticket);
+    if (threadIdx.x == 0) st->avec_rhs = tot[0] / (g.TotalVolume * (g.sigFin - g.sigIni));
+  }
+}
+
+int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
+                  double *rhsp_out, const double *pv, Partials part, SolverState *st) {
+  hipLaunchKernelGGL(k_source, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, uvw[0], uvw[1], uvw[2],
+                     face[0], face[1], face[2], face[3], face[4], src_out, rhsp_out, pv, part.buf, part.ticket, st);
+  return (int)hipGetLastError();
+}
+
+// update_rhsp alone (source3D.F90:255-288) for single-routine use
+__global__ __launch_bounds__(NSX_BLOCK) void k_update_rhsp(Geo g, double *rhs, const double *pv) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const int e = i * 3;
+    const double sv = g.dsigv[k - 1];
+    double dn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
+      const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
+      dn[j] = diff2 * g.dle2[e + j];
+    }
+    const double corr = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv;
+    rhs[s] = rhs[s] - corr;
+  }
+}
+
+int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *pv) {
+  hipLaunchKernelGGL(k_update_rhsp, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, rhs, pv);
+  return (int)hipGetLastError();
+}
+
+// ----------------------------------------------------------- LSM gradient --
+// grandientLSM (gradientLSM.F90:153-201): 3-neighbour least squares in the
+// horizontal, 3-point least squares in sigma, then BCpcenter on the outputs.
+__device__ inline void lsm_grad(const Geo &g, int i, int k, size_t s, const double *f, double &dx, double &dy,
+                                double &ds) {
+  const double fi = f[s];
+  double sumfx = 0.0, sumfy = 0.0;
+  const int e = i * 3;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double d = f[(size_t)g.nbr[e + j] * g.NZ + k] - fi;
+    sumfx = sumfx + g.dxCC[e + j] * d;
+    sumfy = sumfy + g.dyCC[e + j] * d;
+  }
+  const double sx2 = g.sum_xc2[i], sy2 = g.sum_yc2[i], sxy = g.sum_xcyc[i];
+  const double deter = sx2 * sy2 - sxy * sxy;
+  dx = (sy2 * sumfx - sxy * sumfy) / deter;
+  dy = (sx2 * sumfy - sxy * sumfx) / deter;
+  const double dzp = g.sig[k + 1] - g.sig[k];
+  const double dzm = g.sig[k - 1] - g.sig[k];
+  const double s2 = dzp * dzp + dzm * dzm;
+  ds = ((f[s + 1] - fi) * dzp + (f[s - 1] - fi) * dzm) / s2;
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_grad_lsm(Geo g, const double *f, double *dx, double *dy,
+                                                         double *ds) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    double a, b, d;
+    lsm_grad(g, i, k, s, f, a, b, d);
+    dx[s] = a;
+    dy[s] = b;
+    ds[s] = d;
+    if (k == 1 && g.bcbot == 1) {
+      dx[s - 1] = a;
+      dy[s - 1] = b;
+      ds[s - 1] = d;
+    }
+    if (k == g.NZ - 2 && g.bctop == 1) {
+      dx[s + 1] = a;
+      dy[s + 1] = b;
+      ds[s + 1] = d;
+    }
+  }
+}
+
+int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, double *dy, double *ds) {
+  hipLaunchKernelGGL(k_grad_lsm, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, f, dx, dy, ds);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------- corrector ---
+// gradientP(pfnp) (gradientP.F90:103-105 -> grandientLSM) fused with the
+// velocity corrector u = u* - dt grad p (hydro.F90:440-448) and the ghost
+// layers of BCvelcenter3D for the new velocity (hydro.F90:452-455).
+// Bytes/cell: read p, u*,v*,w* = 4; write u,v,w = 3.
+__global__ __launch_bounds__(NSX_BLOCK) void k_corrector(Geo g, const double *p, const double *us,
+                                                          const double *vs, const double *ws, double *un,
+                                                          double *vn, double *wn) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    double dpdx, dpdy, dpds;
+    lsm_grad(g, i, k, s, p, dpdx, dpdy, dpds);
+    const double nu_ = us[s] - g.dt * dpdx;
+    const double nv_ = vs[s] - g.dt * dpdy;
+    const double nw_ = ws[s] - g.dt * dpds;
+    un[s] = nu_;
+    vn[s] = nv_;
+    wn[s] = nw_;
+    if (k == 1) {
+      int sg = vel_ghost_sign(g.bcbot, g.BotBC, 0);
+      if (sg) un[s - 1] = sg > 0 ? nu_ : -nu_;
+      sg = vel_ghost_sign(g.bcbot, g.BotBC, 1);
+      if (sg) vn[s - 1] = sg > 0 ? nv_ : -nv_;
+      sg = vel_ghost_sign(g.bcbot, g.BotBC, 2);
+      if (sg) wn[s - 1] = sg > 0 ? nw_ : -nw_;
+    }
+    if (k == g.NZ - 2) {
+      int sg = vel_ghost_sign(g.bctop, g.TopBC, 0);
+      if (sg) un[s + 1] = sg > 0 ? nu_ : -nu_;
+      sg = vel_ghost_sign(g.bctop, g.TopBC, 1);
+      if (sg) vn[s + 1] = sg > 0 ? nv_ : -nv_;
+      sg = vel_ghost_sign(g.bctop, g.TopBC, 2);
+      if (sg) wn[s + 1] = sg > 0 ? nw_ : -nw_;
+    }
+  }
+}
+
+int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double *const us[3],
+                     double *const unp[3]) {
+  hipLaunchKernelGGL(k_corrector, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, p, us[0], us[1], us[2],
+                     unp[0], unp[1], unp[2]);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------ face update --
+// newu3D Kim-Choi branch (newu3D.F90:121-239): U?FACE -= dt dp/dn, the
+// bottom face divided by dsig(k) (newu3D.F90:164).
+// Bytes/cell: read p, p vertex (0.5), 5 faces = 6.5; write 5 faces.
+__global__ __launch_bounds__(NSX_BLOCK) void k_newu(Geo g, const double *p, const double *pv, double *F1,
+                                                     double *F2, double *F3, double *FT, double *FB) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const int e = i * 3;
+    const double pi = p[s];
+    double dn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
+      const double diff1 = (p[(size_t)g.nbr[e + j] * g.NZ + k] - pi) / g.dlCC[e + j];
+      const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
+      dn[j] = diff1 * g.dle1[e + j] + diff2 * g.dle2[e + j];
+    }
+    const double dT = (p[s + 1] - pi) / g.dsig[k];
+    const double dB = (p[s - 1] - pi) / g.dsig[k];
+    const double c1 = g.dt;
+    F1[s] = F1[s] - dn[0] * c1;
+    F2[s] = F2[s] - dn[1] * c1;
+    F3[s] = F3[s] - dn[2] * c1;
+    FT[s] = FT[s] - dT * c1;
+    FB[s] = FB[s] - dB * c1;
+  }
+}
+
+int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, double *const face[5]) {
+  hipLaunchKernelGGL(k_newu, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, p, pv, face[0], face[1],
+                     face[2], face[3], face[4]);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------ RK2 average --
+// nsmp3D.F90:531-548: u,v,w(n+1) = 0.5 (u(n) + u(n+1)) over every stored
+// entry (ghost layers included).  Bytes/cell: read 6, write 3.
+__global__ __launch_bounds__(NSX_BLOCK) void k_rk2avg(size_t n, const double *u0, const double *v0,
+                                                       const double *w0, double *u1, double *v1, double *w1) {
+  for (size_t t = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * NSX_BLOCK) {
+    u1[t] = 0.5 * (u0[t] + u1[t]);
+    v1[t] = 0.5 * (v0[t] + v1[t]);
+    w1[t] = 0.5 * (w0[t] + w1[t]);
+  }
+}
+
+int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]) {
+  const size_t n = (size_t)g.N0 * g.NZ;
+  hipLaunchKernelGGL(k_rk2avg, dim3(nsx_grid_for((uint32_t)(n < 0xffffffffu ? n : 0xffffffffu))), dim3(NSX_BLOCK),
+                     0, s, n, un[0], un[1], un[2], unp[0], unp[1], unp[2]);
+  return (int)hipGetLastError();
+}
+
+// --------------------------------------------------------- ghost layers ----
+// vertical part of BCpcenter (kind 0, BC.F90:919-938) and BCvelcenter3D
+// (kinds 1..3 = u,v,w, BC.F90:459-527) on a stored field
+__global__ __launch_bounds__(NSX_BLOCK) void k_ghost_layers(Geo g, double *f, int kind) {
+  const int i = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  if (i >= g.N0) return;
+  const size_t b = (size_t)i * g.NZ;
+  int sb, st;
+  if (kind == 0) {
+    sb = g.bcbot == 1 ? 1 : 0;
+    st = g.bctop == 1 ? 1 : 0;
+  } else {
+    sb = vel_ghost_sign(g.bcbot, g.BotBC, kind - 1);
+    st = vel_ghost_sign(g.bctop, g.TopBC, kind - 1);
+  }
+  if (sb) f[b] = sb > 0 ? f[b + 1] : -f[b + 1];
+  if (st) f[b + g.NZ - 1] = st > 0 ? f[b + g.NZ - 2] : -f[b + g.NZ - 2];
+}
+
+int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind) {
+  hipLaunchKernelGGL(k_ghost_layers, dim3((g.N0 + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, f, kind);
+  return (int)hipGetLastError();
+}
+
+// ----------------------------------------------------- compact PPE matrix --
+// PDE_center (PDE.F90:69-121) regenerated: W(i,k) = area/dsig(k) between
+// layers k and k+1 (0 at the walls), diagonal AA0 = -(AAT+AAB+AA1+AA2+AA3)
+// and its reciprocal; A0_check (nullable) receives AA0 for the set-up check.
+__global__ __launch_bounds__(NSX_BLOCK) void k_ppe_build(Geo g, double *W, double *invD, double *A0) {
+  const size_t n = (size_t)g.N0 * g.NZ;
+  for (size_t t = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * NSX_BLOCK) {
+    const int i = (int)(t / g.NZ), k = (int)(t % g.NZ);
+    double w = 0.0;
+    if (k >= 1 && k <= g.NZ - 3) w = g.area[i] / g.dsig[k];
+    if (k == g.NZ - 2 && g.bctop != 1) w = g.area[i] / g.dsig[k];
+    if (k == 0 && g.bcbot != 1) w = g.area[i] / g.dsig[0];
+    W[t] = w;
+  }
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_ppe_diag(Geo g, const double *W, double *invD, double *A0) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const double sv = g.dsigv[k - 1];
+    const double AAT = W[s], AAB = W[s - 1];
+    const double a1 = g.ppec[i * 3] * sv, a2 = g.ppec[i * 3 + 1] * sv, a3 = g.ppec[i * 3 + 2] * sv;
+    const double d = -1.0 * (AAT + AAB + a1 + a2 + a3);
+    invD[s] = 1.0 / d;
+    if (A0) A0[s] = d;
+  }
+}
+
+int launch_ppe_build(const Geo &g, hipStream_t s, double *W, double *invD, double *A0_check) {
+  hipLaunchKernelGGL(k_ppe_build, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, W, invD, A0_check);
+  hipLaunchKernelGGL(k_ppe_diag, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, W, invD, A0_check);
+  return (int)hipGetLastError();
+}

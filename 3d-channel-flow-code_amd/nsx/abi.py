@@ -1,0 +1,88 @@
+"""ctypes declarations of the C ABI in include/nsx.h."""
+from __future__ import annotations
+
+import ctypes
+
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_cp = ctypes.c_char_p
+
+# field ids (enum nsx_field)
+F = dict(UFN=0, VFN=1, WFN=2, PFN=3, UF=4, VF=5, WF=6, PF=7, UFNP=8, VFNP=9, WFNP=10, PFNP=11,
+         RHSUF=12, RHSVF=13, RHSWF=14, EDDY=15, UFV=32, VFV=33, WFV=34, PFV=35,
+         U1FACE=48, U2FACE=49, U3FACE=50, UTFACE=51, UBFACE=52)
+CELL_SCRATCH = 64
+VERT_SCRATCH = 96
+FACE_SCRATCH = 112
+N_CELL_SCRATCH = 16
+N_VERT_SCRATCH = 8
+N_FACE_SCRATCH = 10
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
+
+# symbols the header declares (checked by the CPU tests)
+EXPORTS = [
+    "nsx_create", "nsx_destroy", "nsx_last_error", "nsx_device", "nsx_set_mesh", "nsx_set_geometry",
+    "nsx_set_sigma", "nsx_set_vertex_weights", "nsx_set_bc", "nsx_set_ppe_matrix", "nsx_set_les",
+    "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
+    "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
+    "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
+    "nsx_newu3d", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iters", c_int), ("exit_code", c_int), ("e1norm", c_dbl), ("e2norm", c_dbl),
+                ("n_solves", c_int), ("iters_total", c_int)]
+
+    def as_dict(self):
+        return dict(iters=self.iters, exit=SOLVE_EXIT.get(self.exit_code, str(self.exit_code)), e1=self.e1norm,
+                    e2=self.e2norm, n_solves=self.n_solves, iters_total=self.iters_total)
+
+
+def declare(lib: ctypes.CDLL) -> None:
+    P = ctypes.POINTER
+    sig = {
+        "nsx_create": [c_int, c_int, c_int, c_int, P(c_vp)],
+        "nsx_destroy": [c_vp],
+        "nsx_device": [c_vp],
+        "nsx_set_mesh": [c_vp, c_vp, c_vp, c_vp, c_vp],
+        "nsx_set_geometry": [c_vp] + [c_vp] * 13 + [c_dbl],
+        "nsx_set_sigma": [c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl],
+        "nsx_set_vertex_weights": [c_vp] * 7,
+        "nsx_set_bc": [c_vp] + [c_vp] * 8 + [c_int] * 4,
+        "nsx_set_ppe_matrix": [c_vp] * 7,
+        "nsx_set_les": [c_vp, c_vp],
+        "nsx_set_params": [c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_int, c_int, c_int],
+        "nsx_commit": [c_vp],
+        "nsx_put_field": [c_vp, c_int, c_vp],
+        "nsx_get_field": [c_vp, c_int, c_vp, c_int],
+        "nsx_copy_field": [c_vp, c_int, c_int],
+        "nsx_init_state": [c_vp],
+        "nsx_hydro": [c_vp, c_int, P(SolveInfo)],
+        "nsx_step": [c_vp, c_int, c_int, P(SolveInfo)],
+        "nsx_solve_info_get": [c_vp, P(SolveInfo)],
+        "nsx_bcglobalvc": [c_vp] + [c_int] * 7,
+        "nsx_bcglobalp": [c_vp, c_int, c_int, c_int],
+        "nsx_grad_lsm": [c_vp] + [c_int] * 4,
+        "nsx_calcul_face": [c_vp] + [c_int] * 7,
+        "nsx_sgles": [c_vp] + [c_int] * 3,
+        "nsx_advdiff": [c_vp] + [c_int] * 7,
+        "nsx_source3d": [c_vp] + [c_int] * 4,
+        "nsx_update_rhsp": [c_vp] + [c_int] * 3,
+        "nsx_bicgstab": [c_vp, c_int, c_int, c_int, P(SolveInfo)],
+        "nsx_newu3d": [c_vp, c_int, c_int],
+        "nsx_timing_enable": [c_vp, c_int],
+        "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
+        "nsx_timing_reset": [c_vp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = c_int
+    lib.nsx_last_error.argtypes = [c_vp]
+    lib.nsx_last_error.restype = c_cp

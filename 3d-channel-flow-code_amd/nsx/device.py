@@ -1,0 +1,233 @@
+"""Host-side mirror of the reference's hot-path interface over the HIP library.
+
+`Context` owns one device context (include/nsx.h).  Its methods carry the
+names and argument meaning of the reference's Fortran routines
+(src/Main/hydro.F90, bicgstab.F90, src/BoundaryConditions/BC.F90, ...): they
+take and return numpy arrays in the reference layout ((N_CELL,NZ),
+(N_VERT,NZ-1), (N_CELL0,NZ) Fortran order).  Field state otherwise stays
+resident on the device between calls (`step`, `hydro`).  Every failure of the
+library raises NsxError: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .abi import CELL_SCRATCH, F, FACE_SCRATCH, PREC_FP64, VERT_SCRATCH, SolveInfo
+
+
+class NsxError(RuntimeError):
+    pass
+
+
+def _p(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.int32)
+
+
+class Context:
+    """A device context holding one case (mesh, geometry, matrix, state)."""
+
+    def __init__(self, case):
+        self.lib = _lib.device()
+        self.case = c = case
+        h = ctypes.c_void_p()
+        rc = self.lib.nsx_create(c.N_CELL0, c.N_CELL, c.N_VERT, c.NZ, ctypes.byref(h))
+        if rc != 0:
+            raise NsxError(f"nsx_create failed ({rc}): no usable HIP device")
+        self.h = h
+        A = c.arrays
+        keep = []
+
+        def k(a, conv=_f64):
+            a = conv(a)
+            keep.append(a)
+            return _p(a)
+
+        self._check(self.lib.nsx_set_mesh(h, k(A["No_cp"], _i32), k(A["No_vp"], _i32), k(A["nbe"], _i32),
+                                          k(A["nbev"], _i32)))
+        self._check(self.lib.nsx_set_geometry(h, *[k(A[n]) for n in ("areaCell", "dlVV", "dlCC", "dxCC", "dyCC",
+                                                                      "xnn", "ynn", "dle1", "dle2", "doe",
+                                                                      "sum_xc2", "sum_yc2", "sum_xcyc")],
+                                              c.TotalVolume))
+        p = c.params
+        self._check(self.lib.nsx_set_sigma(h, k(A["sig"]), k(A["dsig"]), k(A["sigv"]), k(A["dsigv"]), p.sigIni,
+                                           p.sigFin))
+        self._check(self.lib.nsx_set_vertex_weights(h, k(A["surrounding"], _i32), k(A["weight"]),
+                                                    k(A["Dimsurrounding"], _i32), k(A["dlVsum"]),
+                                                    k(A["vertex_pair"], _i32), k(A["vertex_edge"], _i32)))
+        self._check(self.lib.nsx_set_bc(h, *[k(A[n], _i32) for n in ("bcuc", "bcvc", "bcwc", "bcpc", "bcuv", "bcvv",
+                                                                      "bcwv", "bcpv")],
+                                        c.bctop, c.bcbot, p.TopBC, p.BotBC))
+        self._check(self.lib.nsx_set_ppe_matrix(h, *[k(A[n]) for n in ("AmC0", "AmC1", "AmC2", "AmC3", "AmCT",
+                                                                        "AmCB")]))
+        self._check(self.lib.nsx_set_les(h, k(A["walld"])))
+        self._check(self.lib.nsx_set_params(h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, p.MaxIters, p.FinRK,
+                                            p.Iturbu))
+        self._check(self.lib.nsx_commit(h))
+
+    # ------------------------------------------------------------ basics ---
+    def _check(self, rc):
+        if rc < 0:
+            raise NsxError(self.lib.nsx_last_error(self.h).decode(errors="replace"))
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nsx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, fid: int):
+        c = self.case
+        if 32 <= fid < 48 or VERT_SCRATCH <= fid < VERT_SCRATCH + 8:
+            return (c.N_VERT, c.NZ - 1)
+        if 48 <= fid < 64 or FACE_SCRATCH <= fid < FACE_SCRATCH + 10:
+            return (c.N_CELL0, c.NZ)
+        return (c.N_CELL, c.NZ)
+
+    def put(self, field, a) -> None:
+        fid = F[field] if isinstance(field, str) else field
+        a = _f64(a)
+        if a.shape != self._shape(fid):
+            raise ValueError(f"field {field}: shape {a.shape}, expected {self._shape(fid)}")
+        self._check(self.lib.nsx_put_field(self.h, fid, _p(a)))
+
+    def get(self, field, fill_ghost_rows: bool = True) -> np.ndarray:
+        fid = F[field] if isinstance(field, str) else field
+        out = np.zeros(self._shape(fid), order="F")
+        self._check(self.lib.nsx_get_field(self.h, fid, _p(out), int(fill_ghost_rows)))
+        return out
+
+    def solve_info(self) -> dict:
+        si = SolveInfo()
+        self._check(self.lib.nsx_solve_info_get(self.h, ctypes.byref(si)))
+        return si.as_dict()
+
+    # ---------------------------------------------------------- stepping ---
+    def init_state(self, u0=None, v0=None, w0=None, p0=None) -> None:
+        """user_initial.F90:229-253 + LoopTimeInitial with the case's initial field."""
+        c = self.case
+        for name, a in (("UFN", c.u0 if u0 is None else u0), ("VFN", c.v0 if v0 is None else v0),
+                        ("WFN", c.w0 if w0 is None else w0), ("PFN", c.p0 if p0 is None else p0)):
+            self.put(name, a)
+        self._check(self.lib.nsx_init_state(self.h))
+
+    def step(self, nsteps: int = 1, precision: int = PREC_FP64) -> dict:
+        """nsteps time steps (nsmp3D.F90:404-564) on the resident state."""
+        si = SolveInfo()
+        self._check(self.lib.nsx_step(self.h, nsteps, precision, ctypes.byref(si)))
+        return si.as_dict()
+
+    def hydro(self, flag_ab: int = 0) -> dict:
+        """One stage from (UF..PF) into (UFNP..PFNP) (hydro.F90:8-501)."""
+        si = SolveInfo()
+        self._check(self.lib.nsx_hydro(self.h, flag_ab, ctypes.byref(si)))
+        return si.as_dict()
+
+    def copy(self, dst, src) -> None:
+        self._check(self.lib.nsx_copy_field(self.h, F[dst] if isinstance(dst, str) else dst,
+                                            F[src] if isinstance(src, str) else src))
+
+    # ---------------------------------------- single routines (reference names)
+    def _cells(self, *arrays):
+        ids = []
+        for n, a in enumerate(arrays):
+            fid = CELL_SCRATCH + n
+            self.put(fid, a)
+            ids.append(fid)
+        return ids
+
+    def BCglobalVC(self, u, v, w, tagdir: int):
+        """BC.F90:202-295: returns (u, v, w) with ghosts filled and (uv, vv, wv)."""
+        cu, cv, cw = self._cells(u, v, w)
+        self._check(self.lib.nsx_bcglobalvc(self.h, cu, cv, cw, VERT_SCRATCH, VERT_SCRATCH + 1, VERT_SCRATCH + 2,
+                                            tagdir))
+        return (self.get(cu), self.get(cv), self.get(cw), self.get(VERT_SCRATCH), self.get(VERT_SCRATCH + 1),
+                self.get(VERT_SCRATCH + 2))
+
+    def BCglobalP(self, p, tagdir: int):
+        """BC.F90:299-363: returns (p with ghosts, pv)."""
+        (cp,) = self._cells(p)
+        self._check(self.lib.nsx_bcglobalp(self.h, cp, VERT_SCRATCH, tagdir))
+        return self.get(cp), self.get(VERT_SCRATCH)
+
+    def grandientLSM(self, f):
+        """gradientLSM.F90:8-211: returns (dfdx, dfdy, dfdsig)."""
+        (cf,) = self._cells(f)
+        o = CELL_SCRATCH + 4
+        self._check(self.lib.nsx_grad_lsm(self.h, o, o + 1, o + 2, cf))
+        return self.get(o), self.get(o + 1), self.get(o + 2)
+
+    def calcul_face(self, f, fv):
+        """interpolationFace.F90:7-238: the five face values of f."""
+        (cf,) = self._cells(f)
+        self.put(VERT_SCRATCH, fv)
+        o = FACE_SCRATCH
+        self._check(self.lib.nsx_calcul_face(self.h, o, o + 1, o + 2, o + 3, o + 4, cf, VERT_SCRATCH))
+        return tuple(self.get(o + q) for q in range(5))
+
+    def sgles(self, u, v, w):
+        """sgm_les.F90:24-275: eddy viscosity of (u,v,w)."""
+        cu, cv, cw = self._cells(u, v, w)
+        self._check(self.lib.nsx_sgles(self.h, cu, cv, cw))
+        return self.get("EDDY")
+
+    def AdvDiffNS(self, rhsu, rhsv, rhsw, u, v, w, eddy):
+        """AdvDiffNS.F90:8-140 with the resident face velocities; returns the three rhs."""
+        ids = self._cells(rhsu, rhsv, rhsw, u, v, w, eddy)
+        self._check(self.lib.nsx_advdiff(self.h, *ids))
+        return self.get(ids[0]), self.get(ids[1]), self.get(ids[2])
+
+    def Source3D(self, u, v, w):
+        """source3D.F90:8-174: returns (src, U1FACE..UBFACE); the faces stay resident."""
+        cs, cu, cv, cw = self._cells(np.zeros_like(u), u, v, w)
+        self._check(self.lib.nsx_source3d(self.h, cs, cu, cv, cw))
+        return (self.get(cs, fill_ghost_rows=False),) + tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE",
+                                                                                      "UTFACE", "UBFACE"))
+
+    def update_rhsp(self, rhs, p):
+        """source3D.F90:176-335: returns (rhs, pv)."""
+        cr, cp = self._cells(rhs, p)
+        self._check(self.lib.nsx_update_rhsp(self.h, cr, cp, VERT_SCRATCH))
+        return self.get(cr, fill_ghost_rows=False), self.get(VERT_SCRATCH)
+
+    def bicgstab(self, phi, rhs):
+        """bicgstab.F90:8-168: returns (phi, phiv, solve info)."""
+        cphi, crhs = self._cells(phi, rhs)
+        si = SolveInfo()
+        self._check(self.lib.nsx_bicgstab(self.h, cphi, VERT_SCRATCH, crhs, ctypes.byref(si)))
+        return self.get(cphi), self.get(VERT_SCRATCH), si.as_dict()
+
+    def newu3D(self, p):
+        """newu3D.F90:8-257 on the resident faces: returns (faces, pv)."""
+        (cp,) = self._cells(p)
+        self._check(self.lib.nsx_newu3d(self.h, cp, VERT_SCRATCH))
+        return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
+
+    # ------------------------------------------------------------- timing ---
+    def timing(self, on: bool) -> None:
+        self._check(self.lib.nsx_timing_enable(self.h, int(on)))
+
+    def timing_reset(self) -> None:
+        self._check(self.lib.nsx_timing_reset(self.h))
+
+    def timer(self, name: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._check(self.lib.nsx_timing_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

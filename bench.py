@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Throughput of the per-time-step hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 2] [--no-cpu]
+
+A step is one RK2 time step (two projection stages: Smagorinsky eddy
+viscosity, advection/diffusion fluxes, predictor, PPE source, BiCGSTAB
+pressure solve, corrector, face-velocity correction) of the channel workload
+on resident device state.  Prints one JSON line:
+  value        prism-cell-steps/s = interior prisms x K / wall time of K steps
+  roofline     the dominant kernel (BiCGSTAB phase A): algorithmic bytes per
+               launch / mean launch time (HIP events on the library's stream)
+  cpu_baseline the oracle (CPU restatement of the reference, 1 core) on a
+               bounded sample of the same workload, extrapolated per step
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "3d-channel-flow-code_amd"))
+
+METRIC = "prism-cell-steps/s: mixed-prec vs FP64 GPU at ≤1e-8 rel err; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per interior prism of each BiCGSTAB phase (FP64):
+# A reads r,p,v,rhat,W,invD and writes p',v'; B reads r,v,W,invD, writes s,t;
+# C reads x,p,s,t,invD,rhat, writes x,r
+PHASE_BYTES = {"bicgstab_a": 8 * 8, "bicgstab_b": 6 * 8, "bicgstab_c": 8 * 8}
+WORKLOAD = {
+    0: "32x16 quads x 32 layers = 32,768 prisms, Re_tau 180, dt 1e-3",
+    1: "128x64 quads x 64 layers = 1,048,576 prisms, Re_tau 180, dt 5e-4, jittered mesh",
+    2: "256x128 quads x 96 layers = 6,291,456 prisms, Re_tau 395, dt 2.5e-4, jittered mesh",
+    3: "64x32 quads x 48 layers = 196,608 prisms, Re_tau 180, dt 5e-4, jittered mesh",
+    4: "512x256 quads x 128 layers = 33,554,432 prisms, Re_tau 590, dt 1e-4, jittered mesh",
+}
+
+
+def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
+    """One projection stage of the oracle on the full mesh with BiCGSTAB capped
+    at `cap` iterations; the step time is extrapolated with the GPU's mean
+    iteration count per solve."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    O = oracle.Oracle(case)
+    O.set_max_iters(cap)
+    for n in ("uf", "vf", "wf", "pf"):
+        O.state(n)[:] = O.state(n[0] + "fn")
+    t0 = time.perf_counter()
+    O.hydro()
+    wall = time.perf_counter() - t0
+    t = O.timers()
+    its = max(O.solver_record()["iter"], 1)
+    t_iter = t["solve"] / its
+    t_rest = t["hydro"] - t["solve"]
+    step_s = case.params.FinRK * (t_rest + mean_iters * t_iter)
+    return {"value": case.n_interior / step_s, "unit": "prism-cell-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C restatement of the reference, 1 thread) on the full mesh: one projection stage "
+                       f"with BiCGSTAB capped at {its} iterations ({wall:.1f} s); step time = "
+                       f"{case.params.FinRK} x (non-solver {t_rest:.2f} s + {mean_iters:.0f} iterations x "
+                       f"{t_iter * 1e3:.1f} ms) with the GPU run's mean iteration count"),
+            "s_per_step": step_s}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cfg", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1234)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of the path is a design note, not code)", file=sys.stderr)
+        return 2
+
+    from nsx.case import CONFIGS, config_case
+    from nsx.device import Context
+
+    cfg = args.cfg
+    steps = args.steps if args.steps is not None else {0: 10, 1: 20, 2: 10, 3: 20, 4: 3}[cfg]
+    case = config_case(cfg, seed=args.seed)
+    ctx = Context(case)
+    ctx.init_state()
+    if args.warmup:
+        ctx.step(args.warmup)
+    info0 = ctx.solve_info()
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    info = ctx.step(steps)
+    wall = time.perf_counter() - t0
+    ctx.timing(False)
+    n = case.n_interior
+    solves = info["n_solves"] - info0["n_solves"]
+    iters = info["iters_total"] - info0["iters_total"]
+    mean_iters = iters / max(solves, 1)
+    kernels = {}
+    for name, b in PHASE_BYTES.items():
+        ms, launches = ctx.timer(name)
+        if launches:
+            avg = ms / launches
+            kernels[name] = {"launches": launches, "avg_ms": avg, "bytes_per_launch": b * n,
+                             "GBps": b * n / (avg * 1e-3) / 1e9}
+    stage_ms, _ = ctx.timer("stage")
+    dom = kernels.get("bicgstab_a", {"GBps": 0.0, "bytes_per_launch": 0})
+    out = {
+        "metric": METRIC,
+        "value": n * steps / wall,
+        "unit": "prism-cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"configs[{cfg}]: {WORKLOAD[cfg]}", "prisms": n, "precision": "fp64",
+                   "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": case.params.eps,
+                   "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": "k_bicg_a (BiCGSTAB phase A)", "achieved": dom["GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
+        "solver": {"mean_iters_per_solve": mean_iters, "solves": solves, "last_e1": info["e1"],
+                   "last_e2": info["e2"]},
+        "kernels": kernels,
+        "non_solver_ms_per_stage": stage_ms / max(solves, 1),
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(case, mean_iters)
+        out["cpu_baseline"] = {k: v for k, v in cb.items() if k != "s_per_step"}
+        out["gpu_over_cpu"] = out["value"] / cb["value"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,174 @@
+/*
+ * nsx.h -- C ABI of the MI355X drop-in for the per-time-step hot path of the
+ * NSMP3D prismatic sigma-grid solver (reference: techi28/3D-channel-flow-code).
+ *
+ * The reference's hot-path routines are Fortran external subroutines with
+ * implicit interfaces, explicit-shape REAL*8 arrays laid out (N_CELL,NZ)
+ * column-major with the triangle index fastest, and module state (USE
+ * geometry / USE les).  This ABI replaces them with entry points over one
+ * device context that keeps every field resident in HBM:
+ *
+ *   - set-up calls take the reference's own module arrays (host pointers,
+ *     reference layout, 1-based connectivity) once;
+ *   - nsx_put_field / nsx_get_field move a field between a host array in the
+ *     reference layout and the device copy (explicit sync points: I/O,
+ *     diagnostics);
+ *   - hot-path calls take field ids (NSX_F_*) of device-resident fields.
+ *
+ * Every call returns 0 on success, > 0 for a soft solver outcome (see
+ * NSX_SOLVE_*), < 0 for an error; nsx_last_error() gives the message.  One
+ * host thread per context; calls are synchronous unless stated otherwise.
+ * Each entry point cites the reference routine it replaces.  The Fortran
+ * binding (ISO_C_BINDING) is 3d-channel-flow-code_amd/fortran/nsx_iface.F90.
+ */
+#ifndef NSX_H
+#define NSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nsx_ctx nsx_ctx;
+
+/* ---- field ids ------------------------------------------------------- */
+enum nsx_field {
+  /* cell fields (N_CELL,NZ) in the reference layout (module variables) */
+  NSX_F_UFN = 0, NSX_F_VFN, NSX_F_WFN, NSX_F_PFN, /* time level n */
+  NSX_F_UF, NSX_F_VF, NSX_F_WF, NSX_F_PF,         /* RK stage input */
+  NSX_F_UFNP, NSX_F_VFNP, NSX_F_WFNP, NSX_F_PFNP, /* stage output / n+1 */
+  NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF,          /* AB2 history (module geometry) */
+  NSX_F_EDDY,                                     /* eddy viscosity (module les) */
+  /* vertex fields (N_VERT,NZ-1) */
+  NSX_F_UFV = 32, NSX_F_VFV, NSX_F_WFV, NSX_F_PFV,
+  /* face-normal velocities (N_CELL0,NZ): U1FACE..UBFACE (module geometry) */
+  NSX_F_U1FACE = 48, NSX_F_U2FACE, NSX_F_U3FACE, NSX_F_UTFACE, NSX_F_UBFACE,
+  /* scratch fields for single-routine calls */
+  NSX_F_CELL_SCRATCH = 64, /* 64..79: cell fields */
+  NSX_F_VERT_SCRATCH = 96, /* 96..103: vertex fields */
+  NSX_F_FACE_SCRATCH = 112 /* 112..121: face fields (N_CELL0,NZ) */
+};
+#define NSX_N_CELL_SCRATCH 16
+#define NSX_N_VERT_SCRATCH 8
+#define NSX_N_FACE_SCRATCH 10
+
+/* ---- solver outcome (bicgstab.F90:462-575) --------------------------- */
+enum nsx_solve_exit {
+  NSX_SOLVE_CONVERGED = 0,   /* min(e1,e2) < eps after iterations */
+  NSX_SOLVE_NO_SOLVE = 1,    /* "Pressure converged without solve" */
+  NSX_SOLVE_BREAKDOWN = 2,   /* |rho|, |omega| or |<r0,v>| < 1e-30 */
+  NSX_SOLVE_MAX_ITERS = 3    /* iter > MaxIters */
+};
+
+typedef struct nsx_solve_info {
+  int iters;       /* BiCGSTAB iterations of the last solve */
+  int exit_code;   /* enum nsx_solve_exit */
+  double e1norm;   /* sum|r| / sum|b| */
+  double e2norm;   /* ||r||_2 / ||b||_2 */
+  int n_solves;    /* solves since nsx_init_state */
+  int iters_total; /* iterations since nsx_init_state */
+} nsx_solve_info;
+
+/* ---- precision of the step --------------------------------------------- */
+enum nsx_precision {
+  NSX_PREC_FP64 = 0,  /* the reference's arithmetic */
+  NSX_PREC_MIXED = 1  /* FP32 stencil/flux/nu_t, FP64 iterative refinement of the PPE */
+};
+
+/* ---- lifecycle ----------------------------------------------------------- */
+/* Sizes of the reference's COMMON /coo/ (common.mpf:323-325). */
+int nsx_create(int n_cell0, int n_cell, int n_vert, int nz, nsx_ctx **ctx);
+int nsx_destroy(nsx_ctx *ctx);
+const char *nsx_last_error(const nsx_ctx *ctx);
+/* device id the context runs on (hipGetDevice at nsx_create) */
+int nsx_device(const nsx_ctx *ctx);
+
+/* ---- set-up: the reference's module arrays, host pointers -------------- */
+/* module variables No_cp (N_CELL,3) incl. ghost rows, No_vp (N_CELL0,3),
+ * nbe (N_CELL0), nbev (N_VERT); read by calcul_geometry.F90 / input.F90 */
+int nsx_set_mesh(nsx_ctx *ctx, const int *No_cp, const int *No_vp, const int *nbe, const int *nbev);
+/* module geometry (module_geometry.F90:92-140): areaCell (N_CELL), the
+ * (N_CELL0,3) edge arrays, the LSM sums (N_CELL0) and TotalVolume */
+int nsx_set_geometry(nsx_ctx *ctx, const double *areaCell, const double *dlVV, const double *dlCC,
+                     const double *dxCC, const double *dyCC, const double *xnn, const double *ynn,
+                     const double *dle1, const double *dle2, const double *doe, const double *sum_xc2,
+                     const double *sum_yc2, const double *sum_xcyc, double TotalVolume);
+/* sig, dsig (NZ), sigv, dsigv (NZ-1) and sigIni/sigFin (common.mpf) */
+int nsx_set_sigma(nsx_ctx *ctx, const double *sig, const double *dsig, const double *sigv, const double *dsigv,
+                  double sigIni, double sigFin);
+/* inverse-distance weights surrounding/weight (N_VERT,10), Dimsurrounding,
+ * dlVsum (N_VERT) and periodic vertex links vertex_pair/vertex_edge (N_VERT) */
+int nsx_set_vertex_weights(nsx_ctx *ctx, const int *surrounding, const double *weight, const int *Dimsurrounding,
+                           const double *dlVsum, const int *vertex_pair, const int *vertex_edge);
+/* boundary flags of update_bc.F90 (bc?c (N_CELL), bc?v (N_VERT)), bctop/bcbot
+ * and TopBC/BotBC (common.mpf).  Horizontal ghosts must be periodic (flag 3);
+ * sigma walls TopBC, BotBC in {0,1}. */
+int nsx_set_bc(nsx_ctx *ctx, const int *bcuc, const int *bcvc, const int *bcwc, const int *bcpc, const int *bcuv,
+               const int *bcvv, const int *bcwv, const int *bcpv, int bctop, int bcbot, int TopBC, int BotBC);
+/* PPE matrix AmC0..AmCB (N_CELL0,NZ) of PDE_center (PDE.F90:69-121).  The
+ * device regenerates it from the geometry; nsx_commit checks that the
+ * regenerated coefficients equal these bit for bit. */
+int nsx_set_ppe_matrix(nsx_ctx *ctx, const double *AmC0, const double *AmC1, const double *AmC2,
+                       const double *AmC3, const double *AmCT, const double *AmCB);
+/* channel wall distance walld(:,k) = |sig(k)| (sgm_les.F90:856-862), per level (NZ) */
+int nsx_set_les(nsx_ctx *ctx, const double *walld_level);
+/* run parameters of dataParameters.txt (input_parameters.F90:255-436) */
+int nsx_set_params(nsx_ctx *ctx, double dt, double Re, double bdx, double bdy, double bdz, double eps, int MaxIters,
+                   int FinRK, int Iturbu);
+/* builds the device data; hot-path calls are valid afterwards */
+int nsx_commit(nsx_ctx *ctx);
+
+/* ---- fields -------------------------------------------------------------- */
+/* host array in the reference layout; ghost rows and ghost layers are ignored */
+int nsx_put_field(nsx_ctx *ctx, int field, const double *host);
+/* host array in the reference layout; periodic ghost rows are filled from
+ * their partners when fill_ghost_rows != 0 (as the reference's BC fills do) */
+int nsx_get_field(nsx_ctx *ctx, int field, double *host, int fill_ghost_rows);
+int nsx_copy_field(nsx_ctx *ctx, int dst, int src);
+
+/* ---- time stepping ------------------------------------------------------- */
+/* user_initial.F90:229-253 and LoopTimeInitial: after ufn..pfn are put, fill
+ * their ghosts, vertex values (tagdir 3), face velocities; zero the AB2 history */
+int nsx_init_state(nsx_ctx *ctx);
+/* one projection stage, hydro.F90:8-501: (UF,VF,WF,PF) -> (UFNP,..,PFNP),
+ * vertex values, faces, AB2 history, eddy.  flag_ab selects AB2 (hydro.F90:225-231). */
+int nsx_hydro(nsx_ctx *ctx, int flag_ab, nsx_solve_info *info);
+/* nsteps time steps of nsmp3D.F90:404-564 on the resident state (FinRK
+ * stages, RK2 average of u,v,w, n <- n+1) */
+int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info *info);
+int nsx_solve_info_get(nsx_ctx *ctx, nsx_solve_info *info);
+
+/* ---- single routines on resident fields (unit parity, piecemeal use) ---- */
+/* BCglobalVC (BC.F90:202-295): ghost fill of u,v,w in place, vertex values */
+int nsx_bcglobalvc(nsx_ctx *ctx, int u, int v, int w, int uv, int vv, int wv, int tagdir);
+/* BCglobalP (BC.F90:299-363) */
+int nsx_bcglobalp(nsx_ctx *ctx, int p, int pv, int tagdir);
+/* grandientLSM (gradientLSM.F90:8-211); gradientP Option 2 (gradientP.F90:8-150) */
+int nsx_grad_lsm(nsx_ctx *ctx, int dfdx, int dfdy, int dfds, int f);
+/* calcul_face (interpolationFace.F90:7-238), Kim-Choi; f1..fB are face fields */
+int nsx_calcul_face(nsx_ctx *ctx, int f1, int f2, int f3, int fT, int fB, int fun, int funv);
+/* sgles (sgm_les.F90:24-275): eddy viscosity into NSX_F_EDDY */
+int nsx_sgles(nsx_ctx *ctx, int u, int v, int w);
+/* AdvDiffNS (AdvDiffNS.F90:8-140): rhs? += diffusion - advection, Gam = eddy + 1/Re,
+ * advecting with the resident U1FACE..UBFACE */
+int nsx_advdiff(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u, int v, int w, int eddy);
+/* Source3D (source3D.F90:8-174): faces into U1FACE..UBFACE, divergence into src */
+int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w);
+/* update_rhsp (source3D.F90:176-335): rhs -= lagged Kim-Choi flux of p */
+int nsx_update_rhsp(nsx_ctx *ctx, int rhs, int p, int pv);
+/* bicgstab (bicgstab.F90:8-168): mean removal, BiCGSTAB_SOLVE, mean removal,
+ * BCglobalP(tagdir 3); solves in place into phi, phiv */
+int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info);
+/* newu3D (newu3D.F90:8-257): U1FACE..UBFACE -= dt grad_n p */
+int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
+
+/* ---- timing ---------------------------------------------------------------- */
+/* per-kernel device time accumulated with HIP events since the last reset;
+ * names: "bicgstab_a", "bicgstab_b", "bicgstab_c", "stage" (all non-solver) */
+int nsx_timing_enable(nsx_ctx *ctx, int on);
+int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);
+int nsx_timing_reset(nsx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSX_H */

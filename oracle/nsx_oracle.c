@@ -27,6 +27,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 typedef struct orc {
   int N0, NC, NV, NZ, NEX;
@@ -52,6 +53,7 @@ typedef struct orc {
   int last_iter, n_solves, iter_total;
   double last_e1, last_e2;
   int last_exit; /* 0 converged, 1 no solve needed, 2 breakdown, 3 max iterations */
+  double t_solve, t_hydro; /* wall seconds spent in bicgstab / hydro */
   char err[256];
 } orc;
 
@@ -625,13 +627,21 @@ done:
   return iter;
 }
 
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* bicgstab, bicgstab.F90:8-168 (flg = 1 under KeyTESTChannel) */
 int orc_bicgstab(orc *o, double *phi, double *phiv, double *rhs) {
   const double *A[6] = {o->AmC0, o->AmC1, o->AmC2, o->AmC3, o->AmCT, o->AmCB};
+  double t0 = now();
   glob_norm(o, rhs, 1);
   int it = orc_bicgstab_solve(o, A, rhs, phi);
   glob_norm(o, phi, 0);
   orc_bcglobalp(o, phi, phiv, 3);
+  o->t_solve += now() - t0;
   return it;
 }
 
@@ -672,6 +682,7 @@ void orc_newu3d(orc *o, double *fun, double *funv) {
  * (ufnp,vfnp,wfnp,pfnp), vertex values, faces, rhs history and eddy. */
 int orc_hydro(orc *o) {
   const int N0 = o->N0, NZ = o->NZ;
+  const double t0 = now();
   size_t nz = (size_t)NCL * NZ;
   double c1 = 1.0, c2 = 0.0;
   if (o->flag_ab == 1) {
@@ -720,6 +731,7 @@ int orc_hydro(orc *o) {
   orc_newu3d(o, o->pfnp, o->pfv);
   if (o->FinRK == 1) o->flag_ab = 1;
   free(T);
+  o->t_hydro += now() - t0;
   return it;
 }
 
@@ -836,6 +848,11 @@ void orc_init_state(orc *o) {
   memcpy(o->wfnp, o->wfn, nz * sizeof(double));
   memcpy(o->pfnp, o->pfn, nz * sizeof(double));
   o->flag_ab = 0;
+}
+
+void orc_timers(orc *o, double *dv) {
+  dv[0] = o->t_solve;
+  dv[1] = o->t_hydro;
 }
 
 void orc_solver_record(orc *o, int *iv, double *dv) {

@@ -48,6 +48,7 @@ def lib() -> ctypes.CDLL:
         L.orc_step.argtypes = [vp]
         L.orc_hydro.argtypes = [vp]
         L.orc_solver_record.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(cd)]
+        L.orc_timers.argtypes = [vp, ctypes.POINTER(cd)]
         for name, n in [("orc_bcglobalvc", 6), ("orc_bcglobalp", 2), ("orc_grad_lsm", 4), ("orc_calcul_face", 7),
                         ("orc_advdiff", 7), ("orc_sgles", 3), ("orc_source3d", 4), ("orc_update_rhsp", 3),
                         ("orc_bicgstab", 3), ("orc_newu3d", 2), ("orc_interpolate_u", 3),
@@ -125,6 +126,18 @@ class Oracle:
         dv = (ctypes.c_double * 2)()
         lib().orc_solver_record(self.h, iv, dv)
         return dict(iter=iv[0], n_solves=iv[1], iter_total=iv[2], exit=iv[3], e1=dv[0], e2=dv[1])
+
+    def timers(self) -> dict:
+        """wall seconds spent in bicgstab and in hydro (solver included)"""
+        dv = (ctypes.c_double * 2)()
+        lib().orc_timers(self.h, dv)
+        return dict(solve=dv[0], hydro=dv[1])
+
+    def set_max_iters(self, n: int) -> None:
+        c = self.case
+        p = c.params
+        lib().orc_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, n, p.FinRK, p.Iturbu, p.TopBC,
+                             p.BotBC, c.bctop, c.bcbot, c.TotalVolume, p.sigIni, p.sigFin)
 
     # -- single routines on caller arrays (reference layout, modified in place)
     def call(self, name: str, *arrays, tagdir=None):

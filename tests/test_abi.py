@@ -1,0 +1,34 @@
+"""The C-ABI library loads on the CPU and exports every entry point of include/nsx.h."""
+import ctypes
+import os
+import re
+
+from conftest import REPO
+from nsx import abi
+
+HEADER = os.path.join(REPO, "include", "nsx.h")
+LIB = os.path.join(REPO, "3d-channel-flow-code_amd", "nsx", "_build", "libnsx.so")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(nsx_\w+)\(", txt, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert declared() == sorted(abi.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    assert os.path.exists(LIB), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    lib = ctypes.CDLL(LIB)
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_create_rejects_bad_sizes_without_touching_the_gpu():
+    lib = ctypes.CDLL(LIB)
+    abi.declare(lib)
+    h = ctypes.c_void_p()
+    assert lib.nsx_create(0, 0, 0, 0, ctypes.byref(h)) == -1
+    assert lib.nsx_create(10, 5, 4, 6, ctypes.byref(h)) == -1
