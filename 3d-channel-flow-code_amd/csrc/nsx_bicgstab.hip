@@ -22,8 +22,13 @@
 //   C: read x, p', s, t, invD, rhat; write x, r            8 x 8 B
 // (neighbour re-reads and the per-column ppec/area are cache hits).
 //
-// Each kernel exits at once when the device flag st->done is set, so the host
-// can queue many iterations ahead and poll rarely.
+// The scalars of a phase come from the partial sums of the phase before it:
+// every block of phase B sums phase A's partials and forms alpha itself,
+// phase C forms omega from B's, and phase A of iteration k+1 forms the norms,
+// the stopping test and rho_{k+1} from C's (see reduce_partials).  Block 0
+// records the scalars in SolverState for the host and for later phases.
+// Each kernel exits at once when st->done is set, so the host can queue many
+// iterations ahead and poll rarely.
 #include "nsx_kernels.h"
 
 #define FOR_EACH_CELL(g, c)                                                         \
@@ -70,12 +75,15 @@ __device__ inline double apply_row(const Geo &g, const PPEOp &op, int i, int k, 
 }
 
 // ------------------------------------------------------------ initial -----
-// bicgstab.F90:420-469 after Glob_Norm(rhs,1): r = rhat = b - A x0, norms
+// bicgstab.F90:420-451 after Glob_Norm(rhs,1) (bicgstab.F90:635-663): the
+// rhs average comes from the partial sums of k_source / k_gnorm_sum; r = rhat
+// = b - A x0 and the partial sums of the norms.
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, PPEOp op, const double *rhs, const double *phi,
-                                                          double *x, double *r, double *rhat, double *partials,
-                                                          unsigned int *ticket, SolverState *st, double eps,
-                                                          int max_iters) {
-  const double avec = st->avec_rhs;
+                                                          double *x, double *r, double *rhat, Partials P,
+                                                          SolverState *st, double eps, int max_iters) {
+  double tsrc[1];
+  reduce_partials<1>(P.src, gridDim.x, tsrc);
+  const double avec = tsrc[0] / (g.TotalVolume * (g.sigFin - g.sigIni));
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
@@ -98,41 +106,93 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, PPEOp op, const 
     acc[3] += fabs(res);
     acc[4] += ph * (area * g.dsig[k]);
   }
-  if (block_partials_and_ticket<5>(acc, partials, ticket)) {
-    double tot[5];
-    final_sums<5>(tot, partials, ticket);
-    if (threadIdx.x == 0) {
-      double b2 = sqrt(tot[0]), r2 = sqrt(tot[1]), b1 = tot[2], r1 = tot[3];
-      if (b2 == 0) b2 = 1.0;
-      if (b1 == 0) b1 = 1.0;
-      st->b1 = b1;
-      st->b2 = b2;
-      st->e2 = r2 / b2;
-      st->e1 = r1 / b1;
-      st->rho = tot[1];  // <rhat, r> with rhat = r
-      st->rhokm = 1.0;
-      st->alpha = 1.0;
-      st->omega = 1.0;
-      st->iter = 0;
-      st->eps = eps;
-      st->max_iters = max_iters;
-      st->xvol = tot[4];
-      const bool conv = (st->e1 < eps) || (st->e2 < eps);
-      st->done = conv ? 1 : 0;
-      st->exit_code = conv ? EXIT_NO_SOLVE : EXIT_CONVERGED;
-    }
+  write_partials<5>(acc, P.init);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->avec = avec;
+    st->eps = eps;
+    st->max_iters = max_iters;
+    st->b_alpha = 1.0;
+    st->c_omega = 1.0;
+    st->a_rho[0] = st->a_rho[1] = 1.0;
+    st->c_iter = 0;
+    st->a_iter = 0;
+    st->done = 0;
+    st->exit_code = EXIT_CONVERGED;
   }
 }
 
 // ------------------------------------------------------------- phase A ----
+// Closes iteration k-1 (norms, stopping test of bicgstab.F90:544-575 and the
+// breakdown guards of :479-490; at k = 0 the norms of :453-469), then
+// p = r + beta (p - omega v), v = A (p / A0), partial <rhat, v>.
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, PPEOp op, const double *r, const double *rhat,
                                                        const double *pold, const double *vold, double *pnew,
-                                                       double *vnew, double *partials, unsigned int *ticket,
-                                                       SolverState *st) {
+                                                       double *vnew, Partials P, SolverState *st) {
   if (st->done) return;
-  const bool first = st->iter == 0;
-  const double beta = first ? 0.0 : (st->rho / st->rhokm) * (st->alpha / st->omega);
-  const double om = st->omega;
+  const int k_it = st->c_iter;
+  const bool first = k_it == 0;
+  double rho, e1, e2, xvol, b1, b2;
+  int stop = 0, code = EXIT_CONVERGED;
+  if (first) {
+    double t[5];
+    reduce_partials<5>(P.init, gridDim.x, t);
+    b2 = sqrt(t[0]);
+    const double r2 = sqrt(t[1]);
+    b1 = t[2];
+    const double r1 = t[3];
+    if (b2 == 0) b2 = 1.0;
+    if (b1 == 0) b1 = 1.0;
+    e2 = r2 / b2;
+    e1 = r1 / b1;
+    rho = t[1];  // <rhat, r0> with rhat = r0
+    xvol = t[4];
+    if (e1 < st->eps || e2 < st->eps) {
+      stop = 1;
+      code = EXIT_NO_SOLVE;
+    }
+  } else {
+    double t[4];
+    reduce_partials<4>(P.c, gridDim.x, t);
+    b1 = st->b1;
+    b2 = st->b2;
+    const double r2 = sqrt(t[0]), r1 = t[1];
+    e2 = r2 / b2;
+    e1 = r1 / b1;
+    rho = t[2];
+    xvol = t[3];
+    double enorm = (double)1.0e10f;
+    if (enorm > e1) enorm = e1;
+    if (enorm > e2) enorm = e2;
+    const double rhokm = st->a_rho[(k_it - 1) & 1];
+    if (enorm < st->eps) {
+      stop = 1;
+      code = EXIT_CONVERGED;
+    } else if (k_it > st->max_iters) {
+      stop = 1;
+      code = EXIT_MAX;
+    } else if (fabs(rhokm) < TINY || fabs(st->c_omega) < TINY) {
+      stop = 1;
+      code = EXIT_BREAKDOWN;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->a_iter = k_it;
+    st->a_rho[k_it & 1] = rho;
+    st->e1 = e1;
+    st->e2 = e2;
+    st->xvol = xvol;
+    if (first) {
+      st->b1 = b1;
+      st->b2 = b2;
+    }
+    if (stop) {
+      st->exit_code = code;
+      st->done = 1;
+    }
+  }
+  if (stop) return;
+  const double om = st->c_omega;
+  const double beta = first ? 0.0 : (rho / st->a_rho[(k_it - 1) & 1]) * (st->b_alpha / om);
   auto pn = [&](size_t x) -> double { return first ? r[x] : r[x] + beta * (pold[x] - om * vold[x]); };
   double acc[1] = {0.0};
   FOR_EACH_CELL(g, c) {
@@ -145,26 +205,27 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, PPEOp op, const dou
     vnew[s] = vs;
     acc[0] += rhat[s] * vs;
   }
-  if (block_partials_and_ticket<1>(acc, partials, ticket)) {
-    double tot[1];
-    final_sums<1>(tot, partials, ticket);
-    if (threadIdx.x == 0) {
-      if (fabs(tot[0]) < TINY) {
-        st->done = 1;
-        st->exit_code = EXIT_BREAKDOWN;
-      } else {
-        st->alpha = st->rho / tot[0];
-      }
-    }
-  }
+  write_partials<1>(acc, P.a);
 }
 
 // ------------------------------------------------------------- phase B ----
+// alpha = rho / <rhat, v> (breakdown guard of :512-517), s = r - alpha v,
+// t = A (s / A0), partial <t,s>, <t,t>
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, PPEOp op, const double *r, const double *v,
-                                                       double *sout, double *tout, double *partials,
-                                                       unsigned int *ticket, SolverState *st) {
+                                                       double *sout, double *tout, Partials P, SolverState *st) {
   if (st->done) return;
-  const double al = st->alpha;
+  double t[1];
+  reduce_partials<1>(P.a, gridDim.x, t);
+  const int k_it = st->a_iter;
+  if (fabs(t[0]) < TINY) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->exit_code = EXIT_BREAKDOWN;
+      st->done = 1;
+    }
+    return;
+  }
+  const double al = st->a_rho[k_it & 1] / t[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->b_alpha = al;
   auto sn = [&](size_t x) -> double { return r[x] - al * v[x]; };
   double acc[2] = {0.0, 0.0};
   FOR_EACH_CELL(g, c) {
@@ -178,19 +239,23 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, PPEOp op, const dou
     acc[0] += ts * ss;
     acc[1] += ts * ts;
   }
-  if (block_partials_and_ticket<2>(acc, partials, ticket)) {
-    double tot[2];
-    final_sums<2>(tot, partials, ticket);
-    if (threadIdx.x == 0) st->omega = tot[0] / tot[1];
-  }
+  write_partials<2>(acc, P.b);
 }
 
 // ------------------------------------------------------------- phase C ----
+// omega = <t,s>/<t,t>, x += alpha p/A0 + omega s/A0, r = s - omega t,
+// partial |r|^2, |r|_1, <rhat, r>, sum x Vol (update_xkrk :1047-1111)
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, PPEOp op, double *x, double *r, const double *rhat,
                                                        const double *p, const double *sv, const double *tv,
-                                                       double *partials, unsigned int *ticket, SolverState *st) {
+                                                       Partials P, SolverState *st) {
   if (st->done) return;
-  const double al = st->alpha, om = st->omega;
+  double t[2];
+  reduce_partials<2>(P.b, gridDim.x, t);
+  const double al = st->b_alpha, om = t[0] / t[1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->c_omega = om;
+    st->c_iter = st->a_iter + 1;
+  }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
@@ -207,57 +272,23 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, PPEOp op, double *x
     acc[2] += rhat[s] * rr;
     acc[3] += xn * (g.area[i] * g.dsig[k]);
   }
-  if (block_partials_and_ticket<4>(acc, partials, ticket)) {
-    double tot[4];
-    final_sums<4>(tot, partials, ticket);
-    if (threadIdx.x == 0) {
-      const double r2 = sqrt(tot[0]), r1 = tot[1];
-      st->e2 = r2 / st->b2;
-      st->e1 = r1 / st->b1;
-      st->rhokm = st->rho;
-      st->iter = st->iter + 1;
-      st->xvol = tot[3];
-      double enorm = (double)1.0e10f;
-      if (enorm > st->e1) enorm = st->e1;
-      if (enorm > st->e2) enorm = st->e2;
-      if (enorm < st->eps) {
-        st->done = 1;
-        st->exit_code = EXIT_CONVERGED;
-      } else if (st->iter > st->max_iters) {
-        st->done = 1;
-        st->exit_code = EXIT_MAX;
-      } else if (fabs(st->rhokm) < TINY || fabs(st->omega) < TINY) {
-        st->done = 1;
-        st->exit_code = EXIT_BREAKDOWN;
-      }
-      st->rho = tot[2];
-    }
-  }
+  write_partials<4>(acc, P.c);
 }
 
 // ---------------------------------------------------------- Glob_Norm -----
-// bicgstab.F90:598-666.  mode 1: sum f (rhs, weights area*dsig(k) applied on
-// use); mode 0: sum f*area*dsig(k).
-__global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_sum(Geo g, const double *f, double *partials,
-                                                          unsigned int *ticket, SolverState *st, int mode) {
+// bicgstab.F90:598-666: partial sums of f (the rhs, tags = 1)
+__global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_sum(Geo g, const double *f, double *P) {
   double acc[1] = {0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
-    acc[0] += mode == 1 ? f[s] : f[s] * (g.area[i] * g.dsig[k]);
+    acc[0] += f[(size_t)i * g.NZ + k];
   }
-  if (block_partials_and_ticket<1>(acc, partials, ticket)) {
-    double tot[1];
-    final_sums<1>(tot, partials, ticket);
-    if (threadIdx.x == 0) {
-      if (mode == 1) st->avec_rhs = tot[0] / (g.TotalVolume * (g.sigFin - g.sigIni));
-      else st->xvol = tot[0];
-    }
-  }
+  write_partials<1>(acc, P);
 }
 
-// phi -= avec (Glob_Norm(phi,0)) and the Neumann ghost layers of BCpcenter
+// phi -= sum(phi Vol)/(TotalVolume (sigFin-sigIni)) (Glob_Norm(phi,0)) and
+// the Neumann ghost layers of BCpcenter
 __global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_apply(Geo g, double *x, const SolverState *st) {
   const double avec = st->xvol / (g.TotalVolume * (g.sigFin - g.sigIni));
   FOR_EACH_CELL(g, c) {
@@ -275,7 +306,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_apply(Geo g, double *x, con
 int launch_bicg_init(const Geo &g, hipStream_t s, PPEOp op, const double *rhs, const double *phi_in, double *x,
                      double *r, double *rhat, Partials part, SolverState *st, double eps, int max_iters) {
   hipLaunchKernelGGL(k_bicg_init, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, op, rhs, phi_in, x, r,
-                     rhat, part.buf, part.ticket, st, eps, max_iters);
+                     rhat, part, st, eps, max_iters);
   return (int)hipGetLastError();
 }
 
@@ -286,24 +317,23 @@ int launch_bicg_iter(const Geo &g, hipStream_t s, PPEOp op, int parity, double *
   double *pold = parity ? p1 : p0, *vold = parity ? v1 : v0;
   double *pnew = parity ? p0 : p1, *vnew = parity ? v0 : v1;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_bicg_a, grid, blk, 0, s, g, op, r, rhat, pold, vold, pnew, vnew, part.buf, part.ticket, st);
+  hipLaunchKernelGGL(k_bicg_a, grid, blk, 0, s, g, op, r, rhat, pold, vold, pnew, vnew, part, st);
   if (ev) {
     (void)hipEventRecord(ev[1], s);
     (void)hipEventRecord(ev[2], s);
   }
-  hipLaunchKernelGGL(k_bicg_b, grid, blk, 0, s, g, op, r, vnew, sv, tv, part.buf, part.ticket + 1, st);
+  hipLaunchKernelGGL(k_bicg_b, grid, blk, 0, s, g, op, r, vnew, sv, tv, part, st);
   if (ev) {
     (void)hipEventRecord(ev[3], s);
     (void)hipEventRecord(ev[4], s);
   }
-  hipLaunchKernelGGL(k_bicg_c, grid, blk, 0, s, g, op, x, r, rhat, pnew, sv, tv, part.buf, part.ticket + 2, st);
+  hipLaunchKernelGGL(k_bicg_c, grid, blk, 0, s, g, op, x, r, rhat, pnew, sv, tv, part, st);
   if (ev) (void)hipEventRecord(ev[5], s);
   return (int)hipGetLastError();
 }
 
-int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part, SolverState *st) {
-  hipLaunchKernelGGL(k_gnorm_sum, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, x, part.buf, part.ticket,
-                     st, 1);
+int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part) {
+  hipLaunchKernelGGL(k_gnorm_sum, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, x, part.src);
   return (int)hipGetLastError();
 }
 

@@ -524,9 +524,12 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   if ((rc = dev_alloc(ctx, &ctx->st_dev, 1))) return rc;
   HIPCHECK(hipMemsetAsync(ctx->st_dev, 0, sizeof(SolverState), ctx->stream));
   HIPCHECK(hipHostMalloc((void **)&ctx->st_host, sizeof(SolverState), hipHostMallocDefault));
-  if ((rc = dev_alloc(ctx, &ctx->part.buf, (size_t)8 * MAXGRID))) return rc;
-  if ((rc = dev_alloc(ctx, &ctx->part.ticket, 8))) return rc;
-  HIPCHECK(hipMemsetAsync(ctx->part.ticket, 0, 8 * sizeof(unsigned int), ctx->stream));
+  double **rows[5] = {&ctx->part.src, &ctx->part.init, &ctx->part.a, &ctx->part.b, &ctx->part.c};
+  const int nrow[5] = {1, 5, 1, 2, 4};
+  for (int q = 0; q < 5; ++q) {
+    if ((rc = dev_alloc(ctx, rows[q], (size_t)nrow[q] * MAXGRID))) return rc;
+    HIPCHECK(hipMemsetAsync(*rows[q], 0, (size_t)nrow[q] * MAXGRID * sizeof(double), ctx->stream));
+  }
 
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
@@ -673,16 +676,17 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
   }
   LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
   const SolverState &h = *ctx->st_host;
-  ctx->info.iters = h.iter;
+  ctx->info.iters = h.a_iter;
   ctx->info.exit_code = h.exit_code;
   ctx->info.e1norm = h.e1;
   ctx->info.e2norm = h.e2;
   ctx->info.n_solves += 1;
-  ctx->info.iters_total += h.iter;
-  ctx->iter_hint = h.iter > 0 ? h.iter : ctx->iter_hint;
+  ctx->info.iters_total += h.a_iter;
+  ctx->iter_hint = h.a_iter > 0 ? h.a_iter : ctx->iter_hint;
   // time only the iterations that ran, not the queued launches that found
   // the solver finished
-  if (ctx->timing && ctx->pending.size() > pend0 + 3 * (size_t)h.iter) ctx->pending.resize(pend0 + 3 * (size_t)h.iter);
+  if (ctx->timing && ctx->pending.size() > pend0 + 3 * (size_t)h.a_iter)
+    ctx->pending.resize(pend0 + 3 * (size_t)h.a_iter);
   collect_timing(ctx);
   return 0;
 }
@@ -737,7 +741,7 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   const double *us[3] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2]};
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
-  LAUNCH(launch_source(g, s, us, facew, nullptr, ctx->rhsp, ctx->slot[NSX_F_PFV], ctx->part, ctx->st_dev));
+  LAUNCH(launch_source(g, s, us, facew, nullptr, ctx->rhsp, ctx->slot[NSX_F_PFV], ctx->part));
   if (ev) {
     (void)hipEventRecord(ev[1], s);
     ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 3});
@@ -793,7 +797,7 @@ extern "C" int nsx_init_state(nsx_ctx *ctx) {
   if ((rc = vertex_of(ctx, 1, pin, pfv, pk, 3))) return rc;
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
-  LAUNCH(launch_source(g, s, uvw, facew, nullptr, nullptr, nullptr, ctx->part, ctx->st_dev));
+  LAUNCH(launch_source(g, s, uvw, facew, nullptr, nullptr, nullptr, ctx->part));
   const size_t nc = (size_t)ctx->N0 * ctx->NZ;
   for (int id : {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY})
     HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, nc * sizeof(double), s));
@@ -971,7 +975,7 @@ extern "C" int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w) {
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   HIPCHECK(hipMemsetAsync(ctx->slot[src], 0, (size_t)ctx->N0 * ctx->NZ * sizeof(double), ctx->stream));
-  LAUNCH(launch_source(ctx->g, ctx->stream, f, facew, ctx->slot[src], nullptr, nullptr, ctx->part, ctx->st_dev));
+  LAUNCH(launch_source(ctx->g, ctx->stream, f, facew, ctx->slot[src], nullptr, nullptr, ctx->part));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -989,7 +993,7 @@ extern "C" int nsx_update_rhsp(nsx_ctx *ctx, int rhs, int p, int pv) {
 extern "C" int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info) {
   NEED_COMMIT(ctx);
   if (kind_ok(ctx, phi, FK_CELL) || kind_ok(ctx, phiv, FK_VERT) || kind_ok(ctx, rhs, FK_CELL)) return -1;
-  LAUNCH(launch_gnorm_sum(ctx->g, ctx->stream, ctx->slot[rhs], ctx->part, ctx->st_dev));
+  LAUNCH(launch_gnorm_sum(ctx->g, ctx->stream, ctx->slot[rhs], ctx->part));
   int rc = solve(ctx, ctx->slot[rhs], ctx->slot[phi], ctx->slot[phi]);
   if (rc) return rc;
   if ((rc = bcglobalp_(ctx, phi, phiv, 3))) return rc;

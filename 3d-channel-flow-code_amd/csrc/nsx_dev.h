@@ -108,9 +108,14 @@ __device__ inline void cell_of(const Geo &g, uint32_t c, int &col, int &k) {
 }
 
 // ---------------------------------------------------------- reductions ---
-// Deterministic grid reduction: every block writes its partial sums to
-// partials[q*G + block]; the last block to finish (atomic ticket) sums the
-// partials in block order.  Same inputs -> same bits, run after run.
+// Deterministic grid reductions without device-scope fences.  A kernel that
+// produces sums writes one partial per block to P[q*G + block]; the kernel
+// that needs the totals runs next on the stream (the kernel boundary makes
+// the partials visible) and every one of its blocks sums the G partials in
+// the same fixed order, so all blocks hold bit-identical totals and take the
+// same decisions.  (A last-block/ticket reduction needs a device-scope
+// release fence per block, which on gfx950 writes back the XCD's L2; with
+// thousands of blocks that costs more than the stencil itself.)
 template <int K>
 __device__ inline void wave_sum(double (&v)[K]) {
 #pragma unroll
@@ -120,43 +125,35 @@ __device__ inline void wave_sum(double (&v)[K]) {
 }
 
 template <int K>
-__device__ inline bool block_partials_and_ticket(double (&v)[K], double *partials, unsigned int *ticket) {
+__device__ inline void write_partials(double (&v)[K], double *P) {
   __shared__ double sh[K][NSX_BLOCK / 64];
-  __shared__ bool last;
   wave_sum<K>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0)
 #pragma unroll
     for (int q = 0; q < K; ++q) sh[q][wid] = v[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       double s = 0.0;
       for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[q][w];
-      partials[(size_t)q * gridDim.x + blockIdx.x] = s;
+      P[(size_t)q * gridDim.x + blockIdx.x] = s;
     }
-    __threadfence();
-    unsigned int prev = atomicAdd(ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  return last;
 }
 
-// Called by all threads of the last block: totals of the K partial rows.
+// totals of the K partial rows written by the previous kernel (grid G); every
+// thread of the block receives them
 template <int K>
-__device__ inline void final_sums(double (&tot)[K], const double *partials, unsigned int *ticket) {
-  __threadfence();
+__device__ inline void reduce_partials(const double *P, uint32_t G, double (&tot)[K]) {
+  __shared__ double sh[K][NSX_BLOCK / 64];
   double v[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) {
     double s = 0.0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += NSX_BLOCK)
-      s += partials[(size_t)q * gridDim.x + b];
+    for (uint32_t b = threadIdx.x; b < G; b += NSX_BLOCK) s += P[(size_t)q * G + b];
     v[q] = s;
   }
-  __shared__ double sh[K][NSX_BLOCK / 64];
   wave_sum<K>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0)
@@ -169,7 +166,7 @@ __device__ inline void final_sums(double (&tot)[K], const double *partials, unsi
     for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[q][w];
     tot[q] = s;
   }
-  if (threadIdx.x == 0) *ticket = 0u;
+  __syncthreads();
 }
 
 #define HIPCHECK(x)                                                     \

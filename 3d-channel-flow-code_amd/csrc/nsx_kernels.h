@@ -3,21 +3,31 @@
 #pragma once
 #include "nsx_dev.h"
 
+// Per-block partial sums of one reduction site: [rows][grid]
 struct Partials {
-  double *buf;            // [8][maxgrid]
-  unsigned int *ticket;   // one per reduction site
+  double *src;   // Glob_Norm(rhs,1) sum (k_source / k_gnorm_sum), 1 row
+  double *init;  // initial residual norms, 5 rows
+  double *a;     // <rhat, v>, 1 row
+  double *b;     // <t,s>, <t,t>, 2 rows
+  double *c;     // |r|^2, |r|_1, <rhat,r>, sum x Vol, 4 rows
 };
 
-// BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals)
+// BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals).  Each
+// field has one writer kernel (block 0 of it) and is read only by later
+// kernels, so no block reads a value that its own kernel may be rewriting.
 struct SolverState {
-  double rho, rhokm, alpha, omega;
-  double b1, b2, e1, e2;
-  double avec_rhs;  // Glob_Norm(rhs,1) average
-  double xvol;      // sum x*Vol of the current iterate (Glob_Norm(phi,0))
-  int iter, done, exit_code, max_iters;
-  double eps;
-  int iters_cap;    // iterations the host allowed so far (host-driven chunks)
-  int pad;
+  double a_rho[2];   // rho_k = <rhat, r_k> by parity of k        (phase A)
+  double b_alpha;    // alpha_k                                    (phase B; init: 1)
+  double c_omega;    // omega_k                                    (phase C; init: 1)
+  double b1, b2;     // |b|_1, |b|_2 (0 -> 1)                      (phase A, k = 0)
+  double e1, e2;     // relative residual norms of x_k             (phase A)
+  double xvol;       // sum x_k Vol for Glob_Norm(phi,0)            (phase A)
+  double avec;       // Glob_Norm(rhs,1) average                   (init)
+  double eps;        //                                            (init)
+  int c_iter;        // iterations completed before phase A runs   (phase C; init: 0)
+  int a_iter;        // k of the last phase A                      (phase A)
+  int done, exit_code;                                        //   (phase A or B)
+  int max_iters, pad;                                         //   (init)
 };
 
 // Compact PPE operator: AmC_j(i,k) = ppec(i,j)*dsigv(k-1), AmCT(i,k) = W(i,k),
@@ -39,7 +49,7 @@ int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], con
                     const double *const face[5], const double *eddy, const double *const rhs_in[3],
                     double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2);
 int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
-                  double *rhsp_out, const double *pv, Partials part, SolverState *st);
+                  double *rhsp_out, const double *pv, Partials part);
 int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *pv);
 int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, double *dy, double *ds);
 int launch_calcul_face(const Geo &g, hipStream_t s, const double *f, const double *fv, double *const out[5]);
@@ -57,4 +67,4 @@ int launch_bicg_iter(const Geo &g, hipStream_t s, PPEOp op, int parity, double *
                      double *p0, double *v0, double *p1, double *v1, double *sv, double *tv, Partials part,
                      SolverState *st, hipEvent_t *ev /* nullable: 6 events, start/end of phases A, B, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
-int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part, SolverState *st);
+int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

@@ -314,7 +314,7 @@ int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], con
 __global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, const double *v, const double *w,
                                                        double *F1, double *F2, double *F3, double *FT, double *FB,
                                                        double *src_out, double *rhsp_out, const double *pv,
-                                                       double *partials, unsigned int *ticket, SolverState *st) {
+                                                       double *partials) {
   double acc[1] = {0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
@@ -360,17 +360,13 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, co
     }
   }
   if (!rhsp_out) return;
-  if (block_partials_and_ticket<1>(acc, partials, ticket)) {
-    double tot[1];
-    final_sums<1>(tot, partials, ticket);
-    if (threadIdx.x == 0) st->avec_rhs = tot[0] / (g.TotalVolume * (g.sigFin - g.sigIni));
-  }
+  write_partials<1>(acc, partials);
 }
 
 int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
-                  double *rhsp_out, const double *pv, Partials part, SolverState *st) {
+                  double *rhsp_out, const double *pv, Partials part) {
   hipLaunchKernelGGL(k_source, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, uvw[0], uvw[1], uvw[2],
-                     face[0], face[1], face[2], face[3], face[4], src_out, rhsp_out, pv, part.buf, part.ticket, st);
+                     face[0], face[1], face[2], face[3], face[4], src_out, rhsp_out, pv, part.src);
   return (int)hipGetLastError();
 }
 

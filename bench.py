@@ -74,6 +74,7 @@ def main() -> int:
     ap.add_argument("--cfg", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -91,7 +92,7 @@ def main() -> int:
     if args.warmup:
         ctx.step(args.warmup)
     info0 = ctx.solve_info()
-    ctx.timing(True)
+    ctx.timing(not args.no_timing)
     ctx.timing_reset()
     t0 = time.perf_counter()
     info = ctx.step(steps)
