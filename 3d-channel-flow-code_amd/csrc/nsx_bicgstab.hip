@@ -1,34 +1,34 @@
 // nsx_bicgstab.hip -- Jacobi-preconditioned BiCGSTAB on the 7-point prism
-// stencil (bicgstab.F90:303-591), three fused phases per iteration, one per
-// global synchronisation point of the algorithm:
+// stencil (bicgstab.F90:303-591).  One iteration is five kernels, split at
+// the algorithm's three global synchronisation points and at the two
+// stencil applications:
 //
-//   A  p = r + beta (p - omega v)   (p = r at iter 0)     update_pk   :726-773
-//      v = A (p / A0)               (Jacobi + SpMV)       diag_precdt :779-925, get_Apk :931-1001
-//      <rhat, v>  -> alpha = rho / <rhat, v>                               :509-520
-//   B  s = r - alpha v ; t = A (s / A0) ; <t,s>, <t,t> -> omega          :523-537
-//   C  x += alpha p/A0 + omega s/A0 ; r = s - omega t                    update_xkrk :1047-1111
-//      ||r||_2^2, ||r||_1, <rhat, r> (next rho), sum x Vol (Glob_Norm)
+//   A1  stop test of k-1, p = r + beta (p - omega v), y = p / A0   update_pk :726-773, diag_precdt :831-836
+//   A2  v = A y, partial <rhat, v>                                 get_Apk :931-1001
+//   B1  alpha = rho / <rhat,v>, s = r - alpha v, z = s / A0        update_sk :1007-1041
+//   B2  t = A z, partial <t,s>, <t,t>
+//   C   omega = <t,s>/<t,t>, x += alpha y + omega z, r = s - omega t,
+//       partial |r|^2, |r|_1, <rhat, r>, sum x Vol                 update_xkrk :1047-1111
 //
-// The stencil input of A and B (p and s) is recomputed at the 7 points from
-// the previous iterates, so p and s never make a separate pass through HBM;
-// p and v are double-buffered across iterations.  The matrix is PDE_center's,
-// regenerated per row from the edge coefficient ppec(i,j), the vertical
-// coefficient W(i,k) and dsigv (bit-identical to AmC0..AmCB), and the Jacobi
-// scaling multiplies by invD = 1/AmC0.
+// Only y and z are gathered at the 7 points, so each stencil reads one
+// neighbour array.  The matrix is PDE_center's, regenerated per row from the
+// per-edge coefficient ppec, the cell area and the sigma spacings (bit-
+// identical to AmC0..AmCB, checked at set-up), so no coefficient array is
+// read; y = p / A0 and z = s / A0 divide exactly as diag_precdt does.
 //
-// Algorithmic HBM bytes per interior prism and iteration (FP64):
-//   A: read r, p, v, rhat, W, invD; write p', v'          8 x 8 B
-//   B: read r, v', W, invD; write s, t                      6 x 8 B
-//   C: read x, p', s, t, invD, rhat; write x, r            8 x 8 B
-// (neighbour re-reads and the per-column ppec/area are cache hits).
+// Algorithmic HBM bytes per interior prism and iteration (FP64, 8 B each):
+//   A1 read r, p, v; write p, y        5      A2 read y, rhat; write v    3
+//   B1 read r, v; write s, z           4      B2 read z, s; write t       3
+//   C  read x, y, z, s, t, rhat; write x, r                              8
+//   total 23 x 8 B = 184 B (neighbour re-reads of y, z are cache hits).
+// Host-side timing brackets each of the five kernels with HIP events.
 //
 // The scalars of a phase come from the partial sums of the phase before it:
-// every block of phase B sums phase A's partials and forms alpha itself,
-// phase C forms omega from B's, and phase A of iteration k+1 forms the norms,
-// the stopping test and rho_{k+1} from C's (see reduce_partials).  Block 0
-// records the scalars in SolverState for the host and for later phases.
-// Each kernel exits at once when st->done is set, so the host can queue many
-// iterations ahead and poll rarely.
+// B1 sums A2's partials and forms alpha itself, C forms omega from B2's, and
+// A1 of iteration k+1 forms the norms, the stopping test and rho_{k+1} from
+// C's (see reduce_partials).  Block 0 records the scalars in SolverState for
+// the host and for later phases.  Each kernel exits at once when st->done is
+// set, so the host can queue many iterations ahead and poll rarely.
 #include "nsx_kernels.h"
 
 #define FOR_EACH_CELL(g, c)                                                         \
@@ -41,16 +41,19 @@
 
 enum { EXIT_CONVERGED = 0, EXIT_NO_SOLVE = 1, EXIT_BREAKDOWN = 2, EXIT_MAX = 3 };
 
-// Row of the PPE matrix at (i,k): diagonal and the 5 off-diagonals
+// Row of the PPE matrix at (i,k), regenerated from the geometry exactly as
+// PDE_center forms it (PDE.F90:76-109): AA_j = (dle1/dlCC)_j dlVV_j dsigv(k-1)
+// (ppec holds the first two factors), AAB = area/dsig(k-1), AAT = area/dsig(k),
+// zero at the sigma walls, AA0 = -(AAT+AAB+AA1+AA2+AA3).
 struct Row {
   double A0, a1, a2, a3, AT, AB;
 };
 
-__device__ inline Row ppe_row(const Geo &g, const PPEOp &op, int i, int k, size_t s) {
+__device__ inline Row ppe_row(const Geo &g, int i, int k) {
   Row r;
-  const double sv = g.dsigv[k - 1];
-  r.AT = op.W[s];
-  r.AB = op.W[s - 1];
+  const double sv = g.dsigv[k - 1], area = g.area[i];
+  r.AT = (k == g.NZ - 2 && g.bctop == 1) ? 0.0 : area / g.dsig[k];
+  r.AB = (k == 1 && g.bcbot == 1) ? 0.0 : area / g.dsig[k - 1];
   r.a1 = g.ppec[i * 3] * sv;
   r.a2 = g.ppec[i * 3 + 1] * sv;
   r.a3 = g.ppec[i * 3 + 2] * sv;
@@ -58,19 +61,15 @@ __device__ inline Row ppe_row(const Geo &g, const PPEOp &op, int i, int k, size_
   return r;
 }
 
-// y = A (f * invD) at (i,k) where f(x) is evaluated at the 7 points; the
-// vertical ghost of a pressure-like vector is the Neumann copy (BCpcenter)
-template <class F>
-__device__ inline double apply_row(const Geo &g, const PPEOp &op, int i, int k, size_t s, F f, double &fs) {
-  const Row A = ppe_row(g, op, i, k, s);
-  fs = f(s);
-  const double ys = fs * op.invD[s];
-  const double yu = (k + 1 <= g.NZ - 2) ? f(s + 1) * op.invD[s + 1] : ys;
-  const double yd = (k - 1 >= 1) ? f(s - 1) * op.invD[s - 1] : ys;
+// A y at (i,k) (get_Apk, bicgstab.F90:976-998): the vertical ghost of y is
+// the Neumann copy of BCpcenter, the periodic ghosts are the partner columns
+__device__ inline double spmv_row(const Geo &g, const Row &A, int i, int k, size_t s, const double *y) {
+  const double ys = y[s];
+  const double yu = (k + 1 <= g.NZ - 2) ? y[s + 1] : ys;
+  const double yd = (k - 1 >= 1) ? y[s - 1] : ys;
   const int e = i * 3;
-  const size_t x0 = (size_t)g.nbr[e] * g.NZ + k, x1 = (size_t)g.nbr[e + 1] * g.NZ + k,
-               x2 = (size_t)g.nbr[e + 2] * g.NZ + k;
-  const double y0 = f(x0) * op.invD[x0], y1 = f(x1) * op.invD[x1], y2 = f(x2) * op.invD[x2];
+  const double y0 = y[(size_t)g.nbr[e] * g.NZ + k], y1 = y[(size_t)g.nbr[e + 1] * g.NZ + k],
+               y2 = y[(size_t)g.nbr[e + 2] * g.NZ + k];
   return A.A0 * ys + A.a1 * y0 + A.a2 * y1 + A.a3 * y2 + A.AT * yu + A.AB * yd;
 }
 
@@ -78,7 +77,7 @@ __device__ inline double apply_row(const Geo &g, const PPEOp &op, int i, int k, 
 // bicgstab.F90:420-451 after Glob_Norm(rhs,1) (bicgstab.F90:635-663): the
 // rhs average comes from the partial sums of k_source / k_gnorm_sum; r = rhat
 // = b - A x0 and the partial sums of the norms.
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, PPEOp op, const double *rhs, const double *phi,
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, const double *rhs, const double *phi,
                                                           double *x, double *r, double *rhat, Partials P,
                                                           SolverState *st, double eps, int max_iters) {
   double tsrc[1];
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, PPEOp op, const 
     const size_t s = (size_t)i * g.NZ + k;
     const double area = g.area[i];
     const double b = rhs[s] - avec * area * g.dsig[k];
-    const Row A = ppe_row(g, op, i, k, s);
+    const Row A = ppe_row(g, i, k);
     const int e = i * 3;
     const double ph = phi[s];
     const double Ax = A.A0 * ph + A.a1 * phi[(size_t)g.nbr[e] * g.NZ + k] + A.a2 * phi[(size_t)g.nbr[e + 1] * g.NZ + k] +
@@ -121,13 +120,13 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init(Geo g, PPEOp op, const 
   }
 }
 
-// ------------------------------------------------------------- phase A ----
-// Closes iteration k-1 (norms, stopping test of bicgstab.F90:544-575 and the
-// breakdown guards of :479-490; at k = 0 the norms of :453-469), then
-// p = r + beta (p - omega v), v = A (p / A0), partial <rhat, v>.
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, PPEOp op, const double *r, const double *rhat,
-                                                       const double *pold, const double *vold, double *pnew,
-                                                       double *vnew, Partials P, SolverState *st) {
+// ------------------------------------------------------------ phase A1 ----
+// Closes iteration k-1 (norms and the stopping test of bicgstab.F90:544-575,
+// the breakdown guards of :479-490; at k = 0 the norms of :453-469), then
+// update_pk (:726-773) p = r + beta (p - omega v) (p = r at k = 0) and
+// diag_precdt Opt 1 (:831-836) y = p / AmC0.  Point-wise, in place.
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a1(Geo g, const double *r, double *p, const double *v,
+                                                        double *y, Partials P, SolverState *st) {
   if (st->done) return;
   const int k_it = st->c_iter;
   const bool first = k_it == 0;
@@ -193,26 +192,38 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, PPEOp op, const dou
   if (stop) return;
   const double om = st->c_omega;
   const double beta = first ? 0.0 : (rho / st->a_rho[(k_it - 1) & 1]) * (st->b_alpha / om);
-  auto pn = [&](size_t x) -> double { return first ? r[x] : r[x] + beta * (pold[x] - om * vold[x]); };
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const double pn = first ? r[s] : r[s] + beta * (p[s] - om * v[s]);
+    p[s] = pn;
+    y[s] = pn / ppe_row(g, i, k).A0;
+  }
+}
+
+// ------------------------------------------------------------ phase A2 ----
+// get_Apk (:931-1001) v = A y, and the partial sums of <rhat, v> (:509)
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a2(Geo g, const double *y, const double *rhat, double *v,
+                                                        Partials P, const SolverState *st) {
+  if (st->done) return;
   double acc[1] = {0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NZ + k;
-    double ps;
-    const double vs = apply_row(g, op, i, k, s, pn, ps);
-    pnew[s] = ps;
-    vnew[s] = vs;
+    const double vs = spmv_row(g, ppe_row(g, i, k), i, k, s, y);
+    v[s] = vs;
     acc[0] += rhat[s] * vs;
   }
   write_partials<1>(acc, P.a);
 }
 
-// ------------------------------------------------------------- phase B ----
-// alpha = rho / <rhat, v> (breakdown guard of :512-517), s = r - alpha v,
-// t = A (s / A0), partial <t,s>, <t,t>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, PPEOp op, const double *r, const double *v,
-                                                       double *sout, double *tout, Partials P, SolverState *st) {
+// ------------------------------------------------------------ phase B1 ----
+// alpha = rho / <rhat, v> (breakdown guard of :512-517), update_sk
+// (:1007-1041) s = r - alpha v, diag_precdt z = s / AmC0.  Point-wise.
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b1(Geo g, const double *r, const double *v, double *sv,
+                                                        double *z, Partials P, SolverState *st) {
   if (st->done) return;
   double t[1];
   reduce_partials<1>(P.a, gridDim.x, t);
@@ -226,28 +237,40 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, PPEOp op, const dou
   }
   const double al = st->a_rho[k_it & 1] / t[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) st->b_alpha = al;
-  auto sn = [&](size_t x) -> double { return r[x] - al * v[x]; };
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NZ + k;
+    const double ss = r[s] - al * v[s];
+    sv[s] = ss;
+    z[s] = ss / ppe_row(g, i, k).A0;
+  }
+}
+
+// ------------------------------------------------------------ phase B2 ----
+// get_Apk t = A z and the partial sums of <t,s>, <t,t> (:535-536)
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b2(Geo g, const double *z, const double *sv, double *tv,
+                                                        Partials P, const SolverState *st) {
+  if (st->done) return;
   double acc[2] = {0.0, 0.0};
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NZ + k;
-    double ss;
-    const double ts = apply_row(g, op, i, k, s, sn, ss);
-    sout[s] = ss;
-    tout[s] = ts;
-    acc[0] += ts * ss;
+    const double ts = spmv_row(g, ppe_row(g, i, k), i, k, s, z);
+    tv[s] = ts;
+    acc[0] += ts * sv[s];
     acc[1] += ts * ts;
   }
   write_partials<2>(acc, P.b);
 }
 
 // ------------------------------------------------------------- phase C ----
-// omega = <t,s>/<t,t>, x += alpha p/A0 + omega s/A0, r = s - omega t,
-// partial |r|^2, |r|_1, <rhat, r>, sum x Vol (update_xkrk :1047-1111)
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, PPEOp op, double *x, double *r, const double *rhat,
-                                                       const double *p, const double *sv, const double *tv,
-                                                       Partials P, SolverState *st) {
+// omega = <t,s>/<t,t>, update_xkrk (:1047-1111) x += alpha y + omega z,
+// r = s - omega t, partial |r|^2, |r|_1, <rhat, r>, sum x Vol
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, double *x, double *r, const double *rhat,
+                                                       const double *y, const double *z, const double *sv,
+                                                       const double *tv, Partials P, SolverState *st) {
   if (st->done) return;
   double t[2];
   reduce_partials<2>(P.b, gridDim.x, t);
@@ -261,9 +284,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, PPEOp op, double *x
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NZ + k;
-    const double d = op.invD[s];
-    const double y = p[s] * d, z = sv[s] * d;
-    const double xn = x[s] + al * y + om * z;
+    const double xn = x[s] + al * y[s] + om * z[s];
     const double rr = sv[s] - om * tv[s];
     x[s] = xn;
     r[s] = rr;
@@ -303,32 +324,41 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_apply(Geo g, double *x, con
 }
 
 // ------------------------------------------------------------ launchers ---
-int launch_bicg_init(const Geo &g, hipStream_t s, PPEOp op, const double *rhs, const double *phi_in, double *x,
-                     double *r, double *rhat, Partials part, SolverState *st, double eps, int max_iters) {
-  hipLaunchKernelGGL(k_bicg_init, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, op, rhs, phi_in, x, r,
+int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
+                     double *rhat, Partials part, SolverState *st, double eps, int max_iters) {
+  hipLaunchKernelGGL(k_bicg_init, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, rhs, phi_in, x, r,
                      rhat, part, st, eps, max_iters);
   return (int)hipGetLastError();
 }
 
-int launch_bicg_iter(const Geo &g, hipStream_t s, PPEOp op, int parity, double *x, double *r, const double *rhat,
-                     double *p0, double *v0, double *p1, double *v1, double *sv, double *tv, Partials part,
-                     SolverState *st, hipEvent_t *ev) {
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
+                     double *y, double *z, double *sv, double *tv, Partials part, SolverState *st, hipEvent_t *ev) {
   const dim3 grid(nsx_grid_for(g.ncell)), blk(NSX_BLOCK);
-  double *pold = parity ? p1 : p0, *vold = parity ? v1 : v0;
-  double *pnew = parity ? p0 : p1, *vnew = parity ? v0 : v1;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_bicg_a, grid, blk, 0, s, g, op, r, rhat, pold, vold, pnew, vnew, part, st);
-  if (ev) {
-    (void)hipEventRecord(ev[1], s);
-    (void)hipEventRecord(ev[2], s);
-  }
-  hipLaunchKernelGGL(k_bicg_b, grid, blk, 0, s, g, op, r, vnew, sv, tv, part, st);
-  if (ev) {
-    (void)hipEventRecord(ev[3], s);
-    (void)hipEventRecord(ev[4], s);
-  }
-  hipLaunchKernelGGL(k_bicg_c, grid, blk, 0, s, g, op, x, r, rhat, pnew, sv, tv, part, st);
+  hipLaunchKernelGGL(k_bicg_a1, grid, blk, 0, s, g, r, p, v, y, part, st);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(k_bicg_a2, grid, blk, 0, s, g, y, rhat, v, part, st);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL(k_bicg_b1, grid, blk, 0, s, g, r, v, sv, z, part, st);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL(k_bicg_b2, grid, blk, 0, s, g, z, sv, tv, part, st);
+  if (ev) (void)hipEventRecord(ev[4], s);
+  hipLaunchKernelGGL(k_bicg_c, grid, blk, 0, s, g, x, r, rhat, y, z, sv, tv, part, st);
   if (ev) (void)hipEventRecord(ev[5], s);
+  return (int)hipGetLastError();
+}
+
+// the regenerated diagonal AmC0 for the set-up check against PDE_center's
+__global__ __launch_bounds__(NSX_BLOCK) void k_ppe_diag(Geo g, double *A0) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    A0[(size_t)i * g.NZ + k] = ppe_row(g, i, k).A0;
+  }
+}
+
+int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0) {
+  hipLaunchKernelGGL(k_ppe_diag, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, A0);
   return (int)hipGetLastError();
 }
 

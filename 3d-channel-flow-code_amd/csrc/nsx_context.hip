@@ -54,7 +54,6 @@ struct nsx_ctx {
   int set_n = 0;                 // pool set holding time level n
   double *ustar[3] = {}, *uv2[3] = {}, *rhsp = nullptr;
   double *sr = nullptr, *srhat = nullptr, *sp[2] = {}, *svv[2] = {}, *ss = nullptr, *st_ = nullptr;
-  double *W = nullptr, *invD = nullptr;
   SolverState *st_dev = nullptr, *st_host = nullptr;
   Partials part{};
   nsx_solve_info info{};
@@ -63,8 +62,8 @@ struct nsx_ctx {
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  Timer t_a, t_b, t_c, t_stage;
-  std::vector<std::pair<size_t, int>> pending;  // (event index, which timer: 0 a,1 b,2 c,3 stage)
+  Timer t_k[5], t_stage;  // A1, A2, B1, B2, C; non-solver stage kernels
+  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..4 solver kernels, 5 stage)
 };
 
 static int nsx_fail(nsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -499,7 +498,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   }
   if ((rc = cell(&ctx->rhsp)) || (rc = cell(&ctx->sr)) || (rc = cell(&ctx->srhat)) || (rc = cell(&ctx->sp[0])) ||
       (rc = cell(&ctx->sp[1])) || (rc = cell(&ctx->svv[0])) || (rc = cell(&ctx->svv[1])) || (rc = cell(&ctx->ss)) ||
-      (rc = cell(&ctx->st_)) || (rc = cell(&ctx->W)) || (rc = cell(&ctx->invD)))
+      (rc = cell(&ctx->st_)))
     return rc;
   // the time-level slots point into pool set 0 (n), 1 (stage), 2 (new)
   ctx->set_n = 0;
@@ -534,9 +533,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
   if ((rc = cell(&A0chk))) return rc;
-  HIPCHECK(hipMemsetAsync(ctx->W, 0, nc * sizeof(double), ctx->stream));
-  HIPCHECK(hipMemsetAsync(ctx->invD, 0, nc * sizeof(double), ctx->stream));
-  LAUNCH(launch_ppe_build(g, ctx->stream, ctx->W, ctx->invD, A0chk));
+  LAUNCH(launch_ppe_diag(g, ctx->stream, A0chk));
   std::vector<double> a0((size_t)nc);
   HIPCHECK(hipMemcpyAsync(a0.data(), A0chk, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -623,7 +620,7 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[4] = {&ctx->t_a, &ctx->t_b, &ctx->t_c, &ctx->t_stage};
+  Timer *t[6] = {&ctx->t_k[0], &ctx->t_k[1], &ctx->t_k[2], &ctx->t_k[3], &ctx->t_k[4], &ctx->t_stage};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -641,10 +638,9 @@ static void collect_timing(nsx_ctx *ctx) {
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
-  PPEOp op{ctx->W, ctx->invD};
-  LAUNCH(launch_bicg_init(g, s, op, rhs, phi_in, x, ctx->sr, ctx->srhat, ctx->part, ctx->st_dev, ctx->eps,
+  LAUNCH(launch_bicg_init(g, s, rhs, phi_in, x, ctx->sr, ctx->srhat, ctx->part, ctx->st_dev, ctx->eps,
                           ctx->MaxIters));
-  int issued = 0, parity = 0;
+  int issued = 0;
   const size_t pend0 = ctx->pending.size();
   // queue about as many iterations as the previous solve needed, then top up
   // in small chunks: few host round trips and few no-op launches
@@ -657,15 +653,12 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
         ev = events(ctx, 6);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_bicg_iter(g, s, op, parity, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->svv[0], ctx->sp[1],
-                              ctx->svv[1], ctx->ss, ctx->st_, ctx->part, ctx->st_dev, ev));
+      LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->svv[0], ctx->sp[1], ctx->svv[1],
+                              ctx->ss, ctx->st_, ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
-        ctx->pending.push_back({base + 0, 0});
-        ctx->pending.push_back({base + 2, 1});
-        ctx->pending.push_back({base + 4, 2});
+        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, q});
       }
-      parity ^= 1;
       ++issued;
     }
     HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -685,8 +678,8 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
   ctx->iter_hint = h.a_iter > 0 ? h.a_iter : ctx->iter_hint;
   // time only the iterations that ran, not the queued launches that found
   // the solver finished
-  if (ctx->timing && ctx->pending.size() > pend0 + 3 * (size_t)h.a_iter)
-    ctx->pending.resize(pend0 + 3 * (size_t)h.a_iter);
+  if (ctx->timing && ctx->pending.size() > pend0 + 5 * (size_t)h.a_iter)
+    ctx->pending.resize(pend0 + 5 * (size_t)h.a_iter);
   collect_timing(ctx);
   return 0;
 }
@@ -744,7 +737,7 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   LAUNCH(launch_source(g, s, us, facew, nullptr, ctx->rhsp, ctx->slot[NSX_F_PFV], ctx->part));
   if (ev) {
     (void)hipEventRecord(ev[1], s);
-    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 3});
+    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 5});
   }
   if ((rc = solve(ctx, ctx->rhsp, X[3], Y[3]))) return rc;
   if (ctx->timing) {
@@ -761,7 +754,7 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   LAUNCH(launch_newu(g, s, Y[3], ctx->slot[NSX_F_PFV], facew));
   if (ev) {
     (void)hipEventRecord(ev[1], s);
-    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 3});
+    ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 5});
     collect_timing(ctx);
   }
   return 0;
@@ -1024,10 +1017,10 @@ extern "C" int nsx_timing_enable(nsx_ctx *ctx, int on) {
 extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches) {
   if (!ctx || !name) return -1;
   const Timer *t = nullptr;
-  if (!strcmp(name, "bicgstab_a")) t = &ctx->t_a;
-  else if (!strcmp(name, "bicgstab_b")) t = &ctx->t_b;
-  else if (!strcmp(name, "bicgstab_c")) t = &ctx->t_c;
-  else if (!strcmp(name, "stage")) t = &ctx->t_stage;
+  static const char *names[5] = {"bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c"};
+  for (int q = 0; q < 5; ++q)
+    if (!strcmp(name, names[q])) t = &ctx->t_k[q];
+  if (!strcmp(name, "stage")) t = &ctx->t_stage;
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -1036,6 +1029,7 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
-  ctx->t_a = ctx->t_b = ctx->t_c = ctx->t_stage = Timer{};
+  for (int q = 0; q < 5; ++q) ctx->t_k[q] = Timer{};
+  ctx->t_stage = Timer{};
   return 0;
 }

@@ -30,14 +30,6 @@ struct SolverState {
   int max_iters, pad;                                         //   (init)
 };
 
-// Compact PPE operator: AmC_j(i,k) = ppec(i,j)*dsigv(k-1), AmCT(i,k) = W(i,k),
-// AmCB(i,k) = W(i,k-1), AmC0 = -(AmCT+AmCB+AmC1+AmC2+AmC3): the coefficients of
-// PDE_center regenerated on the fly, bit for bit; invD = 1/AmC0.
-struct PPEOp {
-  const double *W;     // [N0][NZ]
-  const double *invD;  // [N0][NZ]
-};
-
 int nsx_grid_for(uint32_t n);
 
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
@@ -58,13 +50,13 @@ int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double 
 int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, double *const face[5]);
 int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]);
 int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 1 u, 2 v, 3 w*/);
-int launch_ppe_build(const Geo &g, hipStream_t s, double *W, double *invD, double *A0_check);
+int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0);
 
 // ---- solver
-int launch_bicg_init(const Geo &g, hipStream_t s, PPEOp op, const double *rhs, const double *phi_in, double *x,
-                     double *r, double *rhat, Partials part, SolverState *st, double eps, int max_iters);
-int launch_bicg_iter(const Geo &g, hipStream_t s, PPEOp op, int parity, double *x, double *r, const double *rhat,
-                     double *p0, double *v0, double *p1, double *v1, double *sv, double *tv, Partials part,
-                     SolverState *st, hipEvent_t *ev /* nullable: 6 events, start/end of phases A, B, C */);
+int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
+                     double *rhat, Partials part, SolverState *st, double eps, int max_iters);
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
+                     double *y, double *z, double *sv, double *tv, Partials part, SolverState *st,
+                     hipEvent_t *ev /* nullable: 6 events bracketing A1, A2, B1, B2, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

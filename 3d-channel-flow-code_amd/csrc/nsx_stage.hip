@@ -573,39 +573,3 @@ int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind) {
   hipLaunchKernelGGL(k_ghost_layers, dim3((g.N0 + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, f, kind);
   return (int)hipGetLastError();
 }
-
-// ----------------------------------------------------- compact PPE matrix --
-// PDE_center (PDE.F90:69-121) regenerated: W(i,k) = area/dsig(k) between
-// layers k and k+1 (0 at the walls), diagonal AA0 = -(AAT+AAB+AA1+AA2+AA3)
-// and its reciprocal; A0_check (nullable) receives AA0 for the set-up check.
-__global__ __launch_bounds__(NSX_BLOCK) void k_ppe_build(Geo g, double *W, double *invD, double *A0) {
-  const size_t n = (size_t)g.N0 * g.NZ;
-  for (size_t t = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * NSX_BLOCK) {
-    const int i = (int)(t / g.NZ), k = (int)(t % g.NZ);
-    double w = 0.0;
-    if (k >= 1 && k <= g.NZ - 3) w = g.area[i] / g.dsig[k];
-    if (k == g.NZ - 2 && g.bctop != 1) w = g.area[i] / g.dsig[k];
-    if (k == 0 && g.bcbot != 1) w = g.area[i] / g.dsig[0];
-    W[t] = w;
-  }
-}
-
-__global__ __launch_bounds__(NSX_BLOCK) void k_ppe_diag(Geo g, const double *W, double *invD, double *A0) {
-  FOR_EACH_CELL(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
-    const double sv = g.dsigv[k - 1];
-    const double AAT = W[s], AAB = W[s - 1];
-    const double a1 = g.ppec[i * 3] * sv, a2 = g.ppec[i * 3 + 1] * sv, a3 = g.ppec[i * 3 + 2] * sv;
-    const double d = -1.0 * (AAT + AAB + a1 + a2 + a3);
-    invD[s] = 1.0 / d;
-    if (A0) A0[s] = d;
-  }
-}
-
-int launch_ppe_build(const Geo &g, hipStream_t s, double *W, double *invD, double *A0_check) {
-  hipLaunchKernelGGL(k_ppe_build, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, W, invD, A0_check);
-  hipLaunchKernelGGL(k_ppe_diag, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, W, invD, A0_check);
-  return (int)hipGetLastError();
-}

@@ -163,7 +163,9 @@ int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
 
 /* ---- timing ---------------------------------------------------------------- */
 /* per-kernel device time accumulated with HIP events since the last reset;
- * names: "bicgstab_a", "bicgstab_b", "bicgstab_c", "stage" (all non-solver) */
+ * names: "bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c" (the five
+ * BiCGSTAB kernels, iterations that ran only) and "stage" (all non-solver
+ * kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int nsx_timing_reset(nsx_ctx *ctx);
