@@ -25,7 +25,7 @@ struct Timer {
 }  // namespace
 
 struct nsx_ctx {
-  int N0 = 0, NC = 0, NV = 0, NZ = 0, NEX = 0;
+  int N0 = 0, NC = 0, NV = 0, NZ = 0, NS = 0, NEX = 0;
   int dev = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -58,12 +58,21 @@ struct nsx_ctx {
   Partials part{};
   nsx_solve_info info{};
   int iter_hint = 0;
+  // mixed precision: FP32 inner BiCGSTAB vectors and settings
+  float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *fy = nullptr, *fz = nullptr, *fs = nullptr,
+        *ft = nullptr, *fe = nullptr;
+  int precision = NSX_PREC_FP64;
+  int refine = NSX_REFINE_CASCADE;
+  double delta = 0.1;
+  double eps_inner = 1e-5;
+  int max_passes = 30;
+  int iter_hint32 = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  Timer t_k[5], t_stage;  // A1, A2, B1, B2, C; non-solver stage kernels
-  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..4 solver kernels, 5 stage)
+  Timer t_k[5], t_stage, t_k32[5], t_u;  // FP64 A1, A2, B1, B2, C; non-solver stage kernels; FP32 A1..C; U
+  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..4 FP64 solver, 5 stage, 6..10 FP32, 11 U)
 };
 
 static int nsx_fail(nsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -112,7 +121,7 @@ static int dev_upload(nsx_ctx *ctx, T **p, const std::vector<T> &h) {
 
 static size_t field_elems(const nsx_ctx *c, int kind) {
   if (kind == FK_VERT) return (size_t)c->NV * (c->NZ - 1);
-  return (size_t)c->N0 * c->NZ;
+  return (size_t)c->N0 * c->NS;
 }
 
 // ------------------------------------------------------------- lifecycle --
@@ -125,6 +134,7 @@ extern "C" int nsx_create(int n_cell0, int n_cell, int n_vert, int nz, nsx_ctx *
   ctx->NC = n_cell;
   ctx->NV = n_vert;
   ctx->NZ = nz;
+  ctx->NS = (nz + 3) / 4 * 4;
   hipError_t e = hipGetDevice(&ctx->dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -287,18 +297,38 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   if ((ctx->have & 255) != 255)
     return nsx_fail(ctx, -1, "set-up incomplete (mask %u): mesh, geometry, sigma, vertex weights, bc, ppe matrix, les, params",
                     ctx->have);
-  const int N0 = ctx->N0, NC = ctx->NC, NV = ctx->NV, NZ = ctx->NZ;
+  const int N0 = ctx->N0, NC = ctx->NC, NV = ctx->NV, NZ = ctx->NZ, NS = ctx->NS;
   Geo &g = ctx->g;
   g.N0 = N0;
   g.NV = NV;
   g.NZ = NZ;
+  g.NS = NS;
   g.NZi = NZ - 2;
   g.NL = NZ - 1;
   g.ncell = (uint32_t)N0 * (uint32_t)(NZ - 2);
-  if ((uint64_t)N0 * NZ >= (1ull << 31) || (uint64_t)NV * (NZ - 1) >= (1ull << 31))
+  if ((uint64_t)N0 * NS >= (1ull << 31) || (uint64_t)NV * (NZ - 1) >= (1ull << 31))
     return nsx_fail(ctx, -1, "mesh too large for 32-bit cell numbering");
   g.divNZi = make_fastdiv((uint32_t)g.NZi);
   g.divNL = make_fastdiv((uint32_t)g.NL);
+  g.NQ = NS / 4;
+  g.nquad = (uint32_t)N0 * (uint32_t)g.NQ;
+  g.divNQ = make_fastdiv((uint32_t)g.NQ);
+  {
+    // one grid for the solver's partial-sum chain: every block resident at
+    // once (CUs x the lowest occupancy of those kernels), at most MAXGRID
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, ctx->dev));
+    const int per_cu = solver_blocks_per_cu(g);
+    if (per_cu < 1) return nsx_fail(ctx, -2, "occupancy query of the solver kernels failed");
+    long G = (long)prop.multiProcessorCount * per_cu;
+    const long need = ((long)g.nquad + NSX_BLOCK - 1) / NSX_BLOCK;
+    if (G > need) G = need;
+    if (G > MAXGRID) G = MAXGRID;
+    G = G / NSX_NXCD * NSX_NXCD;
+    if (G < NSX_NXCD) G = NSX_NXCD;
+    g.grid = (int)G;
+  }
+  if (NS > 4096) return nsx_fail(ctx, -1, "NZ = %d exceeds the 4094 layers the solver's LDS level table holds", NZ);
   g.bctop = ctx->bctop;
   g.bcbot = ctx->bcbot;
   g.TopBC = ctx->TopBC;
@@ -446,6 +476,23 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   g.sigv = a_sigv;
   g.dsigv = a_dsigv;
   g.walld = a_walld;
+  {
+    // FP32 copies for the inner solver of the mixed-precision form
+    auto f32 = [](const std::vector<double> &v, size_t n) {
+      std::vector<float> o(n);
+      for (size_t q = 0; q < n; ++q) o[q] = (float)v[q];
+      return o;
+    };
+    float *fa, *fpc, *fds, *fdv;
+    UP(fa, f32(ctx->areaCell, (size_t)N0));
+    UP(fpc, f32(ppec, ppec.size()));
+    UP(fds, f32(ctx->dsig, (size_t)NZ));
+    UP(fdv, f32(ctx->dsigv, (size_t)NZ - 1));
+    g.area_f = fa;
+    g.ppec_f = fpc;
+    g.dsig_f = fds;
+    g.dsigv_f = fdv;
+  }
   g.vptr = d_vptr;
   g.vcell = d_vcell;
   g.vw = a_vw;
@@ -462,7 +509,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
 #undef UP
 
   // fields
-  const size_t nc = (size_t)N0 * NZ, nv = (size_t)NV * (NZ - 1);
+  const size_t nc = (size_t)N0 * NS, nv = (size_t)NV * (NZ - 1);
   auto cell = [&](double **p) { return dev_alloc(ctx, p, nc); };
   for (int s = 0; s < 3; ++s)
     for (int q = 0; q < 4; ++q)
@@ -518,17 +565,28 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
                   ctx->sp[1], ctx->svv[0], ctx->svv[1], ctx->ss, ctx->st_};
   for (double *p : zs) HIPCHECK(hipMemsetAsync(p, 0, nc * sizeof(double), ctx->stream));
   for (int q = 0; q < 3; ++q) HIPCHECK(hipMemsetAsync(ctx->uv2[q], 0, nv * sizeof(double), ctx->stream));
+  float **fvec[9] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->fy, &ctx->fz, &ctx->fs, &ctx->ft, &ctx->fe};
+  for (float **f : fvec) {
+    if ((rc = dev_alloc(ctx, f, nc))) return rc;
+    HIPCHECK(hipMemsetAsync(*f, 0, nc * sizeof(float), ctx->stream));
+  }
 
   // solver state and reduction scratch
   if ((rc = dev_alloc(ctx, &ctx->st_dev, 1))) return rc;
   HIPCHECK(hipMemsetAsync(ctx->st_dev, 0, sizeof(SolverState), ctx->stream));
   HIPCHECK(hipHostMalloc((void **)&ctx->st_host, sizeof(SolverState), hipHostMallocDefault));
-  double **rows[5] = {&ctx->part.src, &ctx->part.init, &ctx->part.a, &ctx->part.b, &ctx->part.c};
-  const int nrow[5] = {1, 5, 1, 2, 4};
-  for (int q = 0; q < 5; ++q) {
+  // one buffer of [rows][MAXGRID] per reduction site; every member of
+  // Partials must be listed here (checked below)
+  double **rows[7] = {&ctx->part.src, &ctx->part.init, &ctx->part.a,    &ctx->part.b,
+                      &ctx->part.c,   &ctx->part.outer, &ctx->part.u};
+  const int nrow[7] = {1, 5, 1, 2, 4, 5, 4};
+  static_assert(sizeof(Partials) == 7 * sizeof(double *), "a Partials member has no buffer");
+  for (int q = 0; q < 7; ++q) {
     if ((rc = dev_alloc(ctx, rows[q], (size_t)nrow[q] * MAXGRID))) return rc;
     HIPCHECK(hipMemsetAsync(*rows[q], 0, (size_t)nrow[q] * MAXGRID * sizeof(double), ctx->stream));
   }
+  for (int q = 0; q < 7; ++q)
+    if (!*rows[q]) return nsx_fail(ctx, -1, "reduction buffer %d is not allocated", q);
 
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
@@ -539,7 +597,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i)
-      if (!(a0[(size_t)i * NZ + k] == ctx->AmC[0][(size_t)i + (size_t)k * N0]))
+      if (!(a0[(size_t)i * NS + k] == ctx->AmC[0][(size_t)i + (size_t)k * N0]))
         return nsx_fail(ctx, -1, "device PPE diagonal differs from AmC0 at (%d,%d)", i + 1, k + 1);
   ctx->committed = true;
   // the host copies are no longer needed except the ghost map of nsx_get_field
@@ -556,15 +614,15 @@ static int check_field(nsx_ctx *ctx, int f) {
 extern "C" int nsx_put_field(nsx_ctx *ctx, int f, const double *host) {
   NEED_COMMIT(ctx);
   if (check_field(ctx, f)) return -1;
-  const int N0 = ctx->N0, NZ = ctx->NZ, NV = ctx->NV;
-  std::vector<double> buf(field_elems(ctx, ctx->kind[f]));
+  const int N0 = ctx->N0, NZ = ctx->NZ, NS = ctx->NS, NV = ctx->NV;
+  std::vector<double> buf(field_elems(ctx, ctx->kind[f]), 0.0);
   if (ctx->kind[f] == FK_VERT) {
     for (int l = 0; l < NZ - 1; ++l)
       for (int v = 0; v < NV; ++v) buf[(size_t)v * (NZ - 1) + l] = host[(size_t)v + (size_t)l * NV];
   } else {
     const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)ctx->NC;
     for (int k = 0; k < NZ; ++k)
-      for (int i = 0; i < N0; ++i) buf[(size_t)i * NZ + k] = host[(size_t)i + (size_t)k * ld];
+      for (int i = 0; i < N0; ++i) buf[(size_t)i * NS + k] = host[(size_t)i + (size_t)k * ld];
   }
   HIPCHECK(hipMemcpyAsync(ctx->slot[f], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -574,7 +632,7 @@ extern "C" int nsx_put_field(nsx_ctx *ctx, int f, const double *host) {
 extern "C" int nsx_get_field(nsx_ctx *ctx, int f, double *host, int fill_ghost_rows) {
   NEED_COMMIT(ctx);
   if (check_field(ctx, f)) return -1;
-  const int N0 = ctx->N0, NZ = ctx->NZ, NV = ctx->NV, NC = ctx->NC;
+  const int N0 = ctx->N0, NZ = ctx->NZ, NS = ctx->NS, NV = ctx->NV, NC = ctx->NC;
   std::vector<double> buf(field_elems(ctx, ctx->kind[f]));
   HIPCHECK(hipMemcpyAsync(buf.data(), ctx->slot[f], buf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -585,12 +643,12 @@ extern "C" int nsx_get_field(nsx_ctx *ctx, int f, double *host, int fill_ghost_r
   }
   const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)NC;
   for (int k = 0; k < NZ; ++k)
-    for (int i = 0; i < N0; ++i) host[(size_t)i + (size_t)k * ld] = buf[(size_t)i * NZ + k];
+    for (int i = 0; i < N0; ++i) host[(size_t)i + (size_t)k * ld] = buf[(size_t)i * NS + k];
   if (fill_ghost_rows && ctx->kind[f] == FK_CELL)
     for (int gh = N0; gh < ctx->NEX; ++gh) {
       const int partner = ctx->No_cp[(size_t)gh + 2 * (size_t)NC] - 1;
       if (partner < 0 || partner >= N0) continue;
-      for (int k = 0; k < NZ; ++k) host[(size_t)gh + (size_t)k * NC] = buf[(size_t)partner * NZ + k];
+      for (int k = 0; k < NZ; ++k) host[(size_t)gh + (size_t)k * NC] = buf[(size_t)partner * NS + k];
     }
   return 0;
 }
@@ -620,7 +678,8 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[6] = {&ctx->t_k[0], &ctx->t_k[1], &ctx->t_k[2], &ctx->t_k[3], &ctx->t_k[4], &ctx->t_stage};
+  Timer *t[12] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   &ctx->t_k[3],   &ctx->t_k[4],  &ctx->t_stage,
+                  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], &ctx->t_k32[3], &ctx->t_k32[4], &ctx->t_u};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -635,7 +694,7 @@ static void collect_timing(nsx_ctx *ctx) {
 // ---------------------------------------------------------- the solver ----
 // bicgstab.F90:8-168: Glob_Norm(rhs,1) [sum done by the caller into
 // st->avec_rhs], BICGSTAB_SOLVE, Glob_Norm(phi,0) and the ghost layers of p.
-static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
   LAUNCH(launch_bicg_init(g, s, rhs, phi_in, x, ctx->sr, ctx->srhat, ctx->part, ctx->st_dev, ctx->eps,
@@ -675,6 +734,8 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
   ctx->info.e2norm = h.e2;
   ctx->info.n_solves += 1;
   ctx->info.iters_total += h.a_iter;
+  ctx->info.outer_passes = 0;
+  ctx->info.iters_fp32 = 0;
   ctx->iter_hint = h.a_iter > 0 ? h.a_iter : ctx->iter_hint;
   // time only the iterations that ran, not the queued launches that found
   // the solver finished
@@ -682,6 +743,163 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
     ctx->pending.resize(pend0 + 5 * (size_t)h.a_iter);
   collect_timing(ctx);
   return 0;
+}
+
+// The mixed-precision form of the same solve: FP64 iterative refinement
+// (defect correction) around the FP32 BiCGSTAB.  Each pass forms the true
+// residual r = b - A x of the FP64 solution in FP64, tests it with the
+// reference's rule (min(|r|_1/|b|_1, |r|_2/|b|_2) < eps), solves A e = r in
+// FP32 to the relative tolerance eps_inner from e = 0, and adds e to x in
+// FP64.  Glob_Norm and the ghost layers follow as in the FP64 solve.
+static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x);
+
+// The FP64 solve (the reference's BiCGSTAB) from the x an FP32 phase has
+// reached; the solve record counts both phases.
+static int finish_fp64(nsx_ctx *ctx, const double *rhs, double *x, int its32, int passes) {
+  const int before = ctx->info.iters_total;
+  int rc = solve_fp64(ctx, rhs, x, x);
+  if (rc) return rc;
+  ctx->info.iters_fp32 = its32;
+  ctx->info.iters += its32;
+  ctx->info.iters_total = before + ctx->info.iters;
+  ctx->info.outer_passes = passes;
+  return 0;
+}
+
+// cascade = 1: a single FP32 pass, then the FP64 solve (the reference's
+// BiCGSTAB) from its result
+static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x, int cascade) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  LAUNCH(launch_mp_begin(g, s, ctx->st_dev));
+  int total = 0, passes = 0;
+  for (int pass = 0;; ++pass) {
+    if (cascade && passes == 1) return finish_fp64(ctx, rhs, x, total, passes);
+    LAUNCH(launch_mp_resid(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                           ctx->eps_inner, ctx->MaxIters, ctx->max_passes, pass == 0));
+    // the outer test decides whether this pass runs an inner solve at all
+    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (ctx->st_host->mp_done) break;
+    const int hint = ctx->iter_hint32;
+    int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
+    int issued = 0;
+    const size_t pstart = ctx->pending.size();
+    for (;;) {
+      for (int c = 0; c < chunk; ++c) {
+        hipEvent_t *ev = nullptr;
+        if (ctx->timing) {
+          ev = events(ctx, 6);
+          if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+        }
+        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
+                                  ctx->ft, ctx->part, ctx->st_dev, ev));
+        if (ev) {
+          const size_t base = (size_t)(ev - &ctx->evpool[0]);
+          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, 6 + q});
+        }
+        ++issued;
+      }
+      HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      if (ctx->st_host->done) break;
+      if (issued > ctx->MaxIters + 4) return nsx_fail(ctx, -4, "inner solver did not stop after %d iterations", issued);
+      chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
+    }
+    const SolverState &h = *ctx->st_host;
+    if (ctx->timing && ctx->pending.size() > pstart + 5 * (size_t)h.a_iter)
+      ctx->pending.resize(pstart + 5 * (size_t)h.a_iter);
+    total += h.a_iter;
+    ++passes;
+    if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
+    if (h.exit_code == NSX_SOLVE_BREAKDOWN || h.exit_code == NSX_SOLVE_MAX_ITERS) {
+      // the FP32 pass failed: its correction is dropped and the FP64 solve
+      // finishes from the current x
+      collect_timing(ctx);
+      return finish_fp64(ctx, rhs, x, total, passes);
+    }
+    LAUNCH(launch_mp_update(g, s, x, ctx->fe));
+  }
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  const SolverState &h = *ctx->st_host;
+  ctx->info.iters = total;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.oe1;
+  ctx->info.e2norm = h.oe2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += total;
+  ctx->info.outer_passes = passes;
+  ctx->info.iters_fp32 = total;
+  collect_timing(ctx);
+  return 0;
+}
+
+// The mixed-precision form with reliable updates: one FP32 BiCGSTAB whose
+// recursive residual is replaced by the true FP64 residual of the FP64
+// solution plus the FP32 correction whenever it has fallen by delta (and at
+// the end), so the Krylov space is kept across refinements (k_rel_u).
+static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  LAUNCH(launch_rel_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                         ctx->delta, ctx->MaxIters));
+  int issued = 0;
+  const size_t pend0 = ctx->pending.size();
+  const int hint = ctx->iter_hint32;
+  int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      hipEvent_t *ev = nullptr;
+      if (ctx->timing) {
+        ev = events(ctx, 7);
+        if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+      }
+      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
+                             ctx->ft, ctx->part, ctx->st_dev, ev));
+      if (ev) {
+        const size_t base = (size_t)(ev - &ctx->evpool[0]);
+        ctx->pending.push_back({base, 11});
+        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 1 + q, 6 + q});
+      }
+      ++issued;
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (ctx->st_host->done) break;
+    if (issued > ctx->MaxIters + 4) return nsx_fail(ctx, -4, "solver did not stop after %d iterations", issued);
+    chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
+  }
+  const SolverState &h = *ctx->st_host;
+  if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
+    // the FP32 iteration failed: the correction since the last update is
+    // dropped and the FP64 solve finishes from x
+    if (ctx->timing && ctx->pending.size() > pend0 + 6 * (size_t)h.a_iter)
+      ctx->pending.resize(pend0 + 6 * (size_t)h.a_iter);
+    collect_timing(ctx);
+    const int its32 = h.a_iter, ups = h.mp_pass;
+    if (its32 > 0) ctx->iter_hint32 = its32;
+    return finish_fp64(ctx, rhs, x, its32, ups);
+  }
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  ctx->info.iters = h.a_iter;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.e1;
+  ctx->info.e2norm = h.e2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += h.a_iter;
+  ctx->info.outer_passes = h.mp_pass;
+  ctx->info.iters_fp32 = h.a_iter;
+  if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
+  if (ctx->timing && ctx->pending.size() > pend0 + 6 * (size_t)h.a_iter)
+    ctx->pending.resize(pend0 + 6 * (size_t)h.a_iter);
+  collect_timing(ctx);
+  return 0;
+}
+
+static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
+  return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
 }
 
 // vertex values of up to three fields; kinds 0 p, 1 u, 2 v, 3 w
@@ -791,7 +1009,7 @@ extern "C" int nsx_init_state(nsx_ctx *ctx) {
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   LAUNCH(launch_source(g, s, uvw, facew, nullptr, nullptr, nullptr, ctx->part));
-  const size_t nc = (size_t)ctx->N0 * ctx->NZ;
+  const size_t nc = (size_t)ctx->N0 * ctx->NS;
   for (int id : {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY})
     HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, nc * sizeof(double), s));
   // LoopTimeInitial: (.)np <- (.)n
@@ -823,9 +1041,17 @@ extern "C" int nsx_hydro(nsx_ctx *ctx, int flag_ab, nsx_solve_info *info) {
 
 extern "C" int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info *info) {
   NEED_COMMIT(ctx);
-  if (precision != NSX_PREC_FP64) return nsx_fail(ctx, -1, "precision %d is not available in this build", precision);
+  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
+    return nsx_fail(ctx, -1, "unknown precision %d", precision);
   hipStream_t s = ctx->stream;
   int worst = 0;
+  const int saved = ctx->precision;
+  ctx->precision = precision;
+  struct Restore {
+    nsx_ctx *c;
+    int p;
+    ~Restore() { c->precision = p; }
+  } restore{ctx, saved};
   for (int n = 0; n < nsteps; ++n) {
     const int N = ctx->set_n, A = (N + 1) % 3, B = (N + 2) % 3;
     int rc;
@@ -967,7 +1193,7 @@ extern "C" int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w) {
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
-  HIPCHECK(hipMemsetAsync(ctx->slot[src], 0, (size_t)ctx->N0 * ctx->NZ * sizeof(double), ctx->stream));
+  HIPCHECK(hipMemsetAsync(ctx->slot[src], 0, (size_t)ctx->N0 * ctx->NS * sizeof(double), ctx->stream));
   LAUNCH(launch_source(ctx->g, ctx->stream, f, facew, ctx->slot[src], nullptr, nullptr, ctx->part));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   return 0;
@@ -1008,6 +1234,24 @@ extern "C" int nsx_newu3d(nsx_ctx *ctx, int p, int pv) {
 }
 
 // ---------------------------------------------------------------- timing --
+extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes) {
+  if (!ctx) return -1;
+  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
+    return nsx_fail(ctx, -1, "unknown precision %d", precision);
+  if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE)
+    return nsx_fail(ctx, -1, "unknown refinement %d", refine);
+  if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
+  if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
+  ctx->precision = precision;
+  ctx->refine = refine;
+  if (refine == NSX_REFINE_RELIABLE)
+    ctx->delta = tol;
+  else
+    ctx->eps_inner = tol;
+  ctx->max_passes = max_passes;
+  return 0;
+}
+
 extern "C" int nsx_timing_enable(nsx_ctx *ctx, int on) {
   if (!ctx) return -1;
   ctx->timing = on != 0;
@@ -1018,9 +1262,13 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
   if (!ctx || !name) return -1;
   const Timer *t = nullptr;
   static const char *names[5] = {"bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c"};
-  for (int q = 0; q < 5; ++q)
+  static const char *names32[5] = {"bicg32_a1", "bicg32_a2", "bicg32_b1", "bicg32_b2", "bicg32_c"};
+  for (int q = 0; q < 5; ++q) {
     if (!strcmp(name, names[q])) t = &ctx->t_k[q];
+    if (!strcmp(name, names32[q])) t = &ctx->t_k32[q];
+  }
   if (!strcmp(name, "stage")) t = &ctx->t_stage;
+  if (!strcmp(name, "rel_u")) t = &ctx->t_u;
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -1029,7 +1277,7 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
-  for (int q = 0; q < 5; ++q) ctx->t_k[q] = Timer{};
-  ctx->t_stage = Timer{};
+  for (int q = 0; q < 5; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
+  ctx->t_stage = ctx->t_u = Timer{};
   return 0;
 }

@@ -1,13 +1,15 @@
 // nsx_dev.h -- device-side data layout and helpers of the nsx HIP library.
 //
 // Layout in HBM (gfx950):
-//   cell field   double[N0][NZ]   column-contiguous: entry (i,k) at i*NZ+k; the
+//   cell field   double[N0][NS]   column-contiguous: entry (i,k) at i*NS+k, NS =
+//                                 NZ rounded up to a multiple of 4 (16-byte
+//                                 aligned runs of 2 doubles / 4 floats); the
 //                                 ghost layers k=0 and k=NZ-1 are stored, the
 //                                 horizontal (periodic) ghost cells are not: a
 //                                 neighbour index nbr[i*3+j] already points at
 //                                 the periodic partner column;
 //   vertex field double[NV][NZ-1] entry (v,l) at v*(NZ-1)+l;
-//   face field   double[N0][NZ]   U1FACE..UBFACE, same indexing as cells.
+//   face field   double[N0][NS]   U1FACE..UBFACE, same indexing as cells.
 // A wavefront walks consecutive k of one column (then the next column), so
 // every access to the cell itself, to its vertical neighbours and to a
 // horizontal neighbour column is a contiguous run.
@@ -51,8 +53,12 @@ enum : signed char { VK_INTERIOR = 0, VK_PAIR = 1, VK_CORNER = 2, VK_OTHER = 3 }
 
 struct Geo {
   int N0, NV, NZ, NZi, NL;  // NZi = NZ-2 interior layers, NL = NZ-1 vertex levels
+  int NS;                   // column stride of cell/face fields: NZ rounded up to a multiple of 4
   uint32_t ncell;           // N0*NZi interior prisms
-  FastDiv divNZi, divNL;
+  int NQ;                   // NS/4 four-layer groups per column
+  int grid;                 // grid of the kernels that produce or consume solver partial sums
+  uint32_t nquad;           // N0*NQ
+  FastDiv divNZi, divNL, divNQ;
   // per cell-edge (N0*3, edge-fastest: index i*3+j)
   const int *nbr;           // neighbour column across edge j (periodic partner resolved), 0-based
   const int *vtx;           // No_vp(i,j), 0-based
@@ -63,6 +69,8 @@ struct Geo {
   // per level
   const double *sig, *dsig;  // NZ
   const double *sigv, *dsigv;  // NZ-1
+  // FP32 copies for the FP32 inner solver of the mixed-precision form
+  const float *area_f, *ppec_f, *dsig_f, *dsigv_f;
   const double *walld;       // NZ
   // vertices: inverse-distance stencil in CSR form (interior cells, reference order)
   const int *vptr, *vcell;

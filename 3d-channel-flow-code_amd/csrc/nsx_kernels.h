@@ -10,6 +10,8 @@ struct Partials {
   double *a;     // <rhat, v>, 1 row
   double *b;     // <t,s>, <t,t>, 2 rows
   double *c;     // |r|^2, |r|_1, <rhat,r>, sum x Vol, 4 rows
+  double *outer; // refinement residual norms of the mixed solve, 5 rows
+  double *u;     // true residual after a reliable update: |r|^2, |r|_1, <rhat,r>, sum x Vol, 4 rows
 };
 
 // BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals).  Each
@@ -28,9 +30,21 @@ struct SolverState {
   int a_iter;        // k of the last phase A                      (phase A)
   int done, exit_code;                                        //   (phase A or B)
   int max_iters, pad;                                         //   (init)
+  // FP64 iterative refinement around the FP32 solve (k_mp_check)
+  double ob1, ob2;   // |b|_1, |b|_2 of the mean-removed rhs
+  double oe1, oe2;   // relative norms of the true residual b - A x
+  int mp_pass;       // refinement residuals formed so far
+  int mp_done;       // outer stop
+  // reliable updates (mixed precision, Krylov space kept across refinements)
+  double delta;      // update when |r| < delta x the largest |r| since the last one  (init)
+  double r0N;        // |r| of the last true residual                  (phase A, update)
+  double maxrr[2];   // largest |r| since the last update, by parity   (phase U; reset in A)
+  int upd;           // this iteration starts from a true residual     (phase U)
+  int pad2;
 };
 
 int nsx_grid_for(uint32_t n);
+int solver_blocks_per_cu(const Geo &g);
 
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
@@ -58,5 +72,20 @@ int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const doubl
 int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
                      double *y, double *z, double *sv, double *tv, Partials part, SolverState *st,
                      hipEvent_t *ev /* nullable: 6 events bracketing A1, A2, B1, B2, C */);
+int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *r, const float *rhat, float *p, float *v,
+                       float *y, float *z, float *sv, float *tv, Partials part, SolverState *st, hipEvent_t *ev);
+// mixed precision: reset, refinement residual (+ outer test, inner reset), correction
+int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st);
+int launch_mp_resid(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
+                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double eps_inner,
+                    int max_iters, int max_passes, int first_pass);
+int launch_mp_update(const Geo &g, hipStream_t s, double *x, const float *e32);
+// reliable updates: initial true residual, then iterations U, A1..C
+int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
+                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double delta,
+                    int max_iters);
+int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, const float *rhat,
+                    float *p, float *v, float *y, float *z, float *sv, float *tv, Partials part, SolverState *st,
+                    hipEvent_t *ev /* nullable: 7 events bracketing U, A1, A2, B1, B2, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

@@ -39,7 +39,7 @@ struct VtxArgs {
 __device__ inline double idw(const Geo &g, int v, int layer, const double *f) {
   double s = 0.0;
   const int b = g.vptr[v], e = g.vptr[v + 1];
-  for (int j = b; j < e; ++j) s = s + g.vw[j] * f[(size_t)g.vcell[j] * g.NZ + layer];
+  for (int j = b; j < e; ++j) s = s + g.vw[j] * f[(size_t)g.vcell[j] * g.NS + layer];
   return s / g.dlVsum[v];
 }
 
@@ -123,7 +123,7 @@ __device__ inline void faces_of(const Geo &g, int i, int k, size_t s, const doub
     const int nb = g.nbr[e];
     const int nv1 = g.vtx[e], nv2 = g.vtx[i * 3 + (j + 1) % 3];
     const double extra = g.doe[e] * (fv[(size_t)nv2 * g.NL + k] - fv[(size_t)nv1 * g.NL + k]) / g.dlVV[e];
-    out[j] = f[(size_t)nb * g.NZ + k] * 0.5 + fi * 0.5 + extra;
+    out[j] = f[(size_t)nb * g.NS + k] * 0.5 + fi * 0.5 + extra;
   }
   out[3] = 0.5 * (fi + f[s + 1]);
   out[4] = 0.5 * (fi + f[s - 1]);
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_calcul_face(Geo g, const double *
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     double fc[5];
     faces_of(g, i, k, s, f, fv, fc);
     o0[s] = fc[0];
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     double U[5], V[5], W[5];
     faces_of(g, i, k, s, u, uv, U);
     faces_of(g, i, k, s, v, vv, V);
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     const int e = i * 3;
     const double area = g.area[i];
     const double sv = g.dsigv[k - 1];
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
         const int ej = e + j;
         const int nb = g.nbr[ej];
         const int jv1 = g.vtx[ej], jv2 = g.vtx[e + (j + 1) % 3];
-        const double diff1 = (phi[(size_t)nb * g.NZ + k] - pi) / g.dlCC[ej];
+        const double diff1 = (phi[(size_t)nb * g.NS + k] - pi) / g.dlCC[ej];
         const double diff2 = (phiv[(size_t)jv2 * g.NL + k] - phiv[(size_t)jv1 * g.NL + k]) / g.dlVV[ej];
         dn[j] = diff1 * g.dle1[ej] + diff2 * g.dle2[ej];
       }
@@ -319,13 +319,13 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, co
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     const int e = i * 3;
     const double ui = u[s], vi = v[s];
     double Uf[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const size_t sn = (size_t)g.nbr[e + j] * g.NZ + k;
+      const size_t sn = (size_t)g.nbr[e + j] * g.NS + k;
       const double Ui = 0.5 * (ui + u[sn]);
       const double Vi = 0.5 * (vi + v[sn]);
       Uf[j] = Ui * g.xnn[e + j] + Vi * g.ynn[e + j];
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, co
 
 int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
                   double *rhsp_out, const double *pv, Partials part) {
-  hipLaunchKernelGGL(k_source, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, uvw[0], uvw[1], uvw[2],
+  hipLaunchKernelGGL(k_source, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, uvw[0], uvw[1], uvw[2],
                      face[0], face[1], face[2], face[3], face[4], src_out, rhsp_out, pv, part.src);
   return (int)hipGetLastError();
 }
@@ -375,7 +375,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_update_rhsp(Geo g, double *rhs, c
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     const int e = i * 3;
     const double sv = g.dsigv[k - 1];
     double dn[3];
@@ -405,7 +405,7 @@ __device__ inline void lsm_grad(const Geo &g, int i, int k, size_t s, const doub
   const int e = i * 3;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const double d = f[(size_t)g.nbr[e + j] * g.NZ + k] - fi;
+    const double d = f[(size_t)g.nbr[e + j] * g.NS + k] - fi;
     sumfx = sumfx + g.dxCC[e + j] * d;
     sumfy = sumfy + g.dyCC[e + j] * d;
   }
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_grad_lsm(Geo g, const double *f, 
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     double a, b, d;
     lsm_grad(g, i, k, s, f, a, b, d);
     dx[s] = a;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_corrector(Geo g, const double *p,
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     double dpdx, dpdy, dpds;
     lsm_grad(g, i, k, s, p, dpdx, dpdy, dpds);
     const double nu_ = us[s] - g.dt * dpdx;
@@ -503,14 +503,14 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_newu(Geo g, const double *p, cons
   FOR_EACH_CELL(g, c) {
     int i, k;
     cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NZ + k;
+    const size_t s = (size_t)i * g.NS + k;
     const int e = i * 3;
     const double pi = p[s];
     double dn[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
-      const double diff1 = (p[(size_t)g.nbr[e + j] * g.NZ + k] - pi) / g.dlCC[e + j];
+      const double diff1 = (p[(size_t)g.nbr[e + j] * g.NS + k] - pi) / g.dlCC[e + j];
       const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
       dn[j] = diff1 * g.dle1[e + j] + diff2 * g.dle2[e + j];
     }
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rk2avg(size_t n, const double *u0
 }
 
 int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]) {
-  const size_t n = (size_t)g.N0 * g.NZ;
+  const size_t n = (size_t)g.N0 * g.NS;
   hipLaunchKernelGGL(k_rk2avg, dim3(nsx_grid_for((uint32_t)(n < 0xffffffffu ? n : 0xffffffffu))), dim3(NSX_BLOCK),
                      0, s, n, un[0], un[1], un[2], unp[0], unp[1], unp[2]);
   return (int)hipGetLastError();
@@ -556,7 +556,7 @@ int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double
 __global__ __launch_bounds__(NSX_BLOCK) void k_ghost_layers(Geo g, double *f, int kind) {
   const int i = blockIdx.x * NSX_BLOCK + threadIdx.x;
   if (i >= g.N0) return;
-  const size_t b = (size_t)i * g.NZ;
+  const size_t b = (size_t)i * g.NS;
   int sb, st;
   if (kind == 0) {
     sb = g.bcbot == 1 ? 1 : 0;

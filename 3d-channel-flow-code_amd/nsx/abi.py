@@ -21,6 +21,9 @@ N_FACE_SCRATCH = 10
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+REFINE_RELIABLE = 0
+REFINE_RESTART = 1
+REFINE_CASCADE = 2
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 
@@ -31,17 +34,19 @@ EXPORTS = [
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
-    "nsx_newu3d", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_newu3d", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
 ]
 
 
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iters", c_int), ("exit_code", c_int), ("e1norm", c_dbl), ("e2norm", c_dbl),
-                ("n_solves", c_int), ("iters_total", c_int)]
+                ("n_solves", c_int), ("iters_total", c_int), ("outer_passes", c_int),
+                ("iters_fp32", c_int)]
 
     def as_dict(self):
         return dict(iters=self.iters, exit=SOLVE_EXIT.get(self.exit_code, str(self.exit_code)), e1=self.e1norm,
-                    e2=self.e2norm, n_solves=self.n_solves, iters_total=self.iters_total)
+                    e2=self.e2norm, n_solves=self.n_solves, iters_total=self.iters_total,
+                    outer_passes=self.outer_passes, iters_fp32=self.iters_fp32)
 
 
 def declare(lib: ctypes.CDLL) -> None:
@@ -76,6 +81,7 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_update_rhsp": [c_vp] + [c_int] * 3,
         "nsx_bicgstab": [c_vp, c_int, c_int, c_int, P(SolveInfo)],
         "nsx_newu3d": [c_vp, c_int, c_int],
+        "nsx_set_precision": [c_vp, c_int, c_int, c_dbl, c_int],
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
         "nsx_timing_reset": [c_vp],

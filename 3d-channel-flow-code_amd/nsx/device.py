@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .abi import CELL_SCRATCH, F, FACE_SCRATCH, PREC_FP64, VERT_SCRATCH, SolveInfo
+from .abi import CELL_SCRATCH, F, FACE_SCRATCH, PREC_FP64, PREC_MIXED, VERT_SCRATCH, SolveInfo  # noqa: F401
 
 
 class NsxError(RuntimeError):
@@ -220,6 +220,14 @@ class Context:
         return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
 
     # ------------------------------------------------------------- timing ---
+    def set_precision(self, precision: int, refine: int = 2, tol: float = 1e-5, max_passes: int = 30) -> None:
+        """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: FP32
+        BiCGSTAB inside FP64 refinement: refine 0 = reliable updates with
+        threshold tol, 1 = restarted defect correction with inner tolerance
+        tol, 2 = one FP32 pass to tol then the FP64 solve); step() takes its
+        own precision and uses this refinement."""
+        self._check(self.lib.nsx_set_precision(self.h, precision, refine, tol, max_passes))
+
     def timing(self, on: bool) -> None:
         self._check(self.lib.nsx_timing_enable(self.h, int(on)))
 

@@ -1,14 +1,27 @@
 #!/usr/bin/env python3
-"""Throughput of the per-time-step hot path on one MI355X.
+"""Throughput of the per-time-step hot path on one MI355X, FP64 and mixed precision.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 2] [--no-cpu]
 
 A step is one RK2 time step (two projection stages: Smagorinsky eddy
 viscosity, advection/diffusion fluxes, predictor, PPE source, BiCGSTAB
 pressure solve, corrector, face-velocity correction) of the channel workload
-on resident device state.  Prints one JSON line:
-  value        prism-cell-steps/s = interior prisms x K / wall time of K steps
-  roofline     the dominant kernel (BiCGSTAB phase A): algorithmic bytes per
+on resident device state.  Two legs run the same steps from the same initial
+state on one GPU:
+  fp64   the reference's arithmetic (BiCGSTAB in FP64);
+  mixed  the pressure solve with an FP32 BiCGSTAB pass to 1e-5 followed by the
+         FP64 BiCGSTAB from its result to the reference's 1e-10 (the cascade
+         refinement; --legs also takes mixed_restart and mixed_reliable, the
+         FP32-only refinements).
+Each leg: W warm-up steps, K timed steps (wall clock, no events), then one
+step with HIP events around every solver kernel for the per-kernel roofline.
+Prints one JSON line:
+  value        prism-cell-steps/s of the mixed leg = interior prisms x K / wall
+  fp64         the same for the FP64 leg; mixed_over_fp64 their ratio
+  rel_err      relative L2 error of u, v, w, p (interior) and max |cell
+               divergence| (glob_cfl's definition) of the mixed leg against the
+               FP64 leg after the same steps
+  roofline     the dominant kernel of the mixed leg: algorithmic bytes per
                launch / mean launch time (HIP events on the library's stream)
   cpu_baseline the oracle (CPU restatement of the reference, 1 core) on a
                bounded sample of the same workload, extrapolated per step
@@ -21,17 +34,19 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "3d-channel-flow-code_amd"))
 
 METRIC = "prism-cell-steps/s: mixed-prec vs FP64 GPU at ≤1e-8 rel err; % HBM peak"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per interior prism of each BiCGSTAB kernel (FP64):
-# A1 reads r,p,v, writes p,y; A2 reads y,rhat, writes v; B1 reads r,v, writes
-# s,z; B2 reads z,s, writes t; C reads x,y,z,s,t,rhat, writes x,r
-PHASE_BYTES = {"bicg_a1": 5 * 8, "bicg_a2": 3 * 8, "bicg_b1": 4 * 8, "bicg_b2": 3 * 8, "bicg_c": 8 * 8}
-KERNEL_NAMES = {"bicg_a1": "k_bicg_a1", "bicg_a2": "k_bicg_a2", "bicg_b1": "k_bicg_b1", "bicg_b2": "k_bicg_b2",
-                "bicg_c": "k_bicg_c"}
+# algorithmic HBM bytes per interior prism of each BiCGSTAB kernel, in units of
+# the vector width (8 B FP64, 4 B FP32): A1 reads r,p,v, writes p,y; A2 reads
+# y,rhat, writes v; B1 reads r,v, writes s,z; B2 reads z,s, writes t; C reads
+# x,y,z,s,t,rhat, writes x,r
+PHASE_UNITS = {"a1": 5, "a2": 3, "b1": 4, "b2": 3, "c": 8}
+KERNEL_NAMES = {"a1": "k_bicg_a1", "a2": "k_bicg_a2", "b1": "k_bicg_b1", "b2": "k_bicg_b2", "c": "k_bicg_c"}
 WORKLOAD = {
     0: "32x16 quads x 32 layers = 32,768 prisms, Re_tau 180, dt 1e-3",
     1: "128x64 quads x 64 layers = 1,048,576 prisms, Re_tau 180, dt 5e-4, jittered mesh",
@@ -44,7 +59,7 @@ WORKLOAD = {
 def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
     """One projection stage of the oracle on the full mesh with BiCGSTAB capped
     at `cap` iterations; the step time is extrapolated with the GPU's mean
-    iteration count per solve."""
+    iteration count per solve (FP64 leg)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
@@ -64,8 +79,70 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
             "sample": (f"oracle (C restatement of the reference, 1 thread) on the full mesh: one projection stage "
                        f"with BiCGSTAB capped at {its} iterations ({wall:.1f} s); step time = "
                        f"{case.params.FinRK} x (non-solver {t_rest:.2f} s + {mean_iters:.0f} iterations x "
-                       f"{t_iter * 1e3:.1f} ms) with the GPU run's mean iteration count"),
+                       f"{t_iter * 1e3:.1f} ms) with the FP64 GPU leg's mean iteration count"),
             "s_per_step": step_s}
+
+
+def max_divergence(case, faces) -> float:
+    """max over interior cells of |sum_j U_j dlVV_j dsig(k-1) + (UT + UB) area|,
+    the divergence check of glob_cfl (cfl.F90:131-167)."""
+    n0, nz = case.N_CELL0, case.NZ
+    dlVV = case.arrays["dlVV"][:n0]
+    dsig = case.dsig
+    area = case.areaCell[:n0, None]
+    U1, U2, U3, UT, UB = (np.asarray(f)[:n0, 1:nz - 1] for f in faces)
+    ds = dsig[None, 0:nz - 2]
+    flux = U1 * dlVV[:, 0:1] * ds + U2 * dlVV[:, 1:2] * ds + U3 * dlVV[:, 2:3] * ds
+    flux = flux + UT * area + UB * area
+    return float(np.abs(flux).max())
+
+
+MIXED_MODES = {"mixed": (2, 1e-5), "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
+
+
+def leg(case, name: str, steps: int, warmup: int) -> dict:
+    from nsx.device import Context
+
+    precision = 0 if name == "fp64" else 1
+    ctx = Context(case)
+    if precision:
+        ctx.set_precision(1, *MIXED_MODES[name])
+    ctx.init_state()
+    if warmup:
+        ctx.step(warmup, precision)
+    info0 = ctx.solve_info()
+    t0 = time.perf_counter()
+    info = ctx.step(steps, precision)
+    wall = time.perf_counter() - t0
+    solves = info["n_solves"] - info0["n_solves"]
+    iters = info["iters_total"] - info0["iters_total"]
+    # per-kernel times: one more step with HIP events around each solver kernel
+    ctx.timing(True)
+    ctx.timing_reset()
+    ctx.step(1, precision)
+    ctx.timing(False)
+    n = case.n_interior
+    kernels = {}
+    for width, prefix in ((8, "bicg_"), (4, "bicg32_")):
+        for ph, units in PHASE_UNITS.items():
+            ms, launches = ctx.timer(prefix + ph)
+            if launches:
+                avg = ms / launches
+                b = units * width * n
+                kernels[prefix + ph] = {"kernel": KERNEL_NAMES[ph] + ("<float>" if width == 4 else "<double>"),
+                                        "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
+                                        "GBps": b / (avg * 1e-3) / 1e9}
+    stage_ms, stage_n = ctx.timer("stage")
+    fields = {f: np.asarray(ctx.get(f))[:case.N_CELL0, 1:-1].copy() for f in ("UFN", "VFN", "WFN", "PFN")}
+    faces = [ctx.get(f) for f in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")]
+    fields["div"] = max_divergence(case, faces)
+    ctx.close()
+    return {"value": n * steps / wall, "ms_per_step": wall / steps * 1e3, "wall": wall,
+            "mean_iters_per_solve": iters / max(solves, 1), "solves": solves, "exit": info["exit"],
+            "last_iters_fp32": info.get("iters_fp32", 0),
+            "last_e1": info["e1"], "last_e2": info["e2"], "outer_passes_last": info.get("outer_passes", 0),
+            "kernels": kernels, "non_solver_ms_per_stage": stage_ms / max(stage_n, 1) * 2 if stage_n else None,
+            "fields": fields}
 
 
 def main() -> int:
@@ -76,77 +153,64 @@ def main() -> int:
     ap.add_argument("--cfg", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
+    ap.add_argument("--legs", default="fp64,mixed", help="comma list of fp64, mixed, mixed_restart, mixed_reliable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of the path is a design note, not code)", file=sys.stderr)
         return 2
 
-    from nsx.case import CONFIGS, config_case
-    from nsx.device import Context
+    from nsx.case import config_case
 
     cfg = args.cfg
     steps = args.steps if args.steps is not None else {0: 10, 1: 20, 2: 10, 3: 20, 4: 3}[cfg]
     case = config_case(cfg, seed=args.seed)
-    ctx = Context(case)
-    ctx.init_state()
-    if args.warmup:
-        ctx.step(args.warmup)
-    info0 = ctx.solve_info()
-    ctx.timing(not args.no_timing)
-    ctx.timing_reset()
-    t0 = time.perf_counter()
-    info = ctx.step(steps)
-    wall = time.perf_counter() - t0
-    ctx.timing(False)
-    n = case.n_interior
-    solves = info["n_solves"] - info0["n_solves"]
-    iters = info["iters_total"] - info0["iters_total"]
-    mean_iters = iters / max(solves, 1)
-    kernels = {}
-    for name, b in PHASE_BYTES.items():
-        ms, launches = ctx.timer(name)
-        if launches:
-            avg = ms / launches
-            kernels[name] = {"launches": launches, "avg_ms": avg, "bytes_per_launch": b * n,
-                             "GBps": b * n / (avg * 1e-3) / 1e9}
-    stage_ms, _ = ctx.timer("stage")
-    # the dominant kernel: the BiCGSTAB kernel with the largest total time
-    dom_name = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"]) if kernels else None
-    dom = kernels.get(dom_name, {"GBps": 0.0, "bytes_per_launch": 0, "avg_ms": 0.0})
-    solver_ms = sum(v["avg_ms"] * v["launches"] for v in kernels.values())
-    solver_bytes = sum(v["bytes_per_launch"] * v["launches"] for v in kernels.values())
+    legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+    res = {}
+    for name in legs:
+        res[name] = leg(case, name, steps, args.warmup)
+        print(f"# leg {name}: {res[name]['value']:.4g} prism-cell-steps/s, {res[name]['ms_per_step']:.1f} ms/step, "
+              f"{res[name]['mean_iters_per_solve']:.1f} iterations/solve", file=sys.stderr, flush=True)
+    head = res.get("mixed", res.get("fp64"))
+    kern = head["kernels"]
+    dom_name = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"]) if kern else None
+    dom = kern.get(dom_name, {"GBps": 0.0, "bytes_per_launch": 0, "avg_us": 0.0, "kernel": None})
     out = {
         "metric": METRIC,
-        "value": n * steps / wall,
+        "value": head["value"],
         "unit": "prism-cell-steps/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 (FP32 BiCGSTAB in FP64 refinement)" if "mixed" in res else "f64",
         "data": "synthetic",
-        "config": {"workload": f"configs[{cfg}]: {WORKLOAD[cfg]}", "prisms": n, "precision": "fp64",
+        "config": {"workload": f"channel {WORKLOAD[cfg]}", "prisms": case.n_interior,
                    "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": case.params.eps,
-                   "seed": args.seed},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES.get(dom_name, None), "achieved": dom["GBps"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_us": dom["avg_ms"] * 1e3, "bytes_per_launch": dom["bytes_per_launch"]},
-        "solver_GBps": solver_bytes / (solver_ms * 1e-3) / 1e9 if solver_ms else 0.0,
-        "solver": {"mean_iters_per_solve": mean_iters, "solves": solves, "last_e1": info["e1"],
-                   "last_e2": info["e2"]},
-        "kernels": kernels,
-        "non_solver_ms_per_stage": stage_ms / max(solves, 1),
+                   "inner_tolerance": 1e-5, "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": dom["avg_us"], "bytes_per_launch": dom["bytes_per_launch"]},
     }
+    for name, r in res.items():
+        out[name] = {k: v for k, v in r.items() if k != "fields"}
+    if "fp64" in res and "mixed" in res:
+        out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+        fa, fb = res["fp64"]["fields"], res["mixed"]["fields"]
+        err = {}
+        for f, nm in (("UFN", "u"), ("VFN", "v"), ("WFN", "w"), ("PFN", "p")):
+            err[nm] = float(np.linalg.norm(fb[f] - fa[f]) / max(np.linalg.norm(fa[f]), 1e-300))
+        err["max_div_fp64"] = fa["div"]
+        err["max_div_mixed"] = fb["div"]
+        out["rel_err"] = err
     if not args.no_cpu:
-        cb = cpu_baseline(case, mean_iters)
+        base = res.get("fp64", head)
+        cb = cpu_baseline(case, base["mean_iters_per_solve"])
         out["cpu_baseline"] = {k: v for k, v in cb.items() if k != "s_per_step"}
         out["gpu_over_cpu"] = out["value"] / cb["value"]
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

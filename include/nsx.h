@@ -66,12 +66,15 @@ typedef struct nsx_solve_info {
   double e2norm;   /* ||r||_2 / ||b||_2 */
   int n_solves;    /* solves since nsx_init_state */
   int iters_total; /* iterations since nsx_init_state */
+  int outer_passes; /* NSX_PREC_MIXED: refinement passes (restart) or reliable updates of the last solve */
+  int iters_fp32;   /* NSX_PREC_MIXED: FP32 iterations of the last solve (included in iters) */
 } nsx_solve_info;
 
 /* ---- precision of the step --------------------------------------------- */
 enum nsx_precision {
   NSX_PREC_FP64 = 0,  /* the reference's arithmetic */
-  NSX_PREC_MIXED = 1  /* FP32 stencil/flux/nu_t, FP64 iterative refinement of the PPE */
+  NSX_PREC_MIXED = 1  /* FP32 BiCGSTAB (SpMV, preconditioner, updates) inside an FP64
+                          iterative refinement of the PPE; inner products and norms in FP64 */
 };
 
 /* ---- lifecycle ----------------------------------------------------------- */
@@ -161,11 +164,30 @@ int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info)
 /* newu3D (newu3D.F90:8-257): U1FACE..UBFACE -= dt grad_n p */
 int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
 
+/* ---- precision form (no counterpart in the reference) ------------------------ */
+/* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
+ * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs
+ * the BiCGSTAB vectors, SpMV and preconditioner in FP32 and keeps the solution,
+ * the refinement residual, inner products and norms in FP64; the solve stops
+ * when the true FP64 residual meets the reference's rule with eps
+ * (nsx_set_params).  Refinement:
+ *   NSX_REFINE_RELIABLE  one FP32 iteration whose residual is replaced by the
+ *                        true FP64 residual whenever it has fallen by tol
+ *                        (delta, e.g. 0.1); max_passes unused;
+ *   NSX_REFINE_RESTART   defect correction: per pass an FP32 BiCGSTAB from
+ *                        e = 0 on A e = b - A x to min(e1,e2) < tol (e.g.
+ *                        1e-5), then x += e; at most max_passes passes;
+ *   NSX_REFINE_CASCADE   one such FP32 pass, then the FP64 BiCGSTAB of
+ *                        NSX_PREC_FP64 started from its result. */
+enum nsx_refine { NSX_REFINE_RELIABLE = 0, NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2 };
+int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
+
 /* ---- timing ---------------------------------------------------------------- */
 /* per-kernel device time accumulated with HIP events since the last reset;
  * names: "bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c" (the five
- * BiCGSTAB kernels, iterations that ran only) and "stage" (all non-solver
- * kernels of a projection stage) */
+ * BiCGSTAB kernels in FP64, iterations that ran only), "bicg32_a1" ..
+ * "bicg32_c" (the same kernels of the FP32 solve), "rel_u" (the reliable-update
+ * kernel) and "stage" (all non-solver kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int nsx_timing_reset(nsx_ctx *ctx);

@@ -51,3 +51,31 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def ppe_residual(case, phi, rhs):
+    """Relative norms (e1, e2) of b - A phi over the interior, with A the
+    stored PDE_center coefficients AmC0..AmCB and b the mean-removed rhs of
+    Glob_Norm(rhs,1) (bicgstab.F90:635-663).  Periodic neighbours are read
+    through the ghost rows' partner (No_cp(g,3)); the wall-neighbour
+    coefficients are zero, so the vertical ghost layers do not enter."""
+    A = case.arrays
+    n0, nz = case.N_CELL0, case.NZ
+    phi = np.asarray(phi)
+    ncp = A["No_cp"].astype(np.int64) - 1
+    col = np.arange(n0)
+    nb = []
+    for j in range(3):
+        jc = ncp[:n0, j].copy()
+        gh = jc >= n0
+        jc[gh] = ncp[jc[gh], 2]
+        nb.append(jc)
+    k = slice(1, nz - 1)
+    Ax = (A["AmC0"][:, k] * phi[col, k] + A["AmC1"][:, k] * phi[nb[0], k] + A["AmC2"][:, k] * phi[nb[1], k]
+          + A["AmC3"][:, k] * phi[nb[2], k] + A["AmCT"][:, k] * phi[col, 2:nz] + A["AmCB"][:, k] * phi[col, 0:nz - 2])
+    vol = case.areaCell[:n0, None] * case.dsig[None, 1:-1]
+    rhs_i = np.asarray(rhs)[:n0, 1:-1]
+    avec = rhs_i.sum() / (case.TotalVolume * (case.params.sigFin - case.params.sigIni))
+    b = rhs_i - avec * vol
+    r = b - Ax
+    return float(np.abs(r).sum() / np.abs(b).sum()), float(np.linalg.norm(r) / np.linalg.norm(b))
