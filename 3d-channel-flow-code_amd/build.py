@@ -2,6 +2,11 @@
 
   libnsx_host.so  host set-up (host/*.c), gcc, no FMA contraction
   libnsx.so       HIP kernels + C-ABI (csrc/*.hip), hipcc for gfx950
+  libnsx_fortran.a + nsx_iface.mod, nsx_dropin.mod
+                  the Fortran host side (fortran/*.F90, amdflang): the
+                  ISO_C_BINDING interfaces and the drop-in subroutines
+  tests/fortran/_build/dropin_driver
+                  the Fortran test driver of the drop-in, linked to libnsx.so
 
 Outputs land in nsx/_build/ so they travel with the source tree.  Run as
 `python build.py [host|hip|all]` or through __graft_entry__.build().
@@ -49,11 +54,39 @@ def build_hip():
     os.replace(tmp, os.path.join(OUT, "libnsx.so"))
 
 
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran():
+    """The Fortran host side; skipped (with a message) when no Fortran compiler exists."""
+    if not os.path.exists(FC):
+        print(f"build: {FC} not found, the Fortran drop-in is not built", flush=True)
+        return
+    fdir = os.path.join(OUT, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    objs = []
+    for src in ("nsx_iface.F90", "nsx_dropin.F90"):
+        obj = os.path.join(fdir, src.replace(".F90", ".o"))
+        _run([FC, "-O2", "-fPIC", "-J", fdir, "-c", os.path.join(HERE, "fortran", src), "-o", obj])
+        objs.append(obj)
+    lib = os.path.join(OUT, "libnsx_fortran.a")
+    if os.path.exists(lib):
+        os.remove(lib)
+    _run(["ar", "rcs", lib, *objs])
+    tdir = os.path.join(REPO, "tests", "fortran", "_build")
+    os.makedirs(tdir, exist_ok=True)
+    rel = os.path.relpath(OUT, tdir)
+    _run([FC, "-O2", "-J", fdir, "-o", os.path.join(tdir, "dropin_driver"),
+          os.path.join(REPO, "tests", "fortran", "dropin_driver.F90"), lib, "-L", OUT, "-lnsx",
+          f"-Wl,-rpath,$ORIGIN/{rel}"])
+
+
 def build_all():
     build_host()
     build_hip()
+    build_fortran()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"host": build_host, "hip": build_hip, "all": build_all}[what]()
+    {"host": build_host, "hip": build_hip, "fortran": build_fortran, "all": build_all}[what]()

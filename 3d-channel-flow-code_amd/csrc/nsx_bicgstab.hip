@@ -42,7 +42,19 @@
 // the host and for later phases.  Each kernel exits at once when st->done is
 // set, so the host can queue many iterations ahead and poll rarely.  Inner
 // products and norms are accumulated in FP64 in both precisions.
+#include <hip/hip_ext.h>
+
 #include "nsx_kernels.h"
+
+// With timing on, a kernel is launched with its own start/stop events
+// (recorded by the dispatch itself, so the elapsed time is the kernel's)
+#define LAUNCH_TIMED(ev, q, kern, grid, blk, lds, s, ...)                                         \
+  do {                                                                                          \
+    if (ev)                                                                                     \
+      hipExtLaunchKernelGGL(kern, grid, blk, lds, s, (ev)[2 * (q)], (ev)[2 * (q) + 1], 0, __VA_ARGS__); \
+    else                                                                                        \
+      hipLaunchKernelGGL(kern, grid, blk, lds, s, __VA_ARGS__);                                 \
+  } while (0)
 
 #define FOR_EACH_CELL(g, c)                                                         \
   uint32_t beg_, end_;                                                              \
@@ -718,17 +730,14 @@ static int iter_impl(const Geo &g, hipStream_t s, T *x, T *r, const T *rhat, T *
                      Partials part, SolverState *st, hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(T));
-  if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_bicg_a1<T>, grid, blk, lds, s, g, r, p, v, y, part, st, (double *)nullptr, (const T *)nullptr);
-  if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_bicg_a2<T>, grid, blk, lds, s, g, y, rhat, v, part, st);
-  if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL(k_bicg_b1<T>, grid, blk, lds, s, g, r, v, sv, z, part, st);
-  if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_bicg_b2<T>, grid, blk, lds, s, g, z, sv, tv, part, st);
-  if (ev) (void)hipEventRecord(ev[4], s);
-  hipLaunchKernelGGL(k_bicg_c<T>, grid, blk, lds, s, g, x, r, rhat, y, z, sv, tv, part, st);
-  if (ev) (void)hipEventRecord(ev[5], s);
+  LAUNCH_TIMED(ev, 0, k_bicg_a1<T>, grid, blk, lds, s, g, (const T *)r, p, (const T *)v, y, part, st,
+               (double *)nullptr, (const T *)nullptr);
+  LAUNCH_TIMED(ev, 1, k_bicg_a2<T>, grid, blk, lds, s, g, (const T *)y, rhat, v, part, (const SolverState *)st);
+  LAUNCH_TIMED(ev, 2, k_bicg_b1<T>, grid, blk, lds, s, g, (const T *)r, (const T *)v, sv, z, part, st);
+  LAUNCH_TIMED(ev, 3, k_bicg_b2<T>, grid, blk, lds, s, g, (const T *)z, (const T *)sv, tv, part,
+               (const SolverState *)st);
+  LAUNCH_TIMED(ev, 4, k_bicg_c<T>, grid, blk, lds, s, g, x, r, rhat, (const T *)y, (const T *)z, (const T *)sv,
+               (const T *)tv, part, st);
   return (int)hipGetLastError();
 }
 
@@ -763,21 +772,17 @@ int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, f
                     hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(float));
-  if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(k_rel_u, grid, blk, 0, s, g, rhs, (const double *)x, (const float *)e, r, rhat, part, st);
-  if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_bicg_a1<float>, grid, blk, lds, s, g, (const float *)r, p, (const float *)v, y, part, st, x,
-                     (const float *)e);
-  if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL(k_bicg_a2<float>, grid, blk, lds, s, g, (const float *)y, rhat, v, part, st);
-  if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_bicg_b1<float>, grid, blk, lds, s, g, (const float *)r, (const float *)v, sv, z, part, st);
-  if (ev) (void)hipEventRecord(ev[4], s);
-  hipLaunchKernelGGL(k_bicg_b2<float>, grid, blk, lds, s, g, (const float *)z, (const float *)sv, tv, part, st);
-  if (ev) (void)hipEventRecord(ev[5], s);
-  hipLaunchKernelGGL(k_bicg_c<float>, grid, blk, lds, s, g, e, r, rhat, (const float *)y, (const float *)z,
-                     (const float *)sv, (const float *)tv, part, st);
-  if (ev) (void)hipEventRecord(ev[6], s);
+  LAUNCH_TIMED(ev, 0, k_rel_u, grid, blk, (size_t)0, s, g, rhs, (const double *)x, (const float *)e, r, rhat, part,
+               st);
+  LAUNCH_TIMED(ev, 1, k_bicg_a1<float>, grid, blk, lds, s, g, (const float *)r, p, (const float *)v, y, part, st, x,
+               (const float *)e);
+  LAUNCH_TIMED(ev, 2, k_bicg_a2<float>, grid, blk, lds, s, g, (const float *)y, rhat, v, part,
+               (const SolverState *)st);
+  LAUNCH_TIMED(ev, 3, k_bicg_b1<float>, grid, blk, lds, s, g, (const float *)r, (const float *)v, sv, z, part, st);
+  LAUNCH_TIMED(ev, 4, k_bicg_b2<float>, grid, blk, lds, s, g, (const float *)z, (const float *)sv, tv, part,
+               (const SolverState *)st);
+  LAUNCH_TIMED(ev, 5, k_bicg_c<float>, grid, blk, lds, s, g, e, r, rhat, (const float *)y, (const float *)z,
+               (const float *)sv, (const float *)tv, part, st);
   return (int)hipGetLastError();
 }
 

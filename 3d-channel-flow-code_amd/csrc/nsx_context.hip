@@ -657,8 +657,8 @@ extern "C" int nsx_copy_field(nsx_ctx *ctx, int dst, int src) {
   NEED_COMMIT(ctx);
   if (check_field(ctx, dst) || check_field(ctx, src)) return -1;
   if (ctx->kind[dst] != ctx->kind[src]) return nsx_fail(ctx, -1, "field kinds differ");
-  HIPCHECK(hipMemcpyAsync(ctx->slot[dst], ctx->slot[src], field_elems(ctx, ctx->kind[src]) * sizeof(double),
-                          hipMemcpyDeviceToDevice, ctx->stream));
+  if (dst == src) return 0;
+  LAUNCH(launch_copy(ctx->stream, field_elems(ctx, ctx->kind[src]), ctx->slot[src], ctx->slot[dst]));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -709,14 +709,14 @@ static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, dou
     for (int c = 0; c < chunk; ++c) {
       hipEvent_t *ev = nullptr;
       if (ctx->timing) {
-        ev = events(ctx, 6);
+        ev = events(ctx, 10);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
       LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->svv[0], ctx->sp[1], ctx->svv[1],
                               ctx->ss, ctx->st_, ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
-        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, q});
+        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, q});
       }
       ++issued;
     }
@@ -789,14 +789,14 @@ static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
       for (int c = 0; c < chunk; ++c) {
         hipEvent_t *ev = nullptr;
         if (ctx->timing) {
-          ev = events(ctx, 6);
+          ev = events(ctx, 10);
           if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
         }
         LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
                                   ctx->ft, ctx->part, ctx->st_dev, ev));
         if (ev) {
           const size_t base = (size_t)(ev - &ctx->evpool[0]);
-          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, 6 + q});
+          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, 6 + q});
         }
         ++issued;
       }
@@ -851,7 +851,7 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
     for (int c = 0; c < chunk; ++c) {
       hipEvent_t *ev = nullptr;
       if (ctx->timing) {
-        ev = events(ctx, 7);
+        ev = events(ctx, 12);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
       LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
@@ -859,7 +859,7 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
         ctx->pending.push_back({base, 11});
-        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 1 + q, 6 + q});
+        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 + 2 * q, 6 + q});
       }
       ++issued;
     }
@@ -933,6 +933,11 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
     if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
     (void)hipEventRecord(ev[0], s);
   }
+  // the reference fills the sigma ghost layers of the stage input in place
+  // (BCvelcenter3D inside BCglobalVC of sgles/AdvDiffNS, BCpcenter of p);
+  // a resident input already holds them, an input put from the host may not
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
+  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
   const double *uvw[3] = {X[0], X[1], X[2]};
   const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
   // vertex values of the stage velocity (tagdir 2), shared by sgles and AdvDiffNS

@@ -65,13 +65,14 @@ int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, 
 int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]);
 int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 1 u, 2 v, 3 w*/);
 int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0);
+int launch_copy(hipStream_t s, size_t n, const double *src, double *dst);
 
 // ---- solver
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
                      double *rhat, Partials part, SolverState *st, double eps, int max_iters);
 int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
                      double *y, double *z, double *sv, double *tv, Partials part, SolverState *st,
-                     hipEvent_t *ev /* nullable: 6 events bracketing A1, A2, B1, B2, C */);
+                     hipEvent_t *ev /* nullable: 10 events, start/stop of A1, A2, B1, B2, C */);
 int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *r, const float *rhat, float *p, float *v,
                        float *y, float *z, float *sv, float *tv, Partials part, SolverState *st, hipEvent_t *ev);
 // mixed precision: reset, refinement residual (+ outer test, inner reset), correction
@@ -86,6 +87,6 @@ int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double
                     int max_iters);
 int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, const float *rhat,
                     float *p, float *v, float *y, float *z, float *sv, float *tv, Partials part, SolverState *st,
-                    hipEvent_t *ev /* nullable: 7 events bracketing U, A1, A2, B1, B2, C */);
+                    hipEvent_t *ev /* nullable: 12 events, start/stop of U, A1, A2, B1, B2, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

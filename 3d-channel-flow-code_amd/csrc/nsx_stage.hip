@@ -573,3 +573,19 @@ int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind) {
   hipLaunchKernelGGL(k_ghost_layers, dim3((g.N0 + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, f, kind);
   return (int)hipGetLastError();
 }
+
+// dst = src over n doubles: the field copy of nsx_copy_field, also the
+// known-size copy that HBM traffic counters are calibrated against (reads and
+// writes 8 n bytes)
+__global__ __launch_bounds__(NSX_BLOCK) void k_copy(size_t n, const double *src, double *dst) {
+  for (size_t q = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; q < n; q += (size_t)gridDim.x * NSX_BLOCK)
+    dst[q] = src[q];
+}
+
+int launch_copy(hipStream_t s, size_t n, const double *src, double *dst) {
+  size_t blocks = (n + NSX_BLOCK - 1) / NSX_BLOCK;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) return 0;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(NSX_BLOCK), 0, s, n, src, dst);
+  return (int)hipGetLastError();
+}

@@ -1,0 +1,213 @@
+!  nsx_dropin.F90 -- Fortran host side of the drop-in: the reference's hot-path
+!  subroutines under their own names and argument lists, over the C ABI.
+!
+!  module nsx_dropin holds the device context and the sizes of COMMON /coo/
+!  (N_CELL0, N_CELL, N_VERT, NZ; common.mpf:323-325).  nsx_dropin_setup hands
+!  the reference's set-up arrays (module geometry after calcul_geometry and
+!  PDE_center, module les, the run parameters) to the device once.  The
+!  external subroutines below replace the reference routines of the same name:
+!  they upload their array arguments, run the routine on the device and
+!  download the results, so a caller that passes host arrays sees the
+!  reference's interface.  Module state of the reference that the device keeps
+!  (U1FACE..UBFACE, rhsuf/rhsvf/rhswf, eddy) stays resident between calls.
+!  For the time loop itself nsx_dropin_step runs whole steps on the resident
+!  state without host transfers.
+!
+!  Errors follow the reference: a fatal error prints and stops; a solver that
+!  does not converge prints and returns (bicgstab.F90:562-566).
+      module nsx_dropin
+      use iso_c_binding
+      use nsx_iface
+      implicit none
+      type(c_ptr), save :: nsx_ctx = c_null_ptr
+      integer, save :: nsx_n_cell0 = 0, nsx_n_cell = 0, nsx_n_vert = 0, nsx_nz = 0
+      type(nsx_solve_info), save :: nsx_last_info
+
+      contains
+
+      subroutine nsx_dropin_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        if (rc < 0) then
+          write(*,'(a,a,a,a)') ' nsx: ', where, ': ', trim(nsx_error_text(nsx_ctx))
+          stop 'nsx: fatal error'
+        else if (rc > 0) then
+          write(*,'(a,a,a,i3)') ' nsx: ', where, ': solver outcome ', rc
+        endif
+      end subroutine nsx_dropin_check
+
+!     returns 0, or the (negative) code of nsx_create when no device is usable
+      integer function nsx_dropin_open(N_CELL0, N_CELL, N_VERT, NZ)
+        integer, intent(in) :: N_CELL0, N_CELL, N_VERT, NZ
+        nsx_n_cell0 = N_CELL0
+        nsx_n_cell = N_CELL
+        nsx_n_vert = N_VERT
+        nsx_nz = NZ
+        nsx_dropin_open = nsx_create(N_CELL0, N_CELL, N_VERT, NZ, nsx_ctx)
+      end function nsx_dropin_open
+
+      subroutine nsx_dropin_setup(No_cp, No_vp, nbe, nbev,               &
+           areaCell, dlVV, dlCC, dxCC, dyCC, xnn, ynn, dle1, dle2, doe,   &
+           sum_xc2, sum_yc2, sum_xcyc, TotalVolume,                       &
+           sig, dsig, sigv, dsigv, sigIni, sigFin,                        &
+           surrounding, weight, Dimsurrounding, dlVsum, vertex_pair,      &
+           vertex_edge, bcuc, bcvc, bcwc, bcpc, bcuv, bcvv, bcwv, bcpv,   &
+           bctop, bcbot, TopBC, BotBC, AmC0, AmC1, AmC2, AmC3, AmCT, AmCB,&
+           walld_level, dt, Re, bdx, bdy, bdz, eps, MaxIters, FinRK, Iturbu)
+        integer(c_int), intent(in) :: No_cp(*), No_vp(*), nbe(*), nbev(*)
+        real(c_double), intent(in) :: areaCell(*), dlVV(*), dlCC(*),     &
+             dxCC(*), dyCC(*), xnn(*), ynn(*), dle1(*), dle2(*), doe(*),  &
+             sum_xc2(*), sum_yc2(*), sum_xcyc(*)
+        real(c_double), intent(in) :: TotalVolume, sigIni, sigFin
+        real(c_double), intent(in) :: sig(*), dsig(*), sigv(*), dsigv(*)
+        integer(c_int), intent(in) :: surrounding(*), Dimsurrounding(*), &
+             vertex_pair(*), vertex_edge(*)
+        real(c_double), intent(in) :: weight(*), dlVsum(*)
+        integer(c_int), intent(in) :: bcuc(*), bcvc(*), bcwc(*), bcpc(*),&
+             bcuv(*), bcvv(*), bcwv(*), bcpv(*)
+        integer(c_int), intent(in) :: bctop, bcbot, TopBC, BotBC
+        real(c_double), intent(in) :: AmC0(*), AmC1(*), AmC2(*), AmC3(*),&
+             AmCT(*), AmCB(*), walld_level(*)
+        real(c_double), intent(in) :: dt, Re, bdx, bdy, bdz, eps
+        integer(c_int), intent(in) :: MaxIters, FinRK, Iturbu
+        call nsx_dropin_check(nsx_set_mesh(nsx_ctx, No_cp, No_vp, nbe,   &
+             nbev), 'nsx_set_mesh')
+        call nsx_dropin_check(nsx_set_geometry(nsx_ctx, areaCell, dlVV,  &
+             dlCC, dxCC, dyCC, xnn, ynn, dle1, dle2, doe, sum_xc2,        &
+             sum_yc2, sum_xcyc, TotalVolume), 'nsx_set_geometry')
+        call nsx_dropin_check(nsx_set_sigma(nsx_ctx, sig, dsig, sigv,    &
+             dsigv, sigIni, sigFin), 'nsx_set_sigma')
+        call nsx_dropin_check(nsx_set_vertex_weights(nsx_ctx,            &
+             surrounding, weight, Dimsurrounding, dlVsum, vertex_pair,    &
+             vertex_edge), 'nsx_set_vertex_weights')
+        call nsx_dropin_check(nsx_set_bc(nsx_ctx, bcuc, bcvc, bcwc, bcpc,&
+             bcuv, bcvv, bcwv, bcpv, bctop, bcbot, TopBC, BotBC),         &
+             'nsx_set_bc')
+        call nsx_dropin_check(nsx_set_ppe_matrix(nsx_ctx, AmC0, AmC1,    &
+             AmC2, AmC3, AmCT, AmCB), 'nsx_set_ppe_matrix')
+        call nsx_dropin_check(nsx_set_les(nsx_ctx, walld_level),         &
+             'nsx_set_les')
+        call nsx_dropin_check(nsx_set_params(nsx_ctx, dt, Re, bdx, bdy,  &
+             bdz, eps, MaxIters, FinRK, Iturbu), 'nsx_set_params')
+        call nsx_dropin_check(nsx_commit(nsx_ctx), 'nsx_commit')
+      end subroutine nsx_dropin_setup
+
+!     the resident time loop: nsteps of nsmp3D.F90:404-564 on the device
+      subroutine nsx_dropin_step(nsteps, precision)
+        integer, intent(in) :: nsteps, precision
+        call nsx_dropin_check(nsx_step(nsx_ctx, nsteps, precision,       &
+             nsx_last_info), 'nsx_step')
+      end subroutine nsx_dropin_step
+
+      subroutine nsx_dropin_put(field, a)
+        integer(c_int), intent(in) :: field
+        real(c_double), intent(in) :: a(*)
+        call nsx_dropin_check(nsx_put_field(nsx_ctx, field, a), 'nsx_put_field')
+      end subroutine nsx_dropin_put
+
+      subroutine nsx_dropin_get(field, a)
+        integer(c_int), intent(in) :: field
+        real(c_double), intent(inout) :: a(*)
+        call nsx_dropin_check(nsx_get_field(nsx_ctx, field, a, 1), 'nsx_get_field')
+      end subroutine nsx_dropin_get
+
+      subroutine nsx_dropin_close()
+        integer(c_int) :: rc
+        if (c_associated(nsx_ctx)) rc = nsx_destroy(nsx_ctx)
+        nsx_ctx = c_null_ptr
+      end subroutine nsx_dropin_close
+
+      end module nsx_dropin
+
+!-----------------------------------------------------------------------
+!  bicgstab.F90:8-168 -- solution of the pressure Poisson equation
+      SUBROUTINE bicgstab(phi,phiv,rhs,                                  &
+                          xc,yc,sig,dsig,No_cp,nbe,                      &
+                          xv,yv,sigv,dsigv,No_vp,nbev,                   &
+                          Hpr,h,etan,                                    &
+                          Hprv,hv,etav)
+      use nsx_dropin
+      implicit none
+      real*8 :: phi(nsx_n_cell,nsx_nz), phiv(nsx_n_vert,nsx_nz-1)
+      real*8 :: rhs(nsx_n_cell,nsx_nz)
+      real*8 :: xc(nsx_n_cell), yc(nsx_n_cell), sig(nsx_nz), dsig(nsx_nz)
+      integer :: No_cp(nsx_n_cell,3), nbe(nsx_n_cell0)
+      real*8 :: xv(nsx_n_vert), yv(nsx_n_vert), sigv(nsx_nz-1), dsigv(nsx_nz-1)
+      integer :: No_vp(nsx_n_cell0,3), nbev(nsx_n_vert)
+      real*8 :: Hpr(nsx_n_cell), h(nsx_n_cell), etan(nsx_n_cell)
+      real*8 :: Hprv(nsx_n_vert), hv(nsx_n_vert), etav(nsx_n_vert)
+      integer(c_int) :: rc
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH, phi)
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH + 1, rhs)
+      rc = nsx_bicgstab(nsx_ctx, NSX_F_CELL_SCRATCH, NSX_F_VERT_SCRATCH, &
+                        NSX_F_CELL_SCRATCH + 1, nsx_last_info)
+      call nsx_dropin_check(rc, 'bicgstab')
+      call nsx_dropin_get(NSX_F_CELL_SCRATCH, phi)
+      call nsx_dropin_get(NSX_F_VERT_SCRATCH, phiv)
+      END SUBROUTINE bicgstab
+
+!-----------------------------------------------------------------------
+!  hydro.F90:8-501 -- one projection stage
+      SUBROUTINE hydro(ufnp,ufn,uf,ufv,                                  &
+                       vfnp,vfn,vf,vfv,                                  &
+                       wfnp,wfn,wf,wfv,                                  &
+                       pfnp,pfn,pf,pfv,                                  &
+                       rhof,viscof,                                      &
+                       rhofv,viscofv,                                    &
+                       xc,yc,sig,dsig,No_cp,nbe,                         &
+                       xv,yv,sigv,dsigv,No_vp,nbev,                      &
+                       Hpr,h,eta,etan,                                   &
+                       Hprv,hv,etav,                                     &
+                       flag_ab)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(nsx_n_cell,nsx_nz) :: ufnp, ufn, uf, vfnp, vfn, vf, &
+           wfnp, wfn, wf, pfnp, pfn, pf, rhof, viscof
+      real*8, dimension(nsx_n_vert,nsx_nz-1) :: ufv, vfv, wfv, pfv, rhofv, viscofv
+      real*8 :: xc(nsx_n_cell), yc(nsx_n_cell), sig(nsx_nz), dsig(nsx_nz)
+      integer :: No_cp(nsx_n_cell,3), nbe(nsx_n_cell0)
+      real*8 :: xv(nsx_n_vert), yv(nsx_n_vert), sigv(nsx_nz-1), dsigv(nsx_nz-1)
+      integer :: No_vp(nsx_n_cell0,3), nbev(nsx_n_vert)
+      real*8 :: Hpr(nsx_n_cell), h(nsx_n_cell), eta(nsx_n_cell), etan(nsx_n_cell)
+      real*8 :: Hprv(nsx_n_vert), hv(nsx_n_vert), etav(nsx_n_vert)
+      integer :: flag_ab
+      integer(c_int) :: rc
+!     stage input (uf..pf) and time level n; the module state of the reference
+!     (faces, AB2 history, eddy) is the device's resident copy
+      call nsx_dropin_put(NSX_F_UF, uf)
+      call nsx_dropin_put(NSX_F_VF, vf)
+      call nsx_dropin_put(NSX_F_WF, wf)
+      call nsx_dropin_put(NSX_F_PF, pf)
+      call nsx_dropin_put(NSX_F_UFN, ufn)
+      call nsx_dropin_put(NSX_F_VFN, vfn)
+      call nsx_dropin_put(NSX_F_WFN, wfn)
+      call nsx_dropin_put(NSX_F_PFN, pfn)
+      rc = nsx_hydro(nsx_ctx, flag_ab, nsx_last_info)
+      call nsx_dropin_check(rc, 'hydro')
+      call nsx_dropin_get(NSX_F_UFNP, ufnp)
+      call nsx_dropin_get(NSX_F_VFNP, vfnp)
+      call nsx_dropin_get(NSX_F_WFNP, wfnp)
+      call nsx_dropin_get(NSX_F_PFNP, pfnp)
+      call nsx_dropin_get(NSX_F_UFV, ufv)
+      call nsx_dropin_get(NSX_F_VFV, vfv)
+      call nsx_dropin_get(NSX_F_WFV, wfv)
+      call nsx_dropin_get(NSX_F_PFV, pfv)
+      END SUBROUTINE hydro
+
+!-----------------------------------------------------------------------
+!  gradientLSM.F90:8-211 -- least-squares cell gradients
+      SUBROUTINE grandientLSM(dfundx,dfundy,dfundsig,                    &
+                              fun,xc,yc,sig,dsig,No_cp,nbe)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(nsx_n_cell,nsx_nz) :: dfundx, dfundy, dfundsig, fun
+      real*8 :: xc(nsx_n_cell), yc(nsx_n_cell), sig(nsx_nz), dsig(nsx_nz)
+      integer :: No_cp(nsx_n_cell,3), nbe(nsx_n_cell0)
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH + 3, fun)
+      call nsx_dropin_check(nsx_grad_lsm(nsx_ctx, NSX_F_CELL_SCRATCH,    &
+           NSX_F_CELL_SCRATCH + 1, NSX_F_CELL_SCRATCH + 2,                &
+           NSX_F_CELL_SCRATCH + 3), 'grandientLSM')
+      call nsx_dropin_get(NSX_F_CELL_SCRATCH, dfundx)
+      call nsx_dropin_get(NSX_F_CELL_SCRATCH + 1, dfundy)
+      call nsx_dropin_get(NSX_F_CELL_SCRATCH + 2, dfundsig)
+      END SUBROUTINE grandientLSM

@@ -1,0 +1,278 @@
+!  nsx_iface.F90 -- ISO_C_BINDING interfaces of the nsx C ABI (include/nsx.h)
+!
+!  One interface per entry point of include/nsx.h, with the same argument
+!  meaning: arrays are passed by address in the reference layout ((N_CELL,NZ),
+!  (N_VERT,NZ-1), (N_CELL0,NZ) column-major, 1-based connectivity), scalars by
+!  value, the context as an opaque c_ptr.  Return codes: 0 ok, > 0 a soft
+!  solver outcome (NSX_SOLVE_*), < 0 an error whose text nsx_last_error gives.
+      module nsx_iface
+      use iso_c_binding
+      implicit none
+
+!     field ids (enum nsx_field)
+      integer(c_int), parameter :: NSX_F_UFN = 0, NSX_F_VFN = 1,        &
+           NSX_F_WFN = 2, NSX_F_PFN = 3, NSX_F_UF = 4, NSX_F_VF = 5,     &
+           NSX_F_WF = 6, NSX_F_PF = 7, NSX_F_UFNP = 8, NSX_F_VFNP = 9,   &
+           NSX_F_WFNP = 10, NSX_F_PFNP = 11, NSX_F_RHSUF = 12,           &
+           NSX_F_RHSVF = 13, NSX_F_RHSWF = 14, NSX_F_EDDY = 15,          &
+           NSX_F_UFV = 32, NSX_F_VFV = 33, NSX_F_WFV = 34,               &
+           NSX_F_PFV = 35, NSX_F_U1FACE = 48, NSX_F_U2FACE = 49,         &
+           NSX_F_U3FACE = 50, NSX_F_UTFACE = 51, NSX_F_UBFACE = 52,      &
+           NSX_F_CELL_SCRATCH = 64, NSX_F_VERT_SCRATCH = 96,             &
+           NSX_F_FACE_SCRATCH = 112
+!     solver outcome (enum nsx_solve_exit)
+      integer(c_int), parameter :: NSX_SOLVE_CONVERGED = 0,             &
+           NSX_SOLVE_NO_SOLVE = 1, NSX_SOLVE_BREAKDOWN = 2,              &
+           NSX_SOLVE_MAX_ITERS = 3
+!     precision and refinement (enum nsx_precision, enum nsx_refine)
+      integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1
+      integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
+           NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2
+
+      type, bind(C) :: nsx_solve_info
+        integer(c_int) :: iters, exit_code
+        real(c_double) :: e1norm, e2norm
+        integer(c_int) :: n_solves, iters_total, outer_passes, iters_fp32
+      end type nsx_solve_info
+
+      interface
+        integer(c_int) function nsx_create(n_cell0, n_cell, n_vert, nz, &
+             ctx) bind(C, name="nsx_create")
+          import :: c_int, c_ptr
+          integer(c_int), value :: n_cell0, n_cell, n_vert, nz
+          type(c_ptr) :: ctx
+        end function
+        integer(c_int) function nsx_destroy(ctx) bind(C, name="nsx_destroy")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        type(c_ptr) function nsx_last_error(ctx)                         &
+             bind(C, name="nsx_last_error")
+          import :: c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function nsx_device(ctx) bind(C, name="nsx_device")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function nsx_set_mesh(ctx, No_cp, No_vp, nbe,     &
+             nbev) bind(C, name="nsx_set_mesh")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(in) :: No_cp(*), No_vp(*), nbe(*), nbev(*)
+        end function
+        integer(c_int) function nsx_set_geometry(ctx, areaCell, dlVV,    &
+             dlCC, dxCC, dyCC, xnn, ynn, dle1, dle2, doe, sum_xc2,        &
+             sum_yc2, sum_xcyc, TotalVolume)                              &
+             bind(C, name="nsx_set_geometry")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: areaCell(*), dlVV(*), dlCC(*),   &
+               dxCC(*), dyCC(*), xnn(*), ynn(*), dle1(*), dle2(*),        &
+               doe(*), sum_xc2(*), sum_yc2(*), sum_xcyc(*)
+          real(c_double), value :: TotalVolume
+        end function
+        integer(c_int) function nsx_set_sigma(ctx, sig, dsig, sigv,      &
+             dsigv, sigIni, sigFin) bind(C, name="nsx_set_sigma")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: sig(*), dsig(*), sigv(*), dsigv(*)
+          real(c_double), value :: sigIni, sigFin
+        end function
+        integer(c_int) function nsx_set_vertex_weights(ctx, surrounding, &
+             weight, Dimsurrounding, dlVsum, vertex_pair, vertex_edge)    &
+             bind(C, name="nsx_set_vertex_weights")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(in) :: surrounding(*), Dimsurrounding(*), &
+               vertex_pair(*), vertex_edge(*)
+          real(c_double), intent(in) :: weight(*), dlVsum(*)
+        end function
+        integer(c_int) function nsx_set_bc(ctx, bcuc, bcvc, bcwc, bcpc,  &
+             bcuv, bcvv, bcwv, bcpv, bctop, bcbot, TopBC, BotBC)          &
+             bind(C, name="nsx_set_bc")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), intent(in) :: bcuc(*), bcvc(*), bcwc(*),        &
+               bcpc(*), bcuv(*), bcvv(*), bcwv(*), bcpv(*)
+          integer(c_int), value :: bctop, bcbot, TopBC, BotBC
+        end function
+        integer(c_int) function nsx_set_ppe_matrix(ctx, AmC0, AmC1,      &
+             AmC2, AmC3, AmCT, AmCB) bind(C, name="nsx_set_ppe_matrix")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: AmC0(*), AmC1(*), AmC2(*),       &
+               AmC3(*), AmCT(*), AmCB(*)
+        end function
+        integer(c_int) function nsx_set_les(ctx, walld_level)            &
+             bind(C, name="nsx_set_les")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), intent(in) :: walld_level(*)
+        end function
+        integer(c_int) function nsx_set_params(ctx, dt, Re, bdx, bdy,    &
+             bdz, eps, MaxIters, FinRK, Iturbu)                           &
+             bind(C, name="nsx_set_params")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double), value :: dt, Re, bdx, bdy, bdz, eps
+          integer(c_int), value :: MaxIters, FinRK, Iturbu
+        end function
+        integer(c_int) function nsx_commit(ctx) bind(C, name="nsx_commit")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function nsx_put_field(ctx, field, host)          &
+             bind(C, name="nsx_put_field")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: field
+          real(c_double), intent(in) :: host(*)
+        end function
+        integer(c_int) function nsx_get_field(ctx, field, host,          &
+             fill_ghost_rows) bind(C, name="nsx_get_field")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: field, fill_ghost_rows
+          real(c_double), intent(inout) :: host(*)
+        end function
+        integer(c_int) function nsx_copy_field(ctx, dst, src)            &
+             bind(C, name="nsx_copy_field")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: dst, src
+        end function
+        integer(c_int) function nsx_init_state(ctx)                      &
+             bind(C, name="nsx_init_state")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function nsx_hydro(ctx, flag_ab, info)            &
+             bind(C, name="nsx_hydro")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: flag_ab
+          type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_step(ctx, nsteps, precision, info)   &
+             bind(C, name="nsx_step")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: nsteps, precision
+          type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_solve_info_get(ctx, info)            &
+             bind(C, name="nsx_solve_info_get")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_bcglobalvc(ctx, u, v, w, uv, vv, wv, &
+             tagdir) bind(C, name="nsx_bcglobalvc")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: u, v, w, uv, vv, wv, tagdir
+        end function
+        integer(c_int) function nsx_bcglobalp(ctx, p, pv, tagdir)        &
+             bind(C, name="nsx_bcglobalp")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: p, pv, tagdir
+        end function
+        integer(c_int) function nsx_grad_lsm(ctx, dfdx, dfdy, dfds, f)   &
+             bind(C, name="nsx_grad_lsm")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: dfdx, dfdy, dfds, f
+        end function
+        integer(c_int) function nsx_calcul_face(ctx, f1, f2, f3, fT, fB, &
+             fun, funv) bind(C, name="nsx_calcul_face")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: f1, f2, f3, fT, fB, fun, funv
+        end function
+        integer(c_int) function nsx_sgles(ctx, u, v, w)                  &
+             bind(C, name="nsx_sgles")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: u, v, w
+        end function
+        integer(c_int) function nsx_advdiff(ctx, rhsu, rhsv, rhsw, u, v, &
+             w, eddy) bind(C, name="nsx_advdiff")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: rhsu, rhsv, rhsw, u, v, w, eddy
+        end function
+        integer(c_int) function nsx_source3d(ctx, src, u, v, w)          &
+             bind(C, name="nsx_source3d")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: src, u, v, w
+        end function
+        integer(c_int) function nsx_update_rhsp(ctx, rhs, p, pv)         &
+             bind(C, name="nsx_update_rhsp")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: rhs, p, pv
+        end function
+        integer(c_int) function nsx_bicgstab(ctx, phi, phiv, rhs, info)  &
+             bind(C, name="nsx_bicgstab")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: phi, phiv, rhs
+          type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_newu3d(ctx, p, pv)                   &
+             bind(C, name="nsx_newu3d")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: p, pv
+        end function
+        integer(c_int) function nsx_set_precision(ctx, precision, refine,&
+             tol, max_passes) bind(C, name="nsx_set_precision")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: precision, refine, max_passes
+          real(c_double), value :: tol
+        end function
+        integer(c_int) function nsx_timing_enable(ctx, on)               &
+             bind(C, name="nsx_timing_enable")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: on
+        end function
+        integer(c_int) function nsx_timing_get(ctx, name, total_ms,      &
+             launches) bind(C, name="nsx_timing_get")
+          import :: c_int, c_ptr, c_char, c_double, c_long_long
+          type(c_ptr), value :: ctx
+          character(kind=c_char), intent(in) :: name(*)
+          real(c_double) :: total_ms
+          integer(c_long_long) :: launches
+        end function
+        integer(c_int) function nsx_timing_reset(ctx)                    &
+             bind(C, name="nsx_timing_reset")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+      end interface
+
+      contains
+
+!     the text of nsx_last_error as a Fortran string
+      function nsx_error_text(ctx) result(msg)
+        type(c_ptr), intent(in) :: ctx
+        character(len=512) :: msg
+        character(kind=c_char), pointer :: s(:)
+        type(c_ptr) :: p
+        integer :: n
+        msg = ' '
+        p = nsx_last_error(ctx)
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, s, [512])
+        do n = 1, 512
+          if (s(n) == c_null_char) exit
+          msg(n:n) = s(n)
+        enddo
+      end function nsx_error_text
+
+      end module nsx_iface

@@ -220,6 +220,12 @@ class Context:
         return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
 
     # ------------------------------------------------------------- timing ---
+    def set_max_iters(self, n: int) -> None:
+        """MaxIters of the pressure solve (the case's other parameters kept)."""
+        p = self.case.params
+        self._check(self.lib.nsx_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, int(n), p.FinRK,
+                                            p.Iturbu))
+
     def set_precision(self, precision: int, refine: int = 2, tol: float = 1e-5, max_passes: int = 30) -> None:
         """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: FP32
         BiCGSTAB inside FP64 refinement: refine 0 = reliable updates with

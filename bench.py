@@ -47,6 +47,8 @@ HBM_PEAK_GBS = 8000.0
 # x,y,z,s,t,rhat, writes x,r
 PHASE_UNITS = {"a1": 5, "a2": 3, "b1": 4, "b2": 3, "c": 8}
 KERNEL_NAMES = {"a1": "k_bicg_a1", "a2": "k_bicg_a2", "b1": "k_bicg_b1", "b2": "k_bicg_b2", "c": "k_bicg_c"}
+# workload tag used in file names of the profiles: quads in x, quads in y, layers
+TAG = {0: "32x16x32", 1: "128x64x64", 2: "256x128x96", 3: "64x32x48", 4: "512x256x128"}
 WORKLOAD = {
     0: "32x16 quads x 32 layers = 32,768 prisms, Re_tau 180, dt 1e-3",
     1: "128x64 quads x 64 layers = 1,048,576 prisms, Re_tau 180, dt 5e-4, jittered mesh",
@@ -100,11 +102,13 @@ def max_divergence(case, faces) -> float:
 MIXED_MODES = {"mixed": (2, 1e-5), "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
 
 
-def leg(case, name: str, steps: int, warmup: int) -> dict:
+def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
     from nsx.device import Context
 
     precision = 0 if name == "fp64" else 1
     ctx = Context(case)
+    if max_iters:
+        ctx.set_max_iters(max_iters)
     if precision:
         ctx.set_precision(1, *MIXED_MODES[name])
     ctx.init_state()
@@ -145,6 +149,19 @@ def leg(case, name: str, steps: int, warmup: int) -> dict:
             "fields": fields}
 
 
+def traffic_of(cfg: int, kernel):
+    """HBM bytes per launch of `kernel` (fetch + write) from the committed
+    rocprofv3 --pmc summary of this workload, or None"""
+    if not kernel:
+        return None
+    from nsx.profsum import read_traffic
+
+    t = read_traffic(os.path.join(REPO, "profiles", f"bicgstab_traffic_channel_{TAG[cfg]}.txt"))
+    if kernel not in t:
+        return None
+    return t[kernel][0] + t[kernel][1]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +171,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--legs", default="fp64,mixed", help="comma list of fp64, mixed, mixed_restart, mixed_reliable")
+    ap.add_argument("--max-iters", type=int, default=None,
+                    help="cap BiCGSTAB iterations (profiling runs only: the line is then marked capped)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy one cell field with k_copy (the known-size copy for HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -166,9 +187,16 @@ def main() -> int:
     steps = args.steps if args.steps is not None else {0: 10, 1: 20, 2: 10, 3: 20, 4: 3}[cfg]
     case = config_case(cfg, seed=args.seed)
     legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+    if args.calibrate_copy:
+        from nsx.device import Context
+        c0 = Context(case)
+        c0.copy(64, 0)  # CELL_SCRATCH <- UFN: N_CELL0 x NS doubles, NS = NZ rounded up to a multiple of 4
+        c0.close()
+        ns = (case.NZ + 3) // 4 * 4
+        print(f"# k_copy calibration: {case.N_CELL0 * ns} doubles", file=sys.stderr, flush=True)
     res = {}
     for name in legs:
-        res[name] = leg(case, name, steps, args.warmup)
+        res[name] = leg(case, name, steps, args.warmup, args.max_iters)
         print(f"# leg {name}: {res[name]['value']:.4g} prism-cell-steps/s, {res[name]['ms_per_step']:.1f} ms/step, "
               f"{res[name]['mean_iters_per_solve']:.1f} iterations/solve", file=sys.stderr, flush=True)
     head = res.get("mixed", res.get("fp64"))
@@ -192,11 +220,13 @@ def main() -> int:
                    "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": case.params.eps,
                    "inner_tolerance": 1e-5, "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(cfg, dom["kernel"]),
                      "avg_launch_us": dom["avg_us"], "bytes_per_launch": dom["bytes_per_launch"]},
     }
     for name, r in res.items():
         out[name] = {k: v for k, v in r.items() if k != "fields"}
+    if args.max_iters:
+        out["capped_max_iters"] = args.max_iters
     if "fp64" in res and "mixed" in res:
         out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
         fa, fb = res["fp64"]["fields"], res["mixed"]["fields"]

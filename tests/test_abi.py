@@ -32,3 +32,11 @@ def test_create_rejects_bad_sizes_without_touching_the_gpu():
     h = ctypes.c_void_p()
     assert lib.nsx_create(0, 0, 0, 0, ctypes.byref(h)) == -1
     assert lib.nsx_create(10, 5, 4, 6, ctypes.byref(h)) == -1
+
+
+def test_fortran_interfaces_cover_the_header():
+    """Every entry point of include/nsx.h has a bind(C) interface in the
+    Fortran binding, under its C name."""
+    f90 = open(os.path.join(REPO, "3d-channel-flow-code_amd", "fortran", "nsx_iface.F90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(nsx_\w+)"\)', f90))
+    assert set(declared()) == bound, sorted(set(declared()) ^ bound)

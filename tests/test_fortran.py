@@ -1,0 +1,125 @@
+"""The Fortran host side: the drop-in subroutines (bicgstab, hydro,
+grandientLSM under the reference's names and argument lists, over the
+ISO_C_BINDING interfaces of include/nsx.h) linked into a Fortran program.
+
+CPU: the drop-in library exports the reference's external names and the test
+driver links and runs (without a GPU it stops at nsx_create with its own
+status).  GPU: the driver's bicgstab and hydro calls against the oracle, with
+the tolerances of the C-ABI parity tests (pressure 1e-6, velocities 1e-8
+relative L2 over interior cells).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from common import interior, rel_err, tiny_case
+from conftest import REPO
+from nsx.case import config_case
+
+DRIVER = os.path.join(REPO, "tests", "fortran", "_build", "dropin_driver")
+FLIB = os.path.join(REPO, "3d-channel-flow-code_amd", "nsx", "_build", "libnsx_fortran.a")
+
+
+def write_case(case, rhs, path):
+    """The stream file dropin_driver.F90 reads (its order and kinds)."""
+    A, p = case.arrays, case.params
+    i32 = lambda a: np.asfortranarray(a, dtype=np.int32).ravel(order="F").tobytes()
+    f64 = lambda a: np.asfortranarray(a, dtype=np.float64).ravel(order="F").tobytes()
+    parts = [np.array([case.N_CELL0, case.N_CELL, case.N_VERT, case.NZ], np.int32).tobytes()]
+    for n in ("No_cp", "No_vp", "nbe", "nbev", "surrounding", "Dimsurrounding", "vertex_pair", "vertex_edge"):
+        parts.append(i32(A[n]))
+    parts.append(i32(np.stack([A[n] for n in ("bcuc", "bcvc", "bcwc", "bcpc")], axis=1)))
+    parts.append(i32(np.stack([A[n] for n in ("bcuv", "bcvv", "bcwv", "bcpv")], axis=1)))
+    parts.append(i32([case.bctop, case.bcbot, p.TopBC, p.BotBC, p.MaxIters, p.FinRK, p.Iturbu]))
+    parts.append(f64(A["areaCell"]))
+    parts.append(f64(np.stack([A[n] for n in ("dlVV", "dlCC", "dxCC", "dyCC", "xnn", "ynn", "dle1", "dle2", "doe")],
+                              axis=2)))
+    parts.append(f64(np.stack([A[n] for n in ("sum_xc2", "sum_yc2", "sum_xcyc")], axis=1)))
+    for n in ("sig", "dsig", "sigv", "dsigv", "weight", "dlVsum"):
+        parts.append(f64(A[n]))
+    parts.append(f64(np.stack([A[n] for n in ("AmC0", "AmC1", "AmC2", "AmC3", "AmCT", "AmCB")], axis=2)))
+    parts.append(f64(A["walld"]))
+    parts.append(f64([case.TotalVolume, p.sigIni, p.sigFin, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps]))
+    parts.append(f64(np.stack([A[n] for n in ("u0", "v0", "w0", "p0")], axis=2)))
+    parts.append(f64(rhs))
+    with open(path, "wb") as f:
+        f.write(b"".join(parts))
+
+
+def read_out(case, path):
+    nc, nz, nv = case.N_CELL, case.NZ, case.N_VERT
+    d = np.fromfile(path, dtype=np.float64)
+    o = 0
+
+    def take(shape):
+        nonlocal o
+        n = int(np.prod(shape))
+        a = d[o:o + n].reshape(shape, order="F")
+        o += n
+        return a
+
+    phi, phiv = take((nc, nz)), take((nv, nz - 1))
+    fnp, fv = take((nc, nz, 4)), take((nv, nz - 1, 4))
+    assert o == d.size
+    return phi, phiv, fnp, fv
+
+
+def _rhs(case, seed=5):
+    rhs = case.cell_field()
+    rhs[:case.N_CELL0, 1:-1] = np.random.default_rng(seed).standard_normal((case.N_CELL0, case.NZ - 2))
+    return rhs
+
+
+def test_dropin_exports_reference_names():
+    if not os.path.exists(FLIB):
+        pytest.skip("the Fortran drop-in is not built (no Fortran compiler)")
+    out = subprocess.run(["nm", FLIB], capture_output=True, text=True, check=True).stdout
+    for sym in ("bicgstab_", "hydro_", "grandientlsm_"):
+        assert f" T {sym}\n" in out, sym
+
+
+def test_driver_links_and_runs(tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.skip("the Fortran test driver is not built (no Fortran compiler)")
+    c, _ = tiny_case()
+    fin, fout = tmp_path / "case.bin", tmp_path / "out.bin"
+    write_case(c, _rhs(c), fin)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    if r.returncode == 3:
+        assert "no usable HIP device" in r.stdout
+    else:
+        assert r.returncode == 0, r.stdout + r.stderr
+        read_out(c, fout)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["tiny", "cfg0"])
+def test_dropin_vs_oracle(tmp_path, which):
+    if not os.path.exists(DRIVER):
+        pytest.skip("the Fortran test driver is not built (no Fortran compiler)")
+    c = tiny_case()[0] if which == "tiny" else config_case(0, jitter=0.1)
+    rhs = _rhs(c)
+    fin, fout = tmp_path / "case.bin", tmp_path / "out.bin"
+    write_case(c, rhs, fin)
+    print(f"dropin_driver {which}", flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    phi, phiv, fnp, fv = read_out(c, fout)
+    # bicgstab from phi = 0
+    O = oracle.Oracle(c)
+    ophi, ophiv = c.cell_field(), O.vert()
+    O.call("bicgstab", ophi, ophiv, rhs.copy(order="F"))
+    assert rel_err(interior(c, phi), interior(c, ophi)) <= 1e-6
+    assert rel_err(phiv, ophiv) <= 1e-6
+    # one hydro stage from the initial state
+    O2 = oracle.Oracle(c)
+    for n in ("uf", "vf", "wf", "pf"):
+        O2.state(n)[:] = O2.state(n[0] + "fn")
+    O2.hydro()
+    for q, (n, tol) in enumerate((("ufnp", 1e-8), ("vfnp", 1e-8), ("wfnp", 1e-8), ("pfnp", 1e-6))):
+        assert rel_err(interior(c, fnp[:, :, q]), interior(c, O2.state(n))) <= tol, n
+    for q, (n, tol) in enumerate((("ufv", 1e-8), ("pfv", 1e-6))):
+        assert rel_err(fv[:, :, 0 if q == 0 else 3], O2.state(n)) <= tol, n
