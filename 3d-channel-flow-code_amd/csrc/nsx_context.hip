@@ -61,6 +61,7 @@ struct nsx_ctx {
   // mixed precision: FP32 inner BiCGSTAB vectors and settings
   float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *fy = nullptr, *fz = nullptr, *fs = nullptr,
         *ft = nullptr, *fe = nullptr;
+  double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CASCADE;
   double delta = 0.1;
@@ -339,12 +340,14 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
 
   // neighbours: periodic ghosts resolved to their partner columns
   std::vector<int> nbr((size_t)N0 * 3), vtx((size_t)N0 * 3);
+  std::vector<signed char> eside((size_t)N0 * 3, 0);
   int nex = N0;
   for (int i = 0; i < N0; ++i)
     for (int j = 0; j < 3; ++j) {
       int jc = ctx->No_cp[(size_t)i + (size_t)j * NC] - 1;
       if (jc < 0 || jc >= NC) return nsx_fail(ctx, -1, "No_cp(%d,%d) = %d out of range", i + 1, j + 1, jc + 1);
       if (jc >= N0) {
+        eside[(size_t)i * 3 + j] = 1;
         if (jc + 1 > nex) nex = jc + 1;
         for (int q = 0; q < 4; ++q)
           if (ctx->bcc[q][jc] != 3)
@@ -436,6 +439,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   double *d[16];
   UP(d_nbr, nbr);
   UP(d_vtx, vtx);
+  {
+    signed char *d_es;
+    UP(d_es, eside);
+    g.eside = d_es;
+  }
   const std::vector<double> *edge[9] = {&ctx->dlVV, &ctx->dlCC, &ctx->dle1, &ctx->dle2, &ctx->doe,
                                         &ctx->xnn, &ctx->ynn, &ctx->dxCC, &ctx->dyCC};
   for (int q = 0; q < 9; ++q) UP(d[q], edge_major(*edge[q], N0));
@@ -587,6 +595,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   }
   for (int q = 0; q < 7; ++q)
     if (!*rows[q]) return nsx_fail(ctx, -1, "reduction buffer %d is not allocated", q);
+  if ((rc = dev_alloc(ctx, &ctx->cfl_part, (size_t)glob_cfl_rows() * MAXGRID))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->cfl_out, 16))) return rc;
 
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
@@ -1235,6 +1245,34 @@ extern "C" int nsx_newu3d(nsx_ctx *ctx, int p, int pv) {
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   LAUNCH(launch_newu(ctx->g, ctx->stream, ctx->slot[p], ctx->slot[pv], facew));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// -------------------------------------------------------------- glob_cfl --
+extern "C" int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out) {
+  NEED_COMMIT(ctx);
+  if (!out) return nsx_fail(ctx, -1, "nsx_glob_cfl: out is NULL");
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL)) return -1;
+  const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                           ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  LAUNCH(launch_glob_cfl(ctx->g, ctx->stream, ctx->slot[u], ctx->slot[v], ctx->slot[w], face, ctx->cfl_part,
+                         ctx->cfl_out));
+  double r[13];
+  HIPCHECK(hipMemcpyAsync(r, ctx->cfl_out, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  out->umax = r[0];
+  out->vmax = r[1];
+  out->wmax = r[2];
+  out->cfl = r[3];
+  out->aveu = r[4];
+  out->avev = r[5];
+  out->avew = r[6];
+  out->flux_total = r[7];
+  out->flux_max = r[8];
+  out->flux_bc = r[9];
+  out->flux_bot = r[10];
+  out->flux_top = r[11];
+  out->flux_side = r[12];
   return 0;
 }
 

@@ -64,6 +64,7 @@ struct Geo {
   const int *vtx;           // No_vp(i,j), 0-based
   const double *dlVV, *dlCC, *dle1, *dle2, *doe, *xnn, *ynn, *dxCC, *dyCC;
   const double *ppec;       // (dle1/dlCC)*dlVV: PPE edge coefficient before the dsigv factor
+  const signed char *eside; // 1 when No_cp(i,j) is not an interior cell (a side face of glob_cfl)
   // per cell (N0)
   const double *area, *sum_xc2, *sum_yc2, *sum_xcyc;
   // per level

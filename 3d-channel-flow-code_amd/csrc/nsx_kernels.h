@@ -66,6 +66,10 @@ int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double
 int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 1 u, 2 v, 3 w*/);
 int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0);
 int launch_copy(hipStream_t s, size_t n, const double *src, double *dst);
+// glob_cfl (nsx_diag.hip): partial rows per block, and the 13 outputs into out
+int glob_cfl_rows();
+int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w,
+                    const double *const face[5], double *part, double *out);
 
 // ---- solver
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,

@@ -29,6 +29,11 @@
       integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
            NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2
 
+      type, bind(C) :: nsx_cfl_info
+        real(c_double) :: umax, vmax, wmax, cfl, aveu, avev, avew
+        real(c_double) :: flux_total, flux_max, flux_bc, flux_bot, flux_top, flux_side
+      end type nsx_cfl_info
+
       type, bind(C) :: nsx_solve_info
         integer(c_int) :: iters, exit_code
         real(c_double) :: e1norm, e2norm
@@ -227,6 +232,13 @@
           import :: c_int, c_ptr
           type(c_ptr), value :: ctx
           integer(c_int), value :: p, pv
+        end function
+        integer(c_int) function nsx_glob_cfl(ctx, u, v, w, info)         &
+             bind(C, name="nsx_glob_cfl")
+          import :: c_int, c_ptr, nsx_cfl_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: u, v, w
+          type(nsx_cfl_info) :: info
         end function
         integer(c_int) function nsx_set_precision(ctx, precision, refine,&
              tol, max_passes) bind(C, name="nsx_set_precision")

@@ -34,7 +34,7 @@ EXPORTS = [
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
-    "nsx_newu3d", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_newu3d", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
 ]
 
 
@@ -47,6 +47,14 @@ class SolveInfo(ctypes.Structure):
         return dict(iters=self.iters, exit=SOLVE_EXIT.get(self.exit_code, str(self.exit_code)), e1=self.e1norm,
                     e2=self.e2norm, n_solves=self.n_solves, iters_total=self.iters_total,
                     outer_passes=self.outer_passes, iters_fp32=self.iters_fp32)
+
+
+class CflInfo(ctypes.Structure):
+    _fields_ = [(n, c_dbl) for n in ("umax", "vmax", "wmax", "cfl", "aveu", "avev", "avew", "flux_total",
+                                      "flux_max", "flux_bc", "flux_bot", "flux_top", "flux_side")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 def declare(lib: ctypes.CDLL) -> None:
@@ -82,6 +90,7 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_bicgstab": [c_vp, c_int, c_int, c_int, P(SolveInfo)],
         "nsx_newu3d": [c_vp, c_int, c_int],
         "nsx_set_precision": [c_vp, c_int, c_int, c_dbl, c_int],
+        "nsx_glob_cfl": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
         "nsx_timing_reset": [c_vp],

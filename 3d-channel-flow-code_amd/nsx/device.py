@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .abi import CELL_SCRATCH, F, FACE_SCRATCH, PREC_FP64, PREC_MIXED, VERT_SCRATCH, SolveInfo  # noqa: F401
+from .abi import CELL_SCRATCH, F, FACE_SCRATCH, PREC_FP64, PREC_MIXED, VERT_SCRATCH, CflInfo, SolveInfo  # noqa: F401
 
 
 class NsxError(RuntimeError):
@@ -220,6 +220,17 @@ class Context:
         return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
 
     # ------------------------------------------------------------- timing ---
+    def glob_cfl(self, u=None, v=None, w=None) -> dict:
+        """glob_cfl (cfl.F90:7-269) on device: resident UFN, VFN, WFN by
+        default, or the given (N_CELL,NZ) arrays, with the resident faces."""
+        if u is None:
+            ids = (F["UFN"], F["VFN"], F["WFN"])
+        else:
+            ids = self._cells(u, v, w)
+        out = CflInfo()
+        self._check(self.lib.nsx_glob_cfl(self.h, *ids, ctypes.byref(out)))
+        return out.as_dict()
+
     def set_max_iters(self, n: int) -> None:
         """MaxIters of the pressure solve (the case's other parameters kept)."""
         p = self.case.params

@@ -18,9 +18,9 @@ step with HIP events around every solver kernel for the per-kernel roofline.
 Prints one JSON line:
   value        prism-cell-steps/s of the mixed leg = interior prisms x K / wall
   fp64         the same for the FP64 leg; mixed_over_fp64 their ratio
-  rel_err      relative L2 error of u, v, w, p (interior) and max |cell
-               divergence| (glob_cfl's definition) of the mixed leg against the
-               FP64 leg after the same steps
+  rel_err      relative L2 error of u, v, w, p (interior) of the mixed leg
+               against the FP64 leg after the same steps, and both legs' max
+               |cell divergence| and CFL number (glob_cfl on the device)
   roofline     the dominant kernel of the mixed leg: algorithmic bytes per
                launch / mean launch time (HIP events on the library's stream)
   cpu_baseline the oracle (CPU restatement of the reference, 1 core) on a
@@ -85,20 +85,6 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
             "s_per_step": step_s}
 
 
-def max_divergence(case, faces) -> float:
-    """max over interior cells of |sum_j U_j dlVV_j dsig(k-1) + (UT + UB) area|,
-    the divergence check of glob_cfl (cfl.F90:131-167)."""
-    n0, nz = case.N_CELL0, case.NZ
-    dlVV = case.arrays["dlVV"][:n0]
-    dsig = case.dsig
-    area = case.areaCell[:n0, None]
-    U1, U2, U3, UT, UB = (np.asarray(f)[:n0, 1:nz - 1] for f in faces)
-    ds = dsig[None, 0:nz - 2]
-    flux = U1 * dlVV[:, 0:1] * ds + U2 * dlVV[:, 1:2] * ds + U3 * dlVV[:, 2:3] * ds
-    flux = flux + UT * area + UB * area
-    return float(np.abs(flux).max())
-
-
 MIXED_MODES = {"mixed": (2, 1e-5), "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
 
 
@@ -138,8 +124,10 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
                                         "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")
     fields = {f: np.asarray(ctx.get(f))[:case.N_CELL0, 1:-1].copy() for f in ("UFN", "VFN", "WFN", "PFN")}
-    faces = [ctx.get(f) for f in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")]
-    fields["div"] = max_divergence(case, faces)
+    # glob_cfl on the device: the largest net cell flux (cfl.F90:131-167) and the CFL number
+    cfl = ctx.glob_cfl()
+    fields["div"] = abs(cfl["flux_max"])
+    fields["cfl"] = cfl["cfl"]
     ctx.close()
     return {"value": n * steps / wall, "ms_per_step": wall / steps * 1e3, "wall": wall,
             "mean_iters_per_solve": iters / max(solves, 1), "solves": solves, "exit": info["exit"],
@@ -235,6 +223,8 @@ def main() -> int:
             err[nm] = float(np.linalg.norm(fb[f] - fa[f]) / max(np.linalg.norm(fa[f]), 1e-300))
         err["max_div_fp64"] = fa["div"]
         err["max_div_mixed"] = fb["div"]
+        err["cfl_fp64"] = fa["cfl"]
+        err["cfl_mixed"] = fb["cfl"]
         out["rel_err"] = err
     if not args.no_cpu:
         base = res.get("fp64", head)

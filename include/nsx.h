@@ -164,6 +164,21 @@ int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info)
 /* newu3D (newu3D.F90:8-257): U1FACE..UBFACE -= dt grad_n p */
 int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
 
+/* ---- diagnostics ------------------------------------------------------------ */
+/* glob_cfl (cfl.F90:7-269, sequential branch) on the cell fields u, v, w and
+ * the resident face velocities U1FACE..UBFACE; the values it prints and
+ * writes to meanuv.dat, returned instead */
+typedef struct nsx_cfl_info {
+  double umax, vmax, wmax; /* signed value of largest magnitude (first in k-then-i order) */
+  double cfl;              /* max over cells of 0.5 dt sum|face flux| / (area dsig(k)) */
+  double aveu, avev, avew; /* volume means, sum phi area dsig(k) / (TotalVolume (sigFin-sigIni)) */
+  double flux_total;       /* sum over cells of the net face flux (lateral faces with dsig(k-1)) */
+  double flux_max;         /* signed net cell flux of largest magnitude */
+  double flux_bc;          /* flux_bot + flux_top + flux_side */
+  double flux_bot, flux_top, flux_side; /* through the bottom/top walls and the side (non-interior) faces */
+} nsx_cfl_info;
+int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
+
 /* ---- precision form (no counterpart in the reference) ------------------------ */
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
  * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs

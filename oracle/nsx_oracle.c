@@ -680,6 +680,62 @@ void orc_newu3d(orc *o, double *fun, double *funv) {
 /* ------------------------------------------------------------- stage ----- */
 /* hydro, hydro.F90:8-501: one projection stage from (uf,vf,wf,pf) into
  * (ufnp,vfnp,wfnp,pfnp), vertex values, faces, rhs history and eddy. */
+/* ------------------------------------------------------------ glob_cfl --- */
+/* cfl.F90:84-200 (sequential branch) on velocities u, v, w and the module
+ * face velocities U1..UB.  out[13]: umax, vmax, wmax (the signed value of
+ * largest magnitude, first in k-then-i order), cfl_max, aveu, avev, avew,
+ * flux_total, flux_max (signed, largest magnitude), flux_bc, flux_bot,
+ * flux_top, flux_side.  A face counts as a side face when its neighbour
+ * No_cp(i,j) is not an interior cell (cfl.F90:150-159). */
+void orc_glob_cfl(orc *o, const double *u, const double *v, const double *w, double *out) {
+  double umax = 0., vmax = 0., wmax = 0., sumu = 0., sumv = 0., sumw = 0.;
+  for (int k = 1; k < o->NZ - 1; ++k)
+    for (int i = 0; i < o->N0; ++i) {
+      const size_t s = CI(i, k);
+      if (fabs(u[s]) > fabs(umax)) umax = u[s];
+      if (fabs(v[s]) > fabs(vmax)) vmax = v[s];
+      if (fabs(w[s]) > fabs(wmax)) wmax = w[s];
+      sumu = u[s] * o->areaCell[i] * o->dsig[k] + sumu;
+      sumv = v[s] * o->areaCell[i] * o->dsig[k] + sumv;
+      sumw = w[s] * o->areaCell[i] * o->dsig[k] + sumw;
+    }
+  double cfl = 0., flux_l = 0., flux_m = 0., flux_i = 0., flux_o = 0., flux_w = 0.;
+  for (int k = 1; k < o->NZ - 1; ++k)
+    for (int i = 0; i < o->N0; ++i) {
+      double flux = 0., flux_cfl = 0.;
+      const double uface[3] = {o->U1[C0(i, k)], o->U2[C0(i, k)], o->U3[C0(i, k)]};
+      for (int j = 0; j < 3; ++j) {
+        flux = flux + uface[j] * G3(o->dlVV, i, j) * o->dsig[k - 1];
+        flux_cfl = flux_cfl + fabs(uface[j] * G3(o->dlVV, i, j) * o->dsig[k]);
+        const int jc = CP(i, j);
+        if (jc < 0 || jc >= o->N0) flux_w = flux_w + uface[j] * G3(o->dlVV, i, j) * o->dsig[k];
+      }
+      flux = flux + o->UT[C0(i, k)] * o->areaCell[i];
+      flux = flux + o->UB[C0(i, k)] * o->areaCell[i];
+      flux_cfl = flux_cfl + fabs(o->UB[C0(i, k)] * o->areaCell[i]) + fabs(o->UT[C0(i, k)] * o->areaCell[i]);
+      flux_cfl = 0.5 * o->dt * flux_cfl / (o->areaCell[i] * o->dsig[k]);
+      if (flux_cfl > cfl) cfl = flux_cfl;
+      flux_l = flux_l + flux;
+      if (fabs(flux_m) < fabs(flux)) flux_m = flux;
+      if (k == 1 && o->bcbot == 1) flux_i = flux_i + o->UB[C0(i, k)] * o->areaCell[i];
+      if (k == o->NZ - 2 && o->bctop == 1) flux_o = flux_o + o->UT[C0(i, k)] * o->areaCell[i];
+    }
+  const double Vol = o->TotalVolume * (o->sigFin - o->sigIni);
+  out[0] = umax;
+  out[1] = vmax;
+  out[2] = wmax;
+  out[3] = cfl;
+  out[4] = sumu / Vol;
+  out[5] = sumv / Vol;
+  out[6] = sumw / Vol;
+  out[7] = flux_l;
+  out[8] = flux_m;
+  out[9] = flux_i + flux_o + flux_w;
+  out[10] = flux_i;
+  out[11] = flux_o;
+  out[12] = flux_w;
+}
+
 int orc_hydro(orc *o) {
   const int N0 = o->N0, NZ = o->NZ;
   const double t0 = now();

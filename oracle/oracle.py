@@ -55,6 +55,7 @@ def lib() -> ctypes.CDLL:
                         ("orc_bcpcenter", 1)]:
             f = getattr(L, name)
             f.argtypes = [vp] + [vp] * n + ([ci] if name in ("orc_bcglobalvc", "orc_bcglobalp") else [])
+        L.orc_glob_cfl.argtypes = [vp, vp, vp, vp, vp]
         L.orc_bicgstab.restype = ci
         L.orc_hydro.restype = ci
         L.orc_step.restype = ci
@@ -145,6 +146,19 @@ class Oracle:
         if tagdir is not None:
             args.append(tagdir)
         return getattr(lib(), "orc_" + name)(*args)
+
+    CFL_KEYS = ("umax", "vmax", "wmax", "cfl", "aveu", "avev", "avew", "flux_total", "flux_max", "flux_bc",
+                "flux_bot", "flux_top", "flux_side")
+
+    def glob_cfl(self, u=None, v=None, w=None) -> dict:
+        """cfl.F90:84-200 on (u, v, w) (default: the state at time level n)
+        and the resident face velocities."""
+        u = self.state("ufn") if u is None else np.asfortranarray(u)
+        v = self.state("vfn") if v is None else np.asfortranarray(v)
+        w = self.state("wfn") if w is None else np.asfortranarray(w)
+        out = np.zeros(13)
+        lib().orc_glob_cfl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(out))
+        return dict(zip(self.CFL_KEYS, out.tolist()))
 
     def cell(self):
         return self.case.cell_field()

@@ -11,7 +11,9 @@
 !
 !  usage: nsx_ref_driver <case.bin> <out.bin>
 !  mode 1 dumps set-up geometry and single-routine results on the initial
-!  field; mode 2 runs nsteps RK steps and dumps the state after each step.
+!  field; mode 2 runs nsteps RK steps, dumps the state after each step and
+!  calls glob_cfl on the final state (its printed diagnostics and meanuv.dat
+!  are the record of that call).
 
       module nsxdump
       implicit none
@@ -543,6 +545,9 @@
             call dump_state(trim(sfx))
          endif
       enddo
+!     the CFL / mass-flux diagnostics of nsmp3D.F90:595-606 on the final state
+      print*, 'NSX_CFL'
+      call glob_cfl(ufn,vfn,wfn,pfn,xc,yc,sig,dsig,No_cp,nbe)
       end subroutine
 
       end program

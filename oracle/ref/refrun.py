@@ -112,4 +112,30 @@ def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workd
         r = subprocess.run([exe, cf, of], cwd=td, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference driver failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
-        return read_dump(of, case), r.stdout
+        out = r.stdout
+        mu = os.path.join(td, "meanuv.dat")
+        if os.path.exists(mu):
+            out += "".join("MEANUV " + ln for ln in open(mu))
+        return read_dump(of, case), out
+
+
+CFL_KEYS = {"MAX U": "umax", "MAX V": "vmax", "MAX W": "wmax", "MAX CFL": "cfl", "Total  flux": "flux_total",
+            "Maxim  flux": "flux_max", "BC     flux": "flux_bc", "BOT    flux": "flux_bot", "TOP    flux": "flux_top",
+            "Side   flux": "flux_side"}
+
+
+def parse_cfl(stdout: str) -> dict:
+    """The values glob_cfl printed after the NSX_CFL marker (and the means
+    it appended to meanuv.dat, f12.5)"""
+    res = {}
+    tail = stdout.split("NSX_CFL", 1)[1] if "NSX_CFL" in stdout else ""
+    for line in tail.splitlines():
+        line = line.strip()
+        if line.startswith("MEANUV "):
+            t, u, v, w = (float(x) for x in line.split()[1:5])
+            res.update(aveu=u, avev=v, avew=w)
+            continue
+        for k, name in CFL_KEYS.items():
+            if line.startswith(k) and "=" in line:
+                res[name] = float(line.split("=", 1)[1])
+    return res

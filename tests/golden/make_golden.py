@@ -8,8 +8,9 @@ what they produce:
                   PPE matrix, and every hot-path routine applied once to the
                   initial field (full arrays, ghost rows included);
   cfg0_steps.json workload 0 (32x16 quads x 32 layers, 10 RK2 steps):
-                  per-step SHA-256 of the state arrays, norms, sample values
-                  and the BiCGSTAB iteration count of every pressure solve;
+                  per-step SHA-256 of the state arrays, norms, sample values,
+                  the BiCGSTAB iteration count of every pressure solve and
+                  glob_cfl's diagnostics of the final state;
   cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
@@ -87,6 +88,9 @@ def main():
                    "s1_eddy"):
             stage[nm] = {"sha256": digest(c, d[nm]), "l2": float(np.linalg.norm(d[nm][:rows_of(c, d[nm])]))}
         rec["stage1"] = stage
+        # glob_cfl on the final state: its printed values (17 digits) and the
+        # means it appends to meanuv.dat (f12.5)
+        rec["glob_cfl"] = refrun.parse_cfl(log)
         with open(os.path.join(HERE, f"cfg{cfg}_steps.json"), "w") as f:
             json.dump(rec, f, indent=1)
         print(f"cfg{cfg}_steps.json:", len(rec["solves"]), "solves")

@@ -243,3 +243,29 @@ def test_full_size_step_properties():
     u = interior(c, G.get("UFN"))
     assert np.isfinite(u).all() and np.abs(u).max() < 5 * c.arrays["u0"].max()
     G.close()
+
+
+def test_glob_cfl_vs_oracle():
+    """glob_cfl on the device against the oracle on the same fields and faces
+    (the oracle's state after three steps of workload 0): the selected
+    extremes and the CFL number are per-cell values and must be equal; the
+    sums differ in order only."""
+    c = config_case(0)
+    O = oracle.Oracle(c)
+    for _ in range(3):
+        O.step()
+    ref = O.glob_cfl()
+    G = _ctx(c)
+    G.init_state()
+    for dev, orc in (("UFN", "ufn"), ("VFN", "vfn"), ("WFN", "wfn"), ("U1FACE", "U1"), ("U2FACE", "U2"),
+                     ("U3FACE", "U3"), ("UTFACE", "UT"), ("UBFACE", "UB")):
+        G.put(dev, O.state(orc))
+    got = G.glob_cfl()
+    G.close()
+    for k in ("umax", "vmax", "wmax", "cfl", "flux_max"):
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    for k in ("aveu", "avev", "avew"):
+        assert abs(got[k] - ref[k]) <= 1e-13 * max(abs(ref["aveu"]), 1.0), (k, got[k], ref[k])
+    scale = abs(ref["flux_max"]) * c.n_interior
+    for k in ("flux_total", "flux_bc", "flux_bot", "flux_top", "flux_side"):
+        assert abs(got[k] - ref[k]) <= 1e-13 * scale, (k, got[k], ref[k])

@@ -197,3 +197,53 @@ def test_ppe_rows_sum_to_zero_and_manufactured_solve():
     vol = c.areaCell[:c.N_CELL0, None] * c.dsig[None, 1:-1]
     xm = x[:c.N_CELL0, 1:-1] - (x[:c.N_CELL0, 1:-1] * vol).sum() / (c.TotalVolume * 2.0)
     np.testing.assert_allclose(phi[:c.N_CELL0, 1:-1], xm, atol=1e-7 * np.abs(xm).max())
+
+
+@pytest.mark.parametrize("cfg", [0, 3])
+def test_glob_cfl_vs_reference(cfg):
+    """glob_cfl (cfl.F90:84-200) on the final state of the recorded steps: the
+    values the compiled reference printed (17 significant digits, so equal as
+    doubles) and the means it wrote to meanuv.dat (f12.5)."""
+    rec = load_steps(cfg)
+    c = config_case(cfg)
+    O = oracle.Oracle(c)
+    for _ in range(rec["nsteps"]):
+        O.step()
+    got = O.glob_cfl()
+    ref = rec["glob_cfl"]
+    for k in ("umax", "vmax", "wmax", "cfl", "flux_total", "flux_max", "flux_bc", "flux_bot", "flux_top",
+              "flux_side"):
+        assert got[k] == ref[k], (k, got[k], ref[k])
+    for k in ("aveu", "avev", "avew"):
+        assert abs(got[k] - ref[k]) <= 0.5e-5 + 1e-12, (k, got[k], ref[k])
+
+
+def test_glob_cfl_closed_form():
+    """Uniform velocity u = 1, v = w = 0 with face velocities of a uniform
+    horizontal flow: the means are (1, 0, 0), every cell's net flux is the
+    sum of its lateral face fluxes and the CFL number follows from the face
+    sums, computed here independently of the oracle."""
+    c = make_case(8, 6, 6, seed=3, jitter=0.1)
+    O = oracle.Oracle(c)
+    n0, nz = c.N_CELL0, c.NZ
+    u = c.cell_field() + 1.0
+    zero = c.cell_field()
+    xnn = c.arrays["xnn"][:n0]
+    dl = c.arrays["dlVV"][:n0]
+    for j, name in enumerate(("U1", "U2", "U3")):
+        O.state(name)[:, :] = xnn[:, j:j + 1]            # face-normal component of (1, 0)
+    O.state("UT")[:, :] = 0.0
+    O.state("UB")[:, :] = 0.0
+    got = O.glob_cfl(u, zero, zero)
+    assert got["umax"] == 1.0 and got["vmax"] == 0.0 and got["wmax"] == 0.0
+    assert abs(got["aveu"] - 1.0) < 1e-13 and got["avev"] == 0.0
+    # closed polygon: sum_j n_j l_j = 0, so every cell's flux vanishes to round-off
+    assert abs(got["flux_max"]) < 1e-13 and abs(got["flux_total"]) < 1e-11
+    area = c.areaCell[:n0]
+    dsig = c.dsig
+    cfl = max(0.5 * c.params.dt * np.abs(xnn * dl * dsig[k]).sum(axis=1).max() / (area * dsig[k]).min()
+              for k in range(1, nz - 1))
+    assert got["cfl"] <= cfl * (1 + 1e-12)
+    per_cell = [0.5 * c.params.dt * (np.abs(xnn * dl).sum(axis=1) * dsig[k]) / (area * dsig[k])
+                for k in range(1, nz - 1)]
+    assert abs(got["cfl"] - max(p.max() for p in per_cell)) <= 1e-14 * got["cfl"]
