@@ -2,21 +2,20 @@
 // stencil (bicgstab.F90:303-591), in FP64 (the reference's arithmetic) and
 // as the FP32 inner solver of an FP64 iterative refinement.
 //
-// One iteration is five kernels, split at the algorithm's three global
-// synchronisation points and at the two stencil applications:
+// One iteration is three kernels, one per global synchronisation point of
+// the algorithm (see "iteration" below for the scaled vectors they carry):
 //
-//   A1  stop test of k-1, p = r + beta (p - omega v), y = p / A0   update_pk :726-773, diag_precdt :831-836
-//   A2  v = A y, partial <rhat, v>                                 get_Apk :931-1001
-//   B1  alpha = rho / <rhat,v>, s = r - alpha v, z = s / A0        update_sk :1007-1041
-//   B2  t = A z, partial <t,s>, <t,t>
-//   C   omega = <t,s>/<t,t>, x += alpha y + omega z, r = s - omega t,
-//       partial |r|^2, |r|_1, <rhat, r>, sum x Vol                 update_xkrk :1047-1111
+//   A  stop test of k-1, beta; y = rt + beta q; v = A y; vt = v/A0;
+//      partial <rhat, v>              update_pk, diag_precdt, get_Apk  :726-1001
+//   B  alpha; z = rt - alpha vt; t = A z; s = A0 z; partial <t,s>, <t,t>
+//                                     update_sk, diag_precdt, get_Apk  :1007-1041
+//   C  omega; r = s - omega t; x += alpha y + omega z; rt = r/A0;
+//      q = y - omega vt; partial |r|^2, |r|_1, <rhat, r>, sum x Vol
+//                                     update_xkrk                      :1047-1111
 //
-// Only y and z are gathered at the 7 points, so each stencil reads one
-// neighbour array.  The matrix is PDE_center's, regenerated per row from the
-// per-edge coefficient ppec, the cell area and the sigma spacings (bit-
-// identical to AmC0..AmCB, checked at set-up), so no coefficient array is
-// read; y = p / A0 and z = s / A0 divide exactly as diag_precdt does.
+// The matrix is PDE_center's, regenerated per row from the per-edge
+// coefficient ppec, the cell area and the sigma spacings (bit-identical to
+// AmC0..AmCB, checked at set-up), so no coefficient array is read.
 //
 // Work decomposition: a thread owns four consecutive layers k0..k0+3 of one
 // column (k0 a multiple of 4, the column stride a multiple of 4), so every
@@ -25,22 +24,14 @@
 // and the five divisions area/dsig(k0-1..k0+3) give AB and AT of all four
 // rows.  The sigma spacings of the block live in LDS.  Slots outside the
 // interior layers (ghost layers, padding) are written as zero in the work
-// vectors and left unchanged in x.  Cell-wise arithmetic keeps the
-// reference's operation order, so in FP64 every element is what the
-// reference computes from the same scalars.
-//
-// Algorithmic HBM bytes per interior prism and iteration (b = 8 FP64, 4 FP32):
-//   A1 read r, p, v; write p, y        5b     A2 read y, rhat; write v    3b
-//   B1 read r, v; write s, z           4b     B2 read z, s; write t       3b
-//   C  read x, y, z, s, t, rhat; write x, r                              8b
-//   total 23b (neighbour re-reads of y, z are cache hits).
+// vectors and left unchanged in x.
 //
 // The scalars of a phase come from the partial sums of the phase before it:
-// B1 sums A2's partials and forms alpha itself, C forms omega from B2's, and
-// A1 of iteration k+1 forms the norms, the stopping test and rho_{k+1} from
-// C's (see reduce_partials).  Block 0 records the scalars in SolverState for
-// the host and for later phases.  Each kernel exits at once when st->done is
-// set, so the host can queue many iterations ahead and poll rarely.  Inner
+// B sums A's partials and forms alpha itself, C forms omega from B's, and A
+// of iteration k+1 forms the norms, the stopping test and rho_{k+1} from C's
+// (see reduce_partials).  Block 0 records the scalars in SolverState for the
+// host and for later phases.  Each kernel exits at once when st->done is set,
+// so the host can queue many iterations ahead and poll rarely.  Inner
 // products and norms are accumulated in FP64 in both precisions.
 #include <hip/hip_ext.h>
 
@@ -171,31 +162,6 @@ __device__ inline void rows4(const Geo &g, const T *lv, int i, int k0, Rows4<T> 
   }
 }
 
-// out = A y on layers k0..k0+3 of column i (get_Apk, bicgstab.F90:976-998):
-// the vertical ghost of y is the Neumann copy of BCpcenter, the periodic
-// ghosts are the partner columns
-template <typename T>
-__device__ inline void spmv4(const Geo &g, const Rows4<T> &R, int i, int k0, size_t s0, const T *y, T (&out)[4]) {
-  T yc[4], y0[4], y1[4], y2[4];
-  ld4(y + s0, yc);
-  const T ym = k0 >= 1 ? y[s0 - 1] : T(0);
-  const T yp = (k0 + 4 <= g.NZ - 2) ? y[s0 + 4] : T(0);
-  const int e = i * 3;
-  ld4(y + (size_t)g.nbr[e] * g.NS + k0, y0);
-  ld4(y + (size_t)g.nbr[e + 1] * g.NS + k0, y1);
-  ld4(y + (size_t)g.nbr[e + 2] * g.NS + k0, y2);
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int k = k0 + u;
-    const T ys = yc[u];
-    const T yu = (k + 1 <= g.NZ - 2) ? (u < 3 ? yc[u < 3 ? u + 1 : 3] : yp) : ys;
-    const T yd = (k - 1 >= 1) ? (u > 0 ? yc[u > 0 ? u - 1 : 0] : ym) : ys;
-    out[u] = R.in[u] ? R.A0[u] * ys + R.a1[u] * y0[u] + R.a2[u] * y1[u] + R.a3[u] * y2[u] + R.AT[u] * yu +
-                           R.AB[u] * yd
-                     : T(0);
-  }
-}
-
 template <class F>
 __device__ inline void for_each_quad(const Geo &g, F &&f) {
   uint32_t beg, end;
@@ -214,10 +180,11 @@ __device__ inline void for_each_quad(const Geo &g, F &&f) {
 // bicgstab.F90:420-451 after Glob_Norm(rhs,1) (bicgstab.F90:635-663): the
 // rhs average comes from the partial sums of k_source / k_gnorm_sum; r = b -
 // A x0 in FP64 with the partial sums |b|^2, |r|^2, |b|_1, |r|_1, sum x0 Vol
-// into `outer`.  FP64 solve (TO = double): r, rhat = r, x = x0.  Refinement
-// pass of the mixed solve (TO = float): r32 = rhat32 = (float) r, the inner
-// solution e = 0, and the inner norms |r32|^2, |r32|_1 into `inner` laid out
-// as the initial partials of an FP32 solve with zero start.
+// into `outer`.  FP64 solve (TO = double): rt = r/A0, rhat = r, x = x0.
+// Refinement pass of the mixed solve (TO = float): rt32 = (float)(r/A0),
+// rhat32 = (float) r, the inner solution e = 0, and the inner norms |r32|^2,
+// |r32|_1 into `inner` laid out as the initial partials of an FP32 solve with
+// zero start.
 template <typename TO>
 __global__ __launch_bounds__(NSX_BLOCK) void k_resid(Geo g, const double *rhs, const double *phi, double *x,
                                                       TO *r, TO *rhat, TO *e, Partials P, double *outer,
@@ -242,7 +209,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_resid(Geo g, const double *rhs, c
                       A.AT * phi[s + 1] + A.AB * phi[s - 1];
     const double res = b - Ax;
     const TO ro = (TO)res;
-    r[s] = ro;
+    r[s] = (TO)(res / A.A0);  // the solver keeps r / A0
     rhat[s] = ro;
     if (e) e[s] = TO(0);
     if (copy_x) x[s] = ph;
@@ -340,7 +307,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rel_u(Geo g, const double *rhs, c
                       A.AB * (x[s - 1] + (double)e[s - 1]);
     const double res = b - Ax;
     const float ro = (float)res;
-    r[s] = ro;
+    r[s] = (float)(res / A.A0);  // the solver keeps r / A0
     acc[0] += res * res;
     acc[1] += fabs(res);
     acc[2] += (double)rhat[s] * (double)ro;
@@ -349,15 +316,27 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rel_u(Geo g, const double *rhs, c
   write_partials<4>(acc, P.u);
 }
 
-// ------------------------------------------------------------ phase A1 ----
+// ----------------------------------------------------------- iteration ----
+// The iteration carries the preconditioned (scaled) vectors: rt = r/A0,
+// q = (p - omega v)/A0, vt = v/A0, so that the two vectors the stencils
+// need, y = p/A0 = rt + beta q and z = s/A0 = rt - alpha vt, are formed at
+// the seven points from stored vectors with one multiply-add, without the
+// pointwise passes that would otherwise write them.  The unscaled s = A0 z
+// and r = s - omega t are formed where needed and never stored.  In exact
+// arithmetic these are the reference's vectors (update_pk, diag_precdt,
+// update_sk, update_xkrk, bicgstab.F90:726-1111); the rounding differs at
+// the last bit per operation.
+//
+// Per interior prism (b = vector width): A reads rt, q, rhat, writes y, vt
+// (5b); B reads rt, vt, writes t (3b); C reads x, y, rt, vt, t, rhat, writes
+// x, rt, q (9b): 17b, against 23b for five passes that store y, z, s.
+
 // Closes iteration k-1 (norms and the stopping test of bicgstab.F90:544-575,
-// the breakdown guards of :479-490; at k = 0 the norms of :453-469), then
-// update_pk (:726-773) p = r + beta (p - omega v) (p = r at k = 0) and
-// diag_precdt Opt 1 (:831-836) y = p / AmC0.  Point-wise, in place.
-template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a1(Geo g, const T *r, T *p, const T *v, T *y, Partials P,
-                                                        SolverState *st, double *xfold, const T *efold) {
-  if (st->done) return;
+// the breakdown guards of :479-490; at k = 0 the norms of :453-469) from C's
+// partial sums (P.init at k = 0, P.u after a reliable update).  Returns
+// false when the solve stops; sets beta (0 at k = 0).
+__device__ inline bool begin_iteration(const Geo &g, Partials P, SolverState *st, double *xfold, const void *efold_v,
+                                       bool efold_f32, double &beta_out, bool &first_out) {
   const int k_it = st->c_iter;
   const bool first = k_it == 0;
   const bool upd = xfold && !first && st->upd;
@@ -403,7 +382,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a1(Geo g, const T *r, T *p, 
         int i, k;
         cell_of(g, c, i, k);
         const size_t s = (size_t)i * g.NS + k;
-        xfold[s] = xfold[s] + (double)efold[s];
+        const double e = efold_f32 ? (double)((const float *)efold_v)[s] : ((const double *)efold_v)[s];
+        xfold[s] = xfold[s] + e;
       }
     }
   } else if (first) {
@@ -472,68 +452,144 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a1(Geo g, const T *r, T *p, 
       st->done = 1;
     }
   }
-  if (stop) return;
-  const T om = (T)st->c_omega;
-  const T beta = first ? T(0) : (T)((rho / st->a_rho[(k_it - 1) & 1]) * (st->b_alpha / st->c_omega));
-  const T *lv = load_levels<T>(g);
-  for_each_quad(g, [&](int i, int k0, size_t s0) {
-    Rows4<T> R;
-    rows4<T>(g, lv, i, k0, R);
-    T rr[4], pp[4], vv[4], yy[4];
-    ld4(r + s0, rr);
-    ld4(p + s0, pp);
-    ld4(v + s0, vv);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const T pn = first ? rr[u] : rr[u] + beta * (pp[u] - om * vv[u]);
-      pp[u] = R.in[u] ? pn : T(0);
-      yy[u] = R.in[u] ? pn / R.A0[u] : T(0);
-    }
-    st4(p + s0, pp);
-    st4(y + s0, yy);
-  });
+  first_out = first;
+  if (stop) return false;
+  beta_out = first ? 0.0 : (rho / st->a_rho[(k_it - 1) & 1]) * (st->b_alpha / st->c_omega);
+  return true;
 }
 
-// ------------------------------------------------------------ phase A2 ----
-// get_Apk (:931-1001) v = A y, and the partial sums of <rhat, v> (:509)
+// a + f b over the four layers at `off` (b not read when first)
 template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a2(Geo g, const T *y, const T *rhat, T *v, Partials P,
-                                                        const SolverState *st) {
+__device__ inline void axpy4(const T *a, const T *b, T f, bool first, size_t off, T (&o)[4]) {
+  ld4(a + off, o);
+  if (!first) {
+    T bb[4];
+    ld4(b + off, bb);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o[u] = o[u] + f * bb[u];
+  }
+}
+
+template <typename T>
+__device__ inline T axpy1(const T *a, const T *b, T f, bool first, size_t off) {
+  return first ? a[off] : a[off] + f * b[off];
+}
+
+// A f on layers k0..k0+3 of column i from the values of f at the seven
+// points (get_Apk, bicgstab.F90:976-998; Neumann vertical ghosts, periodic
+// partners through nbr); rows outside the interior give 0
+template <typename T>
+__device__ inline void stencil4(const Geo &g, const Rows4<T> &R, int k0, const T (&fc)[4], T fm, T fp,
+                                const T (&f0)[4], const T (&f1)[4], const T (&f2)[4], T (&out)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = k0 + u;
+    const T fs = fc[u];
+    const T fu = (k + 1 <= g.NZ - 2) ? (u < 3 ? fc[u < 3 ? u + 1 : 3] : fp) : fs;
+    const T fd = (k - 1 >= 1) ? (u > 0 ? fc[u > 0 ? u - 1 : 0] : fm) : fs;
+    out[u] = R.in[u] ? R.A0[u] * fs + R.a1[u] * f0[u] + R.a2[u] * f1[u] + R.a3[u] * f2[u] + R.AT[u] * fu +
+                           R.AB[u] * fd
+                     : T(0);
+  }
+}
+
+// Seven-point values of f = a + c b around every quad of the block's range
+// (f = a when first).  Each thread forms f on its own four layers and
+// publishes them in LDS; the vertical neighbours and the neighbour columns
+// whose quad lies in the block's current window of NSX_BLOCK quads (the
+// in-quad partner and usually the x-neighbour) are read from there, the rest
+// (the y-neighbour, window edges) are formed from global memory with the same
+// arithmetic, so every value is bit-identical whichever way it was obtained.
+template <typename T, class F>
+__device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, F &&body) {
+  __shared__ __align__(16) T win[NSX_BLOCK][4];
+  uint32_t beg, end;
+  block_range(g.nquad, beg, end);
+  for (uint32_t base = beg; base < end; base += NSX_BLOCK) {
+    const uint32_t t = base + threadIdx.x;
+    const bool act = t < end;
+    const uint32_t lim = end - base < NSX_BLOCK ? end - base : NSX_BLOCK;
+    int i = 0, k0 = 0;
+    size_t s0 = 0;
+    T fc[4] = {T(0), T(0), T(0), T(0)};
+    if (act) {
+      i = (int)fdiv(t, g.divNQ);
+      k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
+      s0 = (size_t)i * g.NS + k0;
+      axpy4(a, b, c, first, s0, fc);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) win[threadIdx.x][u] = fc[u];
+    __syncthreads();
+    if (act) {
+      T fm = T(0), fp = T(0);
+      if (k0 >= 1) fm = threadIdx.x >= 1 ? win[threadIdx.x - 1][3] : axpy1(a, b, c, first, s0 - 1);
+      if (k0 + 4 <= g.NZ - 2) fp = threadIdx.x + 1 < lim ? win[threadIdx.x + 1][0] : axpy1(a, b, c, first, s0 + 4);
+      const int e = i * 3;
+      const uint32_t qq = (uint32_t)(k0 >> 2);
+      T fn[3][4];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int n = g.nbr[e + j];
+        const uint32_t d = (uint32_t)n * (uint32_t)g.NQ + qq - base;
+        if (d < lim) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) fn[j][u] = win[d][u];
+        } else {
+          axpy4(a, b, c, first, (size_t)n * g.NS + k0, fn[j]);
+        }
+      }
+      body(i, k0, s0, fc, fm, fp, fn[0], fn[1], fn[2]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ phase A ----
+// Stop test of k-1 and beta; y = p/A0 = rt + beta q (y = rt at k = 0),
+// v = A y (get_Apk), vt = v/A0, partial sums of <rhat, v> (:509).
+template <typename T>
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const T *q, const T *rhat, T *y, T *vt,
+                                                       Partials P, SolverState *st, double *xfold, const T *efold) {
   if (st->done) return;
+  double beta_d = 0.0;
+  bool first = false;
+  if (!begin_iteration(g, P, st, xfold, efold, sizeof(T) == 4, beta_d, first)) return;
+  const T beta = (T)beta_d;
   const T *lv = load_levels<T>(g);
   double acc[1] = {0.0};
-  for_each_quad(g, [&](int i, int k0, size_t s0) {
+  for_each_quad_gathered(g, rt, q, beta, first, [&](int i, int k0, size_t s0, const T(&yc)[4], T ym, T yp,
+                                                     const T(&y0)[4], const T(&y1)[4], const T(&y2)[4]) {
     Rows4<T> R;
     rows4<T>(g, lv, i, k0, R);
-    T vs[4], rh[4];
-    spmv4<T>(g, R, i, k0, s0, y, vs);
+    T v[4], rh[4];
+    stencil4(g, R, k0, yc, ym, yp, y0, y1, y2, v);
     ld4(rhat + s0, rh);
-    st4(v + s0, vs);
+    T yo[4], vo[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[0] += (double)rh[u] * (double)vs[u];
+    for (int u = 0; u < 4; ++u) {
+      yo[u] = R.in[u] ? yc[u] : T(0);
+      vo[u] = R.in[u] ? v[u] / R.A0[u] : T(0);
+      acc[0] += (double)rh[u] * (double)v[u];
+    }
+    st4(y + s0, yo);
+    st4(vt + s0, vo);
   });
   write_partials<1>(acc, P.a);
 }
 
-// ------------------------------------------------------------ phase B1 ----
-// alpha = rho / <rhat, v> (breakdown guard of :512-517), update_sk
-// (:1007-1041) s = r - alpha v, diag_precdt z = s / AmC0.  Point-wise.
+// ------------------------------------------------------------ phase B ----
+// alpha = rho/<rhat, v> (breakdown guard of :512-517); z = s/A0 = rt -
+// alpha vt, t = A z (get_Apk), s = A0 z, partial sums of <t,s>, <t,t>.
 template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b1(Geo g, const T *r, const T *v, T *sv, T *z, Partials P,
-                                                        SolverState *st) {
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const T *vt, T *tv, Partials P,
+                                                       SolverState *st) {
   if (st->done) return;
-  double t[1];
-  reduce_partials<1>(P.a, gridDim.x, t);
+  double tr[1];
+  reduce_partials<1>(P.a, gridDim.x, tr);
   const int k_it = st->a_iter;
-  if (fabs(t[0]) < TINY) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->exit_code = EXIT_BREAKDOWN;
-      st->done = 1;
-    }
-    return;
-  }
-  const double al = st->a_rho[k_it & 1] / t[0];
-  if (!isfinite(al)) {
+  const double al = st->a_rho[k_it & 1] / tr[0];
+  if (fabs(tr[0]) < TINY || !isfinite(al)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->exit_code = EXIT_BREAKDOWN;
       st->done = 1;
@@ -541,44 +597,20 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b1(Geo g, const T *r, const 
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->b_alpha = al;
-  const T alt = (T)al;
-  const T *lv = load_levels<T>(g);
-  for_each_quad(g, [&](int i, int k0, size_t s0) {
-    Rows4<T> R;
-    rows4<T>(g, lv, i, k0, R);
-    T rr[4], vv[4], ss[4], zz[4];
-    ld4(r + s0, rr);
-    ld4(v + s0, vv);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const T sn = rr[u] - alt * vv[u];
-      ss[u] = R.in[u] ? sn : T(0);
-      zz[u] = R.in[u] ? sn / R.A0[u] : T(0);
-    }
-    st4(sv + s0, ss);
-    st4(z + s0, zz);
-  });
-}
-
-// ------------------------------------------------------------ phase B2 ----
-// get_Apk t = A z and the partial sums of <t,s>, <t,t> (:535-536)
-template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b2(Geo g, const T *z, const T *sv, T *tv, Partials P,
-                                                        const SolverState *st) {
-  if (st->done) return;
+  const T alm = (T)(-al);
   const T *lv = load_levels<T>(g);
   double acc[2] = {0.0, 0.0};
-  for_each_quad(g, [&](int i, int k0, size_t s0) {
+  for_each_quad_gathered(g, rt, vt, alm, false, [&](int i, int k0, size_t s0, const T(&zc)[4], T zm, T zp,
+                                                    const T(&z0)[4], const T(&z1)[4], const T(&z2)[4]) {
     Rows4<T> R;
     rows4<T>(g, lv, i, k0, R);
-    T ts[4], ss[4];
-    spmv4<T>(g, R, i, k0, s0, z, ts);
-    ld4(sv + s0, ss);
-    st4(tv + s0, ts);
+    T t[4];
+    stencil4(g, R, k0, zc, zm, zp, z0, z1, z2, t);
+    st4(tv + s0, t);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double td = (double)ts[u];
-      acc[0] += td * (double)ss[u];
+      const double sd = R.in[u] ? (double)(R.A0[u] * zc[u]) : 0.0, td = (double)t[u];
+      acc[0] += td * sd;
       acc[1] += td * td;
     }
   });
@@ -586,15 +618,17 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b2(Geo g, const T *z, const 
 }
 
 // ------------------------------------------------------------- phase C ----
-// omega = <t,s>/<t,t>, update_xkrk (:1047-1111) x += alpha y + omega z,
-// r = s - omega t, partial |r|^2, |r|_1, <rhat, r>, sum x Vol
+// omega = <t,s>/<t,t>; z = rt - alpha vt, s = A0 z, r = s - omega t,
+// x += alpha y + omega z (update_xkrk :1047-1111), rt = r/A0, q = y - omega
+// vt; partial |r|^2, |r|_1, <rhat, r>, sum x Vol.  After a reliable update
+// the FP32 correction x starts again from zero (A has folded it).
 template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *r, const T *rhat, const T *y, const T *z,
-                                                       const T *sv, const T *tv, Partials P, SolverState *st) {
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, const T *rhat, const T *y,
+                                                       const T *vt, const T *tv, Partials P, SolverState *st) {
   if (st->done) return;
-  double t[2];
-  reduce_partials<2>(P.b, gridDim.x, t);
-  const double al = st->b_alpha, om = t[0] / t[1];
+  double tr[2];
+  reduce_partials<2>(P.b, gridDim.x, tr);
+  const double al = st->b_alpha, om = tr[0] / tr[1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->c_omega = om;
     st->c_iter = st->a_iter + 1;
@@ -602,31 +636,34 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *r, const T
   const T alt = (T)al, omt = (T)om;
   const T *lv = load_levels<T>(g);
   const T *area = Prec<T>::area(g);
-  // after a reliable update A1 has folded the correction into the FP64
-  // solution: the FP32 correction starts again from zero
   const bool fresh = st->upd != 0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for_each_quad(g, [&](int i, int k0, size_t s0) {
-    T xx[4], rr[4], rh[4], yy[4], zz[4], ss[4], tt[4];
+    Rows4<T> R;
+    rows4<T>(g, lv, i, k0, R);
+    T xx[4], rr[4], vv[4], yy[4], tt[4], rh[4], qq[4];
     if (fresh) {
       xx[0] = xx[1] = xx[2] = xx[3] = T(0);
     } else {
       ld4(x + s0, xx);
     }
-    ld4(rhat + s0, rh);
+    ld4(rt + s0, rr);
+    ld4(vt + s0, vv);
     ld4(y + s0, yy);
-    ld4(z + s0, zz);
-    ld4(sv + s0, ss);
     ld4(tv + s0, tt);
+    ld4(rhat + s0, rh);
     const T ar = area[i];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = k0 + u;
-      const bool in = k >= 1 && k <= g.NZ - 2;
-      const T xn = xx[u] + alt * yy[u] + omt * zz[u];
-      const T rn = ss[u] - omt * tt[u];
+      const bool in = R.in[u];
+      const T z = rr[u] - alt * vv[u];
+      const T sn = R.A0[u] * z;
+      const T rn = sn - omt * tt[u];
+      const T xn = xx[u] + alt * yy[u] + omt * z;
       xx[u] = in ? xn : xx[u];
-      rr[u] = in ? rn : T(0);
+      rr[u] = in ? rn / R.A0[u] : T(0);
+      qq[u] = in ? yy[u] - omt * vv[u] : T(0);
       if (in) {
         const double rd = (double)rn;
         acc[0] += rd * rd;
@@ -636,7 +673,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *r, const T
       }
     }
     st4(x + s0, xx);
-    st4(r + s0, rr);
+    st4(rt + s0, rr);
+    st4(q + s0, qq);
   });
   write_partials<4>(acc, P.c);
 }
@@ -726,29 +764,26 @@ int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const doubl
 }
 
 template <typename T>
-static int iter_impl(const Geo &g, hipStream_t s, T *x, T *r, const T *rhat, T *p, T *v, T *y, T *z, T *sv, T *tv,
-                     Partials part, SolverState *st, hipEvent_t *ev) {
+static int iter_impl(const Geo &g, hipStream_t s, T *x, T *rt, const T *rhat, T *q, T *y, T *vt, T *tv, Partials part,
+                     SolverState *st, hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(T));
-  LAUNCH_TIMED(ev, 0, k_bicg_a1<T>, grid, blk, lds, s, g, (const T *)r, p, (const T *)v, y, part, st,
+  LAUNCH_TIMED(ev, 0, k_bicg_a<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)q, rhat, y, vt, part, st,
                (double *)nullptr, (const T *)nullptr);
-  LAUNCH_TIMED(ev, 1, k_bicg_a2<T>, grid, blk, lds, s, g, (const T *)y, rhat, v, part, (const SolverState *)st);
-  LAUNCH_TIMED(ev, 2, k_bicg_b1<T>, grid, blk, lds, s, g, (const T *)r, (const T *)v, sv, z, part, st);
-  LAUNCH_TIMED(ev, 3, k_bicg_b2<T>, grid, blk, lds, s, g, (const T *)z, (const T *)sv, tv, part,
-               (const SolverState *)st);
-  LAUNCH_TIMED(ev, 4, k_bicg_c<T>, grid, blk, lds, s, g, x, r, rhat, (const T *)y, (const T *)z, (const T *)sv,
-               (const T *)tv, part, st);
+  LAUNCH_TIMED(ev, 1, k_bicg_b<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)vt, tv, part, st);
+  LAUNCH_TIMED(ev, 2, k_bicg_c<T>, grid, blk, lds, s, g, x, rt, q, rhat, (const T *)y, (const T *)vt, (const T *)tv,
+               part, st);
   return (int)hipGetLastError();
 }
 
-int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
-                     double *y, double *z, double *sv, double *tv, Partials part, SolverState *st, hipEvent_t *ev) {
-  return iter_impl<double>(g, s, x, r, rhat, p, v, y, z, sv, tv, part, st, ev);
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *y,
+                     double *vt, double *tv, Partials part, SolverState *st, hipEvent_t *ev) {
+  return iter_impl<double>(g, s, x, rt, rhat, q, y, vt, tv, part, st, ev);
 }
 
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *r, const float *rhat, float *p, float *v,
-                       float *y, float *z, float *sv, float *tv, Partials part, SolverState *st, hipEvent_t *ev) {
-  return iter_impl<float>(g, s, x, r, rhat, p, v, y, z, sv, tv, part, st, ev);
+int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *y,
+                       float *vt, float *tv, Partials part, SolverState *st, hipEvent_t *ev) {
+  return iter_impl<float>(g, s, x, rt, rhat, q, y, vt, tv, part, st, ev);
 }
 
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st) {
@@ -767,22 +802,18 @@ int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double
   return (int)hipGetLastError();
 }
 
-int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, const float *rhat,
-                    float *p, float *v, float *y, float *z, float *sv, float *tv, Partials part, SolverState *st,
+int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
+                    const float *rhat, float *q, float *y, float *vt, float *tv, Partials part, SolverState *st,
                     hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(float));
-  LAUNCH_TIMED(ev, 0, k_rel_u, grid, blk, (size_t)0, s, g, rhs, (const double *)x, (const float *)e, r, rhat, part,
+  LAUNCH_TIMED(ev, 0, k_rel_u, grid, blk, (size_t)0, s, g, rhs, (const double *)x, (const float *)e, rt, rhat, part,
                st);
-  LAUNCH_TIMED(ev, 1, k_bicg_a1<float>, grid, blk, lds, s, g, (const float *)r, p, (const float *)v, y, part, st, x,
-               (const float *)e);
-  LAUNCH_TIMED(ev, 2, k_bicg_a2<float>, grid, blk, lds, s, g, (const float *)y, rhat, v, part,
-               (const SolverState *)st);
-  LAUNCH_TIMED(ev, 3, k_bicg_b1<float>, grid, blk, lds, s, g, (const float *)r, (const float *)v, sv, z, part, st);
-  LAUNCH_TIMED(ev, 4, k_bicg_b2<float>, grid, blk, lds, s, g, (const float *)z, (const float *)sv, tv, part,
-               (const SolverState *)st);
-  LAUNCH_TIMED(ev, 5, k_bicg_c<float>, grid, blk, lds, s, g, e, r, rhat, (const float *)y, (const float *)z,
-               (const float *)sv, (const float *)tv, part, st);
+  LAUNCH_TIMED(ev, 1, k_bicg_a<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)q, rhat, y, vt, part,
+               st, x, (const float *)e);
+  LAUNCH_TIMED(ev, 2, k_bicg_b<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)vt, tv, part, st);
+  LAUNCH_TIMED(ev, 3, k_bicg_c<float>, grid, blk, lds, s, g, e, rt, q, rhat, (const float *)y, (const float *)vt,
+               (const float *)tv, part, st);
   return (int)hipGetLastError();
 }
 
@@ -836,11 +867,9 @@ int solver_blocks_per_cu(const Geo &g) {
     const void *f;
     size_t lds;
   } ks[] = {
-      {(const void *)k_bicg_a1<double>, level_lds(g, 8)}, {(const void *)k_bicg_a2<double>, level_lds(g, 8)},
-      {(const void *)k_bicg_b1<double>, level_lds(g, 8)}, {(const void *)k_bicg_b2<double>, level_lds(g, 8)},
-      {(const void *)k_bicg_c<double>, level_lds(g, 8)},  {(const void *)k_bicg_a1<float>, level_lds(g, 4)},
-      {(const void *)k_bicg_a2<float>, level_lds(g, 4)},  {(const void *)k_bicg_b1<float>, level_lds(g, 4)},
-      {(const void *)k_bicg_b2<float>, level_lds(g, 4)},  {(const void *)k_bicg_c<float>, level_lds(g, 4)},
+      {(const void *)k_bicg_a<double>, level_lds(g, 8)}, {(const void *)k_bicg_b<double>, level_lds(g, 8)},
+      {(const void *)k_bicg_c<double>, level_lds(g, 8)}, {(const void *)k_bicg_a<float>, level_lds(g, 4)},
+      {(const void *)k_bicg_b<float>, level_lds(g, 4)},  {(const void *)k_bicg_c<float>, level_lds(g, 4)},
       {(const void *)k_resid<double>, 0},                 {(const void *)k_resid<float>, 0},
       {(const void *)k_rel_u, 0},
   };

@@ -72,8 +72,8 @@ struct nsx_ctx {
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  Timer t_k[5], t_stage, t_k32[5], t_u;  // FP64 A1, A2, B1, B2, C; non-solver stage kernels; FP32 A1..C; U
-  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..4 FP64 solver, 5 stage, 6..10 FP32, 11 U)
+  Timer t_k[3], t_stage, t_k32[3], t_u;  // FP64 A, B, C; non-solver stage kernels; FP32 A, B, C; U
+  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..2 FP64 solver, 5 stage, 6..8 FP32, 11 U)
 };
 
 static int nsx_fail(nsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -688,11 +688,11 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[12] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   &ctx->t_k[3],   &ctx->t_k[4],  &ctx->t_stage,
-                  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], &ctx->t_k32[3], &ctx->t_k32[4], &ctx->t_u};
+  Timer *t[12] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr, nullptr, &ctx->t_stage,
+                  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], nullptr, nullptr, &ctx->t_u};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
+    if (t[pr.second] && hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
       t[pr.second]->ms += ms;
       t[pr.second]->n += 1;
     }
@@ -719,14 +719,14 @@ static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, dou
     for (int c = 0; c < chunk; ++c) {
       hipEvent_t *ev = nullptr;
       if (ctx->timing) {
-        ev = events(ctx, 10);
+        ev = events(ctx, 6);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->svv[0], ctx->sp[1], ctx->svv[1],
-                              ctx->ss, ctx->st_, ctx->part, ctx->st_dev, ev));
+      LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->sp[1], ctx->svv[0], ctx->svv[1],
+                              ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
-        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, q});
+        for (int q = 0; q < 3; ++q) ctx->pending.push_back({base + 2 * q, q});
       }
       ++issued;
     }
@@ -749,8 +749,8 @@ static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, dou
   ctx->iter_hint = h.a_iter > 0 ? h.a_iter : ctx->iter_hint;
   // time only the iterations that ran, not the queued launches that found
   // the solver finished
-  if (ctx->timing && ctx->pending.size() > pend0 + 5 * (size_t)h.a_iter)
-    ctx->pending.resize(pend0 + 5 * (size_t)h.a_iter);
+  if (ctx->timing && ctx->pending.size() > pend0 + 3 * (size_t)h.a_iter)
+    ctx->pending.resize(pend0 + 3 * (size_t)h.a_iter);
   collect_timing(ctx);
   return 0;
 }
@@ -799,14 +799,14 @@ static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
       for (int c = 0; c < chunk; ++c) {
         hipEvent_t *ev = nullptr;
         if (ctx->timing) {
-          ev = events(ctx, 10);
+          ev = events(ctx, 6);
           if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
         }
-        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
-                                  ctx->ft, ctx->part, ctx->st_dev, ev));
+        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fy, ctx->fv, ctx->ft, ctx->part,
+                                  ctx->st_dev, ev));
         if (ev) {
           const size_t base = (size_t)(ev - &ctx->evpool[0]);
-          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, 6 + q});
+          for (int q = 0; q < 3; ++q) ctx->pending.push_back({base + 2 * q, 6 + q});
         }
         ++issued;
       }
@@ -817,8 +817,8 @@ static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
       chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
     }
     const SolverState &h = *ctx->st_host;
-    if (ctx->timing && ctx->pending.size() > pstart + 5 * (size_t)h.a_iter)
-      ctx->pending.resize(pstart + 5 * (size_t)h.a_iter);
+    if (ctx->timing && ctx->pending.size() > pstart + 3 * (size_t)h.a_iter)
+      ctx->pending.resize(pstart + 3 * (size_t)h.a_iter);
     total += h.a_iter;
     ++passes;
     if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
@@ -861,15 +861,15 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
     for (int c = 0; c < chunk; ++c) {
       hipEvent_t *ev = nullptr;
       if (ctx->timing) {
-        ev = events(ctx, 12);
+        ev = events(ctx, 8);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->fy, ctx->fz, ctx->fs,
-                             ctx->ft, ctx->part, ctx->st_dev, ev));
+      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fy, ctx->fv, ctx->ft,
+                             ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
         ctx->pending.push_back({base, 11});
-        for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 + 2 * q, 6 + q});
+        for (int q = 0; q < 3; ++q) ctx->pending.push_back({base + 2 + 2 * q, 6 + q});
       }
       ++issued;
     }
@@ -883,8 +883,8 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
   if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
     // the FP32 iteration failed: the correction since the last update is
     // dropped and the FP64 solve finishes from x
-    if (ctx->timing && ctx->pending.size() > pend0 + 6 * (size_t)h.a_iter)
-      ctx->pending.resize(pend0 + 6 * (size_t)h.a_iter);
+    if (ctx->timing && ctx->pending.size() > pend0 + 4 * (size_t)h.a_iter)
+      ctx->pending.resize(pend0 + 4 * (size_t)h.a_iter);
     collect_timing(ctx);
     const int its32 = h.a_iter, ups = h.mp_pass;
     if (its32 > 0) ctx->iter_hint32 = its32;
@@ -900,8 +900,8 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
   ctx->info.outer_passes = h.mp_pass;
   ctx->info.iters_fp32 = h.a_iter;
   if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
-  if (ctx->timing && ctx->pending.size() > pend0 + 6 * (size_t)h.a_iter)
-    ctx->pending.resize(pend0 + 6 * (size_t)h.a_iter);
+  if (ctx->timing && ctx->pending.size() > pend0 + 4 * (size_t)h.a_iter)
+    ctx->pending.resize(pend0 + 4 * (size_t)h.a_iter);
   collect_timing(ctx);
   return 0;
 }
@@ -1304,9 +1304,9 @@ extern "C" int nsx_timing_enable(nsx_ctx *ctx, int on) {
 extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches) {
   if (!ctx || !name) return -1;
   const Timer *t = nullptr;
-  static const char *names[5] = {"bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c"};
-  static const char *names32[5] = {"bicg32_a1", "bicg32_a2", "bicg32_b1", "bicg32_b2", "bicg32_c"};
-  for (int q = 0; q < 5; ++q) {
+  static const char *names[3] = {"bicg_a", "bicg_b", "bicg_c"};
+  static const char *names32[3] = {"bicg32_a", "bicg32_b", "bicg32_c"};
+  for (int q = 0; q < 3; ++q) {
     if (!strcmp(name, names[q])) t = &ctx->t_k[q];
     if (!strcmp(name, names32[q])) t = &ctx->t_k32[q];
   }
@@ -1320,7 +1320,7 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
-  for (int q = 0; q < 5; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
+  for (int q = 0; q < 3; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
   ctx->t_stage = ctx->t_u = Timer{};
   return 0;
 }

@@ -74,23 +74,25 @@ int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *
 // ---- solver
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
                      double *rhat, Partials part, SolverState *st, double eps, int max_iters);
-int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *r, const double *rhat, double *p, double *v,
-                     double *y, double *z, double *sv, double *tv, Partials part, SolverState *st,
-                     hipEvent_t *ev /* nullable: 10 events, start/stop of A1, A2, B1, B2, C */);
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *r, const float *rhat, float *p, float *v,
-                       float *y, float *z, float *sv, float *tv, Partials part, SolverState *st, hipEvent_t *ev);
+// one iteration, three kernels (A, B, C) on the scaled vectors rt = r/A0,
+// q = (p - omega v)/A0, vt = v/A0, and y, t; x is the solution
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *y,
+                     double *vt, double *tv, Partials part, SolverState *st,
+                     hipEvent_t *ev /* nullable: 6 events, start/stop of A, B, C */);
+int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *y,
+                       float *vt, float *tv, Partials part, SolverState *st, hipEvent_t *ev);
 // mixed precision: reset, refinement residual (+ outer test, inner reset), correction
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st);
 int launch_mp_resid(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
                     float *rhat32, float *e32, Partials part, SolverState *st, double eps, double eps_inner,
                     int max_iters, int max_passes, int first_pass);
 int launch_mp_update(const Geo &g, hipStream_t s, double *x, const float *e32);
-// reliable updates: initial true residual, then iterations U, A1..C
+// reliable updates: initial true residual, then iterations U, A, B, C
 int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
                     float *rhat32, float *e32, Partials part, SolverState *st, double eps, double delta,
                     int max_iters);
-int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, const float *rhat,
-                    float *p, float *v, float *y, float *z, float *sv, float *tv, Partials part, SolverState *st,
-                    hipEvent_t *ev /* nullable: 12 events, start/stop of U, A1, A2, B1, B2, C */);
+int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
+                    const float *rhat, float *q, float *y, float *vt, float *tv, Partials part, SolverState *st,
+                    hipEvent_t *ev /* nullable: 8 events, start/stop of U, A, B, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

@@ -199,10 +199,10 @@ int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int m
 
 /* ---- timing ---------------------------------------------------------------- */
 /* per-kernel device time accumulated with HIP events since the last reset;
- * names: "bicg_a1", "bicg_a2", "bicg_b1", "bicg_b2", "bicg_c" (the five
- * BiCGSTAB kernels in FP64, iterations that ran only), "bicg32_a1" ..
- * "bicg32_c" (the same kernels of the FP32 solve), "rel_u" (the reliable-update
- * kernel) and "stage" (all non-solver kernels of a projection stage) */
+ * names: "bicg_a", "bicg_b", "bicg_c" (the three BiCGSTAB kernels in FP64,
+ * iterations that ran only), "bicg32_a" .. "bicg32_c" (the same kernels of the
+ * FP32 solve), "rel_u" (the reliable-update kernel) and "stage" (all
+ * non-solver kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int nsx_timing_reset(nsx_ctx *ctx);
