@@ -74,6 +74,13 @@ __device__ inline Row ppe_row(const Geo &g, int i, int k) {
   r.a2 = g.ppec[i * 3 + 1] * sv;
   r.a3 = g.ppec[i * 3 + 2] * sv;
   r.A0 = -1.0 * (r.AT + r.AB + r.a1 + r.a2 + r.a3);
+  if (g.trans) {
+    // row i of A^T: the diagonal and the (symmetric) vertical couplings are
+    // row i's, the lateral ones are those of the neighbours' rows towards i
+    r.a1 = g.ppec_t[i * 3] * sv;
+    r.a2 = g.ppec_t[i * 3 + 1] * sv;
+    r.a3 = g.ppec_t[i * 3 + 2] * sv;
+  }
   return r;
 }
 
@@ -84,6 +91,7 @@ template <>
 struct Prec<double> {
   __device__ static const double *area(const Geo &g) { return g.area; }
   __device__ static const double *ppec(const Geo &g) { return g.ppec; }
+  __device__ static const double *ppec_t(const Geo &g) { return g.ppec_t; }
   __device__ static const double *dsig(const Geo &g) { return g.dsig; }
   __device__ static const double *dsigv(const Geo &g) { return g.dsigv; }
 };
@@ -91,6 +99,7 @@ template <>
 struct Prec<float> {
   __device__ static const float *area(const Geo &g) { return g.area_f; }
   __device__ static const float *ppec(const Geo &g) { return g.ppec_f; }
+  __device__ static const float *ppec_t(const Geo &g) { return g.ppec_t_f; }
   __device__ static const float *dsig(const Geo &g) { return g.dsig_f; }
   __device__ static const float *dsigv(const Geo &g) { return g.dsigv_f; }
 };
@@ -159,6 +168,19 @@ __device__ inline void rows4(const Geo &g, const T *lv, int i, int k0, Rows4<T> 
     R.a2[u] = c2 * sv;
     R.a3[u] = c3 * sv;
     R.A0[u] = T(-1) * (R.AT[u] + R.AB[u] + R.a1[u] + R.a2[u] + R.a3[u]);
+  }
+  if (g.trans) {
+    // row i of A^T (see ppe_row)
+    const T *pt = Prec<T>::ppec_t(g) + i * 3;
+    const T d1 = pt[0], d2 = pt[1], d3 = pt[2];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u;
+      const T sv = lv[g.NS + (k >= 1 ? k - 1 : 0)];
+      R.a1[u] = d1 * sv;
+      R.a2[u] = d2 * sv;
+      R.a3[u] = d3 * sv;
+    }
   }
 }
 

@@ -372,6 +372,22 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
       int jc = ctx->No_cp[(size_t)i + (size_t)j * NC] - 1;
       ppec[(size_t)i * 3 + j] = (ctx->bcc[3][jc] == 1) ? 0.0 : co1 * ctx->dlVV[f];
     }
+  // A^T: the lateral coefficient of row i towards neighbour n is row n's
+  // coefficient towards i (the edge of n whose neighbour is i; where several
+  // edges of n lead to i, the one with the closest coefficient)
+  std::vector<double> ppec_t((size_t)N0 * 3);
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int n = nbr[(size_t)i * 3 + j];
+      int best = -1;
+      for (int jj = 0; jj < 3; ++jj)
+        if (nbr[(size_t)n * 3 + jj] == i &&
+            (best < 0 || std::fabs(ppec[(size_t)n * 3 + jj] - ppec[(size_t)i * 3 + j]) <
+                             std::fabs(ppec[(size_t)n * 3 + best] - ppec[(size_t)i * 3 + j])))
+          best = jj;
+      if (best < 0) return nsx_fail(ctx, -1, "cell %d is not a neighbour of its neighbour %d", i + 1, n + 1);
+      ppec_t[(size_t)i * 3 + j] = ppec[(size_t)n * 3 + best];
+    }
   // the compact matrix must reproduce PDE_center's coefficients bit for bit
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i) {
@@ -448,6 +464,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
                                         &ctx->xnn, &ctx->ynn, &ctx->dxCC, &ctx->dyCC};
   for (int q = 0; q < 9; ++q) UP(d[q], edge_major(*edge[q], N0));
   UP(d[9], ppec);
+  UP(d[10], ppec_t);
   g.nbr = d_nbr;
   g.vtx = d_vtx;
   g.dlVV = d[0];
@@ -460,6 +477,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   g.dxCC = d[7];
   g.dyCC = d[8];
   g.ppec = d[9];
+  g.ppec_t = d[10];
+  g.trans = 0;
   double *a_area, *a_sx, *a_sy, *a_sxy, *a_sig, *a_dsig, *a_sigv, *a_dsigv, *a_walld, *a_vw, *a_dl;
   UP(a_area, std::vector<double>(ctx->areaCell.begin(), ctx->areaCell.begin() + N0));
   UP(a_sx, ctx->sum_xc2);
@@ -491,9 +510,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
       for (size_t q = 0; q < n; ++q) o[q] = (float)v[q];
       return o;
     };
-    float *fa, *fpc, *fds, *fdv;
+    float *fa, *fpc, *fds, *fdv, *fpt;
     UP(fa, f32(ctx->areaCell, (size_t)N0));
     UP(fpc, f32(ppec, ppec.size()));
+    UP(fpt, f32(ppec_t, ppec_t.size()));
+    g.ppec_t_f = fpt;
     UP(fds, f32(ctx->dsig, (size_t)NZ));
     UP(fdv, f32(ctx->dsigv, (size_t)NZ - 1));
     g.area_f = fa;
@@ -1231,6 +1252,27 @@ extern "C" int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_
   int rc = solve(ctx, ctx->slot[rhs], ctx->slot[phi], ctx->slot[phi]);
   if (rc) return rc;
   if ((rc = bcglobalp_(ctx, phi, phiv, 3))) return rc;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (info) *info = ctx->info;
+  return ctx->info.exit_code == NSX_SOLVE_CONVERGED ? 0 : ctx->info.exit_code;
+}
+
+// The adjoint of nsx_bicgstab with respect to its right-hand side.  The
+// forward map is phi = N0 A^+ N1 rhs with N1 = Glob_Norm(.,1) (subtract the
+// volume-weighted mean) and N0 = Glob_Norm(.,0) (subtract the mean over the
+// volume); their transposes are each other's form, so the adjoint is the
+// same wrapper on A^T: lam = N0 (A^T)^+ N1 g, solved by the FP64 BiCGSTAB
+// from lam = 0 with the lateral couplings of A^T (ppec_t).
+extern "C" int nsx_bicgstab_adjoint(nsx_ctx *ctx, int lam, int g, nsx_solve_info *info) {
+  NEED_COMMIT(ctx);
+  if (kind_ok(ctx, lam, FK_CELL) || kind_ok(ctx, g, FK_CELL)) return -1;
+  if (lam == g) return nsx_fail(ctx, -1, "nsx_bicgstab_adjoint: lam and g must be different fields");
+  HIPCHECK(hipMemsetAsync(ctx->slot[lam], 0, field_elems(ctx, FK_CELL) * sizeof(double), ctx->stream));
+  LAUNCH(launch_gnorm_sum(ctx->g, ctx->stream, ctx->slot[g], ctx->part));
+  ctx->g.trans = 1;
+  const int rc = solve_fp64(ctx, ctx->slot[g], ctx->slot[lam], ctx->slot[lam]);
+  ctx->g.trans = 0;
+  if (rc) return rc;
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   if (info) *info = ctx->info;
   return ctx->info.exit_code == NSX_SOLVE_CONVERGED ? 0 : ctx->info.exit_code;

@@ -64,6 +64,8 @@ struct Geo {
   const int *vtx;           // No_vp(i,j), 0-based
   const double *dlVV, *dlCC, *dle1, *dle2, *doe, *xnn, *ynn, *dxCC, *dyCC;
   const double *ppec;       // (dle1/dlCC)*dlVV: PPE edge coefficient before the dsigv factor
+  const double *ppec_t;     // ppec of the neighbour's edge back towards i: the lateral couplings of A^T
+  int trans;                // 1 while the solver applies A^T (the adjoint solve)
   const signed char *eside; // 1 when No_cp(i,j) is not an interior cell (a side face of glob_cfl)
   // per cell (N0)
   const double *area, *sum_xc2, *sum_yc2, *sum_xcyc;
@@ -71,7 +73,7 @@ struct Geo {
   const double *sig, *dsig;  // NZ
   const double *sigv, *dsigv;  // NZ-1
   // FP32 copies for the FP32 inner solver of the mixed-precision form
-  const float *area_f, *ppec_f, *dsig_f, *dsigv_f;
+  const float *area_f, *ppec_f, *dsig_f, *dsigv_f, *ppec_t_f;
   const double *walld;       // NZ
   // vertices: inverse-distance stencil in CSR form (interior cells, reference order)
   const int *vptr, *vcell;

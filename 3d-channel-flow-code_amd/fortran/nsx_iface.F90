@@ -233,6 +233,13 @@
           type(c_ptr), value :: ctx
           integer(c_int), value :: p, pv
         end function
+        integer(c_int) function nsx_bicgstab_adjoint(ctx, lam, g, info)  &
+             bind(C, name="nsx_bicgstab_adjoint")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: lam, g
+          type(nsx_solve_info) :: info
+        end function
         integer(c_int) function nsx_glob_cfl(ctx, u, v, w, info)         &
              bind(C, name="nsx_glob_cfl")
           import :: c_int, c_ptr, nsx_cfl_info

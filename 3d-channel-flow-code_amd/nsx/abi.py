@@ -34,7 +34,7 @@ EXPORTS = [
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
-    "nsx_newu3d", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
 ]
 
 
@@ -91,6 +91,7 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_newu3d": [c_vp, c_int, c_int],
         "nsx_set_precision": [c_vp, c_int, c_int, c_dbl, c_int],
         "nsx_glob_cfl": [c_vp, c_int, c_int, c_int, P(CflInfo)],
+        "nsx_bicgstab_adjoint": [c_vp, c_int, c_int, P(SolveInfo)],
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
         "nsx_timing_reset": [c_vp],

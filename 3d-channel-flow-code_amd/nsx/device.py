@@ -220,6 +220,15 @@ class Context:
         return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
 
     # ------------------------------------------------------------- timing ---
+    def bicgstab_adjoint(self, g):
+        """Adjoint of bicgstab() with respect to its right-hand side: returns
+        (lam, solve info) with <g, d phi> = <lam, d rhs>."""
+        (cg,) = self._cells(g)
+        lam = CELL_SCRATCH + 1
+        si = SolveInfo()
+        self._check(self.lib.nsx_bicgstab_adjoint(self.h, lam, cg, ctypes.byref(si)))
+        return self.get(lam), si.as_dict()
+
     def glob_cfl(self, u=None, v=None, w=None) -> dict:
         """glob_cfl (cfl.F90:7-269) on device: resident UFN, VFN, WFN by
         default, or the given (N_CELL,NZ) arrays, with the resident faces."""

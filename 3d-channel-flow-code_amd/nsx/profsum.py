@@ -25,7 +25,8 @@ def _csvs(d: str, suffix: str):
 
 
 def short(name: str) -> str:
-    n = name.split("(")[0].strip()
+    n = name.replace("(anonymous namespace)::", "")
+    n = n.split("(")[0].strip()
     return n[5:] if n.startswith("void ") else n
 
 

@@ -164,6 +164,15 @@ int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info)
 /* newu3D (newu3D.F90:8-257): U1FACE..UBFACE -= dt grad_n p */
 int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
 
+/* ---- differentiation (no counterpart in the reference) ----------------------- */
+/* Adjoint of nsx_bicgstab with respect to its right-hand side: for the map
+ * phi = Glob_Norm0(A^+ Glob_Norm1(rhs)) it returns lam = Glob_Norm0((A^T)^+
+ * Glob_Norm1(g)), so that <g, dphi> = <lam, drhs> (interior cells).  A^T is
+ * the PPE operator with each lateral coupling taken from the neighbour's row;
+ * the solve is the FP64 BiCGSTAB from lam = 0 with the reference's stopping
+ * rule.  lam and g are cell fields, lam != g. */
+int nsx_bicgstab_adjoint(nsx_ctx *ctx, int lam, int g, nsx_solve_info *info);
+
 /* ---- diagnostics ------------------------------------------------------------ */
 /* glob_cfl (cfl.F90:7-269, sequential branch) on the cell fields u, v, w and
  * the resident face velocities U1FACE..UBFACE; the values it prints and
