@@ -139,49 +139,67 @@ __device__ inline T *load_levels(const Geo &g) {
   return lv;
 }
 
-// the rows of layers k0..k0+3 of column i; rows outside the interior are zero
+// Column data of the PPE rows of column i: the area, the edge coefficients
+// ppec that enter the diagonal, and the lateral couplings (ppec again, or
+// ppec_t for the rows of A^T, see ppe_row).
 template <typename T>
-struct Rows4 {
-  T A0[4], a1[4], a2[4], a3[4], AT[4], AB[4];
-  bool in[4];
+struct Col {
+  T area, c[3], d[3];
 };
 
 template <typename T>
-__device__ inline void rows4(const Geo &g, const T *lv, int i, int k0, Rows4<T> &R) {
-  const T area = Prec<T>::area(g)[i];
+__device__ inline void load_col(const Geo &g, int i, Col<T> &C) {
+  C.area = Prec<T>::area(g)[i];
   const T *pc = Prec<T>::ppec(g) + i * 3;
-  const T c1 = pc[0], c2 = pc[1], c3 = pc[2];
-  T q[5];  // q[u] = area / dsig(k0-1+u)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) C.c[j] = pc[j];
+  if (g.trans) {
+    const T *pt = Prec<T>::ppec_t(g) + i * 3;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C.d[j] = pt[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C.d[j] = C.c[j];
+  }
+}
+
+// q[u] = area / dsig(k0-1+u), u = 0..4: AB of layer k0+u is q[u] and its AT
+// is q[u+1], so five divisions serve the four rows of a quad
+template <typename T>
+__device__ inline void quot5(const Col<T> &C, const T *lv, int k0, T (&q)[5]) {
 #pragma unroll
   for (int u = 0; u < 5; ++u) {
     const int k = k0 - 1 + u;
-    q[u] = area / lv[k < 0 ? 0 : k];
+    q[u] = C.area / lv[k < 0 ? 0 : k];
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int k = k0 + u;
-    R.in[u] = k >= 1 && k <= g.NZ - 2;
-    const T sv = lv[g.NS + (k >= 1 ? k - 1 : 0)];
-    R.AT[u] = (k == g.NZ - 2 && g.bctop == 1) ? T(0) : q[u + 1];
-    R.AB[u] = (k == 1 && g.bcbot == 1) ? T(0) : q[u];
-    R.a1[u] = c1 * sv;
-    R.a2[u] = c2 * sv;
-    R.a3[u] = c3 * sv;
-    R.A0[u] = T(-1) * (R.AT[u] + R.AB[u] + R.a1[u] + R.a2[u] + R.a3[u]);
-  }
+}
+
+// the row of layer k0+u of the column (ppe_row's arithmetic); `in` is false
+// outside the interior layers, where the solver's vectors hold zero
+template <typename T>
+struct Row1 {
+  T A0, a1, a2, a3, AT, AB;
+  bool in;
+};
+
+template <typename T>
+__device__ inline Row1<T> row_of(const Geo &g, const T *lv, const Col<T> &C, const T (&q)[5], int k0, int u) {
+  const int k = k0 + u;
+  Row1<T> R;
+  R.in = k >= 1 && k <= g.NZ - 2;
+  const T sv = lv[g.NS + (k >= 1 ? k - 1 : 0)];
+  R.AT = (k == g.NZ - 2 && g.bctop == 1) ? T(0) : q[u + 1];
+  R.AB = (k == 1 && g.bcbot == 1) ? T(0) : q[u];
+  R.a1 = C.c[0] * sv;
+  R.a2 = C.c[1] * sv;
+  R.a3 = C.c[2] * sv;
+  R.A0 = T(-1) * (R.AT + R.AB + R.a1 + R.a2 + R.a3);
   if (g.trans) {
-    // row i of A^T (see ppe_row)
-    const T *pt = Prec<T>::ppec_t(g) + i * 3;
-    const T d1 = pt[0], d2 = pt[1], d3 = pt[2];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = k0 + u;
-      const T sv = lv[g.NS + (k >= 1 ? k - 1 : 0)];
-      R.a1[u] = d1 * sv;
-      R.a2[u] = d2 * sv;
-      R.a3[u] = d3 * sv;
-    }
+    R.a1 = C.d[0] * sv;
+    R.a2 = C.d[1] * sv;
+    R.a3 = C.d[2] * sv;
   }
+  return R;
 }
 
 template <class F>
@@ -497,71 +515,102 @@ __device__ inline T axpy1(const T *a, const T *b, T f, bool first, size_t off) {
   return first ? a[off] : a[off] + f * b[off];
 }
 
-// A f on layers k0..k0+3 of column i from the values of f at the seven
-// points (get_Apk, bicgstab.F90:976-998; Neumann vertical ghosts, periodic
-// partners through nbr); rows outside the interior give 0
+// (A f) at layer k0+u of column i from the values of f at the seven points
+// (get_Apk, bicgstab.F90:976-998; Neumann vertical ghosts, periodic partners
+// through nbr); rows outside the interior give 0
 template <typename T>
-__device__ inline void stencil4(const Geo &g, const Rows4<T> &R, int k0, const T (&fc)[4], T fm, T fp,
-                                const T (&f0)[4], const T (&f1)[4], const T (&f2)[4], T (&out)[4]) {
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int k = k0 + u;
-    const T fs = fc[u];
-    const T fu = (k + 1 <= g.NZ - 2) ? (u < 3 ? fc[u < 3 ? u + 1 : 3] : fp) : fs;
-    const T fd = (k - 1 >= 1) ? (u > 0 ? fc[u > 0 ? u - 1 : 0] : fm) : fs;
-    out[u] = R.in[u] ? R.A0[u] * fs + R.a1[u] * f0[u] + R.a2[u] * f1[u] + R.a3[u] * f2[u] + R.AT[u] * fu +
-                           R.AB[u] * fd
-                     : T(0);
+__device__ inline T apply_row(const Geo &g, const Row1<T> &R, int k0, int u, const T (&fc)[4], T fm, T fp, T f0,
+                              T f1, T f2) {
+  const int k = k0 + u;
+  const T fs = fc[u];
+  const T fu = (k + 1 <= g.NZ - 2) ? (u < 3 ? fc[u < 3 ? u + 1 : 3] : fp) : fs;
+  const T fd = (k - 1 >= 1) ? (u > 0 ? fc[u > 0 ? u - 1 : 0] : fm) : fs;
+  return R.in ? R.A0 * fs + R.a1 * f0 + R.a2 * f1 + R.a3 * f2 + R.AT * fu + R.AB * fd : T(0);
+}
+
+// Windows of NSX_BLOCK consecutive quads, swept band by band.  Blocks are
+// dealt round-robin to the XCDs, so hardware block b runs on XCD b % 8.  XCD
+// x owns the x-th contiguous eighth of the windows, and its G/8 blocks sweep
+// that slice together: block j takes windows j, j + G/8, j + 2G/8, ... of it.
+// At any time one XCD works on one band of G/8 adjacent windows, so the
+// y-neighbour column of a quad (one mesh row, a few hundred columns away)
+// usually lies in the band its own L2 is serving.
+__device__ inline void window_sweep(uint32_t nwin, uint32_t &w, uint32_t &wend, uint32_t &step) {
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  if ((G % NSX_NXCD) == 0) {
+    const uint32_t per = (nwin + NSX_NXCD - 1) / NSX_NXCD, x = b % NSX_NXCD;
+    const uint32_t s0 = x * per < nwin ? x * per : nwin;
+    wend = s0 + per < nwin ? s0 + per : nwin;
+    w = s0 + b / NSX_NXCD;
+    step = G / NSX_NXCD;
+  } else {
+    w = b;
+    wend = nwin;
+    step = G;
   }
 }
 
-// Seven-point values of f = a + c b around every quad of the block's range
-// (f = a when first).  Each thread forms f on its own four layers and
-// publishes them in LDS; the vertical neighbours and the neighbour columns
-// whose quad lies in the block's current window of NSX_BLOCK quads (the
-// in-quad partner and usually the x-neighbour) are read from there, the rest
-// (the y-neighbour, window edges) are formed from global memory with the same
-// arithmetic, so every value is bit-identical whichever way it was obtained.
-template <typename T, class F>
-__device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, F &&body) {
+// Seven-point values of f = a + c b around every quad (f = a when first).
+// Each thread forms f on its own four layers and publishes them in LDS; the
+// vertical neighbours and the neighbour columns whose quad lies in the
+// block's current window of NSX_BLOCK quads (the in-quad partner and usually
+// the x-neighbour) are read from there, the rest (the y-neighbour, window
+// edges) are formed from global memory with the same arithmetic, so every
+// value is bit-identical whichever way it was obtained.  Every global load of
+// a window (own values, neighbour indices, column data, the out-of-window
+// neighbours and what `pre` loads) is issued before the window's barrier, so
+// their latencies overlap; body(i, k0, s0, C, fc, fm, fp, f0, f1, f2) runs
+// after it.
+template <typename T, class Pre, class Body>
+__device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, Pre &&pre,
+                                              Body &&body) {
   __shared__ __align__(16) T win[NSX_BLOCK][4];
-  uint32_t beg, end;
-  block_range(g.nquad, beg, end);
-  for (uint32_t base = beg; base < end; base += NSX_BLOCK) {
+  const uint32_t nwin = (g.nquad + NSX_BLOCK - 1) / NSX_BLOCK;
+  uint32_t w0, wend, step;
+  window_sweep(nwin, w0, wend, step);
+  for (uint32_t wi = w0; wi < wend; wi += step) {
+    const uint32_t base = wi * NSX_BLOCK;
+    const uint32_t end = base + NSX_BLOCK < g.nquad ? base + NSX_BLOCK : g.nquad;
     const uint32_t t = base + threadIdx.x;
     const bool act = t < end;
-    const uint32_t lim = end - base < NSX_BLOCK ? end - base : NSX_BLOCK;
+    const uint32_t lim = end - base;
+    const bool mwin = threadIdx.x >= 1, pwin = threadIdx.x + 1 < lim;
     int i = 0, k0 = 0;
     size_t s0 = 0;
-    T fc[4] = {T(0), T(0), T(0), T(0)};
+    T fc[4] = {T(0), T(0), T(0), T(0)}, fn[3][4], fm = T(0), fp = T(0);
+    uint32_t d[3] = {0u, 0u, 0u};
+    Col<T> C;
     if (act) {
       i = (int)fdiv(t, g.divNQ);
       k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
       s0 = (size_t)i * g.NS + k0;
       axpy4(a, b, c, first, s0, fc);
+      const int e = i * 3;
+      const int n[3] = {g.nbr[e], g.nbr[e + 1], g.nbr[e + 2]};
+      load_col(g, i, C);
+      pre(s0);
+      const uint32_t qq = (uint32_t)(k0 >> 2);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        d[j] = (uint32_t)n[j] * (uint32_t)g.NQ + qq - base;
+        if (d[j] >= lim) axpy4(a, b, c, first, (size_t)n[j] * g.NS + k0, fn[j]);
+      }
+      if (k0 >= 1 && !mwin) fm = axpy1(a, b, c, first, s0 - 1);
+      if (k0 + 4 <= g.NZ - 2 && !pwin) fp = axpy1(a, b, c, first, s0 + 4);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) win[threadIdx.x][u] = fc[u];
     __syncthreads();
     if (act) {
-      T fm = T(0), fp = T(0);
-      if (k0 >= 1) fm = threadIdx.x >= 1 ? win[threadIdx.x - 1][3] : axpy1(a, b, c, first, s0 - 1);
-      if (k0 + 4 <= g.NZ - 2) fp = threadIdx.x + 1 < lim ? win[threadIdx.x + 1][0] : axpy1(a, b, c, first, s0 + 4);
-      const int e = i * 3;
-      const uint32_t qq = (uint32_t)(k0 >> 2);
-      T fn[3][4];
+      if (k0 >= 1 && mwin) fm = win[threadIdx.x - 1][3];
+      if (k0 + 4 <= g.NZ - 2 && pwin) fp = win[threadIdx.x + 1][0];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int n = g.nbr[e + j];
-        const uint32_t d = (uint32_t)n * (uint32_t)g.NQ + qq - base;
-        if (d < lim) {
+      for (int j = 0; j < 3; ++j)
+        if (d[j] < lim) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) fn[j][u] = win[d][u];
-        } else {
-          axpy4(a, b, c, first, (size_t)n * g.NS + k0, fn[j]);
+          for (int u = 0; u < 4; ++u) fn[j][u] = win[d[j]][u];
         }
-      }
-      body(i, k0, s0, fc, fm, fp, fn[0], fn[1], fn[2]);
+      body(i, k0, s0, C, fc, fm, fp, fn[0], fn[1], fn[2]);
     }
     __syncthreads();
   }
@@ -580,23 +629,24 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const 
   const T beta = (T)beta_d;
   const T *lv = load_levels<T>(g);
   double acc[1] = {0.0};
-  for_each_quad_gathered(g, rt, q, beta, first, [&](int i, int k0, size_t s0, const T(&yc)[4], T ym, T yp,
-                                                     const T(&y0)[4], const T(&y1)[4], const T(&y2)[4]) {
-    Rows4<T> R;
-    rows4<T>(g, lv, i, k0, R);
-    T v[4], rh[4];
-    stencil4(g, R, k0, yc, ym, yp, y0, y1, y2, v);
-    ld4(rhat + s0, rh);
-    T yo[4], vo[4];
+  T rh[4];
+  for_each_quad_gathered(
+      g, rt, q, beta, first, [&](size_t s0) { ld4(rhat + s0, rh); },
+      [&](int i, int k0, size_t s0, const Col<T> &C, const T(&yc)[4], T ym, T yp, const T(&y0)[4],
+          const T(&y1)[4], const T(&y2)[4]) {
+        T qv[5], yo[4], vo[4];
+        quot5(C, lv, k0, qv);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      yo[u] = R.in[u] ? yc[u] : T(0);
-      vo[u] = R.in[u] ? v[u] / R.A0[u] : T(0);
-      acc[0] += (double)rh[u] * (double)v[u];
-    }
-    st4(y + s0, yo);
-    st4(vt + s0, vo);
-  });
+        for (int u = 0; u < 4; ++u) {
+          const Row1<T> R = row_of(g, lv, C, qv, k0, u);
+          const T v = apply_row(g, R, k0, u, yc, ym, yp, y0[u], y1[u], y2[u]);
+          yo[u] = R.in ? yc[u] : T(0);
+          vo[u] = R.in ? v / R.A0 : T(0);
+          acc[0] += (double)rh[u] * (double)v;
+        }
+        st4(y + s0, yo);
+        st4(vt + s0, vo);
+      });
   write_partials<1>(acc, P.a);
 }
 
@@ -622,20 +672,22 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const 
   const T alm = (T)(-al);
   const T *lv = load_levels<T>(g);
   double acc[2] = {0.0, 0.0};
-  for_each_quad_gathered(g, rt, vt, alm, false, [&](int i, int k0, size_t s0, const T(&zc)[4], T zm, T zp,
-                                                    const T(&z0)[4], const T(&z1)[4], const T(&z2)[4]) {
-    Rows4<T> R;
-    rows4<T>(g, lv, i, k0, R);
-    T t[4];
-    stencil4(g, R, k0, zc, zm, zp, z0, z1, z2, t);
-    st4(tv + s0, t);
+  for_each_quad_gathered(
+      g, rt, vt, alm, false, [&](size_t) {},
+      [&](int i, int k0, size_t s0, const Col<T> &C, const T(&zc)[4], T zm, T zp, const T(&z0)[4],
+          const T(&z1)[4], const T(&z2)[4]) {
+        T qv[5], t[4];
+        quot5(C, lv, k0, qv);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double sd = R.in[u] ? (double)(R.A0[u] * zc[u]) : 0.0, td = (double)t[u];
-      acc[0] += td * sd;
-      acc[1] += td * td;
-    }
-  });
+        for (int u = 0; u < 4; ++u) {
+          const Row1<T> R = row_of(g, lv, C, qv, k0, u);
+          t[u] = apply_row(g, R, k0, u, zc, zm, zp, z0[u], z1[u], z2[u]);
+          const double sd = R.in ? (double)(R.A0 * zc[u]) : 0.0, td = (double)t[u];
+          acc[0] += td * sd;
+          acc[1] += td * td;
+        }
+        st4(tv + s0, t);
+      });
   write_partials<2>(acc, P.b);
 }
 
@@ -657,41 +709,52 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
   }
   const T alt = (T)al, omt = (T)om;
   const T *lv = load_levels<T>(g);
-  const T *area = Prec<T>::area(g);
   const bool fresh = st->upd != 0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for_each_quad(g, [&](int i, int k0, size_t s0) {
-    Rows4<T> R;
-    rows4<T>(g, lv, i, k0, R);
-    T xx[4], rr[4], vv[4], yy[4], tt[4], rh[4], qq[4];
+    T xx[4], rr[4], vv[4], yy[4], tt[4], rh[4], qq[4], qv[5];
+    // Scheduling fences keep the two 16-byte halves of each vector's run
+    // back to back.  Across a wave one half touches every cache line of the
+    // wave's span, so the second half then hits the lines the first one
+    // brought in; interleaving the six vectors' halves re-requests each line
+    // later and costs 15 % of the kernel in FP64.
     if (fresh) {
       xx[0] = xx[1] = xx[2] = xx[3] = T(0);
     } else {
       ld4(x + s0, xx);
     }
+    __builtin_amdgcn_sched_barrier(0);
     ld4(rt + s0, rr);
+    __builtin_amdgcn_sched_barrier(0);
     ld4(vt + s0, vv);
+    __builtin_amdgcn_sched_barrier(0);
     ld4(y + s0, yy);
+    __builtin_amdgcn_sched_barrier(0);
     ld4(tv + s0, tt);
+    __builtin_amdgcn_sched_barrier(0);
     ld4(rhat + s0, rh);
-    const T ar = area[i];
+    __builtin_amdgcn_sched_barrier(0);
+    Col<T> C;
+    load_col(g, i, C);
+    quot5(C, lv, k0, qv);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = k0 + u;
-      const bool in = R.in[u];
+      const Row1<T> R = row_of(g, lv, C, qv, k0, u);
+      const bool in = R.in;
       const T z = rr[u] - alt * vv[u];
-      const T sn = R.A0[u] * z;
+      const T sn = R.A0 * z;
       const T rn = sn - omt * tt[u];
       const T xn = xx[u] + alt * yy[u] + omt * z;
       xx[u] = in ? xn : xx[u];
-      rr[u] = in ? rn / R.A0[u] : T(0);
+      rr[u] = in ? rn / R.A0 : T(0);
       qq[u] = in ? yy[u] - omt * vv[u] : T(0);
       if (in) {
         const double rd = (double)rn;
         acc[0] += rd * rd;
         acc[1] += fabs(rd);
         acc[2] += (double)rh[u] * rd;
-        acc[3] += (double)xn * (double)(ar * lv[k]);
+        acc[3] += (double)xn * (double)(C.area * lv[k]);
       }
     }
     st4(x + s0, xx);
