@@ -5,12 +5,14 @@
 // One iteration is three kernels, one per global synchronisation point of
 // the algorithm (see "iteration" below for the scaled vectors they carry):
 //
-//   A  stop test of k-1, beta; y = rt + beta q; v = A y; vt = v/A0;
-//      partial <rhat, v>              update_pk, diag_precdt, get_Apk  :726-1001
+//   A  stop test of k-1, beta; y = rt + beta q at the seven points;
+//      v = A y; vt = v/A0; partial <rhat, v>
+//                                     update_pk, diag_precdt, get_Apk  :726-1001
 //   B  alpha; z = rt - alpha vt; t = A z; s = A0 z; partial <t,s>, <t,t>
 //                                     update_sk, diag_precdt, get_Apk  :1007-1041
-//   C  omega; r = s - omega t; x += alpha y + omega z; rt = r/A0;
-//      q = y - omega vt; partial |r|^2, |r|_1, <rhat, r>, sum x Vol
+//   C  omega; y = rt + beta q again; r = s - omega t; x += alpha y +
+//      omega z; rt = r/A0; q = y - omega vt; partial |r|^2, |r|_1,
+//      <rhat, r>, sum x Vol
 //                                     update_xkrk                      :1047-1111
 //
 // The matrix is PDE_center's, regenerated per row from the per-edge
@@ -367,9 +369,13 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rel_u(Geo g, const double *rhs, c
 // update_sk, update_xkrk, bicgstab.F90:726-1111); the rounding differs at
 // the last bit per operation.
 //
-// Per interior prism (b = vector width): A reads rt, q, rhat, writes y, vt
-// (5b); B reads rt, vt, writes t (3b); C reads x, y, rt, vt, t, rhat, writes
-// x, rt, q (9b): 17b, against 23b for five passes that store y, z, s.
+// C forms y = rt + beta q again from the rt and q it reads anyway (the same
+// expression on the same values, so the same bits as A's), instead of
+// reading a y that A would have to store.
+//
+// Per interior prism (b = vector width): A reads rt, q, rhat, writes vt
+// (4b); B reads rt, vt, writes t (3b); C reads x, rt, q, vt, t, rhat, writes
+// x, rt, q (9b): 16b, against 23b for five passes that store y, z, s.
 
 // Closes iteration k-1 (norms and the stopping test of bicgstab.F90:544-575,
 // the breakdown guards of :479-490; at k = 0 the norms of :453-469) from C's
@@ -620,12 +626,13 @@ __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T 
 // Stop test of k-1 and beta; y = p/A0 = rt + beta q (y = rt at k = 0),
 // v = A y (get_Apk), vt = v/A0, partial sums of <rhat, v> (:509).
 template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const T *q, const T *rhat, T *y, T *vt,
-                                                       Partials P, SolverState *st, double *xfold, const T *efold) {
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const T *q, const T *rhat, T *vt, Partials P,
+                                                       SolverState *st, double *xfold, const T *efold) {
   if (st->done) return;
   double beta_d = 0.0;
   bool first = false;
   if (!begin_iteration(g, P, st, xfold, efold, sizeof(T) == 4, beta_d, first)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->a_beta = beta_d;
   const T beta = (T)beta_d;
   const T *lv = load_levels<T>(g);
   double acc[1] = {0.0};
@@ -634,17 +641,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const 
       g, rt, q, beta, first, [&](size_t s0) { ld4(rhat + s0, rh); },
       [&](int i, int k0, size_t s0, const Col<T> &C, const T(&yc)[4], T ym, T yp, const T(&y0)[4],
           const T(&y1)[4], const T(&y2)[4]) {
-        T qv[5], yo[4], vo[4];
+        T qv[5], vo[4];
         quot5(C, lv, k0, qv);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const Row1<T> R = row_of(g, lv, C, qv, k0, u);
           const T v = apply_row(g, R, k0, u, yc, ym, yp, y0[u], y1[u], y2[u]);
-          yo[u] = R.in ? yc[u] : T(0);
           vo[u] = R.in ? v / R.A0 : T(0);
           acc[0] += (double)rh[u] * (double)v;
         }
-        st4(y + s0, yo);
         st4(vt + s0, vo);
       });
   write_partials<1>(acc, P.a);
@@ -692,13 +697,14 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const 
 }
 
 // ------------------------------------------------------------- phase C ----
-// omega = <t,s>/<t,t>; z = rt - alpha vt, s = A0 z, r = s - omega t,
-// x += alpha y + omega z (update_xkrk :1047-1111), rt = r/A0, q = y - omega
-// vt; partial |r|^2, |r|_1, <rhat, r>, sum x Vol.  After a reliable update
-// the FP32 correction x starts again from zero (A has folded it).
+// omega = <t,s>/<t,t>; y = rt + beta q (A's y, formed again), z = rt -
+// alpha vt, s = A0 z, r = s - omega t, x += alpha y + omega z (update_xkrk
+// :1047-1111), rt = r/A0, q = y - omega vt; partial |r|^2, |r|_1, <rhat, r>,
+// sum x Vol.  After a reliable update the FP32 correction x starts again
+// from zero (A has folded it).
 template <typename T>
-__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, const T *rhat, const T *y,
-                                                       const T *vt, const T *tv, Partials P, SolverState *st) {
+__global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, const T *rhat, const T *vt,
+                                                       const T *tv, Partials P, SolverState *st) {
   if (st->done) return;
   double tr[2];
   reduce_partials<2>(P.b, gridDim.x, tr);
@@ -707,7 +713,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
     st->c_omega = om;
     st->c_iter = st->a_iter + 1;
   }
-  const T alt = (T)al, omt = (T)om;
+  const T alt = (T)al, omt = (T)om, bet = (T)st->a_beta;
+  const bool first = st->a_iter == 0;
   const T *lv = load_levels<T>(g);
   const bool fresh = st->upd != 0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -728,7 +735,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
     __builtin_amdgcn_sched_barrier(0);
     ld4(vt + s0, vv);
     __builtin_amdgcn_sched_barrier(0);
-    ld4(y + s0, yy);
+    ld4(q + s0, yy);
     __builtin_amdgcn_sched_barrier(0);
     ld4(tv + s0, tt);
     __builtin_amdgcn_sched_barrier(0);
@@ -737,6 +744,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
     Col<T> C;
     load_col(g, i, C);
     quot5(C, lv, k0, qv);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) yy[u] = first ? rr[u] : rr[u] + bet * yy[u];  // y = rt + beta q, as in A
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = k0 + u;
@@ -849,26 +858,25 @@ int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const doubl
 }
 
 template <typename T>
-static int iter_impl(const Geo &g, hipStream_t s, T *x, T *rt, const T *rhat, T *q, T *y, T *vt, T *tv, Partials part,
+static int iter_impl(const Geo &g, hipStream_t s, T *x, T *rt, const T *rhat, T *q, T *vt, T *tv, Partials part,
                      SolverState *st, hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(T));
-  LAUNCH_TIMED(ev, 0, k_bicg_a<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)q, rhat, y, vt, part, st,
+  LAUNCH_TIMED(ev, 0, k_bicg_a<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)q, rhat, vt, part, st,
                (double *)nullptr, (const T *)nullptr);
   LAUNCH_TIMED(ev, 1, k_bicg_b<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)vt, tv, part, st);
-  LAUNCH_TIMED(ev, 2, k_bicg_c<T>, grid, blk, lds, s, g, x, rt, q, rhat, (const T *)y, (const T *)vt, (const T *)tv,
-               part, st);
+  LAUNCH_TIMED(ev, 2, k_bicg_c<T>, grid, blk, lds, s, g, x, rt, q, rhat, (const T *)vt, (const T *)tv, part, st);
   return (int)hipGetLastError();
 }
 
-int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *y,
-                     double *vt, double *tv, Partials part, SolverState *st, hipEvent_t *ev) {
-  return iter_impl<double>(g, s, x, rt, rhat, q, y, vt, tv, part, st, ev);
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *vt,
+                     double *tv, Partials part, SolverState *st, hipEvent_t *ev) {
+  return iter_impl<double>(g, s, x, rt, rhat, q, vt, tv, part, st, ev);
 }
 
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *y,
-                       float *vt, float *tv, Partials part, SolverState *st, hipEvent_t *ev) {
-  return iter_impl<float>(g, s, x, rt, rhat, q, y, vt, tv, part, st, ev);
+int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *vt,
+                       float *tv, Partials part, SolverState *st, hipEvent_t *ev) {
+  return iter_impl<float>(g, s, x, rt, rhat, q, vt, tv, part, st, ev);
 }
 
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st) {
@@ -888,17 +896,17 @@ int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double
 }
 
 int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
-                    const float *rhat, float *q, float *y, float *vt, float *tv, Partials part, SolverState *st,
+                    const float *rhat, float *q, float *vt, float *tv, Partials part, SolverState *st,
                     hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(float));
   LAUNCH_TIMED(ev, 0, k_rel_u, grid, blk, (size_t)0, s, g, rhs, (const double *)x, (const float *)e, rt, rhat, part,
                st);
-  LAUNCH_TIMED(ev, 1, k_bicg_a<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)q, rhat, y, vt, part,
-               st, x, (const float *)e);
+  LAUNCH_TIMED(ev, 1, k_bicg_a<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)q, rhat, vt, part, st,
+               x, (const float *)e);
   LAUNCH_TIMED(ev, 2, k_bicg_b<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)vt, tv, part, st);
-  LAUNCH_TIMED(ev, 3, k_bicg_c<float>, grid, blk, lds, s, g, e, rt, q, rhat, (const float *)y, (const float *)vt,
-               (const float *)tv, part, st);
+  LAUNCH_TIMED(ev, 3, k_bicg_c<float>, grid, blk, lds, s, g, e, rt, q, rhat, (const float *)vt, (const float *)tv,
+               part, st);
   return (int)hipGetLastError();
 }
 

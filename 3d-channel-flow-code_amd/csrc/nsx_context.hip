@@ -53,14 +53,14 @@ struct nsx_ctx {
   double *pool_cell[3][4] = {};  // three time-level sets of (u,v,w,p)
   int set_n = 0;                 // pool set holding time level n
   double *ustar[3] = {}, *uv2[3] = {}, *rhsp = nullptr;
-  double *sr = nullptr, *srhat = nullptr, *sp[2] = {}, *svv[2] = {}, *ss = nullptr, *st_ = nullptr;
+  // FP64 BiCGSTAB vectors: rt = r/A0, rhat, q = (p - omega v)/A0, vt = v/A0, t = A z
+  double *sr = nullptr, *srhat = nullptr, *sq = nullptr, *svt = nullptr, *stv = nullptr;
   SolverState *st_dev = nullptr, *st_host = nullptr;
   Partials part{};
   nsx_solve_info info{};
   int iter_hint = 0;
   // mixed precision: FP32 inner BiCGSTAB vectors and settings
-  float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *fy = nullptr, *fz = nullptr, *fs = nullptr,
-        *ft = nullptr, *fe = nullptr;
+  float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *ft = nullptr, *fe = nullptr;
   double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CASCADE;
@@ -572,9 +572,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     if ((rc = cell(&ctx->ustar[q]))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->uv2[q], nv))) return rc;
   }
-  if ((rc = cell(&ctx->rhsp)) || (rc = cell(&ctx->sr)) || (rc = cell(&ctx->srhat)) || (rc = cell(&ctx->sp[0])) ||
-      (rc = cell(&ctx->sp[1])) || (rc = cell(&ctx->svv[0])) || (rc = cell(&ctx->svv[1])) || (rc = cell(&ctx->ss)) ||
-      (rc = cell(&ctx->st_)))
+  if ((rc = cell(&ctx->rhsp)) || (rc = cell(&ctx->sr)) || (rc = cell(&ctx->srhat)) || (rc = cell(&ctx->sq)) ||
+      (rc = cell(&ctx->svt)) || (rc = cell(&ctx->stv)))
     return rc;
   // the time-level slots point into pool set 0 (n), 1 (stage), 2 (new)
   ctx->set_n = 0;
@@ -590,11 +589,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   for (int id = 0; id < NSLOT; ++id)
     if (ctx->slot[id] && ctx->kind[id] != FK_NONE && !(id >= NSX_F_UFN && id <= NSX_F_PFNP))
       HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, field_elems(ctx, ctx->kind[id]) * sizeof(double), ctx->stream));
-  double *zs[] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2], ctx->rhsp, ctx->sr, ctx->srhat, ctx->sp[0],
-                  ctx->sp[1], ctx->svv[0], ctx->svv[1], ctx->ss, ctx->st_};
+  double *zs[] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2], ctx->rhsp, ctx->sr, ctx->srhat, ctx->sq, ctx->svt,
+                  ctx->stv};
   for (double *p : zs) HIPCHECK(hipMemsetAsync(p, 0, nc * sizeof(double), ctx->stream));
   for (int q = 0; q < 3; ++q) HIPCHECK(hipMemsetAsync(ctx->uv2[q], 0, nv * sizeof(double), ctx->stream));
-  float **fvec[9] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->fy, &ctx->fz, &ctx->fs, &ctx->ft, &ctx->fe};
+  float **fvec[6] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->ft, &ctx->fe};
   for (float **f : fvec) {
     if ((rc = dev_alloc(ctx, f, nc))) return rc;
     HIPCHECK(hipMemsetAsync(*f, 0, nc * sizeof(float), ctx->stream));
@@ -743,7 +742,7 @@ static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, dou
         ev = events(ctx, 6);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sp[0], ctx->sp[1], ctx->svv[0], ctx->svv[1],
+      LAUNCH(launch_bicg_iter(g, s, x, ctx->sr, ctx->srhat, ctx->sq, ctx->svt, ctx->stv,
                               ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);
@@ -823,7 +822,7 @@ static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
           ev = events(ctx, 6);
           if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
         }
-        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fy, ctx->fv, ctx->ft, ctx->part,
+        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->ft, ctx->part,
                                   ctx->st_dev, ev));
         if (ev) {
           const size_t base = (size_t)(ev - &ctx->evpool[0]);
@@ -885,7 +884,7 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
         ev = events(ctx, 8);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fy, ctx->fv, ctx->ft,
+      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->ft,
                              ctx->part, ctx->st_dev, ev));
       if (ev) {
         const size_t base = (size_t)(ev - &ctx->evpool[0]);

@@ -41,6 +41,7 @@ struct SolverState {
   double maxrr[2];   // largest |r| since the last update, by parity   (phase U; reset in A)
   int upd;           // this iteration starts from a true residual     (phase U)
   int pad2;
+  double a_beta;     // beta_k of y = rt + beta q                      (phase A)
 };
 
 int nsx_grid_for(uint32_t n);
@@ -76,11 +77,11 @@ int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const doubl
                      double *rhat, Partials part, SolverState *st, double eps, int max_iters);
 // one iteration, three kernels (A, B, C) on the scaled vectors rt = r/A0,
 // q = (p - omega v)/A0, vt = v/A0, and y, t; x is the solution
-int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *y,
-                     double *vt, double *tv, Partials part, SolverState *st,
+int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *vt,
+                     double *tv, Partials part, SolverState *st,
                      hipEvent_t *ev /* nullable: 6 events, start/stop of A, B, C */);
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *y,
-                       float *vt, float *tv, Partials part, SolverState *st, hipEvent_t *ev);
+int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *vt,
+                       float *tv, Partials part, SolverState *st, hipEvent_t *ev);
 // mixed precision: reset, refinement residual (+ outer test, inner reset), correction
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st);
 int launch_mp_resid(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
@@ -92,7 +93,7 @@ int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double
                     float *rhat32, float *e32, Partials part, SolverState *st, double eps, double delta,
                     int max_iters);
 int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
-                    const float *rhat, float *q, float *y, float *vt, float *tv, Partials part, SolverState *st,
+                    const float *rhat, float *q, float *vt, float *tv, Partials part, SolverState *st,
                     hipEvent_t *ev /* nullable: 8 events, start/stop of U, A, B, C */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

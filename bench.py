@@ -42,9 +42,9 @@ sys.path.insert(0, os.path.join(REPO, "3d-channel-flow-code_amd"))
 METRIC = "prism-cell-steps/s: mixed-prec vs FP64 GPU at ≤1e-8 rel err; % HBM peak"
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per interior prism of each BiCGSTAB kernel, in units of
-# the vector width (8 B FP64, 4 B FP32): A reads rt,q,rhat, writes y,vt; B
-# reads rt,vt, writes t; C reads x,y,rt,vt,t,rhat, writes x,rt,q
-PHASE_UNITS = {"a": 5, "b": 3, "c": 9}
+# the vector width (8 B FP64, 4 B FP32): A reads rt,q,rhat, writes vt; B
+# reads rt,vt, writes t; C reads x,rt,q,vt,t,rhat, writes x,rt,q
+PHASE_UNITS = {"a": 4, "b": 3, "c": 9}
 KERNEL_NAMES = {"a": "k_bicg_a", "b": "k_bicg_b", "c": "k_bicg_c"}
 # workload tag used in file names of the profiles: quads in x, quads in y, layers
 TAG = {0: "32x16x32", 1: "128x64x64", 2: "256x128x96", 3: "64x32x48", 4: "512x256x128"}
