@@ -296,6 +296,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init_state(SolverState *st, 
     st->mp_done = 0;
     st->delta = delta;
     st->upd = 0;
+    st->ureq[0] = st->ureq[1] = 0;
   }
 }
 
@@ -773,6 +774,286 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
   write_partials<4>(acc, P.c);
 }
 
+// ------------------------------------------------ mixed-precision CG ----
+// The inner solver of NSX_REFINE_CG: Jacobi-preconditioned conjugate
+// gradients in FP32 on the correction e, with reliable updates (Clark et al.
+// 2010) against the FP64 solution x.  PDE_center's operator is symmetric
+// (up to the last bit) and negative semi-definite with the constants as its
+// null space, and the mean-removed right-hand side lies in its range, so CG
+// with M = diag(A) applies (PCG on -A with -M).  Unlike FP32 BiCGSTAB, whose
+// residual peaks on this anisotropic operator push the recursive residual
+// away from the true one, FP32 CG with reliable updates converges in the
+// iteration count of FP64 CG.
+//
+// Start (once per solve, k_cg_start): the norms of b and of r0 = b - A x0
+// from k_resid's partials, the reference's no-solve test, and r0 again with
+// rho_0 = <r0, M^-1 r0> into P.u.  One iteration is then two kernels:
+//
+//   W  after a true residual: x += e and the reference's stopping rule on
+//      it.  Otherwise, from X's partials: when |r| < delta * max|r| since
+//      the last update or the recursive residual meets the stopping rule,
+//      ask this iteration's X for a true residual.  beta = rho_k /
+//      rho_{k-1}; p = rt + beta p at the seven points; w = A p; <p, w>
+//   X  alpha = rho_k / <p, w>; p formed again (same bits as W's); e += alpha
+//      p (from zero after a true residual); then either the recursive
+//      residual, r = A0 rt - alpha w, rt = r/A0, partial rho_{k+1} = <r, rt>,
+//      |r|^2, |r|_1 into P.c; or, when W asked for it, the true residual
+//      r = b - A (x + e) in FP64 with the new e formed at the seven points
+//      from the stored vectors (the same expressions as the threads that
+//      store it), rt = r/A0, the same partials and sum (x+e) Vol into P.u.
+//      The true-residual pass reads e, rt, p at neighbouring prisms, so it
+//      writes the new e, rt, p to a second set of buffers (e_b, rt_b, p_b),
+//      which W and X of the next iteration read; that X writes the first
+//      set again.  W never asks right after a true residual, so two
+//      consecutive iterations never both take this pass.
+//
+// Per interior prism W reads rt, p and writes w (3 x 4 B), X reads e, rt, p,
+// w and writes e, rt, p (7 x 4 B): 40 B per iteration, against 128 B of an
+// FP64 BiCGSTAB iteration, which does two products with A.
+
+// the true residual of x + e at interior prism (i,k), with e(q) given by a
+// function of the slot; `res` and the prism's x + e returned
+template <class E>
+__device__ inline void true_residual(const Geo &g, const double *rhs, const double *x, double avec, int i, int k,
+                                     E &&eq, double &res, double &xs) {
+  const size_t s = (size_t)i * g.NS + k;
+  const double b = rhs[s] - avec * g.area[i] * g.dsig[k];
+  const Row A = ppe_row(g, i, k);
+  const int ee = i * 3;
+  const size_t n0 = (size_t)g.nbr[ee] * g.NS + k, n1 = (size_t)g.nbr[ee + 1] * g.NS + k,
+               n2 = (size_t)g.nbr[ee + 2] * g.NS + k;
+  xs = x[s] + (double)eq(s, true);
+  const double Ax = A.A0 * xs + A.a1 * (x[n0] + (double)eq(n0, true)) + A.a2 * (x[n1] + (double)eq(n1, true)) +
+                    A.a3 * (x[n2] + (double)eq(n2, true)) + A.AT * (x[s + 1] + (double)eq(s + 1, k + 1 <= g.NZ - 2)) +
+                    A.AB * (x[s - 1] + (double)eq(s - 1, k - 1 >= 1));
+  res = b - Ax;
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_cg_start(Geo g, const double *rhs, const double *x, float *rt,
+                                                         Partials P, SolverState *st) {
+  // the norms of b and r0 = b - A x0 (k_resid's partials) and the
+  // reference's no-solve test (bicgstab.F90:453-469)
+  double t[5];
+  reduce_partials<5>(P.init, gridDim.x, t);
+  double b2 = sqrt(t[0]), b1 = t[2];
+  const double r2 = sqrt(t[1]), r1 = t[3];
+  if (b2 == 0) b2 = 1.0;
+  if (b1 == 0) b1 = 1.0;
+  const double e1 = r1 / b1, e2 = r2 / b2;
+  const bool stop = e1 < st->eps || e2 < st->eps;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->b1 = b1;
+    st->b2 = b2;
+    st->e1 = e1;
+    st->e2 = e2;
+    st->xvol = t[4];
+    if (stop) {
+      st->exit_code = EXIT_NO_SOLVE;
+      st->done = 1;
+    }
+  }
+  if (stop) return;
+  const double avec = st->avec;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    double res, xs;
+    true_residual(g, rhs, x, avec, i, k, [](size_t, bool) { return 0.f; }, res, xs);
+    const Row A = ppe_row(g, i, k);
+    const float rtf = (float)(res / A.A0);
+    rt[(size_t)i * g.NS + k] = rtf;
+    acc[0] += res * (double)rtf;
+    acc[1] += res * res;
+    acc[2] += fabs(res);
+    acc[3] += xs * (g.area[i] * g.dsig[k]);
+  }
+  write_partials<4>(acc, P.u);
+}
+
+// whether iteration k starts from a true residual (the start, or X of k-1)
+__device__ inline bool cg_fresh(const SolverState *st, int k) { return k == 0 || st->ureq[(k - 1) & 1]; }
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_cg_w(Geo g, const float *rt, const float *p, const float *rt_b,
+                                                     const float *p_b, const float *e_b, float *w, double *x,
+                                                     Partials P, SolverState *st) {
+  if (st->done) return;
+  const int k_it = st->c_iter;
+  const bool upd = cg_fresh(st, k_it);
+  const bool alt = upd && k_it > 0;  // X of k-1 took the true-residual pass into the second set
+  double rho;
+  int stop = 0, code = EXIT_CONVERGED, req = 0;
+  if (upd) {
+    double t[4];
+    reduce_partials<4>(P.u, (uint32_t)g.grid, t);  // the start and X run on the solver grid
+    rho = t[0];
+    const double r2 = sqrt(t[1]), e1 = t[2] / st->b1, e2 = r2 / st->b2;
+    // x + e is the solution the true residual belongs to (e = 0 at k = 0)
+    if (alt) {
+      FOR_EACH_CELL(g, c) {
+        int i, k;
+        cell_of(g, c, i, k);
+        const size_t s = (size_t)i * g.NS + k;
+        x[s] = x[s] + (double)e_b[s];
+      }
+    }
+    if (!isfinite(t[0]) || !isfinite(t[1])) {
+      stop = 1;
+      code = EXIT_BREAKDOWN;
+    } else if (e1 < st->eps || e2 < st->eps) {
+      stop = 1;
+      code = EXIT_CONVERGED;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->e1 = e1;
+      st->e2 = e2;
+      st->xvol = t[3];
+      st->maxrr[k_it & 1] = r2;
+      st->mp_pass += 1;
+    }
+  } else {
+    double t[3];
+    reduce_partials<3>(P.c, (uint32_t)g.grid, t);
+    rho = t[0];
+    const double rN = sqrt(t[1]), e1 = t[2] / st->b1, e2 = rN / st->b2;
+    const double mrr = fmax(st->maxrr[(k_it - 1) & 1], rN);
+    req = e1 < st->eps || e2 < st->eps || rN < st->delta * mrr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->e1 = e1;
+      st->e2 = e2;
+      st->maxrr[k_it & 1] = mrr;
+    }
+  }
+  if (!stop && k_it >= st->max_iters) {
+    stop = 1;
+    code = EXIT_MAX;
+  }
+  if (!stop && (rho == 0.0 || !isfinite(rho))) {
+    stop = 1;
+    code = EXIT_BREAKDOWN;
+  }
+  const bool first = k_it == 0;
+  const double beta_d = first ? 0.0 : rho / st->a_rho[(k_it - 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->a_iter = k_it;
+    st->a_rho[k_it & 1] = rho;
+    st->a_beta = beta_d;
+    st->ureq[k_it & 1] = req;
+    if (stop) {
+      st->exit_code = code;
+      st->done = 1;
+    }
+  }
+  if (stop) return;
+  const float *lv = load_levels<float>(g);
+  double acc[1] = {0.0};
+  for_each_quad_gathered(
+      g, alt ? rt_b : rt, alt ? p_b : p, (float)beta_d, first, [&](size_t) {},
+      [&](int i, int k0, size_t s0, const Col<float> &C, const float(&pc)[4], float pm, float pp,
+          const float(&p0)[4], const float(&p1)[4], const float(&p2)[4]) {
+        float qv[5], wo[4];
+        quot5(C, lv, k0, qv);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+          wo[u] = apply_row(g, R, k0, u, pc, pm, pp, p0[u], p1[u], p2[u]);
+          acc[0] += (double)pc[u] * (double)wo[u];
+        }
+        st4(w + s0, wo);
+      });
+  write_partials<1>(acc, P.a);
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, float *p, float *e_b, float *rt_b,
+                                                     float *p_b, const float *w, const double *rhs, const double *x,
+                                                     Partials P, SolverState *st) {
+  if (st->done) return;
+  double tr[1];
+  reduce_partials<1>(P.a, (uint32_t)g.grid_cg, tr);  // W runs on the CG grid
+  const int k_it = st->a_iter;
+  const double al = st->a_rho[k_it & 1] / tr[0];
+  if (tr[0] == 0.0 || !isfinite(al)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->exit_code = EXIT_BREAKDOWN;
+      st->done = 1;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->b_alpha = al;
+    st->c_iter = k_it + 1;
+  }
+  const float alf = (float)al, bet = (float)st->a_beta;
+  const bool first = k_it == 0, fresh = cg_fresh(st, k_it);
+  if (st->ureq[k_it & 1]) {
+    // the true residual of x + e_{k+1}; e_{k+1} and p_k at any interior slot
+    // are formed from the stored vectors exactly as the thread that stores
+    // them forms them below (ghost slots keep their stored e)
+    auto pn = [&](size_t q) { return first ? rt[q] : rt[q] + bet * p[q]; };
+    auto en = [&](size_t q, bool in) { return in ? (fresh ? 0.f : e[q]) + alf * pn(q) : e[q]; };
+    const double avec = st->avec;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    FOR_EACH_CELL(g, c) {
+      int i, k;
+      cell_of(g, c, i, k);
+      const size_t s = (size_t)i * g.NS + k;
+      double res, xs;
+      true_residual(g, rhs, x, avec, i, k, en, res, xs);
+      const Row A = ppe_row(g, i, k);
+      const float rtf = (float)(res / A.A0);
+      acc[0] += res * (double)rtf;
+      acc[1] += res * res;
+      acc[2] += fabs(res);
+      acc[3] += xs * (g.area[i] * g.dsig[k]);
+      e_b[s] = en(s, true);
+      p_b[s] = pn(s);
+      rt_b[s] = rtf;
+    }
+    write_partials<4>(acc, P.u);
+    return;
+  }
+  const float *lv = load_levels<float>(g);
+  const float *rt_in = fresh && !first ? rt_b : rt, *p_in = fresh && !first ? p_b : p;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for_each_quad(g, [&](int i, int k0, size_t s0) {
+    float ee[4], rr[4], pp[4], ww[4], qv[5];
+    if (fresh) {
+      ee[0] = ee[1] = ee[2] = ee[3] = 0.f;
+    } else {
+      ld4(e + s0, ee);
+    }
+    ld4(rt_in + s0, rr);
+    ld4(p_in + s0, pp);
+    ld4(w + s0, ww);
+    Col<float> C;
+    load_col(g, i, C);
+    quot5(C, lv, k0, qv);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pp[u] = first ? rr[u] : rr[u] + bet * pp[u];  // p as W formed it
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+      const bool in = R.in;
+      const float en = ee[u] + alf * pp[u];
+      const float rn = R.A0 * rr[u] - alf * ww[u];
+      const float rtn = rn / R.A0;
+      ee[u] = in ? en : ee[u];
+      pp[u] = in ? pp[u] : 0.f;
+      rr[u] = in ? rtn : 0.f;
+      if (in) {
+        const double rd = (double)rn;
+        acc[0] += rd * (double)rtn;
+        acc[1] += rd * rd;
+        acc[2] += fabs(rd);
+      }
+    }
+    st4(e + s0, ee);
+    st4(rt + s0, rr);
+    st4(p + s0, pp);
+  });
+  write_partials<3>(acc, P.c);
+}
+
 // ---------------------------------------------------------- refinement ----
 // After a refinement residual: the outer norms (true FP64 residual of the
 // current x against the mean-removed rhs, the reference's stopping rule
@@ -879,6 +1160,32 @@ int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const f
   return iter_impl<float>(g, s, x, rt, rhat, q, vt, tv, part, st, ev);
 }
 
+int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *rt,
+                   float *scratch, float *e, Partials part, SolverState *st, double eps, double delta, int max_iters) {
+  const dim3 grid(g.grid), blk(NSX_BLOCK);
+  hipLaunchKernelGGL(k_bicg_init_state, dim3(1), blk, 0, s, st, eps, max_iters, delta);
+  // r0 = b - A x0 and its norms (P.init), e = 0, x = x0; rt, rho_0 into P.u
+  hipLaunchKernelGGL(k_resid<float>, grid, blk, 0, s, g, rhs, phi_in, x, rt, scratch, e, part, part.init,
+                     (double *)nullptr, phi_in != x ? 1 : 0, st);
+  hipLaunchKernelGGL(k_cg_start, grid, blk, 0, s, g, rhs, (const double *)x, rt, part, st);
+  return (int)hipGetLastError();
+}
+
+int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *const v[7], Partials part,
+                   SolverState *st, hipEvent_t *ev) {
+  // v: e, rt, p, w, e_b, rt_b, p_b.  The stencil kernel W is latency-bound and
+  // runs with every block its occupancy allows; the streaming kernel X runs
+  // on the solver grid (more concurrent streams slow it down)
+  float *e = v[0], *rt = v[1], *p = v[2], *w = v[3], *e_b = v[4], *rt_b = v[5], *p_b = v[6];
+  const dim3 grid(g.grid), grid_w(g.grid_cg), blk(NSX_BLOCK);
+  const size_t lds = level_lds(g, sizeof(float));
+  LAUNCH_TIMED(ev, 0, k_cg_w, grid_w, blk, lds, s, g, (const float *)rt, (const float *)p, (const float *)rt_b,
+               (const float *)p_b, (const float *)e_b, w, x, part, st);
+  LAUNCH_TIMED(ev, 1, k_cg_x, grid, blk, lds, s, g, e, rt, p, e_b, rt_b, p_b, (const float *)w, rhs,
+               (const double *)x, part, st);
+  return (int)hipGetLastError();
+}
+
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st) {
   hipLaunchKernelGGL(k_bicg_init_state, dim3(1), dim3(NSX_BLOCK), 0, s, st, 0.0, 0, 0.0);
   return (int)hipGetLastError();
@@ -955,6 +1262,16 @@ int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState
 // partial-sum chain reaches (they all run on one grid, g.grid, so that a
 // consumer sums exactly its producer's partials); the grid is sized so that
 // all of its blocks are resident at once, without a second, partial round.
+// resident blocks per CU of the FP32 CG stencil kernel
+int cg_blocks_per_cu(const Geo &g) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cg_w, NSX_BLOCK, level_lds(g, 4)) !=
+          hipSuccess ||
+      n < 1)
+    return -1;
+  return n;
+}
+
 int solver_blocks_per_cu(const Geo &g) {
   struct K {
     const void *f;
@@ -964,7 +1281,8 @@ int solver_blocks_per_cu(const Geo &g) {
       {(const void *)k_bicg_c<double>, level_lds(g, 8)}, {(const void *)k_bicg_a<float>, level_lds(g, 4)},
       {(const void *)k_bicg_b<float>, level_lds(g, 4)},  {(const void *)k_bicg_c<float>, level_lds(g, 4)},
       {(const void *)k_resid<double>, 0},                 {(const void *)k_resid<float>, 0},
-      {(const void *)k_rel_u, 0},
+      {(const void *)k_rel_u, 0},                         {(const void *)k_cg_start, 0},
+      {(const void *)k_cg_x, level_lds(g, 4)},
   };
   int best = 1 << 30;
   for (const K &k : ks) {

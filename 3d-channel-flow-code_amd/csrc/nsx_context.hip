@@ -61,6 +61,7 @@ struct nsx_ctx {
   int iter_hint = 0;
   // mixed precision: FP32 inner BiCGSTAB vectors and settings
   float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *ft = nullptr, *fe = nullptr;
+  float *fe2 = nullptr, *fr2 = nullptr, *fp2 = nullptr;  // FP32 CG: the second set of e, rt, p
   double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CASCADE;
@@ -68,12 +69,15 @@ struct nsx_ctx {
   double eps_inner = 1e-5;
   int max_passes = 30;
   int iter_hint32 = 0;
+  double delta_cg = 0.1;  // reliable-update fraction of NSX_REFINE_CG
+  int iter_hint_cg = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  Timer t_k[3], t_stage, t_k32[3], t_u;  // FP64 A, B, C; non-solver stage kernels; FP32 A, B, C; U
-  std::vector<std::pair<size_t, int>> pending;  // (event index, timer: 0..2 FP64 solver, 5 stage, 6..8 FP32, 11 U)
+  Timer t_k[3], t_stage, t_k32[3], t_u, t_cg[2];  // FP64 A, B, C; stage kernels; FP32 A, B, C; U; FP32 CG W, X
+  // (event index, timer: 0..2 FP64 solver, 5 stage, 6..8 FP32, 11 U, 12..13 FP32 CG)
+  std::vector<std::pair<size_t, int>> pending;
 };
 
 static int nsx_fail(nsx_ctx *ctx, int code, const char *fmt, ...) {
@@ -328,6 +332,14 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     G = G / NSX_NXCD * NSX_NXCD;
     if (G < NSX_NXCD) G = NSX_NXCD;
     g.grid = (int)G;
+    const int per_cu_cg = cg_blocks_per_cu(g);
+    if (per_cu_cg < 1) return nsx_fail(ctx, -2, "occupancy query of the CG kernels failed");
+    long Gc = (long)prop.multiProcessorCount * per_cu_cg;
+    if (Gc > need) Gc = need;
+    if (Gc > MAXGRID) Gc = MAXGRID;
+    Gc = Gc / NSX_NXCD * NSX_NXCD;
+    if (Gc < NSX_NXCD) Gc = NSX_NXCD;
+    g.grid_cg = (int)Gc;
   }
   if (NS > 4096) return nsx_fail(ctx, -1, "NZ = %d exceeds the 4094 layers the solver's LDS level table holds", NZ);
   g.bctop = ctx->bctop;
@@ -593,7 +605,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
                   ctx->stv};
   for (double *p : zs) HIPCHECK(hipMemsetAsync(p, 0, nc * sizeof(double), ctx->stream));
   for (int q = 0; q < 3; ++q) HIPCHECK(hipMemsetAsync(ctx->uv2[q], 0, nv * sizeof(double), ctx->stream));
-  float **fvec[6] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->ft, &ctx->fe};
+  float **fvec[9] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->ft, &ctx->fe, &ctx->fe2, &ctx->fr2, &ctx->fp2};
   for (float **f : fvec) {
     if ((rc = dev_alloc(ctx, f, nc))) return rc;
     HIPCHECK(hipMemsetAsync(*f, 0, nc * sizeof(float), ctx->stream));
@@ -708,8 +720,8 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[12] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr, nullptr, &ctx->t_stage,
-                  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], nullptr, nullptr, &ctx->t_u};
+  Timer *t[14] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr, nullptr, &ctx->t_stage, &ctx->t_k32[0],
+                  &ctx->t_k32[1], &ctx->t_k32[2], nullptr,        nullptr, &ctx->t_u,     &ctx->t_cg[0], &ctx->t_cg[1]};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (t[pr.second] && hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -926,8 +938,71 @@ static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in,
   return 0;
 }
 
+// Mixed precision, NSX_REFINE_CG: FP32 conjugate gradients on the correction
+// with reliable updates against the FP64 solution (k_cg_start, k_cg_w/x); the solve
+// stops when the true FP64 residual meets the reference's rule.  A run that
+// reaches the iteration cap or breaks down leaves x at its last update, and
+// the reference's FP64 BiCGSTAB finishes from there.
+static int solve_cg(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  // CG does one product with A per iteration, BiCGSTAB two
+  const int cap = 2 * ctx->MaxIters + 100;
+  LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                        ctx->delta_cg, cap));
+  float *const v[7] = {ctx->fe, ctx->fr, ctx->fp, ctx->fv, ctx->fe2, ctx->fr2, ctx->fp2};
+  int issued = 0;
+  const size_t pend0 = ctx->pending.size();
+  const int hint = ctx->iter_hint_cg;
+  int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      hipEvent_t *ev = nullptr;
+      if (ctx->timing) {
+        ev = events(ctx, 4);
+        if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+      }
+      LAUNCH(launch_cg_iter(g, s, rhs, x, v, ctx->part, ctx->st_dev, ev));
+      if (ev) {
+        const size_t base = (size_t)(ev - &ctx->evpool[0]);
+        for (int q = 0; q < 2; ++q) ctx->pending.push_back({base + 2 * q, 12 + q});
+      }
+      ++issued;
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (ctx->st_host->done) break;
+    if (issued > cap + 4) return nsx_fail(ctx, -4, "solver did not stop after %d iterations", issued);
+    chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
+  }
+  const SolverState &h = *ctx->st_host;
+  // launches queued after the stop returned at once: keep the timing of
+  // the iterations that ran (W of the stopping iteration included)
+  const size_t keep = pend0 + 2 * (size_t)h.a_iter + 1;
+  if (ctx->timing && ctx->pending.size() > keep) ctx->pending.resize(keep);
+  if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
+    collect_timing(ctx);
+    const int its32 = h.a_iter, ups = h.mp_pass;
+    if (its32 > 0) ctx->iter_hint_cg = its32;
+    return finish_fp64(ctx, rhs, x, its32, ups);
+  }
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  ctx->info.iters = h.a_iter;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.e1;
+  ctx->info.e2norm = h.e2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += h.a_iter;
+  ctx->info.outer_passes = h.mp_pass;
+  ctx->info.iters_fp32 = h.a_iter;
+  if (h.a_iter > 0) ctx->iter_hint_cg = h.a_iter;
+  collect_timing(ctx);
+  return 0;
+}
+
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
   return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
 }
@@ -1322,7 +1397,8 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   if (!ctx) return -1;
   if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
-  if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE)
+  if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE &&
+      refine != NSX_REFINE_CG)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
   if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
@@ -1330,6 +1406,8 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   ctx->refine = refine;
   if (refine == NSX_REFINE_RELIABLE)
     ctx->delta = tol;
+  else if (refine == NSX_REFINE_CG)
+    ctx->delta_cg = tol;
   else
     ctx->eps_inner = tol;
   ctx->max_passes = max_passes;
@@ -1353,6 +1431,9 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
   }
   if (!strcmp(name, "stage")) t = &ctx->t_stage;
   if (!strcmp(name, "rel_u")) t = &ctx->t_u;
+  static const char *names_cg[2] = {"cg32_w", "cg32_x"};
+  for (int q = 0; q < 2; ++q)
+    if (!strcmp(name, names_cg[q])) t = &ctx->t_cg[q];
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -1362,6 +1443,6 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
   for (int q = 0; q < 3; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
-  ctx->t_stage = ctx->t_u = Timer{};
+  ctx->t_stage = ctx->t_u = ctx->t_cg[0] = ctx->t_cg[1] = Timer{};
   return 0;
 }

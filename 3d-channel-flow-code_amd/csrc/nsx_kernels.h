@@ -42,10 +42,12 @@ struct SolverState {
   int upd;           // this iteration starts from a true residual     (phase U)
   int pad2;
   double a_beta;     // beta_k of y = rt + beta q                      (phase A)
+  int ureq[2];       // FP32 CG: W_k asks U_{k+1} for a true residual, by parity of k (phase W)
 };
 
 int nsx_grid_for(uint32_t n);
 int solver_blocks_per_cu(const Geo &g);
+int cg_blocks_per_cu(const Geo &g);
 
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
@@ -95,5 +97,11 @@ int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double
 int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
                     const float *rhat, float *q, float *vt, float *tv, Partials part, SolverState *st,
                     hipEvent_t *ev /* nullable: 8 events, start/stop of U, A, B, C */);
+// mixed precision, FP32 CG with reliable updates: start, then W, X per iteration;
+// v = e, rt, p, w, e_b, rt_b, p_b (the second set takes the true-residual pass)
+int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *rt,
+                   float *scratch, float *e, Partials part, SolverState *st, double eps, double delta, int max_iters);
+int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *const v[7], Partials part,
+                   SolverState *st, hipEvent_t *ev /* nullable: 4 events, start/stop of W, X */);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

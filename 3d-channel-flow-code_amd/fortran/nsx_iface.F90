@@ -27,7 +27,8 @@
 !     precision and refinement (enum nsx_precision, enum nsx_refine)
       integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1
       integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
-           NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2
+           NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2,              &
+           NSX_REFINE_CG = 3
 
       type, bind(C) :: nsx_cfl_info
         real(c_double) :: umax, vmax, wmax, cfl, aveu, avev, avew

@@ -24,6 +24,7 @@ PREC_MIXED = 1
 REFINE_RELIABLE = 0
 REFINE_RESTART = 1
 REFINE_CASCADE = 2
+REFINE_CG = 3
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 

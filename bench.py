@@ -84,7 +84,12 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
             "s_per_step": step_s}
 
 
-MIXED_MODES = {"mixed": (2, 1e-5), "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
+MIXED_MODES = {"mixed": (2, 1e-5), "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1),
+               "mixed_cg": (3, 0.1)}
+# the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
+# writes e, rt, p (units of 4 B per interior prism; the few X launches that
+# form a true FP64 residual move more)
+CG_UNITS = {"w": 3, "x": 7}
 
 
 def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
@@ -121,6 +126,13 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
                 kernels[prefix + ph] = {"kernel": KERNEL_NAMES[ph] + ("<float>" if width == 4 else "<double>"),
                                         "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                                         "GBps": b / (avg * 1e-3) / 1e9}
+    for ph, units in CG_UNITS.items():
+        ms, launches = ctx.timer("cg32_" + ph)
+        if launches:
+            avg = ms / launches
+            b = units * 4 * n
+            kernels["cg32_" + ph] = {"kernel": "k_cg_" + ph, "launches": launches, "avg_us": avg * 1e3,
+                                     "bytes_per_launch": b, "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")
     fields = {f: np.asarray(ctx.get(f))[:case.N_CELL0, 1:-1].copy() for f in ("UFN", "VFN", "WFN", "PFN")}
     # glob_cfl on the device: the largest net cell flux (cfl.F90:131-167) and the CFL number
@@ -214,9 +226,12 @@ def main() -> int:
         out[name] = {k: v for k, v in r.items() if k != "fields"}
     if args.max_iters:
         out["capped_max_iters"] = args.max_iters
-    if "fp64" in res and "mixed" in res:
-        out["mixed_over_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
-        fa, fb = res["fp64"]["fields"], res["mixed"]["fields"]
+    for name in res:
+        if "fp64" not in res or not name.startswith("mixed"):
+            continue
+        sfx = "" if name == "mixed" else name[5:]
+        out["mixed_over_fp64" + sfx] = res[name]["value"] / res["fp64"]["value"]
+        fa, fb = res["fp64"]["fields"], res[name]["fields"]
         err = {}
         for f, nm in (("UFN", "u"), ("VFN", "v"), ("WFN", "w"), ("PFN", "p")):
             err[nm] = float(np.linalg.norm(fb[f] - fa[f]) / max(np.linalg.norm(fa[f]), 1e-300))
@@ -224,7 +239,7 @@ def main() -> int:
         err["max_div_mixed"] = fb["div"]
         err["cfl_fp64"] = fa["cfl"]
         err["cfl_mixed"] = fb["cfl"]
-        out["rel_err"] = err
+        out["rel_err" + sfx] = err
     if not args.no_cpu:
         base = res.get("fp64", head)
         cb = cpu_baseline(case, base["mean_iters_per_solve"])

@@ -202,16 +202,23 @@ int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
  *                        e = 0 on A e = b - A x to min(e1,e2) < tol (e.g.
  *                        1e-5), then x += e; at most max_passes passes;
  *   NSX_REFINE_CASCADE   one such FP32 pass, then the FP64 BiCGSTAB of
- *                        NSX_PREC_FP64 started from its result. */
-enum nsx_refine { NSX_REFINE_RELIABLE = 0, NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2 };
+ *                        NSX_PREC_FP64 started from its result;
+ *   NSX_REFINE_CG        FP32 Jacobi-preconditioned conjugate gradients on
+ *                        the correction, whose residual is replaced by the
+ *                        true FP64 residual whenever it has fallen by tol
+ *                        (delta, e.g. 0.1); max_passes unused.  The operator
+ *                        is symmetric negative semi-definite, and CG's
+ *                        smooth residual keeps FP32 from stalling. */
+enum nsx_refine { NSX_REFINE_RELIABLE = 0, NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2, NSX_REFINE_CG = 3 };
 int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
 
 /* ---- timing ---------------------------------------------------------------- */
 /* per-kernel device time accumulated with HIP events since the last reset;
  * names: "bicg_a", "bicg_b", "bicg_c" (the three BiCGSTAB kernels in FP64,
  * iterations that ran only), "bicg32_a" .. "bicg32_c" (the same kernels of the
- * FP32 solve), "rel_u" (the reliable-update kernel) and "stage" (all
- * non-solver kernels of a projection stage) */
+ * FP32 solve), "rel_u" (the reliable-update kernel), "cg32_w", "cg32_x"
+ * (the two kernels of an FP32 CG iteration) and "stage" (all non-solver
+ * kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);
 int nsx_timing_reset(nsx_ctx *ctx);

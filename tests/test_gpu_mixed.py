@@ -16,7 +16,7 @@ import pytest
 
 import oracle
 from common import interior, ppe_residual, rel_err
-from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_RELIABLE, REFINE_RESTART
+from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_RELIABLE, REFINE_RESTART
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -27,7 +27,8 @@ def _ctx(case):
     return Context(case)
 
 
-MODES = {"reliable": (REFINE_RELIABLE, 0.1), "restart": (REFINE_RESTART, 1e-5), "cascade": (REFINE_CASCADE, 1e-5)}
+MODES = {"reliable": (REFINE_RELIABLE, 0.1), "restart": (REFINE_RESTART, 1e-5), "cascade": (REFINE_CASCADE, 1e-5),
+         "cg": (REFINE_CG, 0.1)}
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
