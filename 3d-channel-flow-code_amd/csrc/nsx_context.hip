@@ -64,7 +64,7 @@ struct nsx_ctx {
   float *fe2 = nullptr, *fr2 = nullptr, *fp2 = nullptr;  // FP32 CG: the second set of e, rt, p
   double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
   int precision = NSX_PREC_FP64;
-  int refine = NSX_REFINE_CASCADE;
+  int refine = NSX_REFINE_CG;
   double delta = 0.1;
   double eps_inner = 1e-5;
   int max_passes = 30;

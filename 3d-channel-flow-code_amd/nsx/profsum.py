@@ -5,7 +5,10 @@
     python 3d-channel-flow-code_amd/nsx/profsum.py traffic FETCH_DIR WRITE_DIR N_COPY
         per-kernel HBM traffic of two --pmc runs (FETCH_SIZE, then WRITE_SIZE,
         KiB per dispatch), each scaled by the factor that makes the k_copy
-        dispatch of N_COPY doubles read and write 8 N_COPY bytes
+        dispatch of N_COPY doubles read and write 8 N_COPY bytes; the median
+        over a kernel's dispatches, so that the few that differ in kind (a
+        solver launch queued after the stop returns at once, the rare FP32 CG
+        update launch that forms a true residual) do not shift it
 
 The traffic summary ends with one machine-readable line per kernel,
 `traffic: <kernel> <fetch bytes> <write bytes> <dispatches>`, which bench.py
@@ -18,6 +21,8 @@ import glob
 import os
 import sys
 from collections import defaultdict
+
+import numpy as np
 
 
 def _csvs(d: str, suffix: str):
@@ -64,11 +69,12 @@ def traffic(fetch_dir: str, write_dir: str, n_copy: int) -> str:
     w_scale = copy_bytes / (sum(wr["k_copy"]) / len(wr["k_copy"]) * 1024.0)
     out = [f"calibration: k_copy of {n_copy} doubles ({copy_bytes / 1e6:.1f} MB each way): "
            f"FETCH_SIZE x {f_scale:.4f}, WRITE_SIZE x {w_scale:.4f}", "",
-           "| kernel | dispatches | fetch MB/dispatch | write MB/dispatch |", "|---|---|---|---|"]
+           "| kernel | dispatches | fetch MB/dispatch (median) | write MB/dispatch (median) |", "|---|---|---|---|"]
     lines = []
-    for k in sorted(set(fe) & set(wr), key=lambda k: -sum(fe[k]) / max(len(fe[k]), 1)):
-        f = sum(fe[k]) / len(fe[k]) * 1024.0 * f_scale
-        w = sum(wr[k]) / len(wr[k]) * 1024.0 * w_scale
+    med = lambda v: float(np.median(np.asarray(v, dtype=float)))
+    for k in sorted(set(fe) & set(wr), key=lambda k: -med(fe[k])):
+        f = med(fe[k]) * 1024.0 * f_scale
+        w = med(wr[k]) * 1024.0 * w_scale
         out.append(f"| {k} | {len(fe[k])} | {f / 1e6:.1f} | {w / 1e6:.1f} |")
         lines.append(f"traffic: {k} {f:.0f} {w:.0f} {len(fe[k])}")
     return "\n".join(out + [""] + lines)

@@ -191,10 +191,10 @@ int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
 /* ---- precision form (no counterpart in the reference) ------------------------ */
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
  * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs
- * the BiCGSTAB vectors, SpMV and preconditioner in FP32 and keeps the solution,
+ * the Krylov vectors, SpMV and preconditioner in FP32 and keeps the solution,
  * the refinement residual, inner products and norms in FP64; the solve stops
  * when the true FP64 residual meets the reference's rule with eps
- * (nsx_set_params).  Refinement:
+ * (nsx_set_params).  Refinement (default NSX_REFINE_CG, delta 0.1):
  *   NSX_REFINE_RELIABLE  one FP32 iteration whose residual is replaced by the
  *                        true FP64 residual whenever it has fallen by tol
  *                        (delta, e.g. 0.1); max_passes unused;
