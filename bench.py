@@ -26,7 +26,9 @@ Prints one JSON line:
   roofline     the dominant kernel of the mixed leg: algorithmic bytes per
                launch / mean launch time (HIP events on the library's stream)
   cpu_baseline the oracle (CPU restatement of the reference, 1 core) on a
-               bounded sample of the same workload, extrapolated per step
+               bounded sample of the same workload, extrapolated per step;
+               cpu_baseline_all_cores the same with the oracle's solver on
+               OMP_NUM_THREADS threads
 """
 from __future__ import annotations
 
@@ -59,14 +61,17 @@ WORKLOAD = {
 }
 
 
-def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
+def cpu_baseline(case, mean_iters: float, cap: int = 20, parallel: bool = False) -> dict:
     """One projection stage of the oracle on the full mesh with BiCGSTAB capped
     at `cap` iterations; the step time is extrapolated with the GPU's mean
-    iteration count per solve (FP64 leg)."""
+    iteration count per solve (FP64 leg).  parallel: the oracle built with
+    OpenMP (the solver's loops over OMP_NUM_THREADS threads; the rest of the
+    stage, 3 % of the serial step, stays on one thread)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    O = oracle.Oracle(case)
+    O = oracle.Oracle(case, parallel=parallel)
+    cores = O.threads()
     O.set_max_iters(cap)
     for n in ("uf", "vf", "wf", "pf"):
         O.state(n)[:] = O.state(n[0] + "fn")
@@ -78,8 +83,9 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20) -> dict:
     t_iter = t["solve"] / its
     t_rest = t["hydro"] - t["solve"]
     step_s = case.params.FinRK * (t_rest + mean_iters * t_iter)
-    return {"value": case.n_interior / step_s, "unit": "prism-cell-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C restatement of the reference, 1 thread) on the full mesh: one projection stage "
+    what = "1 thread" if not parallel else f"OpenMP, {cores} threads on the solver"
+    return {"value": case.n_interior / step_s, "unit": "prism-cell-steps/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle (C restatement of the reference, {what}) on the full mesh: one projection stage "
                        f"with BiCGSTAB capped at {its} iterations ({wall:.1f} s); step time = "
                        f"{case.params.FinRK} x (non-solver {t_rest:.2f} s + {mean_iters:.0f} iterations x "
                        f"{t_iter * 1e3:.1f} ms) with the FP64 GPU leg's mean iteration count"),
@@ -248,6 +254,9 @@ def main() -> int:
         cb = cpu_baseline(case, base["mean_iters_per_solve"])
         out["cpu_baseline"] = {k: v for k, v in cb.items() if k != "s_per_step"}
         out["gpu_over_cpu"] = out["value"] / cb["value"]
+        cbp = cpu_baseline(case, base["mean_iters_per_solve"], parallel=True)
+        out["cpu_baseline_all_cores"] = {k: v for k, v in cbp.items() if k != "s_per_step"}
+        out["gpu_over_cpu_all_cores"] = out["value"] / cbp["value"]
     print(json.dumps(out), flush=True)
     return 0
 

@@ -67,6 +67,24 @@ typedef struct orc {
 
 enum { TAG_P = 0, TAG_U = 1, TAG_V = 2, TAG_W = 3 };
 
+/* The timing build (-fopenmp, liboracle_omp.so) splits the solver's loops
+ * over threads; its sums then run in another order, so only the serial
+ * build (no -fopenmp: these pragmas vanish) is the bit-exact checker. */
+#ifdef _OPENMP
+#include <omp.h>
+#define ORC_PAR _Pragma("omp parallel for collapse(2) schedule(static)")
+#else
+#define ORC_PAR
+#endif
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 static double *dalloc(size_t n) {
   double *p = calloc(n ? n : 1, sizeof(double));
   if (!p) abort();
@@ -108,6 +126,9 @@ void orc_bcpcenter(orc *o, double *phi) {
     for (int i = 0; i < N0; ++i) phi[CI(i, NZ - 1)] = phi[CI(i, NZ - 2)];
   if (o->bcbot == 1)
     for (int i = 0; i < N0; ++i) phi[CI(i, 0)] = phi[CI(i, 1)];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int k = 0; k < NZ; ++k)
     for (int i = 0; i < N0; ++i) {
       if (o->nbe[i] == 0) continue;
@@ -490,6 +511,9 @@ void orc_update_rhsp(orc *o, double *rhs, double *p, double *pv) {
 static void glob_norm(orc *o, double *f, int tags) {
   const int N0 = o->N0, NZ = o->NZ;
   double local_sum = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : local_sum)
+#endif
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i) {
       double Vol = o->areaCell[i] * o->dsig[k];
@@ -497,6 +521,7 @@ static void glob_norm(orc *o, double *f, int tags) {
       else local_sum = f[CI(i, k)] * Vol + local_sum;
     }
   double avec = local_sum / (o->TotalVolume * (o->sigFin - o->sigIni));
+  ORC_PAR
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i) {
       if (tags == 1) f[CI(i, k)] = f[CI(i, k)] - avec * o->areaCell[i] * o->dsig[k];
@@ -506,6 +531,9 @@ static void glob_norm(orc *o, double *f, int tags) {
 
 static double innerprod(orc *o, const double *a, const double *b) {
   double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : s)
+#endif
   for (int k = 1; k < o->NZ - 1; ++k)
     for (int i = 0; i < o->N0; ++i) s = s + a[CI(i, k)] * b[CI(i, k)];
   return s;
@@ -515,6 +543,7 @@ static double innerprod(orc *o, const double *a, const double *b) {
 static void get_apk(orc *o, const double *A[6], double *out, double *x) {
   const int N0 = o->N0, NZ = o->NZ;
   orc_bcpcenter(o, x);
+  ORC_PAR
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i) {
       size_t c = C0(i, k);
@@ -533,6 +562,9 @@ int orc_bicgstab_solve(orc *o, const double *A[6], double *rhs, double *phi) {
   double *r0bar = W, *vk = W + nz, *sk = W + 2 * nz, *tk = W + 3 * nz, *yk = W + 4 * nz, *rk = W + 5 * nz,
          *pk = W + 6 * nz, *zk = W + 7 * nz;
   double b2 = 0.0, r2 = 0.0, b1 = 0.0, r1 = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : b2, r2, b1, r1)
+#endif
   for (int k = 1; k < NZ - 1; ++k)
     for (int i = 0; i < N0; ++i) {
       size_t c = C0(i, k);
@@ -567,13 +599,16 @@ int orc_bicgstab_solve(orc *o, const double *A[6], double *rhs, double *phi) {
     }
     rhok = innerprod(o, r0bar, rk);
     if (iter == 0) {
+      ORC_PAR
       for (int k = 1; k < NZ - 1; ++k)
         for (int i = 0; i < N0; ++i) pk[CI(i, k)] = rk[CI(i, k)];
     } else {
       double beta = (rhok / rhokm) * (alpha / omega);
+      ORC_PAR
       for (int k = 1; k < NZ - 1; ++k)
         for (int i = 0; i < N0; ++i) pk[CI(i, k)] = rk[CI(i, k)] + beta * (pk[CI(i, k)] - omega * vk[CI(i, k)]);
     }
+    ORC_PAR
     for (int k = 1; k < NZ - 1; ++k)
       for (int i = 0; i < N0; ++i) yk[CI(i, k)] = pk[CI(i, k)] / A[0][C0(i, k)];
     get_apk(o, A, vk, yk);
@@ -583,8 +618,10 @@ int orc_bicgstab_solve(orc *o, const double *A[6], double *rhs, double *phi) {
       break;
     }
     alpha = rhok / r0v;
+    ORC_PAR
     for (int k = 1; k < NZ - 1; ++k)
       for (int i = 0; i < N0; ++i) sk[CI(i, k)] = rk[CI(i, k)] - alpha * vk[CI(i, k)];
+    ORC_PAR
     for (int k = 1; k < NZ - 1; ++k)
       for (int i = 0; i < N0; ++i) zk[CI(i, k)] = sk[CI(i, k)] / A[0][C0(i, k)];
     get_apk(o, A, tk, zk);
@@ -593,6 +630,9 @@ int orc_bicgstab_solve(orc *o, const double *A[6], double *rhs, double *phi) {
     /* update_xkrk, bicgstab.F90:1047-1111 */
     r2 = 0.0;
     r1 = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : r2, r1)
+#endif
     for (int k = 1; k < NZ - 1; ++k)
       for (int i = 0; i < N0; ++i) {
         size_t c = CI(i, k);

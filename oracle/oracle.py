@@ -14,28 +14,32 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "liboracle.so")
+# the same source built with -fopenmp: the solver's loops split over threads
+# (sums in another order), for the all-cores CPU baseline only
+LIB_OMP = os.path.join(BUILD, "liboracle_omp.so")
 SRC = os.path.join(HERE, "nsx_oracle.c")
 
-_lib = None
+_libs = {}
 
 
 def build() -> str:
     os.makedirs(BUILD, exist_ok=True)
-    tmp = LIB + ".tmp"
     # -ffp-contract=off and no -march: plain SSE2 double arithmetic in the
     # reference's operation order, the same as the Fortran build
-    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-Wall",
-                    "-o", tmp, SRC, "-lm"], check=True)
-    os.replace(tmp, LIB)
+    for out, extra in ((LIB, []), (LIB_OMP, ["-fopenmp"])):
+        tmp = out + ".tmp"
+        subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-Wall", *extra,
+                        "-o", tmp, SRC, "-lm"], check=True)
+        os.replace(tmp, out)
     return LIB
 
 
-def lib() -> ctypes.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+def lib(parallel: bool = False) -> ctypes.CDLL:
+    path = LIB_OMP if parallel else LIB
+    if path not in _libs:
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(SRC):
             build()
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path)
         vp, ci, cd, cs = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char_p
         L.orc_create.restype = vp
         L.orc_create.argtypes = [ci] * 5
@@ -59,8 +63,9 @@ def lib() -> ctypes.CDLL:
         L.orc_bicgstab.restype = ci
         L.orc_hydro.restype = ci
         L.orc_step.restype = ci
-        _lib = L
-    return _lib
+        L.orc_threads.restype = ci
+        _libs[path] = L
+    return _libs[path]
 
 
 _BORROW = ["No_cp", "No_vp", "nbe", "nbev", "areaCell", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "doe",
@@ -76,8 +81,8 @@ def _ptr(a: np.ndarray) -> int:
 class Oracle:
     """The reference's step on the CPU, holding its own state (reference layout)."""
 
-    def __init__(self, case, *, p0=None):
-        L = lib()
+    def __init__(self, case, *, p0=None, parallel: bool = False):
+        self.L = L = lib(parallel)
         self.case = case
         c = case
         self.h = L.orc_create(c.N_CELL0, c.N_CELL, c.N_VERT, c.NZ, c.N_CELLexact)
@@ -97,14 +102,17 @@ class Oracle:
         L.orc_init_state(self.h)
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.orc_free(self.h)
+        if getattr(self, "h", None) and getattr(self, "L", None) is not None:
+            self.L.orc_free(self.h)
             self.h = None
+
+    def threads(self) -> int:
+        return self.L.orc_threads()
 
     def state(self, name: str) -> np.ndarray:
         """Live view of a state array, shaped like the Fortran array."""
         c = self.case
-        ptr = lib().orc_state(self.h, name.encode())
+        ptr = self.L.orc_state(self.h, name.encode())
         if not ptr:
             raise KeyError(name)
         if name in ("ufv", "vfv", "wfv", "pfv"):
@@ -117,27 +125,27 @@ class Oracle:
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shape, order="F")
 
     def step(self) -> int:
-        return lib().orc_step(self.h)
+        return self.L.orc_step(self.h)
 
     def hydro(self) -> int:
-        return lib().orc_hydro(self.h)
+        return self.L.orc_hydro(self.h)
 
     def solver_record(self) -> dict:
         iv = (ctypes.c_int * 4)()
         dv = (ctypes.c_double * 2)()
-        lib().orc_solver_record(self.h, iv, dv)
+        self.L.orc_solver_record(self.h, iv, dv)
         return dict(iter=iv[0], n_solves=iv[1], iter_total=iv[2], exit=iv[3], e1=dv[0], e2=dv[1])
 
     def timers(self) -> dict:
         """wall seconds spent in bicgstab and in hydro (solver included)"""
         dv = (ctypes.c_double * 2)()
-        lib().orc_timers(self.h, dv)
+        self.L.orc_timers(self.h, dv)
         return dict(solve=dv[0], hydro=dv[1])
 
     def set_max_iters(self, n: int) -> None:
         c = self.case
         p = c.params
-        lib().orc_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, n, p.FinRK, p.Iturbu, p.TopBC,
+        self.L.orc_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, n, p.FinRK, p.Iturbu, p.TopBC,
                              p.BotBC, c.bctop, c.bcbot, c.TotalVolume, p.sigIni, p.sigFin)
 
     # -- single routines on caller arrays (reference layout, modified in place)
@@ -145,7 +153,7 @@ class Oracle:
         args = [self.h] + [_ptr(a) for a in arrays]
         if tagdir is not None:
             args.append(tagdir)
-        return getattr(lib(), "orc_" + name)(*args)
+        return getattr(self.L, "orc_" + name)(*args)
 
     CFL_KEYS = ("umax", "vmax", "wmax", "cfl", "aveu", "avev", "avew", "flux_total", "flux_max", "flux_bc",
                 "flux_bot", "flux_top", "flux_side")
@@ -157,7 +165,7 @@ class Oracle:
         v = self.state("vfn") if v is None else np.asfortranarray(v)
         w = self.state("wfn") if w is None else np.asfortranarray(w)
         out = np.zeros(13)
-        lib().orc_glob_cfl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(out))
+        self.L.orc_glob_cfl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(out))
         return dict(zip(self.CFL_KEYS, out.tolist()))
 
     def cell(self):

@@ -247,3 +247,28 @@ def test_glob_cfl_closed_form():
     per_cell = [0.5 * c.params.dt * (np.abs(xnn * dl).sum(axis=1) * dsig[k]) / (area * dsig[k])
                 for k in range(1, nz - 1)]
     assert abs(got["cfl"] - max(p.max() for p in per_cell)) <= 1e-14 * got["cfl"]
+
+
+def test_parallel_oracle_build_matches_serial():
+    """The OpenMP build of the oracle (all-cores CPU baseline only) solves the
+    same pressure system as the serial checker: its sums run in another
+    order, so the iteration count may move a little and the solution agrees
+    to the solver's tolerance."""
+    import oracle as orc
+    c = config_case(0, jitter=0.1)
+    rng = np.random.default_rng(3)
+    rhs = c.cell_field()
+    rhs[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
+    sols, its = [], []
+    for par in (False, True):
+        O = orc.Oracle(c, parallel=par)
+        phi = c.cell_field()
+        its.append(O.call("bicgstab", phi, O.vert(), rhs.copy(order="F")))
+        sols.append(phi[:c.N_CELL0, 1:-1].copy())
+        if par:
+            assert O.threads() >= 1
+    assert abs(its[1] - its[0]) <= 0.1 * its[0], its
+    a, b = sols
+    a = a - a.mean()
+    b = b - b.mean()
+    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
