@@ -23,7 +23,7 @@ OUT = os.path.join(HERE, "nsx", "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NSX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["nsx_context.hip", "nsx_stage.hip", "nsx_bicgstab.hip", "nsx_diag.hip"]
+HIP_SOURCES = ["nsx_context.hip", "nsx_stage.hip", "nsx_bicgstab.hip", "nsx_diag.hip", "nsx_stats.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # arithmetic order of the reference: no contraction into FMA, so the

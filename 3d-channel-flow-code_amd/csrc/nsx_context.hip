@@ -63,6 +63,10 @@ struct nsx_ctx {
   float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *ft = nullptr, *fe = nullptr;
   float *fe2 = nullptr, *fr2 = nullptr, *fp2 = nullptr;  // FP32 CG: the second set of e, rt, p
   double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
+  // channel statistics (stats.F90), allocated by nsx_sm_init
+  double *sm[SM_N] = {};
+  double *sm_planes = nullptr;  // SM_NPLANE x NZ
+  int sm_n = 0;
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CG;
   double delta = 0.1;
@@ -1389,6 +1393,93 @@ extern "C" int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out
   out->flux_bot = r[10];
   out->flux_top = r[11];
   out->flux_side = r[12];
+  return 0;
+}
+
+// ------------------------------------------------------------ statistics --
+// sm_init (stats.F90:34-84): the accumulators of the channel statistics,
+// allocated on first use (twelve cell fields), set to zero; sample count 0
+extern "C" int nsx_sm_init(nsx_ctx *ctx) {
+  NEED_COMMIT(ctx);
+  const size_t nc = (size_t)ctx->N0 * ctx->NS;
+  int rc;
+  for (int q = 0; q < SM_N; ++q) {
+    if (!ctx->sm[q] && (rc = dev_alloc(ctx, &ctx->sm[q], nc))) return rc;
+    HIPCHECK(hipMemsetAsync(ctx->sm[q], 0, nc * sizeof(double), ctx->stream));
+  }
+  if (!ctx->sm_planes && (rc = dev_alloc(ctx, &ctx->sm_planes, (size_t)SM_NPLANE * ctx->NZ))) return rc;
+  ctx->sm_n = 0;
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// glob_sample (stats.F90:91-197) on resident cell fields; the eddy viscosity
+// term comes from NSX_F_EDDY (the reference's KeyLES build)
+extern "C" int nsx_glob_sample(nsx_ctx *ctx, int u, int v, int w, int p) {
+  NEED_COMMIT(ctx);
+  if (!ctx->sm[0]) return nsx_fail(ctx, -1, "nsx_glob_sample: nsx_sm_init has not been called");
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL) || kind_ok(ctx, p, FK_CELL))
+    return -1;
+  StatsBufs sb;
+  for (int q = 0; q < SM_N; ++q) sb.a[q] = ctx->sm[q];
+  LAUNCH(launch_glob_sample(ctx->g, ctx->stream, ctx->slot[u], ctx->slot[v], ctx->slot[w], ctx->slot[p],
+                           ctx->slot[NSX_F_EDDY], sb));
+  ctx->sm_n += 1;
+  return 0;
+}
+
+extern "C" int nsx_sm_count(nsx_ctx *ctx, int *n) {
+  NEED_COMMIT(ctx);
+  if (!n) return nsx_fail(ctx, -1, "nsx_sm_count: n is NULL");
+  *n = ctx->sm_n;
+  return 0;
+}
+
+// one accumulator (NSX_SM_*) into host(N_CELL,NZ), rows 1..N_CELL0
+extern "C" int nsx_sm_get(nsx_ctx *ctx, int q, double *host) {
+  NEED_COMMIT(ctx);
+  if (q < 0 || q >= SM_N) return nsx_fail(ctx, -1, "nsx_sm_get: no accumulator %d", q);
+  if (!ctx->sm[q]) return nsx_fail(ctx, -1, "nsx_sm_get: nsx_sm_init has not been called");
+  if (!host) return nsx_fail(ctx, -1, "nsx_sm_get: host is NULL");
+  const int N0 = ctx->N0, NZ = ctx->NZ, NS = ctx->NS;
+  const size_t NC = (size_t)ctx->NC;
+  std::vector<double> buf((size_t)N0 * NS);
+  HIPCHECK(hipMemcpyAsync(buf.data(), ctx->sm[q], buf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < NZ; ++k)
+    for (int i = 0; i < N0; ++i) host[(size_t)i + (size_t)k * NC] = buf[(size_t)i * NS + k];
+  return 0;
+}
+
+// the plane sums of glob_put_plane_mm (stats.F90:266-279), computed on the
+// device in the reference's order: planes[q*NZ + k-1] = sum over i of
+// sm_q(i,k)/ns for the nine velocity moments and k = 2..NZ-1, 0 at k = 1, NZ
+extern "C" int nsx_sm_planes(nsx_ctx *ctx, double *planes, int *ns) {
+  NEED_COMMIT(ctx);
+  if (!ctx->sm[0]) return nsx_fail(ctx, -1, "nsx_sm_planes: nsx_sm_init has not been called");
+  if (!planes) return nsx_fail(ctx, -1, "nsx_sm_planes: planes is NULL");
+  StatsBufs sb;
+  for (int q = 0; q < SM_N; ++q) sb.a[q] = ctx->sm[q];
+  const size_t n = (size_t)SM_NPLANE * ctx->NZ;
+  HIPCHECK(hipMemsetAsync(ctx->sm_planes, 0, n * sizeof(double), ctx->stream));
+  LAUNCH(launch_plane_sums(ctx->g, ctx->stream, sb, ctx->sm_n, ctx->sm_planes));
+  HIPCHECK(hipMemcpyAsync(planes, ctx->sm_planes, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  if (ns) *ns = ctx->sm_n;
+  return 0;
+}
+
+// glob_put_plane_mm (stats.F90:204-391): plane sums on the device, then the
+// three files sm-, srms-, stau-NNNNN.dat in dir (the reference's
+// ../output/Stats)
+extern "C" int nsx_glob_put_plane_mm(nsx_ctx *ctx, const char *dir) {
+  NEED_COMMIT(ctx);
+  if (!dir) return nsx_fail(ctx, -1, "nsx_glob_put_plane_mm: dir is NULL");
+  std::vector<double> planes((size_t)SM_NPLANE * ctx->NZ);
+  int ns = 0, rc;
+  if ((rc = nsx_sm_planes(ctx, planes.data(), &ns))) return rc;
+  if (nsx_write_plane_stats(planes.data(), ctx->NZ, ctx->N0, ns, ctx->Re, dir))
+    return nsx_fail(ctx, -5, "nsx_glob_put_plane_mm: cannot write the statistics files in %s", dir);
   return 0;
 }
 

@@ -70,6 +70,17 @@ int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 
 int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0);
 int launch_copy(hipStream_t s, size_t n, const double *src, double *dst);
 // glob_cfl (nsx_diag.hip): partial rows per block, and the 13 outputs into out
+// channel statistics (stats.F90): the twelve accumulators of glob_sample,
+// in the order of module stats; the first nine are glob_put_plane_mm's
+enum { SM_UF = 0, SM_VF, SM_WF, SM_U2F, SM_V2F, SM_W2F, SM_UVF, SM_VWF, SM_WUF, SM_PF, SM_P2F, SM_EF, SM_N };
+enum { SM_NPLANE = 9 };
+struct StatsBufs {
+  double *a[SM_N];
+};
+int launch_glob_sample(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w,
+                       const double *p, const double *eddy /* nullable */, StatsBufs sm);
+int launch_plane_sums(const Geo &g, hipStream_t s, StatsBufs sm, int ns, double *out /* SM_NPLANE x NZ */);
+
 int glob_cfl_rows();
 int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w,
                     const double *const face[5], double *part, double *out);

@@ -29,6 +29,11 @@
       integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
            NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2,              &
            NSX_REFINE_CG = 3
+!     accumulators of the channel statistics (enum nsx_sm, module stats)
+      integer(c_int), parameter :: NSX_SM_UF = 0, NSX_SM_VF = 1,        &
+           NSX_SM_WF = 2, NSX_SM_U2F = 3, NSX_SM_V2F = 4, NSX_SM_W2F = 5,&
+           NSX_SM_UVF = 6, NSX_SM_VWF = 7, NSX_SM_WUF = 8, NSX_SM_PF = 9,&
+           NSX_SM_P2F = 10, NSX_SM_EF = 11
 
       type, bind(C) :: nsx_cfl_info
         real(c_double) :: umax, vmax, wmax, cfl, aveu, avev, avew
@@ -247,6 +252,58 @@
           type(c_ptr), value :: ctx
           integer(c_int), value :: u, v, w
           type(nsx_cfl_info) :: info
+        end function
+        integer(c_int) function nsx_sm_init(ctx)                         &
+             bind(C, name="nsx_sm_init")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function nsx_glob_sample(ctx, u, v, w, p)         &
+             bind(C, name="nsx_glob_sample")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: u, v, w, p
+        end function
+        integer(c_int) function nsx_sm_count(ctx, n)                     &
+             bind(C, name="nsx_sm_count")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int) :: n
+        end function
+        integer(c_int) function nsx_sm_get(ctx, which, host)             &
+             bind(C, name="nsx_sm_get")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: which
+          real(c_double) :: host(*)
+        end function
+        integer(c_int) function nsx_sm_planes(ctx, planes, ns)           &
+             bind(C, name="nsx_sm_planes")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          real(c_double) :: planes(*)
+          integer(c_int) :: ns
+        end function
+        integer(c_int) function nsx_glob_put_plane_mm(ctx, dir)          &
+             bind(C, name="nsx_glob_put_plane_mm")
+          import :: c_int, c_ptr, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), dimension(*) :: dir
+        end function
+        integer(c_int) function nsx_write_plane_stats(planes, nz,        &
+             n_cell0, ns, re, dir) bind(C, name="nsx_write_plane_stats")
+          import :: c_int, c_double, c_char
+          real(c_double) :: planes(*)
+          integer(c_int), value :: nz, n_cell0, ns
+          real(c_double), value :: re
+          character(kind=c_char), dimension(*) :: dir
+        end function
+        integer(c_int) function nsx_format_real(x, kind, w, d, out,      &
+             outlen) bind(C, name="nsx_format_real")
+          import :: c_int, c_double, c_char
+          real(c_double), value :: x
+          integer(c_int), value :: kind, w, d, outlen
+          character(kind=c_char), dimension(*) :: out
         end function
         integer(c_int) function nsx_set_precision(ctx, precision, refine,&
              tol, max_passes) bind(C, name="nsx_set_precision")

@@ -36,7 +36,12 @@ EXPORTS = [
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
     "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
+    "nsx_write_plane_stats", "nsx_format_real",
 ]
+# the accumulators of module stats (nsx_sm_get)
+SM_NAMES = ["sm_uf", "sm_vf", "sm_wf", "sm_u2f", "sm_v2f", "sm_w2f", "sm_uvf", "sm_vwf", "sm_wuf", "sm_pf", "sm_p2f",
+            "sm_ef"]
 
 
 class SolveInfo(ctypes.Structure):
@@ -96,6 +101,14 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
         "nsx_timing_reset": [c_vp],
+        "nsx_sm_init": [c_vp],
+        "nsx_glob_sample": [c_vp, c_int, c_int, c_int, c_int],
+        "nsx_sm_count": [c_vp, P(c_int)],
+        "nsx_sm_get": [c_vp, c_int, c_vp],
+        "nsx_sm_planes": [c_vp, c_vp, P(c_int)],
+        "nsx_glob_put_plane_mm": [c_vp, c_cp],
+        "nsx_write_plane_stats": [c_vp, c_int, c_int, c_int, c_dbl, c_cp],
+        "nsx_format_real": [c_dbl, c_int, c_int, c_int, c_cp, c_int],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

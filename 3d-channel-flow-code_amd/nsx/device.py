@@ -240,6 +240,44 @@ class Context:
         self._check(self.lib.nsx_glob_cfl(self.h, *ids, ctypes.byref(out)))
         return out.as_dict()
 
+    # -- channel statistics (stats.F90)
+    def sm_init(self) -> None:
+        """sm_init (stats.F90:34-84) on the device accumulators."""
+        self._check(self.lib.nsx_sm_init(self.h))
+
+    def glob_sample(self, u=None, v=None, w=None, p=None) -> None:
+        """glob_sample (stats.F90:91-197): resident UFN, VFN, WFN, PFN by
+        default, or the given (N_CELL,NZ) arrays; the eddy viscosity term is
+        the resident NSX_F_EDDY."""
+        if u is None:
+            ids = (F["UFN"], F["VFN"], F["WFN"], F["PFN"])
+        else:
+            ids = self._cells(u, v, w, p)
+        self._check(self.lib.nsx_glob_sample(self.h, *ids))
+
+    def sm_count(self) -> int:
+        n = ctypes.c_int()
+        self._check(self.lib.nsx_sm_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def sm_get(self, name: str) -> np.ndarray:
+        """One accumulator (sm_uf .. sm_ef) as (N_CELL,NZ); rows past N_CELL0 are 0."""
+        from .abi import SM_NAMES
+        out = np.zeros((self.case.N_CELL, self.case.NZ), order="F")
+        self._check(self.lib.nsx_sm_get(self.h, SM_NAMES.index(name), out.ctypes.data))
+        return out
+
+    def sm_planes(self):
+        """(planes (9, NZ), ns): glob_put_plane_mm's plane sums (stats.F90:266-279)."""
+        out = np.zeros(9 * self.case.NZ)
+        ns = ctypes.c_int()
+        self._check(self.lib.nsx_sm_planes(self.h, out.ctypes.data, ctypes.byref(ns)))
+        return out.reshape(9, self.case.NZ), ns.value
+
+    def glob_put_plane_mm(self, directory: str) -> None:
+        """glob_put_plane_mm (stats.F90:204-391): the three statistics files in directory."""
+        self._check(self.lib.nsx_glob_put_plane_mm(self.h, directory.encode()))
+
     def set_max_iters(self, n: int) -> None:
         """MaxIters of the pressure solve (the case's other parameters kept)."""
         p = self.case.params

@@ -188,6 +188,40 @@ typedef struct nsx_cfl_info {
 } nsx_cfl_info;
 int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
 
+/* ---- channel statistics (stats.F90) ---------------------------------------- */
+/* The twelve accumulators of module stats (module_stats.F90), resident on the
+ * device as cell fields. */
+enum nsx_sm {
+  NSX_SM_UF = 0, NSX_SM_VF, NSX_SM_WF, NSX_SM_U2F, NSX_SM_V2F, NSX_SM_W2F,
+  NSX_SM_UVF, NSX_SM_VWF, NSX_SM_WUF, NSX_SM_PF, NSX_SM_P2F, NSX_SM_EF
+};
+/* sm_init (stats.F90:34-84): allocate (first call) and zero the accumulators,
+ * sample count 0 */
+int nsx_sm_init(nsx_ctx *ctx);
+/* glob_sample (stats.F90:91-197): add u, v, w, p, their products and the eddy
+ * viscosity (NSX_F_EDDY) of every column i <= N_CELL0 and every layer k = 1..NZ
+ * (cell field ids); the sample count grows by one */
+int nsx_glob_sample(nsx_ctx *ctx, int u, int v, int w, int p);
+/* sm_n of module stats */
+int nsx_sm_count(nsx_ctx *ctx, int *n);
+/* one accumulator NSX_SM_* into host(N_CELL,NZ) (rows 1..N_CELL0 written) */
+int nsx_sm_get(nsx_ctx *ctx, int which, double *host);
+/* the plane sums of glob_put_plane_mm (stats.F90:266-279), in its order of
+ * summation: planes[q*NZ + k-1] = sum_{i=1..N_CELL0} sm_q(i,k)/sm_n for the
+ * nine velocity moments q = NSX_SM_UF..NSX_SM_WUF and k = 2..NZ-1 (0 at k = 1
+ * and NZ); *ns = sm_n */
+int nsx_sm_planes(nsx_ctx *ctx, double *planes, int *ns);
+/* glob_put_plane_mm (stats.F90:204-391): the plane sums, then the files
+ * <dir>/sm-NNNNN.dat (means and the TKE line), srms-NNNNN.dat (rms) and
+ * stau-NNNNN.dat (stresses), NNNNN = sm_n, in the reference's formats */
+int nsx_glob_put_plane_mm(nsx_ctx *ctx, const char *dir);
+/* the file writing of glob_put_plane_mm alone (stats.F90:301-363), on plane
+ * sums laid out as nsx_sm_planes returns them; needs no device */
+int nsx_write_plane_stats(const double *planes, int NZ, int n_cell0, int ns, double Re, const char *dir);
+/* a REAL(8) formatted as the reference's output does: kind 0 = Ew.d edit
+ * descriptor, kind 1 = list-directed (the compiler of the reference build) */
+int nsx_format_real(double x, int kind, int w, int d, char *out, int outlen);
+
 /* ---- precision form (no counterpart in the reference) ------------------------ */
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
  * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs

@@ -49,6 +49,9 @@ typedef struct orc {
   double *ufv, *vfv, *wfv, *pfv;
   double *U1, *U2, *U3, *UT, *UB; /* U1FACE..UBFACE (N_CELL0,NZ) */
   double *rhsuf, *rhsvf, *rhswf, *eddy;
+  /* channel statistics of module stats (module_stats.F90), (N_CELL,NZ) */
+  double *sm_uf, *sm_vf, *sm_wf, *sm_u2f, *sm_v2f, *sm_w2f, *sm_uvf, *sm_vwf, *sm_wuf, *sm_pf, *sm_p2f, *sm_ef;
+  int sm_n;
   /* solver record */
   int last_iter, n_solves, iter_total;
   double last_e1, last_e2;
@@ -862,6 +865,54 @@ int orc_step(orc *o) {
 }
 
 /* ----------------------------------------------------------- set-up ------ */
+/* ------------------------------------------------------- statistics ---- */
+/* sm_init, stats.F90:34-84 (rows 1..N_CELL0, every layer) */
+void orc_sm_init(orc *o) {
+  double *a[12] = {o->sm_uf, o->sm_vf, o->sm_wf, o->sm_u2f, o->sm_v2f, o->sm_w2f,
+                   o->sm_uvf, o->sm_vwf, o->sm_wuf, o->sm_pf, o->sm_p2f, o->sm_ef};
+  for (int q = 0; q < 12; ++q)
+    for (int k = 0; k < o->NZ; ++k)
+      for (int i = 0; i < o->N0; ++i) a[q][CI(i, k)] = 0.0;
+  o->sm_n = 0;
+}
+
+/* glob_sample, stats.F90:91-197 (KeyLES: the eddy term from o->eddy) */
+void orc_glob_sample(orc *o, const double *phiu, const double *phiv, const double *phiw, const double *phip) {
+  for (int k = 0; k < o->NZ; ++k)
+    for (int i = 0; i < o->N0; ++i) {
+      size_t c = CI(i, k);
+      double tu = phiu[c], tv = phiv[c], tw = phiw[c], tp = phip[c];
+      double tp2 = tp * tp, tu2 = tu * tu, tv2 = tv * tv, tw2 = tw * tw;
+      double tuv = tu * tv, tvw = tv * tw, twu = tw * tu;
+      o->sm_uf[c] = o->sm_uf[c] + tu;
+      o->sm_vf[c] = o->sm_vf[c] + tv;
+      o->sm_wf[c] = o->sm_wf[c] + tw;
+      o->sm_pf[c] = o->sm_pf[c] + tp;
+      o->sm_u2f[c] = o->sm_u2f[c] + tu2;
+      o->sm_v2f[c] = o->sm_v2f[c] + tv2;
+      o->sm_w2f[c] = o->sm_w2f[c] + tw2;
+      o->sm_uvf[c] = o->sm_uvf[c] + tuv;
+      o->sm_vwf[c] = o->sm_vwf[c] + tvw;
+      o->sm_wuf[c] = o->sm_wuf[c] + twu;
+      o->sm_p2f[c] = o->sm_p2f[c] + tp2;
+      o->sm_ef[c] = o->sm_ef[c] + o->eddy[c];
+    }
+  o->sm_n = o->sm_n + 1;
+}
+
+/* the plane sums of glob_put_plane_mm, stats.F90:266-279: planes[q*NZ + k]
+ * (k 0-based) for the nine velocity moments, interior layers; returns sm_n */
+int orc_sm_planes(orc *o, double *planes) {
+  const double *a[9] = {o->sm_uf, o->sm_vf, o->sm_wf, o->sm_u2f, o->sm_v2f, o->sm_w2f, o->sm_uvf, o->sm_vwf, o->sm_wuf};
+  const int ns = o->sm_n;
+  for (int q = 0; q < 9; ++q)
+    for (int k = 0; k < o->NZ; ++k) planes[(size_t)q * o->NZ + k] = 0.0;
+  for (int k = 1; k < o->NZ - 1; ++k)
+    for (int i = 0; i < o->N0; ++i)
+      for (int q = 0; q < 9; ++q) planes[(size_t)q * o->NZ + k] = planes[(size_t)q * o->NZ + k] + a[q][CI(i, k)] / ns;
+  return ns;
+}
+
 orc *orc_create(int N0, int NC, int NV, int NZ, int NEX) {
   orc *o = calloc(1, sizeof(orc));
   o->N0 = N0;
@@ -876,6 +927,9 @@ orc *orc_create(int N0, int NC, int NV, int NZ, int NEX) {
   o->ufv = dalloc(nvz); o->vfv = dalloc(nvz); o->wfv = dalloc(nvz); o->pfv = dalloc(nvz);
   o->U1 = dalloc(n0z); o->U2 = dalloc(n0z); o->U3 = dalloc(n0z); o->UT = dalloc(n0z); o->UB = dalloc(n0z);
   o->rhsuf = dalloc(nz); o->rhsvf = dalloc(nz); o->rhswf = dalloc(nz); o->eddy = dalloc(nz);
+  o->sm_uf = dalloc(nz); o->sm_vf = dalloc(nz); o->sm_wf = dalloc(nz); o->sm_u2f = dalloc(nz);
+  o->sm_v2f = dalloc(nz); o->sm_w2f = dalloc(nz); o->sm_uvf = dalloc(nz); o->sm_vwf = dalloc(nz);
+  o->sm_wuf = dalloc(nz); o->sm_pf = dalloc(nz); o->sm_p2f = dalloc(nz); o->sm_ef = dalloc(nz);
   o->FinRK = 2;
   o->Iturbu = 1;
   o->MaxIters = 20000;
@@ -887,7 +941,8 @@ void orc_free(orc *o) {
   if (!o) return;
   double *own[] = {o->ufn, o->vfn, o->wfn, o->pfn, o->uf, o->vf, o->wf, o->pf, o->ufnp, o->vfnp, o->wfnp, o->pfnp,
                    o->ufv, o->vfv, o->wfv, o->pfv, o->U1, o->U2, o->U3, o->UT, o->UB,
-                   o->rhsuf, o->rhsvf, o->rhswf, o->eddy};
+                   o->rhsuf, o->rhsvf, o->rhswf, o->eddy, o->sm_uf, o->sm_vf, o->sm_wf, o->sm_u2f, o->sm_v2f,
+                   o->sm_w2f, o->sm_uvf, o->sm_vwf, o->sm_wuf, o->sm_pf, o->sm_p2f, o->sm_ef};
   for (size_t n = 0; n < sizeof own / sizeof own[0]; ++n) free(own[n]);
   free(o);
 }
@@ -911,7 +966,8 @@ int orc_set_array(orc *o, const char *name, const void *ptr) {
 
 #define STATE(X)                                                                                            \
   X(ufn) X(vfn) X(wfn) X(pfn) X(uf) X(vf) X(wf) X(pf) X(ufnp) X(vfnp) X(wfnp) X(pfnp) X(ufv) X(vfv) X(wfv) \
-  X(pfv) X(U1) X(U2) X(U3) X(UT) X(UB) X(rhsuf) X(rhsvf) X(rhswf) X(eddy)
+  X(pfv) X(U1) X(U2) X(U3) X(UT) X(UB) X(rhsuf) X(rhsvf) X(rhswf) X(eddy) X(sm_uf) X(sm_vf) X(sm_wf)     \
+  X(sm_u2f) X(sm_v2f) X(sm_w2f) X(sm_uvf) X(sm_vwf) X(sm_wuf) X(sm_pf) X(sm_p2f) X(sm_ef)
 
 double *orc_state(orc *o, const char *name) {
 #define GETS(n) \

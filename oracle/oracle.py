@@ -64,6 +64,10 @@ def lib(parallel: bool = False) -> ctypes.CDLL:
         L.orc_hydro.restype = ci
         L.orc_step.restype = ci
         L.orc_threads.restype = ci
+        L.orc_sm_init.argtypes = [vp]
+        L.orc_glob_sample.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_sm_planes.argtypes = [vp, vp]
+        L.orc_sm_planes.restype = ci
         _libs[path] = L
     return _libs[path]
 
@@ -167,6 +171,22 @@ class Oracle:
         out = np.zeros(13)
         self.L.orc_glob_cfl(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(out))
         return dict(zip(self.CFL_KEYS, out.tolist()))
+
+    # -- channel statistics (stats.F90)
+    def sm_init(self) -> None:
+        self.L.orc_sm_init(self.h)
+
+    def glob_sample(self, u=None, v=None, w=None, p=None) -> None:
+        """glob_sample on (u, v, w, p) (default: the state at time level n)"""
+        arrs = [self.state(n) if a is None else np.asfortranarray(a)
+                for n, a in (("ufn", u), ("vfn", v), ("wfn", w), ("pfn", p))]
+        self.L.orc_glob_sample(self.h, *[_ptr(a) for a in arrs])
+
+    def sm_planes(self):
+        """(planes (9, NZ), ns): the plane sums of glob_put_plane_mm"""
+        out = np.zeros(9 * self.case.NZ)
+        ns = self.L.orc_sm_planes(self.h, _ptr(out))
+        return out.reshape(9, self.case.NZ), ns
 
     def cell(self):
         return self.case.cell_field()

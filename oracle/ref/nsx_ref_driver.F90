@@ -13,7 +13,10 @@
 !  mode 1 dumps set-up geometry and single-routine results on the initial
 !  field; mode 2 runs nsteps RK steps, dumps the state after each step and
 !  calls glob_cfl on the final state (its printed diagnostics and meanuv.dat
-!  are the record of that call).
+!  are the record of that call).  Mode 2 also takes the channel statistics
+!  as nsmp3D.F90:303,579-589 does (sm_init, then glob_sample after every
+!  step, glob_put_plane_mm at the end, whose files go to ../output/Stats of
+!  the working directory) and dumps the twelve accumulators.
 
       module nsxdump
       implicit none
@@ -41,6 +44,7 @@
       USE variables
       USE geometry
       USE les
+      USE stats
       USE nsxdump
 !     explicit interfaces of the routines with assumed-shape dummies, as the
 !     reference PROGRAM has them (nsmp3D.F90:55); grandientLSM's entry there
@@ -453,6 +457,8 @@
                            xv,yv,sigv,Hprv,hv,                 &
                            No_cp,nbe)
       flag_ab = 0
+      call alloc_stats_variables
+      call sm_init()
       do istep=1,nsteps
          time  = time + dt
          nstep = nstep + 1
@@ -544,10 +550,24 @@
             write(sfx,'(a,i0)') '_', istep
             call dump_state(trim(sfx))
          endif
+         call glob_sample(ufnp,vfnp,wfnp,pfnp)
       enddo
 !     the CFL / mass-flux diagnostics of nsmp3D.F90:595-606 on the final state
       print*, 'NSX_CFL'
       call glob_cfl(ufn,vfn,wfn,pfn,xc,yc,sig,dsig,No_cp,nbe)
+      call glob_put_plane_mm()
+      call dump_r8('sm_uf', sm_uf, N_CELL*NZ)
+      call dump_r8('sm_vf', sm_vf, N_CELL*NZ)
+      call dump_r8('sm_wf', sm_wf, N_CELL*NZ)
+      call dump_r8('sm_u2f', sm_u2f, N_CELL*NZ)
+      call dump_r8('sm_v2f', sm_v2f, N_CELL*NZ)
+      call dump_r8('sm_w2f', sm_w2f, N_CELL*NZ)
+      call dump_r8('sm_uvf', sm_uvf, N_CELL*NZ)
+      call dump_r8('sm_vwf', sm_vwf, N_CELL*NZ)
+      call dump_r8('sm_wuf', sm_wuf, N_CELL*NZ)
+      call dump_r8('sm_pf', sm_pf, N_CELL*NZ)
+      call dump_r8('sm_p2f', sm_p2f, N_CELL*NZ)
+      call dump_r8('sm_ef', sm_ef, N_CELL*NZ)
       end subroutine
 
       end program

@@ -106,17 +106,24 @@ def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workd
     """Run the reference; returns (dump dict, stdout text)."""
     exe = ensure_built(case.nn)
     with tempfile.TemporaryDirectory(dir=workdir) as td:
+        # the reference writes its statistics to ../output/Stats of its working directory
+        wd = os.path.join(td, "run")
+        stats_dir = os.path.join(td, "output", "Stats")
+        os.makedirs(wd)
+        os.makedirs(stats_dir)
         cf = os.path.join(td, "case.bin")
         of = os.path.join(td, "out.bin")
         write_case(cf, case, mode=mode, nsteps=nsteps, dump_every=dump_every, p0=p0)
-        r = subprocess.run([exe, cf, of], cwd=td, capture_output=True, text=True)
+        r = subprocess.run([exe, cf, of], cwd=wd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference driver failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
         out = r.stdout
-        mu = os.path.join(td, "meanuv.dat")
+        mu = os.path.join(wd, "meanuv.dat")
         if os.path.exists(mu):
             out += "".join("MEANUV " + ln for ln in open(mu))
-        return read_dump(of, case), out
+        d = read_dump(of, case)
+        d["stats_files"] = {fn: open(os.path.join(stats_dir, fn)).read() for fn in sorted(os.listdir(stats_dir))}
+        return d, out
 
 
 CFL_KEYS = {"MAX U": "umax", "MAX V": "vmax", "MAX W": "wmax", "MAX CFL": "cfl", "Total  flux": "flux_total",

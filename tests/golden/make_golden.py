@@ -9,8 +9,11 @@ what they produce:
                   initial field (full arrays, ghost rows included);
   cfg0_steps.json workload 0 (32x16 quads x 32 layers, 10 RK2 steps):
                   per-step SHA-256 of the state arrays, norms, sample values,
-                  the BiCGSTAB iteration count of every pressure solve and
-                  glob_cfl's diagnostics of the final state;
+                  the BiCGSTAB iteration count of every pressure solve,
+                  glob_cfl's diagnostics of the final state, and the channel
+                  statistics (glob_sample after every step): SHA-256 of the
+                  twelve accumulators (rows 1..N_CELL0) and the three files
+                  glob_put_plane_mm wrote at the end;
   cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
@@ -61,6 +64,7 @@ def main():
 
     c, p0 = tiny_case()
     d, log = refrun.run(c, mode=1, p0=p0)
+    d.pop("stats_files", None)  # mode 1 takes no statistics
     out = {k: (v[:rows_of(c, v)] if isinstance(v, np.ndarray) and v.ndim == 2 else v) for k, v in d.items()}
     out["solves"] = np.array([s[0] for s in refrun.parse_solves(log)], dtype=np.int32)
     np.savez_compressed(os.path.join(HERE, "tiny_unit.npz"), **out)
@@ -91,6 +95,12 @@ def main():
         # glob_cfl on the final state: its printed values (17 digits) and the
         # means it appends to meanuv.dat (f12.5)
         rec["glob_cfl"] = refrun.parse_cfl(log)
+        sm = {}
+        for nm in ("sm_uf", "sm_vf", "sm_wf", "sm_u2f", "sm_v2f", "sm_w2f", "sm_uvf", "sm_vwf", "sm_wuf", "sm_pf",
+                   "sm_p2f", "sm_ef"):
+            a = np.ascontiguousarray(d[nm][:c.N_CELL0], dtype=np.float64) + 0.0
+            sm[nm] = {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "l2": float(np.linalg.norm(a))}
+        rec["stats"] = {"sm_n": nsteps, "accumulators": sm, "files": d["stats_files"]}
         with open(os.path.join(HERE, f"cfg{cfg}_steps.json"), "w") as f:
             json.dump(rec, f, indent=1)
         print(f"cfg{cfg}_steps.json:", len(rec["solves"]), "solves")

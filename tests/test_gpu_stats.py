@@ -1,0 +1,104 @@
+"""Channel statistics (stats.F90) on the device against the oracle and the
+reference's own files.
+
+* glob_sample on the oracle's states (u, v, w, p and the eddy viscosity put
+  on the device): the twelve accumulators and the plane sums of
+  glob_put_plane_mm are bit-identical to the oracle's (element-wise sums in
+  the reference's order; plane sums in its sequential order), and the three
+  files are byte-identical to the ones the library's writer makes from the
+  oracle's plane sums.
+* glob_sample after every one of ten device RK2 steps of the 32x16 quads x 32
+  layers channel: the files hold the reference's values (its own files,
+  tests/golden) to 1e-6 relative or 1e-8 absolute, the spread of two
+  trajectories whose pressure solves stop at a 1e-10 residual.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from common import load_steps
+from nsx.abi import SM_NAMES
+from nsx.case import config_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(case):
+    from nsx.device import Context
+    return Context(case)
+
+
+def test_glob_sample_bitwise_vs_oracle():
+    c = config_case(0)
+    O = oracle.Oracle(c)
+    G = _ctx(c)
+    G.init_state()
+    O.sm_init()
+    G.sm_init()
+    for _ in range(3):
+        O.step()
+        for f, s in (("UFN", "ufn"), ("VFN", "vfn"), ("WFN", "wfn"), ("PFN", "pfn"), ("EDDY", "eddy")):
+            G.put(f, O.state(s))
+        O.glob_sample()
+        G.glob_sample()
+    assert G.sm_count() == 3
+    n0 = c.N_CELL0
+    for nm in SM_NAMES:
+        dev, ref = G.sm_get(nm)[:n0], O.state(nm)[:n0]
+        assert np.array_equal(dev, ref), nm
+    pd, nsd = G.sm_planes()
+    po, nso = O.sm_planes()
+    assert nsd == nso == 3
+    assert np.array_equal(pd, po)
+    from nsx._lib import device
+    with tempfile.TemporaryDirectory() as a, tempfile.TemporaryDirectory() as b:
+        G.glob_put_plane_mm(a)
+        assert device().nsx_write_plane_stats(np.ascontiguousarray(po).ctypes.data, c.NZ, n0, nso, c.params.Re,
+                                              b.encode()) == 0
+        names = sorted(os.listdir(a))
+        assert names == ["sm-00003.dat", "srms-00003.dat", "stau-00003.dat"]
+        for fn in names:
+            assert open(os.path.join(a, fn)).read() == open(os.path.join(b, fn)).read(), fn
+    G.close()
+
+
+def _numbers(text):
+    out = []
+    for ln in text.splitlines():
+        ln = ln.replace("TKE =", "")
+        for tok in ln.split():
+            t = tok.replace("E", "e")
+            if t.startswith(".") or t.startswith("-."):
+                t = t.replace(".", "0.", 1)
+            out.append(float(t))
+    return np.array(out)
+
+
+def test_stats_over_device_steps_vs_reference_files():
+    rec = load_steps(0)
+    c = config_case(0)
+    G = _ctx(c)
+    G.init_state()
+    G.sm_init()
+    for _ in range(rec["nsteps"]):
+        G.step(1, 0)
+        G.glob_sample()
+    with tempfile.TemporaryDirectory() as td:
+        G.glob_put_plane_mm(td)
+        for fn, want in rec["stats"]["files"].items():
+            got = open(os.path.join(td, fn)).read()
+            a, b = _numbers(got), _numbers(want)
+            assert a.shape == b.shape, fn
+            assert np.all(np.abs(a - b) <= 1e-6 * np.abs(b) + 1e-8), (fn, np.max(np.abs(a - b)))
+    G.close()
+
+
+def test_sample_needs_init():
+    c = config_case(0)
+    G = _ctx(c)
+    with pytest.raises(RuntimeError):
+        G.glob_sample()
+    G.close()

@@ -4,6 +4,9 @@ The fixtures in tests/golden were written by the reference's own Fortran
 routines (tests/golden/make_golden.py); every comparison here is bit-exact.
 Closed-form checks pin what a fixture could not tell apart by itself.
 """
+import hashlib
+import os
+
 import numpy as np
 import pytest
 
@@ -109,21 +112,78 @@ def test_oracle_routines_bitwise():
 
 @pytest.mark.parametrize("cfg", [0, 3])
 def test_oracle_trajectory_bitwise(cfg):
-    """Whole RK2 steps (hydro x2, average, update) == the reference's, hash for hash."""
+    """Whole RK2 steps (hydro x2, average, update) == the reference's, hash
+    for hash; with glob_sample after every step, the twelve accumulators of
+    the channel statistics too, and the three files glob_put_plane_mm wrote
+    come out byte for byte from the library's writer fed the oracle's plane
+    sums."""
+    import tempfile
+
+    from nsx._lib import device
     rec = load_steps(cfg)
     c = config_case(cfg)
     O = oracle.Oracle(c)
-    iters = []
+    O.sm_init()
     for s in range(1, rec["nsteps"] + 1):
-        for _ in range(2):
-            pass
         before = O.solver_record()["n_solves"]
         O.step()
         assert O.solver_record()["n_solves"] == before + 2
         for nm, want in rec["steps"][str(s)].items():
             assert digest(c, O.state(STATE_MAP[nm])) == want["sha256"], (s, nm)
+        O.glob_sample()
     assert O.solver_record()["iter_total"] == sum(rec["solves"])
     assert O.solver_record()["iter"] == rec["solves"][-1]
+    st = rec["stats"]
+    for nm, want in st["accumulators"].items():
+        a = np.ascontiguousarray(O.state(nm)[:c.N_CELL0], dtype=np.float64) + 0.0
+        assert hashlib.sha256(a.tobytes()).hexdigest() == want["sha256"], nm
+    planes, ns = O.sm_planes()
+    assert ns == st["sm_n"]
+    lib = device()
+    with tempfile.TemporaryDirectory() as td:
+        assert lib.nsx_write_plane_stats(np.ascontiguousarray(planes).ctypes.data, c.NZ, c.N_CELL0, ns, c.params.Re,
+                                         td.encode()) == 0
+        for fn, text in st["files"].items():
+            assert open(os.path.join(td, fn)).read() == text, fn
+
+
+def test_list_directed_real_format():
+    """The library's list-directed REAL(8) formatting (the "TKE =" line of
+    glob_put_plane_mm) against what amdflang, the compiler of the reference
+    build, printed for 394 values (tests/golden/make_list_directed.py)."""
+    import ctypes
+    import json
+
+    from nsx._lib import device
+    with open(os.path.join(os.path.dirname(__file__), "golden", "list_directed.json")) as f:
+        pairs = json.load(f)["pairs"]
+    buf = ctypes.create_string_buffer(64)
+    lib = device()
+    bad = []
+    for v, want in pairs:
+        assert lib.nsx_format_real(float(v), 1, 0, 0, buf, 64) == 0
+        if buf.value.decode() != want:
+            bad.append((v, want, buf.value.decode()))
+    assert not bad, bad[:5]
+
+
+def test_e_edit_format():
+    """The library's Ew.d output (E12.5, the data lines of glob_put_plane_mm)
+    against what amdflang printed for the same values."""
+    import ctypes
+    import json
+
+    from nsx._lib import device
+    with open(os.path.join(os.path.dirname(__file__), "golden", "list_directed.json")) as f:
+        pairs = json.load(f)["e12.5"]
+    buf = ctypes.create_string_buffer(64)
+    lib = device()
+    bad = []
+    for v, want in pairs:
+        assert lib.nsx_format_real(float(v), 0, 12, 5, buf, 64) == 0
+        if buf.value.decode() != want:
+            bad.append((v, want, buf.value.decode()))
+    assert not bad, bad[:5]
 
 
 # ---------------------------------------------------------- closed forms ----
