@@ -211,3 +211,47 @@
       call nsx_dropin_get(NSX_F_CELL_SCRATCH + 1, dfundy)
       call nsx_dropin_get(NSX_F_CELL_SCRATCH + 2, dfundsig)
       END SUBROUTINE grandientLSM
+
+!-----------------------------------------------------------------------
+!  stats.F90:34-84 -- zero the channel statistics (device accumulators;
+!  the three routines below replace stats.F90 as a whole)
+      SUBROUTINE sm_init()
+      use nsx_dropin
+      implicit none
+      call nsx_dropin_check(nsx_sm_init(nsx_ctx), 'sm_init')
+      END SUBROUTINE sm_init
+
+!-----------------------------------------------------------------------
+!  stats.F90:91-197 -- one sample of u, v, w, p and of the eddy viscosity
+!  (the device's resident copy of module les' eddy)
+      SUBROUTINE glob_sample(phiu,phiv,phiw,phip)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(nsx_n_cell,nsx_nz) :: phiu, phiv, phiw, phip
+      integer(c_int) :: n
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH, phiu)
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH + 1, phiv)
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH + 2, phiw)
+      call nsx_dropin_put(NSX_F_CELL_SCRATCH + 3, phip)
+      call nsx_dropin_check(nsx_glob_sample(nsx_ctx, NSX_F_CELL_SCRATCH, &
+           NSX_F_CELL_SCRATCH + 1, NSX_F_CELL_SCRATCH + 2,                &
+           NSX_F_CELL_SCRATCH + 3), 'glob_sample')
+      call nsx_dropin_check(nsx_sm_count(nsx_ctx, n), 'glob_sample')
+      if (mod(n,10) .eq. 0) then
+         print*, '       =========================================='
+         print*, '               STATS SAMPLED',n,'TIMES'
+         print*, '       =========================================='
+      endif
+      END SUBROUTINE glob_sample
+
+!-----------------------------------------------------------------------
+!  stats.F90:204-391 -- plane-averaged statistics into ../output/Stats
+      SUBROUTINE glob_put_plane_mm()
+      use nsx_dropin
+      implicit none
+      integer(c_int) :: n
+      call nsx_dropin_check(nsx_glob_put_plane_mm(nsx_ctx,               &
+           '../output/Stats'//c_null_char), 'glob_put_plane_mm')
+      call nsx_dropin_check(nsx_sm_count(nsx_ctx, n), 'glob_put_plane_mm')
+      print*,'   --- Plane Averaged Stats printed! ---',n
+      END SUBROUTINE glob_put_plane_mm

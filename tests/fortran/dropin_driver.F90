@@ -5,8 +5,11 @@
 !  sets the device up through nsx_dropin_setup and calls the drop-in
 !  subroutines under the reference's names and argument lists: bicgstab on the
 !  given right-hand side, then one hydro stage (flag_ab = 0) from the initial
-!  fields.  Writes phi, phiv, ufnp, vfnp, wfnp, pfnp, ufv, pfv to the output
-!  stream file.  Exit status 3 when no HIP device is usable.
+!  fields; then the channel statistics: sm_init, glob_sample of the initial
+!  fields and of the stage result, glob_put_plane_mm (its files go to
+!  ../output/Stats of the working directory).  Writes phi, phiv, ufnp, vfnp,
+!  wfnp, pfnp, ufv, pfv to the output stream file.  Exit status 3 when no HIP
+!  device is usable.
       program dropin_driver
       use iso_c_binding
       use nsx_iface
@@ -84,6 +87,11 @@
                  xc, xc, sig, dsig, No_cp, nbe,                          &
                  xv, xv, sigv, dsigv, No_vp, nbev,                       &
                  xc, xc, xc, xc, xv, xv, xv, 0)
+
+      call sm_init()
+      call glob_sample(f0(:,:,1), f0(:,:,2), f0(:,:,3), f0(:,:,4))
+      call glob_sample(fnp(:,:,1), fnp(:,:,2), fnp(:,:,3), fnp(:,:,4))
+      call glob_put_plane_mm()
 
       open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
       write(u) phi, phiv, fnp, fv

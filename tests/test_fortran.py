@@ -1,6 +1,7 @@
 """The Fortran host side: the drop-in subroutines (bicgstab, hydro,
-grandientLSM under the reference's names and argument lists, over the
-ISO_C_BINDING interfaces of include/nsx.h) linked into a Fortran program.
+grandientLSM, sm_init, glob_sample, glob_put_plane_mm under the reference's
+names and argument lists, over the ISO_C_BINDING interfaces of include/nsx.h)
+linked into a Fortran program.
 
 CPU: the drop-in library exports the reference's external names and the test
 driver links and runs (without a GPU it stops at nsx_create with its own
@@ -77,7 +78,7 @@ def test_dropin_exports_reference_names():
     if not os.path.exists(FLIB):
         pytest.skip("the Fortran drop-in is not built (no Fortran compiler)")
     out = subprocess.run(["nm", FLIB], capture_output=True, text=True, check=True).stdout
-    for sym in ("bicgstab_", "hydro_", "grandientlsm_"):
+    for sym in ("bicgstab_", "hydro_", "grandientlsm_", "sm_init_", "glob_sample_", "glob_put_plane_mm_"):
         assert f" T {sym}\n" in out, sym
 
 
@@ -87,7 +88,10 @@ def test_driver_links_and_runs(tmp_path):
     c, _ = tiny_case()
     fin, fout = tmp_path / "case.bin", tmp_path / "out.bin"
     write_case(c, _rhs(c), fin)
-    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    (tmp_path / "run").mkdir()
+    (tmp_path / "output" / "Stats").mkdir(parents=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path / "run")
     if r.returncode == 3:
         assert "no usable HIP device" in r.stdout
     else:
@@ -104,9 +108,14 @@ def test_dropin_vs_oracle(tmp_path, which):
     rhs = _rhs(c)
     fin, fout = tmp_path / "case.bin", tmp_path / "out.bin"
     write_case(c, rhs, fin)
+    (tmp_path / "run").mkdir()
+    stats_dir = tmp_path / "output" / "Stats"
+    stats_dir.mkdir(parents=True)
     print(f"dropin_driver {which}", flush=True)
-    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path / "run")
     assert r.returncode == 0, r.stdout + r.stderr
+    assert "Plane Averaged Stats printed!" in r.stdout
     phi, phiv, fnp, fv = read_out(c, fout)
     # bicgstab from phi = 0
     O = oracle.Oracle(c)
@@ -123,3 +132,33 @@ def test_dropin_vs_oracle(tmp_path, which):
         assert rel_err(interior(c, fnp[:, :, q]), interior(c, O2.state(n))) <= tol, n
     for q, (n, tol) in enumerate((("ufv", 1e-8), ("pfv", 1e-6))):
         assert rel_err(fv[:, :, 0 if q == 0 else 3], O2.state(n)) <= tol, n
+    # the statistics of the initial fields and the stage result (the eddy
+    # viscosity of the stage on both sides), written by glob_put_plane_mm
+    O2.sm_init()
+    O2.glob_sample(c.u0, c.v0, c.w0, c.p0)
+    O2.glob_sample(fnp[:, :, 0], fnp[:, :, 1], fnp[:, :, 2], fnp[:, :, 3])
+    planes, ns = O2.sm_planes()
+    from nsx._lib import device
+    want_dir = tmp_path / "want"
+    want_dir.mkdir()
+    assert device().nsx_write_plane_stats(np.ascontiguousarray(planes).ctypes.data, c.NZ, c.N_CELL0, ns, c.params.Re,
+                                          str(want_dir).encode()) == 0
+    names = sorted(os.listdir(stats_dir))
+    assert names == sorted(os.listdir(want_dir)) == ["sm-00002.dat", "srms-00002.dat", "stau-00002.dat"]
+    # the stage velocities agree to 1e-8; rms and stresses are differences of
+    # nearly equal moments, which widens that by the ratio of mean to
+    # fluctuation squared, and the files round to 5 digits
+    for fn in names:
+        a, b = _numbers(open(stats_dir / fn).read()), _numbers(open(want_dir / fn).read())
+        assert a.shape == b.shape and np.all(np.abs(a - b) <= 1e-4 * np.abs(b) + 1e-10), fn
+
+
+def _numbers(text):
+    out = []
+    for ln in text.splitlines():
+        for tok in ln.replace("TKE =", "").split():
+            t = tok.replace("E", "e")
+            if t.startswith(".") or t.startswith("-."):
+                t = t.replace(".", "0.", 1)
+            out.append(float(t))
+    return np.array(out)
