@@ -5,8 +5,8 @@
   libnsx_fortran.a + nsx_iface.mod, nsx_dropin.mod
                   the Fortran host side (fortran/*.F90, amdflang): the
                   ISO_C_BINDING interfaces and the drop-in subroutines
-  tests/fortran/_build/dropin_driver
-                  the Fortran test driver of the drop-in, linked to libnsx.so
+  tests/fortran/_build/dropin_driver, savetec_driver
+                  the Fortran test drivers of the drop-in, linked to libnsx.so
 
 Outputs land in nsx/_build/ so they travel with the source tree.  Run as
 `python build.py [host|hip|all]` or through __graft_entry__.build().
@@ -23,7 +23,8 @@ OUT = os.path.join(HERE, "nsx", "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NSX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["nsx_context.hip", "nsx_stage.hip", "nsx_bicgstab.hip", "nsx_diag.hip", "nsx_stats.hip"]
+HIP_SOURCES = ["nsx_context.hip", "nsx_stage.hip", "nsx_bicgstab.hip", "nsx_diag.hip", "nsx_stats.hip",
+               "nsx_files.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # arithmetic order of the reference: no contraction into FMA, so the
@@ -76,9 +77,10 @@ def build_fortran():
     tdir = os.path.join(REPO, "tests", "fortran", "_build")
     os.makedirs(tdir, exist_ok=True)
     rel = os.path.relpath(OUT, tdir)
-    _run([FC, "-O2", "-J", fdir, "-o", os.path.join(tdir, "dropin_driver"),
-          os.path.join(REPO, "tests", "fortran", "dropin_driver.F90"), lib, "-L", OUT, "-lnsx",
-          f"-Wl,-rpath,$ORIGIN/{rel}"])
+    for drv in ("dropin_driver", "savetec_driver"):
+        _run([FC, "-O2", "-J", fdir, "-o", os.path.join(tdir, drv),
+              os.path.join(REPO, "tests", "fortran", drv + ".F90"), lib, "-L", OUT, "-lnsx",
+              f"-Wl,-rpath,$ORIGIN/{rel}"])
 
 
 def build_all():

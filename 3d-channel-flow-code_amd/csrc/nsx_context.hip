@@ -1483,6 +1483,42 @@ extern "C" int nsx_glob_put_plane_mm(nsx_ctx *ctx, const char *dir) {
   return 0;
 }
 
+// -------------------------------------------------------------- snapshots --
+extern "C" int nsx_savetec_vertex(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt,
+                                  const double *yvt, const double *zvt) {
+  NEED_COMMIT(ctx);
+  if (!dir || !xvt || !yvt || !zvt) return nsx_fail(ctx, -1, "nsx_savetec_vertex: NULL argument");
+  const size_t nv = (size_t)ctx->NV * (ctx->NZ - 1);
+  std::vector<double> f(4 * nv);
+  for (int q = 0; q < 4; ++q) {
+    int rc = nsx_get_field(ctx, NSX_F_UFV + q, f.data() + q * nv, 0);
+    if (rc) return rc;
+  }
+  if (nsx_write_tec_vertex(dir, counter, time, ctx->N0, ctx->NV, ctx->NZ, ctx->No_vp.data(), xvt, yvt, zvt,
+                           f.data(), f.data() + nv, f.data() + 2 * nv, f.data() + 3 * nv))
+    return nsx_fail(ctx, -5, "nsx_savetec_vertex: cannot write V-%04d.tec in %s", counter, dir);
+  return 0;
+}
+
+extern "C" int nsx_savetec_center(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt,
+                                  const double *yvt, const double *zvt, int u, int v, int w, int p) {
+  NEED_COMMIT(ctx);
+  if (!dir || !xvt || !yvt || !zvt) return nsx_fail(ctx, -1, "nsx_savetec_center: NULL argument");
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL) || kind_ok(ctx, p, FK_CELL))
+    return -1;
+  const size_t nc = (size_t)ctx->NC * ctx->NZ;
+  std::vector<double> f(4 * nc);
+  const int ids[4] = {u, v, w, p};
+  for (int q = 0; q < 4; ++q) {
+    int rc = nsx_get_field(ctx, ids[q], f.data() + q * nc, 0);
+    if (rc) return rc;
+  }
+  if (nsx_write_tec_center(dir, counter, time, ctx->N0, ctx->NC, ctx->NV, ctx->NZ, ctx->No_vp.data(), xvt, yvt, zvt,
+                           f.data(), f.data() + nc, f.data() + 2 * nc, f.data() + 3 * nc))
+    return nsx_fail(ctx, -5, "nsx_savetec_center: cannot write C-%05d.tec in %s", counter, dir);
+  return 0;
+}
+
 // ---------------------------------------------------------------- timing --
 extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes) {
   if (!ctx) return -1;

@@ -255,3 +255,56 @@
       call nsx_dropin_check(nsx_sm_count(nsx_ctx, n), 'glob_put_plane_mm')
       print*,'   --- Plane Averaged Stats printed! ---',n
       END SUBROUTINE glob_put_plane_mm
+
+!-----------------------------------------------------------------------
+!  The Tecplot snapshots of src/Save.  The reference declares both with
+!  assumed-shape dummies (module_interfaces.F90:141-190), so callers pass
+!  array descriptors and these take the same form.  SaveCounter and time
+!  come from the caller's COMMON /integ/ and /times/, declared here with the
+!  member types of common.mpf:221-228,310-346 (a COMMON block is a storage
+!  sequence, so the members must keep the reference's order and kinds).
+!  The files go to ../output/Serial, as the reference's serial build writes.
+
+!  SavetecVertex.F90:185-235 -- V-NNNN.tec of the vertex fields
+      SUBROUTINE SavetecVertex(alphafv,ufv,vfv,wfv,pfv,                  &
+                               alphasv,usv,vsv,wsv,psv,rhosv,            &
+                               xvt,yvt,zvt,No_vp)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(:,:) :: alphafv, ufv, vfv, wfv, pfv
+      real*8, dimension(:,:) :: alphasv, usv, vsv, wsv, psv, rhosv
+      real*8, dimension(:,:) :: xvt, yvt, zvt
+      integer, dimension(:,:) :: No_vp
+      real*8 :: tFin, tIni, dt, tInisave, dtsave, LastTimeSave, time
+      integer :: tlimit, SaveCounter, Numictec, nc_ref, FinRK
+      common/times/tFin,tIni,dt,tInisave,dtsave,LastTimeSave,time,tlimit
+      common/integ/SaveCounter,Numictec,nc_ref,FinRK
+      if (nsx_write_tec_vertex('../output/Serial'//c_null_char,          &
+           SaveCounter, time, nsx_n_cell0, nsx_n_vert, nsx_nz, No_vp,    &
+           xvt, yvt, zvt, ufv, vfv, wfv, pfv) /= 0) then
+        write(*,'(a)') ' nsx: SavetecVertex: cannot write ../output/Serial'
+        stop 'nsx: fatal error'
+      endif
+      END SUBROUTINE SavetecVertex
+
+!  SavetecCenter.F90:185-235 -- C-NNNNN.tec of the cell fields
+      SUBROUTINE SavetecCenter(alphafnp,ufnp,vfnp,wfnp,pfnp,             &
+                               alphasnp,usnp,vsnp,wsnp,psnp,rhos,        &
+                               xvt,yvt,zvt,No_vp)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(:,:) :: alphafnp, ufnp, vfnp, wfnp, pfnp
+      real*8, dimension(:,:) :: alphasnp, usnp, vsnp, wsnp, psnp, rhos
+      real*8, dimension(:,:) :: xvt, yvt, zvt
+      integer, dimension(:,:) :: No_vp
+      real*8 :: tFin, tIni, dt, tInisave, dtsave, LastTimeSave, time
+      integer :: tlimit, SaveCounter, Numictec, nc_ref, FinRK
+      common/times/tFin,tIni,dt,tInisave,dtsave,LastTimeSave,time,tlimit
+      common/integ/SaveCounter,Numictec,nc_ref,FinRK
+      if (nsx_write_tec_center('../output/Serial'//c_null_char,          &
+           SaveCounter, time, nsx_n_cell0, nsx_n_cell, nsx_n_vert,       &
+           nsx_nz, No_vp, xvt, yvt, zvt, ufnp, vfnp, wfnp, pfnp) /= 0) then
+        write(*,'(a)') ' nsx: SavetecCenter: cannot write ../output/Serial'
+        stop 'nsx: fatal error'
+      endif
+      END SUBROUTINE SavetecCenter

@@ -305,6 +305,45 @@
           integer(c_int), value :: kind, w, d, outlen
           character(kind=c_char), dimension(*) :: out
         end function
+        integer(c_int) function nsx_savetec_vertex(ctx, dir, counter,    &
+             time, xvt, yvt, zvt) bind(C, name="nsx_savetec_vertex")
+          import :: c_int, c_ptr, c_double, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter
+          real(c_double), value :: time
+          real(c_double) :: xvt(*), yvt(*), zvt(*)
+        end function
+        integer(c_int) function nsx_savetec_center(ctx, dir, counter,    &
+             time, xvt, yvt, zvt, u, v, w, p)                           &
+             bind(C, name="nsx_savetec_center")
+          import :: c_int, c_ptr, c_double, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, u, v, w, p
+          real(c_double), value :: time
+          real(c_double) :: xvt(*), yvt(*), zvt(*)
+        end function
+        integer(c_int) function nsx_write_tec_vertex(dir, counter, time, &
+             n_cell0, n_vert, nz, no_vp, xvt, yvt, zvt, ufv, vfv, wfv,   &
+             pfv) bind(C, name="nsx_write_tec_vertex")
+          import :: c_int, c_double, c_char
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, n_cell0, n_vert, nz
+          real(c_double), value :: time
+          integer(c_int) :: no_vp(*)
+          real(c_double) :: xvt(*), yvt(*), zvt(*), ufv(*), vfv(*), wfv(*), pfv(*)
+        end function
+        integer(c_int) function nsx_write_tec_center(dir, counter, time, &
+             n_cell0, n_cell, n_vert, nz, no_vp, xvt, yvt, zvt, u, v, w, &
+             p) bind(C, name="nsx_write_tec_center")
+          import :: c_int, c_double, c_char
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, n_cell0, n_cell, n_vert, nz
+          real(c_double), value :: time
+          integer(c_int) :: no_vp(*)
+          real(c_double) :: xvt(*), yvt(*), zvt(*), u(*), v(*), w(*), p(*)
+        end function
         integer(c_int) function nsx_set_precision(ctx, precision, refine,&
              tol, max_passes) bind(C, name="nsx_set_precision")
           import :: c_int, c_ptr, c_double

@@ -278,6 +278,34 @@ class Context:
         """glob_put_plane_mm (stats.F90:204-391): the three statistics files in directory."""
         self._check(self.lib.nsx_glob_put_plane_mm(self.h, directory.encode()))
 
+    # -- Tecplot snapshots (src/Save)
+    def _coords(self, xvt, yvt, zvt):
+        shape = (self.case.N_VERT, self.case.NZ - 1)
+        out = []
+        for a in (xvt, yvt, zvt):
+            a = np.asfortranarray(a, dtype=np.float64)
+            if a.shape != shape:
+                raise ValueError(f"vertex coordinates must be {shape}, got {a.shape}")
+            out.append(a)
+        return out
+
+    def savetec_vertex(self, directory: str, counter: int, time: float, xvt, yvt, zvt) -> None:
+        """SavetecVertex (SavetecVertex.F90:185-235): directory/V-NNNN.tec of
+        the resident vertex fields, with the vertex coordinates xvt, yvt, zvt
+        (N_VERT, NZ-1)."""
+        x, y, z = self._coords(xvt, yvt, zvt)
+        self._check(self.lib.nsx_savetec_vertex(self.h, directory.encode(), int(counter), float(time),
+                                                x.ctypes.data, y.ctypes.data, z.ctypes.data))
+
+    def savetec_center(self, directory: str, counter: int, time: float, xvt, yvt, zvt,
+                       fields=("UFNP", "VFNP", "WFNP", "PFNP")) -> None:
+        """SavetecCenter (SavetecCenter.F90:185-235): directory/C-NNNNN.tec of
+        the resident cell fields named in fields (u, v, w, p)."""
+        x, y, z = self._coords(xvt, yvt, zvt)
+        ids = [F[n] if isinstance(n, str) else int(n) for n in fields]
+        self._check(self.lib.nsx_savetec_center(self.h, directory.encode(), int(counter), float(time),
+                                                x.ctypes.data, y.ctypes.data, z.ctypes.data, *ids))
+
     def set_max_iters(self, n: int) -> None:
         """MaxIters of the pressure solve (the case's other parameters kept)."""
         p = self.case.params

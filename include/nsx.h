@@ -222,6 +222,26 @@ int nsx_write_plane_stats(const double *planes, int NZ, int n_cell0, int ns, dou
  * descriptor, kind 1 = list-directed (the compiler of the reference build) */
 int nsx_format_real(double x, int kind, int w, int d, char *out, int outlen);
 
+/* ---- Tecplot snapshots (src/Save) ------------------------------------------ */
+/* SavetecVertex (SavetecVertex.F90:185-235): <dir>/V-NNNN.tec (NNNN = counter,
+ * the reference's SaveCounter) with the vertex coordinates xvt, yvt, zvt
+ * (N_VERT,NZ-1) of the caller and the resident vertex fields NSX_F_UFV..PFV */
+int nsx_savetec_vertex(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt,
+                       const double *yvt, const double *zvt);
+/* SavetecCenter (SavetecCenter.F90:185-235): <dir>/C-NNNNN.tec with the vertex
+ * coordinates and the resident cell fields u, v, w, p (field ids) */
+int nsx_savetec_center(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt,
+                       const double *yvt, const double *zvt, int u, int v, int w, int p);
+/* the file writing of the two alone, on host arrays in the reference layout
+ * (No_vp (N_CELL0,3) 1-based; vertex fields (N_VERT,NZ-1); cell fields
+ * (N_CELL,NZ)); needs no device */
+int nsx_write_tec_vertex(const char *dir, int counter, double time, int N_CELL0, int N_VERT, int NZ,
+                         const int *No_vp, const double *xvt, const double *yvt, const double *zvt,
+                         const double *ufv, const double *vfv, const double *wfv, const double *pfv);
+int nsx_write_tec_center(const char *dir, int counter, double time, int N_CELL0, int N_CELL, int N_VERT, int NZ,
+                         const int *No_vp, const double *xvt, const double *yvt, const double *zvt,
+                         const double *u, const double *v, const double *w, const double *p);
+
 /* ---- precision form (no counterpart in the reference) ------------------------ */
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
  * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs

@@ -16,7 +16,10 @@
 !  are the record of that call).  Mode 2 also takes the channel statistics
 !  as nsmp3D.F90:303,579-589 does (sm_init, then glob_sample after every
 !  step, glob_put_plane_mm at the end, whose files go to ../output/Stats of
-!  the working directory) and dumps the twelve accumulators.
+!  the working directory) and dumps the twelve accumulators; then it saves
+!  the final state as nsmp3D.F90:617-625 does for ChooseExit 1 and 2
+!  (SavetecVertex, SavetecCenter, SaveCounter 3, into ../output/Serial) and
+!  dumps the vertex coordinates xvt, yvt, zvt those files hold.
 
       module nsxdump
       implicit none
@@ -49,7 +52,8 @@
 !     explicit interfaces of the routines with assumed-shape dummies, as the
 !     reference PROGRAM has them (nsmp3D.F90:55); grandientLSM's entry there
 !     does not match the routine and is left out
-      USE interfaces, only: calcul_geometry, initial, LoopTimeInitial, LoopTimeUpdate
+      USE interfaces, only: calcul_geometry, initial, LoopTimeInitial, LoopTimeUpdate, &
+                            SavetecVertex, SavetecCenter
       implicit none
       character(len=512) :: casefile, outfile
       integer :: hdr(16), ucase, n1, n2, nsteps, mode, irk, istep, nbuf
@@ -568,6 +572,18 @@
       call dump_r8('sm_pf', sm_pf, N_CELL*NZ)
       call dump_r8('sm_p2f', sm_p2f, N_CELL*NZ)
       call dump_r8('sm_ef', sm_ef, N_CELL*NZ)
+      SaveCounter = 3
+      call SavetecVertex(alphafv,ufv,vfv,wfv,pfv,           &
+                         alphasv,usv,vsv,wsv,psv,rhosv,     &
+                         xvt,yvt,zvt,No_vp)
+      call SavetecCenter(alphafnp,ufnp,vfnp,wfnp,pfnp,      &
+                         alphasnp,usnp,vsnp,wsnp,psnp,rhos, &
+                         xvt,yvt,zvt,No_vp)
+      call dump_r8('xvt', xvt, N_VERT*(NZ-1))
+      call dump_r8('yvt', yvt, N_VERT*(NZ-1))
+      call dump_r8('zvt', zvt, N_VERT*(NZ-1))
+      tv(1) = time
+      call dump_r8('time', tv, 1)
       end subroutine
 
       end program

@@ -109,8 +109,10 @@ def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workd
         # the reference writes its statistics to ../output/Stats of its working directory
         wd = os.path.join(td, "run")
         stats_dir = os.path.join(td, "output", "Stats")
+        serial_dir = os.path.join(td, "output", "Serial")
         os.makedirs(wd)
         os.makedirs(stats_dir)
+        os.makedirs(serial_dir)
         cf = os.path.join(td, "case.bin")
         of = os.path.join(td, "out.bin")
         write_case(cf, case, mode=mode, nsteps=nsteps, dump_every=dump_every, p0=p0)
@@ -123,6 +125,7 @@ def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workd
             out += "".join("MEANUV " + ln for ln in open(mu))
         d = read_dump(of, case)
         d["stats_files"] = {fn: open(os.path.join(stats_dir, fn)).read() for fn in sorted(os.listdir(stats_dir))}
+        d["serial_files"] = {fn: open(os.path.join(serial_dir, fn)).read() for fn in sorted(os.listdir(serial_dir))}
         return d, out
 
 

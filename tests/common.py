@@ -79,3 +79,14 @@ def ppe_residual(case, phi, rhs):
     b = rhs_i - avec * vol
     r = b - Ax
     return float(np.abs(r).sum() / np.abs(b).sum()), float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def vertex_coordinates(c):
+    """xvt, yvt, zvt (N_VERT, NZ-1) as initial.F90:380-388 sets them, with
+    the channel's Hprv = 1 and hv = 0"""
+    A = c.arrays
+    nk = c.NZ - 1
+    xvt = np.asfortranarray(np.repeat(A["xv"][:, None], nk, axis=1))
+    yvt = np.asfortranarray(np.repeat(A["yv"][:, None], nk, axis=1))
+    zvt = np.asfortranarray(np.repeat(A["sigv"][None, :] * 1.0 - 0.0, c.N_VERT, axis=0))
+    return xvt, yvt, zvt

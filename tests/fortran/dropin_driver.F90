@@ -7,14 +7,42 @@
 !  given right-hand side, then one hydro stage (flag_ab = 0) from the initial
 !  fields; then the channel statistics: sm_init, glob_sample of the initial
 !  fields and of the stage result, glob_put_plane_mm (its files go to
-!  ../output/Stats of the working directory).  Writes phi, phiv, ufnp, vfnp,
-!  wfnp, pfnp, ufv, pfv to the output stream file.  Exit status 3 when no HIP
-!  device is usable.
+!  ../output/Stats of the working directory); then the Tecplot snapshots
+!  SavetecVertex of the stage's vertex fields and SavetecCenter of its cell
+!  fields, with SaveCounter = 7 and time = 0.0125 in the reference's COMMON
+!  blocks (files in ../output/Serial).  Writes phi, phiv, ufnp, vfnp, wfnp,
+!  pfnp, ufv, vfv, wfv, pfv to the output stream file.  Exit status 3 when no
+!  HIP device is usable.
       program dropin_driver
       use iso_c_binding
       use nsx_iface
       use nsx_dropin
       implicit none
+!     the explicit interfaces of the snapshot writers (module_interfaces.F90:141-190)
+      interface
+        SUBROUTINE SavetecVertex(alphafv,ufv,vfv,wfv,pfv,                &
+                                 alphasv,usv,vsv,wsv,psv,rhosv,          &
+                                 xvt,yvt,zvt,No_vp)
+          real*8, dimension(:,:) :: alphafv, ufv, vfv, wfv, pfv
+          real*8, dimension(:,:) :: alphasv, usv, vsv, wsv, psv, rhosv
+          real*8, dimension(:,:) :: xvt, yvt, zvt
+          integer, dimension(:,:) :: No_vp
+        END SUBROUTINE SavetecVertex
+        SUBROUTINE SavetecCenter(alphafnp,ufnp,vfnp,wfnp,pfnp,           &
+                                 alphasnp,usnp,vsnp,wsnp,psnp,rhos,      &
+                                 xvt,yvt,zvt,No_vp)
+          real*8, dimension(:,:) :: alphafnp, ufnp, vfnp, wfnp, pfnp
+          real*8, dimension(:,:) :: alphasnp, usnp, vsnp, wsnp, psnp, rhos
+          real*8, dimension(:,:) :: xvt, yvt, zvt
+          integer, dimension(:,:) :: No_vp
+        END SUBROUTINE SavetecCenter
+      end interface
+!     COMMON /times/ and /integ/ of the reference (common.mpf:221-228,310-346)
+      real*8 :: tFin, tIni, dt, tInisave, dtsave, LastTimeSave, time
+      integer :: tlimit, SaveCounter, Numictec, nc_ref, FinRK
+      common/times/tFin,tIni,dt,tInisave,dtsave,LastTimeSave,time,tlimit
+      common/integ/SaveCounter,Numictec,nc_ref,FinRK
+      real(c_double), allocatable :: xvt(:,:), yvt(:,:), zvt(:,:)
       character(len=1024) :: fin, fout
       integer :: u, n0, nc, nv, nz, rc
       integer(c_int), allocatable :: No_cp(:,:), No_vp(:,:), nbe(:), nbev(:)
@@ -40,6 +68,8 @@
       read(u) areaCell, e3, lsm, sig, dsig, sigv, dsigv, weight, dlVsum, Am, walld, rpar
       allocate(f0(nc,nz,4), rhs(nc,nz))
       read(u) f0, rhs
+      allocate(xvt(nv,nz-1), yvt(nv,nz-1), zvt(nv,nz-1))
+      read(u) xvt, yvt, zvt
       close(u)
 
       rc = nsx_dropin_open(n0, nc, nv, nz)
@@ -92,6 +122,18 @@
       call glob_sample(f0(:,:,1), f0(:,:,2), f0(:,:,3), f0(:,:,4))
       call glob_sample(fnp(:,:,1), fnp(:,:,2), fnp(:,:,3), fnp(:,:,4))
       call glob_put_plane_mm()
+
+!     the snapshots of the stage (SaveCounter and time from the COMMON blocks)
+      SaveCounter = 7
+      time = 0.0125d0
+      tFin = -1.d0
+      tlimit = -1
+      call SavetecVertex(rhov, fv(:,:,1), fv(:,:,2), fv(:,:,3), fv(:,:,4),&
+                         rhov, rhov, rhov, rhov, rhov, rhov,              &
+                         xvt, yvt, zvt, No_vp)
+      call SavetecCenter(rho, fnp(:,:,1), fnp(:,:,2), fnp(:,:,3),          &
+                         fnp(:,:,4), rho, rho, rho, rho, rho, rho,         &
+                         xvt, yvt, zvt, No_vp)
 
       open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
       write(u) phi, phiv, fnp, fv

@@ -13,7 +13,9 @@ what they produce:
                   glob_cfl's diagnostics of the final state, and the channel
                   statistics (glob_sample after every step): SHA-256 of the
                   twelve accumulators (rows 1..N_CELL0) and the three files
-                  glob_put_plane_mm wrote at the end;
+                  glob_put_plane_mm wrote at the end; SHA-256, size and
+                  header lines of the Tecplot snapshots SavetecVertex and
+                  SavetecCenter wrote of the final state;
   cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
@@ -64,7 +66,8 @@ def main():
 
     c, p0 = tiny_case()
     d, log = refrun.run(c, mode=1, p0=p0)
-    d.pop("stats_files", None)  # mode 1 takes no statistics
+    for k in ("stats_files", "serial_files"):  # mode 1 takes no statistics and writes no snapshots
+        d.pop(k, None)
     out = {k: (v[:rows_of(c, v)] if isinstance(v, np.ndarray) and v.ndim == 2 else v) for k, v in d.items()}
     out["solves"] = np.array([s[0] for s in refrun.parse_solves(log)], dtype=np.int32)
     np.savez_compressed(os.path.join(HERE, "tiny_unit.npz"), **out)
@@ -101,6 +104,16 @@ def main():
             a = np.ascontiguousarray(d[nm][:c.N_CELL0], dtype=np.float64) + 0.0
             sm[nm] = {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "l2": float(np.linalg.norm(a))}
         rec["stats"] = {"sm_n": nsteps, "accumulators": sm, "files": d["stats_files"]}
+        # the Tecplot snapshots of the final state (SaveCounter 3): digests and
+        # the header lines, with the coordinates and time the files hold
+        serial = {}
+        for fn, txt in d["serial_files"].items():
+            lines = txt.split("\n")
+            serial[fn] = {"sha256": hashlib.sha256(txt.encode()).hexdigest(), "size": len(txt.encode()),
+                          "lines": len(lines) - 1, "head": lines[:5]}
+        coords = {nm: hashlib.sha256(np.ascontiguousarray(d[nm], dtype=np.float64).tobytes()).hexdigest()
+                  for nm in ("xvt", "yvt", "zvt")}
+        rec["serial"] = {"counter": 3, "time": float(d["time"][0]), "files": serial, "coords": coords}
         with open(os.path.join(HERE, f"cfg{cfg}_steps.json"), "w") as f:
             json.dump(rec, f, indent=1)
         print(f"cfg{cfg}_steps.json:", len(rec["solves"]), "solves")

@@ -1,5 +1,6 @@
 """The Fortran host side: the drop-in subroutines (bicgstab, hydro,
-grandientLSM, sm_init, glob_sample, glob_put_plane_mm under the reference's
+grandientLSM, sm_init, glob_sample, glob_put_plane_mm, SavetecVertex,
+SavetecCenter under the reference's
 names and argument lists, over the ISO_C_BINDING interfaces of include/nsx.h)
 linked into a Fortran program.
 
@@ -16,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import interior, rel_err, tiny_case
+from common import interior, rel_err, tiny_case, vertex_coordinates
 from conftest import REPO
 from nsx.case import config_case
 
@@ -46,6 +47,7 @@ def write_case(case, rhs, path):
     parts.append(f64([case.TotalVolume, p.sigIni, p.sigFin, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps]))
     parts.append(f64(np.stack([A[n] for n in ("u0", "v0", "w0", "p0")], axis=2)))
     parts.append(f64(rhs))
+    parts.extend(f64(a) for a in vertex_coordinates(case))
     with open(path, "wb") as f:
         f.write(b"".join(parts))
 
@@ -78,7 +80,8 @@ def test_dropin_exports_reference_names():
     if not os.path.exists(FLIB):
         pytest.skip("the Fortran drop-in is not built (no Fortran compiler)")
     out = subprocess.run(["nm", FLIB], capture_output=True, text=True, check=True).stdout
-    for sym in ("bicgstab_", "hydro_", "grandientlsm_", "sm_init_", "glob_sample_", "glob_put_plane_mm_"):
+    for sym in ("bicgstab_", "hydro_", "grandientlsm_", "sm_init_", "glob_sample_", "glob_put_plane_mm_",
+                "savetecvertex_", "saveteccenter_"):
         assert f" T {sym}\n" in out, sym
 
 
@@ -90,6 +93,7 @@ def test_driver_links_and_runs(tmp_path):
     write_case(c, _rhs(c), fin)
     (tmp_path / "run").mkdir()
     (tmp_path / "output" / "Stats").mkdir(parents=True)
+    (tmp_path / "output" / "Serial").mkdir(parents=True)
     r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
                        cwd=tmp_path / "run")
     if r.returncode == 3:
@@ -97,6 +101,43 @@ def test_driver_links_and_runs(tmp_path):
     else:
         assert r.returncode == 0, r.stdout + r.stderr
         read_out(c, fout)
+
+
+SAVETEC = os.path.join(REPO, "tests", "fortran", "_build", "savetec_driver")
+
+
+def test_savetec_dropins_write_reference_files(tmp_path):
+    """SavetecVertex and SavetecCenter of the Fortran drop-in, called under
+    the reference's argument lists with SaveCounter and time in its COMMON
+    blocks, on the oracle's state after the ten steps of the 32x16 quads x 32
+    layers channel: the reference's own two files, byte for byte."""
+    import hashlib
+
+    from common import load_steps
+    if not os.path.exists(SAVETEC):
+        pytest.skip("the Fortran test driver is not built (no Fortran compiler)")
+    rec = load_steps(0)
+    serial = rec["serial"]
+    c = config_case(0)
+    O = oracle.Oracle(c)
+    for _ in range(rec["nsteps"]):
+        O.step()
+    i32 = lambda a: np.asfortranarray(a, dtype=np.int32).ravel(order="F").tobytes()  # noqa: E731
+    f64 = lambda a: np.asfortranarray(a, dtype=np.float64).ravel(order="F").tobytes()  # noqa: E731
+    parts = [i32([c.N_CELL0, c.N_CELL, c.N_VERT, c.NZ, serial["counter"]]), i32(c.arrays["No_vp"]),
+             f64([serial["time"]])]
+    parts += [f64(a) for a in vertex_coordinates(c)]
+    parts += [f64(O.state(n)) for n in ("ufv", "vfv", "wfv", "pfv", "ufnp", "vfnp", "wfnp", "pfnp")]
+    fin = tmp_path / "snap.bin"
+    fin.write_bytes(b"".join(parts))
+    (tmp_path / "run").mkdir()
+    out = tmp_path / "output" / "Serial"
+    out.mkdir(parents=True)
+    r = subprocess.run([SAVETEC, str(fin)], capture_output=True, text=True, timeout=120, cwd=tmp_path / "run")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sorted(os.listdir(out)) == sorted(serial["files"])
+    for fn, want in serial["files"].items():
+        assert hashlib.sha256((out / fn).read_bytes()).hexdigest() == want["sha256"], fn
 
 
 @pytest.mark.gpu
@@ -111,6 +152,8 @@ def test_dropin_vs_oracle(tmp_path, which):
     (tmp_path / "run").mkdir()
     stats_dir = tmp_path / "output" / "Stats"
     stats_dir.mkdir(parents=True)
+    serial_dir = tmp_path / "output" / "Serial"
+    serial_dir.mkdir(parents=True)
     print(f"dropin_driver {which}", flush=True)
     r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
                        cwd=tmp_path / "run")
@@ -151,6 +194,22 @@ def test_dropin_vs_oracle(tmp_path, which):
     for fn in names:
         a, b = _numbers(open(stats_dir / fn).read()), _numbers(open(want_dir / fn).read())
         assert a.shape == b.shape and np.all(np.abs(a - b) <= 1e-4 * np.abs(b) + 1e-10), fn
+    # the snapshots: the library's writers fed the driver's own stage output,
+    # SaveCounter 7 and time 0.0125, give the same bytes
+    snap = tmp_path / "snap"
+    snap.mkdir()
+    xvt, yvt, zvt = vertex_coordinates(c)
+    no_vp = np.asfortranarray(c.arrays["No_vp"], dtype=np.int32)
+    P = lambda a: np.asfortranarray(a, dtype=np.float64).ctypes.data  # noqa: E731
+    fv_, fnp_ = [np.asfortranarray(fv[:, :, q]) for q in range(4)], [np.asfortranarray(fnp[:, :, q]) for q in range(4)]
+    lib = device()
+    assert lib.nsx_write_tec_vertex(str(snap).encode(), 7, 0.0125, c.N_CELL0, c.N_VERT, c.NZ, no_vp.ctypes.data,
+                                    P(xvt), P(yvt), P(zvt), *[P(a) for a in fv_]) == 0
+    assert lib.nsx_write_tec_center(str(snap).encode(), 7, 0.0125, c.N_CELL0, c.N_CELL, c.N_VERT, c.NZ,
+                                    no_vp.ctypes.data, P(xvt), P(yvt), P(zvt), *[P(a) for a in fnp_]) == 0
+    assert sorted(os.listdir(serial_dir)) == sorted(os.listdir(snap)) == ["C-00007.tec", "V-0007.tec"]
+    for fn in ("C-00007.tec", "V-0007.tec"):
+        assert open(serial_dir / fn, "rb").read() == open(snap / fn, "rb").read(), fn
 
 
 def _numbers(text):

@@ -11,6 +11,8 @@ reference's own files.
   layers channel: the files hold the reference's values (its own files,
   tests/golden) to 1e-6 relative or 1e-8 absolute, the spread of two
   trajectories whose pressure solves stop at a 1e-10 residual.
+* the Tecplot snapshots (SavetecVertex, SavetecCenter) of resident fields:
+  byte-identical to the reference's files on the same state.
 """
 import os
 import tempfile
@@ -19,7 +21,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import load_steps
+from common import load_steps, vertex_coordinates
 from nsx.abi import SM_NAMES
 from nsx.case import config_case
 
@@ -94,6 +96,70 @@ def test_stats_over_device_steps_vs_reference_files():
             assert a.shape == b.shape, fn
             assert np.all(np.abs(a - b) <= 1e-6 * np.abs(b) + 1e-8), (fn, np.max(np.abs(a - b)))
     G.close()
+
+
+def _snapshot_numbers(path):
+    """the numbers of a Tecplot snapshot after its header lines (E12.5 blocks
+    then the integer brick connectivity)"""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    body = [ln for ln in lines if not any(k in ln for k in ("TITLE", "VARIABLES", "ZONE", "VARLOCATION", "StrandID"))]
+    return np.array([float(t) for ln in body for t in ln.split()])
+
+
+def test_snapshots_of_resident_fields_match_reference_files():
+    """nsx_savetec_vertex / nsx_savetec_center download the resident fields:
+    fed the oracle's final 10-step state (== the reference's, hash for hash),
+    the files are the reference's byte for byte; after ten device steps they
+    hold the oracle's values up to one unit in the last printed digit plus
+    1e-6 of each field's largest value (the spread of two trajectories whose
+    solves stop at a 1e-10 residual), with the same connectivity."""
+    import hashlib
+
+    rec = load_steps(0)
+    serial = rec["serial"]
+    c = config_case(0)
+    O = oracle.Oracle(c)
+    for _ in range(rec["nsteps"]):
+        O.step()
+    xvt, yvt, zvt = vertex_coordinates(c)
+    G = _ctx(c)
+    G.init_state()
+    for f in ("UFV", "VFV", "WFV", "PFV", "UFNP", "VFNP", "WFNP", "PFNP"):
+        G.put(f, O.state(f.lower()))
+    with tempfile.TemporaryDirectory() as td:
+        G.savetec_vertex(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        G.savetec_center(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        assert sorted(os.listdir(td)) == sorted(serial["files"])
+        for fn, want in serial["files"].items():
+            with open(os.path.join(td, fn), "rb") as f:
+                assert hashlib.sha256(f.read()).hexdigest() == want["sha256"], fn
+        ref = {fn: _snapshot_numbers(os.path.join(td, fn)) for fn in serial["files"]}
+    G2 = _ctx(c)
+    G2.init_state()
+    for _ in range(rec["nsteps"]):
+        G2.step(1, 0)
+    with tempfile.TemporaryDirectory() as td:
+        G2.savetec_vertex(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        G2.savetec_center(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        for fn, b in ref.items():
+            a = _snapshot_numbers(os.path.join(td, fn))
+            assert a.shape == b.shape, fn
+            # field blocks (coordinates and fields), then the connectivity
+            nv, ncl = c.N_VERT * (c.NZ - 1), c.N_CELL0 * (c.NZ - 2)
+            sizes = [nv] * 7 if fn.startswith("V") else [nv] * 3 + [ncl] * 4
+            o = 0
+            for q, n in enumerate(sizes):
+                x, y = a[o:o + n], b[o:o + n]
+                o += n
+                # one unit in the fifth (last printed) digit of E12.5, plus
+                # the spread of the two trajectories, 1e-6 of the field's scale
+                ulp = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(y), 1e-300))) - 4)
+                bad = np.abs(x - y) > 1.5 * ulp + 1e-6 * np.abs(y).max()
+                assert not bad.any(), (fn, q, int(bad.sum()), x[bad][:4], y[bad][:4])
+            assert np.array_equal(a[o:], b[o:]), fn
+    G.close()
+    G2.close()
 
 
 def test_sample_needs_init():
