@@ -557,20 +557,20 @@ __device__ inline void window_sweep(uint32_t nwin, uint32_t &w, uint32_t &wend, 
   }
 }
 
-// Seven-point values of f = a + c b around every quad (f = a when first).
-// Each thread forms f on its own four layers and publishes them in LDS; the
-// vertical neighbours and the neighbour columns whose quad lies in the
-// block's current window of NSX_BLOCK quads (the in-quad partner and usually
-// the x-neighbour) are read from there, the rest (the y-neighbour, window
-// edges) are formed from global memory with the same arithmetic, so every
-// value is bit-identical whichever way it was obtained.  Every global load of
-// a window (own values, neighbour indices, column data, the out-of-window
-// neighbours and what `pre` loads) is issued before the window's barrier, so
-// their latencies overlap; body(i, k0, s0, C, fc, fm, fp, f0, f1, f2) runs
-// after it.
-template <typename T, class Pre, class Body>
-__device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, Pre &&pre,
-                                              Body &&body) {
+// Seven-point values of a vector f formed from stored vectors around every
+// quad.  Each thread forms f on its own four layers (own4) and publishes them
+// in LDS; the vertical neighbours and the neighbour columns whose quad lies
+// in the block's current window of NSX_BLOCK quads (the in-quad partner and
+// usually the x-neighbour) are read from there, the rest (the y-neighbour,
+// window edges) are formed from global memory (val4 for four layers, val1
+// for one) with the same arithmetic, so every value is bit-identical
+// whichever way it was obtained.  Every global load of a window (own values,
+// neighbour indices, column data, the out-of-window neighbours and what `pre`
+// loads) is issued before the window's barrier, so their latencies overlap;
+// body(i, k0, s0, C, fc, fm, fp, f0, f1, f2) runs after it.
+template <typename T, class Own4, class Val4, class Val1, class Pre, class Body>
+__device__ inline void for_each_quad_formed(const Geo &g, Own4 &&own4, Val4 &&val4, Val1 &&val1, Pre &&pre,
+                                            Body &&body) {
   __shared__ __align__(16) T win[NSX_BLOCK][4];
   const uint32_t nwin = (g.nquad + NSX_BLOCK - 1) / NSX_BLOCK;
   uint32_t w0, wend, step;
@@ -591,7 +591,7 @@ __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T 
       i = (int)fdiv(t, g.divNQ);
       k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
       s0 = (size_t)i * g.NS + k0;
-      axpy4(a, b, c, first, s0, fc);
+      own4(s0, fc);
       const int e = i * 3;
       const int n[3] = {g.nbr[e], g.nbr[e + 1], g.nbr[e + 2]};
       load_col(g, i, C);
@@ -600,10 +600,10 @@ __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T 
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         d[j] = (uint32_t)n[j] * (uint32_t)g.NQ + qq - base;
-        if (d[j] >= lim) axpy4(a, b, c, first, (size_t)n[j] * g.NS + k0, fn[j]);
+        if (d[j] >= lim) val4((size_t)n[j] * g.NS + k0, fn[j]);
       }
-      if (k0 >= 1 && !mwin) fm = axpy1(a, b, c, first, s0 - 1);
-      if (k0 + 4 <= g.NZ - 2 && !pwin) fp = axpy1(a, b, c, first, s0 + 4);
+      if (k0 >= 1 && !mwin) fm = val1(s0 - 1);
+      if (k0 + 4 <= g.NZ - 2 && !pwin) fp = val1(s0 + 4);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) win[threadIdx.x][u] = fc[u];
@@ -621,6 +621,15 @@ __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T 
     }
     __syncthreads();
   }
+}
+
+// the seven-point values of f = a + c b (f = a when first)
+template <typename T, class Pre, class Body>
+__device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, Pre &&pre,
+                                              Body &&body) {
+  auto v4 = [&](size_t off, T(&o)[4]) { axpy4(a, b, c, first, off, o); };
+  auto v1 = [&](size_t off) { return axpy1(a, b, c, first, off); };
+  for_each_quad_formed<T>(g, v4, v4, v1, pre, body);
 }
 
 // ------------------------------------------------------------ phase A ----
@@ -847,6 +856,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_start(Geo g, const double *rhs
     st->e1 = e1;
     st->e2 = e2;
     st->xvol = t[4];
+    st->cgs_runs = 0;
     if (stop) {
       st->exit_code = EXIT_NO_SOLVE;
       st->done = 1;
@@ -1054,6 +1064,257 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
   write_partials<3>(acc, P.c);
 }
 
+// --------------------------------------- single-reduction FP32 CG ----
+// NSX_REFINE_CGSR: the CG of NSX_REFINE_CG in the form of Chronopoulos and
+// Gear (J. Comput. Appl. Math. 25, 1989), which needs one global reduction
+// per iteration.  With u = M^-1 r and w = A u:
+//   gamma_i = <r_i, u_i>, delta_i = <w_i, u_i>, beta_i = gamma_i/gamma_{i-1},
+//   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1}),
+//   p_i = u_i + beta_i p_{i-1}, s_i = w_i + beta_i s_{i-1},
+//   e += alpha_i p_i, r_{i+1} = r_i - alpha_i s_i,
+// (beta_0 = 0, alpha_0 = gamma_0/delta_0), so the next product w_{i+1} =
+// A u_{i+1} is formed in the same pass as the updates: u_{i+1} = rt -
+// alpha_i (om + beta_i sg) at the seven points, from the stored scaled
+// vectors rt = u = r/A0, om = w/A0 and sg = s/A0.  One launch per iteration
+// (mode N) reads rt, om, sg, p, e and writes rt, om, sg (into the other of
+// two buffer sets, since it reads them at neighbouring prisms), p and e (in
+// place): 40 B per prism, as the two kernels of NSX_REFINE_CG, in one pass
+// with one reduction.  In exact arithmetic the iterates are CG's.
+//
+// Reliable updates as in NSX_REFINE_CG: when an iteration's residual |r|
+// has fallen below delta x its largest value since the last true residual,
+// or meets the stopping rule, the next launch (mode T) forms the true FP64
+// residual of x + e into rt; the launch after it (mode M) folds e into x,
+// applies the stopping rule to the true residual, and forms om = A rt / A0
+// with gamma and delta.  beta, alpha carry on across the update.  The first
+// launch of a solve is an M without the fold, after k_cg_start.
+//
+// Launch j reads the carried scalars from SolverState::cgs[(j-1)&1] and
+// block 0 writes cgs[j&1]; the partial sums alternate between two buffers
+// the same way, so no block reads a value that its own launch rewrites.
+// Partial rows: M gamma, delta; N gamma, delta, |r|^2, |r|_1; T |r|^2,
+// |r|_1, sum (x+e) Vol.
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, const double *rhs, double *x,
+                                                    const double *Pin, uint32_t Gin, double *Pout, SolverState *st) {
+  CgsCarry c;
+  if (j == 0) {
+    if (st->done) return;  // k_cg_start's no-solve test
+    c.gamma = c.alpha = c.maxr = 0.0;
+    c.mode = -1;
+    c.iter = c.set = c.done = 0;
+  } else {
+    c = st->cgs[(j - 1) & 1];
+    if (c.done) return;
+  }
+  const int prev = c.mode;
+  int mode = 0, stop = 0, code = EXIT_CONVERGED;
+  bool fold = false;
+  double gam = 0.0, del = 0.0;
+  if (prev == -1) {
+    double t[4];
+    reduce_partials<4>(Pin, Gin, t);  // k_cg_start: rho, |r|^2, |r|_1, sum x Vol
+    c.maxr = sqrt(t[1]);
+  } else if (prev == 2) {
+    double t[3];
+    reduce_partials<3>(Pin, Gin, t);
+    const double rN = sqrt(t[0]), e1 = t[1] / st->b1, e2 = rN / st->b2;
+    fold = true;
+    if (!isfinite(t[0]) || !isfinite(t[1])) {
+      stop = 1;
+      code = EXIT_BREAKDOWN;
+    } else if (e1 < st->eps || e2 < st->eps) {
+      stop = 1;
+    }
+    c.maxr = rN;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->e1 = e1;
+      st->e2 = e2;
+      st->xvol = t[2];
+      st->mp_pass += 1;
+    }
+  } else if (prev == 0) {
+    double t[2];
+    reduce_partials<2>(Pin, Gin, t);
+    gam = t[0];
+    del = t[1];
+    mode = 1;
+  } else {
+    double t[4];
+    reduce_partials<4>(Pin, Gin, t);
+    gam = t[0];
+    del = t[1];
+    const double rN = sqrt(t[2]), e1 = t[3] / st->b1, e2 = rN / st->b2;
+    c.maxr = fmax(c.maxr, rN);
+    if (!isfinite(t[2])) {
+      stop = 1;
+      code = EXIT_BREAKDOWN;
+    }
+    mode = (e1 < st->eps || e2 < st->eps || rN < st->delta * c.maxr) ? 2 : 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->e1 = e1;
+      st->e2 = e2;
+    }
+  }
+  double al = 0.0, be = 0.0;
+  const bool first = c.iter == 0;
+  if (mode == 1 && !stop) {
+    if (c.iter >= st->max_iters) {
+      stop = 1;
+      code = EXIT_MAX;
+    } else {
+      be = first ? 0.0 : gam / c.gamma;
+      al = first ? gam / del : gam / (del - be * gam / c.alpha);
+      if (gam == 0.0 || !isfinite(gam) || !isfinite(al) || al == 0.0) {
+        stop = 1;
+        code = EXIT_BREAKDOWN;
+      }
+    }
+  }
+  const int in = c.set;
+  if (mode == 1 && !stop) {
+    c.gamma = gam;
+    c.alpha = al;
+    c.iter += 1;
+    c.set ^= 1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgsCarry o = c;
+    o.mode = mode;
+    o.done = stop;
+    st->cgs[j & 1] = o;
+    st->cgs_runs = j + 1;
+    st->a_iter = c.iter;
+    if (stop) {
+      st->exit_code = code;
+      st->done = 1;
+    }
+  }
+  if (fold) {
+    // x += e, e = 0: the correction since the last update joins x
+    for_each_quad(g, [&](int, int, size_t s0) {
+      float ee[4];
+      ld4(v.e + s0, ee);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[s0 + u] = x[s0 + u] + (double)ee[u];
+      const float z[4] = {0.f, 0.f, 0.f, 0.f};
+      st4(v.e + s0, z);
+    });
+  }
+  if (stop) return;
+  const float *lv = load_levels<float>(g);
+  if (mode == 2) {
+    // the true residual of x + e into rt (T reads x and e at neighbouring
+    // prisms and writes rt only at its own)
+    const double avec = st->avec;
+    const float *e = v.e;
+    float *rt = v.rt[in];
+    double acc[3] = {0.0, 0.0, 0.0};
+    FOR_EACH_CELL(g, q) {
+      int i, k;
+      cell_of(g, q, i, k);
+      double res, xs;
+      true_residual(g, rhs, x, avec, i, k, [&](size_t s, bool) { return e[s]; }, res, xs);
+      const Row A = ppe_row(g, i, k);
+      rt[(size_t)i * g.NS + k] = (float)(res / A.A0);
+      acc[0] += res * res;
+      acc[1] += fabs(res);
+      acc[2] += xs * (g.area[i] * g.dsig[k]);
+    }
+    write_partials<3>(acc, Pout);
+    return;
+  }
+  if (mode == 0) {
+    // om = A rt / A0, gamma = <A0 rt, rt>, delta = <A rt, rt>
+    const float *rt = v.rt[in];
+    float *om = v.om[in];
+    double acc[2] = {0.0, 0.0};
+    for_each_quad_gathered(
+        g, rt, rt, 0.f, true, [&](size_t) {},
+        [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+            const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+          float qv[5], wo[4];
+          quot5(C, lv, k0, qv);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+            const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+            wo[u] = R.in ? wv / R.A0 : 0.f;
+            if (R.in) {
+              acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
+              acc[1] += (double)wv * (double)fc[u];
+            }
+          }
+          st4(om + s0, wo);
+        });
+    write_partials<2>(acc, Pout);
+    return;
+  }
+  // mode N: iteration i with alpha_i, beta_i
+  const float af = (float)al, bf = (float)be;
+  const float *rt_i = v.rt[in], *om_i = v.om[in], *sg_i = v.sg[in];
+  float *rt_o = v.rt[in ^ 1], *om_o = v.om[in ^ 1], *sg_o = v.sg[in ^ 1];
+  // u_{i+1} at a slot, from rt, om, sg there: the same expressions wherever
+  // it is formed
+  auto form4 = [&](size_t off, float(&r)[4], float(&sg)[4], float(&o)[4]) {
+    float w[4], q[4] = {0.f, 0.f, 0.f, 0.f};
+    ld4(rt_i + off, r);
+    ld4(om_i + off, w);
+    if (!first) ld4(sg_i + off, q);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sg[u] = first ? w[u] : w[u] + bf * q[u];
+      o[u] = r[u] - af * sg[u];
+    }
+  };
+  auto val4 = [&](size_t off, float(&o)[4]) {
+    float r[4], sg[4];
+    form4(off, r, sg, o);
+  };
+  auto val1 = [&](size_t q) {
+    const float sg = first ? om_i[q] : om_i[q] + bf * sg_i[q];
+    return rt_i[q] - af * sg;
+  };
+  float ri[4], si[4], pi[4], ei[4];
+  auto own4 = [&](size_t off, float(&o)[4]) { form4(off, ri, si, o); };
+  auto pre = [&](size_t off) {
+    if (!first) ld4(v.p + off, pi);
+    ld4(v.e + off, ei);
+  };
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for_each_quad_formed<float>(
+      g, own4, val4, val1, pre,
+      [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+          const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+        float qv[5], ro[4], wo[4], so[4], po[4], eo[4];
+        quot5(C, lv, k0, qv);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+          const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+          const float pn = first ? ri[u] : ri[u] + bf * pi[u];
+          eo[u] = R.in ? ei[u] + af * pn : ei[u];
+          po[u] = R.in ? pn : 0.f;
+          ro[u] = R.in ? fc[u] : 0.f;
+          so[u] = R.in ? si[u] : 0.f;
+          wo[u] = R.in ? wv / R.A0 : 0.f;
+          if (R.in) {
+            const double rd = (double)R.A0 * (double)fc[u];
+            acc[0] += rd * (double)fc[u];
+            acc[1] += (double)wv * (double)fc[u];
+            acc[2] += rd * rd;
+            acc[3] += fabs(rd);
+          }
+        }
+        st4(v.e + s0, eo);
+        st4(v.p + s0, po);
+        st4(rt_o + s0, ro);
+        st4(om_o + s0, wo);
+        st4(sg_o + s0, so);
+      });
+  write_partials<4>(acc, Pout);
+}
+
 // ---------------------------------------------------------- refinement ----
 // After a refinement residual: the outer norms (true FP64 residual of the
 // current x against the mean-removed rhs, the reference's stopping rule
@@ -1186,6 +1447,18 @@ int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, fl
   return (int)hipGetLastError();
 }
 
+int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
+               SolverState *st, hipEvent_t *ev) {
+  // launch 0 reads k_cg_start's partials (part.u, solver grid); then the
+  // launches alternate between part.c (even) and part.u (odd) on their grid
+  double *Pout = (j & 1) ? part.u : part.c;
+  const double *Pin = j == 0 ? part.u : ((j & 1) ? part.c : part.u);
+  const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs);
+  const size_t lds = level_lds(g, sizeof(float));
+  LAUNCH_TIMED(ev, 0, k_cgs, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+  return (int)hipGetLastError();
+}
+
 int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st) {
   hipLaunchKernelGGL(k_bicg_init_state, dim3(1), dim3(NSX_BLOCK), 0, s, st, 0.0, 0, 0.0);
   return (int)hipGetLastError();
@@ -1266,6 +1539,16 @@ int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState
 int cg_blocks_per_cu(const Geo &g) {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cg_w, NSX_BLOCK, level_lds(g, 4)) !=
+          hipSuccess ||
+      n < 1)
+    return -1;
+  return n;
+}
+
+// resident blocks per CU of the single-reduction FP32 CG kernel
+int cgs_blocks_per_cu(const Geo &g) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cgs, NSX_BLOCK, level_lds(g, 4)) !=
           hipSuccess ||
       n < 1)
     return -1;

@@ -68,18 +68,20 @@ struct nsx_ctx {
   double *sm_planes = nullptr;  // SM_NPLANE x NZ
   int sm_n = 0;
   int precision = NSX_PREC_FP64;
-  int refine = NSX_REFINE_CG;
+  int refine = NSX_REFINE_CGSR;
   double delta = 0.1;
   double eps_inner = 1e-5;
   int max_passes = 30;
   int iter_hint32 = 0;
-  double delta_cg = 0.1;  // reliable-update fraction of NSX_REFINE_CG
+  double delta_cg = 0.1;  // reliable-update fraction of NSX_REFINE_CG and NSX_REFINE_CGSR
   int iter_hint_cg = 0;
+  int iter_hint_cgs = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   Timer t_k[3], t_stage, t_k32[3], t_u, t_cg[2];  // FP64 A, B, C; stage kernels; FP32 A, B, C; U; FP32 CG W, X
+  Timer t_cgs;                                     // single-reduction FP32 CG
   // (event index, timer: 0..2 FP64 solver, 5 stage, 6..8 FP32, 11 U, 12..13 FP32 CG)
   std::vector<std::pair<size_t, int>> pending;
 };
@@ -344,6 +346,14 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     Gc = Gc / NSX_NXCD * NSX_NXCD;
     if (Gc < NSX_NXCD) Gc = NSX_NXCD;
     g.grid_cg = (int)Gc;
+    const int per_cu_cgs = cgs_blocks_per_cu(g);
+    if (per_cu_cgs < 1) return nsx_fail(ctx, -2, "occupancy query of the single-reduction CG kernel failed");
+    long Gs = (long)prop.multiProcessorCount * per_cu_cgs;
+    if (Gs > need) Gs = need;
+    if (Gs > MAXGRID) Gs = MAXGRID;
+    Gs = Gs / NSX_NXCD * NSX_NXCD;
+    if (Gs < NSX_NXCD) Gs = NSX_NXCD;
+    g.grid_cgs = (int)Gs;
   }
   if (NS > 4096) return nsx_fail(ctx, -1, "NZ = %d exceeds the 4094 layers the solver's LDS level table holds", NZ);
   g.bctop = ctx->bctop;
@@ -724,8 +734,9 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[14] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr, nullptr, &ctx->t_stage, &ctx->t_k32[0],
-                  &ctx->t_k32[1], &ctx->t_k32[2], nullptr,        nullptr, &ctx->t_u,     &ctx->t_cg[0], &ctx->t_cg[1]};
+  Timer *t[15] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr,       nullptr,
+                  &ctx->t_stage,  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], nullptr,
+                  nullptr,        &ctx->t_u,      &ctx->t_cg[0],  &ctx->t_cg[1],  &ctx->t_cgs};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (t[pr.second] && hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -1004,9 +1015,67 @@ static int solve_cg(nsx_ctx *ctx, const double *rhs, const double *phi_in, doubl
   return 0;
 }
 
+// Mixed precision, NSX_REFINE_CGSR: the same FP32 CG with reliable updates
+// in the single-reduction form, one kernel per iteration (k_cgs); a launch
+// is an iteration, a true residual or the product after one, as the device
+// decides.  Fallback as in solve_cg.
+static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  const int cap = 2 * ctx->MaxIters + 100;
+  LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                        ctx->delta_cg, cap));
+  const CgsVecs v{ctx->fe, ctx->fp, {ctx->fr, ctx->fr2}, {ctx->fv, ctx->fp2}, {ctx->ft, ctx->fe2}};
+  int issued = 0;
+  const size_t pend0 = ctx->pending.size();
+  const int hint = ctx->iter_hint_cgs;
+  int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      hipEvent_t *ev = nullptr;
+      if (ctx->timing) {
+        ev = events(ctx, 2);
+        if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+      }
+      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev));
+      if (ev) ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 14});
+      ++issued;
+    }
+    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (ctx->st_host->done) break;
+    // an iteration is one launch, a true residual two more
+    if (issued > 3 * cap + 8) return nsx_fail(ctx, -4, "solver did not stop after %d launches", issued);
+    chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
+  }
+  const SolverState &h = *ctx->st_host;
+  // launches queued after the stop returned at once: keep those that ran
+  const size_t keep = pend0 + (size_t)h.cgs_runs;
+  if (ctx->timing && ctx->pending.size() > keep) ctx->pending.resize(keep);
+  if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
+    collect_timing(ctx);
+    const int its32 = h.a_iter, ups = h.mp_pass;
+    if (its32 > 0) ctx->iter_hint_cgs = its32;
+    return finish_fp64(ctx, rhs, x, its32, ups);
+  }
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  ctx->info.iters = h.a_iter;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.e1;
+  ctx->info.e2norm = h.e2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += h.a_iter;
+  ctx->info.outer_passes = h.mp_pass;
+  ctx->info.iters_fp32 = h.a_iter;
+  if (h.a_iter > 0) ctx->iter_hint_cgs = h.a_iter;
+  collect_timing(ctx);
+  return 0;
+}
+
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_CGSR) return solve_cgs(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
   return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
 }
@@ -1525,7 +1594,7 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
   if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE &&
-      refine != NSX_REFINE_CG)
+      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
   if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
@@ -1533,7 +1602,7 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   ctx->refine = refine;
   if (refine == NSX_REFINE_RELIABLE)
     ctx->delta = tol;
-  else if (refine == NSX_REFINE_CG)
+  else if (refine == NSX_REFINE_CG || refine == NSX_REFINE_CGSR)
     ctx->delta_cg = tol;
   else
     ctx->eps_inner = tol;
@@ -1561,6 +1630,7 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
   static const char *names_cg[2] = {"cg32_w", "cg32_x"};
   for (int q = 0; q < 2; ++q)
     if (!strcmp(name, names_cg[q])) t = &ctx->t_cg[q];
+  if (!strcmp(name, "cg32_sr")) t = &ctx->t_cgs;
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -1570,6 +1640,6 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
   for (int q = 0; q < 3; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
-  ctx->t_stage = ctx->t_u = ctx->t_cg[0] = ctx->t_cg[1] = Timer{};
+  ctx->t_stage = ctx->t_u = ctx->t_cg[0] = ctx->t_cg[1] = ctx->t_cgs = Timer{};
   return 0;
 }

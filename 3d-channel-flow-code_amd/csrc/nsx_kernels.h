@@ -17,6 +17,17 @@ struct Partials {
 // BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals).  Each
 // field has one writer kernel (block 0 of it) and is read only by later
 // kernels, so no block reads a value that its own kernel may be rewriting.
+// scalars of the single-reduction FP32 CG passed from launch to launch
+// (launch j reads SolverState::cgs[(j-1)&1], block 0 writes cgs[j&1])
+struct CgsCarry {
+  double gamma, alpha;  // gamma_i and alpha_i of the last iteration
+  double maxr;          // largest |r| since the last true residual
+  int mode;             // this launch's mode: 0 product (M), 1 iteration (N), 2 true residual (T)
+  int iter;             // iterations (launches in mode N) so far
+  int set;              // buffer set holding the current rt, om, sg
+  int done;
+};
+
 struct SolverState {
   double a_rho[2];   // rho_k = <rhat, r_k> by parity of k        (phase A)
   double b_alpha;    // alpha_k                                    (phase B; init: 1)
@@ -43,11 +54,21 @@ struct SolverState {
   int pad2;
   double a_beta;     // beta_k of y = rt + beta q                      (phase A)
   int ureq[2];       // FP32 CG: W_k asks U_{k+1} for a true residual, by parity of k (phase W)
+  CgsCarry cgs[2];   // single-reduction FP32 CG, by parity of the launch
+  int cgs_runs;      // launches of the single-reduction CG that ran (block 0 of each)
+  int pad3;
+};
+
+// the vectors of the single-reduction FP32 CG: e and p in place, the scaled
+// residual rt = r/A0, om = (A rt)/A0 and sg = s/A0 in two sets
+struct CgsVecs {
+  float *e, *p, *rt[2], *om[2], *sg[2];
 };
 
 int nsx_grid_for(uint32_t n);
 int solver_blocks_per_cu(const Geo &g);
 int cg_blocks_per_cu(const Geo &g);
+int cgs_blocks_per_cu(const Geo &g);
 
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
@@ -114,5 +135,9 @@ int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double 
                    float *scratch, float *e, Partials part, SolverState *st, double eps, double delta, int max_iters);
 int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *const v[7], Partials part,
                    SolverState *st, hipEvent_t *ev /* nullable: 4 events, start/stop of W, X */);
+// one launch of the single-reduction FP32 CG (launch j of the solve; its mode
+// is decided on the device); ev: one timed kernel
+int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
+               SolverState *st, hipEvent_t *ev);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

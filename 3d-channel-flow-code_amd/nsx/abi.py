@@ -25,6 +25,7 @@ REFINE_RELIABLE = 0
 REFINE_RESTART = 1
 REFINE_CASCADE = 2
 REFINE_CG = 3
+REFINE_CGSR = 4
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 

@@ -248,7 +248,7 @@ int nsx_write_tec_center(const char *dir, int counter, double time, int N_CELL0,
  * the Krylov vectors, SpMV and preconditioner in FP32 and keeps the solution,
  * the refinement residual, inner products and norms in FP64; the solve stops
  * when the true FP64 residual meets the reference's rule with eps
- * (nsx_set_params).  Refinement (default NSX_REFINE_CG, delta 0.1):
+ * (nsx_set_params).  Refinement (default NSX_REFINE_CGSR, delta 0.1):
  *   NSX_REFINE_RELIABLE  one FP32 iteration whose residual is replaced by the
  *                        true FP64 residual whenever it has fallen by tol
  *                        (delta, e.g. 0.1); max_passes unused;
@@ -262,8 +262,17 @@ int nsx_write_tec_center(const char *dir, int counter, double time, int N_CELL0,
  *                        true FP64 residual whenever it has fallen by tol
  *                        (delta, e.g. 0.1); max_passes unused.  The operator
  *                        is symmetric negative semi-definite, and CG's
- *                        smooth residual keeps FP32 from stalling. */
-enum nsx_refine { NSX_REFINE_RELIABLE = 0, NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2, NSX_REFINE_CG = 3 };
+ *                        smooth residual keeps FP32 from stalling;
+ *   NSX_REFINE_CGSR      the same CG in the single-reduction form of
+ *                        Chronopoulos and Gear: one kernel and one global
+ *                        reduction per iteration; tol as NSX_REFINE_CG. */
+enum nsx_refine {
+  NSX_REFINE_RELIABLE = 0,
+  NSX_REFINE_RESTART = 1,
+  NSX_REFINE_CASCADE = 2,
+  NSX_REFINE_CG = 3,
+  NSX_REFINE_CGSR = 4
+};
 int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
 
 /* ---- timing ---------------------------------------------------------------- */
@@ -271,7 +280,8 @@ int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int m
  * names: "bicg_a", "bicg_b", "bicg_c" (the three BiCGSTAB kernels in FP64,
  * iterations that ran only), "bicg32_a" .. "bicg32_c" (the same kernels of the
  * FP32 solve), "rel_u" (the reliable-update kernel), "cg32_w", "cg32_x"
- * (the two kernels of an FP32 CG iteration) and "stage" (all non-solver
+ * (the two kernels of an FP32 CG iteration), "cg32_sr" (the kernel of the
+ * single-reduction FP32 CG, every launch) and "stage" (all non-solver
  * kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);

@@ -16,7 +16,7 @@ import pytest
 
 import oracle
 from common import interior, ppe_residual, rel_err
-from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_RELIABLE, REFINE_RESTART
+from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_CGSR, REFINE_RELIABLE, REFINE_RESTART
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +28,7 @@ def _ctx(case):
 
 
 MODES = {"reliable": (REFINE_RELIABLE, 0.1), "restart": (REFINE_RESTART, 1e-5), "cascade": (REFINE_CASCADE, 1e-5),
-         "cg": (REFINE_CG, 0.1)}
+         "cg": (REFINE_CG, 0.1), "cgsr": (REFINE_CGSR, 0.1)}
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
@@ -69,6 +69,67 @@ def test_mixed_solve_tiny_exact_start():
     G.close()
     assert info["exit"] == "no solve" and info["iters"] == 0 and info["outer_passes"] == 0
     assert np.all(interior(c, phi) == 0.0)
+
+
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR])
+def test_cg_no_solve(refine):
+    """A zero right-hand side: the start meets the rule, no CG iteration runs."""
+    c = config_case(0, jitter=0.1)
+    G = _ctx(c)
+    G.init_state()
+    G.set_precision(PREC_MIXED, refine, 0.1, 30)
+    phi, pv, info = G.bicgstab(c.cell_field(), c.cell_field())
+    G.close()
+    assert info["exit"] == "no solve" and info["iters"] == 0, info
+    assert np.all(interior(c, phi) == 0.0)
+
+
+def test_single_reduction_cg_tracks_cg():
+    """The single-reduction form runs CG's iteration in another arrangement:
+    in FP32 its iteration and update counts stay within a few percent of the
+    two-kernel form's, and both solutions agree to the solve's tolerance."""
+    c = config_case(0, jitter=0.1)
+    rng = np.random.default_rng(11)
+    rhs = c.cell_field()
+    rhs[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
+    out = {}
+    for refine in (REFINE_CG, REFINE_CGSR):
+        G = _ctx(c)
+        G.init_state()
+        G.set_precision(PREC_MIXED, refine, 0.1, 30)
+        out[refine] = G.bicgstab(c.cell_field(), rhs)
+        G.close()
+    (p1, _, i1), (p2, _, i2) = out[REFINE_CG], out[REFINE_CGSR]
+    assert i1["exit"] == i2["exit"] == "converged", (i1, i2)
+    assert abs(i2["iters"] - i1["iters"]) <= max(3, 0.05 * i1["iters"]), (i1, i2)
+    assert abs(i2["outer_passes"] - i1["outer_passes"]) <= 3, (i1, i2)
+    assert rel_err(interior(c, p2), interior(c, p1)) <= 1e-6
+
+
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR])
+def test_cg_cap_falls_back_to_fp64(refine):
+    """A CG that reaches its cap (2 MaxIters + 100 iterations) keeps x at its
+    last update and the FP64 BiCGSTAB finishes from there: the solution
+    still meets the rule, and the FP32 iterations are those of the cap."""
+    c = config_case(0, jitter=0.1)
+    rng = np.random.default_rng(12)
+    rhs = c.cell_field()
+    rhs[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
+    G = _ctx(c)
+    G.init_state()
+    G.set_precision(PREC_MIXED, refine, 0.1, 30)
+    _, _, free = G.bicgstab(c.cell_field(), rhs)
+    n_cg = free["iters"]
+    if n_cg < 160:
+        pytest.skip(f"CG converges in {n_cg} iterations, too few to cap")
+    m = (n_cg - 130) // 2  # cap = n_cg - 30 or - 31
+    G.set_max_iters(m)
+    phi, _, info = G.bicgstab(c.cell_field(), rhs)
+    G.close()
+    assert info["iters_fp32"] == 2 * m + 100, (free, info)
+    assert info["exit"] == "converged" and info["iters"] > info["iters_fp32"], (free, info)
+    e1, e2 = ppe_residual(c, phi, rhs)
+    assert min(e1, e2) < 2 * c.params.eps, (e1, e2)
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
