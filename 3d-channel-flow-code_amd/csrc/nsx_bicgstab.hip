@@ -1085,8 +1085,11 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // has fallen below delta x its largest value since the last true residual,
 // or meets the stopping rule, the next launch (mode T) forms the true FP64
 // residual of x + e into rt; the launch after it (mode M) folds e into x,
-// applies the stopping rule to the true residual, and forms om = A rt / A0
-// with gamma and delta.  beta, alpha carry on across the update.  The first
+// applies the stopping rule to the true residual, forms om = A rt / A0 with
+// gamma and delta, and sg = A p / A0 afresh (the recurrence for s = A p
+// drifts in FP32, and with it the recursive residual; without the reset the
+// solve stalls once updates come less often than every decade).  beta,
+// alpha carry on across the update.  The first
 // launch of a solve is an M without the fold, after k_cg_start.
 //
 // Launch j reads the carried scalars from SolverState::cgs[(j-1)&1] and
@@ -1247,6 +1250,25 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           }
           st4(om + s0, wo);
         });
+    if (fold && c.iter > 0) {
+      // after a true residual, s = A p afresh: its recurrence drifts in FP32
+      // from the product it stands for, and with it the recursive residual
+      float *sg = v.sg[in];
+      for_each_quad_gathered(
+          g, (const float *)v.p, (const float *)v.p, 0.f, true, [&](size_t) {},
+          [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+              const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+            float qv[5], so[4];
+            quot5(C, lv, k0, qv);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+              const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+              so[u] = R.in ? sv / R.A0 : 0.f;
+            }
+            st4(sg + s0, so);
+          });
+    }
     write_partials<2>(acc, Pout);
     return;
   }

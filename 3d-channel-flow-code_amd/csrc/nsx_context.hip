@@ -1597,6 +1597,10 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
       refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
   if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
+  // the single-reduction recurrences drift faster in FP32: with updates
+  // less often than about every 1.3 decades the solve stalls at its cap
+  if (refine == NSX_REFINE_CGSR && tol < 0.05)
+    return nsx_fail(ctx, -1, "single-reduction CG needs delta >= 0.05 (got %g); NSX_REFINE_CG takes smaller", tol);
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
   ctx->precision = precision;
   ctx->refine = refine;

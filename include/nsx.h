@@ -265,7 +265,8 @@ int nsx_write_tec_center(const char *dir, int counter, double time, int N_CELL0,
  *                        smooth residual keeps FP32 from stalling;
  *   NSX_REFINE_CGSR      the same CG in the single-reduction form of
  *                        Chronopoulos and Gear: one kernel and one global
- *                        reduction per iteration; tol as NSX_REFINE_CG. */
+ *                        reduction per iteration; tol as NSX_REFINE_CG, at
+ *                        least 0.05 (its FP32 recurrences drift faster). */
 enum nsx_refine {
   NSX_REFINE_RELIABLE = 0,
   NSX_REFINE_RESTART = 1,

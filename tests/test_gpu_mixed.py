@@ -84,6 +84,17 @@ def test_cg_no_solve(refine):
     assert np.all(interior(c, phi) == 0.0)
 
 
+def test_single_reduction_cg_rejects_rare_updates():
+    """The single-reduction form needs a reliable update at least every 1.3
+    decades (delta >= 0.05); the two-kernel form takes smaller deltas."""
+    c = config_case(0, jitter=0.1)
+    G = _ctx(c)
+    with pytest.raises(RuntimeError, match="delta >= 0.05"):
+        G.set_precision(PREC_MIXED, REFINE_CGSR, 0.01, 30)
+    G.set_precision(PREC_MIXED, REFINE_CG, 0.01, 30)
+    G.close()
+
+
 def test_single_reduction_cg_tracks_cg():
     """The single-reduction form runs CG's iteration in another arrangement:
     in FP32 its iteration and update counts stay within a few percent of the
