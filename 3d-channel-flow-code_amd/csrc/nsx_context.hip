@@ -1588,6 +1588,45 @@ extern "C" int nsx_savetec_center(nsx_ctx *ctx, const char *dir, int counter, do
   return 0;
 }
 
+extern "C" int nsx_save_paraview_vertex(nsx_ctx *ctx, const char *dir, int counter, const double *xvt,
+                                        const double *yvt, const double *zvt) {
+  NEED_COMMIT(ctx);
+  if (!dir || !xvt || !yvt || !zvt) return nsx_fail(ctx, -1, "nsx_save_paraview_vertex: NULL argument");
+  const size_t nv = (size_t)ctx->NV * (ctx->NZ - 1);
+  std::vector<double> f(3 * nv);
+  for (int q = 0; q < 3; ++q) {
+    int rc = nsx_get_field(ctx, NSX_F_UFV + q, f.data() + q * nv, 0);
+    if (rc) return rc;
+  }
+  if (nsx_write_vtk_vertex(dir, counter, ctx->N0, ctx->NV, ctx->NZ, ctx->No_vp.data(), xvt, yvt, zvt, f.data(),
+                           f.data() + nv, f.data() + 2 * nv, nullptr, nullptr, nullptr))
+    return nsx_fail(ctx, -5, "nsx_save_paraview_vertex: cannot write V-%04d.vtk in %s", counter, dir);
+  return 0;
+}
+
+extern "C" int nsx_savetec_vc(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt,
+                              const double *yvt, const double *zvt, const double *xct, const double *yct,
+                              const double *zct, int u, int v, int w, int p) {
+  NEED_COMMIT(ctx);
+  if (!dir || !xvt || !yvt || !zvt || !xct || !yct || !zct)
+    return nsx_fail(ctx, -1, "nsx_savetec_vc: NULL argument");
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL) || kind_ok(ctx, p, FK_CELL))
+    return -1;
+  const size_t nv = (size_t)ctx->NV * (ctx->NZ - 1), nc = (size_t)ctx->NC * ctx->NZ;
+  std::vector<double> fv(4 * nv), fc(4 * nc);
+  const int ids[4] = {u, v, w, p};
+  for (int q = 0; q < 4; ++q) {
+    int rc = nsx_get_field(ctx, NSX_F_UFV + q, fv.data() + q * nv, 0);
+    if (!rc) rc = nsx_get_field(ctx, ids[q], fc.data() + q * nc, 0);
+    if (rc) return rc;
+  }
+  if (nsx_write_tec_vc(dir, counter, time, ctx->N0, ctx->NC, ctx->NV, ctx->NZ, ctx->No_vp.data(), xvt, yvt, zvt, xct,
+                       yct, zct, fv.data(), fv.data() + nv, fv.data() + 2 * nv, fv.data() + 3 * nv, fc.data(),
+                       fc.data() + nc, fc.data() + 2 * nc, fc.data() + 3 * nc))
+    return nsx_fail(ctx, -5, "nsx_savetec_vc: cannot write VC-%05d.tec in %s", counter, dir);
+  return 0;
+}
+
 // ---------------------------------------------------------------- timing --
 extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes) {
   if (!ctx) return -1;

@@ -340,3 +340,146 @@ extern "C" int nsx_write_tec_center(const char *dir, int counter, double time, i
   put_bricks(f, No_vp, N_CELL0, N_VERT, NZ - 2);
   return fclose(f) == 0 ? 0 : -3;
 }
+
+// the list-directed records of the VTK writer as the reference build's
+// compiler writes them: a leading blank, character items as they are, each
+// integer as a blank and its digits, and a blank between an integer and a
+// character item after it
+static void put_list(FILE *f, const char *a, int nint, const long long *iv, const char *b) {
+  fputc(' ', f);
+  fputs(a, f);
+  for (int j = 0; j < nint; ++j) fprintf(f, " %lld", iv[j]);
+  if (b) {
+    if (nint > 0) fputc(' ', f);
+    fputs(b, f);
+  }
+  fputc('\n', f);
+}
+
+// one write statement of format (n(1x,e12.5)) per vertex of every layer
+// k = 0..nl-1, with the values of up to three (N_VERT, nl) arrays (a NULL
+// array writes zeros)
+static void put_e_rows(FILE *f, int N_VERT, int nl, int n, const double *a, const double *b, const double *c) {
+  const double *v[3] = {a, b, c};
+  char e[64];
+  for (int k = 0; k < nl; ++k)
+    for (int i = 0; i < N_VERT; ++i) {
+      for (int q = 0; q < n; ++q) {
+        fmt_e(e, v[q] ? v[q][(size_t)i + (size_t)k * N_VERT] : 0.0, 12, 5);
+        fputc(' ', f);
+        fputs(e, f);
+      }
+      fputc('\n', f);
+    }
+}
+
+// SaveParaviewVertex (SaveParaviewVertex.F90:185-278): <dir>/V-NNNN.vtk, an
+// unstructured grid of the prisms (0-based vertex numbers) with the vertex
+// fields.  The reference labels ufv "pf" and usv "ps"; the solid-phase
+// velocities usv, vsv, wsv may be NULL (zero, as in the channel)
+extern "C" int nsx_write_vtk_vertex(const char *dir, int counter, int N_CELL0, int N_VERT, int NZ, const int *No_vp,
+                                    const double *xvt, const double *yvt, const double *zvt, const double *ufv,
+                                    const double *vfv, const double *wfv, const double *usv, const double *vsv,
+                                    const double *wsv) {
+  if (!dir || !No_vp || !xvt || !yvt || !zvt || !ufv || !vfv || !wfv || NZ < 3 || counter < 0 || counter > 9999)
+    return -1;
+  char fn[1024];
+  snprintf(fn, sizeof fn, "%s/V-%04d.vtk", dir, counter);
+  FILE *f = fopen(fn, "w");
+  if (!f) return -2;
+  const long long totv = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
+  fputs("# vtk DataFile Version 2.0\nUnstructured Grid Example\nASCII\nDATASET UNSTRUCTURED_GRID\n", f);
+  put_list(f, "POINTS ", 1, &totv, " float");
+  put_e_rows(f, N_VERT, NZ - 1, 3, xvt, yvt, zvt);
+  const long long ce[2] = {tote, 7 * tote};
+  put_list(f, "CELLS ", 2, ce, nullptr);
+  for (int l = 0; l < NZ - 2; ++l)
+    for (int i = 0; i < N_CELL0; ++i) {
+      const long long b0 = (long long)N_VERT * l - 1, t0 = (long long)N_VERT * (l + 1) - 1;
+      const long long v1 = No_vp[i], v2 = No_vp[i + N_CELL0], v3 = No_vp[i + 2 * (size_t)N_CELL0];
+      const long long q[7] = {6, v1 + b0, v2 + b0, v3 + b0, v1 + t0, v2 + t0, v3 + t0};
+      for (int j = 0; j < 7; ++j) {
+        fputc(' ', f);
+        put_i(f, q[j], 8);
+      }
+      fputc('\n', f);
+    }
+  put_list(f, "CELL_TYPES ", 1, &tote, nullptr);
+  for (long long e = 0; e < tote; ++e) fputs("       13\n", f);
+  put_list(f, " ", 0, nullptr, nullptr);
+  put_list(f, "POINT_DATA ", 1, &totv, nullptr);
+  put_list(f, "SCALARS pf     float", 0, nullptr, nullptr);
+  put_list(f, "LOOKUP_TABLE default", 0, nullptr, nullptr);
+  put_e_rows(f, N_VERT, NZ - 1, 1, ufv, nullptr, nullptr);
+  put_list(f, " ", 0, nullptr, nullptr);
+  put_list(f, "SCALARS ps     float", 0, nullptr, nullptr);
+  put_list(f, "LOOKUP_TABLE default", 0, nullptr, nullptr);
+  put_e_rows(f, N_VERT, NZ - 1, 1, usv, nullptr, nullptr);
+  put_list(f, " ", 0, nullptr, nullptr);
+  put_list(f, "VECTORS velocity_f float", 0, nullptr, nullptr);
+  put_e_rows(f, N_VERT, NZ - 1, 3, ufv, vfv, wfv);
+  put_list(f, " ", 0, nullptr, nullptr);
+  put_list(f, "VECTORS velocity_s float", 0, nullptr, nullptr);
+  put_e_rows(f, N_VERT, NZ - 1, 3, usv, vsv, wsv);
+  return fclose(f) == 0 ? 0 : -3;
+}
+
+// SavetecVC (SavetecVertexCenter.F90:212-305): <dir>/VC-NNNNN.tec, the
+// vertices (N_VERT, NZ-1) followed by the cell centres of the interior
+// layers as nodes, each prism split into five bricks around its centre node
+extern "C" int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int N_CELL, int N_VERT, int NZ,
+                                const int *No_vp, const double *xvt, const double *yvt, const double *zvt,
+                                const double *xct, const double *yct, const double *zct, const double *ufv,
+                                const double *vfv, const double *wfv, const double *pfv, const double *u,
+                                const double *v, const double *w, const double *p) {
+  if (!dir || !No_vp || !xvt || !yvt || !zvt || !xct || !yct || !zct || !ufv || !vfv || !wfv || !pfv || !u || !v ||
+      !w || !p || NZ < 3 || counter < 0 || counter > 99999)
+    return -1;
+  char fn[1024];
+  snprintf(fn, sizeof fn, "%s/VC-%05d.tec", dir, counter);
+  FILE *f = fopen(fn, "w");
+  if (!f) return -2;
+  const long long nodes_v = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
+  put_a(f, "TITLE     = \"nsmp 3D prism data\"", 70);
+  fputc('\n', f);
+  put_a(f, "VARIABLES = \"x\",\"y\",\"z\"", 70);
+  fputc('\n', f);
+  put_a(f, "\"uf\",\"vf\",\"wf\",\"pf\"", 70);
+  fputc('\n', f);
+  put_a(f, "ZONE N = ", 11);
+  put_i(f, nodes_v + tote, 7);
+  put_a(f, ", E = ", 8);
+  put_i(f, 5 * tote, 7);
+  put_a(f, ", DATAPACKING= BLOCK, ZONETYPE=FEBRICK", 40);
+  fputc('\n', f);
+  put_a(f, "StrandID = ", 11);
+  put_i(f, 1, 3);
+  put_a(f, ", SolutionTime = ", 17);
+  put_f(f, time, 12, 5);
+  fputc('\n', f);
+  const double *vv[7] = {xvt, yvt, zvt, ufv, vfv, wfv, pfv};
+  const double *cc[7] = {xct, yct, zct, u, v, w, p};
+  for (int q = 0; q < 7; ++q) {
+    put_e_block(f, Block{vv[q], (size_t)N_VERT, 0, N_VERT, 0, NZ - 1});
+    put_e_block(f, Block{cc[q], (size_t)N_CELL, 0, N_CELL0, 1, NZ - 1});
+  }
+  for (int l = 0; l < NZ - 2; ++l)
+    for (int i = 0; i < N_CELL0; ++i) {
+      const long long b0 = (long long)N_VERT * l, t0 = (long long)N_VERT * (l + 1);
+      const long long v1 = No_vp[i], v2 = No_vp[i + N_CELL0], v3 = No_vp[i + 2 * (size_t)N_CELL0];
+      const long long c = (long long)(i + 1) + (long long)N_CELL0 * l + nodes_v;
+      const long long q[5][8] = {{v1 + b0, v1 + b0, c, c, v2 + b0, v3 + b0, c, c},
+                                 {c, c, v1 + t0, v1 + t0, c, c, v3 + t0, v2 + t0},
+                                 {c, c, c, c, v1 + b0, v2 + b0, v2 + t0, v1 + t0},
+                                 {c, c, c, c, v2 + b0, v3 + b0, v3 + t0, v2 + t0},
+                                 {c, c, c, c, v3 + b0, v1 + b0, v1 + t0, v3 + t0}};
+      for (int r = 0; r < 5; ++r) {
+        for (int j = 0; j < 8; ++j) {
+          fputc(' ', f);
+          put_i(f, q[r][j], 8);
+        }
+        fputc('\n', f);
+      }
+    }
+  return fclose(f) == 0 ? 0 : -3;
+}

@@ -308,3 +308,55 @@
         stop 'nsx: fatal error'
       endif
       END SUBROUTINE SavetecCenter
+
+!  SaveParaviewVertex.F90:185-278 -- V-NNNN.vtk of the vertex fields
+      SUBROUTINE SaveParaviewVertex(alphafv,ufv,vfv,wfv,pfv,             &
+                                    alphasv,usv,vsv,wsv,psv,rhosv,       &
+                                    xvt,yvt,zvt,No_vp)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(:,:) :: alphafv, ufv, vfv, wfv, pfv
+      real*8, dimension(:,:) :: alphasv, usv, vsv, wsv, psv, rhosv
+      real*8, dimension(:,:) :: xvt, yvt, zvt
+      integer, dimension(:,:) :: No_vp
+      real*8 :: tFin, tIni, dt, tInisave, dtsave, LastTimeSave, time
+      integer :: tlimit, SaveCounter, Numictec, nc_ref, FinRK
+      common/times/tFin,tIni,dt,tInisave,dtsave,LastTimeSave,time,tlimit
+      common/integ/SaveCounter,Numictec,nc_ref,FinRK
+      if (nsx_write_vtk_vertex('../output/Serial'//c_null_char,          &
+           SaveCounter, nsx_n_cell0, nsx_n_vert, nsx_nz, No_vp,          &
+           xvt, yvt, zvt, ufv, vfv, wfv, usv, vsv, wsv) /= 0) then
+        write(*,'(a)') ' nsx: SaveParaviewVertex: cannot write ../output/Serial'
+        stop 'nsx: fatal error'
+      endif
+      END SUBROUTINE SaveParaviewVertex
+
+!  SavetecVertexCenter.F90:212-305 -- VC-NNNNN.tec, vertices and centres
+      SUBROUTINE SavetecVC(alphafnp,ufnp,vfnp,wfnp,pfnp,                 &
+                           alphasnp,usnp,vsnp,wsnp,psnp,rhos,            &
+                           xct,yct,zct,No_cp,                            &
+                           alphafv,ufv,vfv,wfv,pfv,                      &
+                           alphasv,usv,vsv,wsv,psv,rhosv,                &
+                           xvt,yvt,zvt,No_vp)
+      use nsx_dropin
+      implicit none
+      real*8, dimension(:,:) :: alphafnp, ufnp, vfnp, wfnp, pfnp
+      real*8, dimension(:,:) :: alphasnp, usnp, vsnp, wsnp, psnp, rhos
+      real*8, dimension(:,:) :: xct, yct, zct
+      integer, dimension(:,:) :: No_cp
+      real*8, dimension(:,:) :: alphafv, ufv, vfv, wfv, pfv
+      real*8, dimension(:,:) :: alphasv, usv, vsv, wsv, psv, rhosv
+      real*8, dimension(:,:) :: xvt, yvt, zvt
+      integer, dimension(:,:) :: No_vp
+      real*8 :: tFin, tIni, dt, tInisave, dtsave, LastTimeSave, time
+      integer :: tlimit, SaveCounter, Numictec, nc_ref, FinRK
+      common/times/tFin,tIni,dt,tInisave,dtsave,LastTimeSave,time,tlimit
+      common/integ/SaveCounter,Numictec,nc_ref,FinRK
+      if (nsx_write_tec_vc('../output/Serial'//c_null_char, SaveCounter, &
+           time, nsx_n_cell0, nsx_n_cell, nsx_n_vert, nsx_nz, No_vp,     &
+           xvt, yvt, zvt, xct, yct, zct, ufv, vfv, wfv, pfv,             &
+           ufnp, vfnp, wfnp, pfnp) /= 0) then
+        write(*,'(a)') ' nsx: SavetecVC: cannot write ../output/Serial'
+        stop 'nsx: fatal error'
+      endif
+      END SUBROUTINE SavetecVC

@@ -344,6 +344,46 @@
           integer(c_int) :: no_vp(*)
           real(c_double) :: xvt(*), yvt(*), zvt(*), u(*), v(*), w(*), p(*)
         end function
+        integer(c_int) function nsx_save_paraview_vertex(ctx, dir,       &
+             counter, xvt, yvt, zvt) bind(C, name="nsx_save_paraview_vertex")
+          import :: c_int, c_ptr, c_double, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter
+          real(c_double) :: xvt(*), yvt(*), zvt(*)
+        end function
+        integer(c_int) function nsx_savetec_vc(ctx, dir, counter, time,  &
+             xvt, yvt, zvt, xct, yct, zct, u, v, w, p)                  &
+             bind(C, name="nsx_savetec_vc")
+          import :: c_int, c_ptr, c_double, c_char
+          type(c_ptr), value :: ctx
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, u, v, w, p
+          real(c_double), value :: time
+          real(c_double) :: xvt(*), yvt(*), zvt(*), xct(*), yct(*), zct(*)
+        end function
+        integer(c_int) function nsx_write_vtk_vertex(dir, counter,       &
+             n_cell0, n_vert, nz, no_vp, xvt, yvt, zvt, ufv, vfv, wfv,   &
+             usv, vsv, wsv) bind(C, name="nsx_write_vtk_vertex")
+          import :: c_int, c_double, c_char
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, n_cell0, n_vert, nz
+          integer(c_int) :: no_vp(*)
+          real(c_double) :: xvt(*), yvt(*), zvt(*), ufv(*), vfv(*), wfv(*)
+          real(c_double) :: usv(*), vsv(*), wsv(*)
+        end function
+        integer(c_int) function nsx_write_tec_vc(dir, counter, time,     &
+             n_cell0, n_cell, n_vert, nz, no_vp, xvt, yvt, zvt, xct, yct,&
+             zct, ufv, vfv, wfv, pfv, u, v, w, p)                       &
+             bind(C, name="nsx_write_tec_vc")
+          import :: c_int, c_double, c_char
+          character(kind=c_char), dimension(*) :: dir
+          integer(c_int), value :: counter, n_cell0, n_cell, n_vert, nz
+          real(c_double), value :: time
+          integer(c_int) :: no_vp(*)
+          real(c_double) :: xvt(*), yvt(*), zvt(*), xct(*), yct(*), zct(*)
+          real(c_double) :: ufv(*), vfv(*), wfv(*), pfv(*), u(*), v(*), w(*), p(*)
+        end function
         integer(c_int) function nsx_set_precision(ctx, precision, refine,&
              tol, max_passes) bind(C, name="nsx_set_precision")
           import :: c_int, c_ptr, c_double

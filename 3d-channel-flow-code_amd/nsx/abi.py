@@ -39,7 +39,7 @@ EXPORTS = [
     "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
     "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
     "nsx_write_plane_stats", "nsx_format_real", "nsx_savetec_vertex", "nsx_savetec_center", "nsx_write_tec_vertex",
-    "nsx_write_tec_center",
+    "nsx_write_tec_center", "nsx_save_paraview_vertex", "nsx_savetec_vc", "nsx_write_vtk_vertex", "nsx_write_tec_vc",
 ]
 # the accumulators of module stats (nsx_sm_get)
 SM_NAMES = ["sm_uf", "sm_vf", "sm_wf", "sm_u2f", "sm_v2f", "sm_w2f", "sm_uvf", "sm_vwf", "sm_wuf", "sm_pf", "sm_p2f",
@@ -115,6 +115,10 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_savetec_center": [c_vp, c_cp, c_int, c_dbl, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int],
         "nsx_write_tec_vertex": [c_cp, c_int, c_dbl, c_int, c_int, c_int] + [c_vp] * 8,
         "nsx_write_tec_center": [c_cp, c_int, c_dbl, c_int, c_int, c_int, c_int] + [c_vp] * 8,
+        "nsx_save_paraview_vertex": [c_vp, c_cp, c_int, c_vp, c_vp, c_vp],
+        "nsx_savetec_vc": [c_vp, c_cp, c_int, c_dbl] + [c_vp] * 6 + [c_int] * 4,
+        "nsx_write_vtk_vertex": [c_cp, c_int, c_int, c_int, c_int] + [c_vp] * 10,
+        "nsx_write_tec_vc": [c_cp, c_int, c_dbl, c_int, c_int, c_int, c_int] + [c_vp] * 15,
     }
     for name, args in sig.items():
         f = getattr(lib, name)

@@ -306,6 +306,29 @@ class Context:
         self._check(self.lib.nsx_savetec_center(self.h, directory.encode(), int(counter), float(time),
                                                 x.ctypes.data, y.ctypes.data, z.ctypes.data, *ids))
 
+    def save_paraview_vertex(self, directory: str, counter: int, xvt, yvt, zvt) -> None:
+        """SaveParaviewVertex (SaveParaviewVertex.F90:185-278): directory/V-NNNN.vtk
+        of the resident vertex velocities."""
+        x, y, z = self._coords(xvt, yvt, zvt)
+        self._check(self.lib.nsx_save_paraview_vertex(self.h, directory.encode(), int(counter), x.ctypes.data,
+                                                      y.ctypes.data, z.ctypes.data))
+
+    def savetec_vc(self, directory: str, counter: int, time: float, xvt, yvt, zvt, xct, yct, zct,
+                   fields=("UFNP", "VFNP", "WFNP", "PFNP")) -> None:
+        """SavetecVC (SavetecVertexCenter.F90:212-305): directory/VC-NNNNN.tec of
+        the resident vertex fields and the cell fields named in fields, with
+        the cell-centre coordinates xct, yct, zct (N_CELL, NZ)."""
+        x, y, z = self._coords(xvt, yvt, zvt)
+        cs = []
+        for a in (xct, yct, zct):
+            a = np.asfortranarray(a, dtype=np.float64)
+            if a.shape != (self.case.N_CELL, self.case.NZ):
+                raise ValueError(f"cell coordinates must be {(self.case.N_CELL, self.case.NZ)}, got {a.shape}")
+            cs.append(a)
+        ids = [F[n] if isinstance(n, str) else int(n) for n in fields]
+        self._check(self.lib.nsx_savetec_vc(self.h, directory.encode(), int(counter), float(time), x.ctypes.data,
+                                            y.ctypes.data, z.ctypes.data, *[a.ctypes.data for a in cs], *ids))
+
     def set_max_iters(self, n: int) -> None:
         """MaxIters of the pressure solve (the case's other parameters kept)."""
         p = self.case.params

@@ -241,6 +241,25 @@ int nsx_write_tec_vertex(const char *dir, int counter, double time, int N_CELL0,
 int nsx_write_tec_center(const char *dir, int counter, double time, int N_CELL0, int N_CELL, int N_VERT, int NZ,
                          const int *No_vp, const double *xvt, const double *yvt, const double *zvt,
                          const double *u, const double *v, const double *w, const double *p);
+/* SaveParaviewVertex (SaveParaviewVertex.F90:185-278): <dir>/V-NNNN.vtk with
+ * the resident vertex velocities (the solid-phase ones are zero) */
+int nsx_save_paraview_vertex(nsx_ctx *ctx, const char *dir, int counter, const double *xvt, const double *yvt,
+                             const double *zvt);
+/* SavetecVC (SavetecVertexCenter.F90:212-305): <dir>/VC-NNNNN.tec, vertices
+ * and cell centres as nodes: the caller's coordinates xvt, yvt, zvt
+ * (N_VERT,NZ-1) and xct, yct, zct (N_CELL,NZ), the resident vertex fields
+ * NSX_F_UFV..PFV and the cell fields u, v, w, p (field ids) */
+int nsx_savetec_vc(nsx_ctx *ctx, const char *dir, int counter, double time, const double *xvt, const double *yvt,
+                   const double *zvt, const double *xct, const double *yct, const double *zct, int u, int v, int w,
+                   int p);
+/* their file writing alone, on host arrays (usv, vsv, wsv may be NULL) */
+int nsx_write_vtk_vertex(const char *dir, int counter, int N_CELL0, int N_VERT, int NZ, const int *No_vp,
+                         const double *xvt, const double *yvt, const double *zvt, const double *ufv, const double *vfv,
+                         const double *wfv, const double *usv, const double *vsv, const double *wsv);
+int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int N_CELL, int N_VERT, int NZ,
+                     const int *No_vp, const double *xvt, const double *yvt, const double *zvt, const double *xct,
+                     const double *yct, const double *zct, const double *ufv, const double *vfv, const double *wfv,
+                     const double *pfv, const double *u, const double *v, const double *w, const double *p);
 
 /* ---- precision form (no counterpart in the reference) ------------------------ */
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes

@@ -18,8 +18,9 @@
 !  step, glob_put_plane_mm at the end, whose files go to ../output/Stats of
 !  the working directory) and dumps the twelve accumulators; then it saves
 !  the final state as nsmp3D.F90:617-625 does for ChooseExit 1 and 2
-!  (SavetecVertex, SavetecCenter, SaveCounter 3, into ../output/Serial) and
-!  dumps the vertex coordinates xvt, yvt, zvt those files hold.
+!  (SavetecVertex, SavetecCenter, SaveParaviewVertex, SavetecVC, SaveCounter
+!  3, into ../output/Serial) and dumps the coordinates xvt, yvt, zvt, xct,
+!  yct, zct and the solid-phase vertex velocities those files hold.
 
       module nsxdump
       implicit none
@@ -53,7 +54,7 @@
 !     reference PROGRAM has them (nsmp3D.F90:55); grandientLSM's entry there
 !     does not match the routine and is left out
       USE interfaces, only: calcul_geometry, initial, LoopTimeInitial, LoopTimeUpdate, &
-                            SavetecVertex, SavetecCenter
+                            SavetecVertex, SavetecCenter, SaveParaviewVertex, SavetecVC
       implicit none
       character(len=512) :: casefile, outfile
       integer :: hdr(16), ucase, n1, n2, nsteps, mode, irk, istep, nbuf
@@ -579,6 +580,21 @@
       call SavetecCenter(alphafnp,ufnp,vfnp,wfnp,pfnp,      &
                          alphasnp,usnp,vsnp,wsnp,psnp,rhos, &
                          xvt,yvt,zvt,No_vp)
+      call SaveParaviewVertex(alphafv,ufv,vfv,wfv,pfv,      &
+                              alphasv,usv,vsv,wsv,psv,rhosv,&
+                              xvt,yvt,zvt,No_vp)
+      call SavetecVC(alphafnp,ufnp,vfnp,wfnp,pfnp,          &
+                     alphasnp,usnp,vsnp,wsnp,psnp,rhos,     &
+                     xct,yct,zct,No_cp,                     &
+                     alphafv,ufv,vfv,wfv,pfv,               &
+                     alphasv,usv,vsv,wsv,psv,rhosv,         &
+                     xvt,yvt,zvt,No_vp)
+      call dump_r8('usv', usv, N_VERT*(NZ-1))
+      call dump_r8('vsv', vsv, N_VERT*(NZ-1))
+      call dump_r8('wsv', wsv, N_VERT*(NZ-1))
+      call dump_r8('xct', xct, N_CELL*NZ)
+      call dump_r8('yct', yct, N_CELL*NZ)
+      call dump_r8('zct', zct, N_CELL*NZ)
       call dump_r8('xvt', xvt, N_VERT*(NZ-1))
       call dump_r8('yvt', yvt, N_VERT*(NZ-1))
       call dump_r8('zvt', zvt, N_VERT*(NZ-1))

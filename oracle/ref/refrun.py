@@ -25,7 +25,8 @@ def driver_path(nn: int) -> str:
 
 def ensure_built(nn: int) -> str:
     exe = driver_path(nn)
-    if not os.path.exists(exe):
+    src = os.path.join(HERE, "nsx_ref_driver.F90")
+    if not os.path.exists(exe) or os.path.getmtime(src) > os.path.getmtime(exe):
         subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(nn)], check=True,
                        stdout=subprocess.DEVNULL)
     return exe

@@ -82,7 +82,7 @@ def ppe_residual(case, phi, rhs):
 
 
 def vertex_coordinates(c):
-    """xvt, yvt, zvt (N_VERT, NZ-1) as initial.F90:380-388 sets them, with
+    """xvt, yvt, zvt (N_VERT, NZ-1) as initial.F90:382-388 sets them, with
     the channel's Hprv = 1 and hv = 0"""
     A = c.arrays
     nk = c.NZ - 1
@@ -90,3 +90,13 @@ def vertex_coordinates(c):
     yvt = np.asfortranarray(np.repeat(A["yv"][:, None], nk, axis=1))
     zvt = np.asfortranarray(np.repeat(A["sigv"][None, :] * 1.0 - 0.0, c.N_VERT, axis=0))
     return xvt, yvt, zvt
+
+
+def cell_coordinates(c):
+    """xct, yct, zct (N_CELL, NZ) as initial.F90:372-378 sets them, with the
+    channel's Hpr = 1 and h = 0"""
+    A = c.arrays
+    xct = np.asfortranarray(np.repeat(A["xc"][:, None], c.NZ, axis=1))
+    yct = np.asfortranarray(np.repeat(A["yc"][:, None], c.NZ, axis=1))
+    zct = np.asfortranarray(np.repeat(A["sig"][None, :] * 1.0 - 0.0, c.N_CELL, axis=0))
+    return xct, yct, zct

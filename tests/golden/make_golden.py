@@ -14,8 +14,9 @@ what they produce:
                   statistics (glob_sample after every step): SHA-256 of the
                   twelve accumulators (rows 1..N_CELL0) and the three files
                   glob_put_plane_mm wrote at the end; SHA-256, size and
-                  header lines of the Tecplot snapshots SavetecVertex and
-                  SavetecCenter wrote of the final state;
+                  header lines of the snapshots SavetecVertex, SavetecCenter,
+                  SavetecVC (Tecplot) and SaveParaviewVertex (VTK) wrote of
+                  the final state;
   cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
@@ -112,7 +113,8 @@ def main():
             serial[fn] = {"sha256": hashlib.sha256(txt.encode()).hexdigest(), "size": len(txt.encode()),
                           "lines": len(lines) - 1, "head": lines[:5]}
         coords = {nm: hashlib.sha256(np.ascontiguousarray(d[nm], dtype=np.float64).tobytes()).hexdigest()
-                  for nm in ("xvt", "yvt", "zvt")}
+                  for nm in ("xvt", "yvt", "zvt", "xct", "yct", "zct")}
+        assert not (d["usv"].any() or d["vsv"].any() or d["wsv"].any())  # the solid phase is at rest
         rec["serial"] = {"counter": 3, "time": float(d["time"][0]), "files": serial, "coords": coords}
         with open(os.path.join(HERE, f"cfg{cfg}_steps.json"), "w") as f:
             json.dump(rec, f, indent=1)

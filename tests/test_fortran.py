@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import interior, rel_err, tiny_case, vertex_coordinates
+from common import cell_coordinates, interior, rel_err, tiny_case, vertex_coordinates
 from conftest import REPO
 from nsx.case import config_case
 
@@ -81,7 +81,7 @@ def test_dropin_exports_reference_names():
         pytest.skip("the Fortran drop-in is not built (no Fortran compiler)")
     out = subprocess.run(["nm", FLIB], capture_output=True, text=True, check=True).stdout
     for sym in ("bicgstab_", "hydro_", "grandientlsm_", "sm_init_", "glob_sample_", "glob_put_plane_mm_",
-                "savetecvertex_", "saveteccenter_"):
+                "savetecvertex_", "saveteccenter_", "saveparaviewvertex_", "savetecvc_"):
         assert f" T {sym}\n" in out, sym
 
 
@@ -107,10 +107,11 @@ SAVETEC = os.path.join(REPO, "tests", "fortran", "_build", "savetec_driver")
 
 
 def test_savetec_dropins_write_reference_files(tmp_path):
-    """SavetecVertex and SavetecCenter of the Fortran drop-in, called under
-    the reference's argument lists with SaveCounter and time in its COMMON
-    blocks, on the oracle's state after the ten steps of the 32x16 quads x 32
-    layers channel: the reference's own two files, byte for byte."""
+    """SavetecVertex, SavetecCenter, SaveParaviewVertex and SavetecVC of the
+    Fortran drop-in, called under the reference's argument lists with
+    SaveCounter and time in its COMMON blocks, on the oracle's state after the
+    ten steps of the 32x16 quads x 32 layers channel: the reference's own four
+    files, byte for byte."""
     import hashlib
 
     from common import load_steps
@@ -128,6 +129,7 @@ def test_savetec_dropins_write_reference_files(tmp_path):
              f64([serial["time"]])]
     parts += [f64(a) for a in vertex_coordinates(c)]
     parts += [f64(O.state(n)) for n in ("ufv", "vfv", "wfv", "pfv", "ufnp", "vfnp", "wfnp", "pfnp")]
+    parts += [f64(a) for a in cell_coordinates(c)]
     fin = tmp_path / "snap.bin"
     fin.write_bytes(b"".join(parts))
     (tmp_path / "run").mkdir()

@@ -11,8 +11,9 @@ reference's own files.
   layers channel: the files hold the reference's values (its own files,
   tests/golden) to 1e-6 relative or 1e-8 absolute, the spread of two
   trajectories whose pressure solves stop at a 1e-10 residual.
-* the Tecplot snapshots (SavetecVertex, SavetecCenter) of resident fields:
-  byte-identical to the reference's files on the same state.
+* the snapshots (SavetecVertex, SavetecCenter, SavetecVC, SaveParaviewVertex)
+  of resident fields: byte-identical to the reference's files on the same
+  state.
 """
 import os
 import tempfile
@@ -21,7 +22,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import load_steps, vertex_coordinates
+from common import cell_coordinates, load_steps, vertex_coordinates
 from nsx.abi import SM_NAMES
 from nsx.case import config_case
 
@@ -123,25 +124,33 @@ def test_snapshots_of_resident_fields_match_reference_files():
     for _ in range(rec["nsteps"]):
         O.step()
     xvt, yvt, zvt = vertex_coordinates(c)
+    xct, yct, zct = cell_coordinates(c)
     G = _ctx(c)
     G.init_state()
     for f in ("UFV", "VFV", "WFV", "PFV", "UFNP", "VFNP", "WFNP", "PFNP"):
         G.put(f, O.state(f.lower()))
+
+    def write_all(ctx, td):
+        ctx.savetec_vertex(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        ctx.savetec_center(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        ctx.save_paraview_vertex(td, serial["counter"], xvt, yvt, zvt)
+        ctx.savetec_vc(td, serial["counter"], serial["time"], xvt, yvt, zvt, xct, yct, zct)
+
     with tempfile.TemporaryDirectory() as td:
-        G.savetec_vertex(td, serial["counter"], serial["time"], xvt, yvt, zvt)
-        G.savetec_center(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        write_all(G, td)
         assert sorted(os.listdir(td)) == sorted(serial["files"])
         for fn, want in serial["files"].items():
             with open(os.path.join(td, fn), "rb") as f:
                 assert hashlib.sha256(f.read()).hexdigest() == want["sha256"], fn
-        ref = {fn: _snapshot_numbers(os.path.join(td, fn)) for fn in serial["files"]}
+        # the numeric comparison after device steps covers the two Tecplot
+        # files whose blocks are one field each
+        ref = {fn: _snapshot_numbers(os.path.join(td, fn)) for fn in ("V-0003.tec", "C-00003.tec")}
     G2 = _ctx(c)
     G2.init_state()
     for _ in range(rec["nsteps"]):
         G2.step(1, 0)
     with tempfile.TemporaryDirectory() as td:
-        G2.savetec_vertex(td, serial["counter"], serial["time"], xvt, yvt, zvt)
-        G2.savetec_center(td, serial["counter"], serial["time"], xvt, yvt, zvt)
+        write_all(G2, td)
         for fn, b in ref.items():
             a = _snapshot_numbers(os.path.join(td, fn))
             assert a.shape == b.shape, fn

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import STATE_MAP, digest, load_steps, load_tiny, tiny_case, vertex_coordinates
+from common import STATE_MAP, cell_coordinates, digest, load_steps, load_tiny, tiny_case, vertex_coordinates
 from nsx.case import config_case, make_case
 
 GEOM_CELL = ["xc", "yc", "areaCell", "No_cp", "bcuc", "bcvc", "bcwc", "bcpc"]
@@ -150,16 +150,18 @@ def test_oracle_trajectory_bitwise(cfg):
 
 
 def check_snapshots(lib, c, serial, f):
-    """The Tecplot files SavetecVertex and SavetecCenter wrote of the final
-    state == the library's writers fed the same fields, byte for byte"""
+    """The snapshots SavetecVertex, SavetecCenter, SavetecVC (Tecplot) and
+    SaveParaviewVertex (VTK) wrote of the final state == the library's
+    writers fed the same fields, byte for byte"""
     import tempfile
 
     xvt, yvt, zvt = vertex_coordinates(c)
-    for nm, a in (("xvt", xvt), ("yvt", yvt), ("zvt", zvt)):
+    xct, yct, zct = cell_coordinates(c)
+    for nm, a in (("xvt", xvt), ("yvt", yvt), ("zvt", zvt), ("xct", xct), ("yct", yct), ("zct", zct)):
         assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() == serial["coords"][nm], nm
     P = lambda a: np.asfortranarray(a, dtype=np.float64).ctypes.data  # noqa: E731
     no_vp = np.asfortranarray(c.arrays["No_vp"], dtype=np.int32)
-    keep = [f[n] for n in f] + [xvt, yvt, zvt, no_vp]  # noqa: F841  (alive across the calls)
+    keep = [f[n] for n in f] + [xvt, yvt, zvt, xct, yct, zct, no_vp]  # noqa: F841  (alive across the calls)
     with tempfile.TemporaryDirectory() as td:
         assert lib.nsx_write_tec_vertex(td.encode(), serial["counter"], serial["time"], c.N_CELL0, c.N_VERT, c.NZ,
                                         no_vp.ctypes.data, P(xvt), P(yvt), P(zvt), P(f["ufv"]), P(f["vfv"]),
@@ -167,6 +169,13 @@ def check_snapshots(lib, c, serial, f):
         assert lib.nsx_write_tec_center(td.encode(), serial["counter"], serial["time"], c.N_CELL0, c.N_CELL,
                                         c.N_VERT, c.NZ, no_vp.ctypes.data, P(xvt), P(yvt), P(zvt), P(f["ufnp"]),
                                         P(f["vfnp"]), P(f["wfnp"]), P(f["pfnp"])) == 0
+        assert lib.nsx_write_vtk_vertex(td.encode(), serial["counter"], c.N_CELL0, c.N_VERT, c.NZ, no_vp.ctypes.data,
+                                        P(xvt), P(yvt), P(zvt), P(f["ufv"]), P(f["vfv"]), P(f["wfv"]), None, None,
+                                        None) == 0
+        assert lib.nsx_write_tec_vc(td.encode(), serial["counter"], serial["time"], c.N_CELL0, c.N_CELL, c.N_VERT,
+                                    c.NZ, no_vp.ctypes.data, P(xvt), P(yvt), P(zvt), P(xct), P(yct), P(zct),
+                                    P(f["ufv"]), P(f["vfv"]), P(f["wfv"]), P(f["pfv"]), P(f["ufnp"]), P(f["vfnp"]),
+                                    P(f["wfnp"]), P(f["pfnp"])) == 0
         assert sorted(os.listdir(td)) == sorted(serial["files"])
         for fn, want in serial["files"].items():
             data = open(os.path.join(td, fn), "rb").read()
