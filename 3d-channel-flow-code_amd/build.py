@@ -42,7 +42,7 @@ def _run(cmd):
 
 def build_host():
     os.makedirs(OUT, exist_ok=True)
-    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra",
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra", "-I", os.path.join(REPO, "include"),
           "-o", os.path.join(OUT, "libnsx_host.so"), os.path.join(HERE, "host", "nsx_case.c"), "-lm"])
 
 

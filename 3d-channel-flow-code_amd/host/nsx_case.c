@@ -171,6 +171,107 @@ static void build_sigma(nsx_case *c) {
   sig[NZ - 1] = sig[NZ - 2] + dsig[NZ - 2];
 }
 
+/* ---------------------------------------------------- search indexes --- */
+/* The reference finds periodic partners by scanning every vertex or cell for
+ * each ghost cell and boundary vertex (calcul_geometry.F90:484-562,
+ * 920-1025).  Here the boundary vertices sit in a uniform bucket grid (cell
+ * size h >= 1e-6, so every vertex within 3e-7 of a point lies in its 3x3
+ * buckets) and the boundary cells in a map from edges to cells.  Candidates
+ * are tested with the reference's own conditions in increasing index order,
+ * so the matches, their counts and the match kept are those of the scans. */
+typedef struct {
+  double x0, y0, h;
+  long nbx, nby;
+  long *key; /* sorted bucket keys, with the vertex of each */
+  int *idx;
+  int n;
+} vgrid;
+
+typedef struct {
+  long key;
+  int v;
+} kv_rec;
+
+static int kv_cmp(const void *a, const void *b) {
+  const kv_rec *p = a, *q = b;
+  if (p->key != q->key) return (p->key > q->key) - (p->key < q->key);
+  return (p->v > q->v) - (p->v < q->v);
+}
+
+static int int_cmp(const void *a, const void *b) {
+  int p = *(const int *)a, q = *(const int *)b;
+  return (p > q) - (p < q);
+}
+
+static void vgrid_build(vgrid *g, const double *xv, const double *yv, const int *nbev, int NV) {
+  double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+  int nb = 0;
+  for (int v = 0; v < NV; ++v) {
+    if (nbev[v] == 0) continue;
+    if (nb == 0 || xv[v] < xmin) xmin = xv[v];
+    if (nb == 0 || xv[v] > xmax) xmax = xv[v];
+    if (nb == 0 || yv[v] < ymin) ymin = yv[v];
+    if (nb == 0 || yv[v] > ymax) ymax = yv[v];
+    ++nb;
+  }
+  double h = nb > 0 ? 2.0 * ((xmax - xmin) + (ymax - ymin)) / nb : 1.0;
+  if (!(h >= 1.0e-6)) h = 1.0e-6;
+  g->h = h;
+  g->x0 = xmin - 2.0 * h;
+  g->y0 = ymin - 2.0 * h;
+  g->nbx = (long)((xmax - g->x0) / h) + 3;
+  g->nby = (long)((ymax - g->y0) / h) + 3;
+  kv_rec *r = xcalloc((size_t)nb, sizeof(kv_rec));
+  int m = 0;
+  for (int v = 0; v < NV; ++v) {
+    if (nbev[v] == 0) continue;
+    const long bx = (long)floor((xv[v] - g->x0) / h), by = (long)floor((yv[v] - g->y0) / h);
+    r[m].key = bx * g->nby + by;
+    r[m].v = v;
+    ++m;
+  }
+  qsort(r, (size_t)nb, sizeof(kv_rec), kv_cmp);
+  g->key = xcalloc((size_t)nb, sizeof(long));
+  g->idx = xcalloc((size_t)nb, sizeof(int));
+  for (int j = 0; j < nb; ++j) {
+    g->key[j] = r[j].key;
+    g->idx[j] = r[j].v;
+  }
+  g->n = nb;
+  free(r);
+}
+
+static void vgrid_free(vgrid *g) {
+  free(g->key);
+  free(g->idx);
+}
+
+/* the boundary vertices in the 3x3 buckets around (x, y), ascending and
+ * without repeats, appended to out[*n] (room for cap) */
+static void vgrid_near(const vgrid *g, double x, double y, int *out, int *n, int cap) {
+  const double fx = floor((x - g->x0) / g->h), fy = floor((y - g->y0) / g->h);
+  if (!(fx > -2.0 && fx < (double)g->nbx + 1.0 && fy > -2.0 && fy < (double)g->nby + 1.0)) return;
+  const long bx = (long)fx, by = (long)fy;
+  for (long dx = -1; dx <= 1; ++dx)
+    for (long dy = -1; dy <= 1; ++dy) {
+      const long cx = bx + dx, cy = by + dy;
+      if (cx < 0 || cx >= g->nbx || cy < 0 || cy >= g->nby) continue;
+      const long k = cx * g->nby + cy;
+      int lo = 0, hi = g->n;
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (g->key[mid] < k) lo = mid + 1;
+        else hi = mid;
+      }
+      for (int j = lo; j < g->n && g->key[j] == k && *n < cap; ++j) out[(*n)++] = g->idx[j];
+    }
+  qsort(out, (size_t)*n, sizeof(int), int_cmp);
+  int m = 0;
+  for (int j = 0; j < *n; ++j)
+    if (m == 0 || out[j] != out[m - 1]) out[m++] = out[j];
+  *n = m;
+}
+
 /* ------------------------------------------------------------ geometry --- */
 /* calcul_geometry.F90:290-367 (per-cell), 402-578 (ghost cells, sequential
  * periodic branch), 851-1040 (tags, vertex pairs), 1055-1076 (IDW weights),
@@ -209,7 +310,25 @@ static int build_geometry(nsx_case *c) {
   c->TotalVolume = Volsum;
 
   /* ghost cells (sequential build, KeyTESTpBC) */
-  const double XDIni = 0.0, XDFin = c->Lx, YDIni = 0.0, YDFin = c->Ly;
+  const double XDIni = c->XDIni, XDFin = c->XDFin, YDIni = c->YDIni, YDFin = c->YDFin;
+  vgrid vg;
+  vgrid_build(&vg, xv, yv, nbev, NV);
+  const int cap = 4 * vg.n + 16;
+  int *cand = xcalloc((size_t)cap, sizeof(int));
+  /* edges of the boundary cells, (lower, higher vertex) -> cell, ascending */
+  int nem = 0;
+  kv_rec *em = xcalloc((size_t)3 * N0 + 1, sizeof(kv_rec));
+  for (int i0 = 0; i0 < N0; ++i0) {
+    if (nbe[i0] == 0) continue;
+    for (int j = 0; j < 3; ++j) {
+      const long a1 = VP(i0, j), a2 = VP(i0, (j + 1) % 3);
+      em[nem].key = (a1 < a2 ? a1 : a2) * (long)NV + (a1 < a2 ? a2 : a1);
+      em[nem].v = i0;
+      ++nem;
+    }
+  }
+  qsort(em, (size_t)nem, sizeof(kv_rec), kv_cmp);
+  int gerr = 0;
   int ii = N0;
   for (int i = 0; i < N0; ++i) {
     if (nbe[i] <= 0) continue;
@@ -218,7 +337,8 @@ static int build_geometry(nsx_case *c) {
       if (!(nc < 1 || nc > N0)) continue;
       if (ii >= NC) {
         snprintf(c->err, sizeof c->err, "more ghost cells than N_CELL allows");
-        return -1;
+        gerr = -1;
+        goto done;
       }
       int g = ii; /* 0-based ghost row */
       ii = ii + 1;
@@ -257,34 +377,52 @@ static int build_geometry(nsx_case *c) {
       }
       if (dxl * dyl > F32_1EM7) {
         snprintf(c->err, sizeof c->err, "ghost cell of cell %d edge %d has no periodic partner", i + 1, j + 1);
-        return -1;
+        gerr = -1;
+        goto done;
       }
-      int jv1p = -1, jv2p = -1, ncount = 0;
+      int jv1p = -1, jv2p = -1, ncount = 0, ncand = 0;
+      /* the boundary vertex at each end of the edge shifted across the box */
       xee = xv[jv1] + dxl;
       yee = yv[jv1] + dyl;
-      for (int nv = 0; nv < NV; ++nv)
-        if (nbev[nv] != 0 && fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
+      vgrid_near(&vg, xee, yee, cand, &ncand, cap);
+      for (int q = 0; q < ncand; ++q) {
+        const int nv = cand[q];
+        if (fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
           jv1p = nv;
           ++ncount;
         }
+      }
       if (ncount != 1) goto pairfail;
       ncount = 0;
+      ncand = 0;
       xee = xv[jv2] + dxl;
       yee = yv[jv2] + dyl;
-      for (int nv = 0; nv < NV; ++nv)
-        if (nbev[nv] != 0 && fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
+      vgrid_near(&vg, xee, yee, cand, &ncand, cap);
+      for (int q = 0; q < ncand; ++q) {
+        const int nv = cand[q];
+        if (fabs(xee - xv[nv]) < F32_1EM7 && fabs(yee - yv[nv]) < F32_1EM7) {
           jv2p = nv;
           ++ncount;
         }
+      }
       if (ncount != 1) goto pairfail;
       {
         /* partner cell: the boundary triangle holding edge (jv1p, jv2p); the
-         * reference tests products of 1-based vertex numbers */
+         * reference tests products of 1-based vertex numbers, which hold
+         * only for the cells having both vertices: those of the edge map */
         long p1 = (long)(jv1p + 1) * (jv2p + 1);
         int cc = -1;
         ncount = 0;
-        for (int i0 = 0; i0 < N0; ++i0) {
-          if (nbe[i0] == 0) continue;
+        const long ek = (long)(jv1p < jv2p ? jv1p : jv2p) * NV + (jv1p < jv2p ? jv2p : jv1p);
+        int lo = 0, hi = nem;
+        while (lo < hi) {
+          const int mid = (lo + hi) / 2;
+          if (em[mid].key < ek) lo = mid + 1;
+          else hi = mid;
+        }
+        for (int q = lo; q < nem && em[q].key == ek; ++q) {
+          const int i0 = em[q].v;
+          if (q > lo && em[q - 1].v == i0) continue;
           long a10 = VP(i0, 0) + 1, a20 = VP(i0, 1) + 1, a30 = VP(i0, 2) + 1;
           if (a10 * a20 == p1) {
             if (a10 == jv1p + 1 || a10 == jv2p + 1) { cc = (int)a30 - 1; CP(g, 2) = i0 + 1; ++ncount; }
@@ -305,11 +443,30 @@ static int build_geometry(nsx_case *c) {
       continue;
     pairfail:
       snprintf(c->err, sizeof c->err, "periodic matching failed for cell %d edge %d", i + 1, j + 1);
-      return -1;
+      gerr = -1;
+      goto done;
     }
+  }
+done:
+  free(em);
+  if (gerr) {
+    free(cand);
+    vgrid_free(&vg);
+    return gerr;
   }
   c->N_CELLexact = ii;
   c->N_CELLghost = ii - N0;
+  /* every edge of an interior cell now has a cell behind it (a cell that the
+   * mesh does not mark as a boundary cell keeps its out-of-range entries) */
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (CP(i, j) < 1 || CP(i, j) > ii) {
+        snprintf(c->err, sizeof c->err, "cell %d edge %d has no neighbour (No_cp = %d, nbe = %d)", i + 1, j + 1,
+                 CP(i, j), nbe[i]);
+        free(cand);
+        vgrid_free(&vg);
+        return -1;
+      }
 
   /* vertex tags (calcul_geometry.F90:895-918) and periodic pairs (:920-1025) */
   int *tagXV = c->tagXV, *tagYV = c->tagYV;
@@ -327,12 +484,29 @@ static int build_geometry(nsx_case *c) {
     c->vertex_pair[nv] = -1;
     c->vertex_edge[nv] = -1;
   }
+  /* with every boundary vertex inside the box (to 1e-7), a partner across
+   * it lies within 2e-7 of the point shifted by the box length, so the
+   * buckets around the four shifted points hold every candidate */
+  int inbox = 1;
+  for (int nv = 0; nv < NV && inbox; ++nv)
+    if (nbev[nv] != 0 && !(xv[nv] >= XDIni - 1.0e-7 && xv[nv] <= XDFin + 1.0e-7 && yv[nv] >= YDIni - 1.0e-7 &&
+                           yv[nv] <= YDFin + 1.0e-7))
+      inbox = 0;
   for (int nv = 0; nv < NV; ++nv) {
     double xee = xv[nv], yee = yv[nv];
     int ncount = (tagXV[nv] * tagYV[nv] > 0) ? 2 : ((tagXV[nv] > 0 || tagYV[nv] > 0) ? 1 : 0);
     if (ncount == 1) {
-      int vcount = 0;
-      for (int i = 0; i < NV; ++i) {
+      int vcount = 0, ncand = 0;
+      if (inbox) {
+        const double Lx = XDFin - XDIni, Ly = YDFin - YDIni;
+        vgrid_near(&vg, xee, yee + Ly, cand, &ncand, cap);
+        vgrid_near(&vg, xee, yee - Ly, cand, &ncand, cap);
+        vgrid_near(&vg, xee + Lx, yee, cand, &ncand, cap);
+        vgrid_near(&vg, xee - Lx, yee, cand, &ncand, cap);
+      }
+      const int nscan = inbox ? ncand : NV;
+      for (int q = 0; q < nscan; ++q) {
+        const int i = inbox ? cand[q] : q;
         if (nbev[i] == 0) continue;
         if (fabs(xee - xv[i]) < 1.0e-7 && fabs(yee - yv[i]) > (YDFin - YDIni - 1.0e-7)) {
           ++vcount;
@@ -346,11 +520,15 @@ static int build_geometry(nsx_case *c) {
       if (vcount == 0) c->vertex_pair[nv] = -1;
       else if (vcount != 1) {
         snprintf(c->err, sizeof c->err, "vertex %d: %d periodic partners", nv + 1, vcount);
+        free(cand);
+        vgrid_free(&vg);
         return -1;
       }
     }
     if (ncount == 2) c->vertex_edge[nv] = 1; /* XPB*YPB = 1 */
   }
+  free(cand);
+  vgrid_free(&vg);
 
   /* inverse-distance weights, interior cells only (calcul_geometry.F90:1055-1076) */
   for (int nv = 0; nv < NV; ++nv) {
@@ -544,20 +722,7 @@ static void *add_array(nsx_case *c, const char *name, int type, size_t n) {
   return p;
 }
 
-nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
-                          double jitter, int bed, double Ubulk, double noise, uint64_t seed) {
-  nsx_case *c = xcalloc(1, sizeof(nsx_case));
-  c->nx = nx; c->ny = ny; c->NN = nn; c->NZ = nn + 2;
-  c->Lx = Lx; c->Ly = Ly; c->H = H; c->stretch = stretch;
-  c->jitter = jitter; c->bed = bed; c->Ubulk = Ubulk; c->noise = noise; c->seed = seed;
-  c->N_CELL0 = 2 * nx * ny;
-  c->N_VERT = (nx + 1) * (ny + 1);
-  c->N_CELL = c->N_CELL0 + 100 * nn; /* N_CELLghostMAX = 100*NN (common.mpf:113) */
-  if (nx < 2 || ny < 2 || nn < 2 || 2 * (nx + ny) > 100 * nn) {
-    snprintf(c->err, sizeof c->err, "bad sizes nx=%d ny=%d nn=%d", nx, ny, nn);
-    c->status = -1;
-    return c;
-  }
+static void alloc_case(nsx_case *c) {
   const size_t N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL, NZ = c->NZ;
   c->No_vp = add_array(c, "No_vp", NSX_T_I32, N0 * 3);
   c->No_cp = add_array(c, "No_cp", NSX_T_I32, NC * 3);
@@ -621,7 +786,12 @@ nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H
   c->w0 = add_array(c, "w0", NSX_T_F64, NC * NZ);
   c->p0 = add_array(c, "p0", NSX_T_F64, NC * NZ);
 
-  if (build_mesh(c) || (build_sigma(c), 0) || build_geometry(c) || build_bc(c)) {
+}
+
+/* sigma levels, geometry, flags, PPE matrix, wall distance and the initial
+ * field of a case whose mesh is set */
+static nsx_case *finish_case(nsx_case *c) {
+  if ((build_sigma(c), 0) || build_geometry(c) || build_bc(c)) {
     c->status = -1;
     return c;
   }
@@ -631,6 +801,300 @@ nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H
   build_ic(c);
   c->status = 0;
   return c;
+}
+
+nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
+                          double jitter, int bed, double Ubulk, double noise, uint64_t seed) {
+  nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  c->nx = nx; c->ny = ny; c->NN = nn; c->NZ = nn + 2;
+  c->Lx = Lx; c->Ly = Ly; c->H = H; c->stretch = stretch;
+  c->jitter = jitter; c->bed = bed; c->Ubulk = Ubulk; c->noise = noise; c->seed = seed;
+  c->N_CELL0 = 2 * nx * ny;
+  c->N_VERT = (nx + 1) * (ny + 1);
+  c->N_CELL = c->N_CELL0 + 100 * nn; /* N_CELLghostMAX = 100*NN (common.mpf:113) */
+  c->XDIni = 0.0;
+  c->XDFin = Lx;
+  c->YDIni = 0.0;
+  c->YDFin = Ly;
+  if (nx < 2 || ny < 2 || nn < 2 || 2 * (nx + ny) > 100 * nn) {
+    snprintf(c->err, sizeof c->err, "bad sizes nx=%d ny=%d nn=%d", nx, ny, nn);
+    c->status = -1;
+    return c;
+  }
+  alloc_case(c);
+  if (build_mesh(c)) {
+    c->status = -1;
+    return c;
+  }
+  return finish_case(c);
+}
+
+nsx_case *nsx_case_from_mesh(int N_CELL0, int N_VERT, int nn, const int *No_vp, const double *xv, const double *yv,
+                             const int *No_cp, const double *zbv, const int *nbe, const int *nbev, double XDIni,
+                             double XDFin, double YDIni, double YDFin, double H, double stretch, double Ubulk,
+                             double noise, uint64_t seed) {
+  nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  c->NN = nn;
+  c->NZ = nn + 2;
+  c->N_CELL0 = N_CELL0;
+  c->N_VERT = N_VERT;
+  c->N_CELL = N_CELL0 + 100 * nn; /* N_CELLghostMAX = 100*NN (common.mpf:113) */
+  c->Lx = XDFin - XDIni;
+  c->Ly = YDFin - YDIni;
+  c->XDIni = XDIni;
+  c->XDFin = XDFin;
+  c->YDIni = YDIni;
+  c->YDFin = YDFin;
+  c->H = H;
+  c->stretch = stretch;
+  c->Ubulk = Ubulk;
+  c->noise = noise;
+  c->seed = seed;
+  if (N_CELL0 < 1 || N_VERT < 3 || nn < 2 || !No_vp || !xv || !yv || !No_cp || !nbe || !nbev ||
+      !(XDFin > XDIni) || !(YDFin > YDIni)) {
+    snprintf(c->err, sizeof c->err, "bad mesh arguments");
+    c->status = -1;
+    return c;
+  }
+  alloc_case(c);
+  const size_t N0 = (size_t)c->N_CELL0, NC = (size_t)c->N_CELL;
+  for (size_t n = 0; n < 3 * N0; ++n)
+    if (No_vp[n] < 1 || No_vp[n] > N_VERT) {
+      snprintf(c->err, sizeof c->err, "No_vp(%zu,%zu) = %d is not a vertex", n % N0 + 1, n / N0 + 1, No_vp[n]);
+      c->status = -1;
+      return c;
+    }
+  memcpy(c->No_vp, No_vp, 3 * N0 * sizeof(int));
+  for (int j = 0; j < 3; ++j) memcpy(c->No_cp + (size_t)j * NC, No_cp + (size_t)j * N0, N0 * sizeof(int));
+  memcpy(c->nbe, nbe, N0 * sizeof(int));
+  memcpy(c->nbev, nbev, (size_t)N_VERT * sizeof(int));
+  memcpy(c->xv, xv, (size_t)N_VERT * sizeof(double));
+  memcpy(c->yv, yv, (size_t)N_VERT * sizeof(double));
+  if (zbv) memcpy(c->zbv, zbv, (size_t)N_VERT * sizeof(double));
+  c->bed = 0;
+  return finish_case(c);
+}
+
+/* ----------------------------------------------------- the mesh file --- */
+/* input.F90:147-194.  Each READ starts a record and takes its items field by
+ * field from fixed columns, n fields to a record (3I8, 2F15.5, 10F8.3,
+ * 20I7), the format starting a new record when it runs out.  A record
+ * shorter than its fields is padded with blanks; blanks in a field are
+ * ignored and an all-blank field reads as zero. */
+typedef struct {
+  FILE *f;
+  char *buf;
+  size_t cap;
+  long recno;
+} mesh_reader;
+
+static int next_record(mesh_reader *r) {
+  size_t n = 0;
+  int ch;
+  while ((ch = fgetc(r->f)) != EOF && ch != '\n') {
+    if (n + 2 > r->cap) {
+      r->cap = r->cap ? 2 * r->cap : 256;
+      r->buf = realloc(r->buf, r->cap);
+      if (!r->buf) return -1;
+    }
+    r->buf[n++] = (char)ch;
+  }
+  if (ch == EOF && n == 0) return -1;
+  if (!r->buf) {
+    r->cap = 256;
+    r->buf = malloc(r->cap);
+    if (!r->buf) return -1;
+  }
+  r->buf[n] = 0;
+  ++r->recno;
+  return 0;
+}
+
+/* the non-blank characters of columns [col, col+w) of the record */
+static void field_chars(const char *rec, int col, int w, char *out) {
+  const int len = (int)strlen(rec);
+  int m = 0;
+  for (int q = col; q < col + w; ++q)
+    if (q < len && rec[q] != ' ') out[m++] = rec[q];
+  out[m] = 0;
+}
+
+static int parse_iw(const char *t, long *v) {
+  if (!*t) {
+    *v = 0;
+    return 0;
+  }
+  const char *p = t;
+  int neg = 0;
+  if (*p == '+' || *p == '-') neg = *p++ == '-';
+  if (!*p) return -1;
+  long x = 0;
+  for (; *p; ++p) {
+    if (*p < '0' || *p > '9' || x > 100000000000L) return -1;
+    x = 10 * x + (*p - '0');
+  }
+  *v = neg ? -x : x;
+  return 0;
+}
+
+/* Fw.d input: [sign] digits [. digits] [exponent], the exponent as E, D, e, d
+ * with an optional sign, or a bare sign; without a decimal point the digits
+ * are scaled by 10^-d.  The value is the correctly rounded double. */
+static int parse_fwd(const char *t, int d, double *v) {
+  if (!*t) {
+    *v = 0.0;
+    return 0;
+  }
+  char mant[128];
+  int m = 0, dot = 0, ndig = 0;
+  const char *p = t;
+  if (*p == '+' || *p == '-') mant[m++] = *p++;
+  for (; *p && m < 100; ++p) {
+    if (*p >= '0' && *p <= '9') {
+      mant[m++] = *p;
+      ++ndig;
+    } else if (*p == '.' && !dot) {
+      mant[m++] = '.';
+      dot = 1;
+    } else
+      break;
+  }
+  if (ndig == 0) return -1;
+  long ex = 0;
+  if (*p) {
+    if (*p == 'E' || *p == 'e' || *p == 'D' || *p == 'd') ++p;
+    else if (*p != '+' && *p != '-') return -1;
+    if (parse_iw(p, &ex) || !*p) return -1;
+  }
+  if (!dot) ex -= d;
+  mant[m] = 0;
+  char full[192];
+  snprintf(full, sizeof full, "%se%ld", mant, ex);
+  char *end = NULL;
+  *v = strtod(full, &end);
+  return (end && *end == 0) ? 0 : -1;
+}
+
+/* n items of format (per_rec(Iw)) or (per_rec(Fw.d)) into iv/dv[0..n) (or,
+ * with stride 2, pairs xv(i), yv(i) from 2F15.5 into two arrays) */
+static int read_ints(mesh_reader *r, int n, int per_rec, int w, int *iv, int stride_rows, int ncols) {
+  /* items in the order of the implied DO: for i, for j (row-major over an
+   * (stride_rows, ncols) column-major array when ncols > 1) */
+  char fld[64];
+  int k = 0;
+  for (int item = 0; item < n; ++item) {
+    if (k == 0 && next_record(r)) return -1;
+    field_chars(r->buf, k * w, w, fld);
+    long v;
+    if (parse_iw(fld, &v) || v < -2147483647L || v > 2147483647L) return -1;
+    const int i = ncols > 1 ? item / ncols : item, j = ncols > 1 ? item % ncols : 0;
+    iv[(size_t)i + (size_t)j * stride_rows] = (int)v;
+    k = (k + 1) % per_rec;
+  }
+  return 0;
+}
+
+static int read_reals(mesh_reader *r, int n, int per_rec, int w, int d, double *a, double *b) {
+  char fld[64];
+  int k = 0;
+  for (int item = 0; item < n; ++item) {
+    if (k == 0 && next_record(r)) return -1;
+    field_chars(r->buf, k * w, w, fld);
+    double v;
+    if (parse_fwd(fld, d, &v)) return -1;
+    if (b) {
+      if (item % 2 == 0) a[item / 2] = v;
+      else b[item / 2] = v;
+    } else
+      a[item] = v;
+    k = (k + 1) % per_rec;
+  }
+  return 0;
+}
+
+int nsx_read_mesh(const char *path, int N_CELL0, int N_VERT, int *No_vp, double *xv, double *yv, int *No_cp,
+                  double *zbv, int *nbe, int *nbev) {
+  if (!path || N_CELL0 < 1 || N_VERT < 1) return -1;
+  mesh_reader r = {fopen(path, "r"), NULL, 0, 0};
+  if (!r.f) return -1;
+  int rc = 0;
+  for (int t = 0; t < 4 && !rc; ++t) rc = next_record(&r); /* three titles and the first section's */
+  if (!rc) rc = read_ints(&r, 3 * N_CELL0, 3, 8, No_vp, N_CELL0, 3);
+  if (!rc) rc = next_record(&r);
+  if (!rc) rc = read_reals(&r, 2 * N_VERT, 2, 15, 5, xv, yv);
+  if (!rc) rc = next_record(&r);
+  if (!rc) rc = read_ints(&r, 3 * N_CELL0, 3, 8, No_cp, N_CELL0, 3);
+  if (!rc) rc = next_record(&r);
+  if (!rc) rc = read_reals(&r, N_VERT, 10, 8, 3, zbv, NULL);
+  if (!rc) rc = next_record(&r);
+  if (!rc) rc = read_ints(&r, N_CELL0, 20, 7, nbe, N_CELL0, 1);
+  if (!rc) rc = next_record(&r);
+  if (!rc) rc = read_ints(&r, N_VERT, 20, 7, nbev, N_VERT, 1);
+  const long at = r.recno;
+  fclose(r.f);
+  free(r.buf);
+  return rc ? (int)-(at + 1) : 0;
+}
+
+/* Iw and Fw.d output, asterisks when the value does not fit */
+static void out_iw(FILE *f, long v, int w) {
+  char b[32];
+  const int n = snprintf(b, sizeof b, "%ld", v);
+  if (n > w)
+    for (int q = 0; q < w; ++q) fputc('*', f);
+  else
+    fprintf(f, "%*s", w, b);
+}
+
+static void out_fwd(FILE *f, double v, int w, int d) {
+  char b[64];
+  const int n = snprintf(b, sizeof b, "%.*f", d, v);
+  if (n > w)
+    for (int q = 0; q < w; ++q) fputc('*', f);
+  else
+    fprintf(f, "%*s", w, b);
+}
+
+int nsx_write_mesh(const char *path, int N_CELL0, int N_VERT, const int *No_vp, const double *xv, const double *yv,
+                   const int *No_cp, const double *zbv, const int *nbe, const int *nbev) {
+  if (!path || N_CELL0 < 1 || N_VERT < 1 || !No_vp || !xv || !yv || !No_cp || !nbe || !nbev) return -1;
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, " ---------------------------------------------\n");
+  fprintf(f, "  prismatic channel mesh: %d cells, %d vertices\n", N_CELL0, N_VERT);
+  fprintf(f, " ---------------------------------------------\n");
+  fprintf(f, " --------NUMBERING OF CELL VERTICES----\n");
+  for (int i = 0; i < N_CELL0; ++i) {
+    for (int j = 0; j < 3; ++j) out_iw(f, No_vp[(size_t)i + (size_t)j * N_CELL0], 8);
+    fputc('\n', f);
+  }
+  fprintf(f, " --------COORDINATES OF CELL VERTICES----\n");
+  for (int v = 0; v < N_VERT; ++v) {
+    out_fwd(f, xv[v], 15, 5);
+    out_fwd(f, yv[v], 15, 5);
+    fputc('\n', f);
+  }
+  fprintf(f, " --------NUMBERING OF CELL CENTERS----\n");
+  for (int i = 0; i < N_CELL0; ++i) {
+    for (int j = 0; j < 3; ++j) out_iw(f, No_cp[(size_t)i + (size_t)j * N_CELL0], 8);
+    fputc('\n', f);
+  }
+  fprintf(f, " --------BOTTOM LEVELS ---------------\n");
+  for (int v = 0; v < N_VERT; ++v) {
+    out_fwd(f, zbv ? zbv[v] : 0.0, 8, 3);
+    if (v % 10 == 9 || v == N_VERT - 1) fputc('\n', f);
+  }
+  fprintf(f, " --------TYPE OF BOUNDARY CELLS ---------------\n");
+  for (int i = 0; i < N_CELL0; ++i) {
+    out_iw(f, nbe[i], 7);
+    if (i % 20 == 19 || i == N_CELL0 - 1) fputc('\n', f);
+  }
+  fprintf(f, " --------TYPE OF BOUNDARY VERTEX-----------\n");
+  for (int v = 0; v < N_VERT; ++v) {
+    out_iw(f, nbev[v], 7);
+    if (v % 20 == 19 || v == N_VERT - 1) fputc('\n', f);
+  }
+  return fclose(f) == 0 ? 0 : -1;
 }
 
 int nsx_case_status(const nsx_case *c, char *msg, int len) {

@@ -1,6 +1,7 @@
 /*
- * nsx_case.h -- host-side set-up of a channel case (mesh, geometry, flags,
- * PPE matrix, initial field) in the reference's own array layout.
+ * nsx_case.h -- the case structure behind the set-up API of nsx_setup.h:
+ * mesh, geometry, flags, PPE matrix and initial field of a channel case in
+ * the reference's own array layout.
  *
  * Arrays are column-major like the Fortran modules they mirror, with the
  * reference's 1-based connectivity; names match the Fortran variables
@@ -9,6 +10,8 @@
 #ifndef NSX_CASE_H
 #define NSX_CASE_H
 #include <stdint.h>
+
+#include "nsx_setup.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -38,6 +41,7 @@ typedef struct nsx_case {
   int nx, ny, NN, NZ, N_CELL0, N_VERT, N_CELL, N_CELLexact, N_CELLghost;
   int bctop, bcbot, bed, status;
   double Lx, Ly, H, stretch, jitter, Ubulk, noise, TotalVolume;
+  double XDIni, XDFin, YDIni, YDFin; /* the periodic box (common.mpf:66-69) */
   uint64_t seed;
   char err[256];
   int n_arrays;
@@ -56,18 +60,6 @@ typedef struct nsx_case {
   double *walld;
   double *u0, *v0, *w0, *p0;
 } nsx_case;
-
-double nsx_rng_u01(uint64_t seed, uint32_t stream, uint64_t index);
-
-nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
-                          double jitter, int bed, double Ubulk, double noise, uint64_t seed);
-void nsx_case_free(nsx_case *c);
-int nsx_case_status(const nsx_case *c, char *msg, int len);
-int nsx_case_info(const nsx_case *c, int *out);
-double nsx_case_total_volume(const nsx_case *c);
-int nsx_case_num_arrays(const nsx_case *c);
-const char *nsx_case_array_name(const nsx_case *c, int n);
-void *nsx_case_get(const nsx_case *c, const char *name, int64_t *n, int *type);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,10 @@ def host() -> ctypes.CDLL:
         lib.nsx_case_array_name.restype = c_cp
         lib.nsx_case_get.argtypes = [c_vp, c_cp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(c_int)]
         lib.nsx_case_get.restype = c_vp
+        lib.nsx_case_from_mesh.restype = c_vp
+        lib.nsx_case_from_mesh.argtypes = [c_int, c_int, c_int] + [c_vp] * 7 + [c_dbl] * 8 + [ctypes.c_uint64]
+        lib.nsx_read_mesh.argtypes = [c_cp, c_int, c_int] + [c_vp] * 7
+        lib.nsx_write_mesh.argtypes = [c_cp, c_int, c_int] + [c_vp] * 7
         lib.nsx_rng_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
         lib.nsx_rng_u01.restype = c_dbl
         _host = lib

@@ -1,9 +1,10 @@
 """Channel cases: seeded mesh, geometry, flags, PPE matrix and initial field.
 
 Wraps the host set-up library (host/nsx_case.c), which restates the set-up
-half of the reference (src/Main/calcul_geometry.F90, src/BoundaryConditions/
-update_bc.F90, src/AdvectionDiffusion/PDE.F90) for the synthetic doubly
-periodic channel.  Every array keeps the reference's Fortran layout
+half of the reference (src/Main/input.F90's mesh file, src/Main/
+calcul_geometry.F90, src/BoundaryConditions/update_bc.F90,
+src/AdvectionDiffusion/PDE.F90) for a doubly periodic channel: the synthetic
+seeded meshes, or a mesh given as arrays or as the reference's data.txt.  Every array keeps the reference's Fortran layout
 (column-major, 1-based connectivity, ghost cells after N_CELL0) and name.
 """
 from __future__ import annotations
@@ -105,6 +106,55 @@ def make_case(nx: int, ny: int, nn: int, *, seed: int = 1234, jitter: float = 0.
     lib = _lib.host()
     ptr = lib.nsx_case_create(nx, ny, nn, Lx, Ly, Hh, stretch, jitter, int(bool(bed)), Ubulk, noise,
                               ctypes.c_uint64(seed))
+    return _collect(lib, ptr, nx, ny, nn, params or Params(sigFin=Hh))
+
+
+MESH_KEYS = ("No_vp", "xv", "yv", "No_cp", "zbv", "nbe", "nbev")
+
+
+def read_mesh(path: str, N_CELL0: int, N_VERT: int) -> dict:
+    """The mesh file of the reference (data.txt, input.F90:147-194) as arrays
+    No_vp, No_cp (N_CELL0, 3), xv, yv, zbv, nbev (N_VERT), nbe (N_CELL0)."""
+    m = _mesh_arrays(N_CELL0, N_VERT)
+    rc = _lib.host().nsx_read_mesh(path.encode(), N_CELL0, N_VERT, *[m[k].ctypes.data for k in MESH_KEYS])
+    if rc == -1:
+        raise OSError(f"cannot open {path}")
+    if rc < 0:
+        raise ValueError(f"{path}: record {-rc} cannot be read as the mesh file")
+    return m
+
+
+def write_mesh(path: str, mesh: dict) -> None:
+    """Write mesh arrays (or a Case) in the reference's data.txt format."""
+    get = mesh.arrays.get if isinstance(mesh, Case) else mesh.get
+    n0, nv = get("nbe").shape[0], get("nbev").shape[0]
+    a = {k: np.asfortranarray(get(k)[:n0] if k == "No_cp" else get(k)) for k in MESH_KEYS}
+    if _lib.host().nsx_write_mesh(path.encode(), n0, nv, *[a[k].ctypes.data for k in MESH_KEYS]) != 0:
+        raise OSError(f"cannot write {path}")
+
+
+def _mesh_arrays(n0: int, nv: int) -> dict:
+    return {"No_vp": np.zeros((n0, 3), np.int32, order="F"), "xv": np.zeros(nv), "yv": np.zeros(nv),
+            "No_cp": np.zeros((n0, 3), np.int32, order="F"), "zbv": np.zeros(nv),
+            "nbe": np.zeros(n0, np.int32), "nbev": np.zeros(nv, np.int32)}
+
+
+def case_from_mesh(mesh: dict, nn: int, *, box=(0.0, LX, 0.0, LY), Hh: float = H, stretch: float = 2.0,
+                   Ubulk: float = 15.6, noise: float = 0.05, seed: int = 1234, params: Params | None = None) -> Case:
+    """calcul_geometry + adjust_bc + PDE_center (and the seeded initial field)
+    on a given mesh (read_mesh's arrays) in the doubly periodic box
+    (XDIni, XDFin, YDIni, YDFin).  Raises RuntimeError with the set-up's
+    message."""
+    n0, nv = mesh["nbe"].shape[0], mesh["nbev"].shape[0]
+    a = {k: np.asfortranarray(mesh[k], dtype=np.int32 if k in ("No_vp", "No_cp", "nbe", "nbev") else np.float64)
+         for k in MESH_KEYS}
+    lib = _lib.host()
+    ptr = lib.nsx_case_from_mesh(n0, nv, nn, *[a[k].ctypes.data for k in MESH_KEYS], *[float(b) for b in box], Hh,
+                                 stretch, Ubulk, noise, ctypes.c_uint64(seed))
+    return _collect(lib, ptr, 0, 0, nn, params or Params(sigFin=Hh))
+
+
+def _collect(lib, ptr, nx, ny, nn, params) -> Case:
     try:
         msg = ctypes.create_string_buffer(256)
         if lib.nsx_case_status(ptr, msg, 256) != 0:
@@ -113,7 +163,7 @@ def make_case(nx: int, ny: int, nn: int, *, seed: int = 1234, jitter: float = 0.
         lib.nsx_case_info(ptr, info)
         c = Case(nx=nx, ny=ny, nn=nn, N_CELL0=info[0], N_CELL=info[1], N_VERT=info[2], NZ=info[3],
                  N_CELLexact=info[4], N_CELLghost=info[5], bctop=info[6], bcbot=info[7],
-                 TotalVolume=lib.nsx_case_total_volume(ptr), params=params or Params(sigFin=Hh))
+                 TotalVolume=lib.nsx_case_total_volume(ptr), params=params)
         for a in range(lib.nsx_case_num_arrays(ptr)):
             name = lib.nsx_case_array_name(ptr, a).decode()
             n = ctypes.c_int64()

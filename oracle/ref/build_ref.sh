@@ -23,7 +23,12 @@ TOPBC=${5:-0}
 BOTBC=${6:-0}
 REF=${NSX_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
-OUT=$HERE/../_ref/nn$NN
+# a box other than the default gets a directory of its own
+if [ "$LX" = "6.283185307179586" ] && [ "$LY" = "3.141592653589793" ]; then
+  OUT=$HERE/../_ref/nn$NN
+else
+  OUT=$HERE/../_ref/nn${NN}_box${LX}x${LY}
+fi
 FC=${FC:-/opt/rocm/bin/amdflang}
 FFLAGS=${FFLAGS:--O2}
 

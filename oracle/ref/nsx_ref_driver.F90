@@ -10,6 +10,8 @@
 !  the build container.
 !
 !  usage: nsx_ref_driver <case.bin> <out.bin>
+!  mode 3 dumps the set-up geometry alone (with hdr(11) = 1 the mesh comes
+!  from data.txt through the reference's input.F90);
 !  mode 1 dumps set-up geometry and single-routine results on the initial
 !  field; mode 2 runs nsteps RK steps, dumps the state after each step and
 !  calls glob_cfl on the final state (its printed diagnostics and meanuv.dat
@@ -53,7 +55,7 @@
 !     explicit interfaces of the routines with assumed-shape dummies, as the
 !     reference PROGRAM has them (nsmp3D.F90:55); grandientLSM's entry there
 !     does not match the routine and is left out
-      USE interfaces, only: calcul_geometry, initial, LoopTimeInitial, LoopTimeUpdate, &
+      USE interfaces, only: input, calcul_geometry, initial, LoopTimeInitial, LoopTimeUpdate, &
                             SavetecVertex, SavetecCenter, SaveParaviewVertex, SavetecVC
       implicit none
       character(len=512) :: casefile, outfile
@@ -66,6 +68,7 @@
       real*8, allocatable :: f1(:,:), f2(:,:), f3(:,:), f4(:,:), f5(:,:)
       real*8 :: tv(1)
       integer :: iv(4)
+      integer(8) :: clk0, clk1, clkrate
 
       call get_command_argument(1, casefile)
       call get_command_argument(2, outfile)
@@ -120,6 +123,10 @@
       read(ucase) (xv(n1), n1=1,N_VERT)
       read(ucase) (yv(n1), n1=1,N_VERT)
       read(ucase) (zbv(n1), n1=1,N_VERT)
+!     hdr(11) = 1: the mesh from data.txt in the working directory instead,
+!     read by the reference's own input.F90
+      if (hdr(11) .eq. 1) call input(No_vp,No_cp,nbe,nbev,xv,yv,zbv)
+      call system_clock(clk0, clkrate)
       do n1=1,N_VERT
          hv(n1) = 0.0d0
       enddo
@@ -140,6 +147,8 @@
                      xv,yv,sigv,dsigv,No_vp,nbev)
       call PDE_center(xc,yc,sig,dsig,No_cp,nbe,    &
                       xv,yv,sigv,dsigv,No_vp,nbev)
+      call system_clock(clk1)
+      write(*,'(a,f12.4)') ' NSX_SETUP_SECONDS', dble(clk1-clk0)/dble(clkrate)
 
       time  = 0.0d0
       nstep = 0
@@ -213,6 +222,7 @@
 
       if (mode .eq. 1) call unit_mode
       if (mode .eq. 2) call step_mode
+      if (mode .eq. 3) call setup_mode
       close(udump)
 
       contains
@@ -324,6 +334,17 @@
 
 !     single routines on the initial field; inputs are copied first so that
 !     each call sees the same state
+!     mode 3: the set-up alone, with the mesh as read
+      subroutine setup_mode
+      call dump_geometry
+      call dump_i4('mesh_No_vp', No_vp, 3*N_CELL0)
+      call dump_i4('mesh_nbe', nbe, N_CELL0)
+      call dump_i4('mesh_nbev', nbev, N_VERT)
+      call dump_r8('mesh_xv', xv, N_VERT)
+      call dump_r8('mesh_yv', yv, N_VERT)
+      call dump_r8('mesh_zbv', zbv, N_VERT)
+      end subroutine
+
       subroutine unit_mode
       integer :: m1, m2
       call dump_geometry

@@ -19,25 +19,29 @@ REF_OUT = os.path.join(os.path.dirname(HERE), "_ref")
 MAGIC = 1314084913
 
 
-def driver_path(nn: int) -> str:
-    return os.path.join(REF_OUT, f"nn{nn}", "nsx_ref_driver")
+def driver_path(nn: int, box: tuple | None = None) -> str:
+    sub = f"nn{nn}" if box is None else f"nn{nn}_box{box[0]}x{box[1]}"
+    return os.path.join(REF_OUT, sub, "nsx_ref_driver")
 
 
-def ensure_built(nn: int) -> str:
-    exe = driver_path(nn)
+def ensure_built(nn: int, box: tuple | None = None) -> str:
+    """The reference driver for nn layers; box = (Lx, Ly) as the strings the
+    build writes into common.mpf (default 2*pi x pi)."""
+    exe = driver_path(nn, box)
     src = os.path.join(HERE, "nsx_ref_driver.F90")
     if not os.path.exists(exe) or os.path.getmtime(src) > os.path.getmtime(exe):
-        subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(nn)], check=True,
+        extra = [] if box is None else [str(box[0]), str(box[1])]
+        subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(nn), *extra], check=True,
                        stdout=subprocess.DEVNULL)
     return exe
 
 
 def write_case(path: str, case, *, mode: int, nsteps: int = 1, dump_every: int = 1,
-               p0=None) -> None:
+               p0=None, mesh_from_file: bool = False) -> None:
     p = case.params
     hdr = np.zeros(16, dtype=np.int32)
-    hdr[:10] = [MAGIC, case.N_CELL0, case.N_VERT, case.nn, nsteps, p.FinRK, p.MaxIters, p.Iturbu,
-                mode, dump_every]
+    hdr[:11] = [MAGIC, case.N_CELL0, case.N_VERT, case.nn, nsteps, p.FinRK, p.MaxIters, p.Iturbu,
+                mode, dump_every, 1 if mesh_from_file else 0]
     rhdr = np.zeros(16, dtype=np.float64)
     rhdr[:6] = [p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps]
     n0 = case.N_CELL0
@@ -103,9 +107,12 @@ def parse_solves(log: str) -> list:
     return solves
 
 
-def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workdir: str | None = None):
-    """Run the reference; returns (dump dict, stdout text)."""
-    exe = ensure_built(case.nn)
+def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workdir: str | None = None,
+        mesh_file: str | None = None, box: tuple | None = None):
+    """Run the reference; returns (dump dict, stdout text).  mesh_file: a
+    data.txt the reference's input.F90 reads instead of the case's mesh;
+    box: (Lx, Ly) of a build other than the default 2*pi x pi."""
+    exe = ensure_built(case.nn, box)
     with tempfile.TemporaryDirectory(dir=workdir) as td:
         # the reference writes its statistics to ../output/Stats of its working directory
         wd = os.path.join(td, "run")
@@ -116,7 +123,11 @@ def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workd
         os.makedirs(serial_dir)
         cf = os.path.join(td, "case.bin")
         of = os.path.join(td, "out.bin")
-        write_case(cf, case, mode=mode, nsteps=nsteps, dump_every=dump_every, p0=p0)
+        if mesh_file:
+            import shutil
+            shutil.copy(mesh_file, os.path.join(wd, "data.txt"))
+        write_case(cf, case, mode=mode, nsteps=nsteps, dump_every=dump_every, p0=p0,
+                   mesh_from_file=bool(mesh_file))
         r = subprocess.run([exe, cf, of], cwd=wd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"reference driver failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")

@@ -17,7 +17,10 @@ what they produce:
                   header lines of the snapshots SavetecVertex, SavetecCenter,
                   SavetecVC (Tecplot) and SaveParaviewVertex (VTK) wrote of
                   the final state;
-  cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps.
+  cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps;
+  mesh_setup.json the set-up of a mesh read from the reference's mesh file
+                  (data.txt through its own input.F90): SHA-256 of the file
+                  and of every set-up array.
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
 adding +0.0 (so -0.0 and +0.0 hash alike).  Run in the build container:
@@ -121,5 +124,66 @@ def main():
         print(f"cfg{cfg}_steps.json:", len(rec["solves"]), "solves")
 
 
+# the mesh-file set-up: a jittered 32x16 quads x 32 layers mesh in the box
+# 6.4 x 3.2 (its boundary vertices on multiples of 0.2, exact in data.txt's
+# F15.5 fields), written as data.txt and set up by the reference with its own
+# input.F90 reading the file (a reference build for that box)
+MESH_CASE = dict(nx=32, ny=16, nn=32, seed=5, jitter=0.1, Lx=6.4, Ly=3.2)
+
+
+def setup_views(dump_or_arrays, N_CELL, N_CELLexact, Dimsurrounding):
+    """the canonical part of each set-up array: rows 1..N_CELLexact of cell
+    arrays, the interior layers of the PPE matrix, the used entries of the
+    inverse-distance stencil"""
+    out = {}
+    for k, a in dump_or_arrays.items():
+        a = np.asarray(a)
+        if k.startswith("AmC"):
+            v = a[:, 1:-1]
+        elif k in ("surrounding", "weight"):
+            v = a.copy()
+            for j in range(v.shape[1]):
+                v[Dimsurrounding <= j, j] = 0
+        elif a.ndim >= 1 and a.shape[0] == N_CELL:
+            v = a[:N_CELLexact]
+        else:
+            v = a
+        out[k] = v
+    return out
+
+
+def sha(a) -> str:
+    a = np.ascontiguousarray(a)
+    if a.dtype.kind == "f":
+        a = a + 0.0
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def mesh_setup():
+    import tempfile
+
+    import refrun
+    from nsx.case import make_case, write_mesh
+
+    m = dict(MESH_CASE)
+    c = make_case(m["nx"], m["ny"], m["nn"], seed=m["seed"], jitter=m["jitter"], Lx=m["Lx"], Ly=m["Ly"])
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "data.txt")
+        write_mesh(p, c)
+        text = open(p, "rb").read()
+        d, log = refrun.run(c, mode=3, mesh_file=p, box=(str(m["Lx"]), str(m["Ly"])))
+    skip = {"sizes", "TotalVolume", "serial_files", "stats_files"}
+    geo = {k: v for k, v in d.items() if k not in skip and not k.startswith("mesh_")}
+    views = setup_views(geo, c.N_CELL, int(d["sizes"][0]), np.asarray(d["Dimsurrounding"]))
+    rec = {"case": m, "data_txt_sha256": hashlib.sha256(text).hexdigest(),
+           "sizes": [int(x) for x in d["sizes"]], "TotalVolume": float(d["TotalVolume"][0]),
+           "mesh_read": {k[5:]: sha(v) for k, v in d.items() if k.startswith("mesh_")},
+           "arrays": {k: sha(v) for k, v in views.items()}}
+    with open(os.path.join(HERE, "mesh_setup.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+    print("mesh_setup.json:", len(rec["arrays"]), "arrays")
+
+
 if __name__ == "__main__":
     main()
+    mesh_setup()

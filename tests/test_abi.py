@@ -1,4 +1,5 @@
-"""The C-ABI library loads on the CPU and exports every entry point of include/nsx.h."""
+"""The C-ABI libraries load on the CPU and export every entry point of
+include/nsx.h (libnsx.so) and include/nsx_setup.h (libnsx_host.so)."""
 import ctypes
 import os
 import re
@@ -40,3 +41,13 @@ def test_fortran_interfaces_cover_the_header():
     f90 = open(os.path.join(REPO, "3d-channel-flow-code_amd", "fortran", "nsx_iface.F90")).read()
     bound = set(re.findall(r'bind\(C,\s*name="(nsx_\w+)"\)', f90))
     assert set(declared()) == bound, sorted(set(declared()) ^ bound)
+
+
+def test_setup_library_exports_every_symbol():
+    txt = open(os.path.join(REPO, "include", "nsx_setup.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|void|double|const char \*|nsx_case \*|void \*)\s*\*?(nsx_\w+)\(",
+                                  txt, re.M)))
+    assert len(names) == 12, names
+    lib = ctypes.CDLL(os.path.join(REPO, "3d-channel-flow-code_amd", "nsx", "_build", "libnsx_host.so"))
+    for name in names:
+        assert hasattr(lib, name), name
