@@ -1076,15 +1076,18 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // A u_{i+1} is formed in the same pass as the updates: u_{i+1} = rt -
 // alpha_i (om + beta_i sg) at the seven points, from the stored scaled
 // vectors rt = u = r/A0, om = w/A0 and sg = s/A0.  One launch per iteration
-// (mode N) reads rt, om, sg, p, e and writes rt, om, sg (into the other of
-// two buffer sets, since it reads them at neighbouring prisms), p and e (in
-// place): 40 B per prism, as the two kernels of NSX_REFINE_CG, in one pass
-// with one reduction.  In exact arithmetic the iterates are CG's.
+// (mode N) reads rt, om, sg, p and writes rt, om, sg (into the other of two
+// buffer sets, since it reads them at neighbouring prisms) and p (in place);
+// every other iteration it also reads and writes e (in place), adding the
+// alpha p left pending by the iteration before: 32 or 40 B per prism, against
+// 40 B in the two kernels of NSX_REFINE_CG, in one pass with one reduction.
+// In exact arithmetic the iterates are CG's.
 //
 // Reliable updates as in NSX_REFINE_CG: when an iteration's residual |r|
 // has fallen below delta x its largest value since the last true residual,
 // or meets the stopping rule, the next launch (mode T) forms the true FP64
-// residual of x + e into rt; the launch after it (mode M) folds e into x,
+// residual of x + e (with any pending alpha p) into rt; the launch after it
+// (mode M) folds e (and the pending alpha p) into x,
 // applies the stopping rule to the true residual, forms om = A rt / A0 with
 // gamma and delta, and sg = A p / A0 afresh (the recurrence for s = A p
 // drifts in FP32, and with it the recursive residual; without the reset the
@@ -1095,20 +1098,34 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // Launch j reads the carried scalars from SolverState::cgs[(j-1)&1] and
 // block 0 writes cgs[j&1]; the partial sums alternate between two buffers
 // the same way, so no block reads a value that its own launch rewrites.
+// The launches queued after the stop write their slot too (the stop passed
+// on), or the one after would read the slot of a launch before the stop and
+// run again from it.
 // Partial rows: M gamma, delta; N gamma, delta, |r|^2, |r|_1; T |r|^2,
 // |r|_1, sum (x+e) Vol.
+
+// e with the pending alpha p.  The true-residual launch forms it at every
+// slot and the fold after it forms it again: one fused multiply-add in both,
+// so x takes exactly the values whose residual was tested (an unfused sum in
+// one of them moves x by an FP32 rounding of e, which A can amplify past the
+// stopping rule).
+__device__ inline float e_pending(float e, float p, float a) { return __builtin_fmaf(a, p, e); }
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, const double *rhs, double *x,
                                                     const double *Pin, uint32_t Gin, double *Pout, SolverState *st) {
   CgsCarry c;
   if (j == 0) {
-    if (st->done) return;  // k_cg_start's no-solve test
     c.gamma = c.alpha = c.maxr = 0.0;
     c.mode = -1;
-    c.iter = c.set = c.done = 0;
+    c.iter = c.set = c.pend = c.pad = 0;
+    c.done = st->done;  // k_cg_start's no-solve test (launch 0 itself never stops)
   } else {
     c = st->cgs[(j - 1) & 1];
-    if (c.done) return;
+  }
+  if (c.done) {
+    // pass the stop on: the launch after this one reads this launch's slot
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->cgs[j & 1] = c;
+    return;
   }
   const int prev = c.mode;
   int mode = 0, stop = 0, code = EXIT_CONVERGED;
@@ -1175,12 +1192,17 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     }
   }
   const int in = c.set;
+  // e and the pending alpha p as they stand when this launch starts
+  const bool pend_in = c.pend != 0;
+  const float ap_in = (float)c.alpha;
   if (mode == 1 && !stop) {
     c.gamma = gam;
     c.alpha = al;
     c.iter += 1;
     c.set ^= 1;
+    c.pend = pend_in ? 0 : 1;  // iterations alternate: leave e alone, then add both terms
   }
+  if (fold) c.pend = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     CgsCarry o = c;
     o.mode = mode;
@@ -1194,10 +1216,17 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     }
   }
   if (fold) {
-    // x += e, e = 0: the correction since the last update joins x
+    // x += e (+ the pending alpha p), e = 0: the correction since the last
+    // update joins x
     for_each_quad(g, [&](int, int, size_t s0) {
       float ee[4];
       ld4(v.e + s0, ee);
+      if (pend_in) {
+        float pp[4];
+        ld4(v.p + s0, pp);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ee[u] = e_pending(ee[u], pp[u], ap_in);
+      }
 #pragma unroll
       for (int u = 0; u < 4; ++u) x[s0 + u] = x[s0 + u] + (double)ee[u];
       const float z[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1210,14 +1239,17 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     // the true residual of x + e into rt (T reads x and e at neighbouring
     // prisms and writes rt only at its own)
     const double avec = st->avec;
-    const float *e = v.e;
+    const float *e = v.e, *p = v.p;
     float *rt = v.rt[in];
     double acc[3] = {0.0, 0.0, 0.0};
+    // e at a slot, with the alpha p the last iteration left pending (the fold
+    // of the next launch forms the same sum)
+    auto ecur = [&](size_t s, bool) { return pend_in ? e_pending(e[s], p[s], ap_in) : e[s]; };
     FOR_EACH_CELL(g, q) {
       int i, k;
       cell_of(g, q, i, k);
       double res, xs;
-      true_residual(g, rhs, x, avec, i, k, [&](size_t s, bool) { return e[s]; }, res, xs);
+      true_residual(g, rhs, x, avec, i, k, ecur, res, xs);
       const Row A = ppe_row(g, i, k);
       rt[(size_t)i * g.NS + k] = (float)(res / A.A0);
       acc[0] += res * res;
@@ -1297,11 +1329,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     const float sg = first ? om_i[q] : om_i[q] + bf * sg_i[q];
     return rt_i[q] - af * sg;
   };
+  // e is read and written every other iteration: an iteration that finds no
+  // term pending leaves e alone (its alpha p becomes pending); the next adds
+  // both, alpha_{i-1} p_{i-1} + alpha_i p_i, from the p_{i-1} it reads anyway
+  const bool upd_e = pend_in;
   float ri[4], si[4], pi[4], ei[4];
   auto own4 = [&](size_t off, float(&o)[4]) { form4(off, ri, si, o); };
   auto pre = [&](size_t off) {
     if (!first) ld4(v.p + off, pi);
-    ld4(v.e + off, ei);
+    if (upd_e) ld4(v.e + off, ei);
   };
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for_each_quad_formed<float>(
@@ -1315,7 +1351,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           const Row1<float> R = row_of(g, lv, C, qv, k0, u);
           const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
           const float pn = first ? ri[u] : ri[u] + bf * pi[u];
-          eo[u] = R.in ? ei[u] + af * pn : ei[u];
+          if (upd_e) eo[u] = R.in ? (ei[u] + ap_in * pi[u]) + af * pn : ei[u];
           po[u] = R.in ? pn : 0.f;
           ro[u] = R.in ? fc[u] : 0.f;
           so[u] = R.in ? si[u] : 0.f;
@@ -1328,7 +1364,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             acc[3] += fabs(rd);
           }
         }
-        st4(v.e + s0, eo);
+        if (upd_e) st4(v.e + s0, eo);
         st4(v.p + s0, po);
         st4(rt_o + s0, ro);
         st4(om_o + s0, wo);

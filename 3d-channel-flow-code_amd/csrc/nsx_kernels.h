@@ -26,6 +26,8 @@ struct CgsCarry {
   int iter;             // iterations (launches in mode N) so far
   int set;              // buffer set holding the current rt, om, sg
   int done;
+  int pend;             // e lacks alpha * p of the last iteration (which left e alone)
+  int pad;
 };
 
 struct SolverState {

@@ -5,10 +5,11 @@
     python 3d-channel-flow-code_amd/nsx/profsum.py traffic FETCH_DIR WRITE_DIR N_COPY
         per-kernel HBM traffic of two --pmc runs (FETCH_SIZE, then WRITE_SIZE,
         KiB per dispatch), each scaled by the factor that makes the k_copy
-        dispatch of N_COPY doubles read and write 8 N_COPY bytes; the median
-        over a kernel's dispatches, so that the few that differ in kind (a
-        solver launch queued after the stop returns at once, the rare FP32 CG
-        update launch that forms a true residual) do not shift it
+        dispatch of N_COPY doubles read and write 8 N_COPY bytes; the mean
+        over the kernel's dispatches that did their work (a solver launch
+        queued after the stop returns at once, moving under a tenth of the
+        kernel's largest fetch: those are left out), the same launches the
+        bench's event timing averages over
 
 The traffic summary ends with one machine-readable line per kernel,
 `traffic: <kernel> <fetch bytes> <write bytes> <dispatches>`, which bench.py
@@ -69,14 +70,26 @@ def traffic(fetch_dir: str, write_dir: str, n_copy: int) -> str:
     w_scale = copy_bytes / (sum(wr["k_copy"]) / len(wr["k_copy"]) * 1024.0)
     out = [f"calibration: k_copy of {n_copy} doubles ({copy_bytes / 1e6:.1f} MB each way): "
            f"FETCH_SIZE x {f_scale:.4f}, WRITE_SIZE x {w_scale:.4f}", "",
-           "| kernel | dispatches | fetch MB/dispatch (median) | write MB/dispatch (median) |", "|---|---|---|---|"]
+           "| kernel | dispatches (that worked) | fetch MB/dispatch (mean) | write MB/dispatch (mean) |",
+           "|---|---|---|---|"]
     lines = []
-    med = lambda v: float(np.median(np.asarray(v, dtype=float)))
-    for k in sorted(set(fe) & set(wr), key=lambda k: -med(fe[k])):
-        f = med(fe[k]) * 1024.0 * f_scale
-        w = med(wr[k]) * 1024.0 * w_scale
-        out.append(f"| {k} | {len(fe[k])} | {f / 1e6:.1f} | {w / 1e6:.1f} |")
-        lines.append(f"traffic: {k} {f:.0f} {w:.0f} {len(fe[k])}")
+
+    def worked(fv, wv):
+        f, w = np.asarray(fv, dtype=float), np.asarray(wv, dtype=float)
+        n = min(len(f), len(w))
+        f, w = f[:n], w[:n]
+        keep = f >= 0.1 * f.max() if n else np.zeros(0, bool)
+        return f[keep], w[keep]
+
+    tab = {k: worked(fe[k], wr[k]) for k in set(fe) & set(wr)}
+    for k in sorted(tab, key=lambda k: -float(tab[k][0].mean()) if len(tab[k][0]) else 0.0):
+        fk, wk = tab[k]
+        if not len(fk):
+            continue
+        f = float(fk.mean()) * 1024.0 * f_scale
+        w = float(wk.mean()) * 1024.0 * w_scale
+        out.append(f"| {k} | {len(fe[k])} ({len(fk)}) | {f / 1e6:.1f} | {w / 1e6:.1f} |")
+        lines.append(f"traffic: {k} {f:.0f} {w:.0f} {len(fk)}")
     return "\n".join(out + [""] + lines)
 
 

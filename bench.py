@@ -143,13 +143,14 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
             b = units * 4 * n
             kernels["cg32_" + ph] = {"kernel": "k_cg_" + ph, "launches": launches, "avg_us": avg * 1e3,
                                      "bytes_per_launch": b, "GBps": b / (avg * 1e-3) / 1e9}
-    # the single-reduction CG: one kernel per iteration, 10 units (reads rt,
-    # om, sg, p, e; writes them); its rare true-residual and product launches
-    # are counted with it
+    # the single-reduction CG: one kernel per iteration reading rt, om, sg, p
+    # and writing rt, om, sg, p (8 units), and every other iteration e too
+    # (10 units): 9 units a launch on average; its rare true-residual and
+    # product launches are counted with it
     ms, launches = ctx.timer("cg32_sr")
     if launches:
         avg = ms / launches
-        b = 10 * 4 * n
+        b = 9 * 4 * n
         kernels["cg32_sr"] = {"kernel": "k_cgs", "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                               "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")
