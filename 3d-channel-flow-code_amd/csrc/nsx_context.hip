@@ -1,6 +1,7 @@
 // nsx_context.hip -- the C ABI of include/nsx.h: one device context holding
 // geometry, matrix and every field resident in HBM, the stage/step drivers
 // and the single-routine entry points.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -346,7 +347,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     Gc = Gc / NSX_NXCD * NSX_NXCD;
     if (Gc < NSX_NXCD) Gc = NSX_NXCD;
     g.grid_cg = (int)Gc;
-    const int per_cu_cgs = cgs_blocks_per_cu(g);
+    // the single-reduction CG kernel runs 4 blocks a CU, below its occupancy
+    // (5): at 6.3M prisms 1024 blocks sweep 6 windows each, a band per XCD
+    // that its L2 holds, and took 50.5 us a launch against 52.2 us for 1280
+    // (896: 52.5, 1152: 51.8)
+    const int per_cu_cgs = std::min(cgs_blocks_per_cu(g), 4);
     if (per_cu_cgs < 1) return nsx_fail(ctx, -2, "occupancy query of the single-reduction CG kernel failed");
     long Gs = (long)prop.multiProcessorCount * per_cu_cgs;
     if (Gs > need) Gs = need;
