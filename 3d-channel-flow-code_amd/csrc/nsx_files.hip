@@ -163,8 +163,9 @@ extern "C" int nsx_write_plane_stats(const double *planes, int NZ, int n_cell0, 
   char fn[3][1024];
   const char *stem[3] = {"sm-", "srms-", "stau-"};
   FILE *f[3];
+  for (int q = 0; q < 3; ++q)
+    if (snprintf(fn[q], sizeof fn[q], "%s/%s%05d.dat", dir, stem[q], ns) >= (int)sizeof fn[q]) return -1;
   for (int q = 0; q < 3; ++q) {
-    snprintf(fn[q], sizeof fn[q], "%s/%s%05d.dat", dir, stem[q], ns);
     f[q] = fopen(fn[q], "w");
     if (!f[q]) {
       for (int r = 0; r < q; ++r) fclose(f[r]);
@@ -279,7 +280,7 @@ extern "C" int nsx_write_tec_vertex(const char *dir, int counter, double time, i
       counter > 9999)
     return -1;
   char fn[1024];
-  snprintf(fn, sizeof fn, "%s/V-%04d.tec", dir, counter);
+  if (snprintf(fn, sizeof fn, "%s/V-%04d.tec", dir, counter) >= (int)sizeof fn) return -1;  // path too long
   FILE *f = fopen(fn, "w");
   if (!f) return -2;
   const long long totv = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
@@ -312,7 +313,7 @@ extern "C" int nsx_write_tec_center(const char *dir, int counter, double time, i
   if (!dir || !No_vp || !xvt || !yvt || !zvt || !u || !v || !w || !p || NZ < 3 || counter < 0 || counter > 99999)
     return -1;
   char fn[1024];
-  snprintf(fn, sizeof fn, "%s/C-%05d.tec", dir, counter);
+  if (snprintf(fn, sizeof fn, "%s/C-%05d.tec", dir, counter) >= (int)sizeof fn) return -1;  // path too long
   FILE *f = fopen(fn, "w");
   if (!f) return -2;
   const long long totv = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
@@ -384,7 +385,7 @@ extern "C" int nsx_write_vtk_vertex(const char *dir, int counter, int N_CELL0, i
   if (!dir || !No_vp || !xvt || !yvt || !zvt || !ufv || !vfv || !wfv || NZ < 3 || counter < 0 || counter > 9999)
     return -1;
   char fn[1024];
-  snprintf(fn, sizeof fn, "%s/V-%04d.vtk", dir, counter);
+  if (snprintf(fn, sizeof fn, "%s/V-%04d.vtk", dir, counter) >= (int)sizeof fn) return -1;  // path too long
   FILE *f = fopen(fn, "w");
   if (!f) return -2;
   const long long totv = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
@@ -436,7 +437,7 @@ extern "C" int nsx_write_tec_vc(const char *dir, int counter, double time, int N
       !w || !p || NZ < 3 || counter < 0 || counter > 99999)
     return -1;
   char fn[1024];
-  snprintf(fn, sizeof fn, "%s/VC-%05d.tec", dir, counter);
+  if (snprintf(fn, sizeof fn, "%s/VC-%05d.tec", dir, counter) >= (int)sizeof fn) return -1;  // path too long
   FILE *f = fopen(fn, "w");
   if (!f) return -2;
   const long long nodes_v = (long long)N_VERT * (NZ - 1), tote = (long long)N_CELL0 * (NZ - 2);
