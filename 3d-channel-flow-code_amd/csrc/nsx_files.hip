@@ -196,6 +196,7 @@ extern "C" int nsx_write_plane_stats(const double *planes, int NZ, int n_cell0, 
   fprintf(f[0], " TKE = %s\n", t);
   int rc = 0;
   for (int q = 0; q < 3; ++q)
+    if (ferror(f[q])) rc = -3;
     if (fclose(f[q]) != 0) rc = -3;
   return rc;
 }
@@ -302,7 +303,8 @@ extern "C" int nsx_write_tec_vertex(const char *dir, int counter, double time, i
   const double *vv[7] = {xvt, yvt, zvt, ufv, vfv, wfv, pfv};
   for (int q = 0; q < 7; ++q) put_e_block(f, Block{vv[q], (size_t)N_VERT, 0, N_VERT, 0, NZ - 1});
   put_bricks(f, No_vp, N_CELL0, N_VERT, NZ - 2);
-  return fclose(f) == 0 ? 0 : -3;
+  const int bad = ferror(f);  // a failed write along the way
+  return (fclose(f) == 0 && !bad) ? 0 : -3;
 }
 
 // SavetecCenter (SavetecCenter.F90): <dir>/C-NNNNN.tec of the vertex
@@ -339,7 +341,8 @@ extern "C" int nsx_write_tec_center(const char *dir, int counter, double time, i
   const double *cc[4] = {u, v, w, p};
   for (int q = 0; q < 4; ++q) put_e_block(f, Block{cc[q], (size_t)N_CELL, 0, N_CELL0, 1, NZ - 1});
   put_bricks(f, No_vp, N_CELL0, N_VERT, NZ - 2);
-  return fclose(f) == 0 ? 0 : -3;
+  const int bad = ferror(f);  // a failed write along the way
+  return (fclose(f) == 0 && !bad) ? 0 : -3;
 }
 
 // the list-directed records of the VTK writer as the reference build's
@@ -422,7 +425,8 @@ extern "C" int nsx_write_vtk_vertex(const char *dir, int counter, int N_CELL0, i
   put_list(f, " ", 0, nullptr, nullptr);
   put_list(f, "VECTORS velocity_s float", 0, nullptr, nullptr);
   put_e_rows(f, N_VERT, NZ - 1, 3, usv, vsv, wsv);
-  return fclose(f) == 0 ? 0 : -3;
+  const int bad = ferror(f);  // a failed write along the way
+  return (fclose(f) == 0 && !bad) ? 0 : -3;
 }
 
 // SavetecVC (SavetecVertexCenter.F90:212-305): <dir>/VC-NNNNN.tec, the
@@ -482,5 +486,6 @@ extern "C" int nsx_write_tec_vc(const char *dir, int counter, double time, int N
         fputc('\n', f);
       }
     }
-  return fclose(f) == 0 ? 0 : -3;
+  const int bad = ferror(f);  // a failed write along the way
+  return (fclose(f) == 0 && !bad) ? 0 : -3;
 }
