@@ -195,9 +195,10 @@ extern "C" int nsx_write_plane_stats(const double *planes, int NZ, int n_cell0, 
   fmt_list_real(t, tkesum / (double)(NZ - 2));
   fprintf(f[0], " TKE = %s\n", t);
   int rc = 0;
-  for (int q = 0; q < 3; ++q)
+  for (int q = 0; q < 3; ++q) {
     if (ferror(f[q])) rc = -3;
     if (fclose(f[q]) != 0) rc = -3;
+  }
   return rc;
 }
 
