@@ -1104,6 +1104,65 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // Partial rows: M gamma, delta; N gamma, delta, |r|^2, |r|_1; T |r|^2,
 // |r|_1, sum (x+e) Vol.
 
+// The 4-layer block-Jacobi preconditioner of the single-reduction CG: M is A
+// restricted to the interior layers of one quad (the four layers a thread
+// owns), with the diagonal A's rows have at the sigma ends (the Neumann
+// reflection of apply_row); its couplings to the layers of other quads are
+// dropped.  Rows outside the interior are the identity, so the solves below
+// give them zero.  M is symmetric (AB of a layer is AT of the layer below,
+// the same quotient) and negative definite like A's diagonal blocks.
+struct Blk4 {
+  float d[4], lo[4], up[4];
+};
+
+__device__ inline Blk4 blk4_of(const Geo &g, const Row1<float> (&R)[4], int k0) {
+  Blk4 B;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = k0 + u;
+    if (!R[u].in) {
+      B.d[u] = 1.f;
+      B.lo[u] = B.up[u] = 0.f;
+      continue;
+    }
+    float d = R[u].A0;
+    if (k + 1 > g.NZ - 2) d = d + R[u].AT;
+    if (k - 1 < 1) d = d + R[u].AB;
+    B.d[u] = d;
+    B.lo[u] = (u > 0 && R[u - 1].in) ? R[u].AB : 0.f;
+    B.up[u] = (u < 3 && R[u + 1].in) ? R[u].AT : 0.f;
+  }
+  return B;
+}
+
+// y = M^-1 f (Thomas, four rows)
+__device__ inline void blk4_solve(const Blk4 &B, const float (&f)[4], float (&y)[4]) {
+  float c[4], z[4];
+  float m = B.d[0];
+  c[0] = B.up[0] / m;
+  z[0] = f[0] / m;
+#pragma unroll
+  for (int u = 1; u < 4; ++u) {
+    m = B.d[u] - B.lo[u] * c[u - 1];
+    c[u] = B.up[u] / m;
+    z[u] = (f[u] - B.lo[u] * z[u - 1]) / m;
+  }
+  y[3] = z[3];
+#pragma unroll
+  for (int u = 2; u >= 0; --u) y[u] = z[u] - c[u] * y[u + 1];
+}
+
+// y = M x
+__device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float t = B.d[u] * x[u];
+    if (u > 0) t = t + B.lo[u] * x[u - 1];
+    if (u < 3) t = t + B.up[u] * x[u + 1];
+    y[u] = t;
+  }
+}
+
 // e with the pending alpha p.  The true-residual launch forms it at every
 // slot and the fold after it forms it again: one fused multiply-add in both,
 // so x takes exactly the values whose residual was tested (an unfused sum in
@@ -1111,6 +1170,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // stopping rule).
 __device__ inline float e_pending(float e, float p, float a) { return __builtin_fmaf(a, p, e); }
 
+// BLK: the 4-layer block-Jacobi preconditioner (above) instead of the point
+// Jacobi one; the solve then starts with a true-residual launch, which forms
+// u = M^-1 r0.
+template <bool BLK>
 __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, const double *rhs, double *x,
                                                     const double *Pin, uint32_t Gin, double *Pout, SolverState *st) {
   CgsCarry c;
@@ -1135,6 +1198,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     double t[4];
     reduce_partials<4>(Pin, Gin, t);  // k_cg_start: rho, |r|^2, |r|_1, sum x Vol
     c.maxr = sqrt(t[1]);
+    if (BLK) mode = 2;  // u0 = M^-1 r0 from a true residual
   } else if (prev == 2) {
     double t[3];
     reduce_partials<3>(Pin, Gin, t);
@@ -1151,7 +1215,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       st->e1 = e1;
       st->e2 = e2;
       st->xvol = t[2];
-      st->mp_pass += 1;
+      if (c.iter > 0) st->mp_pass += 1;  // not the start of a BLK solve
     }
   } else if (prev == 0) {
     double t[2];
@@ -1217,8 +1281,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   }
   if (fold) {
     // x += e (+ the pending alpha p), e = 0: the correction since the last
-    // update joins x
-    for_each_quad(g, [&](int, int, size_t s0) {
+    // update joins x (interior layers: e holds zero elsewhere)
+    for_each_quad(g, [&](int, int k0, size_t s0) {
       float ee[4];
       ld4(v.e + s0, ee);
       if (pend_in) {
@@ -1228,7 +1292,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         for (int u = 0; u < 4; ++u) ee[u] = e_pending(ee[u], pp[u], ap_in);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) x[s0 + u] = x[s0 + u] + (double)ee[u];
+      for (int u = 0; u < 4; ++u)
+        if (k0 + u >= 1 && k0 + u <= g.NZ - 2) x[s0 + u] = x[s0 + u] + (double)ee[u];
       const float z[4] = {0.f, 0.f, 0.f, 0.f};
       st4(v.e + s0, z);
     });
@@ -1245,6 +1310,33 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     // e at a slot, with the alpha p the last iteration left pending (the fold
     // of the next launch forms the same sum)
     auto ecur = [&](size_t s, bool) { return pend_in ? e_pending(e[s], p[s], ap_in) : e[s]; };
+    if (BLK) {
+      // by quads: u = M^-1 r over each quad's block
+      for_each_quad(g, [&](int i, int k0, size_t s0) {
+        Col<float> C;
+        load_col(g, i, C);
+        float qv[5], f[4], u4[4];
+        quot5(C, lv, k0, qv);
+        Row1<float> R[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          R[u] = row_of(g, lv, C, qv, k0, u);
+          f[u] = 0.f;
+          if (R[u].in) {
+            double res, xs;
+            true_residual(g, rhs, x, avec, i, k0 + u, ecur, res, xs);
+            f[u] = (float)res;
+            acc[0] += res * res;
+            acc[1] += fabs(res);
+            acc[2] += xs * (g.area[i] * g.dsig[k0 + u]);
+          }
+        }
+        blk4_solve(blk4_of(g, R, k0), f, u4);
+        st4(rt + s0, u4);
+      });
+      write_partials<3>(acc, Pout);
+      return;
+    }
     FOR_EACH_CELL(g, q) {
       int i, k;
       cell_of(g, q, i, k);
@@ -1270,14 +1362,33 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
           float qv[5], wo[4];
           quot5(C, lv, k0, qv);
+          if (BLK) {
+            Row1<float> R[4];
+            float wv[4], mr[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-            const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-            wo[u] = R.in ? wv / R.A0 : 0.f;
-            if (R.in) {
-              acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
-              acc[1] += (double)wv * (double)fc[u];
+            for (int u = 0; u < 4; ++u) {
+              R[u] = row_of(g, lv, C, qv, k0, u);
+              wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+            }
+            const Blk4 B = blk4_of(g, R, k0);
+            blk4_solve(B, wv, wo);
+            blk4_mul(B, fc, mr);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (R[u].in) {
+                acc[0] += (double)mr[u] * (double)fc[u];
+                acc[1] += (double)wv[u] * (double)fc[u];
+              }
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+              const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+              wo[u] = R.in ? wv / R.A0 : 0.f;
+              if (R.in) {
+                acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
+                acc[1] += (double)wv * (double)fc[u];
+              }
             }
           }
           st4(om + s0, wo);
@@ -1292,11 +1403,22 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
               const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
             float qv[5], so[4];
             quot5(C, lv, k0, qv);
+            if (BLK) {
+              Row1<float> R[4];
+              float sv[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-              const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-              so[u] = R.in ? sv / R.A0 : 0.f;
+              for (int u = 0; u < 4; ++u) {
+                R[u] = row_of(g, lv, C, qv, k0, u);
+                sv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+              }
+              blk4_solve(blk4_of(g, R, k0), sv, so);
+            } else {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+                const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                so[u] = R.in ? sv / R.A0 : 0.f;
+              }
             }
             st4(sg + s0, so);
           });
@@ -1346,20 +1468,38 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
         float qv[5], ro[4], wo[4], so[4], po[4], eo[4];
         quot5(C, lv, k0, qv);
+        Row1<float> R[4];
+        float wv[4], mr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-          const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+          R[u] = row_of(g, lv, C, qv, k0, u);
+          wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+        }
+        if (BLK) {
+          // om = M^-1 w, and r = M u for gamma and the norms
+          const Blk4 B = blk4_of(g, R, k0);
+          blk4_solve(B, wv, wo);
+          blk4_mul(B, fc, mr);
+        } else {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            wo[u] = R[u].in ? wv[u] / R[u].A0 : 0.f;
+            mr[u] = R[u].A0 * fc[u];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = R[u].in;
           const float pn = first ? ri[u] : ri[u] + bf * pi[u];
-          if (upd_e) eo[u] = R.in ? (ei[u] + ap_in * pi[u]) + af * pn : ei[u];
-          po[u] = R.in ? pn : 0.f;
-          ro[u] = R.in ? fc[u] : 0.f;
-          so[u] = R.in ? si[u] : 0.f;
-          wo[u] = R.in ? wv / R.A0 : 0.f;
-          if (R.in) {
-            const double rd = (double)R.A0 * (double)fc[u];
+          if (upd_e) eo[u] = in ? (ei[u] + ap_in * pi[u]) + af * pn : ei[u];
+          po[u] = in ? pn : 0.f;
+          ro[u] = in ? fc[u] : 0.f;
+          so[u] = in ? si[u] : 0.f;
+          wo[u] = in ? wo[u] : 0.f;
+          if (in) {
+            const double rd = BLK ? (double)mr[u] : (double)R[u].A0 * (double)fc[u];
             acc[0] += rd * (double)fc[u];
-            acc[1] += (double)wv * (double)fc[u];
+            acc[1] += (double)wv[u] * (double)fc[u];
             acc[2] += rd * rd;
             acc[3] += fabs(rd);
           }
@@ -1506,14 +1646,18 @@ int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, fl
 }
 
 int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
-               SolverState *st, hipEvent_t *ev) {
+               SolverState *st, hipEvent_t *ev, int blk) {
   // launch 0 reads k_cg_start's partials (part.u, solver grid); then the
   // launches alternate between part.c (even) and part.u (odd) on their grid
   double *Pout = (j & 1) ? part.u : part.c;
   const double *Pin = j == 0 ? part.u : ((j & 1) ? part.c : part.u);
   const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs);
   const size_t lds = level_lds(g, sizeof(float));
-  LAUNCH_TIMED(ev, 0, k_cgs, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+  if (blk)
+    LAUNCH_TIMED(ev, 0, k_cgs<true>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+  else
+    LAUNCH_TIMED(ev, 0, k_cgs<false>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout,
+                 st);
   return (int)hipGetLastError();
 }
 
@@ -1605,12 +1749,14 @@ int cg_blocks_per_cu(const Geo &g) {
 
 // resident blocks per CU of the single-reduction FP32 CG kernel
 int cgs_blocks_per_cu(const Geo &g) {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cgs, NSX_BLOCK, level_lds(g, 4)) !=
+  int n = 0, m = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cgs<false>, NSX_BLOCK, level_lds(g, 4)) !=
           hipSuccess ||
-      n < 1)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, (const void *)k_cgs<true>, NSX_BLOCK, level_lds(g, 4)) !=
+          hipSuccess ||
+      n < 1 || m < 1)
     return -1;
-  return n;
+  return n < m ? n : m;
 }
 
 int solver_blocks_per_cu(const Geo &g) {

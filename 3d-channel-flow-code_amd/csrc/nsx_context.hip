@@ -69,7 +69,7 @@ struct nsx_ctx {
   double *sm_planes = nullptr;  // SM_NPLANE x NZ
   int sm_n = 0;
   int precision = NSX_PREC_FP64;
-  int refine = NSX_REFINE_CGSR;
+  int refine = NSX_REFINE_CGSR_BLOCK;
   double delta = 0.1;
   double eps_inner = 1e-5;
   int max_passes = 30;
@@ -1042,7 +1042,7 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
         ev = events(ctx, 2);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev));
+      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev, ctx->refine == NSX_REFINE_CGSR_BLOCK));
       if (ev) ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 14});
       ++issued;
     }
@@ -1080,7 +1080,7 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
-  if (ctx->refine == NSX_REFINE_CGSR) return solve_cgs(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_CGSR || ctx->refine == NSX_REFINE_CGSR_BLOCK) return solve_cgs(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
   return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
 }
@@ -1638,19 +1638,19 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
   if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE &&
-      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR)
+      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
   if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
   // the single-reduction recurrences drift faster in FP32: with updates
   // less often than about every 1.3 decades the solve stalls at its cap
-  if (refine == NSX_REFINE_CGSR && tol < 0.05)
+  if ((refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK) && tol < 0.05)
     return nsx_fail(ctx, -1, "single-reduction CG needs delta >= 0.05 (got %g); NSX_REFINE_CG takes smaller", tol);
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
   ctx->precision = precision;
   ctx->refine = refine;
   if (refine == NSX_REFINE_RELIABLE)
     ctx->delta = tol;
-  else if (refine == NSX_REFINE_CG || refine == NSX_REFINE_CGSR)
+  else if (refine == NSX_REFINE_CG || refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK)
     ctx->delta_cg = tol;
   else
     ctx->eps_inner = tol;

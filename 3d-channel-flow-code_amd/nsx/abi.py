@@ -26,6 +26,7 @@ REFINE_RESTART = 1
 REFINE_CASCADE = 2
 REFINE_CG = 3
 REFINE_CGSR = 4
+REFINE_CGSR_BLOCK = 5
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 

@@ -335,11 +335,12 @@ class Context:
         self._check(self.lib.nsx_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, int(n), p.FinRK,
                                             p.Iturbu))
 
-    def set_precision(self, precision: int, refine: int = 4, tol: float = 0.1, max_passes: int = 30) -> None:
+    def set_precision(self, precision: int, refine: int = 5, tol: float = 0.1, max_passes: int = 30) -> None:
         """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: an FP32
-        Krylov solver inside FP64 refinement: refine 4 = single-reduction CG
-        with reliable updates at threshold tol (the default), 3 = the same CG
-        in two kernels per iteration, 0 = BiCGSTAB with reliable
+        Krylov solver inside FP64 refinement: refine 5 = single-reduction CG
+        with the 4-layer block-Jacobi preconditioner and reliable updates at
+        threshold tol (the default), 4 = the same with point Jacobi, 3 = the
+        point-Jacobi CG in two kernels per iteration, 0 = BiCGSTAB with reliable
         updates at threshold tol, 1 = restarted BiCGSTAB defect correction
         with inner tolerance tol, 2 = one FP32 BiCGSTAB pass to tol then the
         FP64 solve); step() takes its own precision and uses this refinement."""

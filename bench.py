@@ -92,10 +92,12 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20, parallel: bool = False)
             "s_per_step": step_s}
 
 
-# "mixed" is the library's default form: single-reduction FP32 CG with
-# reliable updates; "mixed_cg2" the same CG in two kernels per iteration
-MIXED_MODES = {"mixed": (4, 0.1), "mixed_cg2": (3, 0.1), "mixed_cascade": (2, 1e-5), "mixed_restart": (1, 1e-5),
-               "mixed_reliable": (0, 0.1)}
+# "mixed" is the library's default form: single-reduction FP32 CG with the
+# 4-layer block-Jacobi preconditioner and reliable updates; "mixed_point" the
+# same with the point Jacobi preconditioner; "mixed_cg2" the point-Jacobi CG
+# in two kernels per iteration
+MIXED_MODES = {"mixed": (5, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1), "mixed_cascade": (2, 1e-5),
+               "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
 # the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
 # writes e, rt, p (units of 4 B per interior prism; the few X launches that
 # form a true FP64 residual move more)
@@ -151,7 +153,8 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
     if launches:
         avg = ms / launches
         b = 9 * 4 * n
-        kernels["cg32_sr"] = {"kernel": "k_cgs", "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
+        kname = "k_cgs<true>" if precision and MIXED_MODES[name][0] == 5 else "k_cgs<false>"
+        kernels["cg32_sr"] = {"kernel": kname, "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                               "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")
     fields = {f: np.asarray(ctx.get(f))[:case.N_CELL0, 1:-1].copy() for f in ("UFN", "VFN", "WFN", "PFN")}
@@ -234,7 +237,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32+f64 (single-reduction FP32 CG in FP64 reliable-update refinement)" if "mixed" in res
+        "dtype": ("f32+f64 (block-Jacobi single-reduction FP32 CG in FP64 reliable-update refinement)" if "mixed" in res
                   else ("f64" if "fp64" in res else "f32+f64")),
         "data": "synthetic",
         "config": {"workload": f"channel {WORKLOAD[cfg]}", "prisms": case.n_interior,

@@ -267,7 +267,7 @@ int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int
  * the Krylov vectors, SpMV and preconditioner in FP32 and keeps the solution,
  * the refinement residual, inner products and norms in FP64; the solve stops
  * when the true FP64 residual meets the reference's rule with eps
- * (nsx_set_params).  Refinement (default NSX_REFINE_CGSR, delta 0.1):
+ * (nsx_set_params).  Refinement (default NSX_REFINE_CGSR_BLOCK, delta 0.1):
  *   NSX_REFINE_RELIABLE  one FP32 iteration whose residual is replaced by the
  *                        true FP64 residual whenever it has fallen by tol
  *                        (delta, e.g. 0.1); max_passes unused;
@@ -285,13 +285,18 @@ int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int
  *   NSX_REFINE_CGSR      the same CG in the single-reduction form of
  *                        Chronopoulos and Gear: one kernel and one global
  *                        reduction per iteration; tol as NSX_REFINE_CG, at
- *                        least 0.05 (its FP32 recurrences drift faster). */
+ *                        least 0.05 (its FP32 recurrences drift faster);
+ *   NSX_REFINE_CGSR_BLOCK  the same with a block-Jacobi preconditioner: A
+ *                        restricted to each group of four consecutive
+ *                        layers of a column (a 4x4 tridiagonal solve in
+ *                        the kernel, no extra memory traffic). */
 enum nsx_refine {
   NSX_REFINE_RELIABLE = 0,
   NSX_REFINE_RESTART = 1,
   NSX_REFINE_CASCADE = 2,
   NSX_REFINE_CG = 3,
-  NSX_REFINE_CGSR = 4
+  NSX_REFINE_CGSR = 4,
+  NSX_REFINE_CGSR_BLOCK = 5
 };
 int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
 

@@ -16,7 +16,8 @@ import pytest
 
 import oracle
 from common import interior, ppe_residual, rel_err
-from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_CGSR, REFINE_RELIABLE, REFINE_RESTART
+from nsx.abi import (PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK,
+                     REFINE_RELIABLE, REFINE_RESTART)
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +29,7 @@ def _ctx(case):
 
 
 MODES = {"reliable": (REFINE_RELIABLE, 0.1), "restart": (REFINE_RESTART, 1e-5), "cascade": (REFINE_CASCADE, 1e-5),
-         "cg": (REFINE_CG, 0.1), "cgsr": (REFINE_CGSR, 0.1)}
+         "cg": (REFINE_CG, 0.1), "cgsr": (REFINE_CGSR, 0.1), "cgsr_block": (REFINE_CGSR_BLOCK, 0.1)}
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
@@ -71,7 +72,7 @@ def test_mixed_solve_tiny_exact_start():
     assert np.all(interior(c, phi) == 0.0)
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
 def test_cg_no_solve(refine):
     """A zero right-hand side: the start meets the rule, no CG iteration runs."""
     c = config_case(0, jitter=0.1)
@@ -84,8 +85,9 @@ def test_cg_no_solve(refine):
     assert np.all(interior(c, phi) == 0.0)
 
 
+@pytest.mark.parametrize("refine", [REFINE_CGSR, REFINE_CGSR_BLOCK])
 @pytest.mark.parametrize("tol", [0.1, 0.3])
-def test_single_reduction_cg_solution_meets_rule(tol):
+def test_single_reduction_cg_solution_meets_rule(tol, refine):
     """The x a solve returns is the one whose true residual met the rule:
     host-checked on several right-hand sides.  The launches queued after a
     stop pass it on instead of running again from a slot written before
@@ -93,7 +95,7 @@ def test_single_reduction_cg_solution_meets_rule(tol):
     c = config_case(0, jitter=0.1)
     G = _ctx(c)
     G.init_state()
-    G.set_precision(PREC_MIXED, REFINE_CGSR, tol, 30)
+    G.set_precision(PREC_MIXED, refine, tol, 30)
     for seed in range(5, 10):
         rng = np.random.default_rng(seed)
         rhs = c.cell_field()
@@ -108,7 +110,7 @@ def test_single_reduction_cg_solution_meets_rule(tol):
     G.close()
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
 def test_solve_after_no_solve(refine):
     """A solve right after one that stopped at its start (no launch of the
     iteration ran) starts from its own state, not the previous solve's."""
@@ -163,7 +165,7 @@ def test_single_reduction_cg_tracks_cg():
     assert rel_err(interior(c, p2), interior(c, p1)) <= 1e-6
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
 def test_cg_cap_falls_back_to_fp64(refine):
     """A CG that reaches its cap (2 MaxIters + 100 iterations) keeps x at its
     last update and the FP64 BiCGSTAB finishes from there: the solution
