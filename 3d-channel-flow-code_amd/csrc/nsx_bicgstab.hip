@@ -567,17 +567,19 @@ __device__ inline void window_sweep(uint32_t nwin, uint32_t &w, uint32_t &wend, 
 // whichever way it was obtained.  Every global load of a window (own values,
 // neighbour indices, column data, the out-of-window neighbours and what `pre`
 // loads) is issued before the window's barrier, so their latencies overlap;
-// body(i, k0, s0, C, fc, fm, fp, f0, f1, f2) runs after it.
-template <typename T, class Own4, class Val4, class Val1, class Pre, class Body>
-__device__ inline void for_each_quad_formed(const Geo &g, Own4 &&own4, Val4 &&val4, Val1 &&val1, Pre &&pre,
-                                            Body &&body) {
+// body(i, k0, s0, C, fc, fm, fp, f0, f1, f2) runs after it.  The windows hold
+// W <= NSX_BLOCK quads; post(act) runs on every thread of the block after
+// the bodies of a window (it may synchronise the block).
+template <typename T, class Own4, class Val4, class Val1, class Pre, class Body, class Post>
+__device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&own4, Val4 &&val4, Val1 &&val1,
+                                              Pre &&pre, Body &&body, Post &&post) {
   __shared__ __align__(16) T win[NSX_BLOCK][4];
-  const uint32_t nwin = (g.nquad + NSX_BLOCK - 1) / NSX_BLOCK;
+  const uint32_t nwin = (g.nquad + W - 1) / W;
   uint32_t w0, wend, step;
   window_sweep(nwin, w0, wend, step);
   for (uint32_t wi = w0; wi < wend; wi += step) {
-    const uint32_t base = wi * NSX_BLOCK;
-    const uint32_t end = base + NSX_BLOCK < g.nquad ? base + NSX_BLOCK : g.nquad;
+    const uint32_t base = wi * W;
+    const uint32_t end = base + W < g.nquad ? base + W : g.nquad;
     const uint32_t t = base + threadIdx.x;
     const bool act = t < end;
     const uint32_t lim = end - base;
@@ -619,8 +621,15 @@ __device__ inline void for_each_quad_formed(const Geo &g, Own4 &&own4, Val4 &&va
         }
       body(i, k0, s0, C, fc, fm, fp, fn[0], fn[1], fn[2]);
     }
+    post(act);
     __syncthreads();
   }
+}
+
+template <typename T, class Own4, class Val4, class Val1, class Pre, class Body>
+__device__ inline void for_each_quad_formed(const Geo &g, Own4 &&own4, Val4 &&val4, Val1 &&val1, Pre &&pre,
+                                            Body &&body) {
+  for_each_quad_formed_w<T>(g, NSX_BLOCK, own4, val4, val1, pre, body, [](bool) {});
 }
 
 // the seven-point values of f = a + c b (f = a when first)
@@ -1170,12 +1179,208 @@ __device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4
 // stopping rule).
 __device__ inline float e_pending(float e, float p, float a) { return __builtin_fmaf(a, p, e); }
 
-// BLK: the 4-layer block-Jacobi preconditioner (above) instead of the point
-// Jacobi one; the solve then starts with a true-residual launch, which forms
-// u = M^-1 r0.
-template <bool BLK>
+// The line preconditioner of the single-reduction CG: M is A restricted to
+// the interior layers of one column (the vertical couplings, and the diagonal
+// with the Neumann reflection at the sigma ends); the lateral couplings are
+// dropped.  Line4 holds a quad's four rows: lo[0] couples layer k0 to k0-1
+// (the last row of the quad below), up[3] layer k0+3 to k0+4.  Rows outside
+// the interior are the identity.  M is symmetric and negative definite, and
+// strictly diagonally dominant (its diagonal carries the lateral terms).
+struct Line4 {
+  float d[4], lo[4], up[4];
+};
+
+__device__ inline Line4 line4_identity() {
+  Line4 L;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    L.d[u] = 1.f;
+    L.lo[u] = L.up[u] = 0.f;
+  }
+  return L;
+}
+
+__device__ inline Line4 line4_of(const Geo &g, const Row1<float> (&R)[4], int k0) {
+  Line4 L = line4_identity();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = k0 + u;
+    if (!R[u].in) continue;
+    float d = R[u].A0;
+    if (k + 1 > g.NZ - 2) d = d + R[u].AT;
+    if (k - 1 < 1) d = d + R[u].AB;
+    L.d[u] = d;
+    L.lo[u] = k - 1 >= 1 ? R[u].AB : 0.f;
+    L.up[u] = k + 1 <= g.NZ - 2 ? R[u].AT : 0.f;
+  }
+  return L;
+}
+
+// y = M x on a quad, with x at k0-1 and k0+4 in xm and xp
+__device__ inline void line4_mul(const Line4 &L, const float (&x)[4], float xm, float xp, float (&y)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    float t = L.d[u] * x[u];
+    t = t + L.lo[u] * (u > 0 ? x[u > 0 ? u - 1 : 0] : xm);
+    t = t + L.up[u] * (u < 3 ? x[u < 3 ? u + 1 : 3] : xp);
+    y[u] = t;
+  }
+}
+
+// quads per window of the line preconditioner: whole columns
+__device__ inline uint32_t line_window(const Geo &g) { return (uint32_t)(NSX_BLOCK / g.NQ) * (uint32_t)g.NQ; }
+
+// the position of this thread's quad in its column, in windows of W quads
+// (whole columns from a column's start); -1 past the window's columns
+__device__ inline int line_pos(const Geo &g, uint32_t W) {
+  return threadIdx.x < W ? (int)(threadIdx.x % (uint32_t)g.NQ) : -1;
+}
+
+// y = M^-1 f over the columns of a window of whole columns.  Every thread of
+// the block calls it (it synchronises the block); pos is the position of the
+// thread's quad in its column (0 .. NQ-1), -1 for a thread without a quad,
+// whose rows are the identity.  The threads of a column are consecutive
+// lanes, so the neighbours of the quad at `pos` are lanes -1 and +1.
+//
+// Each quad's rows 0-2 are solved for in terms of the last unknown of the
+// quad below (Y-) and its own last unknown (Y): y_u = z_u + a_u Y- + b_u Y.
+// Row 3 with quad+1's y_0 substituted is then a tridiagonal system in the
+// column's last unknowns, one per quad, solved by parallel cyclic reduction
+// (ceil(log2 NQ) steps); rows 0-2 follow from Y- and Y.
+__device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y)[4], int pos, int nq) {
+  __shared__ float xz[3][NSX_BLOCK], pr[2][4][NSX_BLOCK], yl[NSX_BLOCK];
+  const int lane = threadIdx.x;
+  const bool on = pos >= 0;
+  // rows 0-2: Thomas with three right-hand sides (f, -lo0 e0, -up2 e2)
+  const float i0 = 1.f / L.d[0];
+  const float c0 = L.up[0] * i0;
+  const float i1 = 1.f / (L.d[1] - L.lo[1] * c0);
+  const float c1 = L.up[1] * i1;
+  const float i2 = 1.f / (L.d[2] - L.lo[2] * c1);
+  float z0 = f[0] * i0, z1 = (f[1] - L.lo[1] * z0) * i1, z2 = (f[2] - L.lo[2] * z1) * i2;
+  float a0 = -L.lo[0] * i0, a1 = -L.lo[1] * a0 * i1, a2 = -L.lo[2] * a1 * i2;
+  const float b2 = -L.up[2] * i2;
+  z1 = z1 - c1 * z2;
+  z0 = z0 - c0 * z1;
+  a1 = a1 - c1 * a2;
+  a0 = a0 - c0 * a1;
+  const float b1 = -c1 * b2, b0 = -c0 * b1;
+  xz[0][lane] = z0;
+  xz[1][lane] = a0;
+  xz[2][lane] = b0;
+  __syncthreads();
+  float zn = 0.f, an = 0.f, bn = 0.f;
+  if (on && pos + 1 < nq) {
+    zn = xz[0][lane + 1];
+    an = xz[1][lane + 1];
+    bn = xz[2][lane + 1];
+  }
+  // row 3: lo3 y2 + d3 Y + up3 y0(quad+1) = f3
+  float ra = 0.f, rb = 1.f, rc = 0.f, rg = 0.f;
+  if (on) {
+    ra = L.lo[3] * a2;
+    rb = L.d[3] + L.lo[3] * b2 + L.up[3] * an;
+    rc = L.up[3] * bn;
+    rg = f[3] - L.lo[3] * z2 - L.up[3] * zn;
+  }
+  int buf = 0;
+  for (int h = 1; h < nq; h <<= 1, buf ^= 1) {
+    pr[buf][0][lane] = ra;
+    pr[buf][1][lane] = rb;
+    pr[buf][2][lane] = rc;
+    pr[buf][3][lane] = rg;
+    __syncthreads();
+    if (on) {
+      float k1 = 0.f, k2 = 0.f, am = 0.f, cm = 0.f, gm = 0.f, ap = 0.f, cp = 0.f, gp = 0.f;
+      if (pos >= h) {
+        am = pr[buf][0][lane - h];
+        k1 = ra / pr[buf][1][lane - h];
+        cm = pr[buf][2][lane - h];
+        gm = pr[buf][3][lane - h];
+      }
+      if (pos + h < nq) {
+        ap = pr[buf][0][lane + h];
+        k2 = rc / pr[buf][1][lane + h];
+        cp = pr[buf][2][lane + h];
+        gp = pr[buf][3][lane + h];
+      }
+      rb = rb - cm * k1 - ap * k2;
+      rg = rg - gm * k1 - gp * k2;
+      ra = -am * k1;
+      rc = -cp * k2;
+    }
+  }
+  const float Y = rg / rb;
+  yl[lane] = Y;
+  __syncthreads();
+  const float Ym = on && pos >= 1 ? yl[lane - 1] : 0.f;
+  y[0] = z0 + a0 * Ym + b0 * Y;
+  y[1] = z1 + a1 * Ym + b1 * Y;
+  y[2] = z2 + a2 * Ym + b2 * Y;
+  y[3] = Y;
+}
+
+// out = M^-1 A f with the line preconditioner, over windows of whole columns;
+// with acc, the partial sums <M f, f> and <A f, f> over the interior
+__device__ inline void line_product(const Geo &g, const float *lv, const float *f, float *out, double *acc) {
+  const uint32_t W = line_window(g);
+  const int lpos = line_pos(g, W);
+  Line4 L = line4_identity();
+  float wv[4] = {0.f, 0.f, 0.f, 0.f};
+  int k0s = 0;
+  size_t s0s = 0;
+  auto v4 = [&](size_t off, float(&o)[4]) { ld4(f + off, o); };
+  auto v1 = [&](size_t off) { return f[off]; };
+  for_each_quad_formed_w<float>(
+      g, W, v4, v4, v1, [&](size_t) {},
+      [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+          const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+        float qv[5], mr[4];
+        quot5(C, lv, k0, qv);
+        Row1<float> R[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          R[u] = row_of(g, lv, C, qv, k0, u);
+          wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+        }
+        L = line4_of(g, R, k0);
+        if (acc) {
+          line4_mul(L, fc, fm, fp, mr);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (R[u].in) {
+              acc[0] += (double)mr[u] * (double)fc[u];
+              acc[1] += (double)wv[u] * (double)fc[u];
+            }
+        }
+        k0s = k0;
+        s0s = s0;
+      },
+      [&](bool act) {
+        float wo[4];
+        if (!act) {
+          L = line4_identity();
+#pragma unroll
+          for (int u = 0; u < 4; ++u) wv[u] = 0.f;
+        }
+        line_solve(L, wv, wo, act ? lpos : -1, g.NQ);
+        if (act) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (k0s + u < 1 || k0s + u > g.NZ - 2) wo[u] = 0.f;
+          st4(out + s0s, wo);
+        }
+      });
+}
+
+// PC: the preconditioner, 0 point Jacobi, 1 the 4-layer blocks, 2 the
+// lines (above).  With blocks or lines the solve starts with a true-residual
+// launch, which forms u = M^-1 r0; with lines every phase runs on windows of
+// whole columns.
+template <int PC>
 __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, const double *rhs, double *x,
                                                     const double *Pin, uint32_t Gin, double *Pout, SolverState *st) {
+  constexpr bool BLK = PC == 1, LINE = PC == 2, PRE = PC != 0;
   CgsCarry c;
   if (j == 0) {
     c.gamma = c.alpha = c.maxr = 0.0;
@@ -1198,7 +1403,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     double t[4];
     reduce_partials<4>(Pin, Gin, t);  // k_cg_start: rho, |r|^2, |r|_1, sum x Vol
     c.maxr = sqrt(t[1]);
-    if (BLK) mode = 2;  // u0 = M^-1 r0 from a true residual
+    if (PRE) mode = 2;  // u0 = M^-1 r0 from a true residual
   } else if (prev == 2) {
     double t[3];
     reduce_partials<3>(Pin, Gin, t);
@@ -1215,7 +1420,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       st->e1 = e1;
       st->e2 = e2;
       st->xvol = t[2];
-      if (c.iter > 0) st->mp_pass += 1;  // not the start of a BLK solve
+      if (c.iter > 0) st->mp_pass += 1;  // not the start of a preconditioned solve
     }
   } else if (prev == 0) {
     double t[2];
@@ -1337,6 +1542,46 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       write_partials<3>(acc, Pout);
       return;
     }
+    if (LINE) {
+      // by windows of whole columns: u = M^-1 r over each column
+      const uint32_t W = line_window(g), nwin = (g.nquad + W - 1) / W;
+      const int lpos = line_pos(g, W);
+      uint32_t w0, wend, step;
+      window_sweep(nwin, w0, wend, step);
+      for (uint32_t wi = w0; wi < wend; wi += step) {
+        const uint32_t t = wi * W + threadIdx.x;
+        const bool act = lpos >= 0 && t < g.nquad;
+        Line4 L = line4_identity();
+        float f[4] = {0.f, 0.f, 0.f, 0.f}, u4[4];
+        int i = 0, k0 = 0;
+        if (act) {
+          i = (int)fdiv(t, g.divNQ);
+          k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
+          Col<float> C;
+          load_col(g, i, C);
+          float qv[5];
+          quot5(C, lv, k0, qv);
+          Row1<float> R[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            R[u] = row_of(g, lv, C, qv, k0, u);
+            if (R[u].in) {
+              double res, xs;
+              true_residual(g, rhs, x, avec, i, k0 + u, ecur, res, xs);
+              f[u] = (float)res;
+              acc[0] += res * res;
+              acc[1] += fabs(res);
+              acc[2] += xs * (g.area[i] * g.dsig[k0 + u]);
+            }
+          }
+          L = line4_of(g, R, k0);
+        }
+        line_solve(L, f, u4, act ? lpos : -1, g.NQ);
+        if (act) st4(rt + (size_t)i * g.NS + k0, u4);
+      }
+      write_partials<3>(acc, Pout);
+      return;
+    }
     FOR_EACH_CELL(g, q) {
       int i, k;
       cell_of(g, q, i, k);
@@ -1356,72 +1601,77 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     const float *rt = v.rt[in];
     float *om = v.om[in];
     double acc[2] = {0.0, 0.0};
-    for_each_quad_gathered(
-        g, rt, rt, 0.f, true, [&](size_t) {},
-        [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
-            const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
-          float qv[5], wo[4];
-          quot5(C, lv, k0, qv);
-          if (BLK) {
-            Row1<float> R[4];
-            float wv[4], mr[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              R[u] = row_of(g, lv, C, qv, k0, u);
-              wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-            }
-            const Blk4 B = blk4_of(g, R, k0);
-            blk4_solve(B, wv, wo);
-            blk4_mul(B, fc, mr);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              if (R[u].in) {
-                acc[0] += (double)mr[u] * (double)fc[u];
-                acc[1] += (double)wv[u] * (double)fc[u];
-              }
-          } else {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-              const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-              wo[u] = R.in ? wv / R.A0 : 0.f;
-              if (R.in) {
-                acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
-                acc[1] += (double)wv * (double)fc[u];
-              }
-            }
-          }
-          st4(om + s0, wo);
-        });
-    if (fold && c.iter > 0) {
-      // after a true residual, s = A p afresh: its recurrence drifts in FP32
-      // from the product it stands for, and with it the recursive residual
-      float *sg = v.sg[in];
+    if (LINE) {
+      line_product(g, lv, rt, om, acc);
+      if (fold && c.iter > 0) line_product(g, lv, (const float *)v.p, v.sg[in], (double *)nullptr);
+    } else {
       for_each_quad_gathered(
-          g, (const float *)v.p, (const float *)v.p, 0.f, true, [&](size_t) {},
+          g, rt, rt, 0.f, true, [&](size_t) {},
           [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
               const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
-            float qv[5], so[4];
+            float qv[5], wo[4];
             quot5(C, lv, k0, qv);
             if (BLK) {
               Row1<float> R[4];
-              float sv[4];
-#pragma unroll
+              float wv[4], mr[4];
+  #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 R[u] = row_of(g, lv, C, qv, k0, u);
-                sv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
               }
-              blk4_solve(blk4_of(g, R, k0), sv, so);
+              const Blk4 B = blk4_of(g, R, k0);
+              blk4_solve(B, wv, wo);
+              blk4_mul(B, fc, mr);
+  #pragma unroll
+              for (int u = 0; u < 4; ++u)
+                if (R[u].in) {
+                  acc[0] += (double)mr[u] * (double)fc[u];
+                  acc[1] += (double)wv[u] * (double)fc[u];
+                }
             } else {
-#pragma unroll
+  #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-                const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-                so[u] = R.in ? sv / R.A0 : 0.f;
+                const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                wo[u] = R.in ? wv / R.A0 : 0.f;
+                if (R.in) {
+                  acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
+                  acc[1] += (double)wv * (double)fc[u];
+                }
               }
             }
-            st4(sg + s0, so);
+            st4(om + s0, wo);
           });
+      if (fold && c.iter > 0) {
+        // after a true residual, s = A p afresh: its recurrence drifts in FP32
+        // from the product it stands for, and with it the recursive residual
+        float *sg = v.sg[in];
+        for_each_quad_gathered(
+            g, (const float *)v.p, (const float *)v.p, 0.f, true, [&](size_t) {},
+            [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+                const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+              float qv[5], so[4];
+              quot5(C, lv, k0, qv);
+              if (BLK) {
+                Row1<float> R[4];
+                float sv[4];
+  #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  R[u] = row_of(g, lv, C, qv, k0, u);
+                  sv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                }
+                blk4_solve(blk4_of(g, R, k0), sv, so);
+              } else {
+  #pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  const Row1<float> R = row_of(g, lv, C, qv, k0, u);
+                  const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                  so[u] = R.in ? sv / R.A0 : 0.f;
+                }
+              }
+              st4(sg + s0, so);
+            });
+      }
     }
     write_partials<2>(acc, Pout);
     return;
@@ -1462,8 +1712,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     if (upd_e) ld4(v.e + off, ei);
   };
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for_each_quad_formed<float>(
-      g, own4, val4, val1, pre,
+  // with lines, om = M^-1 w waits for the window's columns (post)
+  Line4 Ln = line4_identity();
+  float wvn[4] = {0.f, 0.f, 0.f, 0.f};
+  int k0n = 0;
+  size_t s0n = 0;
+  const uint32_t W = LINE ? line_window(g) : NSX_BLOCK;
+  const int lpos = LINE ? line_pos(g, W) : -1;
+  for_each_quad_formed_w<float>(
+      g, W, own4, val4, val1, pre,
       [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
           const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
         float qv[5], ro[4], wo[4], so[4], po[4], eo[4];
@@ -1475,7 +1732,18 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           R[u] = row_of(g, lv, C, qv, k0, u);
           wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
         }
-        if (BLK) {
+        if (LINE) {
+          // r = M u for gamma and the norms
+          Ln = line4_of(g, R, k0);
+          line4_mul(Ln, fc, fm, fp, mr);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            wvn[u] = wv[u];
+            wo[u] = 0.f;
+          }
+          k0n = k0;
+          s0n = s0;
+        } else if (BLK) {
           // om = M^-1 w, and r = M u for gamma and the norms
           const Blk4 B = blk4_of(g, R, k0);
           blk4_solve(B, wv, wo);
@@ -1497,7 +1765,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           so[u] = in ? si[u] : 0.f;
           wo[u] = in ? wo[u] : 0.f;
           if (in) {
-            const double rd = BLK ? (double)mr[u] : (double)R[u].A0 * (double)fc[u];
+            const double rd = PRE ? (double)mr[u] : (double)R[u].A0 * (double)fc[u];
             acc[0] += rd * (double)fc[u];
             acc[1] += (double)wv[u] * (double)fc[u];
             acc[2] += rd * rd;
@@ -1507,8 +1775,24 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         if (upd_e) st4(v.e + s0, eo);
         st4(v.p + s0, po);
         st4(rt_o + s0, ro);
-        st4(om_o + s0, wo);
+        if (!LINE) st4(om_o + s0, wo);
         st4(sg_o + s0, so);
+      },
+      [&](bool act) {
+        if (!LINE) return;
+        float wo[4];
+        if (!act) {
+          Ln = line4_identity();
+#pragma unroll
+          for (int u = 0; u < 4; ++u) wvn[u] = 0.f;
+        }
+        line_solve(Ln, wvn, wo, act ? lpos : -1, g.NQ);
+        if (act) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (k0n + u < 1 || k0n + u > g.NZ - 2) wo[u] = 0.f;
+          st4(om_o + s0n, wo);
+        }
       });
   write_partials<4>(acc, Pout);
 }
@@ -1646,18 +1930,19 @@ int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, fl
 }
 
 int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
-               SolverState *st, hipEvent_t *ev, int blk) {
+               SolverState *st, hipEvent_t *ev, int pc) {
   // launch 0 reads k_cg_start's partials (part.u, solver grid); then the
   // launches alternate between part.c (even) and part.u (odd) on their grid
   double *Pout = (j & 1) ? part.u : part.c;
   const double *Pin = j == 0 ? part.u : ((j & 1) ? part.c : part.u);
   const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs);
   const size_t lds = level_lds(g, sizeof(float));
-  if (blk)
-    LAUNCH_TIMED(ev, 0, k_cgs<true>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+  if (pc == 2)
+    LAUNCH_TIMED(ev, 0, k_cgs<2>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+  else if (pc == 1)
+    LAUNCH_TIMED(ev, 0, k_cgs<1>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
   else
-    LAUNCH_TIMED(ev, 0, k_cgs<false>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout,
-                 st);
+    LAUNCH_TIMED(ev, 0, k_cgs<0>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
   return (int)hipGetLastError();
 }
 
@@ -1749,14 +2034,15 @@ int cg_blocks_per_cu(const Geo &g) {
 
 // resident blocks per CU of the single-reduction FP32 CG kernel
 int cgs_blocks_per_cu(const Geo &g) {
-  int n = 0, m = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cgs<false>, NSX_BLOCK, level_lds(g, 4)) !=
-          hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, (const void *)k_cgs<true>, NSX_BLOCK, level_lds(g, 4)) !=
-          hipSuccess ||
-      n < 1 || m < 1)
-    return -1;
-  return n < m ? n : m;
+  const void *ks[3] = {(const void *)k_cgs<0>, (const void *)k_cgs<1>, (const void *)k_cgs<2>};
+  int m = 1 << 30;
+  for (const void *k : ks) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, NSX_BLOCK, level_lds(g, 4)) != hipSuccess || n < 1)
+      return -1;
+    m = n < m ? n : m;
+  }
+  return m;
 }
 
 int solver_blocks_per_cu(const Geo &g) {

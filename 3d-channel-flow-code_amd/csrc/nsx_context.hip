@@ -1020,7 +1020,7 @@ static int solve_cg(nsx_ctx *ctx, const double *rhs, const double *phi_in, doubl
   return 0;
 }
 
-// Mixed precision, NSX_REFINE_CGSR: the same FP32 CG with reliable updates
+// Mixed precision, NSX_REFINE_CGSR (_BLOCK, _LINE): the same FP32 CG with reliable updates
 // in the single-reduction form, one kernel per iteration (k_cgs); a launch
 // is an iteration, a true residual or the product after one, as the device
 // decides.  Fallback as in solve_cg.
@@ -1031,6 +1031,8 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
   LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
                         ctx->delta_cg, cap));
   const CgsVecs v{ctx->fe, ctx->fp, {ctx->fr, ctx->fr2}, {ctx->fv, ctx->fp2}, {ctx->ft, ctx->fe2}};
+  // the preconditioner: point Jacobi, 4-layer blocks or lines
+  const int pc = ctx->refine == NSX_REFINE_CGSR_LINE ? 2 : ctx->refine == NSX_REFINE_CGSR_BLOCK ? 1 : 0;
   int issued = 0;
   const size_t pend0 = ctx->pending.size();
   const int hint = ctx->iter_hint_cgs;
@@ -1042,7 +1044,7 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
         ev = events(ctx, 2);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev, ctx->refine == NSX_REFINE_CGSR_BLOCK));
+      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev, pc));
       if (ev) ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 14});
       ++issued;
     }
@@ -1080,7 +1082,8 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
-  if (ctx->refine == NSX_REFINE_CGSR || ctx->refine == NSX_REFINE_CGSR_BLOCK) return solve_cgs(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_CGSR || ctx->refine == NSX_REFINE_CGSR_BLOCK || ctx->refine == NSX_REFINE_CGSR_LINE)
+    return solve_cgs(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
   return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
 }
@@ -1638,19 +1641,24 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
   if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE &&
-      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK)
+      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK &&
+      refine != NSX_REFINE_CGSR_LINE)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
+  const bool sr = refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK || refine == NSX_REFINE_CGSR_LINE;
+  // a window of the line preconditioner holds whole columns of four-layer quads
+  if (refine == NSX_REFINE_CGSR_LINE && ctx->g.NQ > NSX_BLOCK)
+    return nsx_fail(ctx, -1, "line preconditioner: %d layers per column exceed %d", ctx->g.NS, 4 * NSX_BLOCK);
   if (!(tol > 0.0 && tol < 1.0)) return nsx_fail(ctx, -1, "tolerance %g outside (0,1)", tol);
   // the single-reduction recurrences drift faster in FP32: with updates
   // less often than about every 1.3 decades the solve stalls at its cap
-  if ((refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK) && tol < 0.05)
+  if (sr && tol < 0.05)
     return nsx_fail(ctx, -1, "single-reduction CG needs delta >= 0.05 (got %g); NSX_REFINE_CG takes smaller", tol);
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
   ctx->precision = precision;
   ctx->refine = refine;
   if (refine == NSX_REFINE_RELIABLE)
     ctx->delta = tol;
-  else if (refine == NSX_REFINE_CG || refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK)
+  else if (refine == NSX_REFINE_CG || sr)
     ctx->delta_cg = tol;
   else
     ctx->eps_inner = tol;

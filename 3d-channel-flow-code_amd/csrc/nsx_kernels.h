@@ -138,9 +138,9 @@ int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double 
 int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *const v[7], Partials part,
                    SolverState *st, hipEvent_t *ev /* nullable: 4 events, start/stop of W, X */);
 // one launch of the single-reduction FP32 CG (launch j of the solve; its mode
-// is decided on the device); ev: one timed kernel; blk: the 4-layer
-// block-Jacobi preconditioner instead of the point Jacobi one
+// is decided on the device); ev: one timed kernel; pc: the preconditioner,
+// 0 point Jacobi, 1 4-layer blocks, 2 lines (whole columns)
 int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
-               SolverState *st, hipEvent_t *ev, int blk);
+               SolverState *st, hipEvent_t *ev, int pc);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

@@ -28,7 +28,8 @@
       integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1
       integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
            NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2,              &
-           NSX_REFINE_CG = 3, NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5
+           NSX_REFINE_CG = 3, NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5, &
+           NSX_REFINE_CGSR_LINE = 6
 !     accumulators of the channel statistics (enum nsx_sm, module stats)
       integer(c_int), parameter :: NSX_SM_UF = 0, NSX_SM_VF = 1,        &
            NSX_SM_WF = 2, NSX_SM_U2F = 3, NSX_SM_V2F = 4, NSX_SM_W2F = 5,&

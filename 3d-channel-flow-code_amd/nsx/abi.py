@@ -27,6 +27,7 @@ REFINE_CASCADE = 2
 REFINE_CG = 3
 REFINE_CGSR = 4
 REFINE_CGSR_BLOCK = 5
+REFINE_CGSR_LINE = 6
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 

@@ -339,7 +339,8 @@ class Context:
         """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: an FP32
         Krylov solver inside FP64 refinement: refine 5 = single-reduction CG
         with the 4-layer block-Jacobi preconditioner and reliable updates at
-        threshold tol (the default), 4 = the same with point Jacobi, 3 = the
+        threshold tol (the default), 6 = the same with the line (whole column)
+        preconditioner, 4 = the same with point Jacobi, 3 = the
         point-Jacobi CG in two kernels per iteration, 0 = BiCGSTAB with reliable
         updates at threshold tol, 1 = restarted BiCGSTAB defect correction
         with inner tolerance tol, 2 = one FP32 BiCGSTAB pass to tol then the

@@ -93,10 +93,10 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20, parallel: bool = False)
 
 
 # "mixed" is the library's default form: single-reduction FP32 CG with the
-# 4-layer block-Jacobi preconditioner and reliable updates; "mixed_point" the
-# same with the point Jacobi preconditioner; "mixed_cg2" the point-Jacobi CG
-# in two kernels per iteration
-MIXED_MODES = {"mixed": (5, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1), "mixed_cascade": (2, 1e-5),
+# 4-layer block-Jacobi preconditioner and reliable updates; "mixed_line" the
+# same with the line preconditioner (whole columns), "mixed_point" with point
+# Jacobi; "mixed_cg2" the point-Jacobi CG in two kernels per iteration
+MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1), "mixed_cascade": (2, 1e-5),
                "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
 # the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
 # writes e, rt, p (units of 4 B per interior prism; the few X launches that
@@ -153,7 +153,7 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
     if launches:
         avg = ms / launches
         b = 9 * 4 * n
-        kname = "k_cgs<true>" if precision and MIXED_MODES[name][0] == 5 else "k_cgs<false>"
+        kname = "k_cgs<%d>" % {4: 0, 5: 1, 6: 2}[MIXED_MODES[name][0]]
         kernels["cg32_sr"] = {"kernel": kname, "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                               "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")

@@ -289,14 +289,19 @@ int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int
  *   NSX_REFINE_CGSR_BLOCK  the same with a block-Jacobi preconditioner: A
  *                        restricted to each group of four consecutive
  *                        layers of a column (a 4x4 tridiagonal solve in
- *                        the kernel, no extra memory traffic). */
+ *                        the kernel, no extra memory traffic);
+ *   NSX_REFINE_CGSR_LINE   the same with a line preconditioner: A restricted
+ *                        to the interior layers of each column (a
+ *                        tridiagonal solve over the column in the kernel's
+ *                        shared memory; columns of at most 1024 layers). */
 enum nsx_refine {
   NSX_REFINE_RELIABLE = 0,
   NSX_REFINE_RESTART = 1,
   NSX_REFINE_CASCADE = 2,
   NSX_REFINE_CG = 3,
   NSX_REFINE_CGSR = 4,
-  NSX_REFINE_CGSR_BLOCK = 5
+  NSX_REFINE_CGSR_BLOCK = 5,
+  NSX_REFINE_CGSR_LINE = 6
 };
 int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
 

@@ -17,7 +17,7 @@ import pytest
 import oracle
 from common import interior, ppe_residual, rel_err
 from nsx.abi import (PREC_FP64, PREC_MIXED, REFINE_CASCADE, REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK,
-                     REFINE_RELIABLE, REFINE_RESTART)
+                     REFINE_CGSR_LINE, REFINE_RELIABLE, REFINE_RESTART)
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -29,7 +29,8 @@ def _ctx(case):
 
 
 MODES = {"reliable": (REFINE_RELIABLE, 0.1), "restart": (REFINE_RESTART, 1e-5), "cascade": (REFINE_CASCADE, 1e-5),
-         "cg": (REFINE_CG, 0.1), "cgsr": (REFINE_CGSR, 0.1), "cgsr_block": (REFINE_CGSR_BLOCK, 0.1)}
+         "cg": (REFINE_CG, 0.1), "cgsr": (REFINE_CGSR, 0.1), "cgsr_block": (REFINE_CGSR_BLOCK, 0.1),
+         "cgsr_line": (REFINE_CGSR_LINE, 0.1)}
 
 
 @pytest.mark.parametrize("mode", sorted(MODES))
@@ -72,7 +73,7 @@ def test_mixed_solve_tiny_exact_start():
     assert np.all(interior(c, phi) == 0.0)
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK, REFINE_CGSR_LINE])
 def test_cg_no_solve(refine):
     """A zero right-hand side: the start meets the rule, no CG iteration runs."""
     c = config_case(0, jitter=0.1)
@@ -85,7 +86,7 @@ def test_cg_no_solve(refine):
     assert np.all(interior(c, phi) == 0.0)
 
 
-@pytest.mark.parametrize("refine", [REFINE_CGSR, REFINE_CGSR_BLOCK])
+@pytest.mark.parametrize("refine", [REFINE_CGSR, REFINE_CGSR_BLOCK, REFINE_CGSR_LINE])
 @pytest.mark.parametrize("tol", [0.1, 0.3])
 def test_single_reduction_cg_solution_meets_rule(tol, refine):
     """The x a solve returns is the one whose true residual met the rule:
@@ -110,7 +111,7 @@ def test_single_reduction_cg_solution_meets_rule(tol, refine):
     G.close()
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK, REFINE_CGSR_LINE])
 def test_solve_after_no_solve(refine):
     """A solve right after one that stopped at its start (no launch of the
     iteration ran) starts from its own state, not the previous solve's."""
@@ -130,6 +131,29 @@ def test_solve_after_no_solve(refine):
     assert info["exit"] == "converged", info
     e1, e2 = ppe_residual(c, phi, rhs)
     assert min(e1, e2) < 2 * c.params.eps, (e1, e2)
+
+
+@pytest.mark.parametrize("cfg", [0, 3])
+def test_preconditioners_cut_iterations(cfg):
+    """Point Jacobi, 4-layer blocks and whole lines of the same matrix: each
+    wider preconditioner holds more of A, and CG needs fewer iterations
+    (a wrong block or line solve would slow it down or stall it)."""
+    c = config_case(cfg, jitter=0.1)
+    rng = np.random.default_rng(21)
+    rhs = c.cell_field()
+    rhs[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
+    its = {}
+    for refine in (REFINE_CGSR, REFINE_CGSR_BLOCK, REFINE_CGSR_LINE):
+        G = _ctx(c)
+        G.init_state()
+        G.set_precision(PREC_MIXED, refine, 0.1, 30)
+        phi, _, info = G.bicgstab(c.cell_field(), rhs)
+        G.close()
+        assert info["exit"] == "converged", (refine, info)
+        e1, e2 = ppe_residual(c, phi, rhs)
+        assert min(e1, e2) < 2 * c.params.eps, (refine, e1, e2)
+        its[refine] = info["iters"]
+    assert its[REFINE_CGSR_LINE] < its[REFINE_CGSR_BLOCK] < its[REFINE_CGSR], its
 
 
 def test_single_reduction_cg_rejects_rare_updates():
@@ -165,7 +189,7 @@ def test_single_reduction_cg_tracks_cg():
     assert rel_err(interior(c, p2), interior(c, p1)) <= 1e-6
 
 
-@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK])
+@pytest.mark.parametrize("refine", [REFINE_CG, REFINE_CGSR, REFINE_CGSR_BLOCK, REFINE_CGSR_LINE])
 def test_cg_cap_falls_back_to_fp64(refine):
     """A CG that reaches its cap (2 MaxIters + 100 iterations) keeps x at its
     last update and the FP64 BiCGSTAB finishes from there: the solution
