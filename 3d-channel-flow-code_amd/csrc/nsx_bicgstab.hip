@@ -1085,18 +1085,20 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // A u_{i+1} is formed in the same pass as the updates: u_{i+1} = rt -
 // alpha_i (om + beta_i sg) at the seven points, from the stored scaled
 // vectors rt = u = r/A0, om = w/A0 and sg = s/A0.  One launch per iteration
-// (mode N) reads rt, om, sg, p and writes rt, om, sg (into the other of two
-// buffer sets, since it reads them at neighbouring prisms) and p (in place);
-// every other iteration it also reads and writes e (in place), adding the
-// alpha p left pending by the iteration before: 32 or 40 B per prism, against
-// 40 B in the two kernels of NSX_REFINE_CG, in one pass with one reduction.
-// In exact arithmetic the iterates are CG's.
+// (mode N) reads rt, om, sg and writes rt, om, sg into the other of two
+// buffer sets (it reads them at neighbouring prisms).  p and e are needed
+// only at the launch's own prisms, and only every other iteration touches
+// them: it forms the p the iteration before left pending from that
+// iteration's u (rt of the other set) and the stored p, then its own p,
+// and adds both alpha p terms to e (p and e in place): 24 or 44 B per prism,
+// 34 B on average, against 40 B in the two kernels of NSX_REFINE_CG, in one
+// pass with one reduction.  In exact arithmetic the iterates are CG's.
 //
 // Reliable updates as in NSX_REFINE_CG: when an iteration's residual |r|
 // has fallen below delta x its largest value since the last true residual,
 // or meets the stopping rule, the next launch (mode T) forms the true FP64
-// residual of x + e (with any pending alpha p) into rt; the launch after it
-// (mode M) folds e (and the pending alpha p) into x,
+// residual of x + e (with any pending alpha p, p formed as above) into rt;
+// the launch after it (mode M) folds e (and the pending alpha p) into x,
 // applies the stopping rule to the true residual, forms om = A rt / A0 with
 // gamma and delta, and sg = A p / A0 afresh (the recurrence for s = A p
 // drifts in FP32, and with it the recursive residual; without the reset the
@@ -1178,6 +1180,10 @@ __device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4
 // one of them moves x by an FP32 rounding of e, which A can amplify past the
 // stopping rule).
 __device__ inline float e_pending(float e, float p, float a) { return __builtin_fmaf(a, p, e); }
+
+// p_i = u_i + beta_i p_{i-1} of an iteration that stored no p (u_i alone
+// when it was the first): the same expression wherever it is formed
+__device__ inline float p_lagged(float u, float p, float b, bool first) { return first ? u : u + b * p; }
 
 // The line preconditioner of the single-reduction CG: M is A restricted to
 // the interior layers of one column (the vertical couplings, and the diagonal
@@ -1320,17 +1326,19 @@ __device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y
   y[3] = Y;
 }
 
-// out = M^-1 A f with the line preconditioner, over windows of whole columns;
-// with acc, the partial sums <M f, f> and <A f, f> over the interior
-__device__ inline void line_product(const Geo &g, const float *lv, const float *f, float *out, double *acc) {
+// out = M^-1 A f, f = a + c b (a when first), with the line preconditioner,
+// over windows of whole columns; with acc, the partial sums <M f, f> and
+// <A f, f> over the interior
+__device__ inline void line_product(const Geo &g, const float *lv, const float *a, const float *b, float cc,
+                                    bool first, float *out, double *acc) {
   const uint32_t W = line_window(g);
   const int lpos = line_pos(g, W);
   Line4 L = line4_identity();
   float wv[4] = {0.f, 0.f, 0.f, 0.f};
   int k0s = 0;
   size_t s0s = 0;
-  auto v4 = [&](size_t off, float(&o)[4]) { ld4(f + off, o); };
-  auto v1 = [&](size_t off) { return f[off]; };
+  auto v4 = [&](size_t off, float(&o)[4]) { axpy4(a, b, cc, first, off, o); };
+  auto v1 = [&](size_t off) { return axpy1(a, b, cc, first, off); };
   for_each_quad_formed_w<float>(
       g, W, v4, v4, v1, [&](size_t) {},
       [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
@@ -1383,9 +1391,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   constexpr bool BLK = PC == 1, LINE = PC == 2, PRE = PC != 0;
   CgsCarry c;
   if (j == 0) {
-    c.gamma = c.alpha = c.maxr = 0.0;
+    c.gamma = c.alpha = c.beta = c.maxr = 0.0;
     c.mode = -1;
-    c.iter = c.set = c.pend = c.pad = 0;
+    c.iter = c.set = c.pend = c.pend_e = c.first = c.pad = 0;
     c.done = st->done;  // k_cg_start's no-solve test (launch 0 itself never stops)
   } else {
     c = st->cgs[(j - 1) & 1];
@@ -1462,16 +1470,24 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   }
   const int in = c.set;
   // e and the pending alpha p as they stand when this launch starts
-  const bool pend_in = c.pend != 0;
-  const float ap_in = (float)c.alpha;
+  // p and e as they stand when this launch starts: when pend_in, the last
+  // iteration i stored neither; p_i = u_i + beta_i p_{i-1} from u_i (rt of
+  // the other set, which no launch rewrites before the next iteration) and
+  // the stored p_{i-1}, and e lacks alpha_i p_i unless a fold took it into x
+  const bool pend_in = c.pend != 0, pend_e_in = c.pend_e != 0, pfirst_in = c.first != 0;
+  const float ap_in = (float)c.alpha, bp_in = (float)c.beta;
+  const float *rt_lag = v.rt[in ^ 1];
   if (mode == 1 && !stop) {
     c.gamma = gam;
     c.alpha = al;
+    c.beta = first ? 0.0 : be;
+    c.first = first ? 1 : 0;
     c.iter += 1;
     c.set ^= 1;
-    c.pend = pend_in ? 0 : 1;  // iterations alternate: leave e alone, then add both terms
+    // iterations alternate: leave p and e alone, then form both terms
+    c.pend = c.pend_e = pend_in ? 0 : 1;
   }
-  if (fold) c.pend = 0;
+  if (fold) c.pend_e = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     CgsCarry o = c;
     o.mode = mode;
@@ -1490,11 +1506,12 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     for_each_quad(g, [&](int, int k0, size_t s0) {
       float ee[4];
       ld4(v.e + s0, ee);
-      if (pend_in) {
-        float pp[4];
-        ld4(v.p + s0, pp);
+      if (pend_e_in) {
+        float ua[4], pp[4] = {0.f, 0.f, 0.f, 0.f};
+        ld4(rt_lag + s0, ua);
+        if (!pfirst_in) ld4(v.p + s0, pp);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) ee[u] = e_pending(ee[u], pp[u], ap_in);
+        for (int u = 0; u < 4; ++u) ee[u] = e_pending(ee[u], p_lagged(ua[u], pp[u], bp_in, pfirst_in), ap_in);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -1514,7 +1531,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     double acc[3] = {0.0, 0.0, 0.0};
     // e at a slot, with the alpha p the last iteration left pending (the fold
     // of the next launch forms the same sum)
-    auto ecur = [&](size_t s, bool) { return pend_in ? e_pending(e[s], p[s], ap_in) : e[s]; };
+    auto ecur = [&](size_t s, bool) {
+      return pend_e_in ? e_pending(e[s], p_lagged(rt_lag[s], pfirst_in ? 0.f : p[s], bp_in, pfirst_in), ap_in) : e[s];
+    };
     if (BLK) {
       // by quads: u = M^-1 r over each quad's block
       for_each_quad(g, [&](int i, int k0, size_t s0) {
@@ -1602,8 +1621,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     float *om = v.om[in];
     double acc[2] = {0.0, 0.0};
     if (LINE) {
-      line_product(g, lv, rt, om, acc);
-      if (fold && c.iter > 0) line_product(g, lv, (const float *)v.p, v.sg[in], (double *)nullptr);
+      line_product(g, lv, rt, rt, 0.f, true, om, acc);
+      if (fold && c.iter > 0)
+        line_product(g, lv, pend_in ? rt_lag : (const float *)v.p, (const float *)v.p, pend_in ? bp_in : 0.f,
+                     pend_in ? pfirst_in : true, v.sg[in], (double *)nullptr);
     } else {
       for_each_quad_gathered(
           g, rt, rt, 0.f, true, [&](size_t) {},
@@ -1645,9 +1666,11 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       if (fold && c.iter > 0) {
         // after a true residual, s = A p afresh: its recurrence drifts in FP32
         // from the product it stands for, and with it the recursive residual
+        // (p formed from u and the stored p when it lags)
         float *sg = v.sg[in];
         for_each_quad_gathered(
-            g, (const float *)v.p, (const float *)v.p, 0.f, true, [&](size_t) {},
+            g, pend_in ? rt_lag : (const float *)v.p, (const float *)v.p, pend_in ? bp_in : 0.f,
+            pend_in ? pfirst_in : true, [&](size_t) {},
             [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
                 const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
               float qv[5], so[4];
@@ -1701,15 +1724,22 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     const float sg = first ? om_i[q] : om_i[q] + bf * sg_i[q];
     return rt_i[q] - af * sg;
   };
-  // e is read and written every other iteration: an iteration that finds no
-  // term pending leaves e alone (its alpha p becomes pending); the next adds
-  // both, alpha_{i-1} p_{i-1} + alpha_i p_i, from the p_{i-1} it reads anyway
-  const bool upd_e = pend_in;
-  float ri[4], si[4], pi[4], ei[4];
+  // p and e are read and written every other iteration: an iteration that
+  // finds them current leaves both alone (p_i = u_i + beta_i p_{i-1} and
+  // alpha_i p_i become pending); the next forms p_i from u_i, still in rt of
+  // its output set, and the stored p_{i-1}, then p_{i+1}, stores p_{i+1} and
+  // adds alpha_i p_i + alpha_{i+1} p_{i+1} to e (only the second term after
+  // a fold took the first into x).  The first iteration never finds them
+  // lagging.
+  const bool upd_p = pend_in;
+  float ri[4], si[4], pi[4] = {0.f, 0.f, 0.f, 0.f}, ei[4], ui[4];
   auto own4 = [&](size_t off, float(&o)[4]) { form4(off, ri, si, o); };
   auto pre = [&](size_t off) {
-    if (!first) ld4(v.p + off, pi);
-    if (upd_e) ld4(v.e + off, ei);
+    if (upd_p) {
+      ld4(rt_lag + off, ui);  // own slots of rt_o, read before this thread stores them
+      if (!pfirst_in) ld4(v.p + off, pi);
+      ld4(v.e + off, ei);
+    }
   };
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // with lines, om = M^-1 w waits for the window's columns (post)
@@ -1758,9 +1788,13 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const bool in = R[u].in;
-          const float pn = first ? ri[u] : ri[u] + bf * pi[u];
-          if (upd_e) eo[u] = in ? (ei[u] + ap_in * pi[u]) + af * pn : ei[u];
-          po[u] = in ? pn : 0.f;
+          if (upd_p) {
+            const float pl = p_lagged(ui[u], pi[u], bp_in, pfirst_in);
+            const float pn = ri[u] + bf * pl;
+            const float eb = pend_e_in ? ei[u] + ap_in * pl : ei[u];
+            eo[u] = in ? eb + af * pn : ei[u];
+            po[u] = in ? pn : 0.f;
+          }
           ro[u] = in ? fc[u] : 0.f;
           so[u] = in ? si[u] : 0.f;
           wo[u] = in ? wo[u] : 0.f;
@@ -1772,8 +1806,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             acc[3] += fabs(rd);
           }
         }
-        if (upd_e) st4(v.e + s0, eo);
-        st4(v.p + s0, po);
+        if (upd_p) {
+          st4(v.e + s0, eo);
+          st4(v.p + s0, po);
+        }
         st4(rt_o + s0, ro);
         if (!LINE) st4(om_o + s0, wo);
         st4(sg_o + s0, so);

@@ -21,12 +21,16 @@ struct Partials {
 // (launch j reads SolverState::cgs[(j-1)&1], block 0 writes cgs[j&1])
 struct CgsCarry {
   double gamma, alpha;  // gamma_i and alpha_i of the last iteration
+  double beta;          // beta_i of the last iteration (0 if it was the first)
   double maxr;          // largest |r| since the last true residual
   int mode;             // this launch's mode: 0 product (M), 1 iteration (N), 2 true residual (T)
   int iter;             // iterations (launches in mode N) so far
   int set;              // buffer set holding the current rt, om, sg
   int done;
-  int pend;             // e lacks alpha * p of the last iteration (which left e alone)
+  int pend;             // p lags: it holds p_{i-1} of the last iteration i, which stored no p
+                        // (u_i is in rt of the other set; no p at all if i was the first)
+  int pend_e;           // e lacks alpha_i p_i of that iteration (cleared by the fold into x)
+  int first;            // the last iteration was the solve's first
   int pad;
 };
 

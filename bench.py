@@ -145,14 +145,14 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
             b = units * 4 * n
             kernels["cg32_" + ph] = {"kernel": "k_cg_" + ph, "launches": launches, "avg_us": avg * 1e3,
                                      "bytes_per_launch": b, "GBps": b / (avg * 1e-3) / 1e9}
-    # the single-reduction CG: one kernel per iteration reading rt, om, sg, p
-    # and writing rt, om, sg, p (8 units), and every other iteration e too
-    # (10 units): 9 units a launch on average; its rare true-residual and
-    # product launches are counted with it
+    # the single-reduction CG: one kernel per iteration reading and writing
+    # rt, om, sg (6 units), and every other iteration also reading the last
+    # u, p and e and writing p and e (11 units): 8.5 units a launch on
+    # average; its rare true-residual and product launches are counted with it
     ms, launches = ctx.timer("cg32_sr")
     if launches:
         avg = ms / launches
-        b = 9 * 4 * n
+        b = 34 * n
         kname = "k_cgs<%d>" % {4: 0, 5: 1, 6: 2}[MIXED_MODES[name][0]]
         kernels["cg32_sr"] = {"kernel": kname, "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                               "GBps": b / (avg * 1e-3) / 1e9}

@@ -193,8 +193,11 @@ def test_single_reduction_cg_tracks_cg():
 def test_cg_cap_falls_back_to_fp64(refine):
     """A CG that reaches its cap (2 MaxIters + 100 iterations) keeps x at its
     last update and the FP64 BiCGSTAB finishes from there: the solution
-    still meets the rule, and the FP32 iterations are those of the cap."""
-    c = config_case(0, jitter=0.1)
+    still meets the rule, and the FP32 iterations are those of the cap.
+    On the 64x32 x 48 mesh, where even the line-preconditioned CG needs
+    enough iterations that the FP64 solve is left MaxIters of its own to
+    finish with."""
+    c = config_case(3)
     rng = np.random.default_rng(12)
     rhs = c.cell_field()
     rhs[:c.N_CELL0, 1:-1] = rng.standard_normal((c.N_CELL0, c.NZ - 2))
