@@ -1534,69 +1534,63 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     auto ecur = [&](size_t s, bool) {
       return pend_e_in ? e_pending(e[s], p_lagged(rt_lag[s], pfirst_in ? 0.f : p[s], bp_in, pfirst_in), ap_in) : e[s];
     };
-    if (BLK) {
-      // by quads: u = M^-1 r over each quad's block
-      for_each_quad(g, [&](int i, int k0, size_t s0) {
-        Col<float> C;
-        load_col(g, i, C);
-        float qv[5], f[4], u4[4];
-        quot5(C, lv, k0, qv);
-        Row1<float> R[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          R[u] = row_of(g, lv, C, qv, k0, u);
-          f[u] = 0.f;
-          if (R[u].in) {
-            double res, xs;
-            true_residual(g, rhs, x, avec, i, k0 + u, ecur, res, xs);
-            f[u] = (float)res;
-            acc[0] += res * res;
-            acc[1] += fabs(res);
-            acc[2] += xs * (g.area[i] * g.dsig[k0 + u]);
-          }
-        }
-        blk4_solve(blk4_of(g, R, k0), f, u4);
-        st4(rt + s0, u4);
-      });
-      write_partials<3>(acc, Pout);
-      return;
-    }
-    if (LINE) {
-      // by windows of whole columns: u = M^-1 r over each column
-      const uint32_t W = line_window(g), nwin = (g.nquad + W - 1) / W;
-      const int lpos = line_pos(g, W);
+    if (PRE) {
+      // window by window: the residuals cell by cell (consecutive threads on
+      // consecutive slots, as the vectors are stored, so the neighbour
+      // gathers coalesce) into LDS, then u = M^-1 r over each quad's block or
+      // each column's line.  A window of W quads from quad `base` covers the
+      // slots 4 base .. 4 (base + W) - 1.
+      __shared__ __align__(16) float rs[NSX_BLOCK][4];
+      const uint32_t W = LINE ? line_window(g) : NSX_BLOCK, nwin = (g.nquad + W - 1) / W;
+      const int lpos = LINE ? line_pos(g, W) : -1;
       uint32_t w0, wend, step;
       window_sweep(nwin, w0, wend, step);
       for (uint32_t wi = w0; wi < wend; wi += step) {
-        const uint32_t t = wi * W + threadIdx.x;
-        const bool act = lpos >= 0 && t < g.nquad;
+        const uint32_t base = wi * W;
+        const uint32_t lim = (base + W < g.nquad ? base + W : g.nquad) - base;
+        float *flat = &rs[0][0];
+        for (uint32_t m = threadIdx.x; m < 4 * W; m += NSX_BLOCK) {
+          float f = 0.f;
+          if (m < 4 * lim) {
+            const uint32_t q = base + (m >> 2);
+            const int i = (int)fdiv(q, g.divNQ);
+            const int k = 4 * (int)(q - (uint32_t)i * (uint32_t)g.NQ) + (int)(m & 3);
+            if (k >= 1 && k <= g.NZ - 2) {
+              double res, xs;
+              true_residual(g, rhs, x, avec, i, k, ecur, res, xs);
+              f = (float)res;
+              acc[0] += res * res;
+              acc[1] += fabs(res);
+              acc[2] += xs * (g.area[i] * g.dsig[k]);
+            }
+          }
+          flat[m] = f;
+        }
+        __syncthreads();
+        const bool act = threadIdx.x < lim;
+        float f[4] = {0.f, 0.f, 0.f, 0.f}, u4[4] = {0.f, 0.f, 0.f, 0.f};
         Line4 L = line4_identity();
-        float f[4] = {0.f, 0.f, 0.f, 0.f}, u4[4];
-        int i = 0, k0 = 0;
         if (act) {
-          i = (int)fdiv(t, g.divNQ);
-          k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
+          const uint32_t t = base + threadIdx.x;
+          const int i = (int)fdiv(t, g.divNQ);
+          const int k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) f[u] = rs[threadIdx.x][u];
           Col<float> C;
           load_col(g, i, C);
           float qv[5];
           quot5(C, lv, k0, qv);
           Row1<float> R[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            R[u] = row_of(g, lv, C, qv, k0, u);
-            if (R[u].in) {
-              double res, xs;
-              true_residual(g, rhs, x, avec, i, k0 + u, ecur, res, xs);
-              f[u] = (float)res;
-              acc[0] += res * res;
-              acc[1] += fabs(res);
-              acc[2] += xs * (g.area[i] * g.dsig[k0 + u]);
-            }
-          }
-          L = line4_of(g, R, k0);
+          for (int u = 0; u < 4; ++u) R[u] = row_of(g, lv, C, qv, k0, u);
+          if (LINE)
+            L = line4_of(g, R, k0);
+          else
+            blk4_solve(blk4_of(g, R, k0), f, u4);
         }
-        line_solve(L, f, u4, act ? lpos : -1, g.NQ);
-        if (act) st4(rt + (size_t)i * g.NS + k0, u4);
+        if (LINE) line_solve(L, f, u4, act ? lpos : -1, g.NQ);
+        if (act) st4(rt + (size_t)4 * (base + threadIdx.x), u4);
+        __syncthreads();  // rs is rewritten by the next window
       }
       write_partials<3>(acc, Pout);
       return;
