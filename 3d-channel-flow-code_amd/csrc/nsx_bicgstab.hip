@@ -577,6 +577,19 @@ __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&o
   const uint32_t nwin = (g.nquad + W - 1) / W;
   uint32_t w0, wend, step;
   window_sweep(nwin, w0, wend, step);
+  // the neighbour columns of this thread's quad in window wi, loaded one
+  // window ahead: the gathers of a window would otherwise wait on them
+  int nc[3] = {0, 0, 0};
+  auto nbr_of = [&](uint32_t wi) {
+    const uint32_t t = wi * W + threadIdx.x;
+    if (wi < wend && threadIdx.x < W && t < g.nquad) {
+      const int e = 3 * (int)fdiv(t, g.divNQ);
+      nc[0] = g.nbr[e];
+      nc[1] = g.nbr[e + 1];
+      nc[2] = g.nbr[e + 2];
+    }
+  };
+  nbr_of(w0);
   for (uint32_t wi = w0; wi < wend; wi += step) {
     const uint32_t base = wi * W;
     const uint32_t end = base + W < g.nquad ? base + W : g.nquad;
@@ -594,8 +607,7 @@ __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&o
       k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
       s0 = (size_t)i * g.NS + k0;
       own4(s0, fc);
-      const int e = i * 3;
-      const int n[3] = {g.nbr[e], g.nbr[e + 1], g.nbr[e + 2]};
+      const int n[3] = {nc[0], nc[1], nc[2]};
       load_col(g, i, C);
       pre(s0);
       const uint32_t qq = (uint32_t)(k0 >> 2);
@@ -607,6 +619,7 @@ __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&o
       if (k0 >= 1 && !mwin) fm = val1(s0 - 1);
       if (k0 + 4 <= g.NZ - 2 && !pwin) fp = val1(s0 + 4);
     }
+    nbr_of(wi + step);
 #pragma unroll
     for (int u = 0; u < 4; ++u) win[threadIdx.x][u] = fc[u];
     __syncthreads();
