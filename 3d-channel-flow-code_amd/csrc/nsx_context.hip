@@ -1027,12 +1027,16 @@ static int solve_cg(nsx_ctx *ctx, const double *rhs, const double *phi_in, doubl
 static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
+  // the preconditioner: point Jacobi, 4-layer blocks or lines (a window of
+  // the line preconditioner holds whole columns; nsx_set_precision checks
+  // this too, but may run before the mesh is committed)
+  const int pc = ctx->refine == NSX_REFINE_CGSR_LINE ? 2 : ctx->refine == NSX_REFINE_CGSR_BLOCK ? 1 : 0;
+  if (pc == 2 && g.NQ > NSX_BLOCK)
+    return nsx_fail(ctx, -1, "line preconditioner: %d layers per column exceed %d", g.NS, 4 * NSX_BLOCK);
   const int cap = 2 * ctx->MaxIters + 100;
   LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
                         ctx->delta_cg, cap));
   const CgsVecs v{ctx->fe, ctx->fp, {ctx->fr, ctx->fr2}, {ctx->fv, ctx->fp2}, {ctx->ft, ctx->fe2}};
-  // the preconditioner: point Jacobi, 4-layer blocks or lines
-  const int pc = ctx->refine == NSX_REFINE_CGSR_LINE ? 2 : ctx->refine == NSX_REFINE_CGSR_BLOCK ? 1 : 0;
   int issued = 0;
   const size_t pend0 = ctx->pending.size();
   const int hint = ctx->iter_hint_cgs;

@@ -156,6 +156,19 @@ def test_preconditioners_cut_iterations(cfg):
     assert its[REFINE_CGSR_LINE] < its[REFINE_CGSR_BLOCK] < its[REFINE_CGSR], its
 
 
+def test_line_preconditioner_rejects_tall_columns():
+    """A window of the line preconditioner holds whole columns of four-layer
+    quads: columns of more than 1024 layers are refused, the other forms
+    take them."""
+    from nsx.case import make_case
+    c = make_case(4, 2, 1100, seed=3, jitter=0.0)
+    G = _ctx(c)
+    with pytest.raises(RuntimeError, match="line preconditioner"):
+        G.set_precision(PREC_MIXED, REFINE_CGSR_LINE, 0.1, 30)
+    G.set_precision(PREC_MIXED, REFINE_CGSR_BLOCK, 0.1, 30)
+    G.close()
+
+
 def test_single_reduction_cg_rejects_rare_updates():
     """The single-reduction form needs a reliable update at least every 1.3
     decades (delta >= 0.05); the two-kernel form takes smaller deltas."""
