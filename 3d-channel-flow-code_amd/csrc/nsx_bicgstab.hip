@@ -176,6 +176,18 @@ __device__ inline void quot5(const Col<T> &C, const T *lv, int k0, T (&q)[5]) {
   }
 }
 
+// quot5 of the single-reduction FP32 CG: the LDS level table also holds
+// 1/dsig at lv[2NS..3NS), and q[u] = area * (1/dsig) is a product, not a
+// division (its FP32 operator rounds differently from area/dsig, in the last
+// bit; every launch of that CG forms its rows this way)
+__device__ inline void quot5_r(const Col<float> &C, const float *lv, int NS, int k0, float (&q)[5]) {
+#pragma unroll
+  for (int u = 0; u < 5; ++u) {
+    const int k = k0 - 1 + u;
+    q[u] = C.area * lv[2 * NS + (k < 0 ? 0 : k)];
+  }
+}
+
 // the row of layer k0+u of the column (ppe_row's arithmetic); `in` is false
 // outside the interior layers, where the solver's vectors hold zero
 template <typename T>
@@ -1160,16 +1172,23 @@ __device__ inline Blk4 blk4_of(const Geo &g, const Row1<float> (&R)[4], int k0) 
 }
 
 // y = M^-1 f (Thomas, four rows)
+// the reciprocal of the preconditioner solves: the hardware's (v_rcp_f32,
+// about 1 ulp) rather than a correctly rounded division.  M^-1 needs no
+// correct rounding, only the same arithmetic in every launch.
+__device__ inline float prec_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ inline void blk4_solve(const Blk4 &B, const float (&f)[4], float (&y)[4]) {
+  // one reciprocal per row: it scales both the row's factor and its
+  // right-hand side
   float c[4], z[4];
-  float m = B.d[0];
-  c[0] = B.up[0] / m;
-  z[0] = f[0] / m;
+  float im = prec_rcp(B.d[0]);
+  c[0] = B.up[0] * im;
+  z[0] = f[0] * im;
 #pragma unroll
   for (int u = 1; u < 4; ++u) {
-    m = B.d[u] - B.lo[u] * c[u - 1];
-    c[u] = B.up[u] / m;
-    z[u] = (f[u] - B.lo[u] * z[u - 1]) / m;
+    im = prec_rcp(B.d[u] - B.lo[u] * c[u - 1]);
+    c[u] = B.up[u] * im;
+    z[u] = (f[u] - B.lo[u] * z[u - 1]) * im;
   }
   y[3] = z[3];
 #pragma unroll
@@ -1271,11 +1290,11 @@ __device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y
   const int lane = threadIdx.x;
   const bool on = pos >= 0;
   // rows 0-2: Thomas with three right-hand sides (f, -lo0 e0, -up2 e2)
-  const float i0 = 1.f / L.d[0];
+  const float i0 = prec_rcp(L.d[0]);
   const float c0 = L.up[0] * i0;
-  const float i1 = 1.f / (L.d[1] - L.lo[1] * c0);
+  const float i1 = prec_rcp(L.d[1] - L.lo[1] * c0);
   const float c1 = L.up[1] * i1;
-  const float i2 = 1.f / (L.d[2] - L.lo[2] * c1);
+  const float i2 = prec_rcp(L.d[2] - L.lo[2] * c1);
   float z0 = f[0] * i0, z1 = (f[1] - L.lo[1] * z0) * i1, z2 = (f[2] - L.lo[2] * z1) * i2;
   float a0 = -L.lo[0] * i0, a1 = -L.lo[1] * a0 * i1, a2 = -L.lo[2] * a1 * i2;
   const float b2 = -L.up[2] * i2;
@@ -1313,13 +1332,13 @@ __device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y
       float k1 = 0.f, k2 = 0.f, am = 0.f, cm = 0.f, gm = 0.f, ap = 0.f, cp = 0.f, gp = 0.f;
       if (pos >= h) {
         am = pr[buf][0][lane - h];
-        k1 = ra / pr[buf][1][lane - h];
+        k1 = ra * prec_rcp(pr[buf][1][lane - h]);
         cm = pr[buf][2][lane - h];
         gm = pr[buf][3][lane - h];
       }
       if (pos + h < nq) {
         ap = pr[buf][0][lane + h];
-        k2 = rc / pr[buf][1][lane + h];
+        k2 = rc * prec_rcp(pr[buf][1][lane + h]);
         cp = pr[buf][2][lane + h];
         gp = pr[buf][3][lane + h];
       }
@@ -1329,7 +1348,7 @@ __device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y
       rc = -cp * k2;
     }
   }
-  const float Y = rg / rb;
+  const float Y = rg * prec_rcp(rb);
   yl[lane] = Y;
   __syncthreads();
   const float Ym = on && pos >= 1 ? yl[lane - 1] : 0.f;
@@ -1357,7 +1376,7 @@ __device__ inline void line_product(const Geo &g, const float *lv, const float *
       [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
           const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
         float qv[5], mr[4];
-        quot5(C, lv, k0, qv);
+        quot5_r(C, lv, g.NS, k0, qv);
         Row1<float> R[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1534,7 +1553,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     });
   }
   if (stop) return;
-  const float *lv = load_levels<float>(g);
+  float *lvw = load_levels<float>(g);
+  for (int k = threadIdx.x; k < g.NS; k += NSX_BLOCK) lvw[2 * g.NS + k] = 1.f / lvw[k];
+  __syncthreads();
+  const float *lv = lvw;
   if (mode == 2) {
     // the true residual of x + e into rt (T reads x and e at neighbouring
     // prisms and writes rt only at its own)
@@ -1592,7 +1614,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           Col<float> C;
           load_col(g, i, C);
           float qv[5];
-          quot5(C, lv, k0, qv);
+          quot5_r(C, lv, g.NS, k0, qv);
           Row1<float> R[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) R[u] = row_of(g, lv, C, qv, k0, u);
@@ -1638,7 +1660,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
               const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
             float qv[5], wo[4];
-            quot5(C, lv, k0, qv);
+            quot5_r(C, lv, g.NS, k0, qv);
             if (BLK) {
               Row1<float> R[4];
               float wv[4], mr[4];
@@ -1661,7 +1683,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
               for (int u = 0; u < 4; ++u) {
                 const Row1<float> R = row_of(g, lv, C, qv, k0, u);
                 const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-                wo[u] = R.in ? wv / R.A0 : 0.f;
+                wo[u] = R.in ? wv * prec_rcp(R.A0) : 0.f;
                 if (R.in) {
                   acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
                   acc[1] += (double)wv * (double)fc[u];
@@ -1681,7 +1703,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
                 const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
               float qv[5], so[4];
-              quot5(C, lv, k0, qv);
+              quot5_r(C, lv, g.NS, k0, qv);
               if (BLK) {
                 Row1<float> R[4];
                 float sv[4];
@@ -1696,7 +1718,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
                 for (int u = 0; u < 4; ++u) {
                   const Row1<float> R = row_of(g, lv, C, qv, k0, u);
                   const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-                  so[u] = R.in ? sv / R.A0 : 0.f;
+                  so[u] = R.in ? sv * prec_rcp(R.A0) : 0.f;
                 }
               }
               st4(sg + s0, so);
@@ -1761,7 +1783,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
           const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
         float qv[5], ro[4], wo[4], so[4], po[4], eo[4];
-        quot5(C, lv, k0, qv);
+        quot5_r(C, lv, g.NS, k0, qv);
         Row1<float> R[4];
         float wv[4], mr[4];
 #pragma unroll
@@ -1788,7 +1810,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         } else {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            wo[u] = R[u].in ? wv[u] / R[u].A0 : 0.f;
+            wo[u] = R[u].in ? wv[u] * prec_rcp(R[u].A0) : 0.f;
             mr[u] = R[u].A0 * fc[u];
           }
         }
@@ -1914,6 +1936,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_apply(Geo g, double *x, con
 
 // ------------------------------------------------------------ launchers ---
 static size_t level_lds(const Geo &g, size_t elem) { return 2 * (size_t)g.NS * elem; }
+// k_cgs: dsig, dsigv and 1/dsig in FP32
+static size_t cgs_lds(const Geo &g) { return 3 * (size_t)g.NS * sizeof(float); }
 
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
                      double *rhat, Partials part, SolverState *st, double eps, int max_iters) {
@@ -1979,7 +2003,7 @@ int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x,
   double *Pout = (j & 1) ? part.u : part.c;
   const double *Pin = j == 0 ? part.u : ((j & 1) ? part.c : part.u);
   const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs);
-  const size_t lds = level_lds(g, sizeof(float));
+  const size_t lds = cgs_lds(g);
   if (pc == 2)
     LAUNCH_TIMED(ev, 0, k_cgs<2>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
   else if (pc == 1)
@@ -2081,7 +2105,7 @@ int cgs_blocks_per_cu(const Geo &g) {
   int m = 1 << 30;
   for (const void *k : ks) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, NSX_BLOCK, level_lds(g, 4)) != hipSuccess || n < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, NSX_BLOCK, cgs_lds(g)) != hipSuccess || n < 1)
       return -1;
     m = n < m ? n : m;
   }
