@@ -547,6 +547,24 @@ __device__ inline T apply_row(const Geo &g, const Row1<T> &R, int k0, int u, con
   return R.in ? R.A0 * fs + R.a1 * f0 + R.a2 * f1 + R.a3 * f2 + R.AT * fu + R.AB * fd : T(0);
 }
 
+// apply_row of the FP32 solvers as a chain of fused multiply-adds (the FP64
+// form keeps the reference's separate products and sums)
+template <>
+__device__ inline float apply_row<float>(const Geo &g, const Row1<float> &R, int k0, int u, const float (&fc)[4],
+                                         float fm, float fp, float f0, float f1, float f2) {
+  const int k = k0 + u;
+  const float fs = fc[u];
+  const float fu = (k + 1 <= g.NZ - 2) ? (u < 3 ? fc[u < 3 ? u + 1 : 3] : fp) : fs;
+  const float fd = (k - 1 >= 1) ? (u > 0 ? fc[u > 0 ? u - 1 : 0] : fm) : fs;
+  float r = R.A0 * fs;
+  r = __builtin_fmaf(R.a1, f0, r);
+  r = __builtin_fmaf(R.a2, f1, r);
+  r = __builtin_fmaf(R.a3, f2, r);
+  r = __builtin_fmaf(R.AT, fu, r);
+  r = __builtin_fmaf(R.AB, fd, r);
+  return R.in ? r : 0.f;
+}
+
 // Windows of NSX_BLOCK consecutive quads, swept band by band.  Blocks are
 // dealt round-robin to the XCDs, so hardware block b runs on XCD b % 8.  XCD
 // x owns the x-th contiguous eighth of the windows, and its G/8 blocks sweep
@@ -1741,8 +1759,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     if (!first) ld4(sg_i + off, q);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      sg[u] = first ? w[u] : w[u] + bf * q[u];
-      o[u] = r[u] - af * sg[u];
+      sg[u] = first ? w[u] : __builtin_fmaf(bf, q[u], w[u]);
+      o[u] = __builtin_fmaf(-af, sg[u], r[u]);
     }
   };
   auto val4 = [&](size_t off, float(&o)[4]) {
@@ -1750,8 +1768,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     form4(off, r, sg, o);
   };
   auto val1 = [&](size_t q) {
-    const float sg = first ? om_i[q] : om_i[q] + bf * sg_i[q];
-    return rt_i[q] - af * sg;
+    const float sg = first ? om_i[q] : __builtin_fmaf(bf, sg_i[q], om_i[q]);
+    return __builtin_fmaf(-af, sg, rt_i[q]);
   };
   // p and e are read and written every other iteration: an iteration that
   // finds them current leaves both alone (p_i = u_i + beta_i p_{i-1} and
