@@ -1204,13 +1204,13 @@ __device__ inline void blk4_solve(const Blk4 &B, const float (&f)[4], float (&y)
   z[0] = f[0] * im;
 #pragma unroll
   for (int u = 1; u < 4; ++u) {
-    im = prec_rcp(B.d[u] - B.lo[u] * c[u - 1]);
+    im = prec_rcp(__builtin_fmaf(-B.lo[u], c[u - 1], B.d[u]));
     c[u] = B.up[u] * im;
-    z[u] = (f[u] - B.lo[u] * z[u - 1]) * im;
+    z[u] = __builtin_fmaf(-B.lo[u], z[u - 1], f[u]) * im;
   }
   y[3] = z[3];
 #pragma unroll
-  for (int u = 2; u >= 0; --u) y[u] = z[u] - c[u] * y[u + 1];
+  for (int u = 2; u >= 0; --u) y[u] = __builtin_fmaf(-c[u], y[u + 1], z[u]);
 }
 
 // y = M x
@@ -1218,8 +1218,8 @@ __device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     float t = B.d[u] * x[u];
-    if (u > 0) t = t + B.lo[u] * x[u - 1];
-    if (u < 3) t = t + B.up[u] * x[u + 1];
+    if (u > 0) t = __builtin_fmaf(B.lo[u], x[u - 1], t);
+    if (u < 3) t = __builtin_fmaf(B.up[u], x[u + 1], t);
     y[u] = t;
   }
 }
@@ -1837,9 +1837,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           const bool in = R[u].in;
           if (upd_p) {
             const float pl = p_lagged(ui[u], pi[u], bp_in, pfirst_in);
-            const float pn = ri[u] + bf * pl;
-            const float eb = pend_e_in ? ei[u] + ap_in * pl : ei[u];
-            eo[u] = in ? eb + af * pn : ei[u];
+            const float pn = __builtin_fmaf(bf, pl, ri[u]);
+            const float eb = pend_e_in ? __builtin_fmaf(ap_in, pl, ei[u]) : ei[u];
+            eo[u] = in ? __builtin_fmaf(af, pn, eb) : ei[u];
             po[u] = in ? pn : 0.f;
           }
           ro[u] = in ? fc[u] : 0.f;
