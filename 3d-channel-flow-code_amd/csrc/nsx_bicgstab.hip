@@ -176,11 +176,12 @@ __device__ inline void quot5(const Col<T> &C, const T *lv, int k0, T (&q)[5]) {
   }
 }
 
-// quot5 of the single-reduction FP32 CG: the LDS level table also holds
-// 1/dsig at lv[2NS..3NS), and q[u] = area * (1/dsig) is a product, not a
-// division (its FP32 operator rounds differently from area/dsig, in the last
-// bit; every launch of that CG forms its rows this way)
-__device__ inline void quot5_r(const Col<float> &C, const float *lv, int NS, int k0, float (&q)[5]) {
+// quot5 of the single-reduction CG: the LDS level table also holds 1/dsig
+// at lv[2NS..3NS), and q[u] = area * (1/dsig) is a product, not a division
+// (its operator rounds differently from area/dsig, in the last bit; every
+// launch of that CG forms its rows this way)
+template <typename T>
+__device__ inline void quot5_r(const Col<T> &C, const T *lv, int NS, int k0, T (&q)[5]) {
 #pragma unroll
   for (int u = 0; u < 5; ++u) {
     const int k = k0 - 1 + u;
@@ -305,70 +306,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init_state(SolverState *st, 
   if (threadIdx.x == 0) {
     reset_state(st, eps, max_iters);
     st->mp_pass = 0;
-    st->mp_done = 0;
     st->delta = delta;
-    st->upd = 0;
     st->ureq[0] = st->ureq[1] = 0;
   }
-}
-
-// ------------------------------------------------- reliable update (U) ----
-// Mixed precision with the Krylov space kept: the FP32 iteration runs on,
-// and after iteration k-1 every block sums C's partials and takes the same
-// decision (Sleijpen & van der Vorst's reliable updates, in the form of
-// Clark et al. 2010): when |r| has fallen below delta times the largest
-// |r| since the last update, or the recursive residual meets the stopping
-// rule, the true residual of the FP64 solution x plus the FP32 correction e
-// is formed in FP64, r = b - A (x + e), and replaces the recursive one
-// (stored in FP32), with its partial sums into P.u.  A1 of the same
-// iteration then folds e into x and tests the true residual; C restarts e
-// from zero.  x and e are only read here, so the stencil sees one state.
-__global__ __launch_bounds__(NSX_BLOCK) void k_rel_u(Geo g, const double *rhs, const double *x, const float *e,
-                                                      float *r, const float *rhat, Partials P, SolverState *st) {
-  if (st->done) return;
-  const int k_it = st->c_iter;
-  if (k_it == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->upd = 0;
-    return;
-  }
-  double t[4];
-  reduce_partials<4>(P.c, gridDim.x, t);
-  const double rN = sqrt(t[0]), r1 = t[1];
-  const double e1 = r1 / st->b1, e2 = rN / st->b2;
-  const double mrr = fmax(st->maxrr[(k_it - 1) & 1], rN);
-  const double r0N = st->r0N;
-  const bool conv = e1 < st->eps || e2 < st->eps;
-  const bool upd = conv || (rN < st->delta * mrr && r0N <= mrr);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->upd = upd ? 1 : 0;
-    st->maxrr[k_it & 1] = mrr;
-  }
-  if (!upd) return;
-  const double avec = st->avec;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  FOR_EACH_CELL(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    const double area = g.area[i];
-    const double b = rhs[s] - avec * area * g.dsig[k];
-    const Row A = ppe_row(g, i, k);
-    const int ee = i * 3;
-    const size_t n0 = (size_t)g.nbr[ee] * g.NS + k, n1 = (size_t)g.nbr[ee + 1] * g.NS + k,
-                 n2 = (size_t)g.nbr[ee + 2] * g.NS + k;
-    const double xs = x[s] + (double)e[s];
-    const double Ax = A.A0 * xs + A.a1 * (x[n0] + (double)e[n0]) + A.a2 * (x[n1] + (double)e[n1]) +
-                      A.a3 * (x[n2] + (double)e[n2]) + A.AT * (x[s + 1] + (double)e[s + 1]) +
-                      A.AB * (x[s - 1] + (double)e[s - 1]);
-    const double res = b - Ax;
-    const float ro = (float)res;
-    r[s] = (float)(res / A.A0);  // the solver keeps r / A0
-    acc[0] += res * res;
-    acc[1] += fabs(res);
-    acc[2] += (double)rhat[s] * (double)ro;
-    acc[3] += xs * (area * g.dsig[k]);
-  }
-  write_partials<4>(acc, P.u);
 }
 
 // ----------------------------------------------------------- iteration ----
@@ -392,60 +332,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rel_u(Geo g, const double *rhs, c
 
 // Closes iteration k-1 (norms and the stopping test of bicgstab.F90:544-575,
 // the breakdown guards of :479-490; at k = 0 the norms of :453-469) from C's
-// partial sums (P.init at k = 0, P.u after a reliable update).  Returns
+// partial sums (P.init at k = 0).  Returns
 // false when the solve stops; sets beta (0 at k = 0).
-__device__ inline bool begin_iteration(const Geo &g, Partials P, SolverState *st, double *xfold, const void *efold_v,
-                                       bool efold_f32, double &beta_out, bool &first_out) {
+__device__ inline bool begin_iteration(const Geo &g, Partials P, SolverState *st, double &beta_out,
+                                       bool &first_out) {
   const int k_it = st->c_iter;
   const bool first = k_it == 0;
-  const bool upd = xfold && !first && st->upd;
   double rho, e1, e2, xvol, b1, b2;
   int stop = 0, code = EXIT_CONVERGED;
-  if (upd) {
-    // reliable update: the true residual of x + e (k_rel_u); fold e into x
-    double t[4];
-    reduce_partials<4>(P.u, gridDim.x, t);
-    b1 = st->b1;
-    b2 = st->b2;
-    const double r2 = sqrt(t[0]), r1 = t[1];
-    e2 = r2 / b2;
-    e1 = r1 / b1;
-    rho = t[2];
-    xvol = t[3];
-    double enorm = (double)1.0e10f;
-    if (enorm > e1) enorm = e1;
-    if (enorm > e2) enorm = e2;
-    const double rhokm = st->a_rho[(k_it - 1) & 1];
-    const bool finite = isfinite(t[0]) && isfinite(t[1]) && isfinite(t[2]) && isfinite(t[3]);
-    if (!finite) {
-      // the FP32 correction has overflowed: leave x as it is
-      stop = 1;
-      code = EXIT_BREAKDOWN;
-    } else if (enorm < st->eps) {
-      stop = 1;
-      code = EXIT_CONVERGED;
-    } else if (k_it > st->max_iters) {
-      stop = 1;
-      code = EXIT_MAX;
-    } else if (fabs(rhokm) < TINY || fabs(st->c_omega) < TINY) {
-      stop = 1;
-      code = EXIT_BREAKDOWN;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->r0N = r2;
-      st->maxrr[k_it & 1] = r2;
-      st->mp_pass += 1;
-    }
-    if (finite) {
-      FOR_EACH_CELL(g, c) {
-        int i, k;
-        cell_of(g, c, i, k);
-        const size_t s = (size_t)i * g.NS + k;
-        const double e = efold_f32 ? (double)((const float *)efold_v)[s] : ((const double *)efold_v)[s];
-        xfold[s] = xfold[s] + e;
-      }
-    }
-  } else if (first) {
+  if (first) {
     double t[5];
     reduce_partials<5>(P.init, gridDim.x, t);
     b2 = sqrt(t[0]);
@@ -487,11 +382,6 @@ __device__ inline bool begin_iteration(const Geo &g, Partials P, SolverState *st
       stop = 1;
       code = EXIT_MAX;
     } else if (fabs(rhokm) < TINY || fabs(st->c_omega) < TINY) {
-      stop = 1;
-      code = EXIT_BREAKDOWN;
-    } else if (!isfinite(enorm) || !isfinite(rho) || !isfinite(st->c_omega)) {
-      // no counterpart in the reference (whose FP64 iterates do not
-      // overflow): an overflowed FP32 iteration stops as a breakdown
       stop = 1;
       code = EXIT_BREAKDOWN;
     }
@@ -689,11 +579,11 @@ __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T 
 // v = A y (get_Apk), vt = v/A0, partial sums of <rhat, v> (:509).
 template <typename T>
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const T *q, const T *rhat, T *vt, Partials P,
-                                                       SolverState *st, double *xfold, const T *efold) {
+                                                       SolverState *st) {
   if (st->done) return;
   double beta_d = 0.0;
   bool first = false;
-  if (!begin_iteration(g, P, st, xfold, efold, sizeof(T) == 4, beta_d, first)) return;
+  if (!begin_iteration(g, P, st, beta_d, first)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->a_beta = beta_d;
   const T beta = (T)beta_d;
   const T *lv = load_levels<T>(g);
@@ -762,8 +652,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const 
 // omega = <t,s>/<t,t>; y = rt + beta q (A's y, formed again), z = rt -
 // alpha vt, s = A0 z, r = s - omega t, x += alpha y + omega z (update_xkrk
 // :1047-1111), rt = r/A0, q = y - omega vt; partial |r|^2, |r|_1, <rhat, r>,
-// sum x Vol.  After a reliable update the FP32 correction x starts again
-// from zero (A has folded it).
+// sum x Vol.
 template <typename T>
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, const T *rhat, const T *vt,
                                                        const T *tv, Partials P, SolverState *st) {
@@ -778,7 +667,6 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
   const T alt = (T)al, omt = (T)om, bet = (T)st->a_beta;
   const bool first = st->a_iter == 0;
   const T *lv = load_levels<T>(g);
-  const bool fresh = st->upd != 0;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for_each_quad(g, [&](int i, int k0, size_t s0) {
     T xx[4], rr[4], vv[4], yy[4], tt[4], rh[4], qq[4], qv[5];
@@ -787,11 +675,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
     // wave's span, so the second half then hits the lines the first one
     // brought in; interleaving the six vectors' halves re-requests each line
     // later and costs 15 % of the kernel in FP64.
-    if (fresh) {
-      xx[0] = xx[1] = xx[2] = xx[3] = T(0);
-    } else {
-      ld4(x + s0, xx);
-    }
+    ld4(x + s0, xx);
     __builtin_amdgcn_sched_barrier(0);
     ld4(rt + s0, rr);
     __builtin_amdgcn_sched_barrier(0);
@@ -890,7 +774,8 @@ __device__ inline void true_residual(const Geo &g, const double *rhs, const doub
   res = b - Ax;
 }
 
-__global__ __launch_bounds__(NSX_BLOCK) void k_cg_start(Geo g, const double *rhs, const double *x, float *rt,
+template <typename T>
+__global__ __launch_bounds__(NSX_BLOCK) void k_cg_start(Geo g, const double *rhs, const double *x, T *rt,
                                                          Partials P, SolverState *st) {
   // the norms of b and r0 = b - A x0 (k_resid's partials) and the
   // reference's no-solve test (bicgstab.F90:453-469)
@@ -921,9 +806,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_start(Geo g, const double *rhs
     int i, k;
     cell_of(g, c, i, k);
     double res, xs;
-    true_residual(g, rhs, x, avec, i, k, [](size_t, bool) { return 0.f; }, res, xs);
+    true_residual(g, rhs, x, avec, i, k, [](size_t, bool) { return T(0); }, res, xs);
     const Row A = ppe_row(g, i, k);
-    const float rtf = (float)(res / A.A0);
+    const T rtf = (T)(res / A.A0);
     rt[(size_t)i * g.NS + k] = rtf;
     acc[0] += res * (double)rtf;
     acc[1] += res * res;
@@ -950,8 +835,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_w(Geo g, const float *rt, cons
     reduce_partials<4>(P.u, (uint32_t)g.grid, t);  // the start and X run on the solver grid
     rho = t[0];
     const double r2 = sqrt(t[1]), e1 = t[2] / st->b1, e2 = r2 / st->b2;
-    // x + e is the solution the true residual belongs to (e = 0 at k = 0)
-    if (alt) {
+    // x + e is the solution the true residual belongs to (e = 0 at k = 0);
+    // an overflowed correction is not folded, and the FP64 solve finishes
+    // from x
+    if (alt && isfinite(t[0]) && isfinite(t[1])) {
       FOR_EACH_CELL(g, c) {
         int i, k;
         cell_of(g, c, i, k);
@@ -1165,61 +1052,71 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cg_x(Geo g, float *e, float *rt, 
 // dropped.  Rows outside the interior are the identity, so the solves below
 // give them zero.  M is symmetric (AB of a layer is AT of the layer below,
 // the same quotient) and negative definite like A's diagonal blocks.
+template <typename T>
 struct Blk4 {
-  float d[4], lo[4], up[4];
+  T d[4], lo[4], up[4];
 };
 
-__device__ inline Blk4 blk4_of(const Geo &g, const Row1<float> (&R)[4], int k0) {
-  Blk4 B;
+template <typename T>
+__device__ inline Blk4<T> blk4_of(const Geo &g, const Row1<T> (&R)[4], int k0) {
+  Blk4<T> B;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int k = k0 + u;
     if (!R[u].in) {
-      B.d[u] = 1.f;
-      B.lo[u] = B.up[u] = 0.f;
+      B.d[u] = T(1);
+      B.lo[u] = B.up[u] = T(0);
       continue;
     }
-    float d = R[u].A0;
+    T d = R[u].A0;
     if (k + 1 > g.NZ - 2) d = d + R[u].AT;
     if (k - 1 < 1) d = d + R[u].AB;
     B.d[u] = d;
-    B.lo[u] = (u > 0 && R[u - 1].in) ? R[u].AB : 0.f;
-    B.up[u] = (u < 3 && R[u + 1].in) ? R[u].AT : 0.f;
+    B.lo[u] = (u > 0 && R[u - 1].in) ? R[u].AB : T(0);
+    B.up[u] = (u < 3 && R[u + 1].in) ? R[u].AT : T(0);
   }
   return B;
 }
 
 // y = M^-1 f (Thomas, four rows)
-// the reciprocal of the preconditioner solves: the hardware's (v_rcp_f32,
-// about 1 ulp) rather than a correctly rounded division.  M^-1 needs no
-// correct rounding, only the same arithmetic in every launch.
+// the reciprocal of the preconditioner solves: in FP32 the hardware's
+// (v_rcp_f32, about 1 ulp) rather than a correctly rounded division, in
+// FP64 the division.  M^-1 needs no correct rounding, only the same
+// arithmetic in every launch.
 __device__ inline float prec_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double prec_rcp(double x) { return 1.0 / x; }
 
-__device__ inline void blk4_solve(const Blk4 &B, const float (&f)[4], float (&y)[4]) {
+// a b + c in one rounding
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T>
+__device__ inline void blk4_solve(const Blk4<T> &B, const T (&f)[4], T (&y)[4]) {
   // one reciprocal per row: it scales both the row's factor and its
   // right-hand side
-  float c[4], z[4];
-  float im = prec_rcp(B.d[0]);
+  T c[4], z[4];
+  T im = prec_rcp(B.d[0]);
   c[0] = B.up[0] * im;
   z[0] = f[0] * im;
 #pragma unroll
   for (int u = 1; u < 4; ++u) {
-    im = prec_rcp(__builtin_fmaf(-B.lo[u], c[u - 1], B.d[u]));
+    im = prec_rcp(fma_t(-B.lo[u], c[u - 1], B.d[u]));
     c[u] = B.up[u] * im;
-    z[u] = __builtin_fmaf(-B.lo[u], z[u - 1], f[u]) * im;
+    z[u] = fma_t(-B.lo[u], z[u - 1], f[u]) * im;
   }
   y[3] = z[3];
 #pragma unroll
-  for (int u = 2; u >= 0; --u) y[u] = __builtin_fmaf(-c[u], y[u + 1], z[u]);
+  for (int u = 2; u >= 0; --u) y[u] = fma_t(-c[u], y[u + 1], z[u]);
 }
 
 // y = M x
-__device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4]) {
+template <typename T>
+__device__ inline void blk4_mul(const Blk4<T> &B, const T (&x)[4], T (&y)[4]) {
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    float t = B.d[u] * x[u];
-    if (u > 0) t = __builtin_fmaf(B.lo[u], x[u - 1], t);
-    if (u < 3) t = __builtin_fmaf(B.up[u], x[u + 1], t);
+    T t = B.d[u] * x[u];
+    if (u > 0) t = fma_t(B.lo[u], x[u - 1], t);
+    if (u < 3) t = fma_t(B.up[u], x[u + 1], t);
     y[u] = t;
   }
 }
@@ -1227,13 +1124,15 @@ __device__ inline void blk4_mul(const Blk4 &B, const float (&x)[4], float (&y)[4
 // e with the pending alpha p.  The true-residual launch forms it at every
 // slot and the fold after it forms it again: one fused multiply-add in both,
 // so x takes exactly the values whose residual was tested (an unfused sum in
-// one of them moves x by an FP32 rounding of e, which A can amplify past the
+// one of them moves x by a rounding of e, which A can amplify past the
 // stopping rule).
-__device__ inline float e_pending(float e, float p, float a) { return __builtin_fmaf(a, p, e); }
+template <typename T>
+__device__ inline T e_pending(T e, T p, T a) { return fma_t(a, p, e); }
 
 // p_i = u_i + beta_i p_{i-1} of an iteration that stored no p (u_i alone
 // when it was the first): the same expression wherever it is formed
-__device__ inline float p_lagged(float u, float p, float b, bool first) { return first ? u : u + b * p; }
+template <typename T>
+__device__ inline T p_lagged(T u, T p, T b, bool first) { return first ? u : u + b * p; }
 
 // The line preconditioner of the single-reduction CG: M is A restricted to
 // the interior layers of one column (the vertical couplings, and the diagonal
@@ -1435,8 +1334,8 @@ __device__ inline void line_product(const Geo &g, const float *lv, const float *
 // lines (above).  With blocks or lines the solve starts with a true-residual
 // launch, which forms u = M^-1 r0; with lines every phase runs on windows of
 // whole columns.
-template <int PC>
-__global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, const double *rhs, double *x,
+template <typename T, int PC>
+__global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, const double *rhs, double *x,
                                                     const double *Pin, uint32_t Gin, double *Pout, SolverState *st) {
   constexpr bool BLK = PC == 1, LINE = PC == 2, PRE = PC != 0;
   CgsCarry c;
@@ -1455,7 +1354,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   }
   const int prev = c.mode;
   int mode = 0, stop = 0, code = EXIT_CONVERGED;
-  bool fold = false;
+  bool fold = false, fold_x = true;
   double gam = 0.0, del = 0.0;
   if (prev == -1) {
     double t[4];
@@ -1468,6 +1367,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     const double rN = sqrt(t[0]), e1 = t[1] / st->b1, e2 = rN / st->b2;
     fold = true;
     if (!isfinite(t[0]) || !isfinite(t[1])) {
+      // the correction has overflowed: x keeps its last value (the fold
+      // below only clears e) and the FP64 solve finishes from it
+      fold_x = false;
       stop = 1;
       code = EXIT_BREAKDOWN;
     } else if (e1 < st->eps || e2 < st->eps) {
@@ -1525,8 +1427,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   // the other set, which no launch rewrites before the next iteration) and
   // the stored p_{i-1}, and e lacks alpha_i p_i unless a fold took it into x
   const bool pend_in = c.pend != 0, pend_e_in = c.pend_e != 0, pfirst_in = c.first != 0;
-  const float ap_in = (float)c.alpha, bp_in = (float)c.beta;
-  const float *rt_lag = v.rt[in ^ 1];
+  const T ap_in = (T)c.alpha, bp_in = (T)c.beta;
+  const T *rt_lag = v.rt[in ^ 1];
   if (mode == 1 && !stop) {
     c.gamma = gam;
     c.alpha = al;
@@ -1554,38 +1456,39 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     // x += e (+ the pending alpha p), e = 0: the correction since the last
     // update joins x (interior layers: e holds zero elsewhere)
     for_each_quad(g, [&](int, int k0, size_t s0) {
-      float ee[4];
+      T ee[4];
       ld4(v.e + s0, ee);
       if (pend_e_in) {
-        float ua[4], pp[4] = {0.f, 0.f, 0.f, 0.f};
+        T ua[4], pp[4] = {T(0), T(0), T(0), T(0)};
         ld4(rt_lag + s0, ua);
         if (!pfirst_in) ld4(v.p + s0, pp);
 #pragma unroll
         for (int u = 0; u < 4; ++u) ee[u] = e_pending(ee[u], p_lagged(ua[u], pp[u], bp_in, pfirst_in), ap_in);
       }
+      if (fold_x)
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (k0 + u >= 1 && k0 + u <= g.NZ - 2) x[s0 + u] = x[s0 + u] + (double)ee[u];
-      const float z[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < 4; ++u)
+          if (k0 + u >= 1 && k0 + u <= g.NZ - 2) x[s0 + u] = x[s0 + u] + (double)ee[u];
+      const T z[4] = {T(0), T(0), T(0), T(0)};
       st4(v.e + s0, z);
     });
   }
   if (stop) return;
-  float *lvw = load_levels<float>(g);
-  for (int k = threadIdx.x; k < g.NS; k += NSX_BLOCK) lvw[2 * g.NS + k] = 1.f / lvw[k];
+  T *lvw = load_levels<T>(g);
+  for (int k = threadIdx.x; k < g.NS; k += NSX_BLOCK) lvw[2 * g.NS + k] = T(1) / lvw[k];
   __syncthreads();
-  const float *lv = lvw;
+  const T *lv = lvw;
   if (mode == 2) {
     // the true residual of x + e into rt (T reads x and e at neighbouring
     // prisms and writes rt only at its own)
     const double avec = st->avec;
-    const float *e = v.e, *p = v.p;
-    float *rt = v.rt[in];
+    const T *e = v.e, *p = v.p;
+    T *rt = v.rt[in];
     double acc[3] = {0.0, 0.0, 0.0};
     // e at a slot, with the alpha p the last iteration left pending (the fold
     // of the next launch forms the same sum)
     auto ecur = [&](size_t s, bool) {
-      return pend_e_in ? e_pending(e[s], p_lagged(rt_lag[s], pfirst_in ? 0.f : p[s], bp_in, pfirst_in), ap_in) : e[s];
+      return pend_e_in ? e_pending(e[s], p_lagged(rt_lag[s], pfirst_in ? T(0) : p[s], bp_in, pfirst_in), ap_in) : e[s];
     };
     if (PRE) {
       // window by window: the residuals cell by cell (consecutive threads on
@@ -1593,7 +1496,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       // gathers coalesce) into LDS, then u = M^-1 r over each quad's block or
       // each column's line.  A window of W quads from quad `base` covers the
       // slots 4 base .. 4 (base + W) - 1.
-      __shared__ __align__(16) float rs[NSX_BLOCK][4];
+      __shared__ __align__(16) T rs[NSX_BLOCK][4];
       const uint32_t W = LINE ? line_window(g) : NSX_BLOCK, nwin = (g.nquad + W - 1) / W;
       const int lpos = LINE ? line_pos(g, W) : -1;
       uint32_t w0, wend, step;
@@ -1601,9 +1504,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       for (uint32_t wi = w0; wi < wend; wi += step) {
         const uint32_t base = wi * W;
         const uint32_t lim = (base + W < g.nquad ? base + W : g.nquad) - base;
-        float *flat = &rs[0][0];
+        T *flat = &rs[0][0];
         for (uint32_t m = threadIdx.x; m < 4 * W; m += NSX_BLOCK) {
-          float f = 0.f;
+          T f = T(0);
           if (m < 4 * lim) {
             const uint32_t q = base + (m >> 2);
             const int i = (int)fdiv(q, g.divNQ);
@@ -1611,7 +1514,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             if (k >= 1 && k <= g.NZ - 2) {
               double res, xs;
               true_residual(g, rhs, x, avec, i, k, ecur, res, xs);
-              f = (float)res;
+              f = (T)res;
               acc[0] += res * res;
               acc[1] += fabs(res);
               acc[2] += xs * (g.area[i] * g.dsig[k]);
@@ -1621,7 +1524,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         }
         __syncthreads();
         const bool act = threadIdx.x < lim;
-        float f[4] = {0.f, 0.f, 0.f, 0.f}, u4[4] = {0.f, 0.f, 0.f, 0.f};
+        T f[4] = {T(0), T(0), T(0), T(0)}, u4[4] = {T(0), T(0), T(0), T(0)};
         Line4 L = line4_identity();
         if (act) {
           const uint32_t t = base + threadIdx.x;
@@ -1629,19 +1532,19 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
           const int k0 = 4 * (int)(t - (uint32_t)i * (uint32_t)g.NQ);
 #pragma unroll
           for (int u = 0; u < 4; ++u) f[u] = rs[threadIdx.x][u];
-          Col<float> C;
+          Col<T> C;
           load_col(g, i, C);
-          float qv[5];
+          T qv[5];
           quot5_r(C, lv, g.NS, k0, qv);
-          Row1<float> R[4];
+          Row1<T> R[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) R[u] = row_of(g, lv, C, qv, k0, u);
-          if (LINE)
+          if constexpr (LINE)
             L = line4_of(g, R, k0);
           else
             blk4_solve(blk4_of(g, R, k0), f, u4);
         }
-        if (LINE) line_solve(L, f, u4, act ? lpos : -1, g.NQ);
+        if constexpr (LINE) line_solve(L, f, u4, act ? lpos : -1, g.NQ);
         if (act) st4(rt + (size_t)4 * (base + threadIdx.x), u4);
         __syncthreads();  // rs is rewritten by the next window
       }
@@ -1654,7 +1557,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
       double res, xs;
       true_residual(g, rhs, x, avec, i, k, ecur, res, xs);
       const Row A = ppe_row(g, i, k);
-      rt[(size_t)i * g.NS + k] = (float)(res / A.A0);
+      rt[(size_t)i * g.NS + k] = (T)(res / A.A0);
       acc[0] += res * res;
       acc[1] += fabs(res);
       acc[2] += xs * (g.area[i] * g.dsig[k]);
@@ -1664,30 +1567,30 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   }
   if (mode == 0) {
     // om = A rt / A0, gamma = <A0 rt, rt>, delta = <A rt, rt>
-    const float *rt = v.rt[in];
-    float *om = v.om[in];
+    const T *rt = v.rt[in];
+    T *om = v.om[in];
     double acc[2] = {0.0, 0.0};
-    if (LINE) {
-      line_product(g, lv, rt, rt, 0.f, true, om, acc);
+    if constexpr (LINE) {
+      line_product(g, lv, rt, rt, T(0), true, om, acc);
       if (fold && c.iter > 0)
-        line_product(g, lv, pend_in ? rt_lag : (const float *)v.p, (const float *)v.p, pend_in ? bp_in : 0.f,
+        line_product(g, lv, pend_in ? rt_lag : (const T *)v.p, (const T *)v.p, pend_in ? bp_in : T(0),
                      pend_in ? pfirst_in : true, v.sg[in], (double *)nullptr);
     } else {
       for_each_quad_gathered(
-          g, rt, rt, 0.f, true, [&](size_t) {},
-          [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
-              const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
-            float qv[5], wo[4];
+          g, rt, rt, T(0), true, [&](size_t) {},
+          [&](int, int k0, size_t s0, const Col<T> &C, const T(&fc)[4], T fm, T fp,
+              const T(&f0)[4], const T(&f1)[4], const T(&f2)[4]) {
+            T qv[5], wo[4];
             quot5_r(C, lv, g.NS, k0, qv);
-            if (BLK) {
-              Row1<float> R[4];
-              float wv[4], mr[4];
+            if constexpr (BLK) {
+              Row1<T> R[4];
+              T wv[4], mr[4];
   #pragma unroll
               for (int u = 0; u < 4; ++u) {
                 R[u] = row_of(g, lv, C, qv, k0, u);
                 wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
               }
-              const Blk4 B = blk4_of(g, R, k0);
+              const auto B = blk4_of(g, R, k0);
               blk4_solve(B, wv, wo);
               blk4_mul(B, fc, mr);
   #pragma unroll
@@ -1699,9 +1602,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
             } else {
   #pragma unroll
               for (int u = 0; u < 4; ++u) {
-                const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-                const float wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-                wo[u] = R.in ? wv * prec_rcp(R.A0) : 0.f;
+                const Row1<T> R = row_of(g, lv, C, qv, k0, u);
+                const T wv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                wo[u] = R.in ? wv * prec_rcp(R.A0) : T(0);
                 if (R.in) {
                   acc[0] += (double)R.A0 * (double)fc[u] * (double)fc[u];
                   acc[1] += (double)wv * (double)fc[u];
@@ -1714,17 +1617,17 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         // after a true residual, s = A p afresh: its recurrence drifts in FP32
         // from the product it stands for, and with it the recursive residual
         // (p formed from u and the stored p when it lags)
-        float *sg = v.sg[in];
+        T *sg = v.sg[in];
         for_each_quad_gathered(
-            g, pend_in ? rt_lag : (const float *)v.p, (const float *)v.p, pend_in ? bp_in : 0.f,
+            g, pend_in ? rt_lag : (const T *)v.p, (const T *)v.p, pend_in ? bp_in : T(0),
             pend_in ? pfirst_in : true, [&](size_t) {},
-            [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
-                const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
-              float qv[5], so[4];
+            [&](int, int k0, size_t s0, const Col<T> &C, const T(&fc)[4], T fm, T fp,
+                const T(&f0)[4], const T(&f1)[4], const T(&f2)[4]) {
+              T qv[5], so[4];
               quot5_r(C, lv, g.NS, k0, qv);
               if (BLK) {
-                Row1<float> R[4];
-                float sv[4];
+                Row1<T> R[4];
+                T sv[4];
   #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                   R[u] = row_of(g, lv, C, qv, k0, u);
@@ -1734,9 +1637,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
               } else {
   #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                  const Row1<float> R = row_of(g, lv, C, qv, k0, u);
-                  const float sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
-                  so[u] = R.in ? sv * prec_rcp(R.A0) : 0.f;
+                  const Row1<T> R = row_of(g, lv, C, qv, k0, u);
+                  const T sv = apply_row(g, R, k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
+                  so[u] = R.in ? sv * prec_rcp(R.A0) : T(0);
                 }
               }
               st4(sg + s0, so);
@@ -1747,29 +1650,29 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
     return;
   }
   // mode N: iteration i with alpha_i, beta_i
-  const float af = (float)al, bf = (float)be;
-  const float *rt_i = v.rt[in], *om_i = v.om[in], *sg_i = v.sg[in];
-  float *rt_o = v.rt[in ^ 1], *om_o = v.om[in ^ 1], *sg_o = v.sg[in ^ 1];
+  const T af = (T)al, bf = (T)be;
+  const T *rt_i = v.rt[in], *om_i = v.om[in], *sg_i = v.sg[in];
+  T *rt_o = v.rt[in ^ 1], *om_o = v.om[in ^ 1], *sg_o = v.sg[in ^ 1];
   // u_{i+1} at a slot, from rt, om, sg there: the same expressions wherever
   // it is formed
-  auto form4 = [&](size_t off, float(&r)[4], float(&sg)[4], float(&o)[4]) {
-    float w[4], q[4] = {0.f, 0.f, 0.f, 0.f};
+  auto form4 = [&](size_t off, T(&r)[4], T(&sg)[4], T(&o)[4]) {
+    T w[4], q[4] = {T(0), T(0), T(0), T(0)};
     ld4(rt_i + off, r);
     ld4(om_i + off, w);
     if (!first) ld4(sg_i + off, q);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      sg[u] = first ? w[u] : __builtin_fmaf(bf, q[u], w[u]);
-      o[u] = __builtin_fmaf(-af, sg[u], r[u]);
+      sg[u] = first ? w[u] : fma_t(bf, q[u], w[u]);
+      o[u] = fma_t(-af, sg[u], r[u]);
     }
   };
-  auto val4 = [&](size_t off, float(&o)[4]) {
-    float r[4], sg[4];
+  auto val4 = [&](size_t off, T(&o)[4]) {
+    T r[4], sg[4];
     form4(off, r, sg, o);
   };
   auto val1 = [&](size_t q) {
-    const float sg = first ? om_i[q] : __builtin_fmaf(bf, sg_i[q], om_i[q]);
-    return __builtin_fmaf(-af, sg, rt_i[q]);
+    const T sg = first ? om_i[q] : fma_t(bf, sg_i[q], om_i[q]);
+    return fma_t(-af, sg, rt_i[q]);
   };
   // p and e are read and written every other iteration: an iteration that
   // finds them current leaves both alone (p_i = u_i + beta_i p_{i-1} and
@@ -1779,8 +1682,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   // a fold took the first into x).  The first iteration never finds them
   // lagging.
   const bool upd_p = pend_in;
-  float ri[4], si[4], pi[4] = {0.f, 0.f, 0.f, 0.f}, ei[4], ui[4];
-  auto own4 = [&](size_t off, float(&o)[4]) { form4(off, ri, si, o); };
+  T ri[4], si[4], pi[4] = {T(0), T(0), T(0), T(0)}, ei[4], ui[4];
+  auto own4 = [&](size_t off, T(&o)[4]) { form4(off, ri, si, o); };
   auto pre = [&](size_t off) {
     if (upd_p) {
       ld4(rt_lag + off, ui);  // own slots of rt_o, read before this thread stores them
@@ -1791,44 +1694,44 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   // with lines, om = M^-1 w waits for the window's columns (post)
   Line4 Ln = line4_identity();
-  float wvn[4] = {0.f, 0.f, 0.f, 0.f};
+  T wvn[4] = {T(0), T(0), T(0), T(0)};
   int k0n = 0;
   size_t s0n = 0;
   const uint32_t W = LINE ? line_window(g) : NSX_BLOCK;
   const int lpos = LINE ? line_pos(g, W) : -1;
-  for_each_quad_formed_w<float>(
+  for_each_quad_formed_w<T>(
       g, W, own4, val4, val1, pre,
-      [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
-          const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
-        float qv[5], ro[4], wo[4], so[4], po[4], eo[4];
+      [&](int, int k0, size_t s0, const Col<T> &C, const T(&fc)[4], T fm, T fp,
+          const T(&f0)[4], const T(&f1)[4], const T(&f2)[4]) {
+        T qv[5], ro[4], wo[4], so[4], po[4], eo[4];
         quot5_r(C, lv, g.NS, k0, qv);
-        Row1<float> R[4];
-        float wv[4], mr[4];
+        Row1<T> R[4];
+        T wv[4], mr[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           R[u] = row_of(g, lv, C, qv, k0, u);
           wv[u] = apply_row(g, R[u], k0, u, fc, fm, fp, f0[u], f1[u], f2[u]);
         }
-        if (LINE) {
+        if constexpr (LINE) {
           // r = M u for gamma and the norms
           Ln = line4_of(g, R, k0);
           line4_mul(Ln, fc, fm, fp, mr);
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             wvn[u] = wv[u];
-            wo[u] = 0.f;
+            wo[u] = T(0);
           }
           k0n = k0;
           s0n = s0;
-        } else if (BLK) {
+        } else if constexpr (BLK) {
           // om = M^-1 w, and r = M u for gamma and the norms
-          const Blk4 B = blk4_of(g, R, k0);
+          const auto B = blk4_of(g, R, k0);
           blk4_solve(B, wv, wo);
           blk4_mul(B, fc, mr);
         } else {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            wo[u] = R[u].in ? wv[u] * prec_rcp(R[u].A0) : 0.f;
+            wo[u] = R[u].in ? wv[u] * prec_rcp(R[u].A0) : T(0);
             mr[u] = R[u].A0 * fc[u];
           }
         }
@@ -1836,15 +1739,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         for (int u = 0; u < 4; ++u) {
           const bool in = R[u].in;
           if (upd_p) {
-            const float pl = p_lagged(ui[u], pi[u], bp_in, pfirst_in);
-            const float pn = __builtin_fmaf(bf, pl, ri[u]);
-            const float eb = pend_e_in ? __builtin_fmaf(ap_in, pl, ei[u]) : ei[u];
-            eo[u] = in ? __builtin_fmaf(af, pn, eb) : ei[u];
-            po[u] = in ? pn : 0.f;
+            const T pl = p_lagged(ui[u], pi[u], bp_in, pfirst_in);
+            const T pn = fma_t(bf, pl, ri[u]);
+            const T eb = pend_e_in ? fma_t(ap_in, pl, ei[u]) : ei[u];
+            eo[u] = in ? fma_t(af, pn, eb) : ei[u];
+            po[u] = in ? pn : T(0);
           }
-          ro[u] = in ? fc[u] : 0.f;
-          so[u] = in ? si[u] : 0.f;
-          wo[u] = in ? wo[u] : 0.f;
+          ro[u] = in ? fc[u] : T(0);
+          so[u] = in ? si[u] : T(0);
+          wo[u] = in ? wo[u] : T(0);
           if (in) {
             const double rd = PRE ? (double)mr[u] : (double)R[u].A0 * (double)fc[u];
             acc[0] += rd * (double)fc[u];
@@ -1862,67 +1765,23 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecs v, cons
         st4(sg_o + s0, so);
       },
       [&](bool act) {
-        if (!LINE) return;
-        float wo[4];
-        if (!act) {
-          Ln = line4_identity();
+        if constexpr (LINE) {
+          T wo[4];
+          if (!act) {
+            Ln = line4_identity();
 #pragma unroll
-          for (int u = 0; u < 4; ++u) wvn[u] = 0.f;
-        }
-        line_solve(Ln, wvn, wo, act ? lpos : -1, g.NQ);
-        if (act) {
+            for (int u = 0; u < 4; ++u) wvn[u] = T(0);
+          }
+          line_solve(Ln, wvn, wo, act ? lpos : -1, g.NQ);
+          if (act) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (k0n + u < 1 || k0n + u > g.NZ - 2) wo[u] = 0.f;
-          st4(om_o + s0n, wo);
+            for (int u = 0; u < 4; ++u)
+              if (k0n + u < 1 || k0n + u > g.NZ - 2) wo[u] = T(0);
+            st4(om_o + s0n, wo);
+          }
         }
       });
   write_partials<4>(acc, Pout);
-}
-
-// ---------------------------------------------------------- refinement ----
-// After a refinement residual: the outer norms (true FP64 residual of the
-// current x against the mean-removed rhs, the reference's stopping rule
-// min(|r|_1/|b|_1, |r|_2/|b|_2) < eps), the outer stop, and the reset of
-// the inner FP32 solve.  One block.
-__global__ __launch_bounds__(NSX_BLOCK) void k_mp_check(const double *outer, uint32_t G, SolverState *st, double eps,
-                                                         double eps_inner, int max_iters, int max_passes) {
-  double t[5];
-  reduce_partials<5>(outer, G, t);
-  if (threadIdx.x != 0) return;
-  const int pass = st->mp_pass;
-  double b2 = sqrt(t[0]), b1 = t[2];
-  if (pass == 0) {
-    if (b2 == 0) b2 = 1.0;
-    if (b1 == 0) b1 = 1.0;
-    st->ob1 = b1;
-    st->ob2 = b2;
-  } else {
-    b1 = st->ob1;
-    b2 = st->ob2;
-  }
-  const double e2 = sqrt(t[1]) / b2, e1 = t[3] / b1;
-  st->oe1 = e1;
-  st->oe2 = e2;
-  st->xvol = t[4];
-  const bool conv = e1 < eps || e2 < eps;
-  reset_state(st, eps_inner, max_iters);
-  if (conv || pass >= max_passes) {
-    st->mp_done = 1;
-    st->done = 1;  // the queued inner kernels return at once
-    st->exit_code = conv ? (pass == 0 ? EXIT_NO_SOLVE : EXIT_CONVERGED) : EXIT_MAX;
-  }
-  st->mp_pass = pass + 1;
-}
-
-// x += e over the interior (the FP32 correction of one refinement pass)
-__global__ __launch_bounds__(NSX_BLOCK) void k_mp_update(Geo g, double *x, const float *e) {
-  FOR_EACH_CELL(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    x[s] = x[s] + (double)e[s];
-  }
 }
 
 // ---------------------------------------------------------- Glob_Norm -----
@@ -1954,8 +1813,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_gnorm_apply(Geo g, double *x, con
 
 // ------------------------------------------------------------ launchers ---
 static size_t level_lds(const Geo &g, size_t elem) { return 2 * (size_t)g.NS * elem; }
-// k_cgs: dsig, dsigv and 1/dsig in FP32
-static size_t cgs_lds(const Geo &g) { return 3 * (size_t)g.NS * sizeof(float); }
+// k_cgs: dsig, dsigv and 1/dsig in the vectors' precision
+static size_t cgs_lds(const Geo &g, size_t elem = sizeof(float)) { return 3 * (size_t)g.NS * elem; }
 
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,
                      double *rhat, Partials part, SolverState *st, double eps, int max_iters) {
@@ -1971,8 +1830,7 @@ static int iter_impl(const Geo &g, hipStream_t s, T *x, T *rt, const T *rhat, T 
                      SolverState *st, hipEvent_t *ev) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
   const size_t lds = level_lds(g, sizeof(T));
-  LAUNCH_TIMED(ev, 0, k_bicg_a<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)q, rhat, vt, part, st,
-               (double *)nullptr, (const T *)nullptr);
+  LAUNCH_TIMED(ev, 0, k_bicg_a<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)q, rhat, vt, part, st);
   LAUNCH_TIMED(ev, 1, k_bicg_b<T>, grid, blk, lds, s, g, (const T *)rt, (const T *)vt, tv, part, st);
   LAUNCH_TIMED(ev, 2, k_bicg_c<T>, grid, blk, lds, s, g, x, rt, q, rhat, (const T *)vt, (const T *)tv, part, st);
   return (int)hipGetLastError();
@@ -1983,11 +1841,6 @@ int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const d
   return iter_impl<double>(g, s, x, rt, rhat, q, vt, tv, part, st, ev);
 }
 
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *vt,
-                       float *tv, Partials part, SolverState *st, hipEvent_t *ev) {
-  return iter_impl<float>(g, s, x, rt, rhat, q, vt, tv, part, st, ev);
-}
-
 int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *rt,
                    float *scratch, float *e, Partials part, SolverState *st, double eps, double delta, int max_iters) {
   const dim3 grid(g.grid), blk(NSX_BLOCK);
@@ -1995,7 +1848,18 @@ int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double 
   // r0 = b - A x0 and its norms (P.init), e = 0, x = x0; rt, rho_0 into P.u
   hipLaunchKernelGGL(k_resid<float>, grid, blk, 0, s, g, rhs, phi_in, x, rt, scratch, e, part, part.init,
                      (double *)nullptr, phi_in != x ? 1 : 0, st);
-  hipLaunchKernelGGL(k_cg_start, grid, blk, 0, s, g, rhs, (const double *)x, rt, part, st);
+  hipLaunchKernelGGL(k_cg_start<float>, grid, blk, 0, s, g, rhs, (const double *)x, rt, part, st);
+  return (int)hipGetLastError();
+}
+
+int launch_cg_init64(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *rt,
+                     double *scratch, double *e, Partials part, SolverState *st, double eps, double delta,
+                     int max_iters) {
+  const dim3 grid(g.grid), blk(NSX_BLOCK);
+  hipLaunchKernelGGL(k_bicg_init_state, dim3(1), blk, 0, s, st, eps, max_iters, delta);
+  hipLaunchKernelGGL(k_resid<double>, grid, blk, 0, s, g, rhs, phi_in, x, rt, scratch, e, part, part.init,
+                     (double *)nullptr, phi_in != x ? 1 : 0, st);
+  hipLaunchKernelGGL(k_cg_start<double>, grid, blk, 0, s, g, rhs, (const double *)x, rt, part, st);
   return (int)hipGetLastError();
 }
 
@@ -2023,59 +1887,26 @@ int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x,
   const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs);
   const size_t lds = cgs_lds(g);
   if (pc == 2)
-    LAUNCH_TIMED(ev, 0, k_cgs<2>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+    LAUNCH_TIMED(ev, 0, (k_cgs<float, 2>), dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin,
+                 Pout, st);
   else if (pc == 1)
-    LAUNCH_TIMED(ev, 0, k_cgs<1>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+    LAUNCH_TIMED(ev, 0, (k_cgs<float, 1>), dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin,
+                 Pout, st);
   else
-    LAUNCH_TIMED(ev, 0, k_cgs<0>, dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin, Pout, st);
+    LAUNCH_TIMED(ev, 0, (k_cgs<float, 0>), dim3(g.grid_cgs), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin,
+                 Pout, st);
   return (int)hipGetLastError();
 }
 
-int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st) {
-  hipLaunchKernelGGL(k_bicg_init_state, dim3(1), dim3(NSX_BLOCK), 0, s, st, 0.0, 0, 0.0);
-  return (int)hipGetLastError();
-}
-
-int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
-                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double delta,
-                    int max_iters) {
-  const dim3 grid(g.grid), blk(NSX_BLOCK);
-  hipLaunchKernelGGL(k_bicg_init_state, dim3(1), blk, 0, s, st, eps, max_iters, delta);
-  // the FP64 norms of r0 = b - A x0 seed the first A1 (P.init); r32 = rhat32
-  hipLaunchKernelGGL(k_resid<float>, grid, blk, 0, s, g, rhs, phi_in, x, r32, rhat32, e32, part, part.init,
-                     (double *)nullptr, phi_in != x ? 1 : 0, st);
-  return (int)hipGetLastError();
-}
-
-int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
-                    const float *rhat, float *q, float *vt, float *tv, Partials part, SolverState *st,
-                    hipEvent_t *ev) {
-  const dim3 grid(g.grid), blk(NSX_BLOCK);
-  const size_t lds = level_lds(g, sizeof(float));
-  LAUNCH_TIMED(ev, 0, k_rel_u, grid, blk, (size_t)0, s, g, rhs, (const double *)x, (const float *)e, rt, rhat, part,
-               st);
-  LAUNCH_TIMED(ev, 1, k_bicg_a<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)q, rhat, vt, part, st,
-               x, (const float *)e);
-  LAUNCH_TIMED(ev, 2, k_bicg_b<float>, grid, blk, lds, s, g, (const float *)rt, (const float *)vt, tv, part, st);
-  LAUNCH_TIMED(ev, 3, k_bicg_c<float>, grid, blk, lds, s, g, e, rt, q, rhat, (const float *)vt, (const float *)tv,
-               part, st);
-  return (int)hipGetLastError();
-}
-
-int launch_mp_resid(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
-                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double eps_inner,
-                    int max_iters, int max_passes, int first_pass) {
-  const dim3 grid(g.grid), blk(NSX_BLOCK);
-  // the first pass copies phi_in into x and forms the residual of phi_in;
-  // later passes form the residual of the corrected x
-  hipLaunchKernelGGL(k_resid<float>, grid, blk, 0, s, g, rhs, first_pass ? phi_in : (const double *)x, x, r32,
-                     rhat32, e32, part, part.outer, part.init, first_pass && phi_in != x ? 1 : 0, st);
-  hipLaunchKernelGGL(k_mp_check, dim3(1), blk, 0, s, part.outer, grid.x, st, eps, eps_inner, max_iters, max_passes);
-  return (int)hipGetLastError();
-}
-
-int launch_mp_update(const Geo &g, hipStream_t s, double *x, const float *e32) {
-  hipLaunchKernelGGL(k_mp_update, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, x, e32);
+// the FP64 instance (4-layer blocks): the same schedule on its own grid
+int launch_cgs64(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecsT<double> &v,
+                 Partials part, SolverState *st, hipEvent_t *ev) {
+  double *Pout = (j & 1) ? part.u : part.c;
+  const double *Pin = j == 0 ? part.u : ((j & 1) ? part.c : part.u);
+  const uint32_t Gin = (uint32_t)(j == 0 ? g.grid : g.grid_cgs64);
+  const size_t lds = cgs_lds(g, sizeof(double));
+  LAUNCH_TIMED(ev, 0, (k_cgs<double, 1>), dim3(g.grid_cgs64), dim3(NSX_BLOCK), lds, s, g, j, v, rhs, x, Pin, Gin,
+               Pout, st);
   return (int)hipGetLastError();
 }
 
@@ -2119,7 +1950,7 @@ int cg_blocks_per_cu(const Geo &g) {
 
 // resident blocks per CU of the single-reduction FP32 CG kernel
 int cgs_blocks_per_cu(const Geo &g) {
-  const void *ks[3] = {(const void *)k_cgs<0>, (const void *)k_cgs<1>, (const void *)k_cgs<2>};
+  const void *ks[3] = {(const void *)k_cgs<float, 0>, (const void *)k_cgs<float, 1>, (const void *)k_cgs<float, 2>};
   int m = 1 << 30;
   for (const void *k : ks) {
     int n = 0;
@@ -2130,16 +1961,25 @@ int cgs_blocks_per_cu(const Geo &g) {
   return m;
 }
 
+// resident blocks per CU of the FP64 single-reduction CG kernel
+int cgs64_blocks_per_cu(const Geo &g) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_cgs<double, 1>, NSX_BLOCK,
+                                                   cgs_lds(g, sizeof(double))) != hipSuccess ||
+      n < 1)
+    return -1;
+  return n;
+}
+
 int solver_blocks_per_cu(const Geo &g) {
   struct K {
     const void *f;
     size_t lds;
   } ks[] = {
       {(const void *)k_bicg_a<double>, level_lds(g, 8)}, {(const void *)k_bicg_b<double>, level_lds(g, 8)},
-      {(const void *)k_bicg_c<double>, level_lds(g, 8)}, {(const void *)k_bicg_a<float>, level_lds(g, 4)},
-      {(const void *)k_bicg_b<float>, level_lds(g, 4)},  {(const void *)k_bicg_c<float>, level_lds(g, 4)},
-      {(const void *)k_resid<double>, 0},                 {(const void *)k_resid<float>, 0},
-      {(const void *)k_rel_u, 0},                         {(const void *)k_cg_start, 0},
+      {(const void *)k_bicg_c<double>, level_lds(g, 8)}, {(const void *)k_resid<double>, 0},
+      {(const void *)k_resid<float>, 0},                  {(const void *)k_cg_start<float>, 0},
+      {(const void *)k_cg_start<double>, 0},
       {(const void *)k_cg_x, level_lds(g, 4)},
   };
   int best = 1 << 30;

@@ -56,11 +56,14 @@ struct nsx_ctx {
   double *ustar[3] = {}, *uv2[3] = {}, *rhsp = nullptr;
   // FP64 BiCGSTAB vectors: rt = r/A0, rhat, q = (p - omega v)/A0, vt = v/A0, t = A z
   double *sr = nullptr, *srhat = nullptr, *sq = nullptr, *svt = nullptr, *stv = nullptr;
+  // NSX_PREC_FP64_CG: the single-reduction CG's second buffer set, with the
+  // BiCGSTAB vectors above as its first (a solve uses one solver at a time)
+  double *scg[4] = {};
   SolverState *st_dev = nullptr, *st_host = nullptr;
   Partials part{};
   nsx_solve_info info{};
   int iter_hint = 0;
-  // mixed precision: FP32 inner BiCGSTAB vectors and settings
+  // mixed precision: FP32 CG vectors and settings
   float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fv = nullptr, *ft = nullptr, *fe = nullptr;
   float *fe2 = nullptr, *fr2 = nullptr, *fp2 = nullptr;  // FP32 CG: the second set of e, rt, p
   double *cfl_part = nullptr, *cfl_out = nullptr;  // glob_cfl partials and result
@@ -70,20 +73,17 @@ struct nsx_ctx {
   int sm_n = 0;
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CGSR_BLOCK;
-  double delta = 0.1;
-  double eps_inner = 1e-5;
-  int max_passes = 30;
-  int iter_hint32 = 0;
-  double delta_cg = 0.1;  // reliable-update fraction of NSX_REFINE_CG and NSX_REFINE_CGSR
+  double delta_cg = 0.1;  // reliable-update fraction of the CG forms
   int iter_hint_cg = 0;
   int iter_hint_cgs = 0;
+  int iter_hint_cgs64 = 0;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  Timer t_k[3], t_stage, t_k32[3], t_u, t_cg[2];  // FP64 A, B, C; stage kernels; FP32 A, B, C; U; FP32 CG W, X
-  Timer t_cgs;                                     // single-reduction FP32 CG
-  // (event index, timer: 0..2 FP64 solver, 5 stage, 6..8 FP32, 11 U, 12..13 FP32 CG)
+  Timer t_k[3], t_stage, t_cg[2];  // FP64 A, B, C; stage kernels; FP32 CG W, X
+  Timer t_cgs, t_cgs64;              // single-reduction CG, FP32 and FP64
+  // (event index, timer: 0..2 FP64 solver, 5 stage, 12..13 FP32 CG, 14 single-reduction CG, 15 its FP64 instance)
   std::vector<std::pair<size_t, int>> pending;
 };
 
@@ -359,6 +359,15 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     Gs = Gs / NSX_NXCD * NSX_NXCD;
     if (Gs < NSX_NXCD) Gs = NSX_NXCD;
     g.grid_cgs = (int)Gs;
+    // its FP64 instance: every block its occupancy allows, at most 4 a CU
+    const int per_cu_64 = std::min(cgs64_blocks_per_cu(g), 4);
+    if (per_cu_64 < 1) return nsx_fail(ctx, -2, "occupancy query of the FP64 single-reduction CG kernel failed");
+    long G6 = (long)prop.multiProcessorCount * per_cu_64;
+    if (G6 > need) G6 = need;
+    if (G6 > MAXGRID) G6 = MAXGRID;
+    G6 = G6 / NSX_NXCD * NSX_NXCD;
+    if (G6 < NSX_NXCD) G6 = NSX_NXCD;
+    g.grid_cgs64 = (int)G6;
   }
   if (NS > 4096) return nsx_fail(ctx, -1, "NZ = %d exceeds the 4094 layers the solver's LDS level table holds", NZ);
   g.bctop = ctx->bctop;
@@ -606,6 +615,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   if ((rc = cell(&ctx->rhsp)) || (rc = cell(&ctx->sr)) || (rc = cell(&ctx->srhat)) || (rc = cell(&ctx->sq)) ||
       (rc = cell(&ctx->svt)) || (rc = cell(&ctx->stv)))
     return rc;
+  for (int q = 0; q < 4; ++q)
+    if ((rc = cell(&ctx->scg[q]))) return rc;
   // the time-level slots point into pool set 0 (n), 1 (stage), 2 (new)
   ctx->set_n = 0;
   for (int q = 0; q < 4; ++q) {
@@ -620,8 +631,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   for (int id = 0; id < NSLOT; ++id)
     if (ctx->slot[id] && ctx->kind[id] != FK_NONE && !(id >= NSX_F_UFN && id <= NSX_F_PFNP))
       HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, field_elems(ctx, ctx->kind[id]) * sizeof(double), ctx->stream));
-  double *zs[] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2], ctx->rhsp, ctx->sr, ctx->srhat, ctx->sq, ctx->svt,
-                  ctx->stv};
+  double *zs[] = {ctx->ustar[0], ctx->ustar[1], ctx->ustar[2], ctx->rhsp,      ctx->sr,        ctx->srhat, ctx->sq,
+                  ctx->svt,      ctx->stv,      ctx->scg[0],    ctx->scg[1], ctx->scg[2], ctx->scg[3]};
   for (double *p : zs) HIPCHECK(hipMemsetAsync(p, 0, nc * sizeof(double), ctx->stream));
   for (int q = 0; q < 3; ++q) HIPCHECK(hipMemsetAsync(ctx->uv2[q], 0, nv * sizeof(double), ctx->stream));
   float **fvec[9] = {&ctx->fr, &ctx->frhat, &ctx->fp, &ctx->fv, &ctx->ft, &ctx->fe, &ctx->fe2, &ctx->fr2, &ctx->fp2};
@@ -636,15 +647,14 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   HIPCHECK(hipHostMalloc((void **)&ctx->st_host, sizeof(SolverState), hipHostMallocDefault));
   // one buffer of [rows][MAXGRID] per reduction site; every member of
   // Partials must be listed here (checked below)
-  double **rows[7] = {&ctx->part.src, &ctx->part.init, &ctx->part.a,    &ctx->part.b,
-                      &ctx->part.c,   &ctx->part.outer, &ctx->part.u};
-  const int nrow[7] = {1, 5, 1, 2, 4, 5, 4};
-  static_assert(sizeof(Partials) == 7 * sizeof(double *), "a Partials member has no buffer");
-  for (int q = 0; q < 7; ++q) {
+  double **rows[6] = {&ctx->part.src, &ctx->part.init, &ctx->part.a, &ctx->part.b, &ctx->part.c, &ctx->part.u};
+  const int nrow[6] = {1, 5, 1, 2, 4, 4};
+  static_assert(sizeof(Partials) == 6 * sizeof(double *), "a Partials member has no buffer");
+  for (int q = 0; q < 6; ++q) {
     if ((rc = dev_alloc(ctx, rows[q], (size_t)nrow[q] * MAXGRID))) return rc;
     HIPCHECK(hipMemsetAsync(*rows[q], 0, (size_t)nrow[q] * MAXGRID * sizeof(double), ctx->stream));
   }
-  for (int q = 0; q < 7; ++q)
+  for (int q = 0; q < 6; ++q)
     if (!*rows[q]) return nsx_fail(ctx, -1, "reduction buffer %d is not allocated", q);
   if ((rc = dev_alloc(ctx, &ctx->cfl_part, (size_t)glob_cfl_rows() * MAXGRID))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->cfl_out, 16))) return rc;
@@ -739,9 +749,9 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[15] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr,       nullptr,
-                  &ctx->t_stage,  &ctx->t_k32[0], &ctx->t_k32[1], &ctx->t_k32[2], nullptr,
-                  nullptr,        &ctx->t_u,      &ctx->t_cg[0],  &ctx->t_cg[1],  &ctx->t_cgs};
+  Timer *t[16] = {&ctx->t_k[0], &ctx->t_k[1], &ctx->t_k[2],  nullptr,       nullptr,     &ctx->t_stage,
+                  nullptr,       nullptr,       nullptr,        nullptr,       nullptr,     nullptr,
+                  &ctx->t_cg[0], &ctx->t_cg[1], &ctx->t_cgs,    &ctx->t_cgs64};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (t[pr.second] && hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -807,14 +817,6 @@ static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, dou
   return 0;
 }
 
-// The mixed-precision form of the same solve: FP64 iterative refinement
-// (defect correction) around the FP32 BiCGSTAB.  Each pass forms the true
-// residual r = b - A x of the FP64 solution in FP64, tests it with the
-// reference's rule (min(|r|_1/|b|_1, |r|_2/|b|_2) < eps), solves A e = r in
-// FP32 to the relative tolerance eps_inner from e = 0, and adds e to x in
-// FP64.  Glob_Norm and the ghost layers follow as in the FP64 solve.
-static int solve_fp64(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x);
-
 // The FP64 solve (the reference's BiCGSTAB) from the x an FP32 phase has
 // reached; the solve record counts both phases.
 static int finish_fp64(nsx_ctx *ctx, const double *rhs, double *x, int its32, int passes) {
@@ -825,136 +827,6 @@ static int finish_fp64(nsx_ctx *ctx, const double *rhs, double *x, int its32, in
   ctx->info.iters += its32;
   ctx->info.iters_total = before + ctx->info.iters;
   ctx->info.outer_passes = passes;
-  return 0;
-}
-
-// cascade = 1: a single FP32 pass, then the FP64 solve (the reference's
-// BiCGSTAB) from its result
-static int solve_mixed(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x, int cascade) {
-  const Geo &g = ctx->g;
-  hipStream_t s = ctx->stream;
-  LAUNCH(launch_mp_begin(g, s, ctx->st_dev));
-  int total = 0, passes = 0;
-  for (int pass = 0;; ++pass) {
-    if (cascade && passes == 1) return finish_fp64(ctx, rhs, x, total, passes);
-    LAUNCH(launch_mp_resid(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
-                           ctx->eps_inner, ctx->MaxIters, ctx->max_passes, pass == 0));
-    // the outer test decides whether this pass runs an inner solve at all
-    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    if (ctx->st_host->mp_done) break;
-    const int hint = ctx->iter_hint32;
-    int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
-    int issued = 0;
-    const size_t pstart = ctx->pending.size();
-    for (;;) {
-      for (int c = 0; c < chunk; ++c) {
-        hipEvent_t *ev = nullptr;
-        if (ctx->timing) {
-          ev = events(ctx, 6);
-          if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
-        }
-        LAUNCH(launch_bicg_iter32(g, s, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->ft, ctx->part,
-                                  ctx->st_dev, ev));
-        if (ev) {
-          const size_t base = (size_t)(ev - &ctx->evpool[0]);
-          for (int q = 0; q < 3; ++q) ctx->pending.push_back({base + 2 * q, 6 + q});
-        }
-        ++issued;
-      }
-      HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-      HIPCHECK(hipStreamSynchronize(s));
-      if (ctx->st_host->done) break;
-      if (issued > ctx->MaxIters + 4) return nsx_fail(ctx, -4, "inner solver did not stop after %d iterations", issued);
-      chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
-    }
-    const SolverState &h = *ctx->st_host;
-    if (ctx->timing && ctx->pending.size() > pstart + 3 * (size_t)h.a_iter)
-      ctx->pending.resize(pstart + 3 * (size_t)h.a_iter);
-    total += h.a_iter;
-    ++passes;
-    if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
-    if (h.exit_code == NSX_SOLVE_BREAKDOWN || h.exit_code == NSX_SOLVE_MAX_ITERS) {
-      // the FP32 pass failed: its correction is dropped and the FP64 solve
-      // finishes from the current x
-      collect_timing(ctx);
-      return finish_fp64(ctx, rhs, x, total, passes);
-    }
-    LAUNCH(launch_mp_update(g, s, x, ctx->fe));
-  }
-  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
-  const SolverState &h = *ctx->st_host;
-  ctx->info.iters = total;
-  ctx->info.exit_code = h.exit_code;
-  ctx->info.e1norm = h.oe1;
-  ctx->info.e2norm = h.oe2;
-  ctx->info.n_solves += 1;
-  ctx->info.iters_total += total;
-  ctx->info.outer_passes = passes;
-  ctx->info.iters_fp32 = total;
-  collect_timing(ctx);
-  return 0;
-}
-
-// The mixed-precision form with reliable updates: one FP32 BiCGSTAB whose
-// recursive residual is replaced by the true FP64 residual of the FP64
-// solution plus the FP32 correction whenever it has fallen by delta (and at
-// the end), so the Krylov space is kept across refinements (k_rel_u).
-static int solve_reliable(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
-  const Geo &g = ctx->g;
-  hipStream_t s = ctx->stream;
-  LAUNCH(launch_rel_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
-                         ctx->delta, ctx->MaxIters));
-  int issued = 0;
-  const size_t pend0 = ctx->pending.size();
-  const int hint = ctx->iter_hint32;
-  int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
-  for (;;) {
-    for (int c = 0; c < chunk; ++c) {
-      hipEvent_t *ev = nullptr;
-      if (ctx->timing) {
-        ev = events(ctx, 8);
-        if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
-      }
-      LAUNCH(launch_rel_iter(g, s, rhs, x, ctx->fe, ctx->fr, ctx->frhat, ctx->fp, ctx->fv, ctx->ft,
-                             ctx->part, ctx->st_dev, ev));
-      if (ev) {
-        const size_t base = (size_t)(ev - &ctx->evpool[0]);
-        ctx->pending.push_back({base, 11});
-        for (int q = 0; q < 3; ++q) ctx->pending.push_back({base + 2 + 2 * q, 6 + q});
-      }
-      ++issued;
-    }
-    HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    if (ctx->st_host->done) break;
-    if (issued > ctx->MaxIters + 4) return nsx_fail(ctx, -4, "solver did not stop after %d iterations", issued);
-    chunk = hint > 0 ? (hint / 32 > 4 ? hint / 32 : 4) : 16;
-  }
-  const SolverState &h = *ctx->st_host;
-  if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
-    // the FP32 iteration failed: the correction since the last update is
-    // dropped and the FP64 solve finishes from x
-    if (ctx->timing && ctx->pending.size() > pend0 + 4 * (size_t)h.a_iter)
-      ctx->pending.resize(pend0 + 4 * (size_t)h.a_iter);
-    collect_timing(ctx);
-    const int its32 = h.a_iter, ups = h.mp_pass;
-    if (its32 > 0) ctx->iter_hint32 = its32;
-    return finish_fp64(ctx, rhs, x, its32, ups);
-  }
-  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
-  ctx->info.iters = h.a_iter;
-  ctx->info.exit_code = h.exit_code;
-  ctx->info.e1norm = h.e1;
-  ctx->info.e2norm = h.e2;
-  ctx->info.n_solves += 1;
-  ctx->info.iters_total += h.a_iter;
-  ctx->info.outer_passes = h.mp_pass;
-  ctx->info.iters_fp32 = h.a_iter;
-  if (h.a_iter > 0) ctx->iter_hint32 = h.a_iter;
-  if (ctx->timing && ctx->pending.size() > pend0 + 4 * (size_t)h.a_iter)
-    ctx->pending.resize(pend0 + 4 * (size_t)h.a_iter);
-  collect_timing(ctx);
   return 0;
 }
 
@@ -1024,22 +896,30 @@ static int solve_cg(nsx_ctx *ctx, const double *rhs, const double *phi_in, doubl
 // in the single-reduction form, one kernel per iteration (k_cgs); a launch
 // is an iteration, a true residual or the product after one, as the device
 // decides.  Fallback as in solve_cg.
-static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x, bool fp64 = false) {
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
   // the preconditioner: point Jacobi, 4-layer blocks or lines (a window of
   // the line preconditioner holds whole columns; nsx_set_precision checks
   // this too, but may run before the mesh is committed)
-  const int pc = ctx->refine == NSX_REFINE_CGSR_LINE ? 2 : ctx->refine == NSX_REFINE_CGSR_BLOCK ? 1 : 0;
+  const int pc = fp64 ? 1 : ctx->refine == NSX_REFINE_CGSR_LINE ? 2 : ctx->refine == NSX_REFINE_CGSR_BLOCK ? 1 : 0;
   if (pc == 2 && g.NQ > NSX_BLOCK)
     return nsx_fail(ctx, -1, "line preconditioner: %d layers per column exceed %d", g.NS, 4 * NSX_BLOCK);
   const int cap = 2 * ctx->MaxIters + 100;
-  LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
-                        ctx->delta_cg, cap));
+  // FP64: e = stv, p = sq, rt = (sr, scg0), om = (svt, scg1), sg = (scg2, scg3)
+  const CgsVecsT<double> v64{ctx->stv, ctx->sq, {ctx->sr, ctx->scg[0]}, {ctx->svt, ctx->scg[1]},
+                             {ctx->scg[2], ctx->scg[3]}};
+  if (fp64)
+    LAUNCH(launch_cg_init64(g, s, rhs, phi_in, x, v64.rt[0], ctx->srhat, v64.e, ctx->part, ctx->st_dev, ctx->eps,
+                            ctx->delta_cg, cap));
+  else
+    LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                          ctx->delta_cg, cap));
   const CgsVecs v{ctx->fe, ctx->fp, {ctx->fr, ctx->fr2}, {ctx->fv, ctx->fp2}, {ctx->ft, ctx->fe2}};
   int issued = 0;
   const size_t pend0 = ctx->pending.size();
-  const int hint = ctx->iter_hint_cgs;
+  int &hint_ref = fp64 ? ctx->iter_hint_cgs64 : ctx->iter_hint_cgs;
+  const int hint = hint_ref;
   int chunk = hint > 0 ? (hint - hint / 16 > 4 ? hint - hint / 16 : 4) : 32;
   for (;;) {
     for (int c = 0; c < chunk; ++c) {
@@ -1048,8 +928,11 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
         ev = events(ctx, 2);
         if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
       }
-      LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev, pc));
-      if (ev) ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 14});
+      if (fp64)
+        LAUNCH(launch_cgs64(g, s, issued, rhs, x, v64, ctx->part, ctx->st_dev, ev));
+      else
+        LAUNCH(launch_cgs(g, s, issued, rhs, x, v, ctx->part, ctx->st_dev, ev, pc));
+      if (ev) ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), fp64 ? 15 : 14});
       ++issued;
     }
     HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -1066,7 +949,7 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
   if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN) {
     collect_timing(ctx);
     const int its32 = h.a_iter, ups = h.mp_pass;
-    if (its32 > 0) ctx->iter_hint_cgs = its32;
+    if (its32 > 0) hint_ref = its32;
     return finish_fp64(ctx, rhs, x, its32, ups);
   }
   LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
@@ -1077,19 +960,17 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
   ctx->info.n_solves += 1;
   ctx->info.iters_total += h.a_iter;
   ctx->info.outer_passes = h.mp_pass;
-  ctx->info.iters_fp32 = h.a_iter;
-  if (h.a_iter > 0) ctx->iter_hint_cgs = h.a_iter;
+  ctx->info.iters_fp32 = fp64 ? 0 : h.a_iter;
+  if (h.a_iter > 0) hint_ref = h.a_iter;
   collect_timing(ctx);
   return 0;
 }
 
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  if (ctx->precision == NSX_PREC_FP64_CG) return solve_cgs(ctx, rhs, phi_in, x, true);
   if (ctx->precision != NSX_PREC_MIXED) return solve_fp64(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
-  if (ctx->refine == NSX_REFINE_CGSR || ctx->refine == NSX_REFINE_CGSR_BLOCK || ctx->refine == NSX_REFINE_CGSR_LINE)
-    return solve_cgs(ctx, rhs, phi_in, x);
-  if (ctx->refine == NSX_REFINE_RELIABLE) return solve_reliable(ctx, rhs, phi_in, x);
-  return solve_mixed(ctx, rhs, phi_in, x, ctx->refine == NSX_REFINE_CASCADE);
+  return solve_cgs(ctx, rhs, phi_in, x);
 }
 
 // vertex values of up to three fields; kinds 0 p, 1 u, 2 v, 3 w
@@ -1236,7 +1117,7 @@ extern "C" int nsx_hydro(nsx_ctx *ctx, int flag_ab, nsx_solve_info *info) {
 
 extern "C" int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info *info) {
   NEED_COMMIT(ctx);
-  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
+  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED && precision != NSX_PREC_FP64_CG)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
   hipStream_t s = ctx->stream;
   int worst = 0;
@@ -1642,13 +1523,17 @@ extern "C" int nsx_savetec_vc(nsx_ctx *ctx, const char *dir, int counter, double
 // ---------------------------------------------------------------- timing --
 extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes) {
   if (!ctx) return -1;
-  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED)
+  if (precision != NSX_PREC_FP64 && precision != NSX_PREC_MIXED && precision != NSX_PREC_FP64_CG)
     return nsx_fail(ctx, -1, "unknown precision %d", precision);
-  if (refine != NSX_REFINE_RELIABLE && refine != NSX_REFINE_RESTART && refine != NSX_REFINE_CASCADE &&
-      refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK &&
+  if (refine >= 0 && refine <= 2)
+    return nsx_fail(ctx, -1,
+                    "refinement %d (an FP32 BiCGSTAB form) is not built: FP32 BiCGSTAB breaks down on this "
+                    "operator; use NSX_REFINE_CGSR_BLOCK",
+                    refine);
+  if (refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK &&
       refine != NSX_REFINE_CGSR_LINE)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
-  const bool sr = refine == NSX_REFINE_CGSR || refine == NSX_REFINE_CGSR_BLOCK || refine == NSX_REFINE_CGSR_LINE;
+  const bool sr = refine != NSX_REFINE_CG;
   // a window of the line preconditioner holds whole columns of four-layer quads
   if (refine == NSX_REFINE_CGSR_LINE && ctx->g.NQ > NSX_BLOCK)
     return nsx_fail(ctx, -1, "line preconditioner: %d layers per column exceed %d", ctx->g.NS, 4 * NSX_BLOCK);
@@ -1660,13 +1545,7 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   if (max_passes < 1) return nsx_fail(ctx, -1, "max_passes %d < 1", max_passes);
   ctx->precision = precision;
   ctx->refine = refine;
-  if (refine == NSX_REFINE_RELIABLE)
-    ctx->delta = tol;
-  else if (refine == NSX_REFINE_CG || sr)
-    ctx->delta_cg = tol;
-  else
-    ctx->eps_inner = tol;
-  ctx->max_passes = max_passes;
+  ctx->delta_cg = tol;
   return 0;
 }
 
@@ -1680,17 +1559,14 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
   if (!ctx || !name) return -1;
   const Timer *t = nullptr;
   static const char *names[3] = {"bicg_a", "bicg_b", "bicg_c"};
-  static const char *names32[3] = {"bicg32_a", "bicg32_b", "bicg32_c"};
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < 3; ++q)
     if (!strcmp(name, names[q])) t = &ctx->t_k[q];
-    if (!strcmp(name, names32[q])) t = &ctx->t_k32[q];
-  }
   if (!strcmp(name, "stage")) t = &ctx->t_stage;
-  if (!strcmp(name, "rel_u")) t = &ctx->t_u;
   static const char *names_cg[2] = {"cg32_w", "cg32_x"};
   for (int q = 0; q < 2; ++q)
     if (!strcmp(name, names_cg[q])) t = &ctx->t_cg[q];
   if (!strcmp(name, "cg32_sr")) t = &ctx->t_cgs;
+  if (!strcmp(name, "cg64_sr")) t = &ctx->t_cgs64;
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -1699,7 +1575,7 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
 
 extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
-  for (int q = 0; q < 3; ++q) ctx->t_k[q] = ctx->t_k32[q] = Timer{};
-  ctx->t_stage = ctx->t_u = ctx->t_cg[0] = ctx->t_cg[1] = ctx->t_cgs = Timer{};
+  for (int q = 0; q < 3; ++q) ctx->t_k[q] = Timer{};
+  ctx->t_stage = ctx->t_cg[0] = ctx->t_cg[1] = ctx->t_cgs = ctx->t_cgs64 = Timer{};
   return 0;
 }

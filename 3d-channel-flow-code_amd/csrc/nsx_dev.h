@@ -59,6 +59,7 @@ struct Geo {
   int grid;                 // grid of the kernels that produce or consume solver partial sums
   int grid_cg;              // grid of the FP32 CG stencil kernel (its own occupancy)
   int grid_cgs;             // grid of the single-reduction FP32 CG kernel (its own occupancy)
+  int grid_cgs64;           // grid of its FP64 instance
   uint32_t nquad;           // N0*NQ
   FastDiv divNZi, divNL, divNQ;
   // per cell-edge (N0*3, edge-fastest: index i*3+j)

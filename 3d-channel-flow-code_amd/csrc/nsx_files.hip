@@ -142,6 +142,9 @@ static void fmt_list_real(char *out, double x) {
 
 extern "C" int nsx_format_real(double x, int kind, int w, int d, char *out, int outlen) {
   char b[64];
+  // fmt_e builds the digits in 32-byte buffers and the line in 64 bytes
+  if (kind == 0 && (d < 1 || d > 30 || w < 0 || w > 63)) return -1;
+  if (kind != 0 && kind != 1) return -1;
   if (kind == 0)
     fmt_e(b, x, w, d);
   else

@@ -10,8 +10,7 @@ struct Partials {
   double *a;     // <rhat, v>, 1 row
   double *b;     // <t,s>, <t,t>, 2 rows
   double *c;     // |r|^2, |r|_1, <rhat,r>, sum x Vol, 4 rows
-  double *outer; // refinement residual norms of the mixed solve, 5 rows
-  double *u;     // true residual after a reliable update: |r|^2, |r|_1, <rhat,r>, sum x Vol, 4 rows
+  double *u;     // FP32 CG: true residual after a reliable update, 4 rows
 };
 
 // BiCGSTAB scalars, device resident (bicgstab.F90:355-368 locals).  Each
@@ -47,17 +46,12 @@ struct SolverState {
   int a_iter;        // k of the last phase A                      (phase A)
   int done, exit_code;                                        //   (phase A or B)
   int max_iters, pad;                                         //   (init)
-  // FP64 iterative refinement around the FP32 solve (k_mp_check)
-  double ob1, ob2;   // |b|_1, |b|_2 of the mean-removed rhs
-  double oe1, oe2;   // relative norms of the true residual b - A x
-  int mp_pass;       // refinement residuals formed so far
-  int mp_done;       // outer stop
-  // reliable updates (mixed precision, Krylov space kept across refinements)
-  double delta;      // update when |r| < delta x the largest |r| since the last one  (init)
-  double r0N;        // |r| of the last true residual                  (phase A, update)
-  double maxrr[2];   // largest |r| since the last update, by parity   (phase U; reset in A)
-  int upd;           // this iteration starts from a true residual     (phase U)
+  // reliable updates of the CG forms
+  int mp_pass;       // true residuals formed after the start (reliable updates)
   int pad2;
+  double delta;      // update when |r| < delta x the largest |r| since the last one  (init)
+  double r0N;        // |r| of the initial residual                    (phase A, k = 0)
+  double maxrr[2];   // largest |r| since the last update, by parity
   double a_beta;     // beta_k of y = rt + beta q                      (phase A)
   int ureq[2];       // FP32 CG: W_k asks U_{k+1} for a true residual, by parity of k (phase W)
   CgsCarry cgs[2];   // single-reduction FP32 CG, by parity of the launch
@@ -65,16 +59,20 @@ struct SolverState {
   int pad3;
 };
 
-// the vectors of the single-reduction FP32 CG: e and p in place, the scaled
-// residual rt = r/A0, om = (A rt)/A0 and sg = s/A0 in two sets
-struct CgsVecs {
-  float *e, *p, *rt[2], *om[2], *sg[2];
+// the vectors of the single-reduction CG: e and p in place, the scaled
+// residual rt = r/A0, om = (A rt)/A0 and sg = s/A0 in two sets (FP32 in the
+// precision form, FP64 in its FP64 counterpart)
+template <typename T>
+struct CgsVecsT {
+  T *e, *p, *rt[2], *om[2], *sg[2];
 };
+using CgsVecs = CgsVecsT<float>;
 
 int nsx_grid_for(uint32_t n);
 int solver_blocks_per_cu(const Geo &g);
 int cg_blocks_per_cu(const Geo &g);
 int cgs_blocks_per_cu(const Geo &g);
+int cgs64_blocks_per_cu(const Geo &g);
 
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
@@ -120,21 +118,6 @@ int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const doubl
 int launch_bicg_iter(const Geo &g, hipStream_t s, double *x, double *rt, const double *rhat, double *q, double *vt,
                      double *tv, Partials part, SolverState *st,
                      hipEvent_t *ev /* nullable: 6 events, start/stop of A, B, C */);
-int launch_bicg_iter32(const Geo &g, hipStream_t s, float *x, float *rt, const float *rhat, float *q, float *vt,
-                       float *tv, Partials part, SolverState *st, hipEvent_t *ev);
-// mixed precision: reset, refinement residual (+ outer test, inner reset), correction
-int launch_mp_begin(const Geo &g, hipStream_t s, SolverState *st);
-int launch_mp_resid(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
-                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double eps_inner,
-                    int max_iters, int max_passes, int first_pass);
-int launch_mp_update(const Geo &g, hipStream_t s, double *x, const float *e32);
-// reliable updates: initial true residual, then iterations U, A, B, C
-int launch_rel_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *r32,
-                    float *rhat32, float *e32, Partials part, SolverState *st, double eps, double delta,
-                    int max_iters);
-int launch_rel_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *rt,
-                    const float *rhat, float *q, float *vt, float *tv, Partials part, SolverState *st,
-                    hipEvent_t *ev /* nullable: 8 events, start/stop of U, A, B, C */);
 // mixed precision, FP32 CG with reliable updates: start, then W, X per iteration;
 // v = e, rt, p, w, e_b, rt_b, p_b (the second set takes the true-residual pass)
 int launch_cg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, float *rt,
@@ -146,5 +129,11 @@ int launch_cg_iter(const Geo &g, hipStream_t s, const double *rhs, double *x, fl
 // 0 point Jacobi, 1 4-layer blocks, 2 lines (whole columns)
 int launch_cgs(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecs &v, Partials part,
                SolverState *st, hipEvent_t *ev, int pc);
+// the same CG with FP64 vectors and the 4-layer block preconditioner
+int launch_cg_init64(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *rt,
+                     double *scratch, double *e, Partials part, SolverState *st, double eps, double delta,
+                     int max_iters);
+int launch_cgs64(const Geo &g, hipStream_t s, int j, const double *rhs, double *x, const CgsVecsT<double> &v,
+                 Partials part, SolverState *st, hipEvent_t *ev);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);

@@ -25,10 +25,10 @@
            NSX_SOLVE_NO_SOLVE = 1, NSX_SOLVE_BREAKDOWN = 2,              &
            NSX_SOLVE_MAX_ITERS = 3
 !     precision and refinement (enum nsx_precision, enum nsx_refine)
-      integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1
-      integer(c_int), parameter :: NSX_REFINE_RELIABLE = 0,             &
-           NSX_REFINE_RESTART = 1, NSX_REFINE_CASCADE = 2,              &
-           NSX_REFINE_CG = 3, NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5, &
+      integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1,&
+           NSX_PREC_FP64_CG = 2
+      integer(c_int), parameter :: NSX_REFINE_CG = 3,                   &
+           NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5,               &
            NSX_REFINE_CGSR_LINE = 6
 !     accumulators of the channel statistics (enum nsx_sm, module stats)
       integer(c_int), parameter :: NSX_SM_UF = 0, NSX_SM_VF = 1,        &

@@ -21,9 +21,7 @@ N_FACE_SCRATCH = 10
 
 PREC_FP64 = 0
 PREC_MIXED = 1
-REFINE_RELIABLE = 0
-REFINE_RESTART = 1
-REFINE_CASCADE = 2
+PREC_FP64_CG = 2
 REFINE_CG = 3
 REFINE_CGSR = 4
 REFINE_CGSR_BLOCK = 5

@@ -113,6 +113,14 @@ class Context:
         self._check(self.lib.nsx_get_field(self.h, fid, _p(out), int(fill_ghost_rows)))
         return out
 
+    def device_memory_used(self) -> int:
+        """Bytes of device memory in use on the context's GPU (hipMemGetInfo)."""
+        hip = ctypes.CDLL("libamdhip64.so")
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+            raise NsxError("hipMemGetInfo failed")
+        return total.value - free.value
+
     def solve_info(self) -> dict:
         si = SolveInfo()
         self._check(self.lib.nsx_solve_info_get(self.h, ctypes.byref(si)))
@@ -331,20 +339,32 @@ class Context:
 
     def set_max_iters(self, n: int) -> None:
         """MaxIters of the pressure solve (the case's other parameters kept)."""
+        self.set_params(max_iters=n)
+
+    def set_params(self, eps: float = None, max_iters: int = None, dt: float = None) -> None:
+        """The tolerance eps and MaxIters of the pressure solve and the time
+        step dt (nsx_set_params); the parameters not given keep the values
+        last set (the case's to begin with)."""
         p = self.case.params
-        self._check(self.lib.nsx_set_params(self.h, p.dt, p.Re, p.bdx, p.bdy, p.bdz, p.eps, int(n), p.FinRK,
-                                            p.Iturbu))
+        cur = getattr(self, "_params", {"eps": p.eps, "max_iters": p.MaxIters, "dt": p.dt})
+        if eps is not None:
+            cur["eps"] = float(eps)
+        if max_iters is not None:
+            cur["max_iters"] = int(max_iters)
+        if dt is not None:
+            cur["dt"] = float(dt)
+        self._params = cur
+        self._check(self.lib.nsx_set_params(self.h, cur["dt"], p.Re, p.bdx, p.bdy, p.bdz, cur["eps"],
+                                            cur["max_iters"], p.FinRK, p.Iturbu))
 
     def set_precision(self, precision: int, refine: int = 5, tol: float = 0.1, max_passes: int = 30) -> None:
-        """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: an FP32
-        Krylov solver inside FP64 refinement: refine 5 = single-reduction CG
-        with the 4-layer block-Jacobi preconditioner and reliable updates at
-        threshold tol (the default), 6 = the same with the line (whole column)
-        preconditioner, 4 = the same with point Jacobi, 3 = the
-        point-Jacobi CG in two kernels per iteration, 0 = BiCGSTAB with reliable
-        updates at threshold tol, 1 = restarted BiCGSTAB defect correction
-        with inner tolerance tol, 2 = one FP32 BiCGSTAB pass to tol then the
-        FP64 solve); step() takes its own precision and uses this refinement."""
+        """Arithmetic of the pressure solve (PREC_FP64, or PREC_MIXED: FP32 CG
+        with reliable updates against the FP64 solution: refine 5 =
+        single-reduction CG with the 4-layer block-Jacobi preconditioner and
+        updates at threshold tol (the default), 6 = the same with the line
+        (whole column) preconditioner, 4 = the same with point Jacobi, 3 = the
+        point-Jacobi CG in two kernels per iteration); step() takes its own
+        precision and uses this refinement."""
         self._check(self.lib.nsx_set_precision(self.h, precision, refine, tol, max_passes))
 
     def timing(self, on: bool) -> None:

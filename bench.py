@@ -4,25 +4,37 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cfg 2] [--no-cpu]
 
 A step is one RK2 time step (two projection stages: Smagorinsky eddy
-viscosity, advection/diffusion fluxes, predictor, PPE source, BiCGSTAB
-pressure solve, corrector, face-velocity correction) of the channel workload
-on resident device state.  Two legs run the same steps from the same initial
-state on one GPU:
-  fp64   the reference's arithmetic (BiCGSTAB in FP64);
-  mixed  the pressure solve as FP32 Jacobi-preconditioned conjugate gradients
-         on the correction, with reliable updates (the true FP64 residual
-         replaces the recursive one when it has fallen tenfold) and the
-         reference's stopping rule on the true residual; --legs also takes
-         mixed_cascade (an FP32 BiCGSTAB pass to 1e-5, then the FP64
-         BiCGSTAB), mixed_restart and mixed_reliable (FP32 BiCGSTAB forms).
-Each leg: W warm-up steps, K timed steps (wall clock, no events), then one
-step with HIP events around every solver kernel for the per-kernel roofline.
-Prints one JSON line:
-  value        prism-cell-steps/s of the mixed leg = interior prisms x K / wall
-  fp64         the same for the FP64 leg; mixed_over_fp64 their ratio
+viscosity, advection/diffusion fluxes, predictor, PPE source, pressure
+solve, corrector, face-velocity correction) of the channel workload on
+resident device state.  Legs run the same steps from the same initial state
+on one GPU, each at a pressure-solve tolerance eps (the reference's stopping
+rule min(|r|_1/|b|_1, |r|_2/|b|_2) < eps):
+  fp64     the reference's arithmetic (Jacobi-preconditioned BiCGSTAB in FP64);
+  fp64_cg  the precision form's solver with FP64 vectors: single-reduction
+           CG with the 4-layer block-Jacobi preconditioner (separates the
+           gain of the solver from the gain of FP32);
+  mixed    the precision form: the same CG in FP32 with reliable updates
+           against the FP64 solution, stopped on the true FP64 residual;
+  ref      fp64_cg at eps 1e-14, the tight reference the other legs'
+           errors are measured against.
+The headline legs run at the gate tolerance (--eps-gate, 1e-13), where the
+mixed leg's u, v, w and p agree with the FP64 leg's to the 1e-8 gate with
+margin (at 1e-12 the pressures agree to 2.5e-9 to 2.7e-8 depending on the
+step; at the workload's 1e-10 the FP64 BiCGSTAB's own pressure is 3e-6 from
+the tight reference, the mixed leg's 2e-7).  The fp64 and mixed legs also
+run at --eps-config (1e-10).  Each leg: W warm-up steps, K timed steps (wall
+clock), then one step with HIP events around every solver kernel for the
+per-kernel roofline.  Prints one JSON line:
+  value        prism-cell-steps/s of the mixed leg at the gate tolerance
+               = interior prisms x K / wall
+  fp64, fp64_cg, mixed   the legs at the gate tolerance; mixed_over_fp64 and
+               mixed_over_fp64_cg their ratios
   rel_err      relative L2 error of u, v, w, p (interior) of the mixed leg
-               against the FP64 leg after the same steps, and both legs' max
-               |cell divergence| and CFL number (glob_cfl on the device)
+               against the FP64 leg after the same steps, both legs' max
+               |cell divergence| and CFL number (glob_cfl on the device);
+               gate_passed: every field within 1e-8
+  at_config_eps  the fp64 and mixed legs at the workload's 1e-10
+  vs_ref       each leg's relative errors against the tight reference
   roofline     the dominant kernel of the mixed leg: algorithmic bytes per
                launch / mean launch time (HIP events on the library's stream)
   cpu_baseline the oracle (CPU restatement of the reference, 1 core) on a
@@ -61,7 +73,7 @@ WORKLOAD = {
 }
 
 
-def cpu_baseline(case, mean_iters: float, cap: int = 20, parallel: bool = False) -> dict:
+def cpu_baseline(case, mean_iters: float, cap: int = 100, parallel: bool = False) -> dict:
     """One projection stage of the oracle on the full mesh with BiCGSTAB capped
     at `cap` iterations; the step time is extrapolated with the GPU's mean
     iteration count per solve (FP64 leg).  parallel: the oracle built with
@@ -92,26 +104,34 @@ def cpu_baseline(case, mean_iters: float, cap: int = 20, parallel: bool = False)
             "s_per_step": step_s}
 
 
-# "mixed" is the library's default form: single-reduction FP32 CG with the
-# 4-layer block-Jacobi preconditioner and reliable updates; "mixed_line" the
-# same with the line preconditioner (whole columns), "mixed_point" with point
-# Jacobi; "mixed_cg2" the point-Jacobi CG in two kernels per iteration
-MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1), "mixed_cascade": (2, 1e-5),
-               "mixed_restart": (1, 1e-5), "mixed_reliable": (0, 0.1)}
+# the mixed forms (refinement, reliable-update delta): "mixed" is the
+# library's default, single-reduction FP32 CG with the 4-layer block-Jacobi
+# preconditioner; "mixed_line" the same with the line preconditioner (whole
+# columns), "mixed_point" with point Jacobi; "mixed_cg2" the point-Jacobi CG
+# in two kernels per iteration
+MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1)}
 # the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
 # writes e, rt, p (units of 4 B per interior prism; the few X launches that
 # form a true FP64 residual move more)
 CG_UNITS = {"w": 3, "x": 7}
+# the single-reduction CG: one kernel per iteration reading and writing rt,
+# om, sg (6 units), and every other iteration also reading the last u, p and
+# e and writing p and e (11 units): 8.5 units a launch on average; its rare
+# true-residual and product launches are counted with it
+CGS_UNITS = 8.5
+CGS_KERNEL = {4: "k_cgs<float, 0>", 5: "k_cgs<float, 1>", 6: "k_cgs<float, 2>"}
 
 
-def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
+def leg(case, name: str, steps: int, warmup: int, eps: float, max_iters=None) -> dict:
+    """One leg: `name` is fp64 (BiCGSTAB), fp64_cg (FP64 single-reduction
+    CG) or one of MIXED_MODES, with the pressure solves stopped at eps."""
     from nsx.device import Context
 
-    precision = 0 if name == "fp64" else 1
+    precision = 0 if name == "fp64" else (2 if name == "fp64_cg" else 1)
     ctx = Context(case)
-    if max_iters:
-        ctx.set_max_iters(max_iters)
-    if precision:
+    p = case.params
+    ctx.set_params(eps=eps, max_iters=max_iters or p.MaxIters)
+    if precision == 1:
         ctx.set_precision(1, *MIXED_MODES[name])
     ctx.init_state()
     if warmup:
@@ -129,15 +149,13 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
     ctx.timing(False)
     n = case.n_interior
     kernels = {}
-    for width, prefix in ((8, "bicg_"), (4, "bicg32_")):
-        for ph, units in PHASE_UNITS.items():
-            ms, launches = ctx.timer(prefix + ph)
-            if launches:
-                avg = ms / launches
-                b = units * width * n
-                kernels[prefix + ph] = {"kernel": KERNEL_NAMES[ph] + ("<float>" if width == 4 else "<double>"),
-                                        "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
-                                        "GBps": b / (avg * 1e-3) / 1e9}
+    for ph, units in PHASE_UNITS.items():
+        ms, launches = ctx.timer("bicg_" + ph)
+        if launches:
+            avg = ms / launches
+            b = units * 8 * n
+            kernels["bicg_" + ph] = {"kernel": KERNEL_NAMES[ph] + "<double>", "launches": launches,
+                                     "avg_us": avg * 1e3, "bytes_per_launch": b, "GBps": b / (avg * 1e-3) / 1e9}
     for ph, units in CG_UNITS.items():
         ms, launches = ctx.timer("cg32_" + ph)
         if launches:
@@ -145,16 +163,13 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
             b = units * 4 * n
             kernels["cg32_" + ph] = {"kernel": "k_cg_" + ph, "launches": launches, "avg_us": avg * 1e3,
                                      "bytes_per_launch": b, "GBps": b / (avg * 1e-3) / 1e9}
-    # the single-reduction CG: one kernel per iteration reading and writing
-    # rt, om, sg (6 units), and every other iteration also reading the last
-    # u, p and e and writing p and e (11 units): 8.5 units a launch on
-    # average; its rare true-residual and product launches are counted with it
-    ms, launches = ctx.timer("cg32_sr")
-    if launches:
-        avg = ms / launches
-        b = 34 * n
-        kname = "k_cgs<%d>" % {4: 0, 5: 1, 6: 2}[MIXED_MODES[name][0]]
-        kernels["cg32_sr"] = {"kernel": kname, "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
+    for tname, width in (("cg32_sr", 4), ("cg64_sr", 8)):
+        ms, launches = ctx.timer(tname)
+        if launches:
+            avg = ms / launches
+            b = int(CGS_UNITS * width * n)
+            kname = "k_cgs<double, 1>" if width == 8 else CGS_KERNEL[MIXED_MODES[name][0]]
+            kernels[tname] = {"kernel": kname, "launches": launches, "avg_us": avg * 1e3, "bytes_per_launch": b,
                               "GBps": b / (avg * 1e-3) / 1e9}
     stage_ms, stage_n = ctx.timer("stage")
     fields = {f: np.asarray(ctx.get(f))[:case.N_CELL0, 1:-1].copy() for f in ("UFN", "VFN", "WFN", "PFN")}
@@ -163,12 +178,20 @@ def leg(case, name: str, steps: int, warmup: int, max_iters=None) -> dict:
     fields["div"] = abs(cfl["flux_max"])
     fields["cfl"] = cfl["cfl"]
     ctx.close()
-    return {"value": n * steps / wall, "ms_per_step": wall / steps * 1e3, "wall": wall,
+    return {"eps": eps, "value": n * steps / wall, "ms_per_step": wall / steps * 1e3, "wall": wall,
             "mean_iters_per_solve": iters / max(solves, 1), "solves": solves, "exit": info["exit"],
             "last_iters_fp32": info.get("iters_fp32", 0),
-            "last_e1": info["e1"], "last_e2": info["e2"], "outer_passes_last": info.get("outer_passes", 0),
+            "last_e1": info["e1"], "last_e2": info["e2"], "updates_last": info.get("outer_passes", 0),
             "kernels": kernels, "non_solver_ms_per_stage": stage_ms / max(stage_n, 1) * 2 if stage_n else None,
             "fields": fields}
+
+
+def field_errors(fb: dict, fa: dict) -> dict:
+    """relative L2 errors of u, v, w, p of leg b against leg a (interior prisms)"""
+    err = {}
+    for f, nm in (("UFN", "u"), ("VFN", "v"), ("WFN", "w"), ("PFN", "p")):
+        err[nm] = float(np.linalg.norm(fb[f] - fa[f]) / max(np.linalg.norm(fa[f]), 1e-300))
+    return err
 
 
 def traffic_of(cfg: int, kernel):
@@ -178,10 +201,14 @@ def traffic_of(cfg: int, kernel):
         return None
     from nsx.profsum import read_traffic
 
-    t = read_traffic(os.path.join(REPO, "profiles", f"bicgstab_traffic_channel_{TAG[cfg]}.txt"))
-    if kernel not in t:
+    t = read_traffic(os.path.join(REPO, "profiles", f"solver_traffic_channel_{TAG[cfg]}.txt"))
+    key = kernel.replace(" ", "")
+    if key not in t:
         return None
-    return t[kernel][0] + t[kernel][1]
+    return t[key][0] + t[key][1]
+
+
+GATE = 1e-8
 
 
 def main() -> int:
@@ -192,10 +219,15 @@ def main() -> int:
     ap.add_argument("--cfg", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--legs", default="fp64,mixed",
-                    help="comma list of fp64, mixed, mixed_cascade, mixed_restart, mixed_reliable")
+    ap.add_argument("--eps-gate", type=float, default=1e-13,
+                    help="pressure-solve tolerance of the headline legs (fp64, fp64_cg, mixed)")
+    ap.add_argument("--eps-config", type=float, default=None,
+                    help="the workload's own tolerance (default: the case's, 1e-10); 0 skips those legs")
+    ap.add_argument("--eps-ref", type=float, default=1e-14, help="tolerance of the tight reference; 0 skips it")
+    ap.add_argument("--legs", default="fp64,fp64_cg,mixed",
+                    help="comma list of the gate legs: fp64, fp64_cg, mixed, mixed_line, mixed_point, mixed_cg2")
     ap.add_argument("--max-iters", type=int, default=None,
-                    help="cap BiCGSTAB iterations (profiling runs only: the line is then marked capped)")
+                    help="cap the solver iterations (profiling runs only: the line is then marked capped)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy one cell field with k_copy (the known-size copy for HBM counter calibration)")
     args = ap.parse_args()
@@ -209,6 +241,7 @@ def main() -> int:
     cfg = args.cfg
     steps = args.steps if args.steps is not None else {0: 10, 1: 20, 2: 10, 3: 20, 4: 3}[cfg]
     case = config_case(cfg, seed=args.seed)
+    eps_cfg = case.params.eps if args.eps_config is None else args.eps_config
     legs = [x.strip() for x in args.legs.split(",") if x.strip()]
     if args.calibrate_copy:
         from nsx.device import Context
@@ -217,11 +250,21 @@ def main() -> int:
         c0.close()
         ns = (case.NZ + 3) // 4 * 4
         print(f"# k_copy calibration: {case.N_CELL0 * ns} doubles", file=sys.stderr, flush=True)
-    res = {}
-    for name in legs:
-        res[name] = leg(case, name, steps, args.warmup, args.max_iters)
-        print(f"# leg {name}: {res[name]['value']:.4g} prism-cell-steps/s, {res[name]['ms_per_step']:.1f} ms/step, "
-              f"{res[name]['mean_iters_per_solve']:.1f} iterations/solve", file=sys.stderr, flush=True)
+
+    def run(name, eps):
+        r = leg(case, name, steps, args.warmup, eps, args.max_iters)
+        print(f"# leg {name} at eps {eps:g}: {r['value']:.4g} prism-cell-steps/s, {r['ms_per_step']:.1f} ms/step, "
+              f"{r['mean_iters_per_solve']:.1f} iterations/solve", file=sys.stderr, flush=True)
+        return r
+
+    res = {name: run(name, args.eps_gate) for name in legs}
+    at_cfg = {}
+    if eps_cfg > 0 and eps_cfg != args.eps_gate:
+        for name in ("fp64", "mixed"):
+            if name in legs:
+                at_cfg[name] = run(name, eps_cfg)
+    ref = run("fp64_cg", args.eps_ref) if args.eps_ref > 0 else None
+
     head = res.get("mixed") or res.get("fp64") or next(iter(res.values()))
     kern = head["kernels"]
     dom_name = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"]) if kern else None
@@ -237,12 +280,13 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32+f64 (block-Jacobi single-reduction FP32 CG in FP64 reliable-update refinement)" if "mixed" in res
+        "dtype": ("f32+f64 (block-Jacobi single-reduction FP32 CG with FP64 reliable updates)" if "mixed" in res
                   else ("f64" if "fp64" in res else "f32+f64")),
         "data": "synthetic",
         "config": {"workload": f"channel {WORKLOAD[cfg]}", "prisms": case.n_interior,
-                   "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": case.params.eps,
-                   "reliable_update_delta": MIXED_MODES["mixed"][1], "seed": args.seed},
+                   "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": args.eps_gate,
+                   "ppe_tolerance_workload": case.params.eps, "reliable_update_delta": MIXED_MODES["mixed"][1],
+                   "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(cfg, dom["kernel"]),
                      "avg_launch_us": dom["avg_us"], "bytes_per_launch": dom["bytes_per_launch"]},
@@ -251,20 +295,45 @@ def main() -> int:
         out[name] = {k: v for k, v in r.items() if k != "fields"}
     if args.max_iters:
         out["capped_max_iters"] = args.max_iters
-    for name in res:
-        if "fp64" not in res or not name.startswith("mixed"):
-            continue
-        sfx = "" if name == "mixed" else name[5:]
-        out["mixed_over_fp64" + sfx] = res[name]["value"] / res["fp64"]["value"]
-        fa, fb = res["fp64"]["fields"], res[name]["fields"]
-        err = {}
-        for f, nm in (("UFN", "u"), ("VFN", "v"), ("WFN", "w"), ("PFN", "p")):
-            err[nm] = float(np.linalg.norm(fb[f] - fa[f]) / max(np.linalg.norm(fa[f]), 1e-300))
-        err["max_div_fp64"] = fa["div"]
-        err["max_div_mixed"] = fb["div"]
-        err["cfl_fp64"] = fa["cfl"]
-        err["cfl_mixed"] = fb["cfl"]
-        out["rel_err" + sfx] = err
+
+    def compare(rs, tag):
+        """ratios and errors of the mixed forms against fp64 (and fp64_cg) within one set of legs"""
+        o = {}
+        for name in rs:
+            if not name.startswith("mixed") or "fp64" not in rs:
+                continue
+            sfx = "" if name == "mixed" else name[5:]
+            o["mixed_over_fp64" + sfx] = rs[name]["value"] / rs["fp64"]["value"]
+            if "fp64_cg" in rs:
+                o["mixed_over_fp64_cg" + sfx] = rs[name]["value"] / rs["fp64_cg"]["value"]
+            fa, fb = rs["fp64"]["fields"], rs[name]["fields"]
+            err = field_errors(fb, fa)
+            err["max_div_fp64"] = fa["div"]
+            err["max_div_mixed"] = fb["div"]
+            err["cfl_fp64"] = fa["cfl"]
+            err["cfl_mixed"] = fb["cfl"]
+            err["gate"] = GATE
+            err["gate_passed"] = bool(max(err["u"], err["v"], err["w"], err["p"]) <= GATE)
+            o["rel_err" + sfx] = err
+        if "fp64_cg" in rs and "fp64" in rs:
+            o["fp64_cg_over_fp64"] = rs["fp64_cg"]["value"] / rs["fp64"]["value"]
+        return o
+
+    out.update(compare(res, "gate"))
+    if at_cfg:
+        sub = {"eps": eps_cfg}
+        for name, r in at_cfg.items():
+            sub[name] = {k: v for k, v in r.items() if k not in ("fields", "kernels")}
+        sub.update(compare(at_cfg, "config"))
+        out["at_config_eps"] = sub
+    if ref is not None:
+        vs = {"eps": args.eps_ref, "solver": "fp64_cg", "ms_per_step": ref["ms_per_step"],
+              "mean_iters_per_solve": ref["mean_iters_per_solve"], "exit": ref["exit"]}
+        for name, r in res.items():
+            vs[f"{name}@{args.eps_gate:g}"] = field_errors(r["fields"], ref["fields"])
+        for name, r in at_cfg.items():
+            vs[f"{name}@{eps_cfg:g}"] = field_errors(r["fields"], ref["fields"])
+        out["vs_ref"] = vs
     if not args.no_cpu:
         base = res.get("fp64", head)
         cb = cpu_baseline(case, base["mean_iters_per_solve"])

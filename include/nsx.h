@@ -73,8 +73,13 @@ typedef struct nsx_solve_info {
 /* ---- precision of the step --------------------------------------------- */
 enum nsx_precision {
   NSX_PREC_FP64 = 0,  /* the reference's arithmetic */
-  NSX_PREC_MIXED = 1  /* FP32 BiCGSTAB (SpMV, preconditioner, updates) inside an FP64
-                          iterative refinement of the PPE; inner products and norms in FP64 */
+  NSX_PREC_MIXED = 1, /* the precision form: FP32 conjugate gradients (SpMV, preconditioner,
+                          updates; by default the single-reduction form with the 4-layer
+                          block-Jacobi preconditioner) with reliable updates against the FP64
+                          solution; true residuals, inner products and norms in FP64 */
+  NSX_PREC_FP64_CG = 2 /* the precision form's solver with FP64 vectors: single-reduction CG
+                          with the 4-layer block-Jacobi preconditioner and reliable updates,
+                          all in FP64 (the refinement set by nsx_set_precision is not used) */
 };
 
 /* ---- lifecycle ----------------------------------------------------------- */
@@ -265,39 +270,32 @@ int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int
 /* Arithmetic of the pressure solve in nsx_hydro / nsx_bicgstab (nsx_step takes
  * its own precision and uses the refinement set here).  NSX_PREC_MIXED runs
  * the Krylov vectors, SpMV and preconditioner in FP32 and keeps the solution,
- * the refinement residual, inner products and norms in FP64; the solve stops
- * when the true FP64 residual meets the reference's rule with eps
- * (nsx_set_params).  Refinement (default NSX_REFINE_CGSR_BLOCK, delta 0.1):
- *   NSX_REFINE_RELIABLE  one FP32 iteration whose residual is replaced by the
- *                        true FP64 residual whenever it has fallen by tol
- *                        (delta, e.g. 0.1); max_passes unused;
- *   NSX_REFINE_RESTART   defect correction: per pass an FP32 BiCGSTAB from
- *                        e = 0 on A e = b - A x to min(e1,e2) < tol (e.g.
- *                        1e-5), then x += e; at most max_passes passes;
- *   NSX_REFINE_CASCADE   one such FP32 pass, then the FP64 BiCGSTAB of
- *                        NSX_PREC_FP64 started from its result;
- *   NSX_REFINE_CG        FP32 Jacobi-preconditioned conjugate gradients on
- *                        the correction, whose residual is replaced by the
- *                        true FP64 residual whenever it has fallen by tol
- *                        (delta, e.g. 0.1); max_passes unused.  The operator
- *                        is symmetric negative semi-definite, and CG's
- *                        smooth residual keeps FP32 from stalling;
- *   NSX_REFINE_CGSR      the same CG in the single-reduction form of
- *                        Chronopoulos and Gear: one kernel and one global
- *                        reduction per iteration; tol as NSX_REFINE_CG, at
- *                        least 0.05 (its FP32 recurrences drift faster);
+ * the true residual, inner products and norms in FP64; the solve stops when
+ * the true FP64 residual meets the reference's rule with eps
+ * (nsx_set_params).  The FP32 solver is conjugate gradients on the
+ * correction e; whenever its recursive residual has fallen by tol (delta,
+ * e.g. 0.1) since the last true residual, or meets the rule, the true FP64
+ * residual of x + e replaces it and e is folded into x (reliable updates).
+ * The operator is symmetric negative semi-definite, and CG's smooth residual
+ * keeps FP32 from stalling.  A CG that reaches its cap (2 MaxIters + 100
+ * iterations) or breaks down leaves x at its last update and the FP64
+ * BiCGSTAB finishes.  Refinement (default NSX_REFINE_CGSR_BLOCK, delta 0.1):
+ *   NSX_REFINE_CG          point-Jacobi CG, two kernels per iteration;
+ *   NSX_REFINE_CGSR        the same CG in the single-reduction form of
+ *                          Chronopoulos and Gear: one kernel and one global
+ *                          reduction per iteration; tol at least 0.05 (its
+ *                          FP32 recurrences drift faster);
  *   NSX_REFINE_CGSR_BLOCK  the same with a block-Jacobi preconditioner: A
- *                        restricted to each group of four consecutive
- *                        layers of a column (a 4x4 tridiagonal solve in
- *                        the kernel, no extra memory traffic);
+ *                          restricted to each group of four consecutive
+ *                          layers of a column (a 4x4 tridiagonal solve in
+ *                          the kernel, no extra memory traffic);
  *   NSX_REFINE_CGSR_LINE   the same with a line preconditioner: A restricted
- *                        to the interior layers of each column (a
- *                        tridiagonal solve over the column in the kernel's
- *                        shared memory; columns of at most 1024 layers). */
+ *                          to the interior layers of each column (a
+ *                          tridiagonal solve over the column in the kernel's
+ *                          shared memory; columns of at most 1024 layers).
+ * Values 0..2 (FP32 BiCGSTAB forms) are refused: FP32 BiCGSTAB breaks down on
+ * this operator.  max_passes is unused (at least 1). */
 enum nsx_refine {
-  NSX_REFINE_RELIABLE = 0,
-  NSX_REFINE_RESTART = 1,
-  NSX_REFINE_CASCADE = 2,
   NSX_REFINE_CG = 3,
   NSX_REFINE_CGSR = 4,
   NSX_REFINE_CGSR_BLOCK = 5,
@@ -308,10 +306,9 @@ int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int m
 /* ---- timing ---------------------------------------------------------------- */
 /* per-kernel device time accumulated with HIP events since the last reset;
  * names: "bicg_a", "bicg_b", "bicg_c" (the three BiCGSTAB kernels in FP64,
- * iterations that ran only), "bicg32_a" .. "bicg32_c" (the same kernels of the
- * FP32 solve), "rel_u" (the reliable-update kernel), "cg32_w", "cg32_x"
- * (the two kernels of an FP32 CG iteration), "cg32_sr" (the kernel of the
- * single-reduction FP32 CG, every launch) and "stage" (all non-solver
+ * iterations that ran only), "cg32_w", "cg32_x" (the two kernels of an FP32
+ * CG iteration), "cg32_sr" (the kernel of the single-reduction FP32 CG, every
+ * launch that ran), "cg64_sr" (its FP64 instance) and "stage" (all non-solver
  * kernels of a projection stage) */
 int nsx_timing_enable(nsx_ctx *ctx, int on);
 int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, long long *launches);

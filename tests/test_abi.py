@@ -51,3 +51,20 @@ def test_setup_library_exports_every_symbol():
     lib = ctypes.CDLL(os.path.join(REPO, "3d-channel-flow-code_amd", "nsx", "_build", "libnsx_host.so"))
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_format_real_rejects_out_of_range_widths():
+    """nsx_format_real builds an Ew.d field in fixed buffers: digit counts
+    outside 1..30 and widths outside 0..63 are refused, not written."""
+    lib = ctypes.CDLL(LIB)
+    fr = lib.nsx_format_real
+    fr.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    fr.restype = ctypes.c_int
+    buf = ctypes.create_string_buffer(128)
+    assert fr(1.5, 0, 12, 5, buf, 128) == 0 and buf.value == b" 0.15000E+01"
+    assert fr(0.0, 0, 12, 5, buf, 128) == 0 and buf.value == b" 0.00000E+00"
+    for w, d in ((12, 0), (12, -3), (12, 31), (12, 40), (-1, 5), (64, 5), (200, 5)):
+        assert fr(1.5, 0, w, d, buf, 128) == -1, (w, d)
+        assert fr(0.0, 0, w, d, buf, 128) == -1, (w, d)
+    assert fr(1.5, 2, 12, 5, buf, 128) == -1
+    assert fr(1.5, 1, 0, 0, buf, 128) == 0 and buf.value == b"1.5"
