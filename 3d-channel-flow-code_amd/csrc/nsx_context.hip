@@ -658,6 +658,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     if (!*rows[q]) return nsx_fail(ctx, -1, "reduction buffer %d is not allocated", q);
   if ((rc = dev_alloc(ctx, &ctx->cfl_part, (size_t)glob_cfl_rows() * MAXGRID))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->cfl_out, 16))) return rc;
+  // the static scratch of nsx_glob_cfl_noalloc on this device
+  {
+    const int e = glob_cfl_scratch_init(ctx->dev);
+    if (e != 0) return nsx_fail_hip(ctx, (hipError_t)e, "hipGetSymbolAddress of the glob_cfl scratch", __LINE__);
+  }
 
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
@@ -1341,6 +1346,42 @@ extern "C" int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out
                          ctx->cfl_out));
   double r[13];
   HIPCHECK(hipMemcpyAsync(r, ctx->cfl_out, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHECK(hipStreamSynchronize(ctx->stream));
+  out->umax = r[0];
+  out->vmax = r[1];
+  out->wmax = r[2];
+  out->cfl = r[3];
+  out->aveu = r[4];
+  out->avev = r[5];
+  out->avew = r[6];
+  out->flux_total = r[7];
+  out->flux_max = r[8];
+  out->flux_bc = r[9];
+  out->flux_bot = r[10];
+  out->flux_top = r[11];
+  out->flux_side = r[12];
+  return 0;
+}
+
+// glob_cfl with its partial sums and outputs in the library's static
+// per-device scratch instead of the context's buffers: no allocation of any
+// kind, for callers that cannot allocate.  Calls that overlap on one GPU race
+// on the shared scratch.
+extern "C" int nsx_glob_cfl_noalloc(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out) {
+  NEED_COMMIT(ctx);
+  if (!out) return nsx_fail(ctx, -1, "nsx_glob_cfl_noalloc: out is NULL");
+  if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL)) return -1;
+  const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                           ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  double *res = nullptr;
+  const int rc = launch_glob_cfl_scratch(ctx->g, ctx->stream, ctx->dev, ctx->slot[u], ctx->slot[v], ctx->slot[w],
+                                         face, &res);
+  if (rc == -1)
+    return nsx_fail(ctx, -1, "nsx_glob_cfl_noalloc: a grid of %d blocks exceeds the scratch's 2048", ctx->g.grid);
+  if (rc == -2) return nsx_fail(ctx, -1, "nsx_glob_cfl_noalloc: the scratch of device %d is not set up", ctx->dev);
+  if (rc) return nsx_fail_hip(ctx, (hipError_t)rc, "launch_glob_cfl_scratch", __LINE__);
+  double r[13];
+  HIPCHECK(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   out->umax = r[0];
   out->vmax = r[1];

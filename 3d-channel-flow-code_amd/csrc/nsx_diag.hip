@@ -10,6 +10,8 @@
 // (k-1)*N_CELL0 + i with each candidate.  Sums are in a different order than
 // the reference's loops (round-off only).
 // Bytes/prism: read u, v, w and 5 face velocities = 8 x 8 B.
+#include <mutex>
+
 #include "nsx_kernels.h"
 
 namespace {
@@ -178,9 +180,60 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_glob_cfl_final(Geo g, const doubl
   out[12] = t[6];
 }
 
+// The scratch of nsx_glob_cfl_noalloc: the per-block partial rows of one
+// pass (NROWS rows of at most CFL_MAXGRID blocks) and the 13 outputs (16
+// slots), 262,272 bytes.  Each GPU the library runs on holds one copy until
+// the process ends; every call on a GPU shares it, so calls that overlap on
+// one GPU race.  Host code never passes `scratch` itself to a kernel (in
+// host code the name is a stand-in, not a device address): it passes the
+// device's address that glob_cfl_scratch_init obtained.
+constexpr int CFL_MAXGRID = 2048;
+constexpr size_t CFL_SCRATCH = (size_t)NROWS * CFL_MAXGRID + 16;
+static __device__ __align__(256) double scratch[CFL_SCRATCH];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[MAX_DEVICES] = {};
+
 }  // namespace
 
 int glob_cfl_rows() { return NROWS; }
+
+// The device address of `scratch` on device dev, obtained once per device
+// with dev current (the library's set-up, nsx_commit, calls this); 0 or a
+// hipError_t.
+int glob_cfl_scratch_init(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[dev]) return 0;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return (int)e;
+  if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return (int)e;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess) return (int)e;
+  scratch_ptr[dev] = (double *)p;
+  return 0;
+}
+
+// glob_cfl into the scratch of device dev: partial rows, then the outputs
+// at out_slot; fails when the grid exceeds the scratch (-1) or the scratch
+// of dev was not set up (-2)
+int launch_glob_cfl_scratch(const Geo &g, hipStream_t s, int dev, const double *u, const double *v, const double *w,
+                            const double *const face[5], double **out_slot) {
+  if (g.grid > CFL_MAXGRID) return -1;
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (dev >= 0 && dev < MAX_DEVICES) base = scratch_ptr[dev];
+  }
+  if (!base) return -2;
+  double *part = base, *out = base + (size_t)NROWS * CFL_MAXGRID;
+  *out_slot = out;
+  return launch_glob_cfl(g, s, u, v, w, face, part, out);
+}
 
 int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w,
                     const double *const face[5], double *part, double *out) {

@@ -109,6 +109,12 @@ int launch_plane_sums(const Geo &g, hipStream_t s, StatsBufs sm, int ns, double 
 int glob_cfl_rows();
 int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w,
                     const double *const face[5], double *part, double *out);
+// the same into the library's static per-device scratch (nsx_diag.hip);
+// glob_cfl_scratch_init(dev) must have run once for dev; returns -1 when the
+// grid exceeds the scratch, -2 when it was not set up, else a hipError_t
+int glob_cfl_scratch_init(int dev);
+int launch_glob_cfl_scratch(const Geo &g, hipStream_t s, int dev, const double *u, const double *v, const double *w,
+                            const double *const face[5], double **out_slot);
 
 // ---- solver
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,

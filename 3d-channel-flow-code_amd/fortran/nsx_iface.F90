@@ -254,6 +254,13 @@
           integer(c_int), value :: u, v, w
           type(nsx_cfl_info) :: info
         end function
+        integer(c_int) function nsx_glob_cfl_noalloc(ctx, u, v, w, info) &
+             bind(C, name="nsx_glob_cfl_noalloc")
+          import :: c_int, c_ptr, nsx_cfl_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: u, v, w
+          type(nsx_cfl_info) :: info
+        end function
         integer(c_int) function nsx_sm_init(ctx)                         &
              bind(C, name="nsx_sm_init")
           import :: c_int, c_ptr

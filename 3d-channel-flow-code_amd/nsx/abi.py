@@ -36,7 +36,7 @@ EXPORTS = [
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
-    "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
     "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
     "nsx_write_plane_stats", "nsx_format_real", "nsx_savetec_vertex", "nsx_savetec_center", "nsx_write_tec_vertex",
     "nsx_write_tec_center", "nsx_save_paraview_vertex", "nsx_savetec_vc", "nsx_write_vtk_vertex", "nsx_write_tec_vc",
@@ -99,6 +99,7 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_newu3d": [c_vp, c_int, c_int],
         "nsx_set_precision": [c_vp, c_int, c_int, c_dbl, c_int],
         "nsx_glob_cfl": [c_vp, c_int, c_int, c_int, P(CflInfo)],
+        "nsx_glob_cfl_noalloc": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_bicgstab_adjoint": [c_vp, c_int, c_int, P(SolveInfo)],
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],

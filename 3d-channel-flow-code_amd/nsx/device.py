@@ -248,6 +248,17 @@ class Context:
         self._check(self.lib.nsx_glob_cfl(self.h, *ids, ctypes.byref(out)))
         return out.as_dict()
 
+    def glob_cfl_noalloc(self, u=None, v=None, w=None) -> dict:
+        """glob_cfl through nsx_glob_cfl_noalloc: the same values, with the
+        partial sums in the library's static per-device scratch."""
+        if u is None:
+            ids = (F["UFN"], F["VFN"], F["WFN"])
+        else:
+            ids = self._cells(u, v, w)
+        out = CflInfo()
+        self._check(self.lib.nsx_glob_cfl_noalloc(self.h, *ids, ctypes.byref(out)))
+        return out.as_dict()
+
     # -- channel statistics (stats.F90)
     def sm_init(self) -> None:
         """sm_init (stats.F90:34-84) on the device accumulators."""

@@ -192,6 +192,14 @@ typedef struct nsx_cfl_info {
   double flux_bot, flux_top, flux_side; /* through the bottom/top walls and the side (non-interior) faces */
 } nsx_cfl_info;
 int nsx_glob_cfl(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
+/* No counterpart in the reference: nsx_glob_cfl for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  It allocates nothing: its partial sums and outputs
+ * live in a static per-device scratch of the library (262,272 bytes: 16 rows
+ * of at most 2048 blocks, and the outputs); a context whose grid exceeds 2048
+ * blocks is refused.  The scratch is shared by every call on a GPU and keeps
+ * the last call's contents, so calls that overlap on one GPU race. */
+int nsx_glob_cfl_noalloc(nsx_ctx *ctx, int u, int v, int w, nsx_cfl_info *out);
 
 /* ---- channel statistics (stats.F90) ---------------------------------------- */
 /* The twelve accumulators of module stats (module_stats.F90), resident on the
