@@ -23,6 +23,28 @@ struct Timer {
   long long n = 0;
 };
 
+// Work fields of the differentiation form, allocated on its first use: the
+// checkpoints, the stage inputs of one segment, the bars of the adjoint
+// sweep and the tangents of the tangent-linear run.  A state is nine fields:
+// u, v, w, p and the five face velocities.
+struct AdjWork {
+  std::vector<double *> ckpt;  // checkpoints, 9 fields each
+  std::vector<double *> seg;   // per step of a segment: S^n then S^a, 9 fields each
+  double *fin[9] = {};         // the resident state at the end of the forward run
+  double *Sb[9] = {};          // bars of the state at the current level
+  double *Yb[4] = {}, *Fb[5] = {};
+  double *Xb[2][4] = {}, *Fib[2][5] = {};
+  double *usb[3] = {}, *eb = nullptr, *lam = nullptr, *E = nullptr;
+  double *vb[3] = {}, *pvb = nullptr, *pvb2 = nullptr, *valb = nullptr;  // vertex fields
+  double *V[3] = {};                                                     // vertex fields
+  // tangents: u, v, w, p by pool set, the faces, and the stage's work fields
+  double *T[3][4] = {}, *TF[5] = {}, *TV[3] = {}, *Tpv = nullptr, *TE = nullptr, *Tus[3] = {},
+         *Trhs = nullptr, *zero = nullptr;
+  double *J = nullptr, *Jpart = nullptr;  // two accumulators, 64 partials
+  bool ready = false;
+  bool unconverged = false;  // a solve of the current run reached MaxIters
+};
+
 }  // namespace
 
 struct nsx_ctx {
@@ -59,6 +81,9 @@ struct nsx_ctx {
   // NSX_PREC_FP64_CG: the single-reduction CG's second buffer set, with the
   // BiCGSTAB vectors above as its first (a solve uses one solver at a time)
   double *scg[4] = {};
+  // differentiation form: the body-force switch and its work fields
+  bool body_force = false;
+  AdjWork adj;
   SolverState *st_dev = nullptr, *st_host = nullptr;
   Partials part{};
   nsx_solve_info info{};
@@ -583,7 +608,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   for (int s = 0; s < 3; ++s)
     for (int q = 0; q < 4; ++q)
       if ((rc = cell(&ctx->pool_cell[s][q]))) return rc;
-  int ids_cell[] = {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY};
+  int ids_cell[] = {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY, NSX_F_FORCE};
   for (int id : ids_cell) {
     if ((rc = cell(&ctx->slot[id]))) return rc;
     ctx->kind[id] = FK_CELL;
@@ -1025,7 +1050,8 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                            ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   double *hist[3] = {ctx->slot[NSX_F_RHSUF], ctx->slot[NSX_F_RHSVF], ctx->slot[NSX_F_RHSWF]};
-  LAUNCH(launch_momentum(g, s, uvw, vtx, face, ctx->slot[NSX_F_EDDY], nullptr, hist, nullptr, ctx->ustar, c1, c2));
+  LAUNCH(launch_momentum(g, s, uvw, vtx, face, ctx->slot[NSX_F_EDDY], nullptr, hist, nullptr, ctx->ustar, c1, c2,
+                         ctx->body_force ? ctx->slot[NSX_F_FORCE] : nullptr));
   // update_rhsp's BCglobalP(pf, pfv, 2)
   const double *pin[3] = {X[3], nullptr, nullptr};
   double *pfv[3] = {ctx->slot[NSX_F_PFV], nullptr, nullptr};
@@ -1558,6 +1584,462 @@ extern "C" int nsx_savetec_vc(nsx_ctx *ctx, const char *dir, int counter, double
                        yct, zct, fv.data(), fv.data() + nv, fv.data() + 2 * nv, fv.data() + 3 * nv, fc.data(),
                        fc.data() + nc, fc.data() + 2 * nc, fc.data() + 3 * nc))
     return nsx_fail(ctx, -5, "nsx_savetec_vc: cannot write VC-%05d.tec in %s", counter, dir);
+  return 0;
+}
+
+// ------------------------------------------------- differentiation form --
+// The derivative of the time-averaged bottom-wall shear of the channel with
+// respect to the per-prism streamwise body force (NSX_F_FORCE), by the
+// discrete adjoint of the RK2 steps (nsmp3D.F90:448-548) and projection
+// stages (hydro.F90:8-501), with checkpoints every few steps and the steps
+// between recomputed; and the same derivative along one direction by the
+// tangent-linear model.  Solves run in FP64 (BiCGSTAB) to eps.
+
+static int adj_field(nsx_ctx *ctx, double **p, int kind) {
+  const size_t n = kind == FK_VERT ? (size_t)ctx->NV * (ctx->NZ - 1) : (size_t)ctx->N0 * ctx->NS;
+  int rc = dev_alloc(ctx, p, n);
+  if (rc) return rc;
+  HIPCHECK(hipMemsetAsync(*p, 0, n * sizeof(double), ctx->stream));
+  return 0;
+}
+
+static int adj_alloc(nsx_ctx *ctx, size_t nckpt, size_t nseg) {
+  AdjWork &W = ctx->adj;
+  int rc;
+  if (!W.ready) {
+    for (int q = 0; q < 9; ++q) {
+      if ((rc = adj_field(ctx, &W.fin[q], FK_CELL)) || (rc = adj_field(ctx, &W.Sb[q], FK_CELL))) return rc;
+    }
+    for (int q = 0; q < 4; ++q) {
+      if ((rc = adj_field(ctx, &W.Yb[q], FK_CELL)) || (rc = adj_field(ctx, &W.Xb[0][q], FK_CELL)) ||
+          (rc = adj_field(ctx, &W.Xb[1][q], FK_CELL)))
+        return rc;
+      for (int t = 0; t < 3; ++t)
+        if ((rc = adj_field(ctx, &W.T[t][q], FK_CELL))) return rc;
+    }
+    for (int q = 0; q < 5; ++q)
+      if ((rc = adj_field(ctx, &W.Fb[q], FK_CELL)) || (rc = adj_field(ctx, &W.Fib[0][q], FK_CELL)) ||
+          (rc = adj_field(ctx, &W.Fib[1][q], FK_CELL)) || (rc = adj_field(ctx, &W.TF[q], FK_CELL)))
+        return rc;
+    for (int q = 0; q < 3; ++q)
+      if ((rc = adj_field(ctx, &W.usb[q], FK_CELL)) || (rc = adj_field(ctx, &W.Tus[q], FK_CELL)) ||
+          (rc = adj_field(ctx, &W.vb[q], FK_VERT)) || (rc = adj_field(ctx, &W.V[q], FK_VERT)) ||
+          (rc = adj_field(ctx, &W.TV[q], FK_VERT)))
+        return rc;
+    if ((rc = adj_field(ctx, &W.eb, FK_CELL)) || (rc = adj_field(ctx, &W.lam, FK_CELL)) ||
+        (rc = adj_field(ctx, &W.E, FK_CELL)) || (rc = adj_field(ctx, &W.TE, FK_CELL)) ||
+        (rc = adj_field(ctx, &W.Trhs, FK_CELL)) || (rc = adj_field(ctx, &W.zero, FK_CELL)) ||
+        (rc = adj_field(ctx, &W.pvb, FK_VERT)) || (rc = adj_field(ctx, &W.pvb2, FK_VERT)) ||
+        (rc = adj_field(ctx, &W.valb, FK_VERT)) || (rc = adj_field(ctx, &W.Tpv, FK_VERT)))
+      return rc;
+    if ((rc = dev_alloc(ctx, &W.J, 2)) || (rc = dev_alloc(ctx, &W.Jpart, 64))) return rc;
+    W.ready = true;
+  }
+  while (W.ckpt.size() < 9 * nckpt) {
+    double *p;
+    if ((rc = adj_field(ctx, &p, FK_CELL))) return rc;
+    W.ckpt.push_back(p);
+  }
+  while (W.seg.size() < 18 * nseg) {
+    double *p;
+    if ((rc = adj_field(ctx, &p, FK_CELL))) return rc;
+    W.seg.push_back(p);
+  }
+  return 0;
+}
+
+static size_t cell_bytes(const nsx_ctx *ctx) { return (size_t)ctx->N0 * ctx->NS * sizeof(double); }
+
+// the state of pool set `set` (u, v, w, p) and the faces into dst[9], or back
+static int save_state(nsx_ctx *ctx, int set, double *const dst[9]) {
+  for (int q = 0; q < 9; ++q) {
+    const double *src = q < 4 ? ctx->pool_cell[set][q] : ctx->slot[NSX_F_U1FACE + q - 4];
+    HIPCHECK(hipMemcpyAsync(dst[q], src, cell_bytes(ctx), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return 0;
+}
+
+static int load_state(nsx_ctx *ctx, int set, const double *const src[9]) {
+  for (int q = 0; q < 9; ++q) {
+    double *dst = q < 4 ? ctx->pool_cell[set][q] : ctx->slot[NSX_F_U1FACE + q - 4];
+    HIPCHECK(hipMemcpyAsync(dst, src[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return 0;
+}
+
+// 1 / (sum of the column areas): the mean over the bottom wall
+static double inv_area(const nsx_ctx *ctx) {
+  double a = 0.0;
+  for (int i = 0; i < ctx->N0; ++i) a += ctx->areaCell[i];
+  return 1.0 / a;
+}
+
+// One RK2 step of the resident state (as nsx_step), storing its input state
+// S^n and the second stage's input S^a when sn / sa are given; J += scale x
+// the bottom-wall shear of the new level
+static int fwd_step(nsx_ctx *ctx, double *const *sn, double *const *sa, double scale) {
+  const int N = ctx->set_n, A = (N + 1) % 3, B = (N + 2) % 3;
+  int rc;
+  if (sn && (rc = save_state(ctx, N, sn))) return rc;
+  if ((rc = stage(ctx, N, A, 0))) return rc;
+  if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) ctx->adj.unconverged = true;
+  if (sa && (rc = save_state(ctx, A, sa))) return rc;
+  if ((rc = stage(ctx, A, B, 0))) return rc;
+  if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) ctx->adj.unconverged = true;
+  const double *un[3] = {ctx->pool_cell[N][0], ctx->pool_cell[N][1], ctx->pool_cell[N][2]};
+  double *unp[3] = {ctx->pool_cell[B][0], ctx->pool_cell[B][1], ctx->pool_cell[B][2]};
+  LAUNCH(launch_rk2avg(ctx->g, ctx->stream, un, unp));
+  ctx->set_n = B;
+  point_slots(ctx, B, A, B);
+  if (scale != 0.0) LAUNCH(launch_wall_shear(ctx->g, ctx->stream, ctx->pool_cell[B][0], scale, ctx->adj.Jpart,
+                                             ctx->adj.J));
+  return 0;
+}
+
+static void vert_acts(const Geo &g, const signed char *act[4]) {
+  act[0] = g.vact_u;
+  act[1] = g.vact_v;
+  act[2] = g.vact_w;
+  act[3] = g.vact_p;
+}
+
+// the stage's vertex values of u, v, w (tagdir 2) and eddy viscosity from
+// its input X (ghost layers filled first, as the stage does)
+static int stage_base(nsx_ctx *ctx, double *const X[4]) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  AdjWork &W = ctx->adj;
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
+  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
+  const double *uvw[3] = {X[0], X[1], X[2]};
+  const int vk[3] = {1, 2, 3};
+  int rc = vertex_of(ctx, 3, uvw, W.V, vk, 2);
+  if (rc) return rc;
+  if (ctx->Iturbu == 1)
+    LAUNCH(launch_les(g, s, X[0], X[1], X[2], W.V[0], W.V[1], W.V[2], W.E));
+  else
+    HIPCHECK(hipMemsetAsync(W.E, 0, cell_bytes(ctx), s));
+  return 0;
+}
+
+// the adjoint of one stage (X, Fin) -> (Y, Fout): from the bars Yb of Y
+// (ghost layers included) and Fb of Fout, the bars Xb of X and Fib of Fin;
+// fb += the bar of the body force.  Yb and Fb are used up.
+static int adj_stage(nsx_ctx *ctx, double *const X[4], const double *const Fin[5], double *const Yb[4],
+                     double *const Fb[5], double *const Xb[4], double *const Fib[5], double *fb) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  AdjWork &W = ctx->adj;
+  const size_t cb = cell_bytes(ctx), vbytes = (size_t)ctx->NV * (ctx->NZ - 1) * sizeof(double);
+  for (int q = 0; q < 4; ++q) HIPCHECK(hipMemsetAsync(Xb[q], 0, cb, s));
+  for (int q = 0; q < 5; ++q) HIPCHECK(hipMemsetAsync(Fib[q], 0, cb, s));
+  for (int q = 0; q < 3; ++q) {
+    HIPCHECK(hipMemsetAsync(W.usb[q], 0, cb, s));
+    HIPCHECK(hipMemsetAsync(W.vb[q], 0, vbytes, s));
+  }
+  HIPCHECK(hipMemsetAsync(W.eb, 0, cb, s));
+  HIPCHECK(hipMemsetAsync(W.pvb, 0, vbytes, s));
+  HIPCHECK(hipMemsetAsync(W.pvb2, 0, vbytes, s));
+  int rc = stage_base(ctx, X);
+  if (rc) return rc;
+  const signed char *act[4];
+  vert_acts(g, act);
+  // F'' = F' - dt grad_n(Yp, Pv'): the face bars pass through
+  LAUNCH(launch_adj_newu(g, s, Fb, Yb[3], W.pvb2));
+  // Pv' = vertex values of Yp
+  LAUNCH(launch_adj_vertex(g, s, W.pvb2, act[3], W.valb, Yb[3]));
+  // Y = U* - dt grad_LSM(Yp), then the ghost layers of Y
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_adj_ghost(g, s, Yb[q], q + 1));
+  const double *yb[3] = {Yb[0], Yb[1], Yb[2]};
+  LAUNCH(launch_adj_corrector(g, s, yb, W.usb, Yb[3]));
+  // Yp = N0 A^+ N1 rhsp with its Neumann ghost layers: rhsp_bar = N0 (A^T)^+ N1 Yp_bar
+  LAUNCH(launch_adj_ghost(g, s, Yb[3], 0));
+  HIPCHECK(hipMemsetAsync(W.lam, 0, cb, s));
+  LAUNCH(launch_gnorm_sum(g, s, Yb[3], ctx->part));
+  ctx->g.trans = 1;
+  rc = solve_fp64(ctx, Yb[3], W.lam, W.lam);
+  ctx->g.trans = 0;
+  if (rc) return rc;
+  // F' = faces of U*, rhsp = div F' / dt - lagged flux of Pv
+  const double *fbc[5] = {Fb[0], Fb[1], Fb[2], Fb[3], Fb[4]};
+  LAUNCH(launch_adj_source(g, s, fbc, W.lam, W.usb, W.pvb));
+  // U* = X + dt r, then the ghost layers of U*
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_adj_ghost(g, s, W.usb[q], q + 1));
+  const double *uvw[3] = {X[0], X[1], X[2]}, *vtx[3] = {W.V[0], W.V[1], W.V[2]};
+  const double *usb[3] = {W.usb[0], W.usb[1], W.usb[2]};
+  double *xb[3] = {Xb[0], Xb[1], Xb[2]};
+  LAUNCH(launch_adj_momentum(g, s, uvw, vtx, Fin, W.E, usb, xb, W.vb, Fib, W.eb, fb));
+  // Pv = vertex values of p
+  LAUNCH(launch_adj_vertex(g, s, W.pvb, act[3], W.valb, Xb[3]));
+  // the eddy viscosity
+  if (ctx->Iturbu == 1) LAUNCH(launch_adj_les(g, s, uvw, vtx, W.eb, xb, W.vb));
+  // V = vertex values of u, v, w
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_adj_vertex(g, s, W.vb[q], act[q], W.valb, Xb[q]));
+  // the ghost layers of X
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_adj_ghost(g, s, Xb[q], q + 1));
+  LAUNCH(launch_adj_ghost(g, s, Xb[3], 0));
+  return 0;
+}
+
+// the adjoint of one RK2 step from its stored input state Sn and second
+// stage input Sa: Sb holds the bars of the new level on entry and those of
+// level n on return; fb += the bar of the body force
+static int adj_step(nsx_ctx *ctx, double *const Sn[9], double *const Sa[9], double *const Sb[9], double *fb) {
+  hipStream_t s = ctx->stream;
+  AdjWork &W = ctx->adj;
+  const size_t nc = (size_t)ctx->N0 * ctx->NS;
+  // u, v, w (n+1) = (u(n) + u_b)/2; p and the faces are the second stage's
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_axpby(s, nc, 0.5, Sb[q], 0.0, nullptr));
+  for (int q = 0; q < 4; ++q) HIPCHECK(hipMemcpyAsync(W.Yb[q], Sb[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  for (int q = 0; q < 5; ++q)
+    HIPCHECK(hipMemcpyAsync(W.Fb[q], Sb[4 + q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  double *Xa[4] = {Sa[0], Sa[1], Sa[2], Sa[3]}, *Xn[4] = {Sn[0], Sn[1], Sn[2], Sn[3]};
+  const double *Fa[5] = {Sa[4], Sa[5], Sa[6], Sa[7], Sa[8]}, *Fn[5] = {Sn[4], Sn[5], Sn[6], Sn[7], Sn[8]};
+  int rc = adj_stage(ctx, Xa, Fa, W.Yb, W.Fb, W.Xb[0], W.Fib[0], fb);
+  if (rc) return rc;
+  if ((rc = adj_stage(ctx, Xn, Fn, W.Xb[0], W.Fib[0], W.Xb[1], W.Fib[1], fb))) return rc;
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_axpby(s, nc, 1.0, Sb[q], 1.0, W.Xb[1][q]));
+  HIPCHECK(hipMemcpyAsync(Sb[3], W.Xb[1][3], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  for (int q = 0; q < 5; ++q)
+    HIPCHECK(hipMemcpyAsync(Sb[4 + q], W.Fib[1][q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+// the tangent-linear stage (X, Fin) -> (Y, Fout) along (dX, dF, dforce),
+// before the base stage runs (the faces are updated in place by both): dX
+// gets its ghost layers, dF goes from Fin' to Fout', dY receives Y'
+static int tlm_stage(nsx_ctx *ctx, double *const X[4], double *const dX[4], const double *dforce,
+                     double *const dY[4]) {
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  AdjWork &W = ctx->adj;
+  int rc = stage_base(ctx, X);
+  if (rc) return rc;
+  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, dX[q], q + 1));
+  LAUNCH(launch_ghost_layers(g, s, dX[3], 0));
+  const double *uvw[3] = {X[0], X[1], X[2]}, *vtx[3] = {W.V[0], W.V[1], W.V[2]};
+  const double *duvw[3] = {dX[0], dX[1], dX[2]};
+  const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
+  if ((rc = vertex_of(ctx, 3, duvw, W.TV, vk, 2))) return rc;
+  const double *dvtx[3] = {W.TV[0], W.TV[1], W.TV[2]};
+  if (ctx->Iturbu == 1)
+    LAUNCH(launch_tlm_les(g, s, uvw, vtx, duvw, dvtx, W.TE));
+  else
+    HIPCHECK(hipMemsetAsync(W.TE, 0, cell_bytes(ctx), s));
+  const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
+                           ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
+  const double *dface[5] = {W.TF[0], W.TF[1], W.TF[2], W.TF[3], W.TF[4]};
+  LAUNCH(launch_tlm_momentum(g, s, uvw, vtx, face, W.E, duvw, dvtx, dface, W.TE, dforce, W.Tus));
+  const double *dp[3] = {dX[3], nullptr, nullptr};
+  double *dpv[3] = {W.Tpv, nullptr, nullptr};
+  if ((rc = vertex_of(ctx, 1, dp, dpv, pk, 2))) return rc;
+  const double *dus[3] = {W.Tus[0], W.Tus[1], W.Tus[2]};
+  LAUNCH(launch_source(g, s, dus, W.TF, nullptr, W.Trhs, W.Tpv, ctx->part));
+  if ((rc = solve_fp64(ctx, W.Trhs, W.zero, dY[3]))) return rc;
+  double *dun[3] = {dY[0], dY[1], dY[2]};
+  LAUNCH(launch_corrector(g, s, dY[3], dus, dun));
+  const double *dyp[3] = {dY[3], nullptr, nullptr};
+  if ((rc = vertex_of(ctx, 1, dyp, dpv, pk, 2))) return rc;
+  LAUNCH(launch_newu(g, s, dY[3], W.Tpv, W.TF));
+  return 0;
+}
+
+// one RK2 step of the base state with its tangent (pool-aligned tangent
+// sets W.T, faces' tangent W.TF); J, dJ += scale x the wall shear of the new
+// level and of its tangent
+static int tlm_step(nsx_ctx *ctx, const double *dforce, double scale) {
+  AdjWork &W = ctx->adj;
+  hipStream_t s = ctx->stream;
+  const int N = ctx->set_n, A = (N + 1) % 3, B = (N + 2) % 3;
+  int rc;
+  double *(*P)[4] = ctx->pool_cell;
+  if ((rc = tlm_stage(ctx, P[N], W.T[N], dforce, W.T[A]))) return rc;
+  if ((rc = stage(ctx, N, A, 0))) return rc;
+  if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) W.unconverged = true;
+  if ((rc = tlm_stage(ctx, P[A], W.T[A], dforce, W.T[B]))) return rc;
+  if ((rc = stage(ctx, A, B, 0))) return rc;
+  if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) W.unconverged = true;
+  const double *un[3] = {P[N][0], P[N][1], P[N][2]}, *dn[3] = {W.T[N][0], W.T[N][1], W.T[N][2]};
+  double *unp[3] = {P[B][0], P[B][1], P[B][2]}, *dnp[3] = {W.T[B][0], W.T[B][1], W.T[B][2]};
+  LAUNCH(launch_rk2avg(ctx->g, s, un, unp));
+  LAUNCH(launch_rk2avg(ctx->g, s, dn, dnp));
+  ctx->set_n = B;
+  point_slots(ctx, B, A, B);
+  if (scale != 0.0) {
+    LAUNCH(launch_wall_shear(ctx->g, s, P[B][0], scale, W.Jpart, W.J));
+    LAUNCH(launch_wall_shear(ctx->g, s, W.T[B][0], scale, W.Jpart, W.J + 1));
+  }
+  return 0;
+}
+
+namespace {
+// runs the differentiation form in FP64 and puts back the caller's precision
+struct Fp64Scope {
+  nsx_ctx *c;
+  int p;
+  explicit Fp64Scope(nsx_ctx *ctx) : c(ctx), p(ctx->precision) { ctx->precision = NSX_PREC_FP64; }
+  ~Fp64Scope() { c->precision = p; }
+};
+}  // namespace
+
+static int diff_checks(nsx_ctx *ctx) {
+  if (ctx->FinRK != 2) return nsx_fail(ctx, -1, "the differentiation form needs RK2 steps (FinRK = 2)");
+  return 0;
+}
+
+extern "C" int nsx_set_body_force(nsx_ctx *ctx, int on) {
+  NEED_COMMIT(ctx);
+  ctx->body_force = on != 0;
+  return 0;
+}
+
+extern "C" int nsx_wall_shear_run(nsx_ctx *ctx, int nsteps, int every, double *J, int grad) {
+  NEED_COMMIT(ctx);
+  if (diff_checks(ctx)) return -1;
+  if (nsteps < 1 || every < 1) return nsx_fail(ctx, -1, "nsx_wall_shear_run: nsteps and every must be >= 1");
+  if (grad >= 0 && kind_ok(ctx, grad, FK_CELL)) return -1;
+  if (grad == NSX_F_FORCE) return nsx_fail(ctx, -1, "nsx_wall_shear_run: grad must not be the body force");
+  const int nck = (nsteps + every - 1) / every;
+  int rc = adj_alloc(ctx, grad >= 0 ? (size_t)nck : 0, grad >= 0 ? (size_t)every : 0);
+  if (rc) return rc;
+  Fp64Scope fp64(ctx);
+  AdjWork &W = ctx->adj;
+  hipStream_t s = ctx->stream;
+  const double scale = inv_area(ctx) / nsteps;
+  W.unconverged = false;
+  HIPCHECK(hipMemsetAsync(W.J, 0, 2 * sizeof(double), s));
+  for (int n = 0; n < nsteps; ++n) {
+    if (grad >= 0 && n % every == 0 && (rc = save_state(ctx, ctx->set_n, &W.ckpt[9 * (size_t)(n / every)]))) return rc;
+    if ((rc = fwd_step(ctx, nullptr, nullptr, scale))) return rc;
+  }
+  double jh = 0.0;
+  HIPCHECK(hipMemcpyAsync(&jh, W.J, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (J) *J = jh;
+  if (grad < 0) return W.unconverged ? NSX_SOLVE_MAX_ITERS : 0;
+  // the reverse sweep, segment by segment from the last checkpoint: recompute
+  // the segment's steps storing their stage inputs, then step back through it
+  const int nfinal = ctx->set_n;
+  if ((rc = save_state(ctx, nfinal, W.fin))) return rc;
+  for (int q = 0; q < 9; ++q) HIPCHECK(hipMemsetAsync(W.Sb[q], 0, cell_bytes(ctx), s));
+  double *fb = ctx->slot[grad];
+  HIPCHECK(hipMemsetAsync(fb, 0, cell_bytes(ctx), s));
+  for (int c = nck - 1; c >= 0; --c) {
+    const int n0 = c * every, n1 = std::min(nsteps, n0 + every);
+    if ((rc = load_state(ctx, ctx->set_n, &W.ckpt[9 * (size_t)c]))) return rc;
+    for (int n = n0; n < n1; ++n) {
+      double *const *sn = &W.seg[18 * (size_t)(n - n0)];
+      if ((rc = fwd_step(ctx, sn, sn + 9, 0.0))) return rc;
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      double *const *sn = &W.seg[18 * (size_t)(n - n0)];
+      LAUNCH(launch_wall_shear_seed(ctx->g, s, W.Sb[0], scale));
+      if ((rc = adj_step(ctx, sn, sn + 9, W.Sb, fb))) return rc;
+    }
+  }
+  if ((rc = load_state(ctx, ctx->set_n, W.fin))) return rc;
+  HIPCHECK(hipStreamSynchronize(s));
+  return W.unconverged ? NSX_SOLVE_MAX_ITERS : 0;
+}
+
+extern "C" int nsx_wall_shear_tangent(nsx_ctx *ctx, int nsteps, int dir, double *J, double *dJ) {
+  NEED_COMMIT(ctx);
+  if (diff_checks(ctx)) return -1;
+  if (nsteps < 1) return nsx_fail(ctx, -1, "nsx_wall_shear_tangent: nsteps must be >= 1");
+  if (kind_ok(ctx, dir, FK_CELL)) return -1;
+  int rc = adj_alloc(ctx, 0, 0);
+  if (rc) return rc;
+  Fp64Scope fp64(ctx);
+  AdjWork &W = ctx->adj;
+  hipStream_t s = ctx->stream;
+  for (int t = 0; t < 3; ++t)
+    for (int q = 0; q < 4; ++q) HIPCHECK(hipMemsetAsync(W.T[t][q], 0, cell_bytes(ctx), s));
+  for (int q = 0; q < 5; ++q) HIPCHECK(hipMemsetAsync(W.TF[q], 0, cell_bytes(ctx), s));
+  HIPCHECK(hipMemsetAsync(W.J, 0, 2 * sizeof(double), s));
+  const double scale = inv_area(ctx) / nsteps;
+  W.unconverged = false;
+  for (int n = 0; n < nsteps; ++n)
+    if ((rc = tlm_step(ctx, ctx->slot[dir], scale))) return rc;
+  double jh[2] = {0.0, 0.0};
+  HIPCHECK(hipMemcpyAsync(jh, W.J, sizeof jh, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (J) *J = jh[0];
+  if (dJ) *dJ = jh[1];
+  return W.unconverged ? NSX_SOLVE_MAX_ITERS : 0;
+}
+
+// the nine fields of a state argument: u, v, w, p cell fields and five face
+// fields
+static int state_ids(nsx_ctx *ctx, const int *ids, double *out[9]) {
+  if (!ids) return nsx_fail(ctx, -1, "state field ids are NULL");
+  for (int q = 0; q < 9; ++q) {
+    if (kind_ok(ctx, ids[q], q < 4 ? FK_CELL : FK_FACE)) return -1;
+    out[q] = ctx->slot[ids[q]];
+  }
+  return 0;
+}
+
+extern "C" int nsx_step_tangent(nsx_ctx *ctx, const int *tin, const int *tout, int dforce) {
+  NEED_COMMIT(ctx);
+  if (diff_checks(ctx)) return -1;
+  double *in[9], *out[9];
+  if (state_ids(ctx, tin, in) || state_ids(ctx, tout, out)) return -1;
+  if (dforce >= 0 && kind_ok(ctx, dforce, FK_CELL)) return -1;
+  int rc = adj_alloc(ctx, 0, 0);
+  if (rc) return rc;
+  Fp64Scope fp64(ctx);
+  AdjWork &W = ctx->adj;
+  hipStream_t s = ctx->stream;
+  const int n0 = ctx->set_n;
+  double *slots[12];
+  for (int q = 0; q < 12; ++q) slots[q] = ctx->slot[NSX_F_UFN + q];
+  if ((rc = save_state(ctx, n0, W.fin))) return rc;
+  for (int q = 0; q < 4; ++q)
+    HIPCHECK(hipMemcpyAsync(W.T[n0][q], in[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  for (int q = 0; q < 5; ++q)
+    HIPCHECK(hipMemcpyAsync(W.TF[q], in[4 + q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  if (dforce < 0) HIPCHECK(hipMemsetAsync(W.zero, 0, cell_bytes(ctx), s));
+  if ((rc = tlm_step(ctx, dforce >= 0 ? ctx->slot[dforce] : W.zero, 0.0))) return rc;
+  const int n1 = ctx->set_n;
+  for (int q = 0; q < 4; ++q)
+    HIPCHECK(hipMemcpyAsync(out[q], W.T[n1][q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  for (int q = 0; q < 5; ++q)
+    HIPCHECK(hipMemcpyAsync(out[4 + q], W.TF[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  // the resident state is left as it was
+  ctx->set_n = n0;
+  for (int q = 0; q < 12; ++q) ctx->slot[NSX_F_UFN + q] = slots[q];
+  if ((rc = load_state(ctx, n0, W.fin))) return rc;
+  HIPCHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int nsx_step_adjoint(nsx_ctx *ctx, const int *bin, const int *bout, int gforce) {
+  NEED_COMMIT(ctx);
+  if (diff_checks(ctx)) return -1;
+  double *in[9], *out[9];
+  if (state_ids(ctx, bin, in) || state_ids(ctx, bout, out)) return -1;
+  if (kind_ok(ctx, gforce, FK_CELL)) return -1;
+  int rc = adj_alloc(ctx, 0, 1);
+  if (rc) return rc;
+  Fp64Scope fp64(ctx);
+  AdjWork &W = ctx->adj;
+  hipStream_t s = ctx->stream;
+  const int n0 = ctx->set_n;
+  double *slots[12];
+  for (int q = 0; q < 12; ++q) slots[q] = ctx->slot[NSX_F_UFN + q];
+  if ((rc = save_state(ctx, n0, W.fin))) return rc;
+  double *const *sn = &W.seg[0];
+  if ((rc = fwd_step(ctx, sn, sn + 9, 0.0))) return rc;
+  for (int q = 0; q < 9; ++q)
+    HIPCHECK(hipMemcpyAsync(W.Sb[q], in[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  HIPCHECK(hipMemsetAsync(ctx->slot[gforce], 0, cell_bytes(ctx), s));
+  if ((rc = adj_step(ctx, sn, sn + 9, W.Sb, ctx->slot[gforce]))) return rc;
+  for (int q = 0; q < 9; ++q)
+    HIPCHECK(hipMemcpyAsync(out[q], W.Sb[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+  ctx->set_n = n0;
+  for (int q = 0; q < 12; ++q) ctx->slot[NSX_F_UFN + q] = slots[q];
+  if ((rc = load_state(ctx, n0, W.fin))) return rc;
+  HIPCHECK(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -81,7 +81,8 @@ int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, co
                const double *vv, const double *wv, double *eddy);
 int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
                     const double *const face[5], const double *eddy, const double *const rhs_in[3],
-                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2);
+                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2,
+                    const double *force = nullptr);
 int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
                   double *rhsp_out, const double *pv, Partials part);
 int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *pv);
@@ -115,6 +116,35 @@ int launch_glob_cfl(const Geo &g, hipStream_t s, const double *u, const double *
 int glob_cfl_scratch_init(int dev);
 int launch_glob_cfl_scratch(const Geo &g, hipStream_t s, int dev, const double *u, const double *v, const double *w,
                             const double *const face[5], double **out_slot);
+
+// ---- differentiation form (nsx_adjoint.hip): tangent-linear kernels of the
+// nonlinear stage steps, transposed kernels of every stage step, the
+// bottom-wall-shear objective
+int launch_tlm_les(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                   const double *const duvw[3], const double *const dvtx[3], double *deddy);
+int launch_adj_les(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                   const double *eddyb, double *const uvwb[3], double *const vtxb[3]);
+int launch_tlm_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                        const double *const face[5], const double *eddy, const double *const duvw[3],
+                        const double *const dvtx[3], const double *const dface[5], const double *deddy,
+                        const double *dforce, double *const dustar[3]);
+int launch_adj_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                        const double *const face[5], const double *eddy, const double *const usb[3],
+                        double *const uvwb[3], double *const vtxb[3], double *const faceb[5], double *eddyb,
+                        double *forceb);
+int launch_adj_source(const Geo &g, hipStream_t s, const double *const faceb[5], const double *rhspb,
+                      double *const uvwb[3], double *pvb);
+int launch_adj_corrector(const Geo &g, hipStream_t s, const double *const ub[3], double *const usb[3], double *pb);
+int launch_adj_newu(const Geo &g, hipStream_t s, const double *const faceb[5], double *pb, double *pvb);
+// valb: a vertex-field work buffer
+int launch_adj_vertex(const Geo &g, hipStream_t s, const double *fvb, const signed char *act, double *valb,
+                      double *fb);
+int launch_adj_ghost(const Geo &g, hipStream_t s, double *fb, int kind /*0 p, 1 u, 2 v, 3 w*/);
+// J += scale * bottom-wall shear of u (part: 64 doubles of partial sums)
+int launch_wall_shear(const Geo &g, hipStream_t s, const double *u, double scale, double *part, double *J);
+int launch_wall_shear_seed(const Geo &g, hipStream_t s, double *ub, double scale);
+// y = a y + b x (x may be null: y = a y)
+int launch_axpby(hipStream_t s, size_t n, double a, double *y, double b, const double *x);
 
 // ---- solver
 int launch_bicg_init(const Geo &g, hipStream_t s, const double *rhs, const double *phi_in, double *x, double *r,

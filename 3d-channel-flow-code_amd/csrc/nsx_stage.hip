@@ -221,6 +221,7 @@ int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, co
 // 3 history = 12.5; write u*,v*,w*, 3 history = 6.
 struct MomArgs {
   const double *uvw[3], *vtx[3], *face[5], *eddy, *rhs_in[3];
+  const double *force;  // per-prism streamwise body force in place of bdx, or null
   double *hist[3], *rhs_out[3], *ustar[3];
   double c1, c2;
 };
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
       const double dB = (phi[s - 1] - pi) / g.dsig[k - 1];
       const double src = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv +
                          (dT + dB) * area;
-      double r = a.rhs_in[q] ? a.rhs_in[q][s] : bd[q];
+      double r = a.rhs_in[q] ? a.rhs_in[q][s] : ((q == 0 && a.force) ? a.force[s] : bd[q]);
       r = r + Gam * src / Vol;
       double fc[5];
       faces_of(g, i, k, s, phi, phiv, fc);
@@ -286,8 +287,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
 
 int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
                     const double *const face[5], const double *eddy, const double *const rhs_in[3],
-                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2) {
+                    double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2,
+                    const double *force) {
   MomArgs a;
+  a.force = force;
   for (int q = 0; q < 3; ++q) {
     a.uvw[q] = uvw[q];
     a.vtx[q] = vtx[q];

@@ -15,6 +15,7 @@
            NSX_F_WF = 6, NSX_F_PF = 7, NSX_F_UFNP = 8, NSX_F_VFNP = 9,   &
            NSX_F_WFNP = 10, NSX_F_PFNP = 11, NSX_F_RHSUF = 12,           &
            NSX_F_RHSVF = 13, NSX_F_RHSWF = 14, NSX_F_EDDY = 15,          &
+           NSX_F_FORCE = 16,                                             &
            NSX_F_UFV = 32, NSX_F_VFV = 33, NSX_F_WFV = 34,               &
            NSX_F_PFV = 35, NSX_F_U1FACE = 48, NSX_F_U2FACE = 49,         &
            NSX_F_U3FACE = 50, NSX_F_UTFACE = 51, NSX_F_UBFACE = 52,      &
@@ -246,6 +247,40 @@
           type(c_ptr), value :: ctx
           integer(c_int), value :: lam, g
           type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_set_body_force(ctx, on)              &
+             bind(C, name="nsx_set_body_force")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: on
+        end function
+        integer(c_int) function nsx_wall_shear_run(ctx, nsteps, every,   &
+             J, grad) bind(C, name="nsx_wall_shear_run")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: nsteps, every, grad
+          real(c_double) :: J
+        end function
+        integer(c_int) function nsx_wall_shear_tangent(ctx, nsteps, dir, &
+             J, dJ) bind(C, name="nsx_wall_shear_tangent")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: nsteps, dir
+          real(c_double) :: J, dJ
+        end function
+        integer(c_int) function nsx_step_tangent(ctx, tin, tout, dforce) &
+             bind(C, name="nsx_step_tangent")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int) :: tin(9), tout(9)
+          integer(c_int), value :: dforce
+        end function
+        integer(c_int) function nsx_step_adjoint(ctx, bin, bout, gforce) &
+             bind(C, name="nsx_step_adjoint")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int) :: bin(9), bout(9)
+          integer(c_int), value :: gforce
         end function
         integer(c_int) function nsx_glob_cfl(ctx, u, v, w, info)         &
              bind(C, name="nsx_glob_cfl")

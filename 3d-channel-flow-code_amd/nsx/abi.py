@@ -10,7 +10,7 @@ c_cp = ctypes.c_char_p
 
 # field ids (enum nsx_field)
 F = dict(UFN=0, VFN=1, WFN=2, PFN=3, UF=4, VF=5, WF=6, PF=7, UFNP=8, VFNP=9, WFNP=10, PFNP=11,
-         RHSUF=12, RHSVF=13, RHSWF=14, EDDY=15, UFV=32, VFV=33, WFV=34, PFV=35,
+         RHSUF=12, RHSVF=13, RHSWF=14, EDDY=15, FORCE=16, UFV=32, VFV=33, WFV=34, PFV=35,
          U1FACE=48, U2FACE=49, U3FACE=50, UTFACE=51, UBFACE=52)
 CELL_SCRATCH = 64
 VERT_SCRATCH = 96
@@ -36,7 +36,8 @@ EXPORTS = [
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
-    "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_set_body_force", "nsx_wall_shear_run", "nsx_wall_shear_tangent",
+    "nsx_step_tangent", "nsx_step_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
     "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
     "nsx_write_plane_stats", "nsx_format_real", "nsx_savetec_vertex", "nsx_savetec_center", "nsx_write_tec_vertex",
     "nsx_write_tec_center", "nsx_save_paraview_vertex", "nsx_savetec_vc", "nsx_write_vtk_vertex", "nsx_write_tec_vc",
@@ -101,6 +102,11 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_glob_cfl": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_glob_cfl_noalloc": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_bicgstab_adjoint": [c_vp, c_int, c_int, P(SolveInfo)],
+        "nsx_set_body_force": [c_vp, c_int],
+        "nsx_wall_shear_run": [c_vp, c_int, c_int, P(c_dbl), c_int],
+        "nsx_wall_shear_tangent": [c_vp, c_int, c_int, P(c_dbl), P(c_dbl)],
+        "nsx_step_tangent": [c_vp, P(c_int), P(c_int), c_int],
+        "nsx_step_adjoint": [c_vp, P(c_int), P(c_int), c_int],
         "nsx_timing_enable": [c_vp, c_int],
         "nsx_timing_get": [c_vp, c_cp, P(c_dbl), P(ctypes.c_longlong)],
         "nsx_timing_reset": [c_vp],

@@ -188,3 +188,23 @@ def config_case(cfg: int, *, seed: int = 1234, **over) -> Case:
     p = Params(dt=d["dt"], Re=d["Re"])
     return make_case(d["nx"], d["ny"], d["nn"], seed=seed, jitter=d["jitter"], bed=d["bed"],
                      Ubulk=d["Ubulk"], params=p)
+
+
+def gaussian_random_field(case: Case, seed: int, corr: float = 0.5, std: float = 0.1, mean: float = 0.0,
+                          modes: int = 512) -> np.ndarray:
+    """A seeded Gaussian random field at the prism centres (xc, yc, sig(k)),
+    interior prisms (N_CELL0, NZ-2): mean + std * sqrt(2/M) sum_m cos(k_m . x
+    + phi_m) with wave vectors k_m ~ N(0, I/corr^2) and phases phi_m ~ U(0,
+    2 pi) (random Fourier features), whose covariance tends to std^2
+    exp(-|r|^2 / (2 corr^2)) -- correlation length corr -- as M grows."""
+    rng = np.random.default_rng(seed)
+    k = rng.standard_normal((modes, 3)) / corr
+    phi = rng.uniform(0.0, 2.0 * math.pi, modes)
+    A = case.arrays
+    x = A["xc"][:case.N_CELL0][:, None]
+    y = A["yc"][:case.N_CELL0][:, None]
+    z = A["sig"][1:-1][None, :]
+    out = np.zeros((case.N_CELL0, case.NZ - 2))
+    for m in range(modes):
+        out += np.cos(k[m, 0] * x + k[m, 1] * y + k[m, 2] * z + phi[m])
+    return mean + std * math.sqrt(2.0 / modes) * out

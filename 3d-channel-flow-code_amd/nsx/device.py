@@ -115,7 +115,10 @@ class Context:
 
     def device_memory_used(self) -> int:
         """Bytes of device memory in use on the context's GPU (hipMemGetInfo)."""
-        hip = ctypes.CDLL("libamdhip64.so")
+        import os
+        # the ROCm runtime the library links (a torch wheel may carry another
+        # copy of it under the same name)
+        hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
         free, total = ctypes.c_size_t(), ctypes.c_size_t()
         if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
             raise NsxError("hipMemGetInfo failed")
@@ -226,6 +229,77 @@ class Context:
         (cp,) = self._cells(p)
         self._check(self.lib.nsx_newu3d(self.h, cp, VERT_SCRATCH))
         return tuple(self.get(n) for n in ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")), self.get(VERT_SCRATCH)
+
+    # -------------------------------------------------- differentiation ---
+    def set_body_force(self, force=None) -> None:
+        """The per-prism streamwise body force in place of bdx (an (N_CELL,NZ)
+        array, or interior (N_CELL0, NZ-2)), or None to go back to bdx."""
+        if force is None:
+            self._check(self.lib.nsx_set_body_force(self.h, 0))
+            return
+        self.put("FORCE", self._full_cell(force))
+        self._check(self.lib.nsx_set_body_force(self.h, 1))
+
+    def _full_cell(self, a) -> np.ndarray:
+        c = self.case
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape == (c.N_CELL0, c.NZ - 2):
+            full = c.cell_field()
+            full[:c.N_CELL0, 1:-1] = a
+            return full
+        return a
+
+    def wall_shear(self, nsteps: int, every: int = 5, grad: bool = False):
+        """nsteps RK2 steps from the resident state and the time-averaged
+        bottom-wall shear J; with grad, also dJ/d(body force) by the adjoint
+        (interior prisms, (N_CELL0, NZ-2)).  Returns J or (J, grad)."""
+        J = ctypes.c_double()
+        g = CELL_SCRATCH + 15 if grad else -1
+        rc = self._check(self.lib.nsx_wall_shear_run(self.h, int(nsteps), int(every), ctypes.byref(J), g))
+        if rc:
+            raise NsxError("nsx_wall_shear_run: a pressure solve reached MaxIters")
+        if not grad:
+            return J.value
+        return J.value, self.get(g, fill_ghost_rows=False)[:self.case.N_CELL0, 1:-1].copy()
+
+    def wall_shear_tangent(self, nsteps: int, direction):
+        """The same steps with the tangent-linear model along a body-force
+        direction: (J, dJ)."""
+        d = CELL_SCRATCH + 14
+        self.put(d, self._full_cell(direction))
+        J, dJ = ctypes.c_double(), ctypes.c_double()
+        rc = self._check(self.lib.nsx_wall_shear_tangent(self.h, int(nsteps), d, ctypes.byref(J), ctypes.byref(dJ)))
+        if rc:
+            raise NsxError("nsx_wall_shear_tangent: a pressure solve reached MaxIters")
+        return J.value, dJ.value
+
+    def _state_ids(self, base_cell: int, base_face: int):
+        ids = (ctypes.c_int * 9)(*([base_cell + q for q in range(4)] + [base_face + q for q in range(5)]))
+        return ids
+
+    def step_tangent(self, tin, dforce=None):
+        """One RK2 step's tangent-linear map at the resident state (left
+        unchanged): tin = 9 arrays (u, v, w, p as (N_CELL,NZ), the five faces
+        as (N_CELL0,NZ)); returns the 9 tangents of the new level."""
+        ti, to = self._state_ids(CELL_SCRATCH, FACE_SCRATCH), self._state_ids(CELL_SCRATCH + 4, FACE_SCRATCH + 5)
+        for q in range(9):
+            self.put(ti[q], tin[q])
+        df = -1
+        if dforce is not None:
+            df = CELL_SCRATCH + 8
+            self.put(df, self._full_cell(dforce))
+        self._check(self.lib.nsx_step_tangent(self.h, ti, to, df))
+        return [self.get(to[q], fill_ghost_rows=False) for q in range(9)]
+
+    def step_adjoint(self, bin_):
+        """The transpose of step_tangent: the bars of the new level (9 arrays)
+        give those of level n (9 arrays) and the body-force bar."""
+        bi, bo = self._state_ids(CELL_SCRATCH, FACE_SCRATCH), self._state_ids(CELL_SCRATCH + 4, FACE_SCRATCH + 5)
+        for q in range(9):
+            self.put(bi[q], bin_[q])
+        gf = CELL_SCRATCH + 8
+        self._check(self.lib.nsx_step_adjoint(self.h, bi, bo, gf))
+        return [self.get(bo[q], fill_ghost_rows=False) for q in range(9)], self.get(gf, fill_ghost_rows=False)
 
     # ------------------------------------------------------------- timing ---
     def bicgstab_adjoint(self, g):

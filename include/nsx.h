@@ -38,6 +38,7 @@ enum nsx_field {
   NSX_F_UFNP, NSX_F_VFNP, NSX_F_WFNP, NSX_F_PFNP, /* stage output / n+1 */
   NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF,          /* AB2 history (module geometry) */
   NSX_F_EDDY,                                     /* eddy viscosity (module les) */
+  NSX_F_FORCE,                                    /* per-prism streamwise body force (nsx_set_body_force) */
   /* vertex fields (N_VERT,NZ-1) */
   NSX_F_UFV = 32, NSX_F_VFV, NSX_F_WFV, NSX_F_PFV,
   /* face-normal velocities (N_CELL0,NZ): U1FACE..UBFACE (module geometry) */
@@ -177,6 +178,37 @@ int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
  * the solve is the FP64 BiCGSTAB from lam = 0 with the reference's stopping
  * rule.  lam and g are cell fields, lam != g. */
 int nsx_bicgstab_adjoint(nsx_ctx *ctx, int lam, int g, nsx_solve_info *info);
+
+/* The differentiation form.  The control is a per-prism streamwise body force
+ * in the field NSX_F_FORCE; nsx_set_body_force(ctx, 1) makes the u momentum
+ * of nsx_hydro / nsx_step use it in place of bdx (0 goes back to bdx).  The
+ * objective is the bottom-wall shear of the channel averaged over nsteps
+ * RK2 steps (FinRK = 2):
+ *   J = (1/nsteps) sum_{n=1..nsteps} sum_i area_i nu (u_n(i,2) - u_n(i,1)) / dsig(1) / sum_i area_i,
+ * with u(i,1) the bottom ghost layer (-u(i,2) at a no-slip wall).  Solves run
+ * in FP64 BiCGSTAB to eps (nsx_set_params); the adjoint solves use A^T. */
+int nsx_set_body_force(nsx_ctx *ctx, int on);
+/* nsteps RK2 steps from the resident state (which ends at the new level, as
+ * after nsx_step) and J; when grad >= 0 (a cell field other than
+ * NSX_F_FORCE), dJ/d(force) per prism into grad by the discrete adjoint:
+ * checkpoints of the state every `every` steps, each segment recomputed
+ * forward with its stage inputs stored, then stepped back with the
+ * transposed kernels (reversed RK2 stages and the adjoint pressure solves).
+ * Returns NSX_SOLVE_MAX_ITERS (> 0) when a forward solve reached MaxIters
+ * (J and the gradient are then not those of converged steps); the same for
+ * nsx_wall_shear_tangent. */
+int nsx_wall_shear_run(nsx_ctx *ctx, int nsteps, int every, double *J, int grad);
+/* the same nsteps steps with the tangent-linear model along the force
+ * direction dir (a cell field): J and dJ = sum over prisms of dJ/d(force) dir */
+int nsx_wall_shear_tangent(nsx_ctx *ctx, int nsteps, int dir, double *J, double *dJ);
+/* One RK2 step's tangent-linear map and its transpose, at the resident state
+ * (which is left unchanged).  A state argument is nine field ids: u, v, w, p
+ * (cell fields) and the five face velocities (face fields).  tangent: the
+ * tangent tin of the level-n state and dforce of the body force (cell field,
+ * or -1 for none) give tout at level n+1; adjoint: the bars bin of the new
+ * level give bout of level n and gforce, the bar of the body force. */
+int nsx_step_tangent(nsx_ctx *ctx, const int *tin, const int *tout, int dforce);
+int nsx_step_adjoint(nsx_ctx *ctx, const int *bin, const int *bout, int gforce);
 
 /* ---- diagnostics ------------------------------------------------------------ */
 /* glob_cfl (cfl.F90:7-269, sequential branch) on the cell fields u, v, w and

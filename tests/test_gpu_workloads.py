@@ -85,9 +85,11 @@ def test_largest_workload_512x256x128_two_steps():
     G.set_precision(PREC_MIXED, REFINE_CGSR_BLOCK, 0.1, 30)
     G.init_state()
     info = G.step(2, PREC_MIXED)
-    used = G.device_memory_used()
-    print(f"512x256x128: {case.n_interior} prisms, solve {info}, device memory in use {used / 2**30:.1f} GiB",
-          flush=True)
+    try:
+        used = f"{G.device_memory_used() / 2**30:.1f} GiB"
+    except Exception as e:  # the footprint is logged, not tested
+        used = f"not available ({e})"
+    print(f"512x256x128: {case.n_interior} prisms, solve {info}, device memory in use {used}", flush=True)
     assert info["exit"] == "converged" and info["n_solves"] == 4, info
     assert min(info["e1"], info["e2"]) < case.params.eps, info
     p = interior(case, G.get("PFN"))
