@@ -77,7 +77,7 @@ def build_fortran():
     tdir = os.path.join(REPO, "tests", "fortran", "_build")
     os.makedirs(tdir, exist_ok=True)
     rel = os.path.relpath(OUT, tdir)
-    for drv in ("dropin_driver", "savetec_driver"):
+    for drv in ("dropin_driver", "savetec_driver", "routines_driver"):
         _run([FC, "-O2", "-J", fdir, "-o", os.path.join(tdir, drv),
               os.path.join(REPO, "tests", "fortran", drv + ".F90"), lib, "-L", OUT, "-lnsx",
               f"-Wl,-rpath,$ORIGIN/{rel}"])

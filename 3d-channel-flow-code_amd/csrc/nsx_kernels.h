@@ -82,7 +82,13 @@ int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, co
 int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
                     const double *const face[5], const double *eddy, const double *const rhs_in[3],
                     double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2,
-                    const double *force = nullptr);
+                    const double *force = nullptr, const double *const gam[3] = nullptr);
+// the same two in FP32 arithmetic on the FP64 fields (NSX_PREC_MIXED_FLUX32)
+int launch_les32(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
+                 const double *vv, const double *wv, double *eddy);
+int launch_momentum32(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                      const double *const face[5], const double *eddy, double *const hist[3],
+                      double *const ustar[3], double c1, double c2, const double *force);
 int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], double *const face[5], double *src_out,
                   double *rhsp_out, const double *pv, Partials part);
 int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *pv);

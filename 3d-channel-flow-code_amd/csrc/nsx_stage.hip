@@ -221,6 +221,7 @@ int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, co
 // 3 history = 12.5; write u*,v*,w*, 3 history = 6.
 struct MomArgs {
   const double *uvw[3], *vtx[3], *face[5], *eddy, *rhs_in[3];
+  const double *gam[3];  // per-component diffusivity fields in place of eddy + nu, or null
   const double *force;  // per-prism streamwise body force in place of bdx, or null
   double *hist[3], *rhs_out[3], *ustar[3];
   double c1, c2;
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
       const double src = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv +
                          (dT + dB) * area;
       double r = a.rhs_in[q] ? a.rhs_in[q][s] : ((q == 0 && a.force) ? a.force[s] : bd[q]);
-      r = r + Gam * src / Vol;
+      r = r + (a.gam[q] ? a.gam[q][s] : Gam) * src / Vol;
       double fc[5];
       faces_of(g, i, k, s, phi, phiv, fc);
       const double flux = fc[0] * UF1 + fc[1] * UF2 + fc[2] * UF3 + fc[3] * UFT + fc[4] * UFB;
@@ -288,10 +289,11 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
 int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
                     const double *const face[5], const double *eddy, const double *const rhs_in[3],
                     double *const hist[3], double *const rhs_out[3], double *const ustar[3], double c1, double c2,
-                    const double *force) {
+                    const double *force, const double *const gam[3]) {
   MomArgs a;
   a.force = force;
   for (int q = 0; q < 3; ++q) {
+    a.gam[q] = gam ? gam[q] : nullptr;
     a.uvw[q] = uvw[q];
     a.vtx[q] = vtx[q];
     a.rhs_in[q] = rhs_in ? rhs_in[q] : nullptr;
@@ -304,6 +306,169 @@ int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], con
   a.c1 = c1;
   a.c2 = c2;
   hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, a);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------ FP32 flux and eddy viscosity --
+// The precision form's FP32 arithmetic for the momentum fluxes and the
+// Smagorinsky eddy viscosity (NSX_PREC_MIXED_FLUX32): the stencil
+// differences are formed in FP64 from the stored FP64 fields and rounded to
+// FP32 once (a difference of two nearby values keeps its relative accuracy
+// that way, where the FP32 values themselves would lose it to cancellation);
+// every product, quotient and sum of the fluxes and of the strain rate is
+// FP32; the predictor u* = u + dt r adds the FP32 r to the FP64 u.  In exact
+// arithmetic these are k_momentum and k_les: the advective flux is formed as
+// sum_j (phi_f - phi_i) UF_j + phi_i sum_j UF_j and the face gradients from
+// phi_f - phi_i (plus the constant part that fvm_gradient's swapped edge
+// lengths leave in dv/dx, dv/dy).  The
+// fields stay FP64 in memory, so the bytes are k_momentum's and k_les's.
+
+// the five face values of f at (i,k) minus f(i,k), in FP32 from FP64 differences
+__device__ inline void faces_dev32(const Geo &g, int i, int k, size_t s, const double *f, const double *fv,
+                                   float (&out)[5]) {
+  const double fi = f[s];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int e = i * 3 + j;
+    const int nb = g.nbr[e];
+    const int nv1 = g.vtx[e], nv2 = g.vtx[i * 3 + (j + 1) % 3];
+    const float dv = (float)(fv[(size_t)nv2 * g.NL + k] - fv[(size_t)nv1 * g.NL + k]);
+    const float dc = (float)(f[(size_t)nb * g.NS + k] - fi);
+    out[j] = dc * 0.5f + (float)g.doe[e] * dv / (float)g.dlVV[e];
+  }
+  out[3] = 0.5f * (float)(f[s + 1] - fi);
+  out[4] = 0.5f * (float)(f[s - 1] - fi);
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_les32(Geo g, const double *u, const double *v, const double *w,
+                                                      const double *uv, const double *vv, const double *wv,
+                                                      double *eddy) {
+  const float Cs = 0.1f, karman = 0.415f;
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NS + k;
+    float U[5], V[5], W[5];
+    faces_dev32(g, i, k, s, u, uv, U);
+    faces_dev32(g, i, k, s, v, vv, V);
+    faces_dev32(g, i, k, s, w, wv, W);
+    const int e = i * 3;
+    const float x1 = (float)g.xnn[e], x2 = (float)g.xnn[e + 1], x3 = (float)g.xnn[e + 2];
+    const float y1 = (float)g.ynn[e], y2 = (float)g.ynn[e + 1], y3 = (float)g.ynn[e + 2];
+    const float l1 = (float)g.dlVV[e], l2 = (float)g.dlVV[e + 1], l3 = (float)g.dlVV[e + 2];
+    const float a = (float)g.area[i];
+    // the face values are deviations from the prism's own value, whose
+    // contribution sum_j n_j l_j phi_i vanishes on the closed triangle -- but
+    // not for dvdx, dvdy, whose edge lengths 2 and 3 are swapped: their
+    // constant part v_i sum_j n_j l_j' is kept, as the reference has it
+    const float vi = (float)v[s];
+    const float dudx = (U[0] * x1 * l1 + U[1] * x2 * l2 + U[2] * x3 * l3) / a;
+    const float dvdx = (V[0] * x1 * l1 + V[1] * x2 * l3 + V[2] * x3 * l2) / a + vi * ((x1 * l1 + x2 * l3 + x3 * l2) / a);
+    const float dwdx = (W[0] * x1 * l1 + W[1] * x2 * l2 + W[2] * x3 * l3) / a;
+    const float dudy = (U[0] * y1 * l1 + U[1] * y2 * l2 + U[2] * y3 * l3) / a;
+    const float dvdy = (V[0] * y1 * l1 + V[1] * y2 * l3 + V[2] * y3 * l2) / a + vi * ((y1 * l1 + y2 * l3 + y3 * l2) / a);
+    const float dwdy = (W[0] * y1 * l1 + W[1] * y2 * l2 + W[2] * y3 * l3) / a;
+    const float dz = (float)g.dsig[k];
+    const float dudz = (U[3] - U[4]) / dz, dvdz = (V[3] - V[4]) / dz, dwdz = (W[3] - W[4]) / dz;
+    const float DD = Cs * cbrtf(a * dz);
+    const float lwall = karman * (float)g.walld[k];
+    const float dn1 = (1.0f / DD) * (1.0f / DD) + (1.0f / lwall) * (1.0f / lwall);
+    const float nut = 1.0f / dn1;
+    const float sxx = dudx, sxy = (dudy + dvdx) * 0.5f, sxz = (dudz + dwdx) * 0.5f;
+    const float syy = dvdy, syz = (dvdz + dwdy) * 0.5f, szz = dwdz;
+    const float Sij = sqrtf(2.0f * (sxx * sxx + syy * syy + szz * szz) + 4.0f * (sxy * sxy + sxz * sxz + syz * syz));
+    const double ed = (double)(nut * Sij);
+    eddy[s] = ed;
+    if (k == 1 && g.bcbot == 1) eddy[s - 1] = ed;
+    if (k == g.NZ - 2 && g.bctop == 1) eddy[s + 1] = ed;
+  }
+}
+
+int launch_les32(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
+                 const double *vv, const double *wv, double *eddy) {
+  hipLaunchKernelGGL(k_les32, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
+  return (int)hipGetLastError();
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_momentum32(Geo g, MomArgs a) {
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NS + k;
+    const int e = i * 3;
+    const float area = (float)g.area[i];
+    const float sv = (float)g.dsigv[k - 1];
+    const float Vol = area * sv;
+    const float Gam = (float)(a.eddy[s] + g.nu);
+    float UF[5];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) UF[j] = (float)a.face[j][s] * (float)g.dlVV[e + j] * sv;
+    UF[3] = (float)a.face[3][s] * area;
+    UF[4] = (float)a.face[4][s] * area;
+    const float divf = UF[0] + UF[1] + UF[2] + UF[3] + UF[4];
+    const float bd[3] = {(float)g.bdx, (float)g.bdy, (float)g.bdz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double *phi = a.uvw[q];
+      const double *phiv = a.vtx[q];
+      const double pi = phi[s];
+      float dn[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int ej = e + j;
+        const int nb = g.nbr[ej];
+        const int jv1 = g.vtx[ej], jv2 = g.vtx[e + (j + 1) % 3];
+        const float diff1 = (float)(phi[(size_t)nb * g.NS + k] - pi) / (float)g.dlCC[ej];
+        const float diff2 = (float)(phiv[(size_t)jv2 * g.NL + k] - phiv[(size_t)jv1 * g.NL + k]) / (float)g.dlVV[ej];
+        dn[j] = diff1 * (float)g.dle1[ej] + diff2 * (float)g.dle2[ej];
+      }
+      const float dT = (float)(phi[s + 1] - pi) / (float)g.dsig[k];
+      const float dB = (float)(phi[s - 1] - pi) / (float)g.dsig[k - 1];
+      const float src = dn[0] * (float)g.dlVV[e] * sv + dn[1] * (float)g.dlVV[e + 1] * sv +
+                        dn[2] * (float)g.dlVV[e + 2] * sv + (dT + dB) * area;
+      float fc[5];
+      faces_dev32(g, i, k, s, phi, phiv, fc);
+      const float flux = fc[0] * UF[0] + fc[1] * UF[1] + fc[2] * UF[2] + fc[3] * UF[3] + fc[4] * UF[4] +
+                         (float)pi * divf;
+      const float r0 = a.rhs_in[q] ? (float)a.rhs_in[q][s] : ((q == 0 && a.force) ? (float)a.force[s] : bd[q]);
+      const float r = r0 + Gam * src / Vol - flux / Vol;
+      if (a.rhs_out[q]) a.rhs_out[q][s] = (double)r;
+      if (a.ustar[q]) {
+        const double us = pi + g.dt * (a.c1 * (double)r + a.c2 * a.hist[q][s]);
+        a.hist[q][s] = (double)r;
+        a.ustar[q][s] = us;
+        if (k == 1) {
+          const int sg = vel_ghost_sign(g.bcbot, g.BotBC, q);
+          if (sg) a.ustar[q][s - 1] = sg > 0 ? us : -us;
+        }
+        if (k == g.NZ - 2) {
+          const int sg = vel_ghost_sign(g.bctop, g.TopBC, q);
+          if (sg) a.ustar[q][s + 1] = sg > 0 ? us : -us;
+        }
+      }
+    }
+  }
+}
+
+int launch_momentum32(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],
+                      const double *const face[5], const double *eddy, double *const hist[3],
+                      double *const ustar[3], double c1, double c2, const double *force) {
+  MomArgs a;
+  a.force = force;
+  for (int q = 0; q < 3; ++q) {
+    a.gam[q] = nullptr;
+    a.uvw[q] = uvw[q];
+    a.vtx[q] = vtx[q];
+    a.rhs_in[q] = nullptr;
+    a.hist[q] = hist[q];
+    a.rhs_out[q] = nullptr;
+    a.ustar[q] = ustar[q];
+  }
+  for (int q = 0; q < 5; ++q) a.face[q] = face[q];
+  a.eddy = eddy;
+  a.c1 = c1;
+  a.c2 = c2;
+  hipLaunchKernelGGL(k_momentum32, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, a);
   return (int)hipGetLastError();
 }
 

@@ -27,7 +27,7 @@
            NSX_SOLVE_MAX_ITERS = 3
 !     precision and refinement (enum nsx_precision, enum nsx_refine)
       integer(c_int), parameter :: NSX_PREC_FP64 = 0, NSX_PREC_MIXED = 1,&
-           NSX_PREC_FP64_CG = 2
+           NSX_PREC_FP64_CG = 2, NSX_PREC_MIXED_FLUX32 = 3
       integer(c_int), parameter :: NSX_REFINE_CG = 3,                   &
            NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5,               &
            NSX_REFINE_CGSR_LINE = 6
@@ -234,6 +234,25 @@
           type(c_ptr), value :: ctx
           integer(c_int), value :: phi, phiv, rhs
           type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_bicgstab_solve(ctx, phi, rhs, info)  &
+             bind(C, name="nsx_bicgstab_solve")
+          import :: c_int, c_ptr, nsx_solve_info
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: phi, rhs
+          type(nsx_solve_info) :: info
+        end function
+        integer(c_int) function nsx_advdiff_gam(ctx, rhsu, rhsv, rhsw,   &
+             u, v, w, gamx, gamy, gamz) bind(C, name="nsx_advdiff_gam")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: rhsu, rhsv, rhsw, u, v, w, gamx, gamy, gamz
+        end function
+        integer(c_int) function nsx_interpolate_u(ctx, f1, f2, f3, fT,   &
+             fB, u, v, w) bind(C, name="nsx_interpolate_u")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: f1, f2, f3, fT, fB, u, v, w
         end function
         integer(c_int) function nsx_newu3d(ctx, p, pv)                   &
              bind(C, name="nsx_newu3d")

@@ -22,6 +22,7 @@ N_FACE_SCRATCH = 10
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_FP64_CG = 2
+PREC_MIXED_FLUX32 = 3
 REFINE_CG = 3
 REFINE_CGSR = 4
 REFINE_CGSR_BLOCK = 5
@@ -35,7 +36,8 @@ EXPORTS = [
     "nsx_set_sigma", "nsx_set_vertex_weights", "nsx_set_bc", "nsx_set_ppe_matrix", "nsx_set_les",
     "nsx_set_params", "nsx_commit", "nsx_put_field", "nsx_get_field", "nsx_copy_field", "nsx_init_state",
     "nsx_hydro", "nsx_step", "nsx_solve_info_get", "nsx_bcglobalvc", "nsx_bcglobalp", "nsx_grad_lsm",
-    "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_source3d", "nsx_update_rhsp", "nsx_bicgstab",
+    "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_advdiff_gam", "nsx_interpolate_u", "nsx_source3d",
+    "nsx_update_rhsp", "nsx_bicgstab", "nsx_bicgstab_solve",
     "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_set_body_force", "nsx_wall_shear_run", "nsx_wall_shear_tangent",
     "nsx_step_tangent", "nsx_step_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
     "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
@@ -94,6 +96,9 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_calcul_face": [c_vp] + [c_int] * 7,
         "nsx_sgles": [c_vp] + [c_int] * 3,
         "nsx_advdiff": [c_vp] + [c_int] * 7,
+        "nsx_advdiff_gam": [c_vp] + [c_int] * 9,
+        "nsx_interpolate_u": [c_vp] + [c_int] * 8,
+        "nsx_bicgstab_solve": [c_vp, c_int, c_int, P(SolveInfo)],
         "nsx_source3d": [c_vp] + [c_int] * 4,
         "nsx_update_rhsp": [c_vp] + [c_int] * 3,
         "nsx_bicgstab": [c_vp, c_int, c_int, c_int, P(SolveInfo)],

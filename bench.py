@@ -15,6 +15,9 @@ rule min(|r|_1/|b|_1, |r|_2/|b|_2) < eps):
            gain of the solver from the gain of FP32);
   mixed    the precision form: the same CG in FP32 with reliable updates
            against the FP64 solution, stopped on the true FP64 residual;
+  mixed_flux32  the same with the momentum fluxes and the Smagorinsky eddy
+           viscosity in FP32 arithmetic too (its errors are reported; the
+           gate is the mixed leg's);
   ref      fp64_cg at eps 1e-14, the tight reference the other legs'
            errors are measured against.
 The headline legs run at the gate tolerance (--eps-gate, 1e-13), where the
@@ -109,7 +112,8 @@ def cpu_baseline(case, mean_iters: float, cap: int = 100, parallel: bool = False
 # preconditioner; "mixed_line" the same with the line preconditioner (whole
 # columns), "mixed_point" with point Jacobi; "mixed_cg2" the point-Jacobi CG
 # in two kernels per iteration
-MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1)}
+MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1),
+               "mixed_flux32": (5, 0.1)}
 # the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
 # writes e, rt, p (units of 4 B per interior prism; the few X launches that
 # form a true FP64 residual move more)
@@ -127,12 +131,12 @@ def leg(case, name: str, steps: int, warmup: int, eps: float, max_iters=None) ->
     CG) or one of MIXED_MODES, with the pressure solves stopped at eps."""
     from nsx.device import Context
 
-    precision = 0 if name == "fp64" else (2 if name == "fp64_cg" else 1)
+    precision = {"fp64": 0, "fp64_cg": 2, "mixed_flux32": 3}.get(name, 1)
     ctx = Context(case)
     p = case.params
     ctx.set_params(eps=eps, max_iters=max_iters or p.MaxIters)
-    if precision == 1:
-        ctx.set_precision(1, *MIXED_MODES[name])
+    if precision in (1, 3):
+        ctx.set_precision(precision, *MIXED_MODES[name])
     ctx.init_state()
     if warmup:
         ctx.step(warmup, precision)
@@ -224,8 +228,9 @@ def main() -> int:
     ap.add_argument("--eps-config", type=float, default=None,
                     help="the workload's own tolerance (default: the case's, 1e-10); 0 skips those legs")
     ap.add_argument("--eps-ref", type=float, default=1e-14, help="tolerance of the tight reference; 0 skips it")
-    ap.add_argument("--legs", default="fp64,fp64_cg,mixed",
-                    help="comma list of the gate legs: fp64, fp64_cg, mixed, mixed_line, mixed_point, mixed_cg2")
+    ap.add_argument("--legs", default="fp64,fp64_cg,mixed,mixed_flux32",
+                    help="comma list of the gate legs: fp64, fp64_cg, mixed, mixed_flux32, mixed_line, mixed_point, "
+                         "mixed_cg2")
     ap.add_argument("--max-iters", type=int, default=None,
                     help="cap the solver iterations (profiling runs only: the line is then marked capped)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -78,9 +78,13 @@ enum nsx_precision {
                           updates; by default the single-reduction form with the 4-layer
                           block-Jacobi preconditioner) with reliable updates against the FP64
                           solution; true residuals, inner products and norms in FP64 */
-  NSX_PREC_FP64_CG = 2 /* the precision form's solver with FP64 vectors: single-reduction CG
+  NSX_PREC_FP64_CG = 2, /* the precision form's solver with FP64 vectors: single-reduction CG
                           with the 4-layer block-Jacobi preconditioner and reliable updates,
                           all in FP64 (the refinement set by nsx_set_precision is not used) */
+  NSX_PREC_MIXED_FLUX32 = 3 /* NSX_PREC_MIXED with the momentum fluxes and the Smagorinsky eddy
+                          viscosity in FP32 arithmetic too (stencil differences formed in FP64,
+                          rounded once; fields stay FP64; the predictor adds the FP32 rate to
+                          the FP64 velocity) */
 };
 
 /* ---- lifecycle ----------------------------------------------------------- */
@@ -162,11 +166,22 @@ int nsx_sgles(nsx_ctx *ctx, int u, int v, int w);
 int nsx_advdiff(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u, int v, int w, int eddy);
 /* Source3D (source3D.F90:8-174): faces into U1FACE..UBFACE, divergence into src */
 int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w);
+/* AdvDiffNS (AdvDiffNS.F90:8-140) with the reference's diffusivity fields
+ * Gamx, Gamy, Gamz (cell fields; diffusion3D.F90:187-189 multiplies each
+ * component's face-flux sum by its own): rhs? += Gam? diffusion - advection */
+int nsx_advdiff_gam(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u, int v, int w, int gamx, int gamy, int gamz);
+/* interpolateU (interpolationU.F90:5-190): ghost layers of u, v, w, then
+ * their face-normal velocities into the face fields f1..fB */
+int nsx_interpolate_u(nsx_ctx *ctx, int f1, int f2, int f3, int fT, int fB, int u, int v, int w);
 /* update_rhsp (source3D.F90:176-335): rhs -= lagged Kim-Choi flux of p */
 int nsx_update_rhsp(nsx_ctx *ctx, int rhs, int p, int pv);
 /* bicgstab (bicgstab.F90:8-168): mean removal, BiCGSTAB_SOLVE, mean removal,
  * BCglobalP(tagdir 3); solves in place into phi, phiv */
 int nsx_bicgstab(nsx_ctx *ctx, int phi, int phiv, int rhs, nsx_solve_info *info);
+/* BICGSTAB_SOLVE (bicgstab.F90:303-591) alone: A phi = rhs with the PPE
+ * matrix of the set-up (the matrix bicgstab passes it), from the phi given,
+ * with the reference's stopping rule, in FP64; no mean removal */
+int nsx_bicgstab_solve(nsx_ctx *ctx, int phi, int rhs, nsx_solve_info *info);
 /* newu3D (newu3D.F90:8-257): U1FACE..UBFACE -= dt grad_n p */
 int nsx_newu3d(nsx_ctx *ctx, int p, int pv);
 

@@ -154,6 +154,11 @@ class Oracle:
 
     # -- single routines on caller arrays (reference layout, modified in place)
     def call(self, name: str, *arrays, tagdir=None):
+        f = getattr(self.L, "orc_" + name)
+        want = len(f.argtypes) - 1 - (1 if name in ("bcglobalvc", "bcglobalp") else 0)
+        if len(arrays) != want:
+            # the C routine would read (and write) the wrong buffers
+            raise TypeError(f"orc_{name} takes {want} arrays, got {len(arrays)}")
         args = [self.h] + [_ptr(a) for a in arrays]
         if tagdir is not None:
             args.append(tagdir)

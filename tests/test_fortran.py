@@ -223,3 +223,125 @@ def _numbers(text):
                 t = t.replace(".", "0.", 1)
             out.append(float(t))
     return np.array(out)
+
+
+ROUTINES = os.path.join(REPO, "tests", "fortran", "_build", "routines_driver")
+ROUTINE_NAMES = ("bcglobalvc_", "bcglobalp_", "gradientp_", "calcul_face_", "interpolateu_", "sgles_",
+                 "advdiffns_", "source3d_", "update_rhsp_", "newu3d_", "bicgstab_solve_", "looptimeinitial_",
+                 "looptimeupdate_")
+
+
+def test_dropin_exports_routine_names():
+    """The remaining hot-path routines of src/AdvectionDiffusion, src/Gradient,
+    src/LES, src/Interpolation, src/BoundaryConditions, the pressure solver and
+    src/LoopUpdate link under the reference's external names."""
+    if not os.path.exists(FLIB):
+        pytest.skip("the Fortran drop-in is not built (no Fortran compiler)")
+    out = subprocess.run(["nm", FLIB], capture_output=True, text=True, check=True).stdout
+    for sym in ROUTINE_NAMES:
+        assert f" T {sym}\n" in out, sym
+
+
+def _read_routines(case, path):
+    nc, nz, nv, n0 = case.N_CELL, case.NZ, case.N_VERT, case.N_CELL0
+    d = np.fromfile(path, dtype=np.float64)
+    o = 0
+
+    def take(shape):
+        nonlocal o
+        n = int(np.prod(shape))
+        a = d[o:o + n].reshape(shape, order="F")
+        o += n
+        return a
+
+    r = dict(f0=take((nc, nz, 4)), vtx=take((nv, nz - 1, 4)), grd=take((nc, nz, 3)), fc=take((n0, nz, 5)),
+             fu=take((n0, nz, 5)), eddy=take((nc, nz)), src=take((nc, nz)), rb=take((nc, nz, 3)),
+             r2=take((nc, nz)), pv2=take((nv, nz - 1)), pv3=take((nv, nz - 1)), nf=take((n0, nz, 5)),
+             phi=take((nc, nz)), lev=take((nc, nz, 10)), eta=take((nc,)), cct=take((nc, nz, 3)),
+             cvt=take((nv, nz - 1, 3)))
+    assert o == d.size
+    return r
+
+
+@pytest.mark.gpu
+def test_routine_dropins_vs_oracle(tmp_path):
+    """Each single-routine drop-in, called from Fortran under the reference's
+    name on the tiny jittered mesh, against the oracle's routine on the same
+    inputs: bit for bit where the C-ABI parity tests are (ghost fills, vertex
+    values, gradients, faces, fluxes, divergence, face correction), 1e-14 for
+    the eddy viscosity (device pow), 1e-6 for BICGSTAB_SOLVE (summation
+    order; compared after removing the volume mean, which BICGSTAB_SOLVE
+    leaves to bicgstab's wrapper)."""
+    if not os.path.exists(ROUTINES):
+        pytest.skip("the Fortran test driver is not built (no Fortran compiler)")
+    c, p0 = tiny_case()
+    rhs = _rhs(c)
+    case_file = tmp_path / "case.bin"
+    c.arrays["p0"] = p0
+    write_case(c, rhs, case_file)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([ROUTINES, str(case_file), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _read_routines(c, out)
+    O = oracle.Oracle(c, p0=p0)
+    n0 = c.N_CELL0
+    u, v, w = (np.asfortranarray(c.arrays[n]).copy(order="F") for n in ("u0", "v0", "w0"))
+    p = p0.copy(order="F")
+    uv, vv, wv, pv = O.vert(), O.vert(), O.vert(), O.vert()
+    O.call("bcglobalvc", u, v, w, uv, vv, wv, tagdir=2)
+    O.call("bcglobalp", p, pv, tagdir=3)
+    for q, a in enumerate((u, v, w, p)):
+        assert np.array_equal(got["f0"][:c.N_CELLexact, :, q], a[:c.N_CELLexact]), ("ghost fill", q)
+    for q, a in enumerate((uv, vv, wv, pv)):
+        assert np.array_equal(got["vtx"][:, :, q], a), ("vertex values", q)
+    gx, gy, gs = O.cell(), O.cell(), O.cell()
+    O.call("grad_lsm", gx, gy, gs, p)
+    for q, a in enumerate((gx, gy, gs)):
+        assert np.array_equal(got["grd"][:n0, :, q], a[:n0]), ("gradientP", q)
+    f = [O.face() for _ in range(5)]
+    O.call("calcul_face", *f, u, uv)
+    for q in range(5):
+        assert np.array_equal(got["fc"][:, 1:-1, q], f[q][:, 1:-1]), ("calcul_face", q)
+    O.call("interpolate_u", u.copy(order="F"), v.copy(order="F"), w.copy(order="F"))
+    for q, n in enumerate(("U1", "U2", "U3", "UT", "UB")):
+        assert np.array_equal(got["fu"][:, 1:-1, q], O.state(n)[:, 1:-1]), ("interpolateU", n)
+    O.call("sgles", u.copy(order="F"), v.copy(order="F"), w.copy(order="F"))
+    ed = O.state("eddy").copy(order="F")
+    assert np.abs(interior(c, got["eddy"]) - interior(c, ed)).max() <= 1e-14 * np.abs(interior(c, ed)).max()
+    src = O.cell()
+    O.call("source3d", src, u.copy(order="F"), v.copy(order="F"), w.copy(order="F"))
+    assert np.array_equal(interior(c, got["src"]), interior(c, src))
+    prm = c.params
+    rb = [O.cell() + prm.bdx, O.cell() + prm.bdy, O.cell() + prm.bdz]
+    # the drop-in advected the device's eddy viscosity; the oracle gets the same field
+    O.call("advdiff", rb[0], rb[1], rb[2], u.copy(order="F"), v.copy(order="F"), w.copy(order="F"),
+           got["eddy"].copy(order="F"))
+    for q in range(3):
+        assert np.array_equal(interior(c, got["rb"][:, :, q]), interior(c, rb[q])), ("AdvDiffNS", q)
+    r2 = src.copy(order="F") / prm.dt
+    pv2 = O.vert()
+    O.call("update_rhsp", r2, p.copy(order="F"), pv2)
+    assert np.array_equal(interior(c, got["r2"]), interior(c, r2))
+    assert np.array_equal(got["pv2"], pv2)
+    pv3 = O.vert()
+    O.call("newu3d", p.copy(order="F"), pv3)
+    assert np.array_equal(got["pv3"], pv3)
+    for q, n in enumerate(("U1", "U2", "U3", "UT", "UB")):
+        assert np.array_equal(got["nf"][:, 1:-1, q], O.state(n)[:, 1:-1]), ("newu3D", n)
+    # BICGSTAB_SOLVE against the oracle's bicgstab (the same solve inside its mean removals)
+    phi = c.cell_field()
+    O.call("bicgstab", phi, O.vert(), rhs.copy(order="F"))
+    vol = c.areaCell[:n0, None] * c.dsig[None, 1:-1]
+    gphi = interior(c, got["phi"])
+    gphi = gphi - (gphi * vol).sum() / vol.sum()
+    assert rel_err(gphi, interior(c, phi)) <= 1e-6
+    # LoopTimeInitial (n+1 <- n) then LoopTimeUpdate (n <- n+1, the n+1 fields
+    # having been raised by one): every level field ends at its own value + 1
+    for k in range(10):
+        assert np.all(got["lev"][:, :, k] == k + 2.0), k
+    assert np.all(got["eta"] == 0.5)
+    A = c.arrays
+    assert np.array_equal(got["cct"][:, :, 2], np.repeat(A["sig"][None, :], c.N_CELL, axis=0))
+    # the driver passes x = 0 for the centres and the vertices
+    assert np.all(got["cvt"][:, :, 0] == 0.0) and np.all(got["cct"][:, :, 0] == 0.0)
+    assert np.array_equal(got["cvt"][:, :, 2], np.repeat(A["sigv"][None, :], c.N_VERT, axis=0))

@@ -314,3 +314,32 @@ def test_fp64_cg_steps_vs_fp64():
         G.close()
     errs = {f: rel_err(out[1][f], out[0][f]) for f in out[0]}
     assert max(errs.values()) <= 1e-8, errs
+
+
+@pytest.mark.parametrize("cfg", [0, 3])
+def test_fp32_flux_form_errors(cfg):
+    """The precision form with FP32 flux and eddy-viscosity arithmetic
+    (PREC_MIXED_FLUX32) against the FP64 path over three RK2 steps, both
+    solves at eps 1e-13: the FP32 rate enters u* = u + dt r through dt, so
+    the velocities stay within 1e-6 (measured 2e-8 to 2.3e-7); the eddy
+    viscosity within 1e-6 (2e-7); the pressure, whose right-hand side is
+    the divergence of the FP32 rate, within 1e-5 (2e-6).  These are above
+    the 1e-8 gate of the default precision form, which keeps the fluxes in
+    FP64 (printed: the errors the form is reported with)."""
+    from nsx.abi import PREC_MIXED_FLUX32
+    c = config_case(cfg)
+    out = []
+    for prec in (PREC_FP64, PREC_MIXED_FLUX32):
+        G = _ctx(c)
+        G.set_params(eps=1e-13)
+        if prec == PREC_MIXED_FLUX32:
+            G.set_precision(PREC_MIXED_FLUX32, REFINE_CGSR_BLOCK, 0.1, 30)
+        G.init_state()
+        info = G.step(3, prec)
+        assert info["exit"] == "converged", (prec, info)
+        out.append({f: interior(c, G.get(f)) for f in ("UFN", "VFN", "WFN", "PFN", "EDDY")})
+        G.close()
+    errs = {f: rel_err(out[1][f], out[0][f]) for f in out[0]}
+    print(f"workload {cfg}: FP32 flux form vs FP64:", errs, flush=True)
+    assert max(errs["UFN"], errs["VFN"], errs["WFN"]) <= 1e-6, errs
+    assert errs["PFN"] <= 1e-5 and errs["EDDY"] <= 1e-6, errs
