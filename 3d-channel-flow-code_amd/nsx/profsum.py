@@ -33,7 +33,9 @@ def _csvs(d: str, suffix: str):
 def short(name: str) -> str:
     n = name.replace("(anonymous namespace)::", "")
     n = n.split("(")[0].strip()
-    return n[5:] if n.startswith("void ") else n
+    n = n[5:] if n.startswith("void ") else n
+    # template arguments without spaces, so a name is one word: k_cgs<float,1>
+    return n.replace(", ", ",")
 
 
 def stats(d: str) -> str:

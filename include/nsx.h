@@ -67,7 +67,7 @@ typedef struct nsx_solve_info {
   double e2norm;   /* ||r||_2 / ||b||_2 */
   int n_solves;    /* solves since nsx_init_state */
   int iters_total; /* iterations since nsx_init_state */
-  int outer_passes; /* NSX_PREC_MIXED: refinement passes (restart) or reliable updates of the last solve */
+  int outer_passes; /* NSX_PREC_MIXED, NSX_PREC_FP64_CG: reliable updates of the last solve */
   int iters_fp32;   /* NSX_PREC_MIXED: FP32 iterations of the last solve (included in iters) */
 } nsx_solve_info;
 
@@ -201,7 +201,8 @@ int nsx_bicgstab_adjoint(nsx_ctx *ctx, int lam, int g, nsx_solve_info *info);
  * RK2 steps (FinRK = 2):
  *   J = (1/nsteps) sum_{n=1..nsteps} sum_i area_i nu (u_n(i,2) - u_n(i,1)) / dsig(1) / sum_i area_i,
  * with u(i,1) the bottom ghost layer (-u(i,2) at a no-slip wall).  Solves run
- * in FP64 BiCGSTAB to eps (nsx_set_params); the adjoint solves use A^T. */
+ * in FP64 BiCGSTAB to eps (nsx_set_params); the adjoint solves use A^T, by
+ * the FP64 single-reduction CG at eps/100 (at least 1e-16). */
 int nsx_set_body_force(nsx_ctx *ctx, int on);
 /* nsteps RK2 steps from the resident state (which ends at the new level, as
  * after nsx_step) and J; when grad >= 0 (a cell field other than

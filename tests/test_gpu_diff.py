@@ -18,8 +18,9 @@ kernels:
   to eps = 1e-13, so that the finite differences are not swamped by the
   solver's stopping error (FP64 BiCGSTAB does not reach 1e-14 on every
   perturbed force of this mesh).  Both derivatives are exact up to their
-  pressure solves' stopping errors, which set their 1e-7 agreement (5e-10
-  with solves to 1e-14);
+  pressure solves' stopping errors; the adjoint solves run a hundred times
+  tighter than eps (FP64 single-reduction CG), and the adjoint and tangent
+  gradients agree to about 1e-15;
 - torch.autograd: the gradient of J and of a function of J through
   WallShear.backward equals the adjoint gradient (to 1e-7 of its largest
   entry: the adjoint kernels scatter with atomic adds, so two runs differ by
