@@ -22,6 +22,36 @@ int nsx_grid_for(uint32_t n) {
   for (uint32_t base_ = beg_; base_ < end_; base_ += NSX_BLOCK)                     \
     for (uint32_t c = base_ + threadIdx.x, once_ = 1; once_ && c < end_; once_ = 0)
 
+// The element-wise stencil kernels sweep each XCD's eighth of the cells band
+// by band: the XCD's G/8 blocks take consecutive batches of NSX_BLOCK cells
+// in turn, so the columns a batch gathers from (its horizontal neighbours,
+// mostly in the same or the next batches) are fetched into that XCD's L2
+// while the blocks that own them load them too.  With one contiguous chunk
+// per block, an XCD's blocks were spread over its whole slice and a
+// neighbour column was often evicted between the two loads.  (Kernels that
+// accumulate partial sums keep FOR_EACH_CELL, whose order their sums follow.)
+__device__ inline void band_range(uint32_t n, uint32_t &beg, uint32_t &end, uint32_t &step) {
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  if ((G % NSX_NXCD) == 0) {
+    uint32_t per = (n + NSX_NXCD - 1) / NSX_NXCD;
+    per = (per + NSX_BLOCK - 1) / NSX_BLOCK * NSX_BLOCK;
+    const uint64_t s0 = (uint64_t)(b % NSX_NXCD) * per, s1 = s0 + per;
+    beg = (uint32_t)(s0 + (uint64_t)(b / NSX_NXCD) * NSX_BLOCK < n ? s0 + (uint64_t)(b / NSX_NXCD) * NSX_BLOCK : n);
+    end = (uint32_t)(s1 < n ? s1 : n);
+    step = G / NSX_NXCD * NSX_BLOCK;
+  } else {
+    beg = b * NSX_BLOCK < n ? b * NSX_BLOCK : n;
+    end = n;
+    step = G * NSX_BLOCK;
+  }
+}
+
+#define FOR_EACH_CELL_BAND(g, c)                                                    \
+  uint32_t beg_, end_, step_;                                                       \
+  band_range((g).ncell, beg_, end_, step_);                                         \
+  for (uint32_t base_ = beg_; base_ < end_; base_ += step_)                         \
+    for (uint32_t c = base_ + threadIdx.x, once_ = 1; once_ && c < end_; once_ = 0)
+
 // ------------------------------------------------------- cell -> vertex ----
 // interpolation2D / interpolation3D(P) + BCvelvertex3D / BCpvertex +
 // bc_periodic (interpolation2D.F90:8-150, interpolation3D.F90:8-444,
@@ -36,16 +66,51 @@ struct VtxArgs {
   int tagdir;
 };
 
-__device__ inline double idw(const Geo &g, int v, int layer, const double *f) {
-  double s = 0.0;
+// The inverse-distance sums at layers l (s0) and l + 1 (s1, when two) of
+// vertex v, each added in the stencil's order.  The stencil entries are
+// taken four at a time so that their loads are in flight together.
+template <bool TWO>
+__device__ inline void idw_sums(const Geo &g, int v, int l, const double *f, double &s0, double &s1) {
   const int b = g.vptr[v], e = g.vptr[v + 1];
-  for (int j = b; j < e; ++j) s = s + g.vw[j] * f[(size_t)g.vcell[j] * g.NS + layer];
+  s0 = 0.0;
+  s1 = 0.0;
+  int j = b;
+  for (; j + 4 <= e; j += 4) {
+    double a[4], c[4], w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const size_t o = (size_t)g.vcell[j + q] * g.NS + l;
+      w[q] = g.vw[j + q];
+      a[q] = f[o];
+      if (TWO) c[q] = f[o + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      s0 = s0 + w[q] * a[q];
+      if (TWO) s1 = s1 + w[q] * c[q];
+    }
+  }
+  for (; j < e; ++j) {
+    const size_t o = (size_t)g.vcell[j] * g.NS + l;
+    s0 = s0 + g.vw[j] * f[o];
+    if (TWO) s1 = s1 + g.vw[j] * f[o + 1];
+  }
+}
+
+__device__ inline double idw(const Geo &g, int v, int layer, const double *f) {
+  double s, unused;
+  idw_sums<false>(g, v, layer, f, s, unused);
   return s / g.dlVsum[v];
 }
 
+// Blocks are dealt round-robin to the 8 XCDs; the remap gives each XCD one
+// contiguous range of vertices, so the columns that neighbouring vertices
+// share are fetched into one L2 (grid a multiple of 8).
 template <int NF>
 __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
-  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t lb = (G % NSX_NXCD) == 0 ? (b % NSX_NXCD) * (G / NSX_NXCD) + b / NSX_NXCD : b;
+  const uint32_t t = lb * NSX_BLOCK + threadIdx.x;
   if (t >= (uint32_t)g.NV * g.NL) return;
   const int v = (int)fdiv(t, g.divNL);
   const int l = (int)(t - (uint32_t)v * g.NL);
@@ -61,10 +126,14 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
     const bool zero_wall = (a.tagdir == 3) && ((l == 0 && a.zero_bot[q]) || (l == g.NL - 1 && a.zero_top[q]));
     auto val = [&](int vv) -> double {
       double r;
-      if (a.tagdir == 3)
-        r = (dzB * idw(g, vv, l + 1, f) + dzT * idw(g, vv, l, f)) / (dzT + dzB);
-      else
+      if (a.tagdir == 3) {
+        double sl, su;
+        idw_sums<true>(g, vv, l, f, sl, su);
+        const double ds = g.dlVsum[vv];
+        r = (dzB * (su / ds) + dzT * (sl / ds)) / (dzT + dzB);
+      } else {
         r = idw(g, vv, l, f);
+      }
       if (act[vv] == VA_ZERO) r = 0.;
       if (zero_wall) r = 0.;
       return r;
@@ -105,7 +174,7 @@ int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3],
   }
   a.tagdir = tagdir;
   const uint32_t n = (uint32_t)g.NV * g.NL;
-  const dim3 grid((n + NSX_BLOCK - 1) / NSX_BLOCK);
+  const dim3 grid(((n + NSX_BLOCK - 1) / NSX_BLOCK + NSX_NXCD - 1) / NSX_NXCD * NSX_NXCD);
   if (nf == 3) hipLaunchKernelGGL(k_vertex<3>, grid, dim3(NSX_BLOCK), 0, s, g, a);
   else hipLaunchKernelGGL(k_vertex<1>, grid, dim3(NSX_BLOCK), 0, s, g, a);
   return (int)hipGetLastError();
@@ -131,7 +200,7 @@ __device__ inline void faces_of(const Geo &g, int i, int k, size_t s, const doub
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_calcul_face(Geo g, const double *f, const double *fv, double *o0,
                                                             double *o1, double *o2, double *o3, double *o4) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -161,7 +230,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const
                                                     const double *uv, const double *vv, const double *wv,
                                                     double *eddy) {
   const double Cs = 0.1, karman = (double)0.415f;
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -229,7 +298,7 @@ struct MomArgs {
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
   const double bd[3] = {g.bdx, g.bdy, g.bdz};
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -344,7 +413,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les32(Geo g, const double *u, con
                                                       const double *uv, const double *vv, const double *wv,
                                                       double *eddy) {
   const float Cs = 0.1f, karman = 0.415f;
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -391,7 +460,7 @@ int launch_les32(const Geo &g, hipStream_t s, const double *u, const double *v, 
 }
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_momentum32(Geo g, MomArgs a) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -540,7 +609,7 @@ int launch_source(const Geo &g, hipStream_t s, const double *const uvw[3], doubl
 
 // update_rhsp alone (source3D.F90:255-288) for single-routine use
 __global__ __launch_bounds__(NSX_BLOCK) void k_update_rhsp(Geo g, double *rhs, const double *pv) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -589,7 +658,7 @@ __device__ inline void lsm_grad(const Geo &g, int i, int k, size_t s, const doub
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_grad_lsm(Geo g, const double *f, double *dx, double *dy,
                                                          double *ds) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -624,7 +693,7 @@ int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, do
 __global__ __launch_bounds__(NSX_BLOCK) void k_corrector(Geo g, const double *p, const double *us,
                                                           const double *vs, const double *ws, double *un,
                                                           double *vn, double *wn) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
@@ -668,7 +737,7 @@ int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double 
 // Bytes/cell: read p, p vertex (0.5), 5 faces = 6.5; write 5 faces.
 __global__ __launch_bounds__(NSX_BLOCK) void k_newu(Geo g, const double *p, const double *pv, double *F1,
                                                      double *F2, double *F3, double *FT, double *FB) {
-  FOR_EACH_CELL(g, c) {
+  FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
     const size_t s = (size_t)i * g.NS + k;
