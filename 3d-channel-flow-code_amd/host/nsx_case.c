@@ -20,8 +20,10 @@
  * array is bit-identical to what the Fortran set-up produces for the same
  * vertices; tests check that against the compiled reference.
  *
- * Only the doubly periodic horizontal box (XPB=YPB=1, the reference's channel
- * test) is generated; a ghost cell without a periodic partner is an error.
+ * The box is periodic in x and y (XPB=YPB=1, the reference's channel test)
+ * or walled on either pair of sides (XPB or YPB = 0, wall kinds XBC, YBC):
+ * a wall ghost keeps the centre mirrored across its edge and gets the flags
+ * of update_bc.F90 (wall 0, free slip 1, inflow 2).
  */
 #include <math.h>
 #include <stdint.h>
@@ -363,23 +365,24 @@ static int build_geometry(nsx_case *c) {
       CP(g, 2) = -1;
 
       double dxl = XDFin - XDIni, dyl = YDFin - YDIni;
-      if (xc[g] > XDFin) {
-        dxl = -1.0 * dxl;
-        dyl = 0.;
-      } else if (xc[g] < XDIni) {
-        dyl = 0.;
+      if (c->XPB == 1) {
+        if (xc[g] > XDFin) {
+          dxl = -1.0 * dxl;
+          dyl = 0.;
+        } else if (xc[g] < XDIni) {
+          dyl = 0.;
+        }
       }
-      if (yc[g] > YDFin) {
-        dxl = 0.;
-        dyl = -1.0 * dyl;
-      } else if (yc[g] < YDIni) {
-        dxl = 0.;
+      if (c->YPB == 1) {
+        if (yc[g] > YDFin) {
+          dxl = 0.;
+          dyl = -1.0 * dyl;
+        } else if (yc[g] < YDIni) {
+          dxl = 0.;
+        }
       }
-      if (dxl * dyl > F32_1EM7) {
-        snprintf(c->err, sizeof c->err, "ghost cell of cell %d edge %d has no periodic partner", i + 1, j + 1);
-        gerr = -1;
-        goto done;
-      }
+      /* not periodic: the mirrored centre stays, No_cp(g,3) = -1 */
+      if (dxl * dyl > F32_1EM7) continue;
       int jv1p = -1, jv2p = -1, ncount = 0, ncand = 0;
       /* the boundary vertex at each end of the edge shifted across the box */
       xee = xv[jv1] + dxl;
@@ -495,7 +498,10 @@ done:
   for (int nv = 0; nv < NV; ++nv) {
     double xee = xv[nv], yee = yv[nv];
     int ncount = (tagXV[nv] * tagYV[nv] > 0) ? 2 : ((tagXV[nv] > 0 || tagYV[nv] > 0) ? 1 : 0);
-    if (ncount == 1) {
+    if (ncount == 1 || (ncount == 2 && c->XPB * c->YPB != 1)) {
+      /* a boundary vertex, or a corner of a box periodic in one direction
+       * only (vertex_edge 0, its pair searched the same way) */
+      if (ncount == 2) c->vertex_edge[nv] = 0;
       int vcount = 0, ncand = 0;
       if (inbox) {
         const double Lx = XDFin - XDIni, Ly = YDFin - YDIni;
@@ -508,11 +514,11 @@ done:
       for (int q = 0; q < nscan; ++q) {
         const int i = inbox ? cand[q] : q;
         if (nbev[i] == 0) continue;
-        if (fabs(xee - xv[i]) < 1.0e-7 && fabs(yee - yv[i]) > (YDFin - YDIni - 1.0e-7)) {
+        if (c->YPB == 1 && fabs(xee - xv[i]) < 1.0e-7 && fabs(yee - yv[i]) > (YDFin - YDIni - 1.0e-7)) {
           ++vcount;
           c->vertex_pair[nv] = i + 1;
         }
-        if (fabs(yee - yv[i]) < 1.0e-7 && fabs(xee - xv[i]) > (XDFin - XDIni - 1.0e-7)) {
+        if (c->XPB == 1 && fabs(yee - yv[i]) < 1.0e-7 && fabs(xee - xv[i]) > (XDFin - XDIni - 1.0e-7)) {
           ++vcount;
           c->vertex_pair[nv] = i + 1;
         }
@@ -525,7 +531,7 @@ done:
         return -1;
       }
     }
-    if (ncount == 2) c->vertex_edge[nv] = 1; /* XPB*YPB = 1 */
+    else if (ncount == 2) c->vertex_edge[nv] = 1; /* XPB*YPB = 1 */
   }
   free(cand);
   vgrid_free(&vg);
@@ -606,43 +612,107 @@ done:
 }
 
 /* -------------------------------------------------------- BC flags ------- */
-/* adjust_tagv / adjust_bc (update_bc.F90:4-848) for XPB=YPB=1, TopBC/BotBC walls */
+/* TagXC/TagYC of a cell from its centre (calcul_geometry.F90:857-886): 1 or 2
+ * below or above the box in x (y), 0 inside */
+static void cell_tags(const nsx_case *c, int n, int *tx, int *ty) {
+  *tx = *ty = 0;
+  if (c->xc[n] < c->XDIni) *tx = 1;
+  if (c->xc[n] > c->XDFin) *tx = 2;
+  if (c->yc[n] < c->YDIni) *ty = 1;
+  if (c->yc[n] > c->YDFin) *ty = 2;
+}
+
+/* the flag of a wall ghost for component q (0 u, 1 v, 2 w, 3 p) from its
+ * tags (update_bc.F90:201-516); -9 when no rule applies */
+static int wall_flag(const nsx_case *c, int q, int tx, int ty) {
+  if (q == 3) return 1; /* all Neumann for p */
+  int fB = -9, flag = 0;
+  if (tx > 0) {
+    ++flag;
+    if (c->XBC == 0) fB = 0;
+    else if (c->XBC == 1) fB = q == 0 ? 0 : 1;
+    else if (c->XBC == 2) fB = tx == 1 ? 2 : 1;
+    else fB = -9;
+  }
+  if (ty > 0) {
+    ++flag;
+    if (c->YBC == 0) fB = 0;
+    else if (c->YBC == 1) fB = q == 0 ? 2 : (q == 1 ? 0 : 1);
+    else fB = -9;
+  }
+  return flag == 1 ? fB : -9;
+}
+
+/* the flag of a boundary vertex for component q (update_bc.F90:531-780) */
+static int vertex_flag(const nsx_case *c, int q, int nv) {
+  int fB = -9, flag = 0;
+  const int tx = c->tagXV[nv], ty = c->tagYV[nv];
+  if (tx > 0) {
+    ++flag;
+    if (c->XPB != 0) fB = 3;
+    else if (q == 3) fB = 1;
+    else if (c->XBC == 0) fB = 0;
+    else if (c->XBC == 1) fB = q == 0 ? 0 : 1;
+    else if (c->XBC == 2) fB = tx == 1 ? 2 : 1;
+  }
+  if (ty > 0) {
+    ++flag;
+    if (c->YPB != 0) fB = 3;
+    else if (q == 3) fB = 1;
+    else if (c->YBC == 0) fB = 0;
+    else if (c->YBC == 1) fB = q == 1 ? 0 : 1;
+  }
+  return flag == 1 ? fB : -9;
+}
+
+/* adjust_tagv / adjust_bc (update_bc.F90:4-848), TopBC/BotBC walls */
 static int build_bc(nsx_case *c) {
   const int N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL;
   for (int i = 0; i < NC; ++i) {
     c->bcuc[i] = c->bcvc[i] = c->bcwc[i] = c->bcpc[i] = -1;
     c->tagsp[i] = 0;
   }
+  int *const cf[4] = {c->bcuc, c->bcvc, c->bcwc, c->bcpc};
   for (int i = 0; i < N0; ++i) {
     if (c->nbe[i] == 0) continue;
     for (int j = 0; j < 3; ++j) {
       int nc = c->No_cp[IX2(i, j, NC)];
       if (!(nc < 1 || nc > N0)) continue;
       int ii = c->No_cp[IX2(nc - 1, 2, NC)];
-      int fB;
-      if (ii > 0) fB = 3;
-      else if (ii == 0) fB = -1;
-      else {
-        snprintf(c->err, sizeof c->err, "wall ghost cells are not generated by this set-up");
+      int tx, ty;
+      cell_tags(c, nc - 1, &tx, &ty);
+      if (tx > 0 && ty > 0) {
+        snprintf(c->err, sizeof c->err, "ghost cell %d lies beyond the box in x and y", nc);
         return -1;
       }
-      c->bcuc[nc - 1] = c->bcvc[nc - 1] = c->bcwc[nc - 1] = c->bcpc[nc - 1] = fB;
+      for (int q = 0; q < 4; ++q) {
+        int fB;
+        if (ii > 0) fB = 3;
+        else if (ii == 0) fB = -1;
+        else fB = wall_flag(c, q, tx, ty);
+        if (fB == -9) {
+          snprintf(c->err, sizeof c->err, "no boundary rule for ghost cell %d, component %d", nc, q + 1);
+          return -1;
+        }
+        cf[q][nc - 1] = fB;
+      }
     }
   }
   /* adjust_tagv: corners keep the y tag */
   for (int nv = 0; nv < NV; ++nv)
     if (c->nbev[nv] != 0 && c->tagXV[nv] * c->tagYV[nv] != 0) c->tagXV[nv] = 0;
+  int *const vf[4] = {c->bcuv, c->bcvv, c->bcwv, c->bcpv};
   for (int nv = 0; nv < NV; ++nv) {
     c->bcuv[nv] = c->bcvv[nv] = c->bcwv[nv] = c->bcpv[nv] = -1;
     if (c->nbev[nv] == 0) continue;
-    int flag = 0;
-    if (c->tagXV[nv] > 0) ++flag;
-    if (c->tagYV[nv] > 0) ++flag;
-    if (flag != 1) {
-      snprintf(c->err, sizeof c->err, "boundary vertex %d has %d boundary tags", nv + 1, flag);
-      return -1;
+    for (int q = 0; q < 4; ++q) {
+      const int fB = vertex_flag(c, q, nv);
+      if (fB == -9) {
+        snprintf(c->err, sizeof c->err, "no boundary rule for vertex %d, component %d", nv + 1, q + 1);
+        return -1;
+      }
+      vf[q][nv] = fB;
     }
-    c->bcuv[nv] = c->bcvv[nv] = c->bcwv[nv] = c->bcpv[nv] = 3;
     if (c->vertex_edge[nv] > 0) c->bcpv[nv] = 3;
     if (c->vertex_pair[nv] > 0) c->bcpv[nv] = 3;
   }
@@ -805,7 +875,17 @@ static nsx_case *finish_case(nsx_case *c) {
 
 nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
                           double jitter, int bed, double Ubulk, double noise, uint64_t seed) {
+  return nsx_case_create_walls(nx, ny, nn, Lx, Ly, H, stretch, jitter, bed, Ubulk, noise, seed, 1, 1, 0, 1);
+}
+
+nsx_case *nsx_case_create_walls(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
+                                double jitter, int bed, double Ubulk, double noise, uint64_t seed, int XPB, int YPB,
+                                int XBC, int YBC) {
   nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  c->XPB = XPB;
+  c->YPB = YPB;
+  c->XBC = XBC;
+  c->YBC = YBC;
   c->nx = nx; c->ny = ny; c->NN = nn; c->NZ = nn + 2;
   c->Lx = Lx; c->Ly = Ly; c->H = H; c->stretch = stretch;
   c->jitter = jitter; c->bed = bed; c->Ubulk = Ubulk; c->noise = noise; c->seed = seed;
@@ -816,8 +896,10 @@ nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H
   c->XDFin = Lx;
   c->YDIni = 0.0;
   c->YDFin = Ly;
-  if (nx < 2 || ny < 2 || nn < 2 || 2 * (nx + ny) > 100 * nn) {
-    snprintf(c->err, sizeof c->err, "bad sizes nx=%d ny=%d nn=%d", nx, ny, nn);
+  if (nx < 2 || ny < 2 || nn < 2 || 2 * (nx + ny) > 100 * nn || (XPB != 0 && XPB != 1) || (YPB != 0 && YPB != 1) ||
+      XBC < 0 || XBC > 2 || YBC < 0 || YBC > 1) {
+    snprintf(c->err, sizeof c->err, "bad sizes or boundary kinds nx=%d ny=%d nn=%d XPB=%d YPB=%d XBC=%d YBC=%d", nx,
+             ny, nn, XPB, YPB, XBC, YBC);
     c->status = -1;
     return c;
   }
@@ -834,6 +916,9 @@ nsx_case *nsx_case_from_mesh(int N_CELL0, int N_VERT, int nn, const int *No_vp, 
                              double XDFin, double YDIni, double YDFin, double H, double stretch, double Ubulk,
                              double noise, uint64_t seed) {
   nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  c->XPB = c->YPB = 1;
+  c->XBC = 0;
+  c->YBC = 1;
   c->NN = nn;
   c->NZ = nn + 2;
   c->N_CELL0 = N_CELL0;

@@ -41,7 +41,8 @@ typedef struct nsx_case {
   int nx, ny, NN, NZ, N_CELL0, N_VERT, N_CELL, N_CELLexact, N_CELLghost;
   int bctop, bcbot, bed, status;
   double Lx, Ly, H, stretch, jitter, Ubulk, noise, TotalVolume;
-  double XDIni, XDFin, YDIni, YDFin; /* the periodic box (common.mpf:66-69) */
+  double XDIni, XDFin, YDIni, YDFin; /* the box (common.mpf:66-69) */
+  int XPB, YPB, XBC, YBC;             /* periodicity and wall kinds (common.mpf:72-80) */
   uint64_t seed;
   char err[256];
   int n_arrays;

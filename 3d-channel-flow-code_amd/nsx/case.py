@@ -101,11 +101,14 @@ _FACE3 = {"dxVV", "dyVV", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "xme", "
 
 def make_case(nx: int, ny: int, nn: int, *, seed: int = 1234, jitter: float = 0.0, bed: bool = False,
               Ubulk: float = 15.6, noise: float = 0.05, stretch: float = 2.0, params: Params | None = None,
-              Lx: float = LX, Ly: float = LY, Hh: float = H) -> Case:
-    """Generate a channel case.  Raises RuntimeError with the set-up's message."""
+              Lx: float = LX, Ly: float = LY, Hh: float = H, walls: tuple | None = None) -> Case:
+    """Generate a channel case.  walls = (XPB, YPB, XBC, YBC) for a box with
+    walled sides (default periodic in x and y).  Raises RuntimeError with
+    the set-up's message."""
     lib = _lib.host()
-    ptr = lib.nsx_case_create(nx, ny, nn, Lx, Ly, Hh, stretch, jitter, int(bool(bed)), Ubulk, noise,
-                              ctypes.c_uint64(seed))
+    xpb, ypb, xbc, ybc = walls if walls is not None else (1, 1, 0, 1)
+    ptr = lib.nsx_case_create_walls(nx, ny, nn, Lx, Ly, Hh, stretch, jitter, int(bool(bed)), Ubulk, noise,
+                                    ctypes.c_uint64(seed), int(xpb), int(ypb), int(xbc), int(ybc))
     return _collect(lib, ptr, nx, ny, nn, params or Params(sigFin=Hh))
 
 

@@ -54,6 +54,14 @@ nsx_case *nsx_case_from_mesh(int N_CELL0, int N_VERT, int nn, const int *No_vp, 
  * triangles, optional jitter of interior vertices, an inert bed) */
 nsx_case *nsx_case_create(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch, double jitter,
                           int bed, double Ubulk, double noise, uint64_t seed);
+/* the same with the horizontal boundary conditions of common.mpf:72-80: XPB,
+ * YPB 1 periodic in x (y), 0 walled; XBC 0 no slip, 1 free slip, 2 inflow
+ * and outflow; YBC 0 no slip, 1 free slip.  Wall ghost cells keep the centre
+ * mirrored across their edge and get update_bc.F90's flags (nsx_case_create
+ * is XPB = YPB = 1). */
+nsx_case *nsx_case_create_walls(int nx, int ny, int nn, double Lx, double Ly, double H, double stretch,
+                                double jitter, int bed, double Ubulk, double noise, uint64_t seed, int XPB, int YPB,
+                                int XBC, int YBC);
 void nsx_case_free(nsx_case *c);
 /* 0, or -1 with the set-up's message in msg */
 int nsx_case_status(const nsx_case *c, char *msg, int len);

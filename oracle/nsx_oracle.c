@@ -19,8 +19,10 @@
  * vertex values that interpolateU computes but never reads
  * (interpolationU.F90:99-102; its ghost fill is kept).
  *
- * Supported boundary set: periodic horizontal ghosts (flag 3) and sigma walls
- * (TopBC, BotBC in {0,1}), the configuration of the reference's channel test.
+ * Supported boundary set: horizontal ghosts periodic (flag 3), walls (0) and
+ * free slip (1), as update_bc.F90 assigns them for XPB/YPB and XBC/YBC;
+ * sigma walls TopBC in {0,1,2} (2: the lid, u = 1 at the top) and BotBC in
+ * {0,1}.  Inflow ghosts (flag 2) are not supported.
  */
 #include <math.h>
 #include <stdint.h>
@@ -103,8 +105,13 @@ void orc_bcvelcenter(orc *o, double *phi, int tag) {
     for (int i = 0; i < N0; ++i) phi[CI(i, 0)] = anti ? -phi[CI(i, 1)] : phi[CI(i, 1)];
   }
   if (o->bctop == 1) {
-    int anti = (o->TopBC == 0) || (o->TopBC == 1 && tag == TAG_W);
-    for (int i = 0; i < N0; ++i) phi[CI(i, NZ - 1)] = anti ? -phi[CI(i, NZ - 2)] : phi[CI(i, NZ - 2)];
+    if (o->TopBC == 2 && tag == TAG_U) {
+      /* the lid (BC.F90:512-521) */
+      for (int i = 0; i < N0; ++i) phi[CI(i, NZ - 1)] = 2.0 - phi[CI(i, NZ - 2)];
+    } else {
+      int anti = (o->TopBC == 0) || (o->TopBC == 1 && tag == TAG_W) || o->TopBC == 2;
+      for (int i = 0; i < N0; ++i) phi[CI(i, NZ - 1)] = anti ? -phi[CI(i, NZ - 2)] : phi[CI(i, NZ - 2)];
+    }
   }
   const int *bc = tag == TAG_U ? o->bcuc : (tag == TAG_V ? o->bcvc : o->bcwc);
   for (int k = 0; k < NZ; ++k)
@@ -220,8 +227,12 @@ void orc_bcvelvertex(orc *o, double *phiv, int tag, int tagdir) {
   if (tagdir == 3) {
     if (o->bcbot == 1 && (o->BotBC == 0 || (o->BotBC == 1 && tag == TAG_W)))
       for (int nv = 0; nv < NV; ++nv) phiv[VI(nv, 0)] = 0.;
-    if (o->bctop == 1 && (o->TopBC == 0 || (o->TopBC == 1 && tag == TAG_W)))
+    if (o->bctop == 1 && (o->TopBC == 0 || (o->TopBC == 1 && tag == TAG_W) || (o->TopBC == 2 && tag != TAG_U)))
       for (int nv = 0; nv < NV; ++nv) phiv[VI(nv, NZ - 2)] = 0.;
+    /* the lid: u = 1 at the top level of the interior vertices (BC.F90:771-777) */
+    if (o->bctop == 1 && o->TopBC == 2 && tag == TAG_U)
+      for (int nv = 0; nv < NV; ++nv)
+        if (o->nbev[nv] == 0) phiv[VI(nv, NZ - 2)] = 1.0;
   }
   orc_bc_periodic(o, phiv, tag);
 }

@@ -19,18 +19,29 @@ REF_OUT = os.path.join(os.path.dirname(HERE), "_ref")
 MAGIC = 1314084913
 
 
-def driver_path(nn: int, box: tuple | None = None) -> str:
+# the channel's boundary conditions (TopBC, BotBC, XPB, YPB, XBC, YBC) of the
+# default builds; build_ref.sh gives any other set a directory of its own
+CHANNEL_BC = (0, 0, 1, 1, 0, 1)
+DEFAULT_BOX = ("6.283185307179586", "3.141592653589793")
+
+
+def driver_path(nn: int, box: tuple | None = None, bc: tuple | None = None) -> str:
     sub = f"nn{nn}" if box is None else f"nn{nn}_box{box[0]}x{box[1]}"
+    if bc is not None and tuple(bc) != CHANNEL_BC:
+        sub += "_bc" + "".join(str(int(v)) for v in bc)
     return os.path.join(REF_OUT, sub, "nsx_ref_driver")
 
 
-def ensure_built(nn: int, box: tuple | None = None) -> str:
+def ensure_built(nn: int, box: tuple | None = None, bc: tuple | None = None) -> str:
     """The reference driver for nn layers; box = (Lx, Ly) as the strings the
-    build writes into common.mpf (default 2*pi x pi)."""
-    exe = driver_path(nn, box)
+    build writes into common.mpf (default 2*pi x pi); bc = (TopBC, BotBC,
+    XPB, YPB, XBC, YBC) (default the channel's)."""
+    exe = driver_path(nn, box, bc)
     src = os.path.join(HERE, "nsx_ref_driver.F90")
     if not os.path.exists(exe) or os.path.getmtime(src) > os.path.getmtime(exe):
         extra = [] if box is None else [str(box[0]), str(box[1])]
+        if bc is not None and tuple(bc) != CHANNEL_BC:
+            extra = (extra or list(DEFAULT_BOX)) + ["2.0"] + [str(int(v)) for v in bc]
         subprocess.run(["bash", os.path.join(HERE, "build_ref.sh"), str(nn), *extra], check=True,
                        stdout=subprocess.DEVNULL)
     return exe
@@ -108,11 +119,12 @@ def parse_solves(log: str) -> list:
 
 
 def run(case, *, mode: int, nsteps: int = 1, dump_every: int = 1, p0=None, workdir: str | None = None,
-        mesh_file: str | None = None, box: tuple | None = None):
+        mesh_file: str | None = None, box: tuple | None = None, bc: tuple | None = None):
     """Run the reference; returns (dump dict, stdout text).  mesh_file: a
     data.txt the reference's input.F90 reads instead of the case's mesh;
-    box: (Lx, Ly) of a build other than the default 2*pi x pi."""
-    exe = ensure_built(case.nn, box)
+    box: (Lx, Ly) of a build other than the default 2*pi x pi; bc: the
+    boundary conditions (TopBC, BotBC, XPB, YPB, XBC, YBC) of the build."""
+    exe = ensure_built(case.nn, box, bc)
     with tempfile.TemporaryDirectory(dir=workdir) as td:
         # the reference writes its statistics to ../output/Stats of its working directory
         wd = os.path.join(td, "run")

@@ -42,6 +42,56 @@ def digest(case, a) -> str:
     return hashlib.sha256(b.tobytes()).hexdigest()
 
 
+def sha(a) -> str:
+    a = np.ascontiguousarray(a)
+    if a.dtype.kind == "f":
+        a = a + 0.0
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def setup_views(dump_or_arrays, N_CELL, N_CELLexact, Dimsurrounding):
+    """the canonical part of each set-up array: rows 1..N_CELLexact of cell
+    arrays, the interior layers of the PPE matrix, the used entries of the
+    inverse-distance stencil"""
+    out = {}
+    for k, a in dump_or_arrays.items():
+        a = np.asarray(a)
+        if k.startswith("AmC"):
+            v = a[:, 1:-1]
+        elif k in ("surrounding", "weight"):
+            v = a.copy()
+            for j in range(v.shape[1]):
+                v[Dimsurrounding <= j, j] = 0
+        elif a.ndim >= 1 and a.shape[0] == N_CELL:
+            v = a[:N_CELLexact]
+        else:
+            v = a
+        out[k] = v
+    return out
+
+
+def load_bc():
+    """the boundary-condition fixtures (tests/golden/bc_cases.json)"""
+    with open(os.path.join(GOLDEN, "bc_cases.json")) as f:
+        return json.load(f)
+
+
+def bc_case(rec, name: str, which: str):
+    """the case of boundary set `name` on the tiny mesh (which = "tiny",
+    with its pressure field) or the 3-step mesh (which = "steps")"""
+    from nsx.case import Params
+    e = rec["sets"][name]
+    m = rec["tiny"] if which == "tiny" else rec["steps_case"]
+    c = make_case(m["nx"], m["ny"], m["nn"], seed=m["seed"], jitter=m["jitter"], bed=m.get("bed", False),
+                  walls=tuple(e["walls"]), params=Params(TopBC=e["TopBC"]))
+    if which != "tiny":
+        return c
+    rng = np.random.default_rng(rec["tiny_p_seed"])
+    p0 = np.zeros((c.N_CELL, c.NZ), order="F")
+    p0[:c.N_CELL0, 1:-1] = rng.uniform(-1.0, 1.0, (c.N_CELL0, c.NZ - 2))
+    return c, p0
+
+
 def interior(case, a):
     """Interior cells (i <= N_CELL0, k = 2..NZ-1) of an (N_CELL,NZ) array."""
     return np.asarray(a)[:case.N_CELL0, 1:-1]

@@ -20,7 +20,14 @@ what they produce:
   cfg3_steps.json workload 3's jittered mesh (64x32 x 48 layers), 2 steps;
   mesh_setup.json the set-up of a mesh read from the reference's mesh file
                   (data.txt through its own input.F90): SHA-256 of the file
-                  and of every set-up array.
+                  and of every set-up array;
+  bc_cases.json   boundary conditions other than the channel's, each on
+                  the tiny mesh (set-up arrays and every routine once) and
+                  on a 12x8 quads x 8 layers jittered mesh (3 RK2 steps):
+                  SHA-256 of the set-up arrays, of each routine's output and
+                  of the state after each step, and the solves' iteration
+                  counts.  The sets: a duct (walls in y, no slip), free-slip
+                  walls in x, a walled box, and the lid (TopBC = 2).
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
 adding +0.0 (so -0.0 and +0.0 hash alike).  Run in the build container:
@@ -39,8 +46,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(REPO, "3d-channel-flow-code_amd"))
 sys.path.insert(0, os.path.join(REPO, "oracle", "ref"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
 
 from nsx.case import config_case, make_case  # noqa: E402
+from common import setup_views, sha  # noqa: E402
 
 TINY = dict(nx=6, ny=4, nn=4, seed=7, jitter=0.1, bed=True)
 TINY_P_SEED = 3
@@ -131,34 +140,6 @@ def main():
 MESH_CASE = dict(nx=32, ny=16, nn=32, seed=5, jitter=0.1, Lx=6.4, Ly=3.2)
 
 
-def setup_views(dump_or_arrays, N_CELL, N_CELLexact, Dimsurrounding):
-    """the canonical part of each set-up array: rows 1..N_CELLexact of cell
-    arrays, the interior layers of the PPE matrix, the used entries of the
-    inverse-distance stencil"""
-    out = {}
-    for k, a in dump_or_arrays.items():
-        a = np.asarray(a)
-        if k.startswith("AmC"):
-            v = a[:, 1:-1]
-        elif k in ("surrounding", "weight"):
-            v = a.copy()
-            for j in range(v.shape[1]):
-                v[Dimsurrounding <= j, j] = 0
-        elif a.ndim >= 1 and a.shape[0] == N_CELL:
-            v = a[:N_CELLexact]
-        else:
-            v = a
-        out[k] = v
-    return out
-
-
-def sha(a) -> str:
-    a = np.ascontiguousarray(a)
-    if a.dtype.kind == "f":
-        a = a + 0.0
-    return hashlib.sha256(a.tobytes()).hexdigest()
-
-
 def mesh_setup():
     import tempfile
 
@@ -184,6 +165,55 @@ def mesh_setup():
     print("mesh_setup.json:", len(rec["arrays"]), "arrays")
 
 
+# (name, (XPB, YPB, XBC, YBC), TopBC) of the boundary-condition fixtures
+BC_SETS = [("duct", (1, 0, 0, 0), 0), ("slip_x", (0, 1, 1, 1), 0), ("box", (0, 0, 0, 0), 0),
+           ("lid", (1, 1, 0, 1), 2)]
+BC_STEPS = dict(nx=12, ny=8, nn=8, seed=11, jitter=0.1, nsteps=3)
+
+
+def bc_case(walls, top, nx, ny, nn, seed, jitter, bed=False):
+    from nsx.case import Params
+    return make_case(nx, ny, nn, seed=seed, jitter=jitter, bed=bed, walls=walls, params=Params(TopBC=top))
+
+
+def bc_cases():
+    import refrun
+
+    rec = {"tiny": dict(TINY), "tiny_p_seed": TINY_P_SEED, "steps_case": dict(BC_STEPS), "sets": {}}
+    for name, walls, top in BC_SETS:
+        bc = (top, 0) + tuple(walls)
+        c = bc_case(walls, top, TINY["nx"], TINY["ny"], TINY["nn"], TINY["seed"], TINY["jitter"], TINY["bed"])
+        rng = np.random.default_rng(TINY_P_SEED)
+        p0 = np.zeros((c.N_CELL, c.NZ), order="F")
+        p0[:c.N_CELL0, 1:-1] = rng.uniform(-1.0, 1.0, (c.N_CELL0, c.NZ - 2))
+        d, log = refrun.run(c, mode=1, p0=p0, bc=bc)
+        ne = int(d["sizes"][0])
+        skip = {"sizes", "TotalVolume", "serial_files", "stats_files"}
+        geo_keys = [k for k in d if k not in skip and k in c.arrays]
+        views = setup_views({k: d[k] for k in geo_keys}, c.N_CELL, ne, np.asarray(d["Dimsurrounding"]))
+        unit = {k: digest(c, v) for k, v in d.items()
+                if isinstance(v, np.ndarray) and v.dtype == np.float64 and k not in c.arrays and k not in skip}
+        entry = {"walls": list(walls), "TopBC": top, "sizes": [int(x) for x in d["sizes"]],
+                 "TotalVolume": float(d["TotalVolume"][0]), "setup": {k: sha(v) for k, v in views.items()},
+                 "unit": unit, "unit_solves": [s_[0] for s_ in refrun.parse_solves(log)]}
+        m = BC_STEPS
+        c = bc_case(walls, top, m["nx"], m["ny"], m["nn"], m["seed"], m["jitter"])
+        d, log = refrun.run(c, mode=2, nsteps=m["nsteps"], dump_every=1, bc=bc)
+        steps = {}
+        for s_ in range(0, m["nsteps"] + 1):
+            steps[str(s_)] = {nm: digest(c, d[f"{nm}_{s_}"]) for nm in STEP_ARRAYS if f"{nm}_{s_}" in d}
+        entry["steps"] = steps
+        entry["solves"] = [s_[0] for s_ in refrun.parse_solves(log)]
+        rec["sets"][name] = entry
+        print(f"bc_cases.json: {name}: {len(unit)} routine outputs, {len(entry['solves'])} solves")
+    with open(os.path.join(HERE, "bc_cases.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
 if __name__ == "__main__":
-    main()
-    mesh_setup()
+    if len(sys.argv) > 1 and sys.argv[1] == "bc":
+        bc_cases()
+    else:
+        main()
+        mesh_setup()
+        bc_cases()
