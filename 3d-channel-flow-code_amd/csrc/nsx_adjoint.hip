@@ -549,6 +549,19 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_adj_ghost(Geo g, double *fb, int 
   fb[b + g.NZ - 1] = 0.0;
 }
 
+// transpose of k_wall_ghosts: the bar of a wall ghost column passes to its
+// cell's column, layer by layer (negated for flag 0), and is cleared; two
+// ghosts of one cell (a corner of a walled box) add into it atomically
+__global__ __launch_bounds__(NSX_BLOCK) void k_adj_wall_ghosts(Geo g, double *fb, int kind) {
+  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  if (t >= (uint32_t)g.NGW * (uint32_t)g.NZ) return;
+  const int w = (int)(t / (uint32_t)g.NZ), k = (int)(t - (uint32_t)w * (uint32_t)g.NZ);
+  const size_t d = (size_t)(g.N0 + w) * g.NS + k;
+  const double b = fb[d];
+  fb[d] = 0.0;
+  if (b != 0.0) aadd(fb + (size_t)g.gw_owner[w] * g.NS + k, ((g.gw_neu[w] >> kind) & 1) ? b : -b);
+}
+
 // --------------------------------------------------------- objective ----
 // the bottom-wall shear of the channel, sum_i w_i u(i, 1) with w_i = nu (1 -
 // s) area_i / (dsig(1) sum area) (s the sign of the bottom ghost layer of u:
@@ -680,6 +693,12 @@ int launch_adj_vertex(const Geo &g, hipStream_t s, const double *fvb, const sign
 }
 
 int launch_adj_ghost(const Geo &g, hipStream_t s, double *fb, int kind) {
+  // the forward fills the sigma ghost layers, then the wall ghost columns
+  // (which copy them too): fold in the reverse order
+  if (g.NGW) {
+    const uint32_t n = (uint32_t)g.NGW * (uint32_t)g.NZ;
+    hipLaunchKernelGGL(k_adj_wall_ghosts, dim3((n + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, fb, kind);
+  }
   hipLaunchKernelGGL(k_adj_ghost, dim3((g.N0 + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, fb, kind);
   return (int)hipGetLastError();
 }

@@ -49,6 +49,8 @@ struct AdjWork {
 
 struct nsx_ctx {
   int N0 = 0, NC = 0, NV = 0, NZ = 0, NS = 0, NEX = 0;
+  int NGW = 0;                 // wall ghost columns after the N0 interior ones
+  std::vector<int> gw_ghost;   // the reference's ghost row (0-based) of each wall ghost column
   int dev = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -156,8 +158,11 @@ static int dev_upload(nsx_ctx *ctx, T **p, const std::vector<T> &h) {
   return 0;
 }
 
+// cell fields hold the N0 interior columns and the wall ghost columns; face
+// fields the interior columns
 static size_t field_elems(const nsx_ctx *c, int kind) {
   if (kind == FK_VERT) return (size_t)c->NV * (c->NZ - 1);
+  if (kind == FK_CELL) return (size_t)(c->N0 + c->NGW) * c->NS;
   return (size_t)c->N0 * c->NS;
 }
 
@@ -272,8 +277,9 @@ extern "C" int nsx_set_bc(nsx_ctx *ctx, const int *bcuc, const int *bcvc, const 
     take(ctx->bcc[q], c[q], (size_t)ctx->NC);
     take(ctx->bcv[q], v[q], (size_t)ctx->NV);
   }
-  if (TopBC < 0 || TopBC > 1 || BotBC < 0 || BotBC > 1)
-    return nsx_fail(ctx, -1, "TopBC/BotBC must be 0 (no slip) or 1 (free slip), got %d/%d", TopBC, BotBC);
+  if (TopBC < 0 || TopBC > 2 || BotBC < 0 || BotBC > 1)
+    return nsx_fail(ctx, -1, "TopBC must be 0 (no slip), 1 (free slip) or 2 (the lid), BotBC 0 or 1, got %d/%d",
+                    TopBC, BotBC);
   ctx->bctop = bctop;
   ctx->bcbot = bcbot;
   ctx->TopBC = TopBC;
@@ -403,9 +409,14 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   g.sigIni = ctx->sigIni;
   g.sigFin = ctx->sigFin;
 
-  // neighbours: periodic ghosts resolved to their partner columns
+  // neighbours: periodic ghosts resolved to their partner columns, wall
+  // ghosts (flags 0 and 1 of update_bc.F90) given columns of their own after
+  // the interior ones
   std::vector<int> nbr((size_t)N0 * 3), vtx((size_t)N0 * 3);
   std::vector<signed char> eside((size_t)N0 * 3, 0);
+  std::vector<int> gw_owner;
+  std::vector<unsigned char> gw_neu;
+  ctx->gw_ghost.clear();
   int nex = N0;
   for (int i = 0; i < N0; ++i)
     for (int j = 0; j < 3; ++j) {
@@ -414,13 +425,23 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
       if (jc >= N0) {
         eside[(size_t)i * 3 + j] = 1;
         if (jc + 1 > nex) nex = jc + 1;
-        for (int q = 0; q < 4; ++q)
-          if (ctx->bcc[q][jc] != 3)
-            return nsx_fail(ctx, -1, "ghost cell %d has boundary flag %d: only periodic horizontal ghosts are supported",
-                            jc + 1, ctx->bcc[q][jc]);
-        int partner = ctx->No_cp[(size_t)jc + 2 * (size_t)NC] - 1;
-        if (partner < 0 || partner >= N0) return nsx_fail(ctx, -1, "ghost cell %d has no periodic partner", jc + 1);
-        jc = partner;
+        const int fu = ctx->bcc[0][jc], fv = ctx->bcc[1][jc], fw = ctx->bcc[2][jc], fp = ctx->bcc[3][jc];
+        if (fu == 3 && fv == 3 && fw == 3 && fp == 3) {
+          int partner = ctx->No_cp[(size_t)jc + 2 * (size_t)NC] - 1;
+          if (partner < 0 || partner >= N0) return nsx_fail(ctx, -1, "ghost cell %d has no periodic partner", jc + 1);
+          jc = partner;
+        } else if ((fu == 0 || fu == 1) && (fv == 0 || fv == 1) && (fw == 0 || fw == 1) && fp == 1) {
+          // bit q: flag 1 for kind q (0 p, 1 u, 2 v, 3 w)
+          gw_neu.push_back((unsigned char)(1 | (fu << 1) | (fv << 2) | (fw << 3)));
+          gw_owner.push_back(i);
+          ctx->gw_ghost.push_back(jc);
+          jc = N0 + (int)gw_owner.size() - 1;
+        } else {
+          return nsx_fail(ctx, -1,
+                          "ghost cell %d has flags u %d v %d w %d p %d: supported are periodic ghosts (all 3) and "
+                          "walls (u, v, w 0 or 1, p 1); inflow (2) and parallel (-1) ghosts are not",
+                          jc + 1, fu, fv, fw, fp);
+        }
       }
       nbr[(size_t)i * 3 + j] = jc;
       int v = ctx->No_vp[(size_t)i + (size_t)j * N0] - 1;
@@ -428,6 +449,9 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
       vtx[(size_t)i * 3 + j] = v;
     }
   ctx->NEX = nex;
+  ctx->NGW = (int)gw_owner.size();
+  if ((uint64_t)(N0 + ctx->NGW) * NS >= (1ull << 31))
+    return nsx_fail(ctx, -1, "the mesh with its wall ghost columns exceeds 32-bit cell numbering");
 
   std::vector<double> ppec((size_t)N0 * 3);
   for (int i = 0; i < N0; ++i)
@@ -444,6 +468,11 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   for (int i = 0; i < N0; ++i)
     for (int j = 0; j < 3; ++j) {
       const int n = nbr[(size_t)i * 3 + j];
+      if (n >= N0) {
+        // a wall: the coupling is zero both ways (Neumann p, flag 1)
+        ppec_t[(size_t)i * 3 + j] = ppec[(size_t)i * 3 + j];
+        continue;
+      }
       int best = -1;
       for (int jj = 0; jj < 3; ++jj)
         if (nbr[(size_t)n * 3 + jj] == i &&
@@ -525,6 +554,17 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     UP(d_es, eside);
     g.eside = d_es;
   }
+  g.NGW = ctx->NGW;
+  g.gw_owner = nullptr;
+  g.gw_neu = nullptr;
+  if (ctx->NGW > 0) {
+    int *d_go;
+    unsigned char *d_gn;
+    UP(d_go, gw_owner);
+    UP(d_gn, gw_neu);
+    g.gw_owner = d_go;
+    g.gw_neu = d_gn;
+  }
   const std::vector<double> *edge[9] = {&ctx->dlVV, &ctx->dlCC, &ctx->dle1, &ctx->dle2, &ctx->doe,
                                         &ctx->xnn, &ctx->ynn, &ctx->dxCC, &ctx->dyCC};
   for (int q = 0; q < 9; ++q) UP(d[q], edge_major(*edge[q], N0));
@@ -602,8 +642,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   g.vact_p = d_act[3];
 #undef UP
 
-  // fields
-  const size_t nc = (size_t)N0 * NS, nv = (size_t)NV * (NZ - 1);
+  // fields (cell fields and solver vectors with the wall ghost columns)
+  const size_t nc = (size_t)(N0 + ctx->NGW) * NS, nv = (size_t)NV * (NZ - 1);
   auto cell = [&](double **p) { return dev_alloc(ctx, p, nc); };
   for (int s = 0; s < 3; ++s)
     for (int q = 0; q < 4; ++q)
@@ -724,6 +764,12 @@ extern "C" int nsx_put_field(nsx_ctx *ctx, int f, const double *host) {
     const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)ctx->NC;
     for (int k = 0; k < NZ; ++k)
       for (int i = 0; i < N0; ++i) buf[(size_t)i * NS + k] = host[(size_t)i + (size_t)k * ld];
+    // wall ghost rows are state the caller's BC calls wrote; periodic ghost
+    // rows are not stored (the neighbour index points at the partner)
+    if (ctx->kind[f] == FK_CELL)
+      for (int w = 0; w < ctx->NGW; ++w)
+        for (int k = 0; k < NZ; ++k)
+          buf[(size_t)(N0 + w) * NS + k] = host[(size_t)ctx->gw_ghost[w] + (size_t)k * ld];
   }
   HIPCHECK(hipMemcpyAsync(ctx->slot[f], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -745,12 +791,16 @@ extern "C" int nsx_get_field(nsx_ctx *ctx, int f, double *host, int fill_ghost_r
   const size_t ld = ctx->kind[f] == FK_FACE ? (size_t)N0 : (size_t)NC;
   for (int k = 0; k < NZ; ++k)
     for (int i = 0; i < N0; ++i) host[(size_t)i + (size_t)k * ld] = buf[(size_t)i * NS + k];
-  if (fill_ghost_rows && ctx->kind[f] == FK_CELL)
+  if (fill_ghost_rows && ctx->kind[f] == FK_CELL) {
     for (int gh = N0; gh < ctx->NEX; ++gh) {
       const int partner = ctx->No_cp[(size_t)gh + 2 * (size_t)NC] - 1;
       if (partner < 0 || partner >= N0) continue;
       for (int k = 0; k < NZ; ++k) host[(size_t)gh + (size_t)k * NC] = buf[(size_t)partner * NS + k];
     }
+    for (int w = 0; w < ctx->NGW; ++w)
+      for (int k = 0; k < NZ; ++k)
+        host[(size_t)ctx->gw_ghost[w] + (size_t)k * NC] = buf[(size_t)(N0 + w) * NS + k];
+  }
   return 0;
 }
 
@@ -1005,20 +1055,41 @@ static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *
   return solve_cgs(ctx, rhs, phi_in, x);
 }
 
+// BCpcenter (kind 0) or BCvelcenter3D (kinds 1..3 = u, v, w) on a stored
+// field: the sigma ghost layers, then the wall ghost columns
+static int bc_center(nsx_ctx *ctx, double *f, int kind) {
+  LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, f, kind));
+  LAUNCH(launch_wall_ghosts(ctx->g, ctx->stream, f, kind));
+  return 0;
+}
+
+#define BC_CENTER(f, kind)                 \
+  do {                                     \
+    const int rc_ = bc_center(ctx, f, kind); \
+    if (rc_) return rc_;                   \
+  } while (0)
+
+// the wall ghost columns alone, of a field whose sigma ghost layers the
+// kernel that wrote it filled
+#define BC_WALLS(f, kind) LAUNCH(launch_wall_ghosts(ctx->g, ctx->stream, f, kind))
+
 // vertex values of up to three fields; kinds 0 p, 1 u, 2 v, 3 w
 static int vertex_of(nsx_ctx *ctx, int nf, const double *const f[3], double *const fv[3], const int kinds[3],
                      int tagdir) {
   const Geo &g = ctx->g;
   const signed char *act[3];
-  int zb[3], zt[3];
+  int zb[3], zt[3], lid[3] = {0, 0, 0};
   for (int q = 0; q < nf; ++q) {
     const int kq = kinds[q];
     act[q] = kq == 0 ? g.vact_p : (kq == 1 ? g.vact_u : (kq == 2 ? g.vact_v : g.vact_w));
     // BCvelvertex3D zeroes the wall levels for tagdir 3 (BC.F90:737-787); BCpvertex does not
     zb[q] = kq == 0 ? 0 : (g.bcbot == 1 && (g.BotBC == 0 || (g.BotBC == 1 && kq == 3)));
-    zt[q] = kq == 0 ? 0 : (g.bctop == 1 && (g.TopBC == 0 || (g.TopBC == 1 && kq == 3)));
+    zt[q] = kq == 0 ? 0
+                    : (g.bctop == 1 && (g.TopBC == 0 || (g.TopBC == 1 && kq == 3) || (g.TopBC == 2 && kq != 1)));
+    // the lid sets u = 1 on the top level of the interior vertices (BC.F90:771-777)
+    lid[q] = kq == 1 && g.bctop == 1 && g.TopBC == 2;
   }
-  LAUNCH(launch_vertex(g, ctx->stream, nf, f, fv, act, zb, zt, tagdir));
+  LAUNCH(launch_vertex(g, ctx->stream, nf, f, fv, act, zb, zt, tagdir, lid));
   return 0;
 }
 
@@ -1039,8 +1110,8 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   // the reference fills the sigma ghost layers of the stage input in place
   // (BCvelcenter3D inside BCglobalVC of sgles/AdvDiffNS, BCpcenter of p);
   // a resident input already holds them, an input put from the host may not
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
-  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
+  for (int q = 0; q < 3; ++q) BC_CENTER(X[q], q + 1);
+  BC_CENTER(X[3], 0);
   const double *uvw[3] = {X[0], X[1], X[2]};
   const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
   // vertex values of the stage velocity (tagdir 2), shared by sgles and AdvDiffNS
@@ -1053,6 +1124,7 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
       LAUNCH(launch_les32(g, s, X[0], X[1], X[2], ctx->uv2[0], ctx->uv2[1], ctx->uv2[2], ctx->slot[NSX_F_EDDY]));
     else
       LAUNCH(launch_les(g, s, X[0], X[1], X[2], ctx->uv2[0], ctx->uv2[1], ctx->uv2[2], ctx->slot[NSX_F_EDDY]));
+    BC_WALLS(ctx->slot[NSX_F_EDDY], 0);  // sgles' BCpcenter(eddy)
   }
   const double *vtx[3] = {ctx->uv2[0], ctx->uv2[1], ctx->uv2[2]};
   const double *face[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
@@ -1064,6 +1136,9 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
   else
     LAUNCH(launch_momentum(g, s, uvw, vtx, face, ctx->slot[NSX_F_EDDY], nullptr, hist, nullptr, ctx->ustar, c1, c2,
                            force));
+  // BCglobalVC(ufstar, ...) of hydro.F90:342: the momentum kernels fill the
+  // sigma ghost layers of u*, then the wall ghost columns
+  for (int q = 0; q < 3; ++q) BC_WALLS(ctx->ustar[q], q + 1);
   // update_rhsp's BCglobalP(pf, pfv, 2)
   const double *pin[3] = {X[3], nullptr, nullptr};
   double *pfv[3] = {ctx->slot[NSX_F_PFV], nullptr, nullptr};
@@ -1077,12 +1152,14 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
     ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 5});
   }
   if ((rc = solve(ctx, ctx->rhsp, X[3], Y[3]))) return rc;
+  BC_WALLS(Y[3], 0);  // bicgstab's BCpcenter(phi)
   if (ctx->timing) {
     ev = events(ctx, 2);
     (void)hipEventRecord(ev[0], s);
   }
   double *unp[3] = {Y[0], Y[1], Y[2]};
   LAUNCH(launch_corrector(g, s, Y[3], us, unp));
+  for (int q = 0; q < 3; ++q) BC_WALLS(Y[q], q + 1);  // BCglobalVC(ufnp, ...) of hydro.F90:452
   const double *un[3] = {Y[0], Y[1], Y[2]};
   double *uvf[3] = {ctx->slot[NSX_F_UFV], ctx->slot[NSX_F_VFV], ctx->slot[NSX_F_WFV]};
   if ((rc = vertex_of(ctx, 3, un, uvf, vk, 3))) return rc;
@@ -1115,8 +1192,8 @@ extern "C" int nsx_init_state(nsx_ctx *ctx) {
     if (ctx->slot[NSX_F_UFN] == ctx->pool_cell[st][0]) n = st;
   if (n < 0) return nsx_fail(ctx, -1, "time-level slots are inconsistent");
   double *const *X = ctx->pool_cell[n];
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
-  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
+  for (int q = 0; q < 3; ++q) BC_CENTER(X[q], q + 1);
+  BC_CENTER(X[3], 0);
   const double *uvw[3] = {X[0], X[1], X[2]};
   const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
   double *uvf[3] = {ctx->slot[NSX_F_UFV], ctx->slot[NSX_F_VFV], ctx->slot[NSX_F_WFV]};
@@ -1128,7 +1205,7 @@ extern "C" int nsx_init_state(nsx_ctx *ctx) {
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   LAUNCH(launch_source(g, s, uvw, facew, nullptr, nullptr, nullptr, ctx->part));
-  const size_t nc = (size_t)ctx->N0 * ctx->NS;
+  const size_t nc = field_elems(ctx, FK_CELL);
   for (int id : {NSX_F_RHSUF, NSX_F_RHSVF, NSX_F_RHSWF, NSX_F_EDDY})
     HIPCHECK(hipMemsetAsync(ctx->slot[id], 0, nc * sizeof(double), s));
   // LoopTimeInitial: (.)np <- (.)n
@@ -1218,7 +1295,7 @@ extern "C" int nsx_bcglobalvc(nsx_ctx *ctx, int u, int v, int w, int uv, int vv,
     return -1;
   if (tagdir != 2 && tagdir != 3) return nsx_fail(ctx, -1, "tagdir must be 2 or 3");
   const int ids[3] = {u, v, w};
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   double *fv[3] = {ctx->slot[uv], ctx->slot[vv], ctx->slot[wv]};
   const int vk[3] = {1, 2, 3};
@@ -1229,7 +1306,7 @@ extern "C" int nsx_bcglobalvc(nsx_ctx *ctx, int u, int v, int w, int uv, int vv,
 }
 
 static int bcglobalp_(nsx_ctx *ctx, int p, int pv, int tagdir) {
-  LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[p], 0));
+  BC_CENTER(ctx->slot[p], 0);
   const double *f[3] = {ctx->slot[p], nullptr, nullptr};
   double *fv[3] = {ctx->slot[pv], nullptr, nullptr};
   const int pk[3] = {0, 0, 0};
@@ -1252,6 +1329,10 @@ extern "C" int nsx_grad_lsm(nsx_ctx *ctx, int dfdx, int dfdy, int dfds, int f) {
       kind_ok(ctx, f, FK_CELL))
     return -1;
   LAUNCH(launch_grad_lsm(ctx->g, ctx->stream, ctx->slot[f], ctx->slot[dfdx], ctx->slot[dfdy], ctx->slot[dfds]));
+  // grandientLSM's BCpcenter on its three outputs (gradientLSM.F90:199-201)
+  BC_WALLS(ctx->slot[dfdx], 0);
+  BC_WALLS(ctx->slot[dfdy], 0);
+  BC_WALLS(ctx->slot[dfds], 0);
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -1272,13 +1353,14 @@ extern "C" int nsx_sgles(nsx_ctx *ctx, int u, int v, int w) {
   NEED_COMMIT(ctx);
   if (kind_ok(ctx, u, FK_CELL) || kind_ok(ctx, v, FK_CELL) || kind_ok(ctx, w, FK_CELL)) return -1;
   const int ids[3] = {u, v, w};
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   const int vk[3] = {1, 2, 3};
   int rc = vertex_of(ctx, 3, f, ctx->uv2, vk, 2);
   if (rc) return rc;
   LAUNCH(launch_les(ctx->g, ctx->stream, f[0], f[1], f[2], ctx->uv2[0], ctx->uv2[1], ctx->uv2[2],
                     ctx->slot[NSX_F_EDDY]));
+  BC_WALLS(ctx->slot[NSX_F_EDDY], 0);
   HIPCHECK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -1288,7 +1370,7 @@ extern "C" int nsx_advdiff(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u, in
   const int ids[7] = {rhsu, rhsv, rhsw, u, v, w, eddy};
   for (int q = 0; q < 7; ++q)
     if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[3 + q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[3 + q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   const int vk[3] = {1, 2, 3};
   int rc = vertex_of(ctx, 3, f, ctx->uv2, vk, 2);
@@ -1311,7 +1393,7 @@ extern "C" int nsx_advdiff_gam(nsx_ctx *ctx, int rhsu, int rhsv, int rhsw, int u
   const int ids[9] = {rhsu, rhsv, rhsw, u, v, w, gamx, gamy, gamz};
   for (int q = 0; q < 9; ++q)
     if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[3 + q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[3 + q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   const int vk[3] = {1, 2, 3};
   int rc = vertex_of(ctx, 3, f, ctx->uv2, vk, 2);
@@ -1338,7 +1420,7 @@ extern "C" int nsx_interpolate_u(nsx_ctx *ctx, int f1, int f2, int f3, int fT, i
   const int ids[3] = {u, v, w};
   for (int q = 0; q < 3; ++q)
     if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   double *out[5] = {ctx->slot[f1], ctx->slot[f2], ctx->slot[f3], ctx->slot[fT], ctx->slot[fB]};
   LAUNCH(launch_source(ctx->g, ctx->stream, f, out, nullptr, nullptr, nullptr, ctx->part));
@@ -1351,7 +1433,7 @@ extern "C" int nsx_source3d(nsx_ctx *ctx, int src, int u, int v, int w) {
   const int ids[4] = {src, u, v, w};
   for (int q = 0; q < 4; ++q)
     if (kind_ok(ctx, ids[q], FK_CELL)) return -1;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(ctx->g, ctx->stream, ctx->slot[ids[1 + q]], q + 1));
+  for (int q = 0; q < 3; ++q) BC_CENTER(ctx->slot[ids[1 + q]], q + 1);
   const double *f[3] = {ctx->slot[u], ctx->slot[v], ctx->slot[w]};
   double *facew[5] = {ctx->slot[NSX_F_U1FACE], ctx->slot[NSX_F_U2FACE], ctx->slot[NSX_F_U3FACE],
                       ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
@@ -1667,7 +1749,7 @@ extern "C" int nsx_savetec_vc(nsx_ctx *ctx, const char *dir, int counter, double
 // tangent-linear model.  Solves run in FP64 (BiCGSTAB) to eps.
 
 static int adj_field(nsx_ctx *ctx, double **p, int kind) {
-  const size_t n = kind == FK_VERT ? (size_t)ctx->NV * (ctx->NZ - 1) : (size_t)ctx->N0 * ctx->NS;
+  const size_t n = field_elems(ctx, kind == FK_VERT ? FK_VERT : FK_CELL);
   int rc = dev_alloc(ctx, p, n);
   if (rc) return rc;
   HIPCHECK(hipMemsetAsync(*p, 0, n * sizeof(double), ctx->stream));
@@ -1719,7 +1801,7 @@ static int adj_alloc(nsx_ctx *ctx, size_t nckpt, size_t nseg) {
   return 0;
 }
 
-static size_t cell_bytes(const nsx_ctx *ctx) { return (size_t)ctx->N0 * ctx->NS * sizeof(double); }
+static size_t cell_bytes(const nsx_ctx *ctx) { return field_elems(ctx, FK_CELL) * sizeof(double); }
 
 // the state of pool set `set` (u, v, w, p) and the faces into dst[9], or back
 static int save_state(nsx_ctx *ctx, int set, double *const dst[9]) {
@@ -1780,8 +1862,8 @@ static int stage_base(nsx_ctx *ctx, double *const X[4]) {
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
   AdjWork &W = ctx->adj;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, X[q], q + 1));
-  LAUNCH(launch_ghost_layers(g, s, X[3], 0));
+  for (int q = 0; q < 3; ++q) BC_CENTER(X[q], q + 1);
+  BC_CENTER(X[3], 0);
   const double *uvw[3] = {X[0], X[1], X[2]};
   const int vk[3] = {1, 2, 3};
   int rc = vertex_of(ctx, 3, uvw, W.V, vk, 2);
@@ -1870,7 +1952,7 @@ static int adj_stage(nsx_ctx *ctx, double *const X[4], const double *const Fin[5
 static int adj_step(nsx_ctx *ctx, double *const Sn[9], double *const Sa[9], double *const Sb[9], double *fb) {
   hipStream_t s = ctx->stream;
   AdjWork &W = ctx->adj;
-  const size_t nc = (size_t)ctx->N0 * ctx->NS;
+  const size_t nc = field_elems(ctx, FK_CELL);
   // u, v, w (n+1) = (u(n) + u_b)/2; p and the faces are the second stage's
   for (int q = 0; q < 3; ++q) LAUNCH(launch_axpby(s, nc, 0.5, Sb[q], 0.0, nullptr));
   for (int q = 0; q < 4; ++q) HIPCHECK(hipMemcpyAsync(W.Yb[q], Sb[q], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
@@ -1898,8 +1980,8 @@ static int tlm_stage(nsx_ctx *ctx, double *const X[4], double *const dX[4], cons
   AdjWork &W = ctx->adj;
   int rc = stage_base(ctx, X);
   if (rc) return rc;
-  for (int q = 0; q < 3; ++q) LAUNCH(launch_ghost_layers(g, s, dX[q], q + 1));
-  LAUNCH(launch_ghost_layers(g, s, dX[3], 0));
+  for (int q = 0; q < 3; ++q) BC_CENTER(dX[q], q + 1);
+  BC_CENTER(dX[3], 0);
   const double *uvw[3] = {X[0], X[1], X[2]}, *vtx[3] = {W.V[0], W.V[1], W.V[2]};
   const double *duvw[3] = {dX[0], dX[1], dX[2]};
   const int vk[3] = {1, 2, 3}, pk[3] = {0, 0, 0};
@@ -1913,14 +1995,17 @@ static int tlm_stage(nsx_ctx *ctx, double *const X[4], double *const dX[4], cons
                            ctx->slot[NSX_F_UTFACE], ctx->slot[NSX_F_UBFACE]};
   const double *dface[5] = {W.TF[0], W.TF[1], W.TF[2], W.TF[3], W.TF[4]};
   LAUNCH(launch_tlm_momentum(g, s, uvw, vtx, face, W.E, duvw, dvtx, dface, W.TE, dforce, W.Tus));
+  for (int q = 0; q < 3; ++q) BC_WALLS(W.Tus[q], q + 1);
   const double *dp[3] = {dX[3], nullptr, nullptr};
   double *dpv[3] = {W.Tpv, nullptr, nullptr};
   if ((rc = vertex_of(ctx, 1, dp, dpv, pk, 2))) return rc;
   const double *dus[3] = {W.Tus[0], W.Tus[1], W.Tus[2]};
   LAUNCH(launch_source(g, s, dus, W.TF, nullptr, W.Trhs, W.Tpv, ctx->part));
   if ((rc = solve_fp64(ctx, W.Trhs, W.zero, dY[3]))) return rc;
+  BC_WALLS(dY[3], 0);
   double *dun[3] = {dY[0], dY[1], dY[2]};
   LAUNCH(launch_corrector(g, s, dY[3], dus, dun));
+  for (int q = 0; q < 3; ++q) BC_WALLS(dY[q], q + 1);
   const double *dyp[3] = {dY[3], nullptr, nullptr};
   if ((rc = vertex_of(ctx, 1, dyp, dpv, pk, 2))) return rc;
   LAUNCH(launch_newu(g, s, dY[3], W.Tpv, W.TF));
@@ -1967,6 +2052,9 @@ struct Fp64Scope {
 
 static int diff_checks(nsx_ctx *ctx) {
   if (ctx->FinRK != 2) return nsx_fail(ctx, -1, "the differentiation form needs RK2 steps (FinRK = 2)");
+  if (ctx->TopBC == 2)
+    return nsx_fail(ctx, -1, "the differentiation form does not transpose the lid (TopBC = 2): its vertex values "
+                             "are constants the tangent and adjoint vertex kernels do not hold");
   return 0;
 }
 

@@ -4,10 +4,12 @@
 //   cell field   double[N0][NS]   column-contiguous: entry (i,k) at i*NS+k, NS =
 //                                 NZ rounded up to a multiple of 4 (16-byte
 //                                 aligned runs of 2 doubles / 4 floats); the
-//                                 ghost layers k=0 and k=NZ-1 are stored, the
-//                                 horizontal (periodic) ghost cells are not: a
+//                                 ghost layers k=0 and k=NZ-1 are stored; a
+//                                 periodic horizontal ghost cell is not: the
 //                                 neighbour index nbr[i*3+j] already points at
-//                                 the periodic partner column;
+//                                 its partner column; a wall ghost cell is a
+//                                 column of its own after the N0 interior ones
+//                                 (N0+w), filled from its cell by the wall rule;
 //   vertex field double[NV][NZ-1] entry (v,l) at v*(NZ-1)+l;
 //   face field   double[N0][NS]   U1FACE..UBFACE, same indexing as cells.
 // A wavefront walks consecutive k of one column (then the next column), so
@@ -70,6 +72,10 @@ struct Geo {
   const double *ppec_t;     // ppec of the neighbour's edge back towards i: the lateral couplings of A^T
   int trans;                // 1 while the solver applies A^T (the adjoint solve)
   const signed char *eside; // 1 when No_cp(i,j) is not an interior cell (a side face of glob_cfl)
+  // wall ghost columns N0 .. N0+NGW-1 (walled sides of the box)
+  int NGW;
+  const int *gw_owner;        // the interior column whose edge the ghost lies behind
+  const unsigned char *gw_neu; // bit q set: flag 1 (copy) for kind q (0 p, 1 u, 2 v, 3 w); clear: flag 0
   // per cell (N0)
   const double *area, *sum_xc2, *sum_yc2, *sum_xcyc;
   // per level
@@ -91,12 +97,32 @@ struct Geo {
 };
 
 // velocity ghost layer rule of BCvelcenter3D (BC.F90:459-527) for ZPB = 0:
-// +1 symmetric copy, -1 antisymmetric copy, 0 no fill
+// the derivative of the ghost value with respect to the interior value next
+// to it, +1 symmetric copy, -1 antisymmetric copy (and the lid's 2 - u), 0
+// no fill
 __host__ __device__ inline int vel_ghost_sign(int bcflag, int wallbc, int comp /*0 u,1 v,2 w*/) {
   if (bcflag != 1) return 0;
   if (wallbc == 0) return -1;
   if (wallbc == 1) return comp == 2 ? -1 : 1;
+  if (wallbc == 2) return -1;
   return 0;
+}
+
+// the ghost value itself from the interior value x: no slip -x, free slip x
+// (w: -x), the lid (wallbc 2, the top only) 2 - x for u and -x for v, w;
+// false when bcflag says there is no wall to fill
+__host__ __device__ inline bool vel_ghost(int bcflag, int wallbc, int comp, double x, double &out) {
+  if (bcflag != 1) return false;
+  if (wallbc == 0) {
+    out = -x;
+  } else if (wallbc == 1) {
+    out = comp == 2 ? -x : x;
+  } else if (wallbc == 2) {
+    out = comp == 0 ? 2.0 - x : -x;
+  } else {
+    return false;
+  }
+  return true;
 }
 
 // ------------------------------------------------------------ iteration ---

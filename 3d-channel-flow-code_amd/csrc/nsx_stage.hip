@@ -63,6 +63,7 @@ struct VtxArgs {
   double *fv[3];
   const signed char *act[3];
   int zero_bot[3], zero_top[3];
+  int lid_top[3];  // the lid: 1 on the top level of the interior vertices (BC.F90:771-777)
   int tagdir;
 };
 
@@ -158,12 +159,14 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
       }
       mine = o;
     }
+    if (a.tagdir == 3 && a.lid_top[q] && l == g.NL - 1 && g.vkind[v] == VK_INTERIOR) mine = 1.0;
     a.fv[q][(size_t)v * g.NL + l] = mine;
   }
 }
 
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
-                  const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir) {
+                  const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir,
+                  const int *lid_top) {
   VtxArgs a;
   for (int q = 0; q < 3; ++q) {
     a.f[q] = q < nf ? f[q] : nullptr;
@@ -171,6 +174,7 @@ int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3],
     a.act[q] = q < nf ? act[q] : nullptr;
     a.zero_bot[q] = q < nf ? zero_bot[q] : 0;
     a.zero_top[q] = q < nf ? zero_top[q] : 0;
+    a.lid_top[q] = q < nf && lid_top ? lid_top[q] : 0;
   }
   a.tagdir = tagdir;
   const uint32_t n = (uint32_t)g.NV * g.NL;
@@ -342,14 +346,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
         const double us = pi + g.dt * (a.c1 * r + a.c2 * a.hist[q][s]);
         a.hist[q][s] = r;
         a.ustar[q][s] = us;
-        if (k == 1) {
-          const int sg = vel_ghost_sign(g.bcbot, g.BotBC, q);
-          if (sg) a.ustar[q][s - 1] = sg > 0 ? us : -us;
-        }
-        if (k == g.NZ - 2) {
-          const int sg = vel_ghost_sign(g.bctop, g.TopBC, q);
-          if (sg) a.ustar[q][s + 1] = sg > 0 ? us : -us;
-        }
+        double gv;
+        if (k == 1 && vel_ghost(g.bcbot, g.BotBC, q, us, gv)) a.ustar[q][s - 1] = gv;
+        if (k == g.NZ - 2 && vel_ghost(g.bctop, g.TopBC, q, us, gv)) a.ustar[q][s + 1] = gv;
       }
     }
   }
@@ -506,14 +505,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum32(Geo g, MomArgs a) {
         const double us = pi + g.dt * (a.c1 * (double)r + a.c2 * a.hist[q][s]);
         a.hist[q][s] = (double)r;
         a.ustar[q][s] = us;
-        if (k == 1) {
-          const int sg = vel_ghost_sign(g.bcbot, g.BotBC, q);
-          if (sg) a.ustar[q][s - 1] = sg > 0 ? us : -us;
-        }
-        if (k == g.NZ - 2) {
-          const int sg = vel_ghost_sign(g.bctop, g.TopBC, q);
-          if (sg) a.ustar[q][s + 1] = sg > 0 ? us : -us;
-        }
+        double gv;
+        if (k == 1 && vel_ghost(g.bcbot, g.BotBC, q, us, gv)) a.ustar[q][s - 1] = gv;
+        if (k == g.NZ - 2 && vel_ghost(g.bctop, g.TopBC, q, us, gv)) a.ustar[q][s + 1] = gv;
       }
     }
   }
@@ -705,21 +699,16 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_corrector(Geo g, const double *p,
     un[s] = nu_;
     vn[s] = nv_;
     wn[s] = nw_;
+    double gv;
     if (k == 1) {
-      int sg = vel_ghost_sign(g.bcbot, g.BotBC, 0);
-      if (sg) un[s - 1] = sg > 0 ? nu_ : -nu_;
-      sg = vel_ghost_sign(g.bcbot, g.BotBC, 1);
-      if (sg) vn[s - 1] = sg > 0 ? nv_ : -nv_;
-      sg = vel_ghost_sign(g.bcbot, g.BotBC, 2);
-      if (sg) wn[s - 1] = sg > 0 ? nw_ : -nw_;
+      if (vel_ghost(g.bcbot, g.BotBC, 0, nu_, gv)) un[s - 1] = gv;
+      if (vel_ghost(g.bcbot, g.BotBC, 1, nv_, gv)) vn[s - 1] = gv;
+      if (vel_ghost(g.bcbot, g.BotBC, 2, nw_, gv)) wn[s - 1] = gv;
     }
     if (k == g.NZ - 2) {
-      int sg = vel_ghost_sign(g.bctop, g.TopBC, 0);
-      if (sg) un[s + 1] = sg > 0 ? nu_ : -nu_;
-      sg = vel_ghost_sign(g.bctop, g.TopBC, 1);
-      if (sg) vn[s + 1] = sg > 0 ? nv_ : -nv_;
-      sg = vel_ghost_sign(g.bctop, g.TopBC, 2);
-      if (sg) wn[s + 1] = sg > 0 ? nw_ : -nw_;
+      if (vel_ghost(g.bctop, g.TopBC, 0, nu_, gv)) un[s + 1] = gv;
+      if (vel_ghost(g.bctop, g.TopBC, 1, nv_, gv)) vn[s + 1] = gv;
+      if (vel_ghost(g.bctop, g.TopBC, 2, nw_, gv)) wn[s + 1] = gv;
     }
   }
 }
@@ -770,7 +759,7 @@ int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, 
 
 // ------------------------------------------------------------ RK2 average --
 // nsmp3D.F90:531-548: u,v,w(n+1) = 0.5 (u(n) + u(n+1)) over every stored
-// entry (ghost layers included).  Bytes/cell: read 6, write 3.
+// entry (ghost layers and wall ghost columns included).  Bytes/cell: read 6, write 3.
 __global__ __launch_bounds__(NSX_BLOCK) void k_rk2avg(size_t n, const double *u0, const double *v0,
                                                        const double *w0, double *u1, double *v1, double *w1) {
   for (size_t t = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * NSX_BLOCK) {
@@ -781,7 +770,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_rk2avg(size_t n, const double *u0
 }
 
 int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]) {
-  const size_t n = (size_t)g.N0 * g.NS;
+  const size_t n = (size_t)(g.N0 + g.NGW) * g.NS;
   hipLaunchKernelGGL(k_rk2avg, dim3(nsx_grid_for((uint32_t)(n < 0xffffffffu ? n : 0xffffffffu))), dim3(NSX_BLOCK),
                      0, s, n, un[0], un[1], un[2], unp[0], unp[1], unp[2]);
   return (int)hipGetLastError();
@@ -794,16 +783,41 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_ghost_layers(Geo g, double *f, in
   const int i = blockIdx.x * NSX_BLOCK + threadIdx.x;
   if (i >= g.N0) return;
   const size_t b = (size_t)i * g.NS;
-  int sb, st;
   if (kind == 0) {
-    sb = g.bcbot == 1 ? 1 : 0;
-    st = g.bctop == 1 ? 1 : 0;
-  } else {
-    sb = vel_ghost_sign(g.bcbot, g.BotBC, kind - 1);
-    st = vel_ghost_sign(g.bctop, g.TopBC, kind - 1);
+    if (g.bcbot == 1) f[b] = f[b + 1];
+    if (g.bctop == 1) f[b + g.NZ - 1] = f[b + g.NZ - 2];
+    return;
   }
-  if (sb) f[b] = sb > 0 ? f[b + 1] : -f[b + 1];
-  if (st) f[b + g.NZ - 1] = st > 0 ? f[b + g.NZ - 2] : -f[b + g.NZ - 2];
+  double gv;
+  if (vel_ghost(g.bcbot, g.BotBC, kind - 1, f[b + 1], gv)) f[b] = gv;
+  if (vel_ghost(g.bctop, g.TopBC, kind - 1, f[b + g.NZ - 2], gv)) f[b + g.NZ - 1] = gv;
+}
+
+// ------------------------------------------------------ wall ghost cells ---
+// the horizontal part of BCvelcenter3D (kinds 1..3 = u,v,w, BC.F90:540-581)
+// and BCpcenter (kind 0, BC.F90:945-964) for wall ghosts: every stored layer
+// of the ghost column takes its cell's value, copied (flag 1) or negated
+// (flag 0: 0 - phi for the velocities, -phi for p, as the reference writes
+// them).  Periodic ghosts need no fill: the neighbour index points at the
+// partner.  Run after the sigma ghost layers, whose values it copies too.
+__global__ __launch_bounds__(NSX_BLOCK) void k_wall_ghosts(Geo g, double *f, int kind) {
+  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  if (t >= (uint32_t)g.NGW * (uint32_t)g.NZ) return;
+  const int w = (int)(t / (uint32_t)g.NZ), k = (int)(t - (uint32_t)w * (uint32_t)g.NZ);
+  const double x = f[(size_t)g.gw_owner[w] * g.NS + k];
+  double y;
+  if ((g.gw_neu[w] >> kind) & 1)
+    y = x;
+  else
+    y = kind == 0 ? -x : 0.0 - x;
+  f[(size_t)(g.N0 + w) * g.NS + k] = y;
+}
+
+int launch_wall_ghosts(const Geo &g, hipStream_t s, double *f, int kind) {
+  if (g.NGW == 0) return 0;
+  const uint32_t n = (uint32_t)g.NGW * (uint32_t)g.NZ;
+  hipLaunchKernelGGL(k_wall_ghosts, dim3((n + NSX_BLOCK - 1) / NSX_BLOCK), dim3(NSX_BLOCK), 0, s, g, f, kind);
+  return (int)hipGetLastError();
 }
 
 int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind) {

@@ -113,8 +113,12 @@ int nsx_set_sigma(nsx_ctx *ctx, const double *sig, const double *dsig, const dou
 int nsx_set_vertex_weights(nsx_ctx *ctx, const int *surrounding, const double *weight, const int *Dimsurrounding,
                            const double *dlVsum, const int *vertex_pair, const int *vertex_edge);
 /* boundary flags of update_bc.F90 (bc?c (N_CELL), bc?v (N_VERT)), bctop/bcbot
- * and TopBC/BotBC (common.mpf).  Horizontal ghosts must be periodic (flag 3);
- * sigma walls TopBC, BotBC in {0,1}. */
+ * and TopBC/BotBC (common.mpf).  A horizontal ghost cell is periodic (flag 3
+ * for u, v, w and p) or a wall (u, v, w 0 no slip or 1 copy, p 1, the flags
+ * update_bc assigns to walled sides, BC.F90:540-581, 945-964); inflow (2)
+ * and parallel (-1) ghosts are refused at nsx_commit.  Sigma walls: TopBC 0
+ * no slip, 1 free slip, 2 the lid (u = 1 at the top, BC.F90:512-521,
+ * 771-777); BotBC 0 or 1. */
 int nsx_set_bc(nsx_ctx *ctx, const int *bcuc, const int *bcvc, const int *bcwc, const int *bcpc, const int *bcuv,
                const int *bcvv, const int *bcwv, const int *bcpv, int bctop, int bcbot, int TopBC, int BotBC);
 /* PPE matrix AmC0..AmCB (N_CELL0,NZ) of PDE_center (PDE.F90:69-121).  The
@@ -131,10 +135,12 @@ int nsx_set_params(nsx_ctx *ctx, double dt, double Re, double bdx, double bdy, d
 int nsx_commit(nsx_ctx *ctx);
 
 /* ---- fields -------------------------------------------------------------- */
-/* host array in the reference layout; ghost rows and ghost layers are ignored */
+/* host array in the reference layout; periodic ghost rows are ignored (the
+ * device reads the partner), the wall ghost rows of a cell field are taken */
 int nsx_put_field(nsx_ctx *ctx, int field, const double *host);
-/* host array in the reference layout; periodic ghost rows are filled from
- * their partners when fill_ghost_rows != 0 (as the reference's BC fills do) */
+/* host array in the reference layout; when fill_ghost_rows != 0, periodic
+ * ghost rows are filled from their partners (as the reference's BC fills do)
+ * and wall ghost rows from the device's wall ghost columns */
 int nsx_get_field(nsx_ctx *ctx, int field, double *host, int fill_ghost_rows);
 int nsx_copy_field(nsx_ctx *ctx, int dst, int src);
 

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from common import interior, rel_err, tiny_case
+from common import bc_case, interior, load_bc, rel_err, tiny_case
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -33,10 +33,14 @@ def _rows(case, a):
     return np.asarray(a)[:case.N_CELLexact]
 
 
-@pytest.fixture(scope="module", params=["tiny", "cfg0"])
+# the tiny mesh, workload 0, and the tiny mesh with the boundary sets of the
+# reference's walled builds (a duct, free-slip x walls, a walled box, the lid)
+@pytest.fixture(scope="module", params=["tiny", "cfg0", "bc_duct", "bc_slip_x", "bc_box", "bc_lid"])
 def pair(request):
     if request.param == "tiny":
         c, p0 = tiny_case()
+    elif request.param.startswith("bc_"):
+        c, p0 = bc_case(load_bc(), request.param[3:], "tiny")
     else:
         c = config_case(0, jitter=0.1)
         rng = np.random.default_rng(11)
