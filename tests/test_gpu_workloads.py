@@ -11,13 +11,14 @@ at the workloads' 1e-10 the FP64 BiCGSTAB's own pressure is 3e-6 from a
 solve to 1e-14, the precision form's 2e-7: the gap is the FP64 path's
 stopping error, not FP32's).  The largest workload is checked by properties
 (converged solves, zero volume mean of p, finite and bounded fields), since
-no second implementation finishes it in seconds.
+no second implementation finishes it in seconds, and its differentiation
+form by the agreement of the adjoint gradient with the tangent-linear run.
 """
 import numpy as np
 import pytest
 
 from common import interior, rel_err
-from nsx.abi import PREC_FP64, PREC_MIXED, REFINE_CGSR_BLOCK
+from nsx.abi import CELL_SCRATCH, FACE_SCRATCH, PREC_FP64, PREC_MIXED, REFINE_CGSR_BLOCK
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -99,3 +100,47 @@ def test_largest_workload_512x256x128_two_steps():
     assert np.isfinite(u).all() and np.abs(u).max() < 5 * case.arrays["u0"].max()
     assert np.isfinite(p).all()
     G.close()
+
+
+def test_largest_workload_512x256x128_adjoint_ten_steps():
+    """The differentiation form at the largest workload: 20 RK2 steps of the
+    precision form from the initial field, then the bottom-wall shear J over
+    10 further steps and dJ/d(body force) by the checkpointed adjoint
+    (checkpoints every 5 steps, segments recomputed); from the same state the
+    tangent-linear run along a smooth direction gives the same J bit for bit
+    and the gradient's projection on the direction to 1e-6.  The device
+    footprint with the adjoint's checkpoints and work fields is printed."""
+    case = config_case(4)
+    G = _ctx(case)
+    n0, nz = case.N_CELL0, case.NZ
+    G.set_body_force(np.ones((n0, nz - 2)))
+    G.set_precision(PREC_MIXED, REFINE_CGSR_BLOCK, 0.1, 30)
+    G.init_state()
+    info = G.step(20, PREC_MIXED)
+    assert info["exit"] == "converged", info
+    state = ("UFN", "VFN", "WFN", "PFN")
+    faces = ("U1FACE", "U2FACE", "U3FACE", "UTFACE", "UBFACE")
+    for q, n in enumerate(state):
+        G.copy(CELL_SCRATCH + q, n)
+    for q, n in enumerate(faces):
+        G.copy(FACE_SCRATCH + q, n)
+    J, grad = G.wall_shear(10, 5, grad=True)
+    try:
+        used = f"{G.device_memory_used() / 2**30:.1f} GiB"
+    except Exception as e:  # the footprint is logged, not tested
+        used = f"not available ({e})"
+    for q, n in enumerate(state):
+        G.copy(n, CELL_SCRATCH + q)
+    for q, n in enumerate(faces):
+        G.copy(n, FACE_SCRATCH + q)
+    x = case.xc[:n0, None] / (case.xc[:n0].max() - case.xc[:n0].min())
+    zeta = case.sig[None, 1:-1] / case.sig[1:-1].max()
+    d = np.sin(2 * np.pi * x) * zeta * (1.0 - zeta) + 0.5
+    J2, dJ = G.wall_shear_tangent(10, d)
+    G.close()
+    adj = float(np.sum(grad * d))
+    print(f"512x256x128 adjoint over 10 steps: J {J:.15e}, adjoint {adj:.12e}, tangent {dJ:.12e}, "
+          f"device memory in use {used}", flush=True)
+    assert np.isfinite(grad).all() and np.abs(grad).max() > 0
+    assert J2 == J
+    assert abs(adj - dJ) <= 1e-6 * abs(dJ), (adj, dJ)
