@@ -679,6 +679,15 @@ static int build_bc(nsx_case *c) {
       int nc = c->No_cp[IX2(i, j, NC)];
       if (!(nc < 1 || nc > N0)) continue;
       int ii = c->No_cp[IX2(nc - 1, 2, NC)];
+      if (ii == -1) {
+        /* a wall: the Kim-Choi split of its face is the plain normal one
+         * (update_bc.F90:160-171) */
+        const size_t f = IX2(i, j, N0);
+        c->tagsp[nc - 1] = 1;
+        c->dle1[f] = 1.0;
+        c->dle2[f] = 0.;
+        c->dle3[f] = 0.0;
+      }
       int tx, ty;
       cell_tags(c, nc - 1, &tx, &ty);
       if (tx > 0 && ty > 0) {
