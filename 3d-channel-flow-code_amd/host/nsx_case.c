@@ -611,6 +611,83 @@ done:
   return 0;
 }
 
+/* ------------------------------------------ Tecplot cell-centre elements --- */
+/* calcul_geometry.F90:1590-1665: around each vertex, a fan of triangles over
+ * the centres of the cells that share it, (initc, neigc, next) with initc
+ * fixed, walking from cell to cell across the edges that touch the vertex.
+ * The walk starts at the first cell (in cell order) that has the vertex on a
+ * ghost edge (a boundary vertex, initc the ghost) or as a corner (an interior
+ * vertex) and stops at a ghost or, dropping its last triangle, back at
+ * initc.  The reference finds each start by scanning every cell; here the
+ * first occurrences are collected in one pass in the same order.  Cell
+ * numbers are 1-based as the reference stores them. */
+static int build_tecplot(nsx_case *c) {
+  const int N0 = c->N_CELL0, NV = c->N_VERT, NC = c->N_CELL;
+  const int *No_vp = c->No_vp, *No_cp = c->No_cp;
+#define VP(i, j) (No_vp[IX2(i, j, N0)] - 1)
+#define CP(i, j) No_cp[IX2(i, j, NC)]
+  int *first_b = xcalloc((size_t)NV, sizeof(int)), *first_v = xcalloc((size_t)NV, sizeof(int));
+  for (int v = 0; v < NV; ++v) first_b[v] = first_v[v] = -1;
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int jv = VP(i, j);
+      if (first_v[jv] < 0) first_v[jv] = 3 * i + j;
+      if (c->nbe[i] != 0 && CP(i, j) > N0) {
+        const int jv1 = VP(i, j), jv2 = VP(i, (j + 1) % 3);
+        if (first_b[jv1] < 0) first_b[jv1] = 3 * i + j;
+        if (first_b[jv2] < 0) first_b[jv2] = 3 * i + j;
+      }
+    }
+  const int cap = 5 * NV;
+  int num = 0, initc = 0, lastc = 0, neigc = 0, rc = 0;
+  for (int nv = 0; nv < NV && !rc; ++nv) {
+    const int f = c->nbev[nv] != 0 ? first_b[nv] : first_v[nv];
+    if (f >= 0) {
+      const int i = f / 3, j = f % 3;
+      if (c->nbev[nv] != 0) {
+        initc = CP(i, j);
+        lastc = initc;
+        neigc = i + 1;
+      } else {
+        initc = i + 1;
+        lastc = i + 1;
+        neigc = CP(i, j);
+      }
+    }
+    for (int steps = 0; neigc >= 1 && neigc <= N0; ++steps) {
+      if (num >= cap || steps > 64) {
+        snprintf(c->err, sizeof c->err, "Tecplot cell-centre walk around vertex %d does not close", nv + 1);
+        rc = -1;
+        break;
+      }
+      c->ic1tec[num] = initc;
+      c->ic2tec[num] = neigc;
+      ++num;
+      int next = 0;
+      for (int j = 0; j < 3; ++j) {
+        const int jc = CP(neigc - 1, j);
+        if (jc == lastc) continue;
+        const int jv1 = VP(neigc - 1, j), jv2 = VP(neigc - 1, (j + 1) % 3);
+        if (jv1 == nv || jv2 == nv) {
+          c->ic3tec[num - 1] = jc;
+          lastc = neigc;
+          neigc = jc;
+          next = jc == initc ? 2 : 1;
+          break;
+        }
+      }
+      if (next == 2) --num;
+      if (next != 1) break;
+    }
+  }
+  c->Numictec[0] = num;
+  free(first_b);
+  free(first_v);
+#undef VP
+#undef CP
+  return rc;
+}
+
 /* -------------------------------------------------------- BC flags ------- */
 /* TagXC/TagYC of a cell from its centre (calcul_geometry.F90:857-886): 1 or 2
  * below or above the box in x (y), 0 inside */
@@ -864,17 +941,25 @@ static void alloc_case(nsx_case *c) {
   c->v0 = add_array(c, "v0", NSX_T_F64, NC * NZ);
   c->w0 = add_array(c, "w0", NSX_T_F64, NC * NZ);
   c->p0 = add_array(c, "p0", NSX_T_F64, NC * NZ);
+  c->ic1tec = add_array(c, "ic1tec", NSX_T_I32, 5 * NV);
+  c->ic2tec = add_array(c, "ic2tec", NSX_T_I32, 5 * NV);
+  c->ic3tec = add_array(c, "ic3tec", NSX_T_I32, 5 * NV);
+  c->Numictec = add_array(c, "Numictec", NSX_T_I32, 1);
+  c->grid = add_array(c, "grid", NSX_T_I32, 3);
 
 }
 
 /* sigma levels, geometry, flags, PPE matrix, wall distance and the initial
  * field of a case whose mesh is set */
 static nsx_case *finish_case(nsx_case *c) {
-  if ((build_sigma(c), 0) || build_geometry(c) || build_bc(c)) {
+  if ((build_sigma(c), 0) || build_geometry(c) || build_tecplot(c) || build_bc(c)) {
     c->status = -1;
     return c;
   }
   build_ppe(c);
+  c->grid[0] = c->nx;
+  c->grid[1] = c->ny;
+  c->grid[2] = c->NN;
   /* channel wall distance: |sig(k)| (sgm_les.F90:856-862) */
   for (int k = 0; k < c->NZ; ++k) c->walld[k] = fabs(c->sig[k]);
   build_ic(c);
@@ -1189,6 +1274,153 @@ int nsx_write_mesh(const char *path, int N_CELL0, int N_VERT, const int *No_vp, 
     if (v % 20 == 19 || v == N_VERT - 1) fputc('\n', f);
   }
   return fclose(f) == 0 ? 0 : -1;
+}
+
+/* --------------------------------------------- the cached binary set-up --- */
+/* One file holds a finished set-up: the sizes and scalars of the case, then
+ * every array by name, type and length, then a 64-bit checksum of all the
+ * bytes before it.  Loading it skips the geometry, flag and matrix build. */
+static const char CASE_MAGIC[8] = {'N', 'S', 'X', 'C', 'A', 'S', 'E', '1'};
+
+typedef struct {
+  FILE *f;
+  uint64_t h;
+  int err;
+} case_io;
+
+/* word-wise multiply-xor hash, for detecting a corrupt or truncated file */
+static void io_hash(case_io *io, const void *p, size_t n) {
+  const unsigned char *b = p;
+  uint64_t h = io->h;
+  size_t q = 0;
+  for (; q + 8 <= n; q += 8) {
+    uint64_t w;
+    memcpy(&w, b + q, 8);
+    h = (h ^ w) * 0x100000001b3ULL;
+    h ^= h >> 29;
+  }
+  for (; q < n; ++q) h = (h ^ b[q]) * 0x100000001b3ULL;
+  io->h = h;
+}
+
+static void io_write(case_io *io, const void *p, size_t n) {
+  if (io->err) return;
+  if (fwrite(p, 1, n, io->f) != n) io->err = 1;
+  io_hash(io, p, n);
+}
+
+static void io_read(case_io *io, void *p, size_t n) {
+  if (io->err) return;
+  if (fread(p, 1, n, io->f) != n) {
+    io->err = 1;
+    return;
+  }
+  io_hash(io, p, n);
+}
+
+static void case_header(nsx_case *c, int32_t *iv, double *dv) {
+  int32_t *const ip[17] = {&c->nx, &c->ny, &c->NN, &c->NZ, &c->N_CELL0, &c->N_VERT, &c->N_CELL, &c->N_CELLexact,
+                           &c->N_CELLghost, &c->bctop, &c->bcbot, &c->bed, &c->XPB, &c->YPB, &c->XBC, &c->YBC,
+                           &c->n_arrays};
+  double *const dp[12] = {&c->Lx, &c->Ly, &c->H, &c->stretch, &c->jitter, &c->Ubulk, &c->noise, &c->TotalVolume,
+                          &c->XDIni, &c->XDFin, &c->YDIni, &c->YDFin};
+  if (iv) {
+    for (int q = 0; q < 17; ++q) iv[q] = *ip[q];
+    for (int q = 0; q < 12; ++q) dv[q] = *dp[q];
+  }
+}
+
+int nsx_case_save(const nsx_case *c, const char *path) {
+  if (!c || c->status != 0 || !path) return -1;
+  case_io io = {fopen(path, "wb"), 0xcbf29ce484222325ULL, 0};
+  if (!io.f) return -1;
+  int32_t iv[17];
+  double dv[12];
+  case_header((nsx_case *)c, iv, dv);
+  const int32_t version = 1;
+  io_write(&io, CASE_MAGIC, 8);
+  io_write(&io, &version, 4);
+  io_write(&io, iv, sizeof iv);
+  io_write(&io, dv, sizeof dv);
+  io_write(&io, &c->seed, 8);
+  for (int a = 0; a < c->n_arrays; ++a) {
+    const nsx_case_array *A = &c->arrays[a];
+    const int32_t type = A->type;
+    io_write(&io, A->name, sizeof A->name);
+    io_write(&io, &type, 4);
+    io_write(&io, &A->n, 8);
+    io_write(&io, A->ptr, (size_t)A->n * (A->type == NSX_T_F64 ? sizeof(double) : sizeof(int)));
+  }
+  const uint64_t h = io.h;
+  if (!io.err && fwrite(&h, 1, 8, io.f) != 8) io.err = 1;
+  if (fclose(io.f) != 0) io.err = 1;
+  return io.err ? -1 : 0;
+}
+
+nsx_case *nsx_case_load(const char *path) {
+  nsx_case *c = xcalloc(1, sizeof(nsx_case));
+  case_io io = {path ? fopen(path, "rb") : NULL, 0xcbf29ce484222325ULL, 0};
+  c->status = -1;
+  if (!io.f) {
+    snprintf(c->err, sizeof c->err, "cannot open %s", path ? path : "(null)");
+    return c;
+  }
+  char magic[8];
+  int32_t version = 0, iv[17];
+  double dv[12];
+  io_read(&io, magic, 8);
+  io_read(&io, &version, 4);
+  io_read(&io, iv, sizeof iv);
+  io_read(&io, dv, sizeof dv);
+  io_read(&io, &c->seed, 8);
+  if (io.err || memcmp(magic, CASE_MAGIC, 8) != 0 || version != 1) {
+    snprintf(c->err, sizeof c->err, "%s is not a cached set-up of this version", path);
+    fclose(io.f);
+    return c;
+  }
+  c->nx = iv[0]; c->ny = iv[1]; c->NN = iv[2]; c->NZ = iv[3]; c->N_CELL0 = iv[4]; c->N_VERT = iv[5];
+  c->N_CELL = iv[6]; c->N_CELLexact = iv[7]; c->N_CELLghost = iv[8]; c->bctop = iv[9]; c->bcbot = iv[10];
+  c->bed = iv[11]; c->XPB = iv[12]; c->YPB = iv[13]; c->XBC = iv[14]; c->YBC = iv[15];
+  c->Lx = dv[0]; c->Ly = dv[1]; c->H = dv[2]; c->stretch = dv[3]; c->jitter = dv[4]; c->Ubulk = dv[5];
+  c->noise = dv[6]; c->TotalVolume = dv[7]; c->XDIni = dv[8]; c->XDFin = dv[9]; c->YDIni = dv[10];
+  c->YDFin = dv[11];
+  if (c->N_CELL0 < 1 || c->N_VERT < 3 || c->NN < 2 || c->NZ != c->NN + 2 || c->N_CELL < c->N_CELL0 ||
+      (int64_t)c->N_CELL * c->NZ > ((int64_t)1 << 34)) {
+    snprintf(c->err, sizeof c->err, "%s: sizes out of range", path);
+    fclose(io.f);
+    return c;
+  }
+  alloc_case(c);
+  if (iv[16] != c->n_arrays) {
+    snprintf(c->err, sizeof c->err, "%s holds %d arrays, this build %d", path, iv[16], c->n_arrays);
+    fclose(io.f);
+    return c;
+  }
+  for (int a = 0; a < c->n_arrays && !io.err; ++a) {
+    nsx_case_array *A = &c->arrays[a];
+    char name[sizeof A->name];
+    int32_t type = 0;
+    int64_t n = 0;
+    io_read(&io, name, sizeof name);
+    io_read(&io, &type, 4);
+    io_read(&io, &n, 8);
+    if (io.err || memcmp(name, A->name, sizeof name) != 0 || type != A->type || n != A->n) {
+      snprintf(c->err, sizeof c->err, "%s: array %d is not %s of this build", path, a, A->name);
+      fclose(io.f);
+      return c;
+    }
+    io_read(&io, A->ptr, (size_t)n * (type == NSX_T_F64 ? sizeof(double) : sizeof(int)));
+  }
+  const uint64_t h = io.h;
+  uint64_t stored = 0;
+  if (!io.err && fread(&stored, 1, 8, io.f) != 8) io.err = 1;
+  fclose(io.f);
+  if (io.err || stored != h) {
+    snprintf(c->err, sizeof c->err, "%s is truncated or corrupt (checksum)", path);
+    return c;
+  }
+  c->status = 0;
+  return c;
 }
 
 int nsx_case_status(const nsx_case *c, char *msg, int len) {

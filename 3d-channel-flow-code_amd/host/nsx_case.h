@@ -60,6 +60,8 @@ typedef struct nsx_case {
   double *AmC0, *AmC1, *AmC2, *AmC3, *AmCT, *AmCB;
   double *walld;
   double *u0, *v0, *w0, *p0;
+  int *ic1tec, *ic2tec, *ic3tec, *Numictec; /* Tecplot cell-centre elements (5 N_VERT), their count */
+  int *grid;                                /* nx, ny, NN of a generated mesh (0, 0, NN for a mesh file) */
 } nsx_case;
 
 #ifdef __cplusplus

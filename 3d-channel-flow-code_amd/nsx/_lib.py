@@ -38,6 +38,9 @@ def host() -> ctypes.CDLL:
         lib.nsx_case_create_walls.restype = c_vp
         lib.nsx_case_create_walls.argtypes = [c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_int,
                                               c_dbl, c_dbl, ctypes.c_uint64, c_int, c_int, c_int, c_int]
+        lib.nsx_case_save.argtypes = [c_vp, c_cp]
+        lib.nsx_case_load.restype = c_vp
+        lib.nsx_case_load.argtypes = [c_cp]
         lib.nsx_case_free.argtypes = [c_vp]
         lib.nsx_case_status.argtypes = [c_vp, c_cp, c_int]
         lib.nsx_case_info.argtypes = [c_vp, ctypes.POINTER(c_int)]

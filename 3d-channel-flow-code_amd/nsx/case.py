@@ -101,15 +101,34 @@ _FACE3 = {"dxVV", "dyVV", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "xme", "
 
 def make_case(nx: int, ny: int, nn: int, *, seed: int = 1234, jitter: float = 0.0, bed: bool = False,
               Ubulk: float = 15.6, noise: float = 0.05, stretch: float = 2.0, params: Params | None = None,
-              Lx: float = LX, Ly: float = LY, Hh: float = H, walls: tuple | None = None) -> Case:
+              Lx: float = LX, Ly: float = LY, Hh: float = H, walls: tuple | None = None,
+              save: str | None = None) -> Case:
     """Generate a channel case.  walls = (XPB, YPB, XBC, YBC) for a box with
-    walled sides (default periodic in x and y).  Raises RuntimeError with
-    the set-up's message."""
+    walled sides (default periodic in x and y); save: also write the
+    finished set-up to this file (load_case reads it back).  Raises
+    RuntimeError with the set-up's message."""
     lib = _lib.host()
     xpb, ypb, xbc, ybc = walls if walls is not None else (1, 1, 0, 1)
     ptr = lib.nsx_case_create_walls(nx, ny, nn, Lx, Ly, Hh, stretch, jitter, int(bool(bed)), Ubulk, noise,
                                     ctypes.c_uint64(seed), int(xpb), int(ypb), int(xbc), int(ybc))
+    if save is not None and lib.nsx_case_status(ptr, None, 0) == 0:
+        if lib.nsx_case_save(ptr, save.encode()) != 0:
+            lib.nsx_case_free(ptr)
+            raise OSError(f"cannot write the set-up to {save}")
     return _collect(lib, ptr, nx, ny, nn, params or Params(sigFin=Hh))
+
+
+def load_case(path: str, params: Params | None = None) -> Case:
+    """A set-up written by make_case(save=...) (nsx_case_load: no geometry,
+    flag or matrix build).  Raises RuntimeError when the file is missing,
+    foreign, truncated or corrupt."""
+    lib = _lib.host()
+    ptr = lib.nsx_case_load(path.encode())
+    n = ctypes.c_int64()
+    t = ctypes.c_int()
+    g = lib.nsx_case_get(ptr, b"grid", ctypes.byref(n), ctypes.byref(t))
+    grid = np.ctypeslib.as_array(ctypes.cast(g, ctypes.POINTER(ctypes.c_int32)), shape=(3,)).copy() if g else (0, 0, 0)
+    return _collect(lib, ptr, int(grid[0]), int(grid[1]), int(grid[2]), params or Params())
 
 
 MESH_KEYS = ("No_vp", "xv", "yv", "No_cp", "zbv", "nbe", "nbev")

@@ -63,6 +63,12 @@ nsx_case *nsx_case_create_walls(int nx, int ny, int nn, double Lx, double Ly, do
                                 double jitter, int bed, double Ubulk, double noise, uint64_t seed, int XPB, int YPB,
                                 int XBC, int YBC);
 void nsx_case_free(nsx_case *c);
+/* the cached binary set-up: nsx_case_save writes a finished case (sizes,
+ * scalars, every array, a checksum) to path, 0 or -1; nsx_case_load reads it
+ * back without rebuilding the geometry, flags and matrix (check
+ * nsx_case_status: a missing, foreign, truncated or corrupt file is refused) */
+int nsx_case_save(const nsx_case *c, const char *path);
+nsx_case *nsx_case_load(const char *path);
 /* 0, or -1 with the set-up's message in msg */
 int nsx_case_status(const nsx_case *c, char *msg, int len);
 /* N_CELL0, N_CELL, N_VERT, NZ, N_CELLexact, N_CELLghost, bctop, bcbot */

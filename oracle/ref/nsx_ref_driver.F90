@@ -309,6 +309,17 @@
       call dump_r8('dsig', dsig, NZ)
       call dump_r8('sigv', sigv, NZ-1)
       call dump_r8('dsigv', dsigv, NZ-1)
+!     the Tecplot cell-centre elements (entries past Numictec are unset)
+      do n1 = Numictec+1, 5*N_VERT
+         ic1tec(n1) = 0
+         ic2tec(n1) = 0
+         ic3tec(n1) = 0
+      enddo
+      call dump_i4('ic1tec', ic1tec, 5*N_VERT)
+      call dump_i4('ic2tec', ic2tec, 5*N_VERT)
+      call dump_i4('ic3tec', ic3tec, 5*N_VERT)
+      iv(1) = Numictec
+      call dump_i4('Numictec', iv, 1)
       end subroutine
 
       subroutine dump_state(sfx)

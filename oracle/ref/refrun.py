@@ -70,6 +70,9 @@ def write_case(path: str, case, *, mode: int, nsteps: int = 1, dump_every: int =
             f.write(np.asfortranarray(a[:n0]).tobytes(order="F"))
 
 
+FLAT = {"ic1tec", "ic2tec", "ic3tec", "Numictec"}
+
+
 def read_dump(path: str, case) -> dict:
     out = {}
     with open(path, "rb") as f:
@@ -83,7 +86,8 @@ def read_dump(path: str, case) -> dict:
         dt = np.float64 if typ == 1 else np.int32
         arr = np.frombuffer(data, dt, n, pos).copy()
         pos += n * arr.itemsize
-        out[name] = _shape(arr, case)
+        # the Tecplot connectivity lists stay flat whatever their length
+        out[name] = arr if name in FLAT else _shape(arr, case)
     return out
 
 

@@ -18,7 +18,7 @@ GEOM_CELL = ["xc", "yc", "areaCell", "No_cp", "bcuc", "bcvc", "bcwc", "bcpc"]
 GEOM_FULL = ["dxVV", "dyVV", "dlVV", "xnn", "ynn", "dxCC", "dyCC", "dlCC", "xme", "yme", "xmo", "ymo", "doe",
              "dle1", "dle2", "dle3", "sum_xc2", "sum_yc2", "sum_xcyc", "dlVsum", "Dimsurrounding",
              "vertex_pair", "vertex_edge", "tagXV", "tagYV", "bcuv", "bcvv", "bcwv", "bcpv", "sig", "dsig", "sigv",
-             "dsigv"]
+             "dsigv", "ic1tec", "ic2tec", "ic3tec", "Numictec"]
 
 
 def test_setup_geometry_bitwise():

@@ -10,6 +10,8 @@ mesh (nsx_case_from_mesh).
   (tests/golden/mesh_setup.json: SHA-256 of every set-up array).
 * On the synthetic meshes the set-up from arrays equals the generator's,
   which the geometry tests pin to the reference.
+* The cached binary set-up (nsx_case_save / nsx_case_load) gives back every
+  array bit for bit and refuses a truncated, corrupt or foreign file.
 """
 import hashlib
 import json
@@ -20,7 +22,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from nsx.case import case_from_mesh, config_case, make_case, read_mesh, write_mesh
+from nsx.case import case_from_mesh, config_case, load_case, make_case, read_mesh, write_mesh
 
 MESH_KEYS = ("No_vp", "xv", "yv", "No_cp", "zbv", "nbe", "nbev")
 
@@ -102,6 +104,8 @@ def test_setup_from_arrays_equals_generator(kw):
     s = case_from_mesh(mesh, c.nn, Ubulk=15.6, seed=1234)
     assert (s.N_CELLexact, s.N_CELLghost, s.TotalVolume) == (c.N_CELLexact, c.N_CELLghost, c.TotalVolume)
     for k, a in c.arrays.items():
+        if k == "grid":  # the generator's nx, ny: a mesh file has none
+            continue
         assert np.array_equal(s.arrays[k], a), k
 
 
@@ -125,3 +129,34 @@ def test_mesh_errors(tmp_path):
     # periodic partners are not found, as in the reference
     with pytest.raises(RuntimeError, match="periodic matching failed"):
         case_from_mesh(mesh, c.nn)
+
+
+@pytest.mark.parametrize("walls", [None, (1, 0, 0, 0)])
+def test_cached_setup_round_trip(tmp_path, walls):
+    """A set-up written to its cache file and read back: the same sizes,
+    volume and every array, bit for bit (the periodic channel and a duct)."""
+    path = str(tmp_path / "case.bin")
+    c = make_case(24, 12, 16, seed=9, jitter=0.1, walls=walls, save=path)
+    d = load_case(path, params=c.params)
+    for k in ("nx", "ny", "nn", "N_CELL0", "N_CELL", "N_VERT", "NZ", "N_CELLexact", "N_CELLghost", "bctop",
+              "bcbot", "TotalVolume"):
+        assert getattr(d, k) == getattr(c, k), k
+    assert sorted(d.arrays) == sorted(c.arrays)
+    for k in c.arrays:
+        assert c.arrays[k].dtype == d.arrays[k].dtype and np.array_equal(c.arrays[k], d.arrays[k]), k
+
+
+def test_cached_setup_refuses_damaged_files(tmp_path):
+    path = str(tmp_path / "case.bin")
+    make_case(8, 6, 4, seed=3, save=path)
+    good = open(path, "rb").read()
+    cases = {"truncated": good[:len(good) // 2], "corrupt": good[:200] + bytes([good[200] ^ 1]) + good[201:],
+             "foreign": b"NOTACASE" + good[8:]}
+    for what, data in cases.items():
+        p = str(tmp_path / (what + ".bin"))
+        with open(p, "wb") as f:
+            f.write(data)
+        with pytest.raises(RuntimeError):
+            load_case(p)
+    with pytest.raises(RuntimeError):
+        load_case(str(tmp_path / "missing.bin"))
