@@ -273,3 +273,22 @@ def test_glob_cfl_vs_oracle():
     scale = abs(ref["flux_max"]) * c.n_interior
     for k in ("flux_total", "flux_bc", "flux_bot", "flux_top", "flux_side"):
         assert abs(got[k] - ref[k]) <= 1e-13 * scale, (k, got[k], ref[k])
+
+
+def test_cached_setup_drives_the_device(tmp_path):
+    """A device context set up from the cached set-up file (nsx_case_load)
+    steps exactly as one set up from the freshly built case: the same bits
+    after two RK2 steps."""
+    from nsx.case import load_case, make_case
+    path = str(tmp_path / "case.bin")
+    c = make_case(32, 16, 32, seed=1234, jitter=0.1, save=path)
+    d = load_case(path, params=c.params)
+    out = []
+    for case in (c, d):
+        G = _ctx(case)
+        G.init_state()
+        G.step(2)
+        out.append([G.get(n) for n in ("UFN", "VFN", "WFN", "PFN")])
+        G.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
