@@ -519,7 +519,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_adj_vertex_idw(Geo g, const doubl
   if (t >= (uint32_t)g.NV * g.NL) return;
   const int v = (int)fdiv(t, g.divNL);
   const int l = (int)(t - (uint32_t)v * g.NL);
-  if (act[v] == VA_ZERO) return;
+  if (act[v] == VA_ZERO || act[v] == VA_INFLOW) return;
   const double b = valb[(size_t)v * g.NL + l];
   if (b == 0.0) return;
   const double r = b / g.dlVsum[v];

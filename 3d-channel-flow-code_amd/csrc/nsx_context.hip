@@ -50,6 +50,7 @@ struct AdjWork {
 struct nsx_ctx {
   int N0 = 0, NC = 0, NV = 0, NZ = 0, NS = 0, NEX = 0;
   int NGW = 0;                 // wall ghost columns after the N0 interior ones
+  bool has_inflow = false;     // a ghost or vertex flagged inflow (2)
   std::vector<int> gw_ghost;   // the reference's ghost row (0-based) of each wall ghost column
   int dev = 0;
   hipStream_t stream = nullptr;
@@ -415,7 +416,7 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   std::vector<int> nbr((size_t)N0 * 3), vtx((size_t)N0 * 3);
   std::vector<signed char> eside((size_t)N0 * 3, 0);
   std::vector<int> gw_owner;
-  std::vector<unsigned char> gw_neu;
+  std::vector<unsigned char> gw_neu, gw_inf;
   ctx->gw_ghost.clear();
   int nex = N0;
   for (int i = 0; i < N0; ++i)
@@ -430,16 +431,17 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
           int partner = ctx->No_cp[(size_t)jc + 2 * (size_t)NC] - 1;
           if (partner < 0 || partner >= N0) return nsx_fail(ctx, -1, "ghost cell %d has no periodic partner", jc + 1);
           jc = partner;
-        } else if ((fu == 0 || fu == 1) && (fv == 0 || fv == 1) && (fw == 0 || fw == 1) && fp == 1) {
-          // bit q: flag 1 for kind q (0 p, 1 u, 2 v, 3 w)
-          gw_neu.push_back((unsigned char)(1 | (fu << 1) | (fv << 2) | (fw << 3)));
+        } else if (fu >= 0 && fu <= 2 && fv >= 0 && fv <= 2 && fw >= 0 && fw <= 2 && fp == 1) {
+          // bit q: flag 1 (copy), resp. 2 (inflow), for kind q (0 p, 1 u, 2 v, 3 w)
+          gw_neu.push_back((unsigned char)(1 | ((fu == 1) << 1) | ((fv == 1) << 2) | ((fw == 1) << 3)));
+          gw_inf.push_back((unsigned char)(((fu == 2) << 1) | ((fv == 2) << 2) | ((fw == 2) << 3)));
           gw_owner.push_back(i);
           ctx->gw_ghost.push_back(jc);
           jc = N0 + (int)gw_owner.size() - 1;
         } else {
           return nsx_fail(ctx, -1,
                           "ghost cell %d has flags u %d v %d w %d p %d: supported are periodic ghosts (all 3) and "
-                          "walls (u, v, w 0 or 1, p 1); inflow (2) and parallel (-1) ghosts are not",
+                          "walls (u, v, w 0, 1 or 2, p 1); parallel (-1) ghosts are not",
                           jc + 1, fu, fv, fw, fp);
         }
       }
@@ -450,6 +452,10 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     }
   ctx->NEX = nex;
   ctx->NGW = (int)gw_owner.size();
+  ctx->has_inflow = false;
+  for (unsigned char b : gw_inf) ctx->has_inflow = ctx->has_inflow || b != 0;
+  for (int q = 0; q < 3; ++q)
+    for (int v = 0; v < NV; ++v) ctx->has_inflow = ctx->has_inflow || (ctx->nbev[v] != 0 && ctx->bcv[q][v] == 2);
   if ((uint64_t)(N0 + ctx->NGW) * NS >= (1ull << 31))
     return nsx_fail(ctx, -1, "the mesh with its wall ghost columns exceeds 32-bit cell numbering");
 
@@ -526,8 +532,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     vkind[v] = ctx->vertex_edge[v] == 1 ? VK_CORNER : (ctx->vertex_pair[v] > 0 ? VK_PAIR : VK_OTHER);
     for (int q = 0; q < 4; ++q) {
       const int fl = ctx->bcv[q][v];
-      if (fl == 2) return nsx_fail(ctx, -1, "inflow vertex flags are not supported (vertex %d)", v + 1);
-      vact[q][v] = fl == 0 ? VA_ZERO : (fl == 3 ? VA_PERIODIC : VA_NONE);
+      vact[q][v] = fl == 0 ? VA_ZERO : (fl == 2 ? VA_INFLOW : (fl == 3 ? VA_PERIODIC : VA_NONE));
+      if (fl == 2 && q == 3) return nsx_fail(ctx, -1, "vertex %d: a pressure vertex flagged inflow", v + 1);
     }
   }
   std::vector<int> vpair(NV);
@@ -557,13 +563,16 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
   g.NGW = ctx->NGW;
   g.gw_owner = nullptr;
   g.gw_neu = nullptr;
+  g.gw_inf = nullptr;
   if (ctx->NGW > 0) {
     int *d_go;
-    unsigned char *d_gn;
+    unsigned char *d_gn, *d_gi;
     UP(d_go, gw_owner);
     UP(d_gn, gw_neu);
+    UP(d_gi, gw_inf);
     g.gw_owner = d_go;
     g.gw_neu = d_gn;
+    g.gw_inf = d_gi;
   }
   const std::vector<double> *edge[9] = {&ctx->dlVV, &ctx->dlCC, &ctx->dle1, &ctx->dle2, &ctx->doe,
                                         &ctx->xnn, &ctx->ynn, &ctx->dxCC, &ctx->dyCC};
@@ -1079,6 +1088,7 @@ static int vertex_of(nsx_ctx *ctx, int nf, const double *const f[3], double *con
   const Geo &g = ctx->g;
   const signed char *act[3];
   int zb[3], zt[3], lid[3] = {0, 0, 0};
+  double infl[3] = {0.0, 0.0, 0.0};
   for (int q = 0; q < nf; ++q) {
     const int kq = kinds[q];
     act[q] = kq == 0 ? g.vact_p : (kq == 1 ? g.vact_u : (kq == 2 ? g.vact_v : g.vact_w));
@@ -1088,8 +1098,10 @@ static int vertex_of(nsx_ctx *ctx, int nf, const double *const f[3], double *con
                     : (g.bctop == 1 && (g.TopBC == 0 || (g.TopBC == 1 && kq == 3) || (g.TopBC == 2 && kq != 1)));
     // the lid sets u = 1 on the top level of the interior vertices (BC.F90:771-777)
     lid[q] = kq == 1 && g.bctop == 1 && g.TopBC == 2;
+    // inflow() of Functions.F90:109-121 at an inflow vertex: 1 for v, 0 for u and w
+    infl[q] = kq == 2 ? 1.0 : 0.0;
   }
-  LAUNCH(launch_vertex(g, ctx->stream, nf, f, fv, act, zb, zt, tagdir, lid));
+  LAUNCH(launch_vertex(g, ctx->stream, nf, f, fv, act, zb, zt, tagdir, lid, infl));
   return 0;
 }
 
@@ -2055,6 +2067,9 @@ static int diff_checks(nsx_ctx *ctx) {
   if (ctx->TopBC == 2)
     return nsx_fail(ctx, -1, "the differentiation form does not transpose the lid (TopBC = 2): its vertex values "
                              "are constants the tangent and adjoint vertex kernels do not hold");
+  if (ctx->has_inflow)
+    return nsx_fail(ctx, -1, "the differentiation form does not take inflow boundaries (flag 2): their ghost and "
+                             "vertex values hold constants the tangent kernels would add");
   return 0;
 }
 

@@ -49,7 +49,7 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 
 // Vertex boundary action per field: what BCvelvertex3D / BCpvertex and
 // bc_periodic do to a vertex value (BC.F90:603-1408).
-enum : signed char { VA_NONE = 0, VA_ZERO = 1, VA_PERIODIC = 3 };
+enum : signed char { VA_NONE = 0, VA_ZERO = 1, VA_INFLOW = 2, VA_PERIODIC = 3 };
 // Periodic vertex kind (vertex_edge / vertex_pair of calcul_geometry.F90:920-1025)
 enum : signed char { VK_INTERIOR = 0, VK_PAIR = 1, VK_CORNER = 2, VK_OTHER = 3 };
 
@@ -75,7 +75,8 @@ struct Geo {
   // wall ghost columns N0 .. N0+NGW-1 (walled sides of the box)
   int NGW;
   const int *gw_owner;        // the interior column whose edge the ghost lies behind
-  const unsigned char *gw_neu; // bit q set: flag 1 (copy) for kind q (0 p, 1 u, 2 v, 3 w); clear: flag 0
+  const unsigned char *gw_neu; // bit q set: flag 1 (copy) for kind q (0 p, 1 u, 2 v, 3 w); clear: flag 0 or 2
+  const unsigned char *gw_inf; // bit q set: flag 2 (inflow: 2 inflow() - phi) for kind q
   // per cell (N0)
   const double *area, *sum_xc2, *sum_yc2, *sum_xcyc;
   // per level

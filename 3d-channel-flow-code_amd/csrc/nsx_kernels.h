@@ -77,7 +77,7 @@ int cgs64_blocks_per_cu(const Geo &g);
 // ---- stage kernels (stream-ordered launches; return hipError_t as int)
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
                   const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir,
-                  const int *lid_top = nullptr);
+                  const int *lid_top = nullptr, const double *inflow = nullptr);
 int launch_wall_ghosts(const Geo &g, hipStream_t s, double *f, int kind);
 int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
                const double *vv, const double *wv, double *eddy);

@@ -64,6 +64,7 @@ struct VtxArgs {
   const signed char *act[3];
   int zero_bot[3], zero_top[3];
   int lid_top[3];  // the lid: 1 on the top level of the interior vertices (BC.F90:771-777)
+  double inflow[3];  // the value of an inflow vertex (VA_INFLOW, BC.F90:724-731)
   int tagdir;
 };
 
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
         r = idw(g, vv, l, f);
       }
       if (act[vv] == VA_ZERO) r = 0.;
+      if (act[vv] == VA_INFLOW) r = a.inflow[q];
       if (zero_wall) r = 0.;
       return r;
     };
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
 
 int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3], double *const fv[3],
                   const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir,
-                  const int *lid_top) {
+                  const int *lid_top, const double *inflow) {
   VtxArgs a;
   for (int q = 0; q < 3; ++q) {
     a.f[q] = q < nf ? f[q] : nullptr;
@@ -175,6 +177,7 @@ int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3],
     a.zero_bot[q] = q < nf ? zero_bot[q] : 0;
     a.zero_top[q] = q < nf ? zero_top[q] : 0;
     a.lid_top[q] = q < nf && lid_top ? lid_top[q] : 0;
+    a.inflow[q] = q < nf && inflow ? inflow[q] : 0.0;
   }
   a.tagdir = tagdir;
   const uint32_t n = (uint32_t)g.NV * g.NL;
@@ -301,7 +304,6 @@ struct MomArgs {
 };
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
-  const double bd[3] = {g.bdx, g.bdy, g.bdz};
   FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
@@ -335,7 +337,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
       const double dB = (phi[s - 1] - pi) / g.dsig[k - 1];
       const double src = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv +
                          (dT + dB) * area;
-      double r = a.rhs_in[q] ? a.rhs_in[q][s] : ((q == 0 && a.force) ? a.force[s] : bd[q]);
+      double r = a.rhs_in[q] ? a.rhs_in[q][s]
+                             : ((q == 0 && a.force) ? a.force[s] : (q == 0 ? g.bdx : (q == 1 ? g.bdy : g.bdz)));
       r = r + (a.gam[q] ? a.gam[q][s] : Gam) * src / Vol;
       double fc[5];
       faces_of(g, i, k, s, phi, phiv, fc);
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_ghost_layers(Geo g, double *f, in
 // and BCpcenter (kind 0, BC.F90:945-964) for wall ghosts: every stored layer
 // of the ghost column takes its cell's value, copied (flag 1) or negated
 // (flag 0: 0 - phi for the velocities, -phi for p, as the reference writes
-// them).  Periodic ghosts need no fill: the neighbour index points at the
+// them), or for an inflow side (flag 2) 2 inflow() - phi.  Periodic ghosts need no fill: the neighbour index points at the
 // partner.  Run after the sigma ghost layers, whose values it copies too.
 __global__ __launch_bounds__(NSX_BLOCK) void k_wall_ghosts(Geo g, double *f, int kind) {
   const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
@@ -808,6 +811,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_wall_ghosts(Geo g, double *f, int
   double y;
   if ((g.gw_neu[w] >> kind) & 1)
     y = x;
+  else if ((g.gw_inf[w] >> kind) & 1)
+    y = 2.0 * (kind == 2 ? 1.0 : 0.0) - x;  // inflow() of Functions.F90:109-121: 1 for v, 0 for u, w
   else
     y = kind == 0 ? -x : 0.0 - x;
   f[(size_t)(g.N0 + w) * g.NS + k] = y;

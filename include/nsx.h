@@ -114,9 +114,10 @@ int nsx_set_vertex_weights(nsx_ctx *ctx, const int *surrounding, const double *w
                            const double *dlVsum, const int *vertex_pair, const int *vertex_edge);
 /* boundary flags of update_bc.F90 (bc?c (N_CELL), bc?v (N_VERT)), bctop/bcbot
  * and TopBC/BotBC (common.mpf).  A horizontal ghost cell is periodic (flag 3
- * for u, v, w and p) or a wall (u, v, w 0 no slip or 1 copy, p 1, the flags
- * update_bc assigns to walled sides, BC.F90:540-581, 945-964); inflow (2)
- * and parallel (-1) ghosts are refused at nsx_commit.  Sigma walls: TopBC 0
+ * for u, v, w and p) or a wall (u, v, w 0 no slip, 1 copy or 2 inflow with
+ * the reference's constant inflow() of Functions.F90:109-121, p 1: the flags
+ * update_bc assigns to walled sides, BC.F90:540-581, 945-964); parallel (-1)
+ * ghosts are refused at nsx_commit.  Sigma walls: TopBC 0
  * no slip, 1 free slip, 2 the lid (u = 1 at the top, BC.F90:512-521,
  * 771-777); BotBC 0 or 1. */
 int nsx_set_bc(nsx_ctx *ctx, const int *bcuc, const int *bcvc, const int *bcwc, const int *bcpc, const int *bcuv,

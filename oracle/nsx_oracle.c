@@ -19,10 +19,10 @@
  * vertex values that interpolateU computes but never reads
  * (interpolationU.F90:99-102; its ghost fill is kept).
  *
- * Supported boundary set: horizontal ghosts periodic (flag 3), walls (0) and
- * free slip (1), as update_bc.F90 assigns them for XPB/YPB and XBC/YBC;
- * sigma walls TopBC in {0,1,2} (2: the lid, u = 1 at the top) and BotBC in
- * {0,1}.  Inflow ghosts (flag 2) are not supported.
+ * Supported boundary set: horizontal ghosts periodic (flag 3), walls (0),
+ * free slip (1) and inflow (2, the constant inflow() of Functions.F90), as
+ * update_bc.F90 assigns them for XPB/YPB and XBC/YBC; sigma walls TopBC in
+ * {0,1,2} (2: the lid, u = 1 at the top) and BotBC in {0,1}.
  */
 #include <math.h>
 #include <stdint.h>
@@ -97,7 +97,11 @@ static double *dalloc(size_t n) {
 }
 
 /* ------------------------------------------------------------ BC cells --- */
-/* BCvelcenter3D, BC.F90:367-594 (ZPB = 0; flag 2 inflow not supported) */
+/* inflow() of Functions.F90:109-121: 1 for tagBC 2 (v), 0 otherwise, at any
+ * point (the reference's comment says u; its test is flag 2) */
+static double inflow_value(int tag) { return tag == TAG_V ? 1.0 : 0.0; }
+
+/* BCvelcenter3D, BC.F90:367-594 (ZPB = 0) */
 void orc_bcvelcenter(orc *o, double *phi, int tag) {
   const int N0 = o->N0, NZ = o->NZ;
   if (o->bcbot == 1) {
@@ -123,6 +127,7 @@ void orc_bcvelcenter(orc *o, double *phi, int tag) {
         double fB = 0.;
         if (flag == 0) fB = 0. - phi[CI(i, k)];
         if (flag == 1) fB = phi[CI(i, k)];
+        if (flag == 2) fB = 2.0 * inflow_value(tag) - phi[CI(i, k)];
         if (flag == 3) fB = phi[CI(o->No_cp[(size_t)nc + 2 * (size_t)NCL] - 1, k)];
         if (flag != -1) phi[CI(nc, k)] = fB;
       }
@@ -221,9 +226,12 @@ void orc_bc_periodic(orc *o, double *phiv, int tag) {
 void orc_bcvelvertex(orc *o, double *phiv, int tag, int tagdir) {
   const int NV = o->NV, NZ = o->NZ;
   const int *bc = tag == TAG_U ? o->bcuv : (tag == TAG_V ? o->bcvv : o->bcwv);
-  for (int nv = 0; nv < NV; ++nv)
+  for (int nv = 0; nv < NV; ++nv) {
     if (o->nbev[nv] != 0 && bc[nv] == 0)
       for (int k = 0; k < NZ - 1; ++k) phiv[VI(nv, k)] = 0.;
+    if (o->nbev[nv] != 0 && bc[nv] == 2)
+      for (int k = 0; k < NZ - 1; ++k) phiv[VI(nv, k)] = inflow_value(tag);
+  }
   if (tagdir == 3) {
     if (o->bcbot == 1 && (o->BotBC == 0 || (o->BotBC == 1 && tag == TAG_W)))
       for (int nv = 0; nv < NV; ++nv) phiv[VI(nv, 0)] = 0.;

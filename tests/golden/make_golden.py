@@ -27,7 +27,9 @@ what they produce:
                   SHA-256 of the set-up arrays, of each routine's output and
                   of the state after each step, and the solves' iteration
                   counts.  The sets: a duct (walls in y, no slip), free-slip
-                  walls in x, a walled box, and the lid (TopBC = 2).
+                  walls in x, a walled box, the lid (TopBC = 2), inflow
+                  and outflow in x (XBC = 2) and free-slip walls in y
+                  (YBC = 1, whose u ghosts the reference flags as inflow).
 
 Hashes are taken over the interior-and-ghost rows (1..N_CELLexact) after
 adding +0.0 (so -0.0 and +0.0 hash alike).  Run in the build container:
@@ -167,7 +169,7 @@ def mesh_setup():
 
 # (name, (XPB, YPB, XBC, YBC), TopBC) of the boundary-condition fixtures
 BC_SETS = [("duct", (1, 0, 0, 0), 0), ("slip_x", (0, 1, 1, 1), 0), ("box", (0, 0, 0, 0), 0),
-           ("lid", (1, 1, 0, 1), 2)]
+           ("lid", (1, 1, 0, 1), 2), ("inflow_x", (0, 1, 2, 1), 0), ("slip_y", (1, 0, 0, 1), 0)]
 BC_STEPS = dict(nx=12, ny=8, nn=8, seed=11, jitter=0.1, nsteps=3)
 
 

@@ -1,17 +1,21 @@
 """Boundary conditions other than the channel's: walled sides and the lid.
 
 tests/golden/bc_cases.json holds what the compiled reference produced for
-four boundary sets (tests/golden/make_golden.py, reference builds with XPB,
+six boundary sets (tests/golden/make_golden.py, reference builds with XPB,
 YPB, XBC, YBC and TopBC set in common.mpf):
   duct    periodic in x, no-slip walls in y (ghost flags u,v,w 0, p 1);
   slip_x  free-slip walls in x, periodic in y (u 0; v, w, p 1);
   box     no-slip walls on all four sides;
   lid     the channel with TopBC = 2 (u ghost 2 - u at the top, u = 1 at
-          the top level of the interior vertices).
+          the top level of the interior vertices);
+  inflow_x  inflow and outflow in x (XBC = 2): on the inflow side the ghosts
+          are 2 inflow() - phi and the vertices inflow(), whose value is 1
+          for v and 0 for u and w (Functions.F90:109-121); Neumann outflow;
+  slip_y  free-slip walls in y (YBC = 1), where update_bc flags u inflow.
 The host set-up, every hot-path routine of the oracle on the 6x4 x 4 mesh
 and three RK2 steps on a 12x8 x 8 mesh are compared hash for hash.  Closed
-forms pin the ghost rules themselves (a ghost is -u or u of its cell, the
-lid's is 2 - u)."""
+forms pin the ghost rules themselves (a ghost is -u or u of its cell, an
+inflow ghost of v is 2 - v, the lid's top ghost of u is 2 - u)."""
 import numpy as np
 import pytest
 
@@ -70,9 +74,10 @@ def test_bc_oracle_trajectory_bitwise(name):
 
 def test_ghost_rules_closed_form():
     """The ghost rules on a field that is 1 + the cell index: a wall ghost of
-    u is -u of its cell (no slip), a free-slip ghost of v is v, a periodic
-    ghost is its partner's value, and the lid's top ghost of u is 2 - u."""
-    for name, checks in (("duct", "wall"), ("slip_x", "slip"), ("lid", "lid")):
+    u is -u of its cell (no slip), a free-slip ghost of v is v, an inflow
+    ghost of v is 2 - v and of u is -u, a periodic ghost is its partner's
+    value, and the lid's top ghost of u is 2 - u."""
+    for name, checks in (("duct", "wall"), ("slip_x", "slip"), ("lid", "lid"), ("inflow_x", "inflow")):
         c, _ = bc_case(REC, name, "tiny")
         O = oracle.Oracle(c)
         N0, NZ = c.N_CELL0, c.NZ
@@ -95,8 +100,13 @@ def test_ghost_rules_closed_form():
                     assert np.array_equal(u[g, 1:-1], -owner)
                 if c.bcvc[g] == 1:
                     assert np.array_equal(v[g, 1:-1], v[i, 1:-1])
+                if c.bcvc[g] == 2:
+                    assert np.array_equal(v[g, 1:-1], 2.0 - v[i, 1:-1])
+                    assert np.array_equal(u[g, 1:-1], -owner) and c.bcuc[g] == 2
         if checks == "lid":
             assert np.array_equal(u[:N0, NZ - 1], 2.0 - u[:N0, NZ - 2])
             assert np.array_equal(v[:N0, NZ - 1], -v[:N0, NZ - 2])
         if checks == "slip":
             assert (c.bcvc[N0:c.N_CELLexact] == 1).sum() > 0
+        if checks == "inflow":
+            assert (c.bcvc[N0:c.N_CELLexact] == 2).sum() > 0

@@ -1,7 +1,8 @@
 """The HIP library on boundary sets other than the channel's, against the
 oracle (which equals the compiled reference on them hash for hash,
 tests/test_bc.py): walled sides (wall ghost columns filled by the rules of
-BCvelcenter3D and BCpcenter) and the lid (TopBC = 2).
+BCvelcenter3D and BCpcenter: no slip, free slip, inflow) and the lid
+(TopBC = 2).
 
   * three RK2 steps on the 12x8 quads x 8 layers mesh of each set, FP64:
     velocities within 1e-8, pressure within 1e-6 (the solver's summation
@@ -106,11 +107,12 @@ def test_bc_step_adjoint_is_transpose_of_tangent(name):
     assert abs(lhs - rhs) <= 1e-8 * max(abs(lhs), abs(rhs)), (lhs, rhs)
 
 
-def test_lid_refused_by_the_differentiation_form():
-    c = bc_case(REC, "lid", "steps")
+@pytest.mark.parametrize("name,why", [("lid", "lid"), ("inflow_x", "inflow")])
+def test_refused_by_the_differentiation_form(name, why):
+    c = bc_case(REC, name, "steps")
     G = _ctx(c, 1e-12)
     G.init_state()
-    with pytest.raises(RuntimeError, match="lid"):
+    with pytest.raises(RuntimeError, match=why):
         G.wall_shear(2, 1)
     G.close()
 

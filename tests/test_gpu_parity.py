@@ -34,8 +34,10 @@ def _rows(case, a):
 
 
 # the tiny mesh, workload 0, and the tiny mesh with the boundary sets of the
-# reference's walled builds (a duct, free-slip x walls, a walled box, the lid)
-@pytest.fixture(scope="module", params=["tiny", "cfg0", "bc_duct", "bc_slip_x", "bc_box", "bc_lid"])
+# reference's walled builds (a duct, free-slip x walls, a walled box, the lid,
+# inflow and outflow in x, free-slip y walls)
+@pytest.fixture(scope="module", params=["tiny", "cfg0", "bc_duct", "bc_slip_x", "bc_box", "bc_lid", "bc_inflow_x",
+                                        "bc_slip_y"])
 def pair(request):
     if request.param == "tiny":
         c, p0 = tiny_case()
