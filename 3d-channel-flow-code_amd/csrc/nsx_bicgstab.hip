@@ -106,25 +106,6 @@ struct Prec<float> {
   __device__ static const float *dsigv(const Geo &g) { return g.dsigv_f; }
 };
 
-__device__ inline void ld4(const double *p, double (&o)[4]) {
-  const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
-  o[0] = a.x;
-  o[1] = a.y;
-  o[2] = b.x;
-  o[3] = b.y;
-}
-__device__ inline void ld4(const float *p, float (&o)[4]) {
-  const float4 a = *(const float4 *)p;
-  o[0] = a.x;
-  o[1] = a.y;
-  o[2] = a.z;
-  o[3] = a.w;
-}
-__device__ inline void st4(double *p, const double (&o)[4]) {
-  *(double2 *)p = make_double2(o[0], o[1]);
-  *(double2 *)(p + 2) = make_double2(o[2], o[3]);
-}
-__device__ inline void st4(float *p, const float (&o)[4]) { *(float4 *)p = make_float4(o[0], o[1], o[2], o[3]); }
 
 // sigma spacings of the block in LDS: lv[0..NS) = dsig, lv[NS..2NS) = dsigv
 // (padding slots hold 1 so that no masked-out row divides by zero)
@@ -462,19 +443,37 @@ __device__ inline float apply_row<float>(const Geo &g, const Row1<float> &R, int
 // At any time one XCD works on one band of G/8 adjacent windows, so the
 // y-neighbour column of a quad (one mesh row, a few hundred columns away)
 // usually lies in the band its own L2 is serving.
-__device__ inline void window_sweep(uint32_t nwin, uint32_t &w, uint32_t &wend, uint32_t &step) {
+//
+// With rev the blocks sweep their slices from the far end: block j takes the
+// slice's windows from its last one down.  The single-reduction CG alternates
+// the direction launch by launch, so a launch first reads the windows the
+// launch before it wrote last, while they are still in the chip's last-level
+// cache.
+struct Sweep {
+  uint32_t w, end, step, lo, hi;
+  bool rev;
+  // the window that sweep position wi (w <= wi < end) stands for
+  __device__ uint32_t at(uint32_t wi) const { return rev ? lo + hi - 1 - wi : wi; }
+};
+
+__device__ inline Sweep window_sweep(uint32_t nwin, bool rev = false) {
   const uint32_t G = gridDim.x, b = blockIdx.x;
+  Sweep S;
+  S.rev = rev;
   if ((G % NSX_NXCD) == 0) {
     const uint32_t per = (nwin + NSX_NXCD - 1) / NSX_NXCD, x = b % NSX_NXCD;
     const uint32_t s0 = x * per < nwin ? x * per : nwin;
-    wend = s0 + per < nwin ? s0 + per : nwin;
-    w = s0 + b / NSX_NXCD;
-    step = G / NSX_NXCD;
+    S.lo = s0;
+    S.hi = S.end = s0 + per < nwin ? s0 + per : nwin;
+    S.w = s0 + b / NSX_NXCD;
+    S.step = G / NSX_NXCD;
   } else {
-    w = b;
-    wend = nwin;
-    step = G;
+    S.lo = 0;
+    S.w = b;
+    S.hi = S.end = nwin;
+    S.step = G;
   }
+  return S;
 }
 
 // Seven-point values of a vector f formed from stored vectors around every
@@ -492,16 +491,16 @@ __device__ inline void window_sweep(uint32_t nwin, uint32_t &w, uint32_t &wend, 
 // the bodies of a window (it may synchronise the block).
 template <typename T, class Own4, class Val4, class Val1, class Pre, class Body, class Post>
 __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&own4, Val4 &&val4, Val1 &&val1,
-                                              Pre &&pre, Body &&body, Post &&post) {
+                                              Pre &&pre, Body &&body, Post &&post, bool rev = false) {
   __shared__ __align__(16) T win[NSX_BLOCK][4];
   const uint32_t nwin = (g.nquad + W - 1) / W;
-  uint32_t w0, wend, step;
-  window_sweep(nwin, w0, wend, step);
-  // the neighbour columns of this thread's quad in window wi, loaded one
-  // window ahead: the gathers of a window would otherwise wait on them
+  const Sweep S = window_sweep(nwin, rev);
+  const uint32_t w0 = S.w, wend = S.end, step = S.step;
+  // the neighbour columns of this thread's quad at sweep position wi, loaded
+  // one window ahead: the gathers of a window would otherwise wait on them
   int nc[3] = {0, 0, 0};
   auto nbr_of = [&](uint32_t wi) {
-    const uint32_t t = wi * W + threadIdx.x;
+    const uint32_t t = S.at(wi) * W + threadIdx.x;
     if (wi < wend && threadIdx.x < W && t < g.nquad) {
       const int e = 3 * (int)fdiv(t, g.divNQ);
       nc[0] = g.nbr[e];
@@ -511,7 +510,7 @@ __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&o
   };
   nbr_of(w0);
   for (uint32_t wi = w0; wi < wend; wi += step) {
-    const uint32_t base = wi * W;
+    const uint32_t base = S.at(wi) * W;
     const uint32_t end = base + W < g.nquad ? base + W : g.nquad;
     const uint32_t t = base + threadIdx.x;
     const bool act = t < end;
@@ -561,17 +560,17 @@ __device__ inline void for_each_quad_formed_w(const Geo &g, uint32_t W, Own4 &&o
 
 template <typename T, class Own4, class Val4, class Val1, class Pre, class Body>
 __device__ inline void for_each_quad_formed(const Geo &g, Own4 &&own4, Val4 &&val4, Val1 &&val1, Pre &&pre,
-                                            Body &&body) {
-  for_each_quad_formed_w<T>(g, NSX_BLOCK, own4, val4, val1, pre, body, [](bool) {});
+                                            Body &&body, bool rev = false) {
+  for_each_quad_formed_w<T>(g, NSX_BLOCK, own4, val4, val1, pre, body, [](bool) {}, rev);
 }
 
 // the seven-point values of f = a + c b (f = a when first)
 template <typename T, class Pre, class Body>
 __device__ inline void for_each_quad_gathered(const Geo &g, const T *a, const T *b, T c, bool first, Pre &&pre,
-                                              Body &&body) {
+                                              Body &&body, bool rev = false) {
   auto v4 = [&](size_t off, T(&o)[4]) { axpy4(a, b, c, first, off, o); };
   auto v1 = [&](size_t off) { return axpy1(a, b, c, first, off); };
-  for_each_quad_formed<T>(g, v4, v4, v1, pre, body);
+  for_each_quad_formed<T>(g, v4, v4, v1, pre, body, rev);
 }
 
 // ------------------------------------------------------------ phase A ----
@@ -1279,7 +1278,7 @@ __device__ inline void line_solve(const Line4 &L, const float (&f)[4], float (&y
 // over windows of whole columns; with acc, the partial sums <M f, f> and
 // <A f, f> over the interior
 __device__ inline void line_product(const Geo &g, const float *lv, const float *a, const float *b, float cc,
-                                    bool first, float *out, double *acc) {
+                                    bool first, float *out, double *acc, bool rev) {
   const uint32_t W = line_window(g);
   const int lpos = line_pos(g, W);
   Line4 L = line4_identity();
@@ -1327,7 +1326,8 @@ __device__ inline void line_product(const Geo &g, const float *lv, const float *
             if (k0s + u < 1 || k0s + u > g.NZ - 2) wo[u] = 0.f;
           st4(out + s0s, wo);
         }
-      });
+      },
+      rev);
 }
 
 // PC: the preconditioner, 0 point Jacobi, 1 the 4-layer blocks, 2 the
@@ -1474,6 +1474,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
     });
   }
   if (stop) return;
+  // odd launches sweep the windows from the far end (window_sweep)
+  const bool rev = (j & 1) != 0;
   T *lvw = load_levels<T>(g);
   for (int k = threadIdx.x; k < g.NS; k += NSX_BLOCK) lvw[2 * g.NS + k] = T(1) / lvw[k];
   __syncthreads();
@@ -1499,10 +1501,9 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
       __shared__ __align__(16) T rs[NSX_BLOCK][4];
       const uint32_t W = LINE ? line_window(g) : NSX_BLOCK, nwin = (g.nquad + W - 1) / W;
       const int lpos = LINE ? line_pos(g, W) : -1;
-      uint32_t w0, wend, step;
-      window_sweep(nwin, w0, wend, step);
-      for (uint32_t wi = w0; wi < wend; wi += step) {
-        const uint32_t base = wi * W;
+      const Sweep S = window_sweep(nwin, rev);
+      for (uint32_t wi = S.w; wi < S.end; wi += S.step) {
+        const uint32_t base = S.at(wi) * W;
         const uint32_t lim = (base + W < g.nquad ? base + W : g.nquad) - base;
         T *flat = &rs[0][0];
         for (uint32_t m = threadIdx.x; m < 4 * W; m += NSX_BLOCK) {
@@ -1571,10 +1572,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
     T *om = v.om[in];
     double acc[2] = {0.0, 0.0};
     if constexpr (LINE) {
-      line_product(g, lv, rt, rt, T(0), true, om, acc);
+      line_product(g, lv, rt, rt, T(0), true, om, acc, rev);
       if (fold && c.iter > 0)
         line_product(g, lv, pend_in ? rt_lag : (const T *)v.p, (const T *)v.p, pend_in ? bp_in : T(0),
-                     pend_in ? pfirst_in : true, v.sg[in], (double *)nullptr);
+                     pend_in ? pfirst_in : true, v.sg[in], (double *)nullptr, rev);
     } else {
       for_each_quad_gathered(
           g, rt, rt, T(0), true, [&](size_t) {},
@@ -1612,7 +1613,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
               }
             }
             st4(om + s0, wo);
-          });
+          },
+          rev);
       if (fold && c.iter > 0) {
         // after a true residual, s = A p afresh: its recurrence drifts in FP32
         // from the product it stands for, and with it the recursive residual
@@ -1643,7 +1645,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
                 }
               }
               st4(sg + s0, so);
-            });
+            },
+            rev);
       }
     }
     write_partials<2>(acc, Pout);
@@ -1780,7 +1783,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_cgs(Geo g, int j, CgsVecsT<T> v, 
             st4(om_o + s0n, wo);
           }
         }
-      });
+      },
+      rev);
   write_partials<4>(acc, Pout);
 }
 

@@ -97,6 +97,28 @@ struct Geo {
   double dt, nu, bdx, bdy, bdz;
 };
 
+// the four layers at p (16-byte aligned: a run k0..k0+3 with k0 = 4m) as
+// 16-byte accesses
+__device__ inline void ld4(const double *p, double (&o)[4]) {
+  const double2 a = *(const double2 *)p, b = *(const double2 *)(p + 2);
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = b.x;
+  o[3] = b.y;
+}
+__device__ inline void ld4(const float *p, float (&o)[4]) {
+  const float4 a = *(const float4 *)p;
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = a.z;
+  o[3] = a.w;
+}
+__device__ inline void st4(double *p, const double (&o)[4]) {
+  *(double2 *)p = make_double2(o[0], o[1]);
+  *(double2 *)(p + 2) = make_double2(o[2], o[3]);
+}
+__device__ inline void st4(float *p, const float (&o)[4]) { *(float4 *)p = make_float4(o[0], o[1], o[2], o[3]); }
+
 // velocity ghost layer rule of BCvelcenter3D (BC.F90:459-527) for ZPB = 0:
 // the derivative of the ghost value with respect to the interior value next
 // to it, +1 symmetric copy, -1 antisymmetric copy (and the lid's 2 - u), 0

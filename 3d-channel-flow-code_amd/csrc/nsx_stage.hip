@@ -52,6 +52,116 @@ __device__ inline void band_range(uint32_t n, uint32_t &beg, uint32_t &end, uint
   for (uint32_t base_ = beg_; base_ < end_; base_ += step_)                         \
     for (uint32_t c = base_ + threadIdx.x, once_ = 1; once_ && c < end_; once_ = 0)
 
+// Runs of NSX_RUN layers: k_les and k_momentum take the layers k0 ..
+// k0+NSX_RUN-1 (k0 a multiple of NSX_RUN) of one column per thread.  The
+// column data (neighbours, vertices, edge lengths) is loaded once for the
+// run, each run of a cell or face field is read with 16-byte accesses, and a
+// wave's accesses to a field span 64 NSX_RUN consecutive slots.  Every
+// prism's arithmetic is the one-prism form's, in its order.  Slots outside
+// the interior layers 1..NZ-2 (the ghost layers and the padding, in a
+// column's first and last runs) are computed on whatever they hold and never
+// stored.
+#ifndef NSX_RUN
+#define NSX_RUN 2
+#endif
+constexpr int RL = NSX_RUN;
+static_assert(RL == 1 || RL == 2 || RL == 4, "a run divides the 4-slot column padding");
+
+struct Run {
+  int i, k0;
+  size_t s0;
+  bool act[RL], full;
+};
+
+// runs in the cell fields: N0 columns of NS / RL runs
+__host__ __device__ inline uint32_t nruns(const Geo &g) { return (uint32_t)g.N0 * (uint32_t)(g.NS / RL); }
+
+__device__ inline Run run_at(const Geo &g, uint32_t t) {
+  Run R;
+  // t / (NS/RL) = (t / (4/RL)) / NQ, NQ = NS/4
+  const uint32_t i = fdiv(t / (uint32_t)(4 / RL), g.divNQ);
+  R.i = (int)i;
+  R.k0 = RL * (int)(t - i * (uint32_t)(g.NS / RL));
+  R.s0 = (size_t)i * g.NS + R.k0;
+  R.full = true;
+#pragma unroll
+  for (int u = 0; u < RL; ++u) {
+    const int k = R.k0 + u;
+    R.act[u] = k >= 1 && k <= g.NZ - 2;
+    R.full = R.full && R.act[u];
+  }
+  return R;
+}
+
+__device__ inline void ldr(const double *p, double (&o)[RL]) {
+  if constexpr (RL == 1) {
+    o[0] = *p;
+  } else {
+#pragma unroll
+    for (int h = 0; h < RL / 2; ++h) {
+      const double2 a = ((const double2 *)p)[h];
+      o[2 * h] = a.x;
+      o[2 * h + 1] = a.y;
+    }
+  }
+}
+
+// the run's active slots of o into p: 16-byte stores when every slot is active
+__device__ inline void str(double *p, const double (&o)[RL], const Run &R) {
+  if (RL > 1 && R.full) {
+#pragma unroll
+    for (int h = 0; h < RL / 2; ++h) ((double2 *)p)[h] = make_double2(o[2 * h], o[2 * h + 1]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < RL; ++u)
+      if (R.act[u]) p[u] = o[u];
+  }
+}
+
+#define FOR_EACH_RUN_BAND(g, t)                                                     \
+  uint32_t beg_, end_, step_;                                                       \
+  band_range(nruns(g), beg_, end_, step_);                                          \
+  for (uint32_t base_ = beg_; base_ < end_; base_ += step_)                         \
+    for (uint32_t t = base_ + threadIdx.x, once_ = 1; once_ && t < end_; once_ = 0)
+
+// A cell field around a run: c[u+1] = f at layer k0+u of the column, c[0]
+// and c[RL+1] the layers below and above (loaded only where an active layer
+// needs them), n[j][u] = f at layer k0+u of neighbour column j, v[j][u] =
+// the vertex field at level k0+u of the triangle's j-th vertex (active
+// layers only: the levels run 0..NZ-2).
+struct FieldRun {
+  double c[RL + 2], n[3][RL], v[3][RL];
+};
+
+__device__ inline void field_run(const Geo &g, const Run &R, const int (&nb)[3], const int (&vx)[3], const double *f,
+                                 const double *fv, FieldRun &F) {
+  double m[RL];
+  ldr(f + R.s0, m);
+#pragma unroll
+  for (int u = 0; u < RL; ++u) F.c[u + 1] = m[u];
+  F.c[0] = R.act[0] ? f[R.s0 - 1] : 0.0;
+  F.c[RL + 1] = R.act[RL - 1] ? f[R.s0 + RL] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ldr(f + (size_t)nb[j] * g.NS + R.k0, F.n[j]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int u = 0; u < RL; ++u) F.v[j][u] = R.act[u] ? fv[(size_t)vx[j] * g.NL + R.k0 + u] : 0.0;
+}
+
+// faces_of (below) at layer k0+u from a FieldRun: the same expressions
+__device__ inline void faces_run(const FieldRun &F, int u, const double (&doe)[3], const double (&lvv)[3],
+                                 double (&out)[5]) {
+  const double fi = F.c[u + 1];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double extra = doe[j] * (F.v[(j + 1) % 3][u] - F.v[j][u]) / lvv[j];
+    out[j] = F.n[j][u] * 0.5 + fi * 0.5 + extra;
+  }
+  out[3] = 0.5 * (fi + F.c[u + 2]);
+  out[4] = 0.5 * (fi + F.c[u]);
+}
+
 // ------------------------------------------------------- cell -> vertex ----
 // interpolation2D / interpolation3D(P) + BCvelvertex3D / BCpvertex +
 // bc_periodic (interpolation2D.F90:8-150, interpolation3D.F90:8-444,
@@ -237,54 +347,83 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const
                                                     const double *uv, const double *vv, const double *wv,
                                                     double *eddy) {
   const double Cs = 0.1, karman = (double)0.415f;
-  FOR_EACH_CELL_BAND(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    double U[5], V[5], W[5];
-    faces_of(g, i, k, s, u, uv, U);
-    faces_of(g, i, k, s, v, vv, V);
-    faces_of(g, i, k, s, w, wv, W);
-    const int e = i * 3;
-    const double x1 = g.xnn[e], x2 = g.xnn[e + 1], x3 = g.xnn[e + 2];
-    const double y1 = g.ynn[e], y2 = g.ynn[e + 1], y3 = g.ynn[e + 2];
-    const double l1 = g.dlVV[e], l2 = g.dlVV[e + 1], l3 = g.dlVV[e + 2];
-    const double a = g.area[i];
-    double dudx = U[0] * x1 * l1 + U[1] * x2 * l2 + U[2] * x3 * l3;
-    double dvdx = V[0] * x1 * l1 + V[1] * x2 * l3 + V[2] * x3 * l2;
-    double dwdx = W[0] * x1 * l1 + W[1] * x2 * l2 + W[2] * x3 * l3;
-    double dudy = U[0] * y1 * l1 + U[1] * y2 * l2 + U[2] * y3 * l3;
-    double dvdy = V[0] * y1 * l1 + V[1] * y2 * l3 + V[2] * y3 * l2;
-    double dwdy = W[0] * y1 * l1 + W[1] * y2 * l2 + W[2] * y3 * l3;
-    dudx = dudx / a;
-    dvdx = dvdx / a;
-    dwdx = dwdx / a;
-    dudy = dudy / a;
-    dvdy = dvdy / a;
-    dwdy = dwdy / a;
-    const double dz = g.dsig[k];
-    const double dudz = (U[3] - U[4]) / dz, dvdz = (V[3] - V[4]) / dz, dwdz = (W[3] - W[4]) / dz;
-    double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
-    const double lwall = karman * g.walld[k];
-    const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
-    const double dn2 = 1.0 / dn1;
-    const double del = sqrt(dn2);
-    DD = del * del;
-    const double nut = 1.0 * DD;
-    const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
-    const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
-    const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
-    const double ed = nut * Sij;
-    eddy[s] = ed;
+  FOR_EACH_RUN_BAND(g, t) {
+    const Run R = run_at(g, t);
+    const int e = R.i * 3;
+    int nb[3], vx[3];
+    double doe[3], lvv[3], xn[3], yn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nb[j] = g.nbr[e + j];
+      vx[j] = g.vtx[e + j];
+      doe[j] = g.doe[e + j];
+      lvv[j] = g.dlVV[e + j];
+      xn[j] = g.xnn[e + j];
+      yn[j] = g.ynn[e + j];
+    }
+    const double x1 = xn[0], x2 = xn[1], x3 = xn[2];
+    const double y1 = yn[0], y2 = yn[1], y3 = yn[2];
+    const double l1 = lvv[0], l2 = lvv[1], l3 = lvv[2];
+    const double a = g.area[R.i];
+    // the x, y, z derivatives of u, v, w at each layer of the run, one field
+    // at a time (dvdx and dvdy take the swapped edge lengths)
+    double dx[3][RL], dy[3][RL], dz_[3][RL];
+    const double *fs[3] = {u, v, w}, *fvs[3] = {uv, vv, wv};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      FieldRun F;
+      field_run(g, R, nb, vx, fs[q], fvs[q], F);
+#pragma unroll
+      for (int m = 0; m < RL; ++m) {
+        double P[5];
+        faces_run(F, m, doe, lvv, P);
+        double ddx, ddy;
+        if (q == 1) {
+          ddx = P[0] * x1 * l1 + P[1] * x2 * l3 + P[2] * x3 * l2;
+          ddy = P[0] * y1 * l1 + P[1] * y2 * l3 + P[2] * y3 * l2;
+        } else {
+          ddx = P[0] * x1 * l1 + P[1] * x2 * l2 + P[2] * x3 * l3;
+          ddy = P[0] * y1 * l1 + P[1] * y2 * l2 + P[2] * y3 * l3;
+        }
+        dx[q][m] = ddx / a;
+        dy[q][m] = ddy / a;
+        dz_[q][m] = (P[3] - P[4]) / (R.act[m] ? g.dsig[R.k0 + m] : 1.0);
+      }
+    }
+    double ed[RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      const double dudx = dx[0][m], dvdx = dx[1][m], dwdx = dx[2][m];
+      const double dudy = dy[0][m], dvdy = dy[1][m], dwdy = dy[2][m];
+      const double dudz = dz_[0][m], dvdz = dz_[1][m], dwdz = dz_[2][m];
+      const double dz = R.act[m] ? g.dsig[k] : 1.0;
+      double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
+      const double lwall = karman * (R.act[m] ? g.walld[k] : 1.0);
+      const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
+      const double dn2 = 1.0 / dn1;
+      const double del = sqrt(dn2);
+      DD = del * del;
+      const double nut = 1.0 * DD;
+      const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
+      const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
+      const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
+      ed[m] = nut * Sij;
+    }
+    str(eddy + R.s0, ed, R);
     // BCpcenter(eddy): Neumann ghost layers
-    if (k == 1 && g.bcbot == 1) eddy[s - 1] = ed;
-    if (k == g.NZ - 2 && g.bctop == 1) eddy[s + 1] = ed;
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      if (k == 1 && g.bcbot == 1) eddy[R.s0 + m - 1] = ed[m];
+      if (k == g.NZ - 2 && g.bctop == 1) eddy[R.s0 + m + 1] = ed[m];
+    }
   }
 }
 
 int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
                const double *vv, const double *wv, double *eddy) {
-  hipLaunchKernelGGL(k_les, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
+  hipLaunchKernelGGL(k_les, dim3(nsx_grid_for(nruns(g))), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
   return (int)hipGetLastError();
 }
 
@@ -304,54 +443,100 @@ struct MomArgs {
 };
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
-  FOR_EACH_CELL_BAND(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    const int e = i * 3;
-    const double area = g.area[i];
-    const double sv = g.dsigv[k - 1];
-    const double Vol = area * sv;
-    const double Gam = a.eddy[s] + g.nu;
-    const double UF1 = a.face[0][s] * g.dlVV[e] * sv;
-    const double UF2 = a.face[1][s] * g.dlVV[e + 1] * sv;
-    const double UF3 = a.face[2][s] * g.dlVV[e + 2] * sv;
-    const double UFT = a.face[3][s] * area;
-    const double UFB = a.face[4][s] * area;
+  FOR_EACH_RUN_BAND(g, t) {
+    const Run R = run_at(g, t);
+    const int e = R.i * 3;
+    int nb[3], vx[3];
+    double doe[3], lvv[3], lcc[3], le1[3], le2[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nb[j] = g.nbr[e + j];
+      vx[j] = g.vtx[e + j];
+      doe[j] = g.doe[e + j];
+      lvv[j] = g.dlVV[e + j];
+      lcc[j] = g.dlCC[e + j];
+      le1[j] = g.dle1[e + j];
+      le2[j] = g.dle2[e + j];
+    }
+    const double area = g.area[R.i];
+    // per layer: dsigv(k-1), dsig(k), dsig(k-1) (1 in the slots never stored)
+    double sv[RL], dsk[RL], dsb[RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      sv[m] = R.act[m] ? g.dsigv[k - 1] : 1.0;
+      dsk[m] = R.act[m] ? g.dsig[k] : 1.0;
+      dsb[m] = R.act[m] ? g.dsig[k - 1] : 1.0;
+    }
+    double ed[RL], fr[5][RL];
+    ldr(a.eddy + R.s0, ed);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) ldr(a.face[f] + R.s0, fr[f]);
+    double Vol[RL], Gam[RL], UF[5][RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      Vol[m] = area * sv[m];
+      Gam[m] = ed[m] + g.nu;
+      UF[0][m] = fr[0][m] * lvv[0] * sv[m];
+      UF[1][m] = fr[1][m] * lvv[1] * sv[m];
+      UF[2][m] = fr[2][m] * lvv[2] * sv[m];
+      UF[3][m] = fr[3][m] * area;
+      UF[4][m] = fr[4][m] * area;
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const double *phi = a.uvw[q];
-      const double *phiv = a.vtx[q];
-      const double pi = phi[s];
-      double dn[3];
+      FieldRun F;
+      field_run(g, R, nb, vx, a.uvw[q], a.vtx[q], F);
+      double rin[RL], gm[RL], hs[RL];
+      if (a.rhs_in[q]) {
+        ldr(a.rhs_in[q] + R.s0, rin);
+      } else if (q == 0 && a.force) {
+        ldr(a.force + R.s0, rin);
+      } else {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int ej = e + j;
-        const int nb = g.nbr[ej];
-        const int jv1 = g.vtx[ej], jv2 = g.vtx[e + (j + 1) % 3];
-        const double diff1 = (phi[(size_t)nb * g.NS + k] - pi) / g.dlCC[ej];
-        const double diff2 = (phiv[(size_t)jv2 * g.NL + k] - phiv[(size_t)jv1 * g.NL + k]) / g.dlVV[ej];
-        dn[j] = diff1 * g.dle1[ej] + diff2 * g.dle2[ej];
+        for (int m = 0; m < RL; ++m) rin[m] = q == 0 ? g.bdx : (q == 1 ? g.bdy : g.bdz);
       }
-      const double dT = (phi[s + 1] - pi) / g.dsig[k];
-      const double dB = (phi[s - 1] - pi) / g.dsig[k - 1];
-      const double src = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv +
-                         (dT + dB) * area;
-      double r = a.rhs_in[q] ? a.rhs_in[q][s]
-                             : ((q == 0 && a.force) ? a.force[s] : (q == 0 ? g.bdx : (q == 1 ? g.bdy : g.bdz)));
-      r = r + (a.gam[q] ? a.gam[q][s] : Gam) * src / Vol;
-      double fc[5];
-      faces_of(g, i, k, s, phi, phiv, fc);
-      const double flux = fc[0] * UF1 + fc[1] * UF2 + fc[2] * UF3 + fc[3] * UFT + fc[4] * UFB;
-      r = r - flux / Vol;
-      if (a.rhs_out[q]) a.rhs_out[q][s] = r;
+      if (a.gam[q]) {
+        ldr(a.gam[q] + R.s0, gm);
+      } else {
+#pragma unroll
+        for (int m = 0; m < RL; ++m) gm[m] = Gam[m];
+      }
+      if (a.ustar[q]) ldr(a.hist[q] + R.s0, hs);
+      double rr[RL], us[RL];
+#pragma unroll
+      for (int m = 0; m < RL; ++m) {
+        const double pi = F.c[m + 1];
+        double dn[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double diff1 = (F.n[j][m] - pi) / lcc[j];
+          const double diff2 = (F.v[(j + 1) % 3][m] - F.v[j][m]) / lvv[j];
+          dn[j] = diff1 * le1[j] + diff2 * le2[j];
+        }
+        const double dT = (F.c[m + 2] - pi) / dsk[m];
+        const double dB = (F.c[m] - pi) / dsb[m];
+        const double src = dn[0] * lvv[0] * sv[m] + dn[1] * lvv[1] * sv[m] + dn[2] * lvv[2] * sv[m] + (dT + dB) * area;
+        double r = rin[m];
+        r = r + gm[m] * src / Vol[m];
+        double fc[5];
+        faces_run(F, m, doe, lvv, fc);
+        const double flux = fc[0] * UF[0][m] + fc[1] * UF[1][m] + fc[2] * UF[2][m] + fc[3] * UF[3][m] + fc[4] * UF[4][m];
+        r = r - flux / Vol[m];
+        rr[m] = r;
+        if (a.ustar[q]) us[m] = pi + g.dt * (a.c1 * r + a.c2 * hs[m]);
+      }
+      if (a.rhs_out[q]) str(a.rhs_out[q] + R.s0, rr, R);
       if (a.ustar[q]) {
-        const double us = pi + g.dt * (a.c1 * r + a.c2 * a.hist[q][s]);
-        a.hist[q][s] = r;
-        a.ustar[q][s] = us;
-        double gv;
-        if (k == 1 && vel_ghost(g.bcbot, g.BotBC, q, us, gv)) a.ustar[q][s - 1] = gv;
-        if (k == g.NZ - 2 && vel_ghost(g.bctop, g.TopBC, q, us, gv)) a.ustar[q][s + 1] = gv;
+        str(a.hist[q] + R.s0, rr, R);
+        str(a.ustar[q] + R.s0, us, R);
+#pragma unroll
+        for (int m = 0; m < RL; ++m) {
+          const int k = R.k0 + m;
+          double gv;
+          if (k == 1 && vel_ghost(g.bcbot, g.BotBC, q, us[m], gv)) a.ustar[q][R.s0 + m - 1] = gv;
+          if (k == g.NZ - 2 && vel_ghost(g.bctop, g.TopBC, q, us[m], gv)) a.ustar[q][R.s0 + m + 1] = gv;
+        }
       }
     }
   }
@@ -376,7 +561,7 @@ int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], con
   a.eddy = eddy;
   a.c1 = c1;
   a.c2 = c2;
-  hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, a);
+  hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(nruns(g))), dim3(NSX_BLOCK), 0, s, g, a);
   return (int)hipGetLastError();
 }
 
