@@ -2013,7 +2013,11 @@ static int tlm_stage(nsx_ctx *ctx, double *const X[4], double *const dX[4], cons
   if ((rc = vertex_of(ctx, 1, dp, dpv, pk, 2))) return rc;
   const double *dus[3] = {W.Tus[0], W.Tus[1], W.Tus[2]};
   LAUNCH(launch_source(g, s, dus, W.TF, nullptr, W.Trhs, W.Tpv, ctx->part));
-  if ((rc = solve_fp64(ctx, W.Trhs, W.zero, dY[3]))) return rc;
+  // the tangent pressure by the FP64 single-reduction CG, as the adjoint
+  // stage solves with A^T: at eps 1e-13 the reference's BiCGSTAB can stall on
+  // its recursive residual and break down, depending on the last bits of the
+  // right-hand side, where the CG tests its true residual in FP64
+  if ((rc = solve_cgs(ctx, W.Trhs, W.zero, dY[3], true))) return rc;
   BC_WALLS(dY[3], 0);
   double *dun[3] = {dY[0], dY[1], dY[2]};
   LAUNCH(launch_corrector(g, s, dY[3], dus, dun));

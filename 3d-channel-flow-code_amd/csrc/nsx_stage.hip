@@ -28,8 +28,9 @@ int nsx_grid_for(uint32_t n) {
 // mostly in the same or the next batches) are fetched into that XCD's L2
 // while the blocks that own them load them too.  With one contiguous chunk
 // per block, an XCD's blocks were spread over its whole slice and a
-// neighbour column was often evicted between the two loads.  (Kernels that
-// accumulate partial sums keep FOR_EACH_CELL, whose order their sums follow.)
+// neighbour column was often evicted between the two loads.  A kernel that
+// accumulates partial sums (k_source) adds them in the order of its sweep,
+// which the fixed grid makes the same on every run.
 __device__ inline void band_range(uint32_t n, uint32_t &beg, uint32_t &end, uint32_t &step) {
   const uint32_t G = gridDim.x, b = blockIdx.x;
   if ((G % NSX_NXCD) == 0) {
@@ -52,40 +53,47 @@ __device__ inline void band_range(uint32_t n, uint32_t &beg, uint32_t &end, uint
   for (uint32_t base_ = beg_; base_ < end_; base_ += step_)                         \
     for (uint32_t c = base_ + threadIdx.x, once_ = 1; once_ && c < end_; once_ = 0)
 
-// Runs of NSX_RUN layers: k_les and k_momentum take the layers k0 ..
-// k0+NSX_RUN-1 (k0 a multiple of NSX_RUN) of one column per thread.  The
-// column data (neighbours, vertices, edge lengths) is loaded once for the
-// run, each run of a cell or face field is read with 16-byte accesses, and a
-// wave's accesses to a field span 64 NSX_RUN consecutive slots.  Every
-// prism's arithmetic is the one-prism form's, in its order.  Slots outside
-// the interior layers 1..NZ-2 (the ghost layers and the padding, in a
-// column's first and last runs) are computed on whatever they hold and never
-// stored.
+// Runs of layers: the stencil kernels below take the layers k0 .. k0+RN-1
+// (k0 a multiple of RN) of one column per thread.  The column data
+// (neighbours, vertices, edge lengths) is loaded once for the run, each run
+// of a cell or face field is read with 16-byte accesses, and a wave's
+// accesses to a field span 64 RN consecutive slots.  Every prism's
+// arithmetic is the one-prism form's, in its order.  Slots outside the
+// interior layers 1..NZ-2 (the ghost layers and the padding, in a column's
+// first and last runs) are computed on whatever they hold and never stored.
+// RL is the run the kernels use.  k_les keeps one prism per thread: with
+// its FP64 cube root per prism, runs of one or two layers took 205 us at
+// 6.3M prisms against 175 us.
 #ifndef NSX_RUN
 #define NSX_RUN 2
 #endif
 constexpr int RL = NSX_RUN;
 static_assert(RL == 1 || RL == 2 || RL == 4, "a run divides the 4-slot column padding");
 
+template <int RN>
 struct Run {
   int i, k0;
   size_t s0;
-  bool act[RL], full;
+  bool act[RN], full;
 };
 
-// runs in the cell fields: N0 columns of NS / RL runs
-__host__ __device__ inline uint32_t nruns(const Geo &g) { return (uint32_t)g.N0 * (uint32_t)(g.NS / RL); }
+// runs in the cell fields: N0 columns of NS / RN runs
+template <int RN>
+__host__ __device__ inline uint32_t nruns(const Geo &g) {
+  return (uint32_t)g.N0 * (uint32_t)(g.NS / RN);
+}
 
-__device__ inline Run run_at(const Geo &g, uint32_t t) {
-  Run R;
-  // t / (NS/RL) = (t / (4/RL)) / NQ, NQ = NS/4
-  const uint32_t i = fdiv(t / (uint32_t)(4 / RL), g.divNQ);
+template <int RN>
+__device__ inline Run<RN> run_at(const Geo &g, uint32_t t) {
+  Run<RN> R;
+  // t / (NS/RN) = (t / (4/RN)) / NQ, NQ = NS/4
+  const uint32_t i = fdiv(t / (uint32_t)(4 / RN), g.divNQ);
   R.i = (int)i;
-  R.k0 = RL * (int)(t - i * (uint32_t)(g.NS / RL));
+  R.k0 = RN * (int)(t - i * (uint32_t)(g.NS / RN));
   R.s0 = (size_t)i * g.NS + R.k0;
   R.full = true;
 #pragma unroll
-  for (int u = 0; u < RL; ++u) {
+  for (int u = 0; u < RN; ++u) {
     const int k = R.k0 + u;
     R.act[u] = k >= 1 && k <= g.NZ - 2;
     R.full = R.full && R.act[u];
@@ -93,12 +101,13 @@ __device__ inline Run run_at(const Geo &g, uint32_t t) {
   return R;
 }
 
-__device__ inline void ldr(const double *p, double (&o)[RL]) {
-  if constexpr (RL == 1) {
+template <int RN>
+__device__ inline void ldr(const double *p, double (&o)[RN]) {
+  if constexpr (RN == 1) {
     o[0] = *p;
   } else {
 #pragma unroll
-    for (int h = 0; h < RL / 2; ++h) {
+    for (int h = 0; h < RN / 2; ++h) {
       const double2 a = ((const double2 *)p)[h];
       o[2 * h] = a.x;
       o[2 * h + 1] = a.y;
@@ -107,50 +116,70 @@ __device__ inline void ldr(const double *p, double (&o)[RL]) {
 }
 
 // the run's active slots of o into p: 16-byte stores when every slot is active
-__device__ inline void str(double *p, const double (&o)[RL], const Run &R) {
-  if (RL > 1 && R.full) {
+template <int RN>
+__device__ inline void str(double *p, const double (&o)[RN], const Run<RN> &R) {
+  if (RN > 1 && R.full) {
 #pragma unroll
-    for (int h = 0; h < RL / 2; ++h) ((double2 *)p)[h] = make_double2(o[2 * h], o[2 * h + 1]);
+    for (int h = 0; h < RN / 2; ++h) ((double2 *)p)[h] = make_double2(o[2 * h], o[2 * h + 1]);
   } else {
 #pragma unroll
-    for (int u = 0; u < RL; ++u)
+    for (int u = 0; u < RN; ++u)
       if (R.act[u]) p[u] = o[u];
   }
 }
 
-#define FOR_EACH_RUN_BAND(g, t)                                                     \
+#define FOR_EACH_RUN_BAND(g, t, RN)                                                 \
   uint32_t beg_, end_, step_;                                                       \
-  band_range(nruns(g), beg_, end_, step_);                                          \
+  band_range(nruns<RN>(g), beg_, end_, step_);                                      \
   for (uint32_t base_ = beg_; base_ < end_; base_ += step_)                         \
     for (uint32_t t = base_ + threadIdx.x, once_ = 1; once_ && t < end_; once_ = 0)
 
 // A cell field around a run: c[u+1] = f at layer k0+u of the column, c[0]
-// and c[RL+1] the layers below and above (loaded only where an active layer
+// and c[RN+1] the layers below and above (loaded only where an active layer
 // needs them), n[j][u] = f at layer k0+u of neighbour column j, v[j][u] =
 // the vertex field at level k0+u of the triangle's j-th vertex (active
-// layers only: the levels run 0..NZ-2).
+// layers only: the levels run 0..NZ-2; none when fv is null).
+template <int RN>
 struct FieldRun {
-  double c[RL + 2], n[3][RL], v[3][RL];
+  double c[RN + 2], n[3][RN], v[3][RN];
 };
 
-__device__ inline void field_run(const Geo &g, const Run &R, const int (&nb)[3], const int (&vx)[3], const double *f,
-                                 const double *fv, FieldRun &F) {
-  double m[RL];
+template <int RN>
+__device__ inline void field_run(const Geo &g, const Run<RN> &R, const int (&nb)[3], const int (&vx)[3],
+                                 const double *f, const double *fv, FieldRun<RN> &F) {
+  double m[RN];
   ldr(f + R.s0, m);
 #pragma unroll
-  for (int u = 0; u < RL; ++u) F.c[u + 1] = m[u];
+  for (int u = 0; u < RN; ++u) F.c[u + 1] = m[u];
   F.c[0] = R.act[0] ? f[R.s0 - 1] : 0.0;
-  F.c[RL + 1] = R.act[RL - 1] ? f[R.s0 + RL] : 0.0;
+  F.c[RN + 1] = R.act[RN - 1] ? f[R.s0 + RN] : 0.0;
 #pragma unroll
   for (int j = 0; j < 3; ++j) ldr(f + (size_t)nb[j] * g.NS + R.k0, F.n[j]);
+  if (!fv) return;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
-    for (int u = 0; u < RL; ++u) F.v[j][u] = R.act[u] ? fv[(size_t)vx[j] * g.NL + R.k0 + u] : 0.0;
+    for (int u = 0; u < RN; ++u) F.v[j][u] = R.act[u] ? fv[(size_t)vx[j] * g.NL + R.k0 + u] : 0.0;
+}
+
+// the column data of a run's edges that the kernels below use
+struct EdgeRun {
+  int nb[3], vx[3];
+};
+
+__device__ inline EdgeRun edges_of(const Geo &g, int i) {
+  EdgeRun E;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    E.nb[j] = g.nbr[i * 3 + j];
+    E.vx[j] = g.vtx[i * 3 + j];
+  }
+  return E;
 }
 
 // faces_of (below) at layer k0+u from a FieldRun: the same expressions
-__device__ inline void faces_run(const FieldRun &F, int u, const double (&doe)[3], const double (&lvv)[3],
+template <int RN>
+__device__ inline void faces_run(const FieldRun<RN> &F, int u, const double (&doe)[3], const double (&lvv)[3],
                                  double (&out)[5]) {
   const double fi = F.c[u + 1];
 #pragma unroll
@@ -178,101 +207,129 @@ struct VtxArgs {
   int tagdir;
 };
 
-// The inverse-distance sums at layers l (s0) and l + 1 (s1, when two) of
-// vertex v, each added in the stencil's order.  The stencil entries are
-// taken four at a time so that their loads are in flight together.
-template <bool TWO>
-__device__ inline void idw_sums(const Geo &g, int v, int l, const double *f, double &s0, double &s1) {
+// The inverse-distance sums of vertex v at the layers l, l + 1 and l + 2
+// (s[0..NL_-1], NL_ of them), each added in the stencil's order.  The stencil
+// entries are taken four at a time so that their loads are in flight
+// together; a cell's layers l, l + 1 (16-byte aligned: l is even) are one
+// load.  Layers past `nl` are not read (their sums stay 0).
+template <int NL_>
+__device__ inline void idw_sums(const Geo &g, int v, int l, int nl, const double *f, double (&s)[NL_]) {
   const int b = g.vptr[v], e = g.vptr[v + 1];
-  s0 = 0.0;
-  s1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < NL_; ++q) s[q] = 0.0;
+  auto add = [&](int j0, int n) {
+    double a[4][NL_], w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < n) {
+        const double *c = f + (size_t)g.vcell[j0 + q] * g.NS + l;
+        w[q] = g.vw[j0 + q];
+        const double2 d = *(const double2 *)c;
+        a[q][0] = d.x;
+        if (NL_ > 1) a[q][NL_ > 1 ? 1 : 0] = nl > 1 ? d.y : 0.0;
+        if (NL_ > 2) a[q][NL_ > 2 ? 2 : 0] = nl > 2 ? c[2] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < n)
+#pragma unroll
+        for (int m = 0; m < NL_; ++m) s[m] = s[m] + w[q] * a[q][m];
+  };
   int j = b;
-  for (; j + 4 <= e; j += 4) {
-    double a[4], c[4], w[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const size_t o = (size_t)g.vcell[j + q] * g.NS + l;
-      w[q] = g.vw[j + q];
-      a[q] = f[o];
-      if (TWO) c[q] = f[o + 1];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      s0 = s0 + w[q] * a[q];
-      if (TWO) s1 = s1 + w[q] * c[q];
-    }
-  }
-  for (; j < e; ++j) {
-    const size_t o = (size_t)g.vcell[j] * g.NS + l;
-    s0 = s0 + g.vw[j] * f[o];
-    if (TWO) s1 = s1 + g.vw[j] * f[o + 1];
-  }
+  for (; j + 4 <= e; j += 4) add(j, 4);
+  if (j < e) add(j, e - j);
 }
 
-__device__ inline double idw(const Geo &g, int v, int layer, const double *f) {
-  double s, unused;
-  idw_sums<false>(g, v, layer, f, s, unused);
-  return s / g.dlVsum[v];
-}
-
-// Blocks are dealt round-robin to the 8 XCDs; the remap gives each XCD one
-// contiguous range of vertices, so the columns that neighbouring vertices
-// share are fetched into one L2 (grid a multiple of 8).
+// Threads take runs of two vertex levels l0 = 2m, l0 + 1 of one vertex (the
+// second only when l0 + 1 < NL): the stencil is read once for both, and
+// each cell's layers come as one 16-byte load.  Blocks are dealt round-robin
+// to the 8 XCDs; the remap gives each XCD one contiguous range of vertices,
+// so the columns that neighbouring vertices share are fetched into one L2
+// (grid a multiple of 8).
 template <int NF>
 __global__ __launch_bounds__(NSX_BLOCK) void k_vertex(Geo g, VtxArgs a) {
   const uint32_t G = gridDim.x, b = blockIdx.x;
   const uint32_t lb = (G % NSX_NXCD) == 0 ? (b % NSX_NXCD) * (G / NSX_NXCD) + b / NSX_NXCD : b;
   const uint32_t t = lb * NSX_BLOCK + threadIdx.x;
-  if (t >= (uint32_t)g.NV * g.NL) return;
-  const int v = (int)fdiv(t, g.divNL);
-  const int l = (int)(t - (uint32_t)v * g.NL);
-  double dzT = 0.0, dzB = 0.0;
-  if (a.tagdir == 3) {
-    dzT = fabs(g.sigv[l] - g.sig[l + 1]);
-    dzB = fabs(g.sigv[l] - g.sig[l]);
-  }
+  const uint32_t nrv = (uint32_t)(g.NL + 1) / 2;
+  if (t >= (uint32_t)g.NV * nrv) return;
+  const int v = (int)(t / nrv);
+  const int l0 = 2 * (int)(t - (uint32_t)v * nrv);
+  const int nl = l0 + 1 < g.NL ? 2 : 1;  // levels in this run
+  double dzT[2] = {0.0, 0.0}, dzB[2] = {0.0, 0.0};
+  if (a.tagdir == 3)
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (m < nl) {
+        dzT[m] = fabs(g.sigv[l0 + m] - g.sig[l0 + m + 1]);
+        dzB[m] = fabs(g.sigv[l0 + m] - g.sig[l0 + m]);
+      }
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const double *f = a.f[q];
     const signed char *act = a.act[q];
-    const bool zero_wall = (a.tagdir == 3) && ((l == 0 && a.zero_bot[q]) || (l == g.NL - 1 && a.zero_top[q]));
-    auto val = [&](int vv) -> double {
-      double r;
+    bool zero_wall[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int l = l0 + m;
+      zero_wall[m] = (a.tagdir == 3) && ((l == 0 && a.zero_bot[q]) || (l == g.NL - 1 && a.zero_top[q]));
+    }
+    // the values of vertex vv at the run's levels (interpolation3D: the two
+    // layers around the level weighted by their sigma distances)
+    auto val = [&](int vv, double (&r)[2]) {
       if (a.tagdir == 3) {
-        double sl, su;
-        idw_sums<true>(g, vv, l, f, sl, su);
+        double sm[3];
+        idw_sums<3>(g, vv, l0, nl + 1, f, sm);
         const double ds = g.dlVsum[vv];
-        r = (dzB * (su / ds) + dzT * (sl / ds)) / (dzT + dzB);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) r[m] = (dzB[m] * (sm[m + 1] / ds) + dzT[m] * (sm[m] / ds)) / (dzT[m] + dzB[m]);
       } else {
-        r = idw(g, vv, l, f);
+        double sm[2];
+        idw_sums<2>(g, vv, l0, nl, f, sm);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) r[m] = sm[m] / g.dlVsum[vv];
       }
-      if (act[vv] == VA_ZERO) r = 0.;
-      if (act[vv] == VA_INFLOW) r = a.inflow[q];
-      if (zero_wall) r = 0.;
-      return r;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        if (act[vv] == VA_ZERO) r[m] = 0.;
+        if (act[vv] == VA_INFLOW) r[m] = a.inflow[q];
+        if (zero_wall[m]) r[m] = 0.;
+      }
     };
-    double mine = val(v);
+    double mine[2];
+    val(v, mine);
     if (act[v] == VA_PERIODIC) {
-      double o = 0.0;
+      double o[2] = {0.0, 0.0}, r[2];
       const signed char kind = g.vkind[v];
       if (kind == VK_CORNER) {
         double sd = 0.0;
         for (int q4 = 0; q4 < g.ncorner; ++q4) {
           const int c = g.corner[q4];
-          o = o + val(c) * g.dlVsum[c];
+          val(c, r);
+#pragma unroll
+          for (int m = 0; m < 2; ++m) o[m] = o[m] + r[m] * g.dlVsum[c];
           sd = sd + g.dlVsum[c];
         }
-        o = o / sd;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) o[m] = o[m] / sd;
       } else if (kind == VK_PAIR) {
         const int p = g.vpair[v];
         const double sd = g.dlVsum[v] + g.dlVsum[p];
-        o = g.dlVsum[v] * mine + g.dlVsum[p] * val(p);
-        o = o / sd;
+        val(p, r);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) o[m] = (g.dlVsum[v] * mine[m] + g.dlVsum[p] * r[m]) / sd;
       }
-      mine = o;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) mine[m] = o[m];
     }
-    if (a.tagdir == 3 && a.lid_top[q] && l == g.NL - 1 && g.vkind[v] == VK_INTERIOR) mine = 1.0;
-    a.fv[q][(size_t)v * g.NL + l] = mine;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (m < nl) {
+        const int l = l0 + m;
+        double out = mine[m];
+        if (a.tagdir == 3 && a.lid_top[q] && l == g.NL - 1 && g.vkind[v] == VK_INTERIOR) out = 1.0;
+        a.fv[q][(size_t)v * g.NL + l] = out;
+      }
   }
 }
 
@@ -290,7 +347,7 @@ int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3],
     a.inflow[q] = q < nf && inflow ? inflow[q] : 0.0;
   }
   a.tagdir = tagdir;
-  const uint32_t n = (uint32_t)g.NV * g.NL;
+  const uint32_t n = (uint32_t)g.NV * (uint32_t)((g.NL + 1) / 2);
   const dim3 grid(((n + NSX_BLOCK - 1) / NSX_BLOCK + NSX_NXCD - 1) / NSX_NXCD * NSX_NXCD);
   if (nf == 3) hipLaunchKernelGGL(k_vertex<3>, grid, dim3(NSX_BLOCK), 0, s, g, a);
   else hipLaunchKernelGGL(k_vertex<1>, grid, dim3(NSX_BLOCK), 0, s, g, a);
@@ -347,83 +404,54 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const
                                                     const double *uv, const double *vv, const double *wv,
                                                     double *eddy) {
   const double Cs = 0.1, karman = (double)0.415f;
-  FOR_EACH_RUN_BAND(g, t) {
-    const Run R = run_at(g, t);
-    const int e = R.i * 3;
-    int nb[3], vx[3];
-    double doe[3], lvv[3], xn[3], yn[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      nb[j] = g.nbr[e + j];
-      vx[j] = g.vtx[e + j];
-      doe[j] = g.doe[e + j];
-      lvv[j] = g.dlVV[e + j];
-      xn[j] = g.xnn[e + j];
-      yn[j] = g.ynn[e + j];
-    }
-    const double x1 = xn[0], x2 = xn[1], x3 = xn[2];
-    const double y1 = yn[0], y2 = yn[1], y3 = yn[2];
-    const double l1 = lvv[0], l2 = lvv[1], l3 = lvv[2];
-    const double a = g.area[R.i];
-    // the x, y, z derivatives of u, v, w at each layer of the run, one field
-    // at a time (dvdx and dvdy take the swapped edge lengths)
-    double dx[3][RL], dy[3][RL], dz_[3][RL];
-    const double *fs[3] = {u, v, w}, *fvs[3] = {uv, vv, wv};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      FieldRun F;
-      field_run(g, R, nb, vx, fs[q], fvs[q], F);
-#pragma unroll
-      for (int m = 0; m < RL; ++m) {
-        double P[5];
-        faces_run(F, m, doe, lvv, P);
-        double ddx, ddy;
-        if (q == 1) {
-          ddx = P[0] * x1 * l1 + P[1] * x2 * l3 + P[2] * x3 * l2;
-          ddy = P[0] * y1 * l1 + P[1] * y2 * l3 + P[2] * y3 * l2;
-        } else {
-          ddx = P[0] * x1 * l1 + P[1] * x2 * l2 + P[2] * x3 * l3;
-          ddy = P[0] * y1 * l1 + P[1] * y2 * l2 + P[2] * y3 * l3;
-        }
-        dx[q][m] = ddx / a;
-        dy[q][m] = ddy / a;
-        dz_[q][m] = (P[3] - P[4]) / (R.act[m] ? g.dsig[R.k0 + m] : 1.0);
-      }
-    }
-    double ed[RL];
-#pragma unroll
-    for (int m = 0; m < RL; ++m) {
-      const int k = R.k0 + m;
-      const double dudx = dx[0][m], dvdx = dx[1][m], dwdx = dx[2][m];
-      const double dudy = dy[0][m], dvdy = dy[1][m], dwdy = dy[2][m];
-      const double dudz = dz_[0][m], dvdz = dz_[1][m], dwdz = dz_[2][m];
-      const double dz = R.act[m] ? g.dsig[k] : 1.0;
-      double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
-      const double lwall = karman * (R.act[m] ? g.walld[k] : 1.0);
-      const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
-      const double dn2 = 1.0 / dn1;
-      const double del = sqrt(dn2);
-      DD = del * del;
-      const double nut = 1.0 * DD;
-      const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
-      const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
-      const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
-      ed[m] = nut * Sij;
-    }
-    str(eddy + R.s0, ed, R);
+  FOR_EACH_CELL_BAND(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const size_t s = (size_t)i * g.NS + k;
+    double U[5], V[5], W[5];
+    faces_of(g, i, k, s, u, uv, U);
+    faces_of(g, i, k, s, v, vv, V);
+    faces_of(g, i, k, s, w, wv, W);
+    const int e = i * 3;
+    const double x1 = g.xnn[e], x2 = g.xnn[e + 1], x3 = g.xnn[e + 2];
+    const double y1 = g.ynn[e], y2 = g.ynn[e + 1], y3 = g.ynn[e + 2];
+    const double l1 = g.dlVV[e], l2 = g.dlVV[e + 1], l3 = g.dlVV[e + 2];
+    const double a = g.area[i];
+    double dudx = U[0] * x1 * l1 + U[1] * x2 * l2 + U[2] * x3 * l3;
+    double dvdx = V[0] * x1 * l1 + V[1] * x2 * l3 + V[2] * x3 * l2;
+    double dwdx = W[0] * x1 * l1 + W[1] * x2 * l2 + W[2] * x3 * l3;
+    double dudy = U[0] * y1 * l1 + U[1] * y2 * l2 + U[2] * y3 * l3;
+    double dvdy = V[0] * y1 * l1 + V[1] * y2 * l3 + V[2] * y3 * l2;
+    double dwdy = W[0] * y1 * l1 + W[1] * y2 * l2 + W[2] * y3 * l3;
+    dudx = dudx / a;
+    dvdx = dvdx / a;
+    dwdx = dwdx / a;
+    dudy = dudy / a;
+    dvdy = dvdy / a;
+    dwdy = dwdy / a;
+    const double dz = g.dsig[k];
+    const double dudz = (U[3] - U[4]) / dz, dvdz = (V[3] - V[4]) / dz, dwdz = (W[3] - W[4]) / dz;
+    double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
+    const double lwall = karman * g.walld[k];
+    const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
+    const double dn2 = 1.0 / dn1;
+    const double del = sqrt(dn2);
+    DD = del * del;
+    const double nut = 1.0 * DD;
+    const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
+    const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
+    const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
+    const double ed = nut * Sij;
+    eddy[s] = ed;
     // BCpcenter(eddy): Neumann ghost layers
-#pragma unroll
-    for (int m = 0; m < RL; ++m) {
-      const int k = R.k0 + m;
-      if (k == 1 && g.bcbot == 1) eddy[R.s0 + m - 1] = ed[m];
-      if (k == g.NZ - 2 && g.bctop == 1) eddy[R.s0 + m + 1] = ed[m];
-    }
+    if (k == 1 && g.bcbot == 1) eddy[s - 1] = ed;
+    if (k == g.NZ - 2 && g.bctop == 1) eddy[s + 1] = ed;
   }
 }
 
 int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
                const double *vv, const double *wv, double *eddy) {
-  hipLaunchKernelGGL(k_les, dim3(nsx_grid_for(nruns(g))), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
+  hipLaunchKernelGGL(k_les, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, u, v, w, uv, vv, wv, eddy);
   return (int)hipGetLastError();
 }
 
@@ -443,8 +471,8 @@ struct MomArgs {
 };
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
-  FOR_EACH_RUN_BAND(g, t) {
-    const Run R = run_at(g, t);
+  FOR_EACH_RUN_BAND(g, t, RL) {
+    const Run<RL> R = run_at<RL>(g, t);
     const int e = R.i * 3;
     int nb[3], vx[3];
     double doe[3], lvv[3], lcc[3], le1[3], le2[3];
@@ -485,7 +513,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_momentum(Geo g, MomArgs a) {
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      FieldRun F;
+      FieldRun<RL> F;
       field_run(g, R, nb, vx, a.uvw[q], a.vtx[q], F);
       double rin[RL], gm[RL], hs[RL];
       if (a.rhs_in[q]) {
@@ -561,7 +589,7 @@ int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], con
   a.eddy = eddy;
   a.c1 = c1;
   a.c2 = c2;
-  hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(nruns(g))), dim3(NSX_BLOCK), 0, s, g, a);
+  hipLaunchKernelGGL(k_momentum, dim3(nsx_grid_for(nruns<RL>(g))), dim3(NSX_BLOCK), 0, s, g, a);
   return (int)hipGetLastError();
 }
 
@@ -735,48 +763,88 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_source(Geo g, const double *u, co
                                                        double *src_out, double *rhsp_out, const double *pv,
                                                        double *partials) {
   double acc[1] = {0.0};
-  FOR_EACH_CELL(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    const int e = i * 3;
-    const double ui = u[s], vi = v[s];
-    double Uf[3];
+  FOR_EACH_RUN_BAND(g, t, RL) {
+    const Run<RL> R = run_at<RL>(g, t);
+    const int e = R.i * 3;
+    const EdgeRun E = edges_of(g, R.i);
+    double xn[3], yn[3], lvv[3], le2[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const size_t sn = (size_t)g.nbr[e + j] * g.NS + k;
-      const double Ui = 0.5 * (ui + u[sn]);
-      const double Vi = 0.5 * (vi + v[sn]);
-      Uf[j] = Ui * g.xnn[e + j] + Vi * g.ynn[e + j];
+      xn[j] = g.xnn[e + j];
+      yn[j] = g.ynn[e + j];
+      lvv[j] = g.dlVV[e + j];
+      le2[j] = g.dle2[e + j];
     }
-    const double UT = 0.5 * (w[s] + w[s + 1]);
-    const double UB = -1.0 * (0.5 * (w[s] + w[s - 1]));
-    F1[s] = Uf[0];
-    F2[s] = Uf[1];
-    F3[s] = Uf[2];
-    FT[s] = UT;
-    FB[s] = UB;
-    const double sv = g.dsigv[k - 1], area = g.area[i];
-    double resid = 0.0;
+    const double area = g.area[R.i];
+    double ur[RL], vr[RL], wc[RL + 2], un[3][RL], vn[3][RL];
+    ldr(u + R.s0, ur);
+    ldr(v + R.s0, vr);
+    {
+      double m[RL];
+      ldr(w + R.s0, m);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) resid = resid + Uf[j] * g.dlVV[e + j] * sv;
-    resid = resid + UT * area;
-    resid = resid + UB * area;
-    if (src_out) src_out[s] = resid;
-    if (rhsp_out) {
-      double rp = resid / g.dt;
-      double dn[3];
+      for (int q = 0; q < RL; ++q) wc[q + 1] = m[q];
+      wc[0] = R.act[0] ? w[R.s0 - 1] : 0.0;
+      wc[RL + 1] = R.act[RL - 1] ? w[R.s0 + RL] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ldr(u + (size_t)E.nb[j] * g.NS + R.k0, un[j]);
+      ldr(v + (size_t)E.nb[j] * g.NS + R.k0, vn[j]);
+    }
+    double pvr[3][RL];
+    if (rhsp_out)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int q = 0; q < RL; ++q) pvr[j][q] = R.act[q] ? pv[(size_t)E.vx[j] * g.NL + R.k0 + q] : 0.0;
+    double o1[RL], o2[RL], o3[RL], oT[RL], oB[RL], rs[RL], rp_[RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      const double ui = ur[m], vi = vr[m];
+      double Uf[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
-        const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
-        dn[j] = diff2 * g.dle2[e + j];
+        const double Ui = 0.5 * (ui + un[j][m]);
+        const double Vi = 0.5 * (vi + vn[j][m]);
+        Uf[j] = Ui * xn[j] + Vi * yn[j];
       }
-      const double corr = dn[0] * g.dlVV[e] * sv + dn[1] * g.dlVV[e + 1] * sv + dn[2] * g.dlVV[e + 2] * sv;
-      rp = rp - corr;
-      rhsp_out[s] = rp;
-      acc[0] += rp;
+      const double UT = 0.5 * (wc[m + 1] + wc[m + 2]);
+      const double UB = -1.0 * (0.5 * (wc[m + 1] + wc[m]));
+      o1[m] = Uf[0];
+      o2[m] = Uf[1];
+      o3[m] = Uf[2];
+      oT[m] = UT;
+      oB[m] = UB;
+      const double sv = R.act[m] ? g.dsigv[k - 1] : 1.0;
+      double resid = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) resid = resid + Uf[j] * lvv[j] * sv;
+      resid = resid + UT * area;
+      resid = resid + UB * area;
+      rs[m] = resid;
+      if (rhsp_out) {
+        double rp = resid / g.dt;
+        double dn[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double diff2 = (pvr[(j + 1) % 3][m] - pvr[j][m]) / lvv[j];
+          dn[j] = diff2 * le2[j];
+        }
+        const double corr = dn[0] * lvv[0] * sv + dn[1] * lvv[1] * sv + dn[2] * lvv[2] * sv;
+        rp = rp - corr;
+        rp_[m] = rp;
+        if (R.act[m]) acc[0] += rp;
+      }
     }
+    str(F1 + R.s0, o1, R);
+    str(F2 + R.s0, o2, R);
+    str(F3 + R.s0, o3, R);
+    str(FT + R.s0, oT, R);
+    str(FB + R.s0, oB, R);
+    if (src_out) str(src_out + R.s0, rs, R);
+    if (rhsp_out) str(rhsp_out + R.s0, rp_, R);
   }
   if (!rhsp_out) return;
   write_partials<1>(acc, partials);
@@ -816,54 +884,82 @@ int launch_update_rhsp(const Geo &g, hipStream_t s, double *rhs, const double *p
 
 // ----------------------------------------------------------- LSM gradient --
 // grandientLSM (gradientLSM.F90:153-201): 3-neighbour least squares in the
-// horizontal, 3-point least squares in sigma, then BCpcenter on the outputs.
-__device__ inline void lsm_grad(const Geo &g, int i, int k, size_t s, const double *f, double &dx, double &dy,
-                                double &ds) {
-  const double fi = f[s];
-  double sumfx = 0.0, sumfy = 0.0;
-  const int e = i * 3;
+// horizontal, 3-point least squares in sigma, then BCpcenter on the outputs:
+// the column's least-squares sums, and the gradient at layer k0+m of a run
+struct LsmCol {
+  double dxc[3], dyc[3], sx2, sy2, sxy;
+};
+
+__device__ inline LsmCol lsm_col(const Geo &g, int i) {
+  LsmCol L;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const double d = f[(size_t)g.nbr[e + j] * g.NS + k] - fi;
-    sumfx = sumfx + g.dxCC[e + j] * d;
-    sumfy = sumfy + g.dyCC[e + j] * d;
+    L.dxc[j] = g.dxCC[i * 3 + j];
+    L.dyc[j] = g.dyCC[i * 3 + j];
   }
-  const double sx2 = g.sum_xc2[i], sy2 = g.sum_yc2[i], sxy = g.sum_xcyc[i];
-  const double deter = sx2 * sy2 - sxy * sxy;
-  dx = (sy2 * sumfx - sxy * sumfy) / deter;
-  dy = (sx2 * sumfy - sxy * sumfx) / deter;
-  const double dzp = g.sig[k + 1] - g.sig[k];
-  const double dzm = g.sig[k - 1] - g.sig[k];
+  L.sx2 = g.sum_xc2[i];
+  L.sy2 = g.sum_yc2[i];
+  L.sxy = g.sum_xcyc[i];
+  return L;
+}
+
+template <int RN>
+__device__ inline void lsm_run(const Geo &g, const LsmCol &L, const FieldRun<RN> &F, const Run<RN> &R, int m,
+                               double &dx, double &dy, double &ds) {
+  const double fi = F.c[m + 1];
+  double sumfx = 0.0, sumfy = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double d = F.n[j][m] - fi;
+    sumfx = sumfx + L.dxc[j] * d;
+    sumfy = sumfy + L.dyc[j] * d;
+  }
+  const double deter = L.sx2 * L.sy2 - L.sxy * L.sxy;
+  dx = (L.sy2 * sumfx - L.sxy * sumfy) / deter;
+  dy = (L.sx2 * sumfy - L.sxy * sumfx) / deter;
+  const int k = R.k0 + m;
+  const bool a = R.act[m];
+  const double sk = a ? g.sig[k] : 0.0;
+  const double dzp = (a ? g.sig[k + 1] : 1.0) - sk;
+  const double dzm = (a ? g.sig[k - 1] : 1.0) - sk;
   const double s2 = dzp * dzp + dzm * dzm;
-  ds = ((f[s + 1] - fi) * dzp + (f[s - 1] - fi) * dzm) / s2;
+  ds = ((F.c[m + 2] - fi) * dzp + (F.c[m] - fi) * dzm) / s2;
 }
 
 __global__ __launch_bounds__(NSX_BLOCK) void k_grad_lsm(Geo g, const double *f, double *dx, double *dy,
                                                          double *ds) {
-  FOR_EACH_CELL_BAND(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    double a, b, d;
-    lsm_grad(g, i, k, s, f, a, b, d);
-    dx[s] = a;
-    dy[s] = b;
-    ds[s] = d;
-    if (k == 1 && g.bcbot == 1) {
-      dx[s - 1] = a;
-      dy[s - 1] = b;
-      ds[s - 1] = d;
-    }
-    if (k == g.NZ - 2 && g.bctop == 1) {
-      dx[s + 1] = a;
-      dy[s + 1] = b;
-      ds[s + 1] = d;
+  FOR_EACH_RUN_BAND(g, t, RL) {
+    const Run<RL> R = run_at<RL>(g, t);
+    const EdgeRun E = edges_of(g, R.i);
+    const LsmCol L = lsm_col(g, R.i);
+    FieldRun<RL> F;
+    field_run(g, R, E.nb, E.vx, f, nullptr, F);
+    double a[RL], b[RL], d[RL];
+#pragma unroll
+    for (int m = 0; m < RL; ++m) lsm_run(g, L, F, R, m, a[m], b[m], d[m]);
+    str(dx + R.s0, a, R);
+    str(dy + R.s0, b, R);
+    str(ds + R.s0, d, R);
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      const size_t s = R.s0 + m;
+      if (k == 1 && g.bcbot == 1) {
+        dx[s - 1] = a[m];
+        dy[s - 1] = b[m];
+        ds[s - 1] = d[m];
+      }
+      if (k == g.NZ - 2 && g.bctop == 1) {
+        dx[s + 1] = a[m];
+        dy[s + 1] = b[m];
+        ds[s + 1] = d[m];
+      }
     }
   }
 }
 
 int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, double *dy, double *ds) {
-  hipLaunchKernelGGL(k_grad_lsm, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, f, dx, dy, ds);
+  hipLaunchKernelGGL(k_grad_lsm, dim3(nsx_grid_for(nruns<RL>(g))), dim3(NSX_BLOCK), 0, s, g, f, dx, dy, ds);
   return (int)hipGetLastError();
 }
 
@@ -875,35 +971,49 @@ int launch_grad_lsm(const Geo &g, hipStream_t s, const double *f, double *dx, do
 __global__ __launch_bounds__(NSX_BLOCK) void k_corrector(Geo g, const double *p, const double *us,
                                                           const double *vs, const double *ws, double *un,
                                                           double *vn, double *wn) {
-  FOR_EACH_CELL_BAND(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    double dpdx, dpdy, dpds;
-    lsm_grad(g, i, k, s, p, dpdx, dpdy, dpds);
-    const double nu_ = us[s] - g.dt * dpdx;
-    const double nv_ = vs[s] - g.dt * dpdy;
-    const double nw_ = ws[s] - g.dt * dpds;
-    un[s] = nu_;
-    vn[s] = nv_;
-    wn[s] = nw_;
-    double gv;
-    if (k == 1) {
-      if (vel_ghost(g.bcbot, g.BotBC, 0, nu_, gv)) un[s - 1] = gv;
-      if (vel_ghost(g.bcbot, g.BotBC, 1, nv_, gv)) vn[s - 1] = gv;
-      if (vel_ghost(g.bcbot, g.BotBC, 2, nw_, gv)) wn[s - 1] = gv;
+  FOR_EACH_RUN_BAND(g, t, RL) {
+    const Run<RL> R = run_at<RL>(g, t);
+    const EdgeRun E = edges_of(g, R.i);
+    const LsmCol L = lsm_col(g, R.i);
+    FieldRun<RL> P;
+    field_run(g, R, E.nb, E.vx, p, nullptr, P);
+    double a[RL], b[RL], c[RL];
+    ldr(us + R.s0, a);
+    ldr(vs + R.s0, b);
+    ldr(ws + R.s0, c);
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      double dpdx, dpdy, dpds;
+      lsm_run(g, L, P, R, m, dpdx, dpdy, dpds);
+      a[m] = a[m] - g.dt * dpdx;
+      b[m] = b[m] - g.dt * dpdy;
+      c[m] = c[m] - g.dt * dpds;
     }
-    if (k == g.NZ - 2) {
-      if (vel_ghost(g.bctop, g.TopBC, 0, nu_, gv)) un[s + 1] = gv;
-      if (vel_ghost(g.bctop, g.TopBC, 1, nv_, gv)) vn[s + 1] = gv;
-      if (vel_ghost(g.bctop, g.TopBC, 2, nw_, gv)) wn[s + 1] = gv;
+    str(un + R.s0, a, R);
+    str(vn + R.s0, b, R);
+    str(wn + R.s0, c, R);
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      const size_t s = R.s0 + m;
+      double gv;
+      if (k == 1) {
+        if (vel_ghost(g.bcbot, g.BotBC, 0, a[m], gv)) un[s - 1] = gv;
+        if (vel_ghost(g.bcbot, g.BotBC, 1, b[m], gv)) vn[s - 1] = gv;
+        if (vel_ghost(g.bcbot, g.BotBC, 2, c[m], gv)) wn[s - 1] = gv;
+      }
+      if (k == g.NZ - 2) {
+        if (vel_ghost(g.bctop, g.TopBC, 0, a[m], gv)) un[s + 1] = gv;
+        if (vel_ghost(g.bctop, g.TopBC, 1, b[m], gv)) vn[s + 1] = gv;
+        if (vel_ghost(g.bctop, g.TopBC, 2, c[m], gv)) wn[s + 1] = gv;
+      }
     }
   }
 }
 
 int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double *const us[3],
                      double *const unp[3]) {
-  hipLaunchKernelGGL(k_corrector, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, p, us[0], us[1], us[2],
+  hipLaunchKernelGGL(k_corrector, dim3(nsx_grid_for(nruns<RL>(g))), dim3(NSX_BLOCK), 0, s, g, p, us[0], us[1], us[2],
                      unp[0], unp[1], unp[2]);
   return (int)hipGetLastError();
 }
@@ -914,33 +1024,57 @@ int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double 
 // Bytes/cell: read p, p vertex (0.5), 5 faces = 6.5; write 5 faces.
 __global__ __launch_bounds__(NSX_BLOCK) void k_newu(Geo g, const double *p, const double *pv, double *F1,
                                                      double *F2, double *F3, double *FT, double *FB) {
-  FOR_EACH_CELL_BAND(g, c) {
-    int i, k;
-    cell_of(g, c, i, k);
-    const size_t s = (size_t)i * g.NS + k;
-    const int e = i * 3;
-    const double pi = p[s];
-    double dn[3];
+  FOR_EACH_RUN_BAND(g, t, RL) {
+    const Run<RL> R = run_at<RL>(g, t);
+    const int e = R.i * 3;
+    const EdgeRun E = edges_of(g, R.i);
+    double lcc[3], lvv[3], le1[3], le2[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const int jv1 = g.vtx[e + j], jv2 = g.vtx[e + (j + 1) % 3];
-      const double diff1 = (p[(size_t)g.nbr[e + j] * g.NS + k] - pi) / g.dlCC[e + j];
-      const double diff2 = (pv[(size_t)jv2 * g.NL + k] - pv[(size_t)jv1 * g.NL + k]) / g.dlVV[e + j];
-      dn[j] = diff1 * g.dle1[e + j] + diff2 * g.dle2[e + j];
+      lcc[j] = g.dlCC[e + j];
+      lvv[j] = g.dlVV[e + j];
+      le1[j] = g.dle1[e + j];
+      le2[j] = g.dle2[e + j];
     }
-    const double dT = (p[s + 1] - pi) / g.dsig[k];
-    const double dB = (p[s - 1] - pi) / g.dsig[k];
+    FieldRun<RL> P;
+    field_run(g, R, E.nb, E.vx, p, pv, P);
+    double f1[RL], f2[RL], f3[RL], fT[RL], fB[RL];
+    ldr(F1 + R.s0, f1);
+    ldr(F2 + R.s0, f2);
+    ldr(F3 + R.s0, f3);
+    ldr(FT + R.s0, fT);
+    ldr(FB + R.s0, fB);
     const double c1 = g.dt;
-    F1[s] = F1[s] - dn[0] * c1;
-    F2[s] = F2[s] - dn[1] * c1;
-    F3[s] = F3[s] - dn[2] * c1;
-    FT[s] = FT[s] - dT * c1;
-    FB[s] = FB[s] - dB * c1;
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int k = R.k0 + m;
+      const double pi = P.c[m + 1];
+      double dn[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double diff1 = (P.n[j][m] - pi) / lcc[j];
+        const double diff2 = (P.v[(j + 1) % 3][m] - P.v[j][m]) / lvv[j];
+        dn[j] = diff1 * le1[j] + diff2 * le2[j];
+      }
+      const double dsk = R.act[m] ? g.dsig[k] : 1.0;
+      const double dT = (P.c[m + 2] - pi) / dsk;
+      const double dB = (P.c[m] - pi) / dsk;
+      f1[m] = f1[m] - dn[0] * c1;
+      f2[m] = f2[m] - dn[1] * c1;
+      f3[m] = f3[m] - dn[2] * c1;
+      fT[m] = fT[m] - dT * c1;
+      fB[m] = fB[m] - dB * c1;
+    }
+    str(F1 + R.s0, f1, R);
+    str(F2 + R.s0, f2, R);
+    str(F3 + R.s0, f3, R);
+    str(FT + R.s0, fT, R);
+    str(FB + R.s0, fB, R);
   }
 }
 
 int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, double *const face[5]) {
-  hipLaunchKernelGGL(k_newu, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, p, pv, face[0], face[1],
+  hipLaunchKernelGGL(k_newu, dim3(nsx_grid_for(nruns<RL>(g))), dim3(NSX_BLOCK), 0, s, g, p, pv, face[0], face[1],
                      face[2], face[3], face[4]);
   return (int)hipGetLastError();
 }
