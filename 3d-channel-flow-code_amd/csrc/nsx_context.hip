@@ -102,6 +102,16 @@ struct nsx_ctx {
   int precision = NSX_PREC_FP64;
   int refine = NSX_REFINE_CGSR_BLOCK;
   double delta_cg = 0.1;  // reliable-update fraction of the CG forms
+  // initial guess of nsx_step's pressure solves in the precision forms
+  // (nsx_set_initial_guess): the last two solutions of each RK stage, which
+  // ring slot takes the next one, how many are stored, the extrapolation,
+  // and the stage nsx_step is running (-1 outside it: the stage's input
+  // pressure, as the reference starts)
+  int guess_mode = NSX_GUESS_PREVIOUS;
+  double *phist[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int hslot[2] = {0, 0}, nhist[2] = {0, 0};
+  double *pguess = nullptr;
+  int guess_stage = -1;
   int iter_hint_cg = 0;
   int iter_hint_cgs = 0;
   int iter_hint_cgs64 = 0;
@@ -166,6 +176,12 @@ static size_t field_elems(const nsx_ctx *c, int kind) {
   if (kind == FK_CELL) return (size_t)(c->N0 + c->NGW) * c->NS;
   return (size_t)c->N0 * c->NS;
 }
+
+static size_t cell_bytes(const nsx_ctx *ctx) { return field_elems(ctx, FK_CELL) * sizeof(double); }
+
+// the stored pressure solutions of nsx_set_initial_guess no longer lead to
+// the resident state: start again from two new ones
+static void reset_guess(nsx_ctx *ctx) { ctx->nhist[0] = ctx->nhist[1] = ctx->hslot[0] = ctx->hslot[1] = 0; }
 
 // ------------------------------------------------------------- lifecycle --
 extern "C" int nsx_create(int n_cell0, int n_cell, int n_vert, int nz, nsx_ctx **out) {
@@ -764,6 +780,7 @@ static int check_field(nsx_ctx *ctx, int f) {
 extern "C" int nsx_put_field(nsx_ctx *ctx, int f, const double *host) {
   NEED_COMMIT(ctx);
   if (check_field(ctx, f)) return -1;
+  reset_guess(ctx);  // a new state: the stored solutions no longer lead to it
   const int N0 = ctx->N0, NZ = ctx->NZ, NS = ctx->NS, NV = ctx->NV;
   std::vector<double> buf(field_elems(ctx, ctx->kind[f]), 0.0);
   if (ctx->kind[f] == FK_VERT) {
@@ -1163,7 +1180,24 @@ static int stage(nsx_ctx *ctx, int in, int out, int flag_ab) {
     (void)hipEventRecord(ev[1], s);
     ctx->pending.push_back({(size_t)(ev - &ctx->evpool[0]), 5});
   }
-  if ((rc = solve(ctx, ctx->rhsp, X[3], Y[3]))) return rc;
+  {
+    // the initial guess: the stage's input pressure, or (nsx_step with
+    // NSX_GUESS_EXTRAPOLATE) 2 p_k - p_{k-1} from the last two solutions of
+    // this RK stage
+    const int gs = ctx->guess_stage;
+    const double *guess = X[3];
+    if (gs >= 0 && ctx->nhist[gs] >= 2) {
+      const int nw = ctx->hslot[gs] ^ 1;
+      LAUNCH(launch_extrapolate(g, s, ctx->phist[gs][nw], ctx->phist[gs][nw ^ 1], ctx->pguess));
+      guess = ctx->pguess;
+    }
+    if ((rc = solve(ctx, ctx->rhsp, guess, Y[3]))) return rc;
+    if (gs >= 0) {
+      HIPCHECK(hipMemcpyAsync(ctx->phist[gs][ctx->hslot[gs]], Y[3], cell_bytes(ctx), hipMemcpyDeviceToDevice, s));
+      ctx->hslot[gs] ^= 1;
+      ctx->nhist[gs] += 1;
+    }
+  }
   BC_WALLS(Y[3], 0);  // bicgstab's BCpcenter(phi)
   if (ctx->timing) {
     ev = events(ctx, 2);
@@ -1196,6 +1230,7 @@ static void point_slots(nsx_ctx *ctx, int n, int stage_in, int newest) {
 
 extern "C" int nsx_init_state(nsx_ctx *ctx) {
   NEED_COMMIT(ctx);
+  reset_guess(ctx);
   const Geo &g = ctx->g;
   hipStream_t s = ctx->stream;
   // the UFN..PFN slots hold the initial field; make them pool set 0
@@ -1258,14 +1293,23 @@ extern "C" int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info 
   struct Restore {
     nsx_ctx *c;
     int p;
-    ~Restore() { c->precision = p; }
+    ~Restore() {
+      c->precision = p;
+      c->guess_stage = -1;
+    }
   } restore{ctx, saved};
+  // extrapolated initial guesses in the precision forms only; steps without
+  // them leave the stored solutions stale, so they are dropped
+  const bool extrap = ctx->guess_mode == NSX_GUESS_EXTRAPOLATE && precision != NSX_PREC_FP64;
+  if (!extrap) reset_guess(ctx);
   for (int n = 0; n < nsteps; ++n) {
     const int N = ctx->set_n, A = (N + 1) % 3, B = (N + 2) % 3;
     int rc;
     if (ctx->FinRK == 2) {
+      ctx->guess_stage = extrap ? 0 : -1;
       if ((rc = stage(ctx, N, A, 0))) return rc;
       if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
+      ctx->guess_stage = extrap ? 1 : -1;
       if ((rc = stage(ctx, A, B, 0))) return rc;
       if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
       const double *un[3] = {ctx->pool_cell[N][0], ctx->pool_cell[N][1], ctx->pool_cell[N][2]};
@@ -1276,6 +1320,7 @@ extern "C" int nsx_step(nsx_ctx *ctx, int nsteps, int precision, nsx_solve_info 
     } else {
       // Euler, then AB2 from the second step on (hydro.F90:469-478)
       const int flag_ab = ctx->info.n_solves > 0 ? 1 : 0;
+      ctx->guess_stage = extrap ? 0 : -1;
       if ((rc = stage(ctx, N, A, flag_ab))) return rc;
       if (ctx->info.exit_code == NSX_SOLVE_MAX_ITERS) worst = NSX_SOLVE_MAX_ITERS;
       ctx->set_n = A;
@@ -1813,7 +1858,6 @@ static int adj_alloc(nsx_ctx *ctx, size_t nckpt, size_t nseg) {
   return 0;
 }
 
-static size_t cell_bytes(const nsx_ctx *ctx) { return field_elems(ctx, FK_CELL) * sizeof(double); }
 
 // the state of pool set `set` (u, v, w, p) and the faces into dst[9], or back
 static int save_state(nsx_ctx *ctx, int set, double *const dst[9]) {
@@ -2259,6 +2303,24 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
   ctx->precision = precision;
   ctx->refine = refine;
   ctx->delta_cg = tol;
+  return 0;
+}
+
+extern "C" int nsx_set_initial_guess(nsx_ctx *ctx, int mode) {
+  if (!ctx) return -1;
+  if (mode != NSX_GUESS_PREVIOUS && mode != NSX_GUESS_EXTRAPOLATE)
+    return nsx_fail(ctx, -1, "unknown initial-guess mode %d", mode);
+  if (mode == NSX_GUESS_EXTRAPOLATE && !ctx->pguess) {
+    if (!ctx->committed) return nsx_fail(ctx, -1, "nsx_set_initial_guess: the mesh is not committed");
+    const size_t nc = field_elems(ctx, FK_CELL);
+    int rc;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        if ((rc = dev_alloc(ctx, &ctx->phist[a][b], nc))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->pguess, nc))) return rc;
+  }
+  ctx->guess_mode = mode;
+  reset_guess(ctx);
   return 0;
 }
 

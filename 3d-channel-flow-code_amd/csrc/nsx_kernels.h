@@ -100,6 +100,8 @@ int launch_corrector(const Geo &g, hipStream_t s, const double *p, const double 
                      double *const unp[3]);
 int launch_newu(const Geo &g, hipStream_t s, const double *p, const double *pv, double *const face[5]);
 int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double *const unp[3]);
+// o = 2 a - b over the cell fields (the extrapolated initial guess)
+int launch_extrapolate(const Geo &g, hipStream_t s, const double *a, const double *b, double *o);
 int launch_ghost_layers(const Geo &g, hipStream_t s, double *f, int kind /*0 p, 1 u, 2 v, 3 w*/);
 int launch_ppe_diag(const Geo &g, hipStream_t s, double *A0);
 int launch_copy(hipStream_t s, size_t n, const double *src, double *dst);

@@ -1098,6 +1098,20 @@ int launch_rk2avg(const Geo &g, hipStream_t s, const double *const un[3], double
   return (int)hipGetLastError();
 }
 
+// the initial guess 2 a - b of an extrapolating pressure solve, a the newer
+// of two solutions
+__global__ __launch_bounds__(NSX_BLOCK) void k_extrapolate(size_t n, const double *a, const double *b, double *o) {
+  for (size_t t = (size_t)blockIdx.x * NSX_BLOCK + threadIdx.x; t < n; t += (size_t)gridDim.x * NSX_BLOCK)
+    o[t] = a[t] + (a[t] - b[t]);
+}
+
+int launch_extrapolate(const Geo &g, hipStream_t s, const double *a, const double *b, double *o) {
+  const size_t n = (size_t)(g.N0 + g.NGW) * g.NS;
+  hipLaunchKernelGGL(k_extrapolate, dim3(nsx_grid_for((uint32_t)(n < 0xffffffffu ? n : 0xffffffffu))),
+                     dim3(NSX_BLOCK), 0, s, n, a, b, o);
+  return (int)hipGetLastError();
+}
+
 // --------------------------------------------------------- ghost layers ----
 // vertical part of BCpcenter (kind 0, BC.F90:919-938) and BCvelcenter3D
 // (kinds 1..3 = u,v,w, BC.F90:459-527) on a stored field

@@ -31,6 +31,9 @@
       integer(c_int), parameter :: NSX_REFINE_CG = 3,                   &
            NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5,               &
            NSX_REFINE_CGSR_LINE = 6
+!     initial guess of nsx_step's pressure solves (enum nsx_guess)
+      integer(c_int), parameter :: NSX_GUESS_PREVIOUS = 0,              &
+           NSX_GUESS_EXTRAPOLATE = 1
 !     accumulators of the channel statistics (enum nsx_sm, module stats)
       integer(c_int), parameter :: NSX_SM_UF = 0, NSX_SM_VF = 1,        &
            NSX_SM_WF = 2, NSX_SM_U2F = 3, NSX_SM_V2F = 4, NSX_SM_W2F = 5,&
@@ -452,6 +455,12 @@
           type(c_ptr), value :: ctx
           integer(c_int), value :: precision, refine, max_passes
           real(c_double), value :: tol
+        end function
+        integer(c_int) function nsx_set_initial_guess(ctx, mode)         &
+             bind(C, name="nsx_set_initial_guess")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: ctx
+          integer(c_int), value :: mode
         end function
         integer(c_int) function nsx_timing_enable(ctx, on)               &
              bind(C, name="nsx_timing_enable")

@@ -23,6 +23,9 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_FP64_CG = 2
 PREC_MIXED_FLUX32 = 3
+# initial guess of nsx_step's pressure solves in the precision forms
+GUESS_PREVIOUS = 0
+GUESS_EXTRAPOLATE = 1
 REFINE_CG = 3
 REFINE_CGSR = 4
 REFINE_CGSR_BLOCK = 5
@@ -39,7 +42,7 @@ EXPORTS = [
     "nsx_calcul_face", "nsx_sgles", "nsx_advdiff", "nsx_advdiff_gam", "nsx_interpolate_u", "nsx_source3d",
     "nsx_update_rhsp", "nsx_bicgstab", "nsx_bicgstab_solve",
     "nsx_newu3d", "nsx_bicgstab_adjoint", "nsx_set_body_force", "nsx_wall_shear_run", "nsx_wall_shear_tangent",
-    "nsx_step_tangent", "nsx_step_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
+    "nsx_step_tangent", "nsx_step_adjoint", "nsx_glob_cfl", "nsx_glob_cfl_noalloc", "nsx_set_precision", "nsx_set_initial_guess", "nsx_timing_enable", "nsx_timing_get", "nsx_timing_reset",
     "nsx_sm_init", "nsx_glob_sample", "nsx_sm_count", "nsx_sm_get", "nsx_sm_planes", "nsx_glob_put_plane_mm",
     "nsx_write_plane_stats", "nsx_format_real", "nsx_savetec_vertex", "nsx_savetec_center", "nsx_write_tec_vertex",
     "nsx_write_tec_center", "nsx_save_paraview_vertex", "nsx_savetec_vc", "nsx_write_vtk_vertex", "nsx_write_tec_vc",
@@ -104,6 +107,7 @@ def declare(lib: ctypes.CDLL) -> None:
         "nsx_bicgstab": [c_vp, c_int, c_int, c_int, P(SolveInfo)],
         "nsx_newu3d": [c_vp, c_int, c_int],
         "nsx_set_precision": [c_vp, c_int, c_int, c_dbl, c_int],
+        "nsx_set_initial_guess": [c_vp, c_int],
         "nsx_glob_cfl": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_glob_cfl_noalloc": [c_vp, c_int, c_int, c_int, P(CflInfo)],
         "nsx_bicgstab_adjoint": [c_vp, c_int, c_int, P(SolveInfo)],

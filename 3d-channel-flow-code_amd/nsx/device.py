@@ -452,6 +452,13 @@ class Context:
         precision and uses this refinement."""
         self._check(self.lib.nsx_set_precision(self.h, precision, refine, tol, max_passes))
 
+    def set_initial_guess(self, mode: int) -> None:
+        """Initial guess of step()'s pressure solves in the precision forms:
+        GUESS_PREVIOUS (the stage's input pressure, the reference's) or
+        GUESS_EXTRAPOLATE (2 p_k - p_{k-1} from the same RK stage's last two
+        solutions)."""
+        self._check(self.lib.nsx_set_initial_guess(self.h, mode))
+
     def timing(self, on: bool) -> None:
         self._check(self.lib.nsx_timing_enable(self.h, int(on)))
 

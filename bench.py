@@ -20,6 +20,11 @@ rule min(|r|_1/|b|_1, |r|_2/|b|_2) < eps):
            gate is the mixed leg's);
   ref      fp64_cg at eps 1e-14, the tight reference the other legs'
            errors are measured against.
+The precision-form legs (fp64_cg, mixed, mixed_flux32) start each pressure
+solve from 2 p_k - p_{k-1} of the same RK stage's last two solutions
+(--guess extrapolate, NSX_GUESS_EXTRAPOLATE: about 5 % fewer iterations);
+fp64 and ref start from the stage's input pressure, as the reference does
+(--guess previous sets that for every leg).
 The headline legs run at the gate tolerance (--eps-gate, 1e-13), where the
 mixed leg's u, v, w and p agree with the FP64 leg's to the 1e-8 gate with
 margin (at 1e-12 the pressures agree to 2.5e-9 to 2.7e-8 depending on the
@@ -126,9 +131,11 @@ CGS_UNITS = 8.5
 CGS_KERNEL = {4: "k_cgs<float, 0>", 5: "k_cgs<float, 1>", 6: "k_cgs<float, 2>"}
 
 
-def leg(case, name: str, steps: int, warmup: int, eps: float, max_iters=None) -> dict:
+def leg(case, name: str, steps: int, warmup: int, eps: float, max_iters=None, guess: int = 0) -> dict:
     """One leg: `name` is fp64 (BiCGSTAB), fp64_cg (FP64 single-reduction
-    CG) or one of MIXED_MODES, with the pressure solves stopped at eps."""
+    CG) or one of MIXED_MODES, with the pressure solves stopped at eps; guess
+    is the initial-guess mode of the precision forms (0 the stage's input
+    pressure, 1 extrapolated)."""
     from nsx.device import Context
 
     precision = {"fp64": 0, "fp64_cg": 2, "mixed_flux32": 3}.get(name, 1)
@@ -137,6 +144,8 @@ def leg(case, name: str, steps: int, warmup: int, eps: float, max_iters=None) ->
     ctx.set_params(eps=eps, max_iters=max_iters or p.MaxIters)
     if precision in (1, 3):
         ctx.set_precision(precision, *MIXED_MODES[name])
+    if guess and precision != 0:
+        ctx.set_initial_guess(guess)
     ctx.init_state()
     if warmup:
         ctx.step(warmup, precision)
@@ -233,6 +242,8 @@ def main() -> int:
                          "mixed_cg2")
     ap.add_argument("--max-iters", type=int, default=None,
                     help="cap the solver iterations (profiling runs only: the line is then marked capped)")
+    ap.add_argument("--guess", choices=("extrapolate", "previous"), default="extrapolate",
+                    help="initial guess of the precision-form legs' pressure solves")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy one cell field with k_copy (the known-size copy for HBM counter calibration)")
     args = ap.parse_args()
@@ -256,8 +267,10 @@ def main() -> int:
         ns = (case.NZ + 3) // 4 * 4
         print(f"# k_copy calibration: {case.N_CELL0 * ns} doubles", file=sys.stderr, flush=True)
 
-    def run(name, eps):
-        r = leg(case, name, steps, args.warmup, eps, args.max_iters)
+    guess = 1 if args.guess == "extrapolate" else 0
+
+    def run(name, eps, g=None):
+        r = leg(case, name, steps, args.warmup, eps, args.max_iters, guess if g is None else g)
         print(f"# leg {name} at eps {eps:g}: {r['value']:.4g} prism-cell-steps/s, {r['ms_per_step']:.1f} ms/step, "
               f"{r['mean_iters_per_solve']:.1f} iterations/solve", file=sys.stderr, flush=True)
         return r
@@ -268,7 +281,7 @@ def main() -> int:
         for name in ("fp64", "mixed"):
             if name in legs:
                 at_cfg[name] = run(name, eps_cfg)
-    ref = run("fp64_cg", args.eps_ref) if args.eps_ref > 0 else None
+    ref = run("fp64_cg", args.eps_ref, 0) if args.eps_ref > 0 else None
 
     head = res.get("mixed") or res.get("fp64") or next(iter(res.values()))
     kern = head["kernels"]
@@ -291,6 +304,7 @@ def main() -> int:
         "config": {"workload": f"channel {WORKLOAD[cfg]}", "prisms": case.n_interior,
                    "time_scheme": "RK2 (FinRK=2)", "les": "Smagorinsky Cs=0.1", "ppe_tolerance": args.eps_gate,
                    "ppe_tolerance_workload": case.params.eps, "reliable_update_delta": MIXED_MODES["mixed"][1],
+                   "initial_guess": {"precision_forms": args.guess, "fp64": "previous"},
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(cfg, dom["kernel"]),
