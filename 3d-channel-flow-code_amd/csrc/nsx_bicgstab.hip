@@ -476,6 +476,22 @@ __device__ inline Sweep window_sweep(uint32_t nwin, bool rev = false) {
   return S;
 }
 
+// for_each_quad in the order of window_sweep: windows of NSX_BLOCK quads,
+// band by band per XCD, from the far end with rev
+template <class F>
+__device__ inline void for_each_quad_swept(const Geo &g, bool rev, F &&f) {
+  const uint32_t nwin = (g.nquad + NSX_BLOCK - 1) / NSX_BLOCK;
+  const Sweep S = window_sweep(nwin, rev);
+  for (uint32_t wi = S.w; wi < S.end; wi += S.step) {
+    const uint32_t t = S.at(wi) * NSX_BLOCK + threadIdx.x;
+    if (t < g.nquad) {
+      const uint32_t i = fdiv(t, g.divNQ);
+      const int k0 = 4 * (int)(t - i * (uint32_t)g.NQ);
+      f((int)i, k0, (size_t)i * g.NS + k0);
+    }
+  }
+}
+
 // Seven-point values of a vector f formed from stored vectors around every
 // quad.  Each thread forms f on its own four layers (own4) and publishes them
 // in LDS; the vertical neighbours and the neighbour columns whose quad lies
@@ -580,6 +596,10 @@ template <typename T>
 __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const T *q, const T *rhat, T *vt, Partials P,
                                                        SolverState *st) {
   if (st->done) return;
+  // the three phases alternate their sweep direction launch by launch
+  // (window_sweep): A and C of iteration k sweep from the far end when k is
+  // odd, B the other way
+  const bool rev = (st->c_iter & 1) != 0;
   double beta_d = 0.0;
   bool first = false;
   if (!begin_iteration(g, P, st, beta_d, first)) return;
@@ -602,7 +622,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_a(Geo g, const T *rt, const 
           acc[0] += (double)rh[u] * (double)v;
         }
         st4(vt + s0, vo);
-      });
+      },
+      rev);
   write_partials<1>(acc, P.a);
 }
 
@@ -625,6 +646,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const 
     return;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->b_alpha = al;
+  const bool rev = (k_it & 1) == 0;
   const T alm = (T)(-al);
   const T *lv = load_levels<T>(g);
   double acc[2] = {0.0, 0.0};
@@ -643,7 +665,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_b(Geo g, const T *rt, const 
           acc[1] += td * td;
         }
         st4(tv + s0, t);
-      });
+      },
+      rev);
   write_partials<2>(acc, P.b);
 }
 
@@ -665,9 +688,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_c(Geo g, T *x, T *rt, T *q, 
   }
   const T alt = (T)al, omt = (T)om, bet = (T)st->a_beta;
   const bool first = st->a_iter == 0;
+  const bool rev = (st->a_iter & 1) != 0;
   const T *lv = load_levels<T>(g);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for_each_quad(g, [&](int i, int k0, size_t s0) {
+  for_each_quad_swept(g, rev, [&](int i, int k0, size_t s0) {
     T xx[4], rr[4], vv[4], yy[4], tt[4], rh[4], qq[4], qv[5];
     // Scheduling fences keep the two 16-byte halves of each vector's run
     // back to back.  Across a wave one half touches every cache line of the
