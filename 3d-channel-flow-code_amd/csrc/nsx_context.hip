@@ -754,6 +754,13 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
     if (e != 0) return nsx_fail_hip(ctx, (hipError_t)e, "hipGetSymbolAddress of the glob_cfl scratch", __LINE__);
   }
 
+  // the eddy viscosity's geometric factor (Cs Delta)^2 per prism
+  {
+    double *nutg = nullptr;
+    if ((rc = dev_alloc(ctx, &nutg, (size_t)N0 * NS))) return rc;
+    g.nutg = nutg;
+    LAUNCH(launch_les_length(g, ctx->stream, nutg));
+  }
   // compact PPE operator, and a device-side check of its diagonal
   double *A0chk = nullptr;
   if ((rc = cell(&A0chk))) return rc;

@@ -85,6 +85,9 @@ struct Geo {
   // FP32 copies for the FP32 inner solver of the mixed-precision form
   const float *area_f, *ppec_f, *dsig_f, *dsigv_f, *ppec_t_f;
   const double *walld;       // NZ
+  // per prism (i*NS+k, interior layers): (Cs Delta)^2 of sgles with Mason's
+  // wall damping, which depends on the geometry only (set at commit)
+  const double *nutg;
   // vertices: inverse-distance stencil in CSR form (interior cells, reference order)
   const int *vptr, *vcell;
   const double *vw, *dlVsum;

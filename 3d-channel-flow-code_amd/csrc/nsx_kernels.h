@@ -79,6 +79,8 @@ int launch_vertex(const Geo &g, hipStream_t s, int nf, const double *const f[3],
                   const signed char *const act[3], const int zero_bot[3], const int zero_top[3], int tagdir,
                   const int *lid_top = nullptr, const double *inflow = nullptr);
 int launch_wall_ghosts(const Geo &g, hipStream_t s, double *f, int kind);
+// (Cs Delta)^2 with Mason's damping per prism, once at commit (Geo::nutg)
+int launch_les_length(const Geo &g, hipStream_t s, double *nutg);
 int launch_les(const Geo &g, hipStream_t s, const double *u, const double *v, const double *w, const double *uv,
                const double *vv, const double *wv, double *eddy);
 int launch_momentum(const Geo &g, hipStream_t s, const double *const uvw[3], const double *const vtx[3],

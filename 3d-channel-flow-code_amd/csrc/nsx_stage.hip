@@ -399,11 +399,37 @@ int launch_calcul_face(const Geo &g, hipStream_t s, const double *f, const doubl
 // (utau = 1, walld = |sig(k)|), Mason power 2 evaluated as products and a
 // square root, as the compiled reference does.  dvdx and dvdy take dlVV of
 // edge 3 for edge 2 and of edge 2 for edge 3 (sgm_les.F90:444-446,453-455).
-// Bytes/cell: read u,v,w (+3 vertex fields at 0.5) = 4.5, write eddy = 1.
+// The length scale, (Cs Delta)^2 with Mason's damping (Delta = (area
+// dsig)^(1/3), lwall = karman walld, 1/l^2 = 1/(Cs Delta)^2 + 1/lwall^2, the
+// device pow for the cube root), depends on the geometry only: k_les_length
+// forms it once at commit with the same expressions, and k_les reads it.
+// Bytes/cell of k_les: read u,v,w (+3 vertex fields at 0.5) and the length
+// scale = 5.5, write eddy = 1.
+__global__ __launch_bounds__(NSX_BLOCK) void k_les_length(Geo g, double *nutg) {
+  const double Cs = 0.1, karman = (double)0.415f;
+  FOR_EACH_CELL_BAND(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    const double a = g.area[i];
+    const double dz = g.dsig[k];
+    double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
+    const double lwall = karman * g.walld[k];
+    const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
+    const double dn2 = 1.0 / dn1;
+    const double del = sqrt(dn2);
+    DD = del * del;
+    nutg[(size_t)i * g.NS + k] = 1.0 * DD;
+  }
+}
+
+int launch_les_length(const Geo &g, hipStream_t s, double *nutg) {
+  hipLaunchKernelGGL(k_les_length, dim3(nsx_grid_for(g.ncell)), dim3(NSX_BLOCK), 0, s, g, nutg);
+  return (int)hipGetLastError();
+}
+
 __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const double *v, const double *w,
                                                     const double *uv, const double *vv, const double *wv,
                                                     double *eddy) {
-  const double Cs = 0.1, karman = (double)0.415f;
   FOR_EACH_CELL_BAND(g, c) {
     int i, k;
     cell_of(g, c, i, k);
@@ -431,13 +457,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_les(Geo g, const double *u, const
     dwdy = dwdy / a;
     const double dz = g.dsig[k];
     const double dudz = (U[3] - U[4]) / dz, dvdz = (V[3] - V[4]) / dz, dwdz = (W[3] - W[4]) / dz;
-    double DD = Cs * pow(a * dz * 1.0, 1.0 / 3.0);
-    const double lwall = karman * g.walld[k];
-    const double dn1 = (1.0 / DD) * (1.0 / DD) + (1.0 / lwall) * (1.0 / lwall);
-    const double dn2 = 1.0 / dn1;
-    const double del = sqrt(dn2);
-    DD = del * del;
-    const double nut = 1.0 * DD;
+    const double nut = g.nutg[s];
     const double sxx = (dudx + dudx) / 2.0, sxy = (dudy + dvdx) / 2.0, sxz = (dudz + dwdx) / 2.0;
     const double syy = (dvdy + dvdy) / 2.0, syz = (dvdz + dwdy) / 2.0, szz = (dwdz + dwdz) / 2.0;
     const double Sij = sqrt(2.0 * (sxx * sxx + syy * syy + szz * szz) + 4.0 * (sxy * sxy + sxz * sxz + syz * syz));
