@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NSX_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["nsx_context.hip", "nsx_stage.hip", "nsx_bicgstab.hip", "nsx_diag.hip", "nsx_stats.hip",
-               "nsx_files.hip", "nsx_adjoint.hip"]
+               "nsx_files.hip", "nsx_adjoint.hip", "nsx_modal.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # arithmetic order of the reference: no contraction into FMA, so the

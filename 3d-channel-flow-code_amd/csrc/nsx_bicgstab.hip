@@ -287,6 +287,8 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_bicg_init_state(SolverState *st, 
   if (threadIdx.x == 0) {
     reset_state(st, eps, max_iters);
     st->mp_pass = 0;
+    st->m_stop = 0;
+    st->m_iters = 0;
     st->delta = delta;
     st->ureq[0] = st->ureq[1] = 0;
   }
@@ -2017,4 +2019,601 @@ int solver_blocks_per_cu(const Geo &g) {
     if (n < best) best = n;
   }
   return best;
+}
+
+// ------------------------------------- CG with the vertical-mode preconditioner
+// (NSX_REFINE_MODAL; the tables and their meaning: nsx_modal.hip).  A
+// residual r (FP32, column-contiguous) goes to modal coefficients column by
+// column, rh_m = sum_k Phi[k][m] r(k+1); the lowest m0 go through one
+// V-cycle of the 2D aggregation multigrid, the others are divided by their
+// diagonal d_i + mu_m area_i; u = -Phi x (the operator is -(L (x) S + D (x)
+// T)).  Both transforms are GEMMs of the N0 columns against the mode table:
+// on the matrix cores (v_mfma_f32_16x16x4_f32, FP32 in and out) when the
+// columns have at most 128 slots, else on the vector units.
+
+// --- matrix-core transforms.  A wave forms 16 columns x all modes (or slots):
+// its lane (j, g) = (lane & 15, lane >> 4) loads float4s of column c0 + j at
+// slots 16 t + 4 g .. +3, which are the A operand of four MFMAs (k index
+// 4 g + q of step (t, q)); the table's float4 for (t, tile, lane) holds the
+// matching B operands, laid out by the host so that one ds_read_b128 per lane
+// fetches them.  The result D[i][j] of a 16 x 16 tile sits in lane j + 16 (i /
+// 4), register i % 4.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f32x4 mfma4(const float4 &a, const float4 &b, f32x4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
+}
+
+// the operand table (n <= 64 T * T float4s) into LDS, every load issued
+// before the first store
+template <int T>
+__device__ inline void lds_table(float4 *dst, const float4 *src, int n) {
+  constexpr int U = (64 * T * T + NSX_BLOCK - 1) / NSX_BLOCK;
+  float4 v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int q = threadIdx.x + u * NSX_BLOCK;
+    if (q < n) v[u] = src[q];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int q = threadIdx.x + u * NSX_BLOCK;
+    if (q < n) dst[q] = v[u];
+  }
+  __syncthreads();
+}
+
+// the A operands of a wave's tile: column c0 + (lane & 15), slots 16 t + 4
+// (lane >> 4) (raw: the slots outside the interior are masked where used)
+template <int T>
+__device__ inline void load_cols(const float *r, int NS, int N0, int c, int gq, float4 (&a)[T]) {
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const int k = 16 * t + 4 * gq;
+    a[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c < N0 && k < NS) a[t] = *(const float4 *)(r + (size_t)c * NS + k);
+  }
+}
+
+template <int MT>
+__device__ inline void load_modes(const float *xh, int N0, int c, int gq, float4 (&a)[MT]) {
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    a[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (c < N0) a[t] = *(const float4 *)(xh + (size_t)c * (16 * MT) + 16 * t + 4 * gq);
+  }
+}
+
+// T: slot tiles of 16 (NS <= 16 T), MT = mp / 16 mode tiles (T - 1 or T).
+// Two blocks a CU (two waves a SIMD), each wave loads its next tile's
+// operands before it forms the current one.
+template <int T, int MT>
+__global__ __launch_bounds__(NSX_BLOCK, 2) void k_modal_fwd_mfma(Geo g, ModalDev M, const float *r,
+                                                                  const SolverState *st) {
+  if (st->m_stop) return;
+  extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
+  float4 *tab = reinterpret_cast<float4 *>(nsx_dyn_lds);
+  const int m0 = M.m0, nzi = M.nzi;
+  lds_table<T>(tab, M.tfwd, T * MT * 64);
+  const int lane = threadIdx.x & 63, j = lane & 15, gq = lane >> 4;
+  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (NSX_BLOCK / 64);
+  int tile = blockIdx.x * (NSX_BLOCK / 64) + (threadIdx.x >> 6);
+  float4 a[T], an[T];
+  load_cols<T>(r, g.NS, g.N0, tile * 16 + j, gq, a);
+  for (; tile < ntile; tile += stride) {
+    if (tile + stride < ntile) load_cols<T>(r, g.NS, g.N0, (tile + stride) * 16 + j, gq, an);
+    const int c0 = tile * 16;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // one slot tile at a time: the scheduler keeps a step's table operands
+    // (MT float4s) in flight, not the whole table
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int k = 16 * t + 4 * gq;
+      float4 x = a[t];
+      x.x = (k >= 1 && k <= nzi) ? x.x : 0.f;
+      x.y = (k + 1 <= nzi) ? x.y : 0.f;
+      x.z = (k + 2 <= nzi) ? x.z : 0.f;
+      x.w = (k + 3 <= nzi) ? x.w : 0.f;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[mt] = mfma4(x, tab[(t * MT + mt) * 64 + lane], acc[mt]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    float dg[4], ar[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = min(c0 + 4 * gq + q, g.N0 - 1);
+      dg[q] = M.lev[0].diag[c];
+      ar[q] = M.lev[0].area[c];
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int m = 16 * mt + j;
+      const float mum = M.mu[m];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 4 * gq + q;
+        if (c >= g.N0) continue;
+        const float v = acc[mt][q], d = dg[q] + mum * ar[q];
+        if (m < m0) {
+          M.b[0][(size_t)c * m0 + m] = v;
+          M.z[0][(size_t)c * m0 + m] = v / d;
+        } else {
+          M.xh[(size_t)c * (16 * MT) + m] = v / d;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) a[t] = an[t];
+  }
+}
+
+// u = -Phi x from the modal coefficients xh (N0 x mp), every slot of the
+// column (zero outside the interior); partial sums of <r, u>
+template <int T, int MT>
+__global__ __launch_bounds__(NSX_BLOCK, 2) void k_modal_bwd_mfma(Geo g, ModalDev M, const float *r, float *u,
+                                                                  double *P, const SolverState *st) {
+  if (st->m_stop) return;
+  extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
+  float4 *tab = reinterpret_cast<float4 *>(nsx_dyn_lds);
+  lds_table<T>(tab, M.tbwd, T * MT * 64);
+  const int lane = threadIdx.x & 63, j = lane & 15, gq = lane >> 4;
+  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (NSX_BLOCK / 64);
+  int tile = blockIdx.x * (NSX_BLOCK / 64) + (threadIdx.x >> 6);
+  double acc2[1] = {0.0};
+  float4 a[MT], an[MT];
+  load_modes<MT>(M.xh, g.N0, tile * 16 + j, gq, a);
+  for (; tile < ntile; tile += stride) {
+    if (tile + stride < ntile) load_modes<MT>(M.xh, g.N0, (tile + stride) * 16 + j, gq, an);
+    const int c0 = tile * 16;
+    f32x4 acc[T];
+#pragma unroll
+    for (int nt = 0; nt < T; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+#pragma unroll
+      for (int nt = 0; nt < T; ++nt) acc[nt] = mfma4(a[t], tab[(t * T + nt) * 64 + lane], acc[nt]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int nt = 0; nt < T; ++nt) {
+      const int k = 16 * nt + j;
+      if (k >= g.NS) continue;
+      const bool in = k >= 1 && k <= M.nzi;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 4 * gq + q;
+        if (c >= g.N0) continue;
+        const size_t sl = (size_t)c * g.NS + k;
+        const float uv = 0.f - acc[nt][q];
+        u[sl] = uv;
+        if (in) acc2[0] += (double)r[sl] * (double)uv;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MT; ++t) a[t] = an[t];
+  }
+  write_partials<1>(acc2, P);
+}
+
+// --- vector-unit transforms for columns of more than 128 slots
+constexpr int MODAL_COLS = 32;  // columns a block transforms together
+constexpr int MODAL_CSTR = 36;  // LDS row stride of the column tile (float4-aligned, spreads the banks)
+
+__device__ inline bool modal_lds_fits(const ModalDev &M) {
+  return (size_t)M.nzp * M.nzp + (size_t)MODAL_CSTR * M.nzp <= 16384;
+}
+static size_t modal_lds(const ModalDev &M) {
+  const size_t t = (size_t)M.nzp * M.nzp + (size_t)MODAL_CSTR * M.nzp;
+  return (t <= 16384 ? t : (size_t)MODAL_CSTR * M.nzp) * sizeof(float);
+}
+
+// out[c][j] = sum_k tab[k][j] in[k][c] for the block's MODAL_COLS columns: a
+// thread forms a 4 x 4 tile (4 columns, 4 of j) from a float4 of the table
+// and a float4 of the column tile per k (tab: nzp x nzp row-major, in LDS or
+// global; tile: [k][MODAL_CSTR] in LDS); f(c, j, value) stores
+template <class F>
+__device__ inline void modal_tiles(const float *tab, const float *tile, int nzp, int nk, F &&f) {
+  const int nq = nzp / 4, nt = (MODAL_COLS / 4) * nq;
+  for (int q = threadIdx.x; q < nt; q += NSX_BLOCK) {
+    const int cq = q / nq, jq = q - cq * nq;
+    float a[4][4] = {};
+    for (int k = 0; k < nk; ++k) {
+      const float4 t = *(const float4 *)(tab + (size_t)k * nzp + 4 * jq);
+      const float4 x = *(const float4 *)(tile + (size_t)k * MODAL_CSTR + 4 * cq);
+      const float tv[4] = {t.x, t.y, t.z, t.w}, xv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) a[cc][jj] = fmaf(tv[jj], xv[cc], a[cc][jj]);
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) f(4 * cq + cc, 4 * jq + jj, a[cc][jj]);
+  }
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_fwd_vec(Geo g, ModalDev M, const float *r,
+                                                              const SolverState *st) {
+  if (st->m_stop) return;
+  extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
+  float *lds = reinterpret_cast<float *>(nsx_dyn_lds);
+  const int nzp = M.nzp, nzi = M.nzi, m0 = M.m0, MP = M.mp;
+  const bool in_lds = modal_lds_fits(M);
+  const float *ph = in_lds ? lds : M.phi;
+  float *rc = in_lds ? lds + (size_t)nzp * nzp : lds;
+  if (in_lds)
+    for (int q = threadIdx.x; q < nzp * nzp; q += NSX_BLOCK) lds[q] = M.phi[q];
+  const int ngrp = (g.N0 + MODAL_COLS - 1) / MODAL_COLS;
+  for (int grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+    __syncthreads();
+    const int c0 = grp * MODAL_COLS;
+    for (int q = threadIdx.x; q < MODAL_COLS * nzp; q += NSX_BLOCK) {
+      const int c = q / nzp, k = q - c * nzp, i = c0 + c;
+      rc[(size_t)k * MODAL_CSTR + c] = (i < g.N0 && k < nzi) ? r[(size_t)i * g.NS + k + 1] : 0.f;
+    }
+    __syncthreads();
+    modal_tiles(ph, rc, nzp, nzi, [&](int c, int m, float v) {
+      const int i = c0 + c;
+      if (i >= g.N0 || m >= nzi) return;
+      const float d = M.lev[0].diag[i] + M.mu[m] * M.lev[0].area[i];
+      if (m < m0) {
+        M.b[0][(size_t)i * m0 + m] = v;
+        M.z[0][(size_t)i * m0 + m] = v / d;
+      } else {
+        M.xh[(size_t)i * MP + m] = v / d;
+      }
+    });
+  }
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_bwd_vec(Geo g, ModalDev M, const float *r, float *u, double *P,
+                                                              const SolverState *st) {
+  if (st->m_stop) return;
+  extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
+  float *lds = reinterpret_cast<float *>(nsx_dyn_lds);
+  const int nzp = M.nzp, nzi = M.nzi, MP = M.mp;
+  const bool in_lds = modal_lds_fits(M);
+  const float *pt = in_lds ? lds : M.phiT;
+  float *xc = in_lds ? lds + (size_t)nzp * nzp : lds;
+  if (in_lds)
+    for (int q = threadIdx.x; q < nzp * nzp; q += NSX_BLOCK) lds[q] = M.phiT[q];
+  double acc[1] = {0.0};
+  const int ngrp = (g.N0 + MODAL_COLS - 1) / MODAL_COLS;
+  for (int grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+    __syncthreads();
+    const int c0 = grp * MODAL_COLS;
+    for (int q = threadIdx.x; q < MODAL_COLS * nzp; q += NSX_BLOCK) {
+      const int c = q / nzp, m = q - c * nzp, i = c0 + c;
+      xc[(size_t)m * MODAL_CSTR + c] = (i < g.N0 && m < nzi) ? M.xh[(size_t)i * MP + m] : 0.f;
+    }
+    __syncthreads();
+    modal_tiles(pt, xc, nzp, nzi, [&](int c, int k, float v) {
+      const int i = c0 + c;
+      if (i >= g.N0 || k >= nzi) return;
+      const size_t s = (size_t)i * g.NS + k + 1;
+      u[s] = 0.f - v;
+      acc[0] += (double)r[s] * (double)(0.f - v);
+    });
+  }
+  // the slots outside the interior layers
+  const uint32_t nout = (uint32_t)(g.NS - nzi);
+  for (uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x; t < (uint32_t)g.N0 * nout; t += gridDim.x * NSX_BLOCK) {
+    const uint32_t i = t / nout, j = t - i * nout;
+    const int k = j == 0 ? 0 : nzi + (int)j;
+    u[(size_t)i * g.NS + k] = 0.f;
+  }
+  write_partials<1>(acc, P);
+}
+
+// --- the batched 2D V-cycle on n_l x m0 arrays (node-major, mode fastest):
+// the operator of mode m at level l is d_i(m) - W, d_i(m) = diag_i + mu_m
+// area_i.  One damped Jacobi sweep from zero before the coarse correction
+// (x = omega z, z = b / d, written by the producer of b) and one after it; the
+// restriction forms the residual of the first where it needs it, and the
+// post-smoothing forms the prolonged y = omega z + P x_c at the node and its
+// neighbours: one launch per level each way.  Rows with many entries are
+// split over S slices of lanes (lane m + 16 s), summed by shuffles in a fixed
+// order.
+constexpr float MG_OMEGA = 0.8f;
+
+template <int S>
+__device__ inline float slice_sum(float v) {
+  if (S >= 2) v += __shfl_xor(v, 16, 64);
+  if (S >= 4) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// b_c[J] = sum over the children i of J of (b - A x)_i with x = omega z:
+// (1 - omega) sum_i b_i + omega sum_j RW[J][j] z_j (RW = R W, the weights of
+// the children summed per fine neighbour); z_c = b_c / d_J.  A wave per J.
+__global__ __launch_bounds__(NSX_BLOCK) void k_mg_down(ModalLevelDev C, const float *mu, int m0, const float *bf,
+                                                        const float *zf, float *bc, float *zc, const SolverState *st) {
+  if (st->m_stop) return;
+  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  const int J = (int)(t >> 6), m = (int)(t & 15), sl = (int)((t >> 4) & 3);
+  if (J >= C.n) return;  // whole waves
+  float v = 0.f;
+  if (m < m0) {
+    for (int q = C.cptr[J] + sl; q < C.cptr[J + 1]; q += 4) v += bf[(size_t)C.child[q] * m0 + m];
+    float a = 0.f;
+    for (int e = C.rptr[J] + sl; e < C.rptr[J + 1]; e += 4) a = fmaf(C.rw[e], zf[(size_t)C.rcol[e] * m0 + m], a);
+    v = fmaf(MG_OMEGA, a, (1.f - MG_OMEGA) * v);
+  }
+  v = slice_sum<4>(v);
+  if (sl == 0 && m < m0) {
+    bc[(size_t)J * m0 + m] = v;
+    zc[(size_t)J * m0 + m] = v / (C.diag[J] + mu[m] * C.area[J]);
+  }
+}
+
+// x = C^-1 b on the coarsest level, C^-1 laid out [J][K][m]: a block per J,
+// 16 slices of K
+__global__ __launch_bounds__(NSX_BLOCK) void k_mg_coarse(int nc, int m0, const float *cinvT, const float *b, float *x,
+                                                          int ostride, const SolverState *st) {
+  if (st->m_stop) return;
+  __shared__ float sh[NSX_BLOCK / 64][16];
+  const int J = blockIdx.x, m = threadIdx.x & 15, sl = threadIdx.x >> 4;
+  float v = 0.f;
+  if (m < m0) {
+    const float *row = cinvT + (size_t)J * nc * m0 + m;
+    for (int K = sl; K < nc; K += NSX_BLOCK / 16) v = fmaf(row[(size_t)K * m0], b[(size_t)K * m0 + m], v);
+  }
+  v = slice_sum<4>(v);
+  if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][m] = v;
+  __syncthreads();
+  if (threadIdx.x < 16 && m < m0) {
+    float s = 0.f;
+    for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[w][m];
+    x[(size_t)J * ostride + m] = s;
+  }
+}
+
+// x = y + omega (b - A y) / d with y = omega z + x_c(agg) at the node and its
+// neighbours; S slices per row (S = 4: a wave per row)
+template <int S>
+__global__ __launch_bounds__(NSX_BLOCK) void k_mg_up(ModalLevelDev F, const float *mu, int m0, const float *xc,
+                                                      const float *bf, const float *zf, float *xf, int ostride,
+                                                      const SolverState *st) {
+  if (st->m_stop) return;
+  const uint32_t t = blockIdx.x * NSX_BLOCK + threadIdx.x;
+  const int m = (int)(t & 15), sl = (int)((t >> 4) & (S - 1));
+  const uint32_t i = t / (16 * S);
+  if (S > 1 && i >= (uint32_t)F.n) return;  // whole waves
+  const bool act = i < (uint32_t)F.n && m < m0;
+  float acc = 0.f;
+  if (act)
+    for (int e = F.ptr[i] + sl; e < F.ptr[i + 1]; e += S)
+      acc = fmaf(F.w[e], fmaf(MG_OMEGA, zf[(size_t)F.col[e] * m0 + m], xc[(size_t)F.eagg[e] * m0 + m]), acc);
+  acc = slice_sum<S>(acc);
+  if (act && sl == 0) {
+    const size_t q = (size_t)i * m0 + m;
+    const float di = F.diag[i] + mu[m] * F.area[i];
+    const float yi = fmaf(MG_OMEGA, zf[q], xc[(size_t)F.agg[i] * m0 + m]);
+    const float ay = fmaf(di, yi, -acc);
+    xf[(size_t)i * ostride + m] = yi + MG_OMEGA * (bf[q] - ay) / di;
+  }
+}
+
+template <int T, int MT>
+static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
+                                const SolverState *st, bool fwd) {
+  const size_t lds = (size_t)T * MT * 64 * sizeof(float4);
+  if (fwd)
+    hipLaunchKernelGGL((k_modal_fwd_mfma<T, MT>), dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
+  else
+    hipLaunchKernelGGL((k_modal_bwd_mfma<T, MT>), dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, u, P, st);
+}
+
+template <int T>
+static void launch_transforms_mt(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
+                                 const SolverState *st, bool fwd) {
+  if (T > 1 && M.mt == T - 1)
+    launch_transforms_t<T, (T > 1 ? T - 1 : 1)>(g, s, M, r, u, P, st, fwd);
+  else
+    launch_transforms_t<T, T>(g, s, M, r, u, P, st, fwd);
+}
+
+// the matrix-core kernels take NS <= 128 with mp / 16 = T - 1 or T (always
+// the case: nzi = NZ - 2 and NS = NZ rounded up to 4); else the vector units
+static void launch_transform(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
+                             const SolverState *st, bool fwd) {
+  if (M.mt == M.ntile || M.mt == M.ntile - 1) switch (M.ntile) {
+      case 1: return launch_transforms_mt<1>(g, s, M, r, u, P, st, fwd);
+      case 2: return launch_transforms_mt<2>(g, s, M, r, u, P, st, fwd);
+      case 3: return launch_transforms_mt<3>(g, s, M, r, u, P, st, fwd);
+      case 4: return launch_transforms_mt<4>(g, s, M, r, u, P, st, fwd);
+      case 5: return launch_transforms_mt<5>(g, s, M, r, u, P, st, fwd);
+      case 6: return launch_transforms_mt<6>(g, s, M, r, u, P, st, fwd);
+      case 7: return launch_transforms_mt<7>(g, s, M, r, u, P, st, fwd);
+      case 8: return launch_transforms_mt<8>(g, s, M, r, u, P, st, fwd);
+      default: break;
+    }
+  const size_t lds = modal_lds(M);
+  if (fwd)
+    hipLaunchKernelGGL(k_modal_fwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
+  else
+    hipLaunchKernelGGL(k_modal_bwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, u, P, st);
+}
+
+int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
+                       const SolverState *st) {
+  const dim3 blk(NSX_BLOCK);
+  launch_transform(g, s, M, r, u, part.a, st, true);
+  const int m0 = M.m0, L = M.nlev;
+  auto grid = [](size_t threads) { return dim3((unsigned)((threads + NSX_BLOCK - 1) / NSX_BLOCK)); };
+  for (int l = 0; l + 1 < L; ++l)
+    hipLaunchKernelGGL(k_mg_down, grid((size_t)M.lev[l + 1].n * 64), blk, 0, s, M.lev[l + 1], M.mu, m0, M.b[l],
+                       M.z[l], M.b[l + 1], M.z[l + 1], st);
+  // the coarsest solve and the last prolongation write the low modes into xh
+  hipLaunchKernelGGL(k_mg_coarse, dim3(M.nc), blk, 0, s, M.nc, m0, M.cinv, M.b[L - 1], L == 1 ? M.xh : M.x[L - 1],
+                     L == 1 ? M.mp : m0, st);
+  for (int l = L - 2; l >= 0; --l) {
+    float *out = l == 0 ? M.xh : M.x[l];
+    const int os = l == 0 ? M.mp : m0;
+    if (l == 0)
+      hipLaunchKernelGGL(k_mg_up<1>, grid((size_t)M.lev[l].n * 16), blk, 0, s, M.lev[l], M.mu, m0, M.x[l + 1],
+                         M.b[l], M.z[l], out, os, st);
+    else
+      hipLaunchKernelGGL(k_mg_up<4>, grid((size_t)M.lev[l].n * 64), blk, 0, s, M.lev[l], M.mu, m0, M.x[l + 1],
+                         M.b[l], M.z[l], out, os, st);
+  }
+  launch_transform(g, s, M, r, u, part.a, st, false);
+  return (int)hipGetLastError();
+}
+
+// p_out = u + beta p_in at the seven points, w = A p_out, <p, w>
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_spmv(Geo g, const float *u, const float *pin, float *pout,
+                                                           float *w, Partials P, SolverState *st, int first,
+                                                           uint32_t na) {
+  if (st->done || st->m_stop) return;
+  double t[1];
+  reduce_partials<1>(P.a, na, t);
+  const double rho = t[0];
+  // every block forms the same rho, so all return together
+  if (!first && fabs(rho) < NSX_MODAL_DROP * fabs(st->m_rho0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->m_stop = 1;
+    return;
+  }
+  const double beta = first ? 0.0 : rho / st->m_rho_prev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->m_rho = rho;
+    if (first) st->m_rho0 = rho;
+  }
+  float *lvw = load_levels<float>(g);
+  for (int k = threadIdx.x; k < g.NS; k += NSX_BLOCK) lvw[2 * g.NS + k] = 1.f / lvw[k];
+  __syncthreads();
+  const float *lv = lvw;
+  double acc[1] = {0.0};
+  for_each_quad_gathered(
+      g, u, pin, (float)beta, first != 0, [&](size_t) {},
+      [&](int, int k0, size_t s0, const Col<float> &C, const float(&fc)[4], float fm, float fp,
+          const float(&f0)[4], const float(&f1)[4], const float(&f2)[4]) {
+        float qv[5], wo[4], po[4];
+        quot5_r(C, lv, g.NS, k0, qv);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const Row1<float> R = row_of(g, lv, C, qv, k0, v);
+          const float wv = apply_row(g, R, k0, v, fc, fm, fp, f0[v], f1[v], f2[v]);
+          wo[v] = R.in ? wv : 0.f;
+          po[v] = R.in ? fc[v] : 0.f;
+          if (R.in) acc[0] += (double)fc[v] * (double)wv;
+        }
+        st4(pout + s0, po);
+        st4(w + s0, wo);
+      });
+  write_partials<1>(acc, P.b);
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_update(Geo g, const float *p, const float *w, float *e, float *r,
+                                                             Partials P, SolverState *st) {
+  if (st->done || st->m_stop) return;
+  double t[1];
+  reduce_partials<1>(P.b, gridDim.x, t);
+  const float alpha = (float)(st->m_rho / t[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->m_rho_prev = st->m_rho;
+    st->m_iters += 1;
+  }
+  for_each_quad(g, [&](int, int, size_t s0) {
+    float pp[4], ww[4], ee[4], rr[4];
+    ld4(p + s0, pp);
+    ld4(w + s0, ww);
+    ld4(e + s0, ee);
+    ld4(r + s0, rr);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      ee[v] = fmaf(alpha, pp[v], ee[v]);
+      rr[v] = fmaf(-alpha, ww[v], rr[v]);
+    }
+    st4(e + s0, ee);
+    st4(r + s0, rr);
+  });
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_true(Geo g, const double *rhs, const double *x, const float *e,
+                                                           float *r, Partials P, SolverState *st) {
+  if (st->done) return;
+  const double avec = st->avec;
+  double acc[3] = {0.0, 0.0, 0.0};
+  FOR_EACH_CELL(g, c) {
+    int i, k;
+    cell_of(g, c, i, k);
+    double res, xs;
+    true_residual(g, rhs, x, avec, i, k, [&](size_t q, bool) { return e[q]; }, res, xs);
+    r[(size_t)i * g.NS + k] = (float)res;
+    acc[0] += res * res;
+    acc[1] += fabs(res);
+    acc[2] += xs * (g.area[i] * g.dsig[k]);
+  }
+  write_partials<3>(acc, P.c);
+}
+
+__global__ __launch_bounds__(NSX_BLOCK) void k_modal_fold(Geo g, double *x, float *e, Partials P, SolverState *st,
+                                                           int cap) {
+  if (st->done) return;
+  const int iters = st->m_iters;
+  double t[3];
+  reduce_partials<3>(P.c, gridDim.x, t);
+  const double rN = sqrt(t[0]), e1 = t[1] / st->b1, e2 = rN / st->b2;
+  const bool finite = isfinite(t[0]) && isfinite(t[1]);
+  int stop = 0, code = EXIT_CONVERGED;
+  if (!finite) {
+    stop = 1;
+    code = EXIT_BREAKDOWN;
+  } else if (e1 < st->eps || e2 < st->eps) {
+    stop = 1;
+  } else if (iters >= cap) {
+    stop = 1;
+    code = EXIT_MAX;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->e1 = e1;
+    st->e2 = e2;
+    st->xvol = t[2];
+    st->a_iter = iters;
+    st->mp_pass += 1;
+    st->m_stop = 0;
+    if (stop) {
+      st->done = 1;
+      st->exit_code = code;
+    }
+  }
+  // x += e, e = 0 (x keeps its value when the residual was not finite)
+  for_each_quad(g, [&](int, int k0, size_t s0) {
+    float ee[4];
+    ld4(e + s0, ee);
+    if (finite)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (k0 + v >= 1 && k0 + v <= g.NZ - 2) x[s0 + v] = x[s0 + v] + (double)ee[v];
+    const float z[4] = {0.f, 0.f, 0.f, 0.f};
+    st4(e + s0, z);
+  });
+}
+
+int launch_modal_spmv(const Geo &g, hipStream_t s, const ModalDev &M, const float *u, const float *pin, float *pout,
+                      float *w, Partials part, SolverState *st, int first) {
+  hipLaunchKernelGGL(k_modal_spmv, dim3(g.grid), dim3(NSX_BLOCK), cgs_lds(g), s, g, u, pin, pout, w, part, st,
+                     first, (uint32_t)M.agrid);
+  return (int)hipGetLastError();
+}
+
+int launch_modal_update(const Geo &g, hipStream_t s, const float *p, const float *w, float *e, float *r,
+                        Partials part, SolverState *st) {
+  hipLaunchKernelGGL(k_modal_update, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, p, w, e, r, part, st);
+  return (int)hipGetLastError();
+}
+
+int launch_modal_true(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, Partials part,
+                      SolverState *st, int cap) {
+  hipLaunchKernelGGL(k_modal_true, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, rhs, (const double *)x, (const float *)e,
+                     r, part, st);
+  hipLaunchKernelGGL(k_modal_fold, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, x, e, part, st, cap);
+  return (int)hipGetLastError();
 }

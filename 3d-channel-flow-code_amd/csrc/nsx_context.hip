@@ -112,6 +112,12 @@ struct nsx_ctx {
   int hslot[2] = {0, 0}, nhist[2] = {0, 0};
   double *pguess = nullptr;
   int guess_stage = -1;
+  // the vertical-mode preconditioner (NSX_REFINE_MODAL): the column graph
+  // kept from nsx_commit, the device tables once built
+  std::vector<int> h_nbr;
+  std::vector<double> h_ppec;
+  ModalDev modal{};
+  bool modal_ok = false;
   int iter_hint_cg = 0;
   int iter_hint_cgs = 0;
   int iter_hint_cgs64 = 0;
@@ -483,6 +489,8 @@ extern "C" int nsx_commit(nsx_ctx *ctx) {
       int jc = ctx->No_cp[(size_t)i + (size_t)j * NC] - 1;
       ppec[(size_t)i * 3 + j] = (ctx->bcc[3][jc] == 1) ? 0.0 : co1 * ctx->dlVV[f];
     }
+  ctx->h_nbr = nbr;
+  ctx->h_ppec = ppec;
   // A^T: the lateral coefficient of row i towards neighbour n is row n's
   // coefficient towards i (the edge of n whose neighbour is i; where several
   // edges of n lead to i, the one with the closest coefficient)
@@ -1080,11 +1088,198 @@ static int solve_cgs(nsx_ctx *ctx, const double *rhs, const double *phi_in, doub
   return 0;
 }
 
+// the vertical-mode preconditioner's tables (nsx_modal.hip), built on the
+// host and uploaded at the first solve that uses them
+static int modal_setup(nsx_ctx *ctx) {
+  if (ctx->modal_ok) return 0;
+  const Geo &g = ctx->g;
+  const int N0 = ctx->N0, NZ = ctx->NZ, nzi = NZ - 2;
+  // vertical couplings per unit area: between interior layers k and k+1,
+  // 1/dsig(k) (the quotient area/dsig(k) of the rows); a coupling to a ghost
+  // layer that is not Neumann stays on the diagonal
+  std::vector<double> tlay((size_t)nzi + 1, 0.0), svl((size_t)nzi);
+  for (int k = 1; k <= NZ - 3; ++k) tlay[k] = 1.0 / ctx->dsig[k];
+  tlay[0] = g.bcbot == 1 ? 0.0 : 1.0 / ctx->dsig[0];
+  tlay[nzi] = g.bctop == 1 ? 0.0 : 1.0 / ctx->dsig[NZ - 2];
+  for (int k = 1; k <= NZ - 2; ++k) svl[k - 1] = ctx->dsigv[k - 1];
+  ModalHost H;
+  std::string err;
+  const int m0 = std::min(NSX_MODAL_M0, nzi);
+  if (modal_build_host(N0, nzi, ctx->h_nbr.data(), ctx->h_ppec.data(), ctx->areaCell.data(), tlay.data(),
+                       svl.data(), m0, H, err))
+    return nsx_fail(ctx, -1, "%s", err.c_str());
+  ModalDev &M = ctx->modal;
+  M = ModalDev{};
+  const int nzp = (nzi + 3) / 4 * 4, MP = (nzi + 15) / 16 * 16, MT = MP / 16, T = (g.NS + 15) / 16;
+  M.m0 = H.m0;
+  M.nzi = nzi;
+  M.nzp = nzp;
+  M.mp = MP;
+  M.mt = MT;
+  M.ntile = T;
+  M.nlev = (int)H.levels.size();
+  M.nc = H.nc;
+  {
+    // transform blocks: two per CU, four waves of 16 columns each
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, ctx->dev));
+    const int waves = (N0 + 15) / 16, want = (waves + 3) / 4;
+    M.agrid = std::max(1, std::min(want, 2 * prop.multiProcessorCount));
+  }
+  // the tables: phi' (slot, m) = phi(slot - 1, m) on the interior slots
+  auto phis = [&](int slot, int m) -> float {
+    return (slot >= 1 && slot <= nzi && m < nzi) ? H.phi[(size_t)(slot - 1) * nzi + m] : 0.f;
+  };
+  std::vector<float> ph((size_t)nzp * nzp, 0.f), pt((size_t)nzp * nzp, 0.f), mu((size_t)MP, 1.f);
+  for (int k = 0; k < nzi; ++k)
+    for (int m = 0; m < nzi; ++m) {
+      ph[(size_t)k * nzp + m] = H.phi[(size_t)k * nzi + m];
+      pt[(size_t)m * nzp + k] = H.phi[(size_t)k * nzi + m];
+    }
+  for (int m = 0; m < nzi; ++m) mu[m] = H.mu[m];
+  // matrix-core operands: forward, lane (j, q4) of (t, mode tile mt) holds
+  // phi'(16 t + 4 q4 + q, 16 mt + j); backward, of (mode group t, slot tile
+  // nt), phi'(16 nt + j, 16 t + 4 q4 + q)
+  std::vector<float> tf((size_t)T * MT * 256, 0.f), tb((size_t)MT * T * 256, 0.f);
+  for (int t = 0; t < T; ++t)
+    for (int mt = 0; mt < MT; ++mt)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int q = 0; q < 4; ++q) {
+          const int j = lane & 15, q4 = lane >> 4;
+          tf[(((size_t)t * MT + mt) * 64 + lane) * 4 + q] = phis(16 * t + 4 * q4 + q, 16 * mt + j);
+        }
+  for (int t = 0; t < MT; ++t)
+    for (int nt = 0; nt < T; ++nt)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int q = 0; q < 4; ++q) {
+          const int j = lane & 15, q4 = lane >> 4;
+          tb[(((size_t)t * T + nt) * 64 + lane) * 4 + q] = phis(16 * nt + j, 16 * t + 4 * q4 + q);
+        }
+  int rc;
+  float *d_ph, *d_pt, *d_mu, *d_ci, *d_xh, *d_tf, *d_tb;
+  if ((rc = dev_upload(ctx, &d_ph, ph)) || (rc = dev_upload(ctx, &d_pt, pt)) || (rc = dev_upload(ctx, &d_mu, mu)) ||
+      (rc = dev_upload(ctx, &d_tf, tf)) || (rc = dev_upload(ctx, &d_tb, tb)) ||
+      (rc = dev_alloc(ctx, &d_xh, (size_t)N0 * MP)))
+    return rc;
+  {
+    // the coarsest inverses laid out [J][K][m] for coalesced reads
+    const int nc = H.nc, m0v = H.m0;
+    std::vector<float> ct((size_t)nc * nc * m0v, 0.f);
+    for (int m = 0; m < m0v; ++m)
+      for (int J = 0; J < nc; ++J)
+        for (int K = 0; K < nc; ++K)
+          ct[((size_t)J * nc + K) * m0v + m] = H.cinv[((size_t)m * nc + J) * nc + K];
+    if ((rc = dev_upload(ctx, &d_ci, ct))) return rc;
+  }
+  HIPCHECK(hipMemset(d_xh, 0, (size_t)N0 * MP * sizeof(float)));
+  M.phi = d_ph;
+  M.phiT = d_pt;
+  M.tfwd = reinterpret_cast<const float4 *>(d_tf);
+  M.tbwd = reinterpret_cast<const float4 *>(d_tb);
+  M.mu = d_mu;
+  M.cinv = d_ci;
+  M.xh = d_xh;
+  for (int l = 0; l < M.nlev; ++l) {
+    const ModalHost::Level &h = H.levels[l];
+    ModalLevelDev &d = M.lev[l];
+    d.n = h.n;
+    auto up_i = [&](const std::vector<int> &v, const int *&dst) -> int {
+      if (v.empty()) return 0;
+      int *q;
+      const int r = dev_upload(ctx, &q, v);
+      dst = q;
+      return r;
+    };
+    auto up_f = [&](const std::vector<float> &v, const float *&dst) -> int {
+      if (v.empty()) return 0;
+      float *q;
+      const int r = dev_upload(ctx, &q, v);
+      dst = q;
+      return r;
+    };
+    if ((rc = up_i(h.ptr, d.ptr)) || (rc = up_i(h.col, d.col)) || (rc = up_f(h.w, d.w)) ||
+        (rc = up_f(h.diag, d.diag)) || (rc = up_f(h.area, d.area)) || (rc = up_i(h.agg, d.agg)) ||
+        (rc = up_i(h.eagg, d.eagg)) || (rc = up_i(h.cptr, d.cptr)) || (rc = up_i(h.child, d.child)) ||
+        (rc = up_i(h.rptr, d.rptr)) || (rc = up_i(h.rcol, d.rcol)) || (rc = up_f(h.rw, d.rw)))
+      return rc;
+    const size_t nm = (size_t)h.n * (size_t)M.m0;
+    if ((rc = dev_alloc(ctx, &M.b[l], nm)) || (rc = dev_alloc(ctx, &M.z[l], nm)) ||
+        (rc = dev_alloc(ctx, &M.x[l], nm)))
+      return rc;
+    HIPCHECK(hipMemset(M.x[l], 0, nm * sizeof(float)));
+  }
+  ctx->modal_ok = true;
+  return 0;
+}
+
+// Mixed precision, NSX_REFINE_MODAL: FP32 CG preconditioned by the vertical
+// modes (launch_modal_apply) in cycles.  e gathers a cycle's correction in
+// FP32; a cycle starts from the true FP64 residual with beta = 0 and ends
+// when rho has fallen by NSX_MODAL_DROP (FP32 recursion holds the residual to
+// about 1e-5 of the cycle's start; keeping p across the update, the next beta
+// would divide the true residual's rho by the drifted recursive one) or after
+// the iterations launched for it.  Then the true FP64 residual of x + e
+// replaces r, e joins x and the reference's stopping rule is tested on it.
+// The cap and a breakdown leave x at its last update and the FP64 BiCGSTAB
+// finishes, as for the other CG forms.
+static int solve_modal(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
+  int rc = modal_setup(ctx);
+  if (rc) return rc;
+  const Geo &g = ctx->g;
+  hipStream_t s = ctx->stream;
+  const int cap = 2 * ctx->MaxIters + 100;
+  // r0 = b - A x0 (FP32 into frhat), e = 0, x = x0, the norms of b and the
+  // reference's no-solve test
+  LAUNCH(launch_cg_init(g, s, rhs, phi_in, x, ctx->fr, ctx->frhat, ctx->fe, ctx->part, ctx->st_dev, ctx->eps,
+                        ctx->delta_cg, cap));
+  float *r = ctx->frhat, *u = ctx->fr, *w = ctx->fv, *e = ctx->fe;
+  float *p[2] = {ctx->fp, ctx->fp2};
+  HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  int launched = 0, cycle = NSX_MODAL_FIRST, before = 0;
+  if (!ctx->st_host->done) {
+    LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
+    for (;;) {
+      for (int c = 0; c < cycle; ++c) {
+        LAUNCH(launch_modal_spmv(g, s, ctx->modal, u, p[launched & 1], p[(launched + 1) & 1], w, ctx->part,
+                                 ctx->st_dev, c == 0));
+        LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev));
+        ++launched;
+        if (c + 1 < cycle) LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
+      }
+      LAUNCH(launch_modal_true(g, s, rhs, x, e, r, ctx->part, ctx->st_dev, cap));
+      HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      if (ctx->st_host->done) break;
+      // the next cycle: as long as this one ran, a few more
+      const int used = ctx->st_host->m_iters - before;
+      before = ctx->st_host->m_iters;
+      cycle = std::min(NSX_MODAL_CYCLE, std::max(8, used + 4));
+      LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
+    }
+  }
+  const SolverState &h = *ctx->st_host;
+  const int it = h.m_iters;
+  if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN)
+    return finish_fp64(ctx, rhs, x, it, h.mp_pass);
+  LAUNCH(launch_gnorm_apply(g, s, x, ctx->st_dev));
+  ctx->info.iters = it;
+  ctx->info.exit_code = h.exit_code;
+  ctx->info.e1norm = h.e1;
+  ctx->info.e2norm = h.e2;
+  ctx->info.n_solves += 1;
+  ctx->info.iters_total += it;
+  ctx->info.outer_passes = h.mp_pass;
+  ctx->info.iters_fp32 = it;
+  return 0;
+}
+
 static int solve(nsx_ctx *ctx, const double *rhs, const double *phi_in, double *x) {
   if (ctx->precision == NSX_PREC_FP64_CG) return solve_cgs(ctx, rhs, phi_in, x, true);
   if (ctx->precision != NSX_PREC_MIXED && ctx->precision != NSX_PREC_MIXED_FLUX32)
     return solve_fp64(ctx, rhs, phi_in, x);
   if (ctx->refine == NSX_REFINE_CG) return solve_cg(ctx, rhs, phi_in, x);
+  if (ctx->refine == NSX_REFINE_MODAL) return solve_modal(ctx, rhs, phi_in, x);
   return solve_cgs(ctx, rhs, phi_in, x);
 }
 
@@ -2295,9 +2490,9 @@ extern "C" int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double
                     "operator; use NSX_REFINE_CGSR_BLOCK",
                     refine);
   if (refine != NSX_REFINE_CG && refine != NSX_REFINE_CGSR && refine != NSX_REFINE_CGSR_BLOCK &&
-      refine != NSX_REFINE_CGSR_LINE)
+      refine != NSX_REFINE_CGSR_LINE && refine != NSX_REFINE_MODAL)
     return nsx_fail(ctx, -1, "unknown refinement %d", refine);
-  const bool sr = refine != NSX_REFINE_CG;
+  const bool sr = refine != NSX_REFINE_CG && refine != NSX_REFINE_MODAL;
   // a window of the line preconditioner holds whole columns of four-layer quads
   if (refine == NSX_REFINE_CGSR_LINE && ctx->g.NQ > NSX_BLOCK)
     return nsx_fail(ctx, -1, "line preconditioner: %d layers per column exceed %d", ctx->g.NS, 4 * NSX_BLOCK);

@@ -2,6 +2,7 @@
 // pressure solver (nsx_bicgstab.hip), called by the context (nsx_context.hip).
 #pragma once
 #include "nsx_dev.h"
+#include "nsx_modal.h"
 
 // Per-block partial sums of one reduction site: [rows][grid]
 struct Partials {
@@ -57,6 +58,12 @@ struct SolverState {
   CgsCarry cgs[2];   // single-reduction FP32 CG, by parity of the launch
   int cgs_runs;      // launches of the single-reduction CG that ran (block 0 of each)
   int pad3;
+  // the CG with the vertical-mode preconditioner (NSX_REFINE_MODAL)
+  double m_rho;       // rho_i = <r_i, M^-1 r_i> of the iteration running     (product kernel)
+  double m_rho_prev;  // rho of the last completed iteration                   (update kernel)
+  double m_rho0;      // rho of the cycle's first iteration                    (product kernel)
+  int m_stop;         // the cycle has met its drop: its remaining launches return (product kernel; fold: 0)
+  int m_iters;        // iterations completed                                   (update kernel)
 };
 
 // the vectors of the single-reduction CG: e and p in place, the scaled
@@ -185,3 +192,24 @@ int launch_cgs64(const Geo &g, hipStream_t s, int j, const double *rhs, double *
                  Partials part, SolverState *st, hipEvent_t *ev);
 int launch_gnorm_apply(const Geo &g, hipStream_t s, double *x, const SolverState *st);
 int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part);
+
+// ---- the CG with the vertical-mode preconditioner (NSX_REFINE_MODAL)
+// u = M^-1 r (modal transform, the low modes' V-cycle, the others' diagonal,
+// transform back; u is stored at every slot, zero outside the interior) and
+// the partial sums of <r, u> into P.a
+int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
+                       const SolverState *st);
+// p_out = u + beta p_in (beta = 0 when first: the cycle's first iteration) at
+// the seven points, w = A p_out, partials of <p, w> into P.b; rho from P.a
+// (M.agrid partials) kept in the state; a rho below NSX_MODAL_DROP times the
+// cycle's first ends the cycle (st->m_stop)
+int launch_modal_spmv(const Geo &g, hipStream_t s, const ModalDev &M, const float *u, const float *pin, float *pout,
+                      float *w, Partials part, SolverState *st, int first);
+// alpha = rho / <p, w>; e += alpha p, r -= alpha w
+int launch_modal_update(const Geo &g, hipStream_t s, const float *p, const float *w, float *e, float *r,
+                        Partials part, SolverState *st);
+// the true FP64 residual of x + e into r and its partials (P.c); then x += e,
+// e = 0, a new cycle (m_stop = 0) and the reference's stopping rule (st: e1,
+// e2, xvol, done, exit code; the iteration cap after st->m_iters iterations)
+int launch_modal_true(const Geo &g, hipStream_t s, const double *rhs, double *x, float *e, float *r, Partials part,
+                      SolverState *st, int cap);

@@ -30,7 +30,7 @@
            NSX_PREC_FP64_CG = 2, NSX_PREC_MIXED_FLUX32 = 3
       integer(c_int), parameter :: NSX_REFINE_CG = 3,                   &
            NSX_REFINE_CGSR = 4, NSX_REFINE_CGSR_BLOCK = 5,               &
-           NSX_REFINE_CGSR_LINE = 6
+           NSX_REFINE_CGSR_LINE = 6, NSX_REFINE_MODAL = 7
 !     initial guess of nsx_step's pressure solves (enum nsx_guess)
       integer(c_int), parameter :: NSX_GUESS_PREVIOUS = 0,              &
            NSX_GUESS_EXTRAPOLATE = 1

@@ -30,6 +30,7 @@ REFINE_CG = 3
 REFINE_CGSR = 4
 REFINE_CGSR_BLOCK = 5
 REFINE_CGSR_LINE = 6
+REFINE_MODAL = 7
 
 SOLVE_EXIT = {0: "converged", 1: "no solve", 2: "breakdown", 3: "max iterations"}
 

@@ -118,7 +118,7 @@ def cpu_baseline(case, mean_iters: float, cap: int = 100, parallel: bool = False
 # columns), "mixed_point" with point Jacobi; "mixed_cg2" the point-Jacobi CG
 # in two kernels per iteration
 MIXED_MODES = {"mixed": (5, 0.1), "mixed_line": (6, 0.1), "mixed_point": (4, 0.1), "mixed_cg2": (3, 0.1),
-               "mixed_flux32": (5, 0.1)}
+               "mixed_flux32": (5, 0.1), "mixed_modal": (7, 0.1)}
 # the FP32 CG kernels: W reads rt, p and writes w; X reads e, rt, p, w and
 # writes e, rt, p (units of 4 B per interior prism; the few X launches that
 # form a true FP64 residual move more)

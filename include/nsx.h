@@ -356,13 +356,23 @@ int nsx_write_tec_vc(const char *dir, int counter, double time, int N_CELL0, int
  *                          to the interior layers of each column (a
  *                          tridiagonal solve over the column in the kernel's
  *                          shared memory; columns of at most 1024 layers).
+ *   NSX_REFINE_MODAL       CG preconditioned by the vertical modes of the
+ *                          operator: per column, the generalized eigenvectors
+ *                          of the vertical couplings split A into one 2D
+ *                          problem per mode; the 16 lowest get a V-cycle of
+ *                          aggregation multigrid on the column graph, the
+ *                          others their diagonal.  The FP32 CG runs in
+ *                          cycles that start from the true FP64 residual and
+ *                          end when <r, M^-1 r> has fallen by 1e-10 (tol
+ *                          unused).  Its tables are built at the first solve.
  * Values 0..2 (FP32 BiCGSTAB forms) are refused: FP32 BiCGSTAB breaks down on
  * this operator.  max_passes is unused (at least 1). */
 enum nsx_refine {
   NSX_REFINE_CG = 3,
   NSX_REFINE_CGSR = 4,
   NSX_REFINE_CGSR_BLOCK = 5,
-  NSX_REFINE_CGSR_LINE = 6
+  NSX_REFINE_CGSR_LINE = 6,
+  NSX_REFINE_MODAL = 7
 };
 int nsx_set_precision(nsx_ctx *ctx, int precision, int refine, double tol, int max_passes);
 
