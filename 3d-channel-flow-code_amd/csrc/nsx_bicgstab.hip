@@ -2031,13 +2031,14 @@ int solver_blocks_per_cu(const Geo &g) {
 // on the matrix cores (v_mfma_f32_16x16x4_f32, FP32 in and out) when the
 // columns have at most 128 slots, else on the vector units.
 
-// --- matrix-core transforms.  A wave forms 16 columns x all modes (or slots):
-// its lane (j, g) = (lane & 15, lane >> 4) loads float4s of column c0 + j at
-// slots 16 t + 4 g .. +3, which are the A operand of four MFMAs (k index
-// 4 g + q of step (t, q)); the table's float4 for (t, tile, lane) holds the
-// matching B operands, laid out by the host so that one ds_read_b128 per lane
-// fetches them.  The result D[i][j] of a 16 x 16 tile sits in lane j + 16 (i /
-// 4), register i % 4.
+// --- matrix-core transforms.  A wave forms 16 columns x all modes (or
+// interior layers): its lane (j, g) = (lane & 15, lane >> 4) holds float4s of
+// column c0 + j at interior layers (or modes) 16 t + 4 g .. +3, which are the
+// A operand of four MFMAs (k index 4 g + q of step (t, q)); the table's float4
+// for (t, tile, lane) holds the matching B operands, laid out by the host so
+// that one ds_read_b128 per lane fetches them.  The result D[i][j] of a 16 x
+// 16 tile sits in lane j + 16 (i / 4), register i % 4.  Layers and modes are
+// both padded to mp = 16 MT (zero rows and columns of the table).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline f32x4 mfma4(const float4 &a, const float4 &b, f32x4 c) {
@@ -2047,34 +2048,54 @@ __device__ inline f32x4 mfma4(const float4 &a, const float4 &b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
 }
 
-// the operand table (n <= 64 T * T float4s) into LDS, every load issued
-// before the first store
-template <int T>
-__device__ inline void lds_table(float4 *dst, const float4 *src, int n) {
-  constexpr int U = (64 * T * T + NSX_BLOCK - 1) / NSX_BLOCK;
+// the transform kernels' blocks: 4 waves share one operand table in LDS
+constexpr int MODAL_TB = 256;
+
+// the operand table (64 MT^2 float4s) into LDS, every load issued before the
+// first store
+template <int MT>
+__device__ inline void lds_table(float4 *dst, const float4 *src) {
+  constexpr int n = 64 * MT * MT, U = (n + MODAL_TB - 1) / MODAL_TB;
   float4 v[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int q = threadIdx.x + u * NSX_BLOCK;
+    const int q = threadIdx.x + u * MODAL_TB;
     if (q < n) v[u] = src[q];
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int q = threadIdx.x + u * NSX_BLOCK;
+    const int q = threadIdx.x + u * MODAL_TB;
     if (q < n) dst[q] = v[u];
   }
   __syncthreads();
 }
 
-// the A operands of a wave's tile: column c0 + (lane & 15), slots 16 t + 4
-// (lane >> 4) (raw: the slots outside the interior are masked where used)
-template <int T>
-__device__ inline void load_cols(const float *r, int NS, int N0, int c, int gq, float4 (&a)[T]) {
+// LDS written by some lanes of a wave, then read by others: ordered at the
+// wavefront's scope
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the A operands of the forward transform: interior layers 16 t + 4 g .. +3 of
+// column c (slots one above; four-byte aligned, so one unaligned dwordx4
+// where all four are interior, single loads at the top, zero above nzi)
+template <int MT>
+__device__ inline void load_cols(const float *r, int NS, int N0, int nzi, int c, int gq, float4 (&a)[MT]) {
 #pragma unroll
-  for (int t = 0; t < T; ++t) {
+  for (int t = 0; t < MT; ++t) {
     const int k = 16 * t + 4 * gq;
     a[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (c < N0 && k < NS) a[t] = *(const float4 *)(r + (size_t)c * NS + k);
+    if (c >= N0) continue;
+    const float *src = r + (size_t)c * NS + 1 + k;
+    if (k + 4 <= nzi) {
+      __builtin_memcpy(&a[t], src, sizeof(float4));
+    } else {
+      if (k < nzi) a[t].x = src[0];
+      if (k + 1 < nzi) a[t].y = src[1];
+      if (k + 2 < nzi) a[t].z = src[2];
+    }
   }
 }
 
@@ -2087,40 +2108,33 @@ __device__ inline void load_modes(const float *xh, int N0, int c, int gq, float4
   }
 }
 
-// T: slot tiles of 16 (NS <= 16 T), MT = mp / 16 mode tiles (T - 1 or T).
-// Two blocks a CU (two waves a SIMD), each wave loads its next tile's
-// operands before it forms the current one.
-template <int T, int MT>
-__global__ __launch_bounds__(NSX_BLOCK, 2) void k_modal_fwd_mfma(Geo g, ModalDev M, const float *r,
-                                                                  const SolverState *st) {
+// MT = mp / 16.  Two blocks of 4 waves a CU; a wave with more than one tile
+// loads its next tile's operands before it forms the current one.
+template <int MT>
+__global__ __launch_bounds__(MODAL_TB, 1) void k_modal_fwd_mfma(Geo g, ModalDev M, const float *r,
+                                                                 const SolverState *st) {
   if (st->m_stop) return;
   extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
   float4 *tab = reinterpret_cast<float4 *>(nsx_dyn_lds);
   const int m0 = M.m0, nzi = M.nzi;
-  lds_table<T>(tab, M.tfwd, T * MT * 64);
+  lds_table<MT>(tab, M.tfwd);
   const int lane = threadIdx.x & 63, j = lane & 15, gq = lane >> 4;
-  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (NSX_BLOCK / 64);
-  int tile = blockIdx.x * (NSX_BLOCK / 64) + (threadIdx.x >> 6);
-  float4 a[T], an[T];
-  load_cols<T>(r, g.NS, g.N0, tile * 16 + j, gq, a);
+  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (MODAL_TB / 64);
+  int tile = blockIdx.x * (MODAL_TB / 64) + (threadIdx.x >> 6);
+  float4 a[MT], an[MT];
+  load_cols<MT>(r, g.NS, g.N0, nzi, tile * 16 + j, gq, a);
   for (; tile < ntile; tile += stride) {
-    if (tile + stride < ntile) load_cols<T>(r, g.NS, g.N0, (tile + stride) * 16 + j, gq, an);
+    if (tile + stride < ntile) load_cols<MT>(r, g.NS, g.N0, nzi, (tile + stride) * 16 + j, gq, an);
     const int c0 = tile * 16;
     f32x4 acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // one slot tile at a time: the scheduler keeps a step's table operands
+    // one layer tile at a time: the scheduler keeps a step's table operands
     // (MT float4s) in flight, not the whole table
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const int k = 16 * t + 4 * gq;
-      float4 x = a[t];
-      x.x = (k >= 1 && k <= nzi) ? x.x : 0.f;
-      x.y = (k + 1 <= nzi) ? x.y : 0.f;
-      x.z = (k + 2 <= nzi) ? x.z : 0.f;
-      x.w = (k + 3 <= nzi) ? x.w : 0.f;
+    for (int t = 0; t < MT; ++t) {
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) acc[mt] = mfma4(x, tab[(t * MT + mt) * 64 + lane], acc[mt]);
+      for (int mt = 0; mt < MT; ++mt) acc[mt] = mfma4(a[t], tab[(t * MT + mt) * 64 + lane], acc[mt]);
       __builtin_amdgcn_sched_barrier(0);
     }
     float dg[4], ar[4];
@@ -2148,59 +2162,97 @@ __global__ __launch_bounds__(NSX_BLOCK, 2) void k_modal_fwd_mfma(Geo g, ModalDev
       }
     }
 #pragma unroll
-    for (int t = 0; t < T; ++t) a[t] = an[t];
+    for (int t = 0; t < MT; ++t) a[t] = an[t];
   }
 }
 
-// u = -Phi x from the modal coefficients xh (N0 x mp), every slot of the
-// column (zero outside the interior); partial sums of <r, u>
-template <int T, int MT>
-__global__ __launch_bounds__(NSX_BLOCK, 2) void k_modal_bwd_mfma(Geo g, ModalDev M, const float *r, float *u,
-                                                                  double *P, const SolverState *st) {
+// u = -Phi x from the modal coefficients xh (N0 x mp) at the interior layers,
+// zero at the other slots; partial sums of <r, u>.  STAGE: the wave's 16
+// columns (one contiguous stretch of u) are assembled in LDS and leave, with
+// r read beside them for <r, u>, as aligned float4s (else each lane stores
+// its outputs where they go, in 64-byte runs that straddle two segments).
+template <int MT, bool STAGE>
+__global__ __launch_bounds__(MODAL_TB, 1) void k_modal_bwd_mfma(Geo g, ModalDev M, const float *r, float *u,
+                                                                 double *P, const SolverState *st) {
   if (st->m_stop) return;
   extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
   float4 *tab = reinterpret_cast<float4 *>(nsx_dyn_lds);
-  lds_table<T>(tab, M.tbwd, T * MT * 64);
+  const int nzi = M.nzi, NS = g.NS;
+  lds_table<MT>(tab, M.tbwd);
   const int lane = threadIdx.x & 63, j = lane & 15, gq = lane >> 4;
-  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (NSX_BLOCK / 64);
-  int tile = blockIdx.x * (NSX_BLOCK / 64) + (threadIdx.x >> 6);
+  float *stg = reinterpret_cast<float *>(tab + 64 * MT * MT) + (size_t)(threadIdx.x >> 6) * 16 * NS;
+  const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (MODAL_TB / 64);
+  int tile = blockIdx.x * (MODAL_TB / 64) + (threadIdx.x >> 6);
   double acc2[1] = {0.0};
   float4 a[MT], an[MT];
   load_modes<MT>(M.xh, g.N0, tile * 16 + j, gq, a);
   for (; tile < ntile; tile += stride) {
     if (tile + stride < ntile) load_modes<MT>(M.xh, g.N0, (tile + stride) * 16 + j, gq, an);
     const int c0 = tile * 16;
-    f32x4 acc[T];
+    f32x4 acc[MT];
 #pragma unroll
-    for (int nt = 0; nt < T; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < MT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
 #pragma unroll
-      for (int nt = 0; nt < T; ++nt) acc[nt] = mfma4(a[t], tab[(t * T + nt) * 64 + lane], acc[nt]);
+      for (int nt = 0; nt < MT; ++nt) acc[nt] = mfma4(a[t], tab[(t * MT + nt) * 64 + lane], acc[nt]);
       __builtin_amdgcn_sched_barrier(0);
     }
+    const int nout = NS - nzi;  // slot 0 and nzi + 1 .. NS - 1
+    if (STAGE) {
+      for (int q = lane; q < 16 * nout; q += 64) {
+        const int cl = q / nout, o = q - cl * nout;
+        stg[cl * NS + (o == 0 ? 0 : nzi + o)] = 0.f;
+      }
 #pragma unroll
-    for (int nt = 0; nt < T; ++nt) {
-      const int k = 16 * nt + j;
-      if (k >= g.NS) continue;
-      const bool in = k >= 1 && k <= M.nzi;
+      for (int nt = 0; nt < MT; ++nt) {
+        const int k = 16 * nt + j;
+        if (k >= nzi) continue;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int c = c0 + 4 * gq + q;
-        if (c >= g.N0) continue;
-        const size_t sl = (size_t)c * g.NS + k;
-        const float uv = 0.f - acc[nt][q];
-        u[sl] = uv;
-        if (in) acc2[0] += (double)r[sl] * (double)uv;
+        for (int q = 0; q < 4; ++q) stg[(4 * gq + q) * NS + k + 1] = 0.f - acc[nt][q];
+      }
+      wave_lds_sync();
+      const int ncol = min(16, g.N0 - c0), nf = ncol * NS;
+      float *ut = u + (size_t)c0 * NS;
+      const float *rt = r + (size_t)c0 * NS;
+      for (int o = 4 * lane; o < nf; o += 256) {
+        const float4 uv = *(const float4 *)(stg + o);
+        const float4 rv = *(const float4 *)(rt + o);
+        *(float4 *)(ut + o) = uv;
+        const int s0 = o % NS;  // NS is a multiple of 4: the float4 stays in one column
+        const float uu[4] = {uv.x, uv.y, uv.z, uv.w}, rr[4] = {rv.x, rv.y, rv.z, rv.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (s0 + q >= 1 && s0 + q <= nzi) acc2[0] += (double)rr[q] * (double)uu[q];
+      }
+      wave_lds_sync();
+    } else {
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt) {
+        const int k = 16 * nt + j;
+        if (k >= nzi) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = c0 + 4 * gq + q;
+          if (c >= g.N0) continue;
+          const size_t sl = (size_t)c * NS + k + 1;
+          const float uv = 0.f - acc[nt][q];
+          u[sl] = uv;
+          acc2[0] += (double)r[sl] * (double)uv;
+        }
+      }
+      for (int q = lane; q < 16 * nout; q += 64) {
+        const int c = c0 + q / nout, o = q % nout;
+        if (c < g.N0) u[(size_t)c * NS + (o == 0 ? 0 : nzi + o)] = 0.f;
       }
     }
 #pragma unroll
     for (int t = 0; t < MT; ++t) a[t] = an[t];
   }
-  write_partials<1>(acc2, P);
+  write_partials<1, MODAL_TB>(acc2, P);
 }
 
-// --- vector-unit transforms for columns of more than 128 slots
+// --- vector-unit transforms for columns of more than 128 interior layers
 constexpr int MODAL_COLS = 32;  // columns a block transforms together
 constexpr int MODAL_CSTR = 36;  // LDS row stride of the column tile (float4-aligned, spreads the banks)
 
@@ -2400,40 +2452,38 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_mg_up(ModalLevelDev F, const floa
   }
 }
 
-template <int T, int MT>
+template <int MT>
 static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
                                 const SolverState *st, bool fwd) {
-  const size_t lds = (size_t)T * MT * 64 * sizeof(float4);
-  if (fwd)
-    hipLaunchKernelGGL((k_modal_fwd_mfma<T, MT>), dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
+  const size_t lds = (size_t)MT * MT * 64 * sizeof(float4);
+  if (fwd) {
+    hipLaunchKernelGGL(k_modal_fwd_mfma<MT>, dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
+    return;
+  }
+  // the staged backward transform where table and staging fit 64 KB of LDS
+  const size_t stage = (size_t)(MODAL_TB / 64) * 16 * g.NS * sizeof(float);
+  if (lds + stage <= 65536)
+    hipLaunchKernelGGL((k_modal_bwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r, u, P,
+                       st);
   else
-    hipLaunchKernelGGL((k_modal_bwd_mfma<T, MT>), dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, u, P, st);
+    hipLaunchKernelGGL((k_modal_bwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P, st);
 }
 
-template <int T>
-static void launch_transforms_mt(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
-                                 const SolverState *st, bool fwd) {
-  if (T > 1 && M.mt == T - 1)
-    launch_transforms_t<T, (T > 1 ? T - 1 : 1)>(g, s, M, r, u, P, st, fwd);
-  else
-    launch_transforms_t<T, T>(g, s, M, r, u, P, st, fwd);
-}
-
-// the matrix-core kernels take NS <= 128 with mp / 16 = T - 1 or T (always
-// the case: nzi = NZ - 2 and NS = NZ rounded up to 4); else the vector units
+// the matrix-core kernels take at most 128 interior layers (mp / 16 <= 8);
+// above, the vector units
 static void launch_transform(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
                              const SolverState *st, bool fwd) {
-  if (M.mt == M.ntile || M.mt == M.ntile - 1) switch (M.ntile) {
-      case 1: return launch_transforms_mt<1>(g, s, M, r, u, P, st, fwd);
-      case 2: return launch_transforms_mt<2>(g, s, M, r, u, P, st, fwd);
-      case 3: return launch_transforms_mt<3>(g, s, M, r, u, P, st, fwd);
-      case 4: return launch_transforms_mt<4>(g, s, M, r, u, P, st, fwd);
-      case 5: return launch_transforms_mt<5>(g, s, M, r, u, P, st, fwd);
-      case 6: return launch_transforms_mt<6>(g, s, M, r, u, P, st, fwd);
-      case 7: return launch_transforms_mt<7>(g, s, M, r, u, P, st, fwd);
-      case 8: return launch_transforms_mt<8>(g, s, M, r, u, P, st, fwd);
-      default: break;
-    }
+  switch (M.mt) {
+    case 1: return launch_transforms_t<1>(g, s, M, r, u, P, st, fwd);
+    case 2: return launch_transforms_t<2>(g, s, M, r, u, P, st, fwd);
+    case 3: return launch_transforms_t<3>(g, s, M, r, u, P, st, fwd);
+    case 4: return launch_transforms_t<4>(g, s, M, r, u, P, st, fwd);
+    case 5: return launch_transforms_t<5>(g, s, M, r, u, P, st, fwd);
+    case 6: return launch_transforms_t<6>(g, s, M, r, u, P, st, fwd);
+    case 7: return launch_transforms_t<7>(g, s, M, r, u, P, st, fwd);
+    case 8: return launch_transforms_t<8>(g, s, M, r, u, P, st, fwd);
+    default: break;
+  }
   const size_t lds = modal_lds(M);
   if (fwd)
     hipLaunchKernelGGL(k_modal_fwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
@@ -2442,9 +2492,11 @@ static void launch_transform(const Geo &g, hipStream_t s, const ModalDev &M, con
 }
 
 int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
-                       const SolverState *st) {
+                       const SolverState *st, hipEvent_t *ev) {
   const dim3 blk(NSX_BLOCK);
+  if (ev) (void)hipEventRecord(ev[0], s);
   launch_transform(g, s, M, r, u, part.a, st, true);
+  if (ev) (void)hipEventRecord(ev[1], s);
   const int m0 = M.m0, L = M.nlev;
   auto grid = [](size_t threads) { return dim3((unsigned)((threads + NSX_BLOCK - 1) / NSX_BLOCK)); };
   for (int l = 0; l + 1 < L; ++l)
@@ -2463,7 +2515,9 @@ int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const flo
       hipLaunchKernelGGL(k_mg_up<4>, grid((size_t)M.lev[l].n * 64), blk, 0, s, M.lev[l], M.mu, m0, M.x[l + 1],
                          M.b[l], M.z[l], out, os, st);
   }
+  if (ev) (void)hipEventRecord(ev[2], s);
   launch_transform(g, s, M, r, u, part.a, st, false);
+  if (ev) (void)hipEventRecord(ev[3], s);
   return (int)hipGetLastError();
 }
 

@@ -127,6 +127,7 @@ struct nsx_ctx {
   size_t evused = 0;
   Timer t_k[3], t_stage, t_cg[2];  // FP64 A, B, C; stage kernels; FP32 CG W, X
   Timer t_cgs, t_cgs64;              // single-reduction CG, FP32 and FP64
+  Timer t_mod[5];                    // vertical-mode CG: forward transform, V-cycle, backward, product, update
   // (event index, timer: 0..2 FP64 solver, 5 stage, 12..13 FP32 CG, 14 single-reduction CG, 15 its FP64 instance)
   std::vector<std::pair<size_t, int>> pending;
 };
@@ -870,9 +871,10 @@ static hipEvent_t *events(nsx_ctx *ctx, int n) {
 static void collect_timing(nsx_ctx *ctx) {
   if (!ctx->timing) return;
   (void)hipStreamSynchronize(ctx->stream);
-  Timer *t[16] = {&ctx->t_k[0], &ctx->t_k[1], &ctx->t_k[2],  nullptr,       nullptr,     &ctx->t_stage,
-                  nullptr,       nullptr,       nullptr,        nullptr,       nullptr,     nullptr,
-                  &ctx->t_cg[0], &ctx->t_cg[1], &ctx->t_cgs,    &ctx->t_cgs64};
+  Timer *t[21] = {&ctx->t_k[0],   &ctx->t_k[1],   &ctx->t_k[2],   nullptr,        nullptr,      &ctx->t_stage,
+                  nullptr,         nullptr,         nullptr,         nullptr,         nullptr,      nullptr,
+                  &ctx->t_cg[0],   &ctx->t_cg[1],   &ctx->t_cgs,     &ctx->t_cgs64,   &ctx->t_mod[0],
+                  &ctx->t_mod[1],  &ctx->t_mod[2],  &ctx->t_mod[3],  &ctx->t_mod[4]};
   for (auto &pr : ctx->pending) {
     float ms = 0.f;
     if (t[pr.second] && hipEventElapsedTime(&ms, ctx->evpool[pr.first], ctx->evpool[pr.first + 1]) == hipSuccess) {
@@ -1110,26 +1112,22 @@ static int modal_setup(nsx_ctx *ctx) {
     return nsx_fail(ctx, -1, "%s", err.c_str());
   ModalDev &M = ctx->modal;
   M = ModalDev{};
-  const int nzp = (nzi + 3) / 4 * 4, MP = (nzi + 15) / 16 * 16, MT = MP / 16, T = (g.NS + 15) / 16;
+  const int nzp = (nzi + 3) / 4 * 4, MP = (nzi + 15) / 16 * 16, MT = MP / 16;
   M.m0 = H.m0;
   M.nzi = nzi;
   M.nzp = nzp;
   M.mp = MP;
   M.mt = MT;
-  M.ntile = T;
   M.nlev = (int)H.levels.size();
   M.nc = H.nc;
   {
-    // transform blocks: two per CU, four waves of 16 columns each
+    // transform blocks: two per CU, four waves of 16 columns each (the
+    // vector-unit transforms take the same grid)
     hipDeviceProp_t prop;
     HIPCHECK(hipGetDeviceProperties(&prop, ctx->dev));
     const int waves = (N0 + 15) / 16, want = (waves + 3) / 4;
     M.agrid = std::max(1, std::min(want, 2 * prop.multiProcessorCount));
   }
-  // the tables: phi' (slot, m) = phi(slot - 1, m) on the interior slots
-  auto phis = [&](int slot, int m) -> float {
-    return (slot >= 1 && slot <= nzi && m < nzi) ? H.phi[(size_t)(slot - 1) * nzi + m] : 0.f;
-  };
   std::vector<float> ph((size_t)nzp * nzp, 0.f), pt((size_t)nzp * nzp, 0.f), mu((size_t)MP, 1.f);
   for (int k = 0; k < nzi; ++k)
     for (int m = 0; m < nzi; ++m) {
@@ -1137,23 +1135,18 @@ static int modal_setup(nsx_ctx *ctx) {
       pt[(size_t)m * nzp + k] = H.phi[(size_t)k * nzi + m];
     }
   for (int m = 0; m < nzi; ++m) mu[m] = H.mu[m];
-  // matrix-core operands: forward, lane (j, q4) of (t, mode tile mt) holds
-  // phi'(16 t + 4 q4 + q, 16 mt + j); backward, of (mode group t, slot tile
-  // nt), phi'(16 nt + j, 16 t + 4 q4 + q)
-  std::vector<float> tf((size_t)T * MT * 256, 0.f), tb((size_t)MT * T * 256, 0.f);
-  for (int t = 0; t < T; ++t)
+  // matrix-core operands: forward, lane (j, q4) of (layer group t, mode tile
+  // mt) holds phi(16 t + 4 q4 + q, 16 mt + j); backward, of (mode group t,
+  // layer tile nt), phi(16 nt + j, 16 t + 4 q4 + q)
+  auto phik = [&](int k, int m) -> float { return (k < nzi && m < nzi) ? H.phi[(size_t)k * nzi + m] : 0.f; };
+  std::vector<float> tf((size_t)MT * MT * 256, 0.f), tb((size_t)MT * MT * 256, 0.f);
+  for (int t = 0; t < MT; ++t)
     for (int mt = 0; mt < MT; ++mt)
       for (int lane = 0; lane < 64; ++lane)
         for (int q = 0; q < 4; ++q) {
           const int j = lane & 15, q4 = lane >> 4;
-          tf[(((size_t)t * MT + mt) * 64 + lane) * 4 + q] = phis(16 * t + 4 * q4 + q, 16 * mt + j);
-        }
-  for (int t = 0; t < MT; ++t)
-    for (int nt = 0; nt < T; ++nt)
-      for (int lane = 0; lane < 64; ++lane)
-        for (int q = 0; q < 4; ++q) {
-          const int j = lane & 15, q4 = lane >> 4;
-          tb[(((size_t)t * T + nt) * 64 + lane) * 4 + q] = phis(16 * nt + j, 16 * t + 4 * q4 + q);
+          tf[(((size_t)t * MT + mt) * 64 + lane) * 4 + q] = phik(16 * t + 4 * q4 + q, 16 * mt + j);
+          tb[(((size_t)t * MT + mt) * 64 + lane) * 4 + q] = phik(16 * mt + j, 16 * t + 4 * q4 + q);
         }
   int rc;
   float *d_ph, *d_pt, *d_mu, *d_ci, *d_xh, *d_tf, *d_tb;
@@ -1238,26 +1231,40 @@ static int solve_modal(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
   HIPCHECK(hipStreamSynchronize(s));
   int launched = 0, cycle = NSX_MODAL_FIRST, before = 0;
   if (!ctx->st_host->done) {
-    LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
     for (;;) {
+      const size_t pend0 = ctx->pending.size();
       for (int c = 0; c < cycle; ++c) {
+        // with timing: fwd, V-cycle, bwd, product, update between six events
+        hipEvent_t *ev = nullptr;
+        if (ctx->timing) {
+          ev = events(ctx, 6);
+          if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
+        }
+        LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev, ev));
         LAUNCH(launch_modal_spmv(g, s, ctx->modal, u, p[launched & 1], p[(launched + 1) & 1], w, ctx->part,
                                  ctx->st_dev, c == 0));
+        if (ev) HIPCHECK(hipEventRecord(ev[4], s));
         LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev));
+        if (ev) {
+          HIPCHECK(hipEventRecord(ev[5], s));
+          const size_t base = (size_t)(ev - &ctx->evpool[0]);
+          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, 16 + q});
+        }
         ++launched;
-        if (c + 1 < cycle) LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
       }
       LAUNCH(launch_modal_true(g, s, rhs, x, e, r, ctx->part, ctx->st_dev, cap));
       HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
       HIPCHECK(hipStreamSynchronize(s));
-      if (ctx->st_host->done) break;
-      // the next cycle: as long as this one ran, a few more
       const int used = ctx->st_host->m_iters - before;
       before = ctx->st_host->m_iters;
+      // timing: only the iterations that ran (after a drop the launches return at once)
+      if (ctx->timing && ctx->pending.size() > pend0 + 5 * (size_t)used) ctx->pending.resize(pend0 + 5 * (size_t)used);
+      if (ctx->st_host->done) break;
+      // the next cycle: as long as this one ran, a few more
       cycle = std::min(NSX_MODAL_CYCLE, std::max(8, used + 4));
-      LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev));
     }
   }
+  collect_timing(ctx);
   const SolverState &h = *ctx->st_host;
   const int it = h.m_iters;
   if (h.exit_code == NSX_SOLVE_MAX_ITERS || h.exit_code == NSX_SOLVE_BREAKDOWN)
@@ -2544,6 +2551,9 @@ extern "C" int nsx_timing_get(nsx_ctx *ctx, const char *name, double *total_ms, 
     if (!strcmp(name, names_cg[q])) t = &ctx->t_cg[q];
   if (!strcmp(name, "cg32_sr")) t = &ctx->t_cgs;
   if (!strcmp(name, "cg64_sr")) t = &ctx->t_cgs64;
+  static const char *names_mod[5] = {"modal_fwd", "modal_mg", "modal_bwd", "modal_spmv", "modal_update"};
+  for (int q = 0; q < 5; ++q)
+    if (!strcmp(name, names_mod[q])) t = &ctx->t_mod[q];
   if (!t) return nsx_fail(ctx, -1, "unknown timer %s", name);
   if (total_ms) *total_ms = t->ms;
   if (launches) *launches = t->n;
@@ -2554,5 +2564,6 @@ extern "C" int nsx_timing_reset(nsx_ctx *ctx) {
   if (!ctx) return -1;
   for (int q = 0; q < 3; ++q) ctx->t_k[q] = Timer{};
   ctx->t_stage = ctx->t_cg[0] = ctx->t_cg[1] = ctx->t_cgs = ctx->t_cgs64 = Timer{};
+  for (int q = 0; q < 5; ++q) ctx->t_mod[q] = Timer{};
   return 0;
 }

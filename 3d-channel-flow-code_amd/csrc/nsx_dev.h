@@ -190,9 +190,10 @@ __device__ inline void wave_sum(double (&v)[K]) {
     for (int q = 0; q < K; ++q) v[q] += __shfl_down(v[q], off, 64);
 }
 
-template <int K>
+// (BLK: the block's threads)
+template <int K, int BLK = NSX_BLOCK>
 __device__ inline void write_partials(double (&v)[K], double *P) {
-  __shared__ double sh[K][NSX_BLOCK / 64];
+  __shared__ double sh[K][BLK / 64];
   wave_sum<K>(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0)
@@ -203,7 +204,7 @@ __device__ inline void write_partials(double (&v)[K], double *P) {
 #pragma unroll
     for (int q = 0; q < K; ++q) {
       double s = 0.0;
-      for (int w = 0; w < NSX_BLOCK / 64; ++w) s += sh[q][w];
+      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
       P[(size_t)q * gridDim.x + blockIdx.x] = s;
     }
 }

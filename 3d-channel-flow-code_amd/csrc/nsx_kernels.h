@@ -197,8 +197,10 @@ int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part
 // u = M^-1 r (modal transform, the low modes' V-cycle, the others' diagonal,
 // transform back; u is stored at every slot, zero outside the interior) and
 // the partial sums of <r, u> into P.a
+// (ev: null, or four events recorded before the forward transform, after
+// it, after the V-cycle and after the backward transform)
 int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
-                       const SolverState *st);
+                       const SolverState *st, hipEvent_t *ev = nullptr);
 // p_out = u + beta p_in (beta = 0 when first: the cycle's first iteration) at
 // the seven points, w = A p_out, partials of <p, w> into P.b; rho from P.a
 // (M.agrid partials) kept in the state; a rho below NSX_MODAL_DROP times the

@@ -59,14 +59,13 @@ struct ModalLevelDev {
 struct ModalDev {
   int m0, nzi, nlev, nc;
   int mp;     // modes padded to a multiple of 16 (the row length of xh)
-  int mt;     // mp / 16
-  int ntile;  // slot tiles of 16 of a column (NS <= 16 ntile); above 8 the vector-unit transforms
+  int mt;     // mp / 16; above 8 the vector-unit transforms
   int agrid;  // blocks of the transform kernels (the partial rows of <r, u>)
   int nzp;    // nzi rounded up to a multiple of 4 (the vector-unit tables)
   const float *phi;   // nzp x nzp, [k][m]
   const float *phiT;  // nzp x nzp, [m][k]
-  const float4 *tfwd;  // matrix-core operands of the forward transform, [t][mode tile][lane] float4
-  const float4 *tbwd;  // and of the backward one, [mode group][slot tile][lane]
+  const float4 *tfwd;  // matrix-core operands of the forward transform, [layer group][mode tile][lane] float4
+  const float4 *tbwd;  // and of the backward one, [mode group][layer tile][lane]
   const float *mu;    // mp (padding modes: 1)
   const float *cinv;  // the coarsest level's inverses, [J][K][m]
   ModalLevelDev lev[NSX_MODAL_MAXLEV];
