@@ -13,12 +13,16 @@ stopping error, not FP32's).  The largest workload is checked by properties
 (converged solves, zero volume mean of p, finite and bounded fields), since
 no second implementation finishes it in seconds, and its differentiation
 form by the agreement of the adjoint gradient with the tangent-linear run.
+The gate and the largest workload run for both precision forms: the block
+CG (NSX_REFINE_CGSR_BLOCK) and the vertical-mode CG (NSX_REFINE_MODAL; at
+128 interior layers its transforms take the matrix-core kernels with the
+widest tiles and the backward one without LDS staging).
 """
 import numpy as np
 import pytest
 
 from common import interior, rel_err
-from nsx.abi import CELL_SCRATCH, FACE_SCRATCH, PREC_FP64, PREC_MIXED, REFINE_CGSR_BLOCK
+from nsx.abi import CELL_SCRATCH, FACE_SCRATCH, PREC_FP64, PREC_MIXED, REFINE_CGSR_BLOCK, REFINE_MODAL
 from nsx.case import config_case
 
 pytestmark = pytest.mark.gpu
@@ -35,13 +39,13 @@ def _ctx(case, eps=None):
     return c
 
 
-def _mixed_vs_fp64(cfg, steps, eps):
+def _mixed_vs_fp64(cfg, steps, eps, refine=REFINE_CGSR_BLOCK):
     case = config_case(cfg)
     out = {}
     for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
         G = _ctx(case, eps)
         if prec == PREC_MIXED:
-            G.set_precision(PREC_MIXED, REFINE_CGSR_BLOCK, 0.1, 30)
+            G.set_precision(PREC_MIXED, refine, 0.1, 30)
         G.init_state()
         info = G.step(steps, prec)
         assert info["exit"] == "converged" and info["n_solves"] == 2 * steps, (name, info)
@@ -53,14 +57,16 @@ def _mixed_vs_fp64(cfg, steps, eps):
     return case, out
 
 
-def test_precision_form_meets_gate_256x128x96():
+@pytest.mark.parametrize("refine", [REFINE_CGSR_BLOCK, REFINE_MODAL], ids=["block", "modal"])
+def test_precision_form_meets_gate_256x128x96(refine):
     """Two RK2 steps of the 6.3M-prism channel: the precision form against
     the FP64 path, u, v, w and p within 1e-8 and the same largest cell
     divergence."""
-    case, out = _mixed_vs_fp64(2, 2, EPS_GATE)
+    case, out = _mixed_vs_fp64(2, 2, EPS_GATE, refine)
     a, b = out["fp64"], out["mixed"]
     errs = {n: rel_err(b[n], a[n]) for n in ("UFN", "VFN", "WFN", "PFN")}
-    print("256x128x96 mixed vs fp64 at eps 1e-13:", errs, "max div", a["div"], b["div"], flush=True)
+    print(f"256x128x96 mixed (refinement {refine}) vs fp64 at eps 1e-13:", errs, "max div", a["div"], b["div"],
+          flush=True)
     assert max(errs.values()) <= GATE, errs
     assert abs(b["div"] - a["div"]) <= 1e-6 * a["div"], (a["div"], b["div"])
 
@@ -77,20 +83,22 @@ def test_ten_steps_128x64x64_mixed_vs_fp64():
     assert np.isfinite(u).all() and np.abs(u).max() < 5 * case.arrays["u0"].max()
 
 
-def test_largest_workload_512x256x128_two_steps():
+@pytest.mark.parametrize("refine", [REFINE_CGSR_BLOCK, REFINE_MODAL], ids=["block", "modal"])
+def test_largest_workload_512x256x128_two_steps(refine):
     """Two RK2 steps of the precision form on the 33.5M-prism channel at the
     workload's tolerance: converged solves, a pressure of zero volume mean,
     finite and bounded velocities; the device footprint is printed."""
     case = config_case(4)
     G = _ctx(case)
-    G.set_precision(PREC_MIXED, REFINE_CGSR_BLOCK, 0.1, 30)
+    G.set_precision(PREC_MIXED, refine, 0.1, 30)
     G.init_state()
     info = G.step(2, PREC_MIXED)
     try:
         used = f"{G.device_memory_used() / 2**30:.1f} GiB"
     except Exception as e:  # the footprint is logged, not tested
         used = f"not available ({e})"
-    print(f"512x256x128: {case.n_interior} prisms, solve {info}, device memory in use {used}", flush=True)
+    print(f"512x256x128 (refinement {refine}): {case.n_interior} prisms, solve {info}, device memory in use {used}",
+          flush=True)
     assert info["exit"] == "converged" and info["n_solves"] == 4, info
     assert min(info["e1"], info["e2"]) < case.params.eps, info
     p = interior(case, G.get("PFN"))
