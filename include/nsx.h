@@ -76,8 +76,9 @@ enum nsx_precision {
   NSX_PREC_FP64 = 0,  /* the reference's arithmetic */
   NSX_PREC_MIXED = 1, /* the precision form: FP32 conjugate gradients (SpMV, preconditioner,
                           updates; by default the single-reduction form with the 4-layer
-                          block-Jacobi preconditioner) with reliable updates against the FP64
-                          solution; true residuals, inner products and norms in FP64 */
+                          block-Jacobi preconditioner, or with NSX_REFINE_MODAL the
+                          vertical-mode preconditioner) with reliable updates against the
+                          FP64 solution; true residuals, inner products and norms in FP64 */
   NSX_PREC_FP64_CG = 2, /* the precision form's solver with FP64 vectors: single-reduction CG
                           with the 4-layer block-Jacobi preconditioner and reliable updates,
                           all in FP64 (the refinement set by nsx_set_precision is not used) */
