@@ -2452,51 +2452,51 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_mg_up(ModalLevelDev F, const floa
   }
 }
 
+// (ev: null, or a start/stop pair the dispatch records)
 template <int MT>
 static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
-                                const SolverState *st, bool fwd) {
+                                const SolverState *st, bool fwd, hipEvent_t *ev) {
   const size_t lds = (size_t)MT * MT * 64 * sizeof(float4);
   if (fwd) {
-    hipLaunchKernelGGL(k_modal_fwd_mfma<MT>, dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
+    LAUNCH_TIMED(ev, 0, k_modal_fwd_mfma<MT>, dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
     return;
   }
   // the staged backward transform where table and staging fit 64 KB of LDS
   const size_t stage = (size_t)(MODAL_TB / 64) * 16 * g.NS * sizeof(float);
   if (lds + stage <= 65536)
-    hipLaunchKernelGGL((k_modal_bwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r, u, P,
-                       st);
+    LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r, u, P,
+                 st);
   else
-    hipLaunchKernelGGL((k_modal_bwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P, st);
+    LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P, st);
 }
 
 // the matrix-core kernels take at most 128 interior layers (mp / 16 <= 8);
 // above, the vector units
 static void launch_transform(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
-                             const SolverState *st, bool fwd) {
+                             const SolverState *st, bool fwd, hipEvent_t *ev) {
   switch (M.mt) {
-    case 1: return launch_transforms_t<1>(g, s, M, r, u, P, st, fwd);
-    case 2: return launch_transforms_t<2>(g, s, M, r, u, P, st, fwd);
-    case 3: return launch_transforms_t<3>(g, s, M, r, u, P, st, fwd);
-    case 4: return launch_transforms_t<4>(g, s, M, r, u, P, st, fwd);
-    case 5: return launch_transforms_t<5>(g, s, M, r, u, P, st, fwd);
-    case 6: return launch_transforms_t<6>(g, s, M, r, u, P, st, fwd);
-    case 7: return launch_transforms_t<7>(g, s, M, r, u, P, st, fwd);
-    case 8: return launch_transforms_t<8>(g, s, M, r, u, P, st, fwd);
+    case 1: return launch_transforms_t<1>(g, s, M, r, u, P, st, fwd, ev);
+    case 2: return launch_transforms_t<2>(g, s, M, r, u, P, st, fwd, ev);
+    case 3: return launch_transforms_t<3>(g, s, M, r, u, P, st, fwd, ev);
+    case 4: return launch_transforms_t<4>(g, s, M, r, u, P, st, fwd, ev);
+    case 5: return launch_transforms_t<5>(g, s, M, r, u, P, st, fwd, ev);
+    case 6: return launch_transforms_t<6>(g, s, M, r, u, P, st, fwd, ev);
+    case 7: return launch_transforms_t<7>(g, s, M, r, u, P, st, fwd, ev);
+    case 8: return launch_transforms_t<8>(g, s, M, r, u, P, st, fwd, ev);
     default: break;
   }
   const size_t lds = modal_lds(M);
   if (fwd)
-    hipLaunchKernelGGL(k_modal_fwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
+    LAUNCH_TIMED(ev, 0, k_modal_fwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, st);
   else
-    hipLaunchKernelGGL(k_modal_bwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, u, P, st);
+    LAUNCH_TIMED(ev, 0, k_modal_bwd_vec, dim3(M.agrid), dim3(NSX_BLOCK), lds, s, g, M, r, u, P, st);
 }
 
 int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
                        const SolverState *st, hipEvent_t *ev) {
   const dim3 blk(NSX_BLOCK);
-  if (ev) (void)hipEventRecord(ev[0], s);
-  launch_transform(g, s, M, r, u, part.a, st, true);
-  if (ev) (void)hipEventRecord(ev[1], s);
+  launch_transform(g, s, M, r, u, part.a, st, true, ev);
+  if (ev) (void)hipEventRecord(ev[2], s);
   const int m0 = M.m0, L = M.nlev;
   auto grid = [](size_t threads) { return dim3((unsigned)((threads + NSX_BLOCK - 1) / NSX_BLOCK)); };
   for (int l = 0; l + 1 < L; ++l)
@@ -2515,9 +2515,8 @@ int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const flo
       hipLaunchKernelGGL(k_mg_up<4>, grid((size_t)M.lev[l].n * 64), blk, 0, s, M.lev[l], M.mu, m0, M.x[l + 1],
                          M.b[l], M.z[l], out, os, st);
   }
-  if (ev) (void)hipEventRecord(ev[2], s);
-  launch_transform(g, s, M, r, u, part.a, st, false);
   if (ev) (void)hipEventRecord(ev[3], s);
+  launch_transform(g, s, M, r, u, part.a, st, false, ev ? ev + 4 : nullptr);
   return (int)hipGetLastError();
 }
 
@@ -2652,15 +2651,15 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_modal_fold(Geo g, double *x, floa
 }
 
 int launch_modal_spmv(const Geo &g, hipStream_t s, const ModalDev &M, const float *u, const float *pin, float *pout,
-                      float *w, Partials part, SolverState *st, int first) {
-  hipLaunchKernelGGL(k_modal_spmv, dim3(g.grid), dim3(NSX_BLOCK), cgs_lds(g), s, g, u, pin, pout, w, part, st,
-                     first, (uint32_t)M.agrid);
+                      float *w, Partials part, SolverState *st, int first, hipEvent_t *ev) {
+  LAUNCH_TIMED(ev, 0, k_modal_spmv, dim3(g.grid), dim3(NSX_BLOCK), cgs_lds(g), s, g, u, pin, pout, w, part, st,
+               first, (uint32_t)M.agrid);
   return (int)hipGetLastError();
 }
 
 int launch_modal_update(const Geo &g, hipStream_t s, const float *p, const float *w, float *e, float *r,
-                        Partials part, SolverState *st) {
-  hipLaunchKernelGGL(k_modal_update, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, p, w, e, r, part, st);
+                        Partials part, SolverState *st, hipEvent_t *ev) {
+  LAUNCH_TIMED(ev, 0, k_modal_update, dim3(g.grid), dim3(NSX_BLOCK), 0, s, g, p, w, e, r, part, st);
   return (int)hipGetLastError();
 }
 

@@ -1234,21 +1234,21 @@ static int solve_modal(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
     for (;;) {
       const size_t pend0 = ctx->pending.size();
       for (int c = 0; c < cycle; ++c) {
-        // with timing: fwd, V-cycle, bwd, product, update between six events
+        // with timing: start/stop pairs of the forward transform, the
+        // V-cycle, the backward transform, the product and the update
         hipEvent_t *ev = nullptr;
         if (ctx->timing) {
-          ev = events(ctx, 6);
+          ev = events(ctx, 10);
           if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
         }
         LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev, ev));
         LAUNCH(launch_modal_spmv(g, s, ctx->modal, u, p[launched & 1], p[(launched + 1) & 1], w, ctx->part,
-                                 ctx->st_dev, c == 0));
-        if (ev) HIPCHECK(hipEventRecord(ev[4], s));
-        LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev));
+                                 ctx->st_dev, c == 0, ev ? ev + 6 : nullptr));
+        LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev,
+                                   ev ? ev + 8 : nullptr));
         if (ev) {
-          HIPCHECK(hipEventRecord(ev[5], s));
           const size_t base = (size_t)(ev - &ctx->evpool[0]);
-          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + q, 16 + q});
+          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, 16 + q});
         }
         ++launched;
       }

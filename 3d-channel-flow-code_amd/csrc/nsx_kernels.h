@@ -197,8 +197,9 @@ int launch_gnorm_sum(const Geo &g, hipStream_t s, const double *x, Partials part
 // u = M^-1 r (modal transform, the low modes' V-cycle, the others' diagonal,
 // transform back; u is stored at every slot, zero outside the interior) and
 // the partial sums of <r, u> into P.a
-// (ev: null, or four events recorded before the forward transform, after
-// it, after the V-cycle and after the backward transform)
+// (ev: null, or six events: the forward transform's start and stop (recorded
+// by its dispatch), the V-cycle's start and stop (recorded on the stream
+// around its launches), the backward transform's start and stop)
 int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, Partials part,
                        const SolverState *st, hipEvent_t *ev = nullptr);
 // p_out = u + beta p_in (beta = 0 when first: the cycle's first iteration) at
@@ -206,10 +207,10 @@ int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const flo
 // (M.agrid partials) kept in the state; a rho below NSX_MODAL_DROP times the
 // cycle's first ends the cycle (st->m_stop)
 int launch_modal_spmv(const Geo &g, hipStream_t s, const ModalDev &M, const float *u, const float *pin, float *pout,
-                      float *w, Partials part, SolverState *st, int first);
+                      float *w, Partials part, SolverState *st, int first, hipEvent_t *ev = nullptr);
 // alpha = rho / <p, w>; e += alpha p, r -= alpha w
 int launch_modal_update(const Geo &g, hipStream_t s, const float *p, const float *w, float *e, float *r,
-                        Partials part, SolverState *st);
+                        Partials part, SolverState *st, hipEvent_t *ev = nullptr);
 // the true FP64 residual of x + e into r and its partials (P.c); then x += e,
 // e = 0, a new cycle (m_stop = 0) and the reference's stopping rule (st: e1,
 // e2, xvol, done, exit code; the iteration cap after st->m_iters iterations)
