@@ -2109,16 +2109,23 @@ __device__ inline void load_modes(const float *xh, int N0, int c, int gq, float4
 }
 
 // MT = mp / 16.  Two blocks of 4 waves a CU; a wave with more than one tile
-// loads its next tile's operands before it forms the current one.
-template <int MT>
+// loads its next tile's operands before it forms the current one.  STAGE:
+// the wave's 16 rows of xh (one contiguous stretch) are assembled in LDS and
+// leave as aligned float4s; the low modes' entries written there are
+// overwritten by the V-cycle's last prolongation.  The diagonal scaling
+// multiplies by v_rcp_f32 of d (a fixed diagonal: the preconditioner stays
+// symmetric).
+template <int MT, bool STAGE>
 __global__ __launch_bounds__(MODAL_TB, 1) void k_modal_fwd_mfma(Geo g, ModalDev M, const float *r,
                                                                  const SolverState *st) {
   if (st->m_stop) return;
   extern __shared__ __align__(16) unsigned char nsx_dyn_lds[];
   float4 *tab = reinterpret_cast<float4 *>(nsx_dyn_lds);
   const int m0 = M.m0, nzi = M.nzi;
+  constexpr int MP = 16 * MT;
   lds_table<MT>(tab, M.tfwd);
   const int lane = threadIdx.x & 63, j = lane & 15, gq = lane >> 4;
+  float *stg = reinterpret_cast<float *>(tab + 64 * MT * MT) + (size_t)(threadIdx.x >> 6) * 16 * MP;
   const int ntile = (g.N0 + 15) >> 4, stride = gridDim.x * (MODAL_TB / 64);
   int tile = blockIdx.x * (MODAL_TB / 64) + (threadIdx.x >> 6);
   float4 a[MT], an[MT];
@@ -2151,15 +2158,23 @@ __global__ __launch_bounds__(MODAL_TB, 1) void k_modal_fwd_mfma(Geo g, ModalDev 
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int c = c0 + 4 * gq + q;
+        const float v = acc[mt][q], x = v * __builtin_amdgcn_rcpf(dg[q] + mum * ar[q]);
+        if (STAGE) stg[(4 * gq + q) * MP + m] = x;
         if (c >= g.N0) continue;
-        const float v = acc[mt][q], d = dg[q] + mum * ar[q];
         if (m < m0) {
           M.b[0][(size_t)c * m0 + m] = v;
-          M.z[0][(size_t)c * m0 + m] = v / d;
-        } else {
-          M.xh[(size_t)c * (16 * MT) + m] = v / d;
+          M.z[0][(size_t)c * m0 + m] = x;
+        } else if (!STAGE) {
+          M.xh[(size_t)c * MP + m] = x;
         }
       }
+    }
+    if (STAGE) {
+      wave_lds_sync();
+      const int nf = min(16, g.N0 - c0) * MP;
+      float *xt = M.xh + (size_t)c0 * MP;
+      for (int o = 4 * lane; o < nf; o += 256) *(float4 *)(xt + o) = *(const float4 *)(stg + o);
+      wave_lds_sync();
     }
 #pragma unroll
     for (int t = 0; t < MT; ++t) a[t] = an[t];
@@ -2458,7 +2473,12 @@ static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, 
                                 const SolverState *st, bool fwd, hipEvent_t *ev) {
   const size_t lds = (size_t)MT * MT * 64 * sizeof(float4);
   if (fwd) {
-    LAUNCH_TIMED(ev, 0, k_modal_fwd_mfma<MT>, dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
+    // the staged forward transform where table and staging fit 64 KB of LDS
+    const size_t fstage = (size_t)(MODAL_TB / 64) * 16 * (16 * MT) * sizeof(float);
+    if (lds + fstage <= 65536)
+      LAUNCH_TIMED(ev, 0, (k_modal_fwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + fstage, s, g, M, r, st);
+    else
+      LAUNCH_TIMED(ev, 0, (k_modal_fwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
     return;
   }
   // the staged backward transform where table and staging fit 64 KB of LDS
