@@ -148,7 +148,8 @@ def modal_kernels(case) -> dict:
     mt = (nzi + 15) // 16
     m0 = min(16, nzi)
     n = case.n_interior
-    fwd = "k_modal_fwd_mfma<%d>" % mt if mt <= 8 else "k_modal_fwd_vec"
+    fstaged = mt * mt * 1024 + 4 * 16 * 16 * mt * 4 <= 65536
+    fwd = ("k_modal_fwd_mfma<%d, %s>" % (mt, "true" if fstaged else "false")) if mt <= 8 else "k_modal_fwd_vec"
     ns = (case.NZ + 3) // 4 * 4
     staged = mt * mt * 1024 + 4 * 16 * ns * 4 <= 65536
     bwd = ("k_modal_bwd_mfma<%d, %s>" % (mt, "true" if staged else "false")) if mt <= 8 else "k_modal_bwd_vec"
