@@ -2387,7 +2387,7 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_modal_bwd_vec(Geo g, ModalDev M, 
 // neighbours: one launch per level each way.  Rows with many entries are
 // split over S slices of lanes (lane m + 16 s), summed by shuffles in a fixed
 // order.
-constexpr float MG_OMEGA = 0.8f;
+constexpr float MG_OMEGA = NSX_MODAL_OMEGA;
 
 template <int S>
 __device__ inline float slice_sum(float v) {
@@ -2398,7 +2398,8 @@ __device__ inline float slice_sum(float v) {
 
 // b_c[J] = sum over the children i of J of (b - A x)_i with x = omega z:
 // (1 - omega) sum_i b_i + omega sum_j RW[J][j] z_j (RW = R W, the weights of
-// the children summed per fine neighbour); z_c = b_c / d_J.  A wave per J.
+// the children summed per fine neighbour), one CSR over the fine nodes with
+// a coefficient on b and one on z; z_c = b_c / d_J.  A wave per J.
 __global__ __launch_bounds__(NSX_BLOCK) void k_mg_down(ModalLevelDev C, const float *mu, int m0, const float *bf,
                                                         const float *zf, float *bc, float *zc, const SolverState *st) {
   if (st->m_stop) return;
@@ -2407,10 +2408,10 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_mg_down(ModalLevelDev C, const fl
   if (J >= C.n) return;  // whole waves
   float v = 0.f;
   if (m < m0) {
-    for (int q = C.cptr[J] + sl; q < C.cptr[J + 1]; q += 4) v += bf[(size_t)C.child[q] * m0 + m];
-    float a = 0.f;
-    for (int e = C.rptr[J] + sl; e < C.rptr[J + 1]; e += 4) a = fmaf(C.rw[e], zf[(size_t)C.rcol[e] * m0 + m], a);
-    v = fmaf(MG_OMEGA, a, (1.f - MG_OMEGA) * v);
+    for (int e = C.dptr[J] + sl; e < C.dptr[J + 1]; e += 4) {
+      const size_t q = (size_t)C.dcol[e] * m0 + m;
+      v = fmaf(C.db[e], bf[q], fmaf(C.dz[e], zf[q], v));
+    }
   }
   v = slice_sum<4>(v);
   if (sl == 0 && m < m0) {

@@ -1192,8 +1192,8 @@ static int modal_setup(nsx_ctx *ctx) {
     };
     if ((rc = up_i(h.ptr, d.ptr)) || (rc = up_i(h.col, d.col)) || (rc = up_f(h.w, d.w)) ||
         (rc = up_f(h.diag, d.diag)) || (rc = up_f(h.area, d.area)) || (rc = up_i(h.agg, d.agg)) ||
-        (rc = up_i(h.eagg, d.eagg)) || (rc = up_i(h.cptr, d.cptr)) || (rc = up_i(h.child, d.child)) ||
-        (rc = up_i(h.rptr, d.rptr)) || (rc = up_i(h.rcol, d.rcol)) || (rc = up_f(h.rw, d.rw)))
+        (rc = up_i(h.eagg, d.eagg)) || (rc = up_i(h.dptr, d.dptr)) || (rc = up_i(h.dcol, d.dcol)) ||
+        (rc = up_f(h.db, d.db)) || (rc = up_f(h.dz, d.dz)))
       return rc;
     const size_t nm = (size_t)h.n * (size_t)M.m0;
     if ((rc = dev_alloc(ctx, &M.b[l], nm)) || (rc = dev_alloc(ctx, &M.z[l], nm)) ||

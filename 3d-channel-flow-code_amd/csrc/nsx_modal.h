@@ -18,6 +18,8 @@
 #define NSX_MODAL_DROP 1e-10
 #define NSX_MODAL_FIRST 30
 #define NSX_MODAL_CYCLE 60
+// the damping of the V-cycle's Jacobi sweeps
+#define NSX_MODAL_OMEGA 0.8f
 
 // host tables
 struct ModalHost {
@@ -30,11 +32,12 @@ struct ModalHost {
     std::vector<float> w, diag, area;
     std::vector<int> agg;        // coarse node of each node (all levels but the coarsest)
     std::vector<int> eagg;       // agg of each edge's far node
-    // all but level 0: the children of each node in the level below, and the
-    // restricted weights RW[J][j] = sum of w_ij over the children i of J (CSR
-    // over the fine neighbours j)
-    std::vector<int> cptr, child, rptr, rcol;
-    std::vector<float> rw;
+    // all but level 0: the restriction of the pre-smoothed residual, one CSR
+    // over the level below: row J holds its children i (db = 1 - omega on b_i)
+    // and the fine neighbours j of its children (dz = omega x the sum of w_ij
+    // over those children, on z_j), merged per fine node
+    std::vector<int> dptr, dcol;
+    std::vector<float> db, dz;
   };
   std::vector<Level> levels;  // level 0: the N0 columns
   std::vector<float> cinv;    // m0 dense nc x nc inverses of the coarsest level
@@ -52,8 +55,8 @@ int modal_build_host(int N0, int nzi, const int *nbr, const double *ppec, const 
 // device view (pointers into device memory)
 struct ModalLevelDev {
   int n;
-  const int *ptr, *col, *agg, *eagg, *cptr, *child, *rptr, *rcol;
-  const float *w, *rw, *diag, *area;
+  const int *ptr, *col, *agg, *eagg, *dptr, *dcol;
+  const float *w, *db, *dz, *diag, *area;
 };
 
 struct ModalDev {

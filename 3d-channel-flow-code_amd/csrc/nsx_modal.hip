@@ -249,33 +249,33 @@ int modal_build_host(int N0, int nzi, const int *nbr, const double *ppec, const 
       // children of each coarse node, for the restriction
       ModalHost::Level &c = H.levels[l + 1];
       const int nc = lev[l + 1].n;
-      c.cptr.assign(nc + 1, 0);
-      for (int i = 0; i < G.n; ++i) c.cptr[aggs[l][i] + 1]++;
-      for (int J = 0; J < nc; ++J) c.cptr[J + 1] += c.cptr[J];
-      c.child.assign(G.n, 0);
-      std::vector<int> fill(c.cptr.begin(), c.cptr.end() - 1);
-      for (int i = 0; i < G.n; ++i) c.child[fill[aggs[l][i]]++] = i;
-      // RW: the children's weights merged per fine neighbour
-      c.rptr.assign(nc + 1, 0);
-      std::vector<std::pair<int, double>> row;
+      std::vector<std::vector<int>> kids(nc);
+      for (int i = 0; i < G.n; ++i) kids[aggs[l][i]].push_back(i);
+      c.dptr.assign(nc + 1, 0);
+      std::vector<std::pair<int, std::pair<double, double>>> row;
       for (int J = 0; J < nc; ++J) {
         row.clear();
-        for (int q = c.cptr[J]; q < c.cptr[J + 1]; ++q) {
-          const int i = c.child[q];
-          for (int e = G.ptr[i]; e < G.ptr[i + 1]; ++e) row.emplace_back(G.col[e], G.w[e]);
+        for (int i : kids[J]) {
+          row.push_back({i, {1.0 - (double)NSX_MODAL_OMEGA, 0.0}});
+          for (int e = G.ptr[i]; e < G.ptr[i + 1]; ++e)
+            row.push_back({G.col[e], {0.0, (double)NSX_MODAL_OMEGA * G.w[e]}});
         }
-        std::stable_sort(row.begin(), row.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) {
-          return a.first < b.first;
-        });
+        std::stable_sort(row.begin(), row.end(),
+                         [](const std::pair<int, std::pair<double, double>> &a,
+                            const std::pair<int, std::pair<double, double>> &b) { return a.first < b.first; });
         for (size_t q = 0; q < row.size();) {
-          double w = 0.0;
+          double vb = 0.0, vz = 0.0;
           size_t r = q;
-          for (; r < row.size() && row[r].first == row[q].first; ++r) w += row[r].second;
-          c.rcol.push_back(row[q].first);
-          c.rw.push_back((float)w);
+          for (; r < row.size() && row[r].first == row[q].first; ++r) {
+            vb += row[r].second.first;
+            vz += row[r].second.second;
+          }
+          c.dcol.push_back(row[q].first);
+          c.db.push_back((float)vb);
+          c.dz.push_back((float)vz);
           q = r;
         }
-        c.rptr[J + 1] = (int)c.rcol.size();
+        c.dptr[J + 1] = (int)c.dcol.size();
       }
     }
   }
