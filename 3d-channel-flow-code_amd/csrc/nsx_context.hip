@@ -1229,37 +1229,28 @@ static int solve_modal(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
   float *p[2] = {ctx->fp, ctx->fp2};
   HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
-  int launched = 0, chunk = NSX_MODAL_FIRST, before = 0;
+  int launched = 0, cycle = NSX_MODAL_FIRST, before = 0;
   if (!ctx->st_host->done) {
     for (;;) {
       const size_t pend0 = ctx->pending.size();
-      // a cycle runs in chunks until its drop test has ended it (or it has
-      // NSX_MODAL_CYCLE iterations): ending it on a count would restart CG
-      // while its recursion still holds
-      for (int c = 0;;) {
-        for (int q = 0; q < chunk; ++q, ++c) {
-          // with timing: start/stop pairs of the forward transform, the
-          // V-cycle, the backward transform, the product and the update
-          hipEvent_t *ev = nullptr;
-          if (ctx->timing) {
-            ev = events(ctx, 10);
-            if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
-          }
-          LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev, ev));
-          LAUNCH(launch_modal_spmv(g, s, ctx->modal, u, p[launched & 1], p[(launched + 1) & 1], w, ctx->part,
-                                   ctx->st_dev, c == 0, ev ? ev + 6 : nullptr));
-          LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev,
-                                     ev ? ev + 8 : nullptr));
-          if (ev) {
-            const size_t base = (size_t)(ev - &ctx->evpool[0]);
-            for (int k = 0; k < 5; ++k) ctx->pending.push_back({base + 2 * k, 16 + k});
-          }
-          ++launched;
+      for (int c = 0; c < cycle; ++c) {
+        // with timing: start/stop pairs of the forward transform, the
+        // V-cycle, the backward transform, the product and the update
+        hipEvent_t *ev = nullptr;
+        if (ctx->timing) {
+          ev = events(ctx, 10);
+          if (!ev) return nsx_fail(ctx, -2, "hipEventCreate failed");
         }
-        HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        if (ctx->st_host->m_stop || c >= NSX_MODAL_CYCLE) break;
-        chunk = NSX_MODAL_MORE;
+        LAUNCH(launch_modal_apply(g, s, ctx->modal, r, u, ctx->part, ctx->st_dev, ev));
+        LAUNCH(launch_modal_spmv(g, s, ctx->modal, u, p[launched & 1], p[(launched + 1) & 1], w, ctx->part,
+                                 ctx->st_dev, c == 0, ev ? ev + 6 : nullptr));
+        LAUNCH(launch_modal_update(g, s, p[(launched + 1) & 1], w, e, r, ctx->part, ctx->st_dev,
+                                   ev ? ev + 8 : nullptr));
+        if (ev) {
+          const size_t base = (size_t)(ev - &ctx->evpool[0]);
+          for (int q = 0; q < 5; ++q) ctx->pending.push_back({base + 2 * q, 16 + q});
+        }
+        ++launched;
       }
       LAUNCH(launch_modal_true(g, s, rhs, x, e, r, ctx->part, ctx->st_dev, cap));
       HIPCHECK(hipMemcpyAsync(ctx->st_host, ctx->st_dev, sizeof(SolverState), hipMemcpyDeviceToHost, s));
@@ -1269,8 +1260,8 @@ static int solve_modal(nsx_ctx *ctx, const double *rhs, const double *phi_in, do
       // timing: only the iterations that ran (after a drop the launches return at once)
       if (ctx->timing && ctx->pending.size() > pend0 + 5 * (size_t)used) ctx->pending.resize(pend0 + 5 * (size_t)used);
       if (ctx->st_host->done) break;
-      // the next cycle's first chunk: about as long as this cycle ran
-      chunk = std::min(NSX_MODAL_CYCLE, std::max(8, used + 2));
+      // the next cycle: as long as this one ran, a few more
+      cycle = std::min(NSX_MODAL_CYCLE, std::max(8, used + 4));
     }
   }
   collect_timing(ctx);

@@ -12,14 +12,12 @@
 #define NSX_MODAL_M0 16
 // A cycle of the FP32 CG starts from a true FP64 residual (beta = 0) and ends
 // when rho = <r, M^-1 r> has fallen below NSX_MODAL_DROP times its first
-// value (about 1e-5 in the residual, what FP32 recursion holds to), or at
-// NSX_MODAL_CYCLE iterations.  The host launches it in chunks: NSX_MODAL_FIRST
-// iterations for the first cycle, then the last cycle's count + 2 (at least
-// 8), and NSX_MODAL_MORE more while the drop has not come
+// value (about 1e-5 in the residual, what FP32 recursion holds to), or after
+// the iterations the host launched for it: NSX_MODAL_FIRST for the first
+// cycle, then the last cycle's count + 4 within [8, NSX_MODAL_CYCLE]
 #define NSX_MODAL_DROP 1e-10
 #define NSX_MODAL_FIRST 30
-#define NSX_MODAL_MORE 8
-#define NSX_MODAL_CYCLE 400
+#define NSX_MODAL_CYCLE 60
 // the damping of the V-cycle's Jacobi sweeps
 #define NSX_MODAL_OMEGA 0.8f
 
