@@ -2469,22 +2469,38 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_mg_up(ModalLevelDev F, const floa
 }
 
 // (ev: null, or a start/stop pair the dispatch records)
+// staged kernels past 64 KB of LDS: the kernel's limit raised once (false
+// when the device does not allow it)
+static bool lds_allow(const void *kern, size_t bytes, int optin) {
+  if (bytes <= 65536) return true;
+  if (bytes > (size_t)optin) return false;
+  return hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess;
+}
+
 template <int MT>
 static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, const float *r, float *u, double *P,
                                 const SolverState *st, bool fwd, hipEvent_t *ev) {
   const size_t lds = (size_t)MT * MT * 64 * sizeof(float4);
   if (fwd) {
-    // the staged forward transform where table and staging fit 64 KB of LDS
+    // the staged forward transform where table and staging fit the LDS a
+    // block may have
     const size_t fstage = (size_t)(MODAL_TB / 64) * 16 * (16 * MT) * sizeof(float);
-    if (lds + fstage <= 65536)
+    static const bool fok =
+        lds_allow(reinterpret_cast<const void *>(&k_modal_fwd_mfma<MT, true>), lds + fstage, M.lds_optin);
+    if (fok)
       LAUNCH_TIMED(ev, 0, (k_modal_fwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + fstage, s, g, M, r, st);
     else
       LAUNCH_TIMED(ev, 0, (k_modal_fwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, st);
     return;
   }
-  // the staged backward transform where table and staging fit 64 KB of LDS
+  // the staged backward transform where table and staging fit the LDS a
+  // block may have
   const size_t stage = (size_t)(MODAL_TB / 64) * 16 * g.NS * sizeof(float);
-  if (lds + stage <= 65536)
+  static size_t allowed = 65536;  // the staged kernel's LDS limit as set so far
+  if (lds + stage > allowed &&
+      lds_allow(reinterpret_cast<const void *>(&k_modal_bwd_mfma<MT, true>), lds + stage, M.lds_optin))
+    allowed = lds + stage;
+  if (lds + stage <= allowed)
     LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r, u, P,
                  st);
   else

@@ -1127,6 +1127,7 @@ static int modal_setup(nsx_ctx *ctx) {
     HIPCHECK(hipGetDeviceProperties(&prop, ctx->dev));
     const int waves = (N0 + 15) / 16, want = (waves + 3) / 4;
     M.agrid = std::max(1, std::min(want, 2 * prop.multiProcessorCount));
+    M.lds_optin = (int)std::max(prop.sharedMemPerBlockOptin, (size_t)65536);
   }
   std::vector<float> ph((size_t)nzp * nzp, 0.f), pt((size_t)nzp * nzp, 0.f), mu((size_t)MP, 1.f);
   for (int k = 0; k < nzi; ++k)

@@ -64,6 +64,7 @@ struct ModalDev {
   int mp;     // modes padded to a multiple of 16 (the row length of xh)
   int mt;     // mp / 16; above 8 the vector-unit transforms
   int agrid;  // blocks of the transform kernels (the partial rows of <r, u>)
+  int lds_optin;  // the most dynamic LDS a block may opt into (sharedMemPerBlockOptin)
   int nzp;    // nzi rounded up to a multiple of 4 (the vector-unit tables)
   const float *phi;   // nzp x nzp, [k][m]
   const float *phiT;  // nzp x nzp, [m][k]

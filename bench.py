@@ -148,10 +148,13 @@ def modal_kernels(case) -> dict:
     mt = (nzi + 15) // 16
     m0 = min(16, nzi)
     n = case.n_interior
-    fstaged = mt * mt * 1024 + 4 * 16 * 16 * mt * 4 <= 65536
+    # the staged transforms where table and staging fit the LDS a block may
+    # opt into (160 KB on gfx950)
+    lds_optin = 160 * 1024
+    fstaged = mt * mt * 1024 + 4 * 16 * 16 * mt * 4 <= lds_optin
     fwd = ("k_modal_fwd_mfma<%d, %s>" % (mt, "true" if fstaged else "false")) if mt <= 8 else "k_modal_fwd_vec"
     ns = (case.NZ + 3) // 4 * 4
-    staged = mt * mt * 1024 + 4 * 16 * ns * 4 <= 65536
+    staged = mt * mt * 1024 + 4 * 16 * ns * 4 <= lds_optin
     bwd = ("k_modal_bwd_mfma<%d, %s>" % (mt, "true" if staged else "false")) if mt <= 8 else "k_modal_bwd_vec"
     return {"modal_fwd": (fwd, 4 * (2 * n + n0 * m0)), "modal_bwd": (bwd, 4 * 3 * n),
             "modal_spmv": ("k_modal_spmv", 16 * n), "modal_update": ("k_modal_update", 24 * n),
