@@ -149,7 +149,8 @@ def modal_kernels(case) -> dict:
     m0 = min(16, nzi)
     n = case.n_interior
     # the staged transforms where table and staging fit the LDS a block may
-    # opt into (160 KB on gfx950)
+    # opt into (the MI355X allowed the 98 KB of the 128-layer workload; 160 KB
+    # is the CU's LDS)
     lds_optin = 160 * 1024
     fstaged = mt * mt * 1024 + 4 * 16 * 16 * mt * 4 <= lds_optin
     fwd = ("k_modal_fwd_mfma<%d, %s>" % (mt, "true" if fstaged else "false")) if mt <= 8 else "k_modal_fwd_vec"
