@@ -13,6 +13,10 @@ are scaled by their diagonal.
   * a 24x16 quads x 140 layers channel, whose columns hold more slots than
     the matrix-core transforms take (128): the vector-unit transforms, same
     checks against the FP64 BiCGSTAB;
+  * an 8x6 quads x 6 layers channel with a bed: fewer layers than the
+    multigrid's 16 modes (all 6 go through it), a single level (the dense
+    coarse solve on the 96 columns), one mode tile of the transforms; a
+    stage of nsx_hydro in the precision form against the FP64 path;
   * the 64x32 quads x 48 layers channel: a random right-hand side solved to
     eps 1e-13 in a fraction of the block-Jacobi instance's iterations, and
     four RK2 steps of the precision form at eps 1e-13 within 1e-8 of the
@@ -79,6 +83,31 @@ def test_modal_solve_tall_columns():
     assert min(e1, e2) < 2 * c.params.eps, (e1, e2)
     assert rel_err(interior(c, phim), interior(c, phi64)) <= 1e-6
     assert rel_err(pvm, pv64) <= 1e-6
+
+
+def test_modal_tiny_mesh_single_level():
+    from nsx.device import Context
+    c = make_case(8, 6, 6, seed=3, jitter=0.1, bed=True)
+    rhs, _, (phi64, pv64, _), (phim, pvm, im) = _solve_pair(c, 9)
+    assert im["exit"] == "converged" and im["iters_fp32"] > 0, im
+    e1, e2 = ppe_residual(c, phim, rhs)
+    assert min(e1, e2) < 2 * c.params.eps, (e1, e2)
+    assert rel_err(interior(c, phim), interior(c, phi64)) <= 1e-6
+    out = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        G = Context(c)
+        G.set_params(eps=1e-13)
+        G.set_precision(prec, REFINE_MODAL, 0.1, 30)
+        G.init_state()
+        for a, b in (("UF", "UFN"), ("VF", "VFN"), ("WF", "WFN"), ("PF", "PFN")):
+            G.copy(a, b)
+        info = G.hydro(0)
+        assert info["exit"] == "converged", (prec, info)
+        out[prec] = {f: interior(c, G.get(f)) for f in ("UFNP", "VFNP", "WFNP", "PFNP")}
+        G.close()
+    for f in ("UFNP", "VFNP", "WFNP", "PFNP"):
+        e = rel_err(out[PREC_MIXED][f], out[PREC_FP64][f])
+        assert e <= 1e-8, (f, e)
 
 
 def test_modal_iterations_and_steps_64x32x48():
