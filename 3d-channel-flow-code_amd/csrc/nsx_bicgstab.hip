@@ -2108,6 +2108,44 @@ __device__ inline void load_modes(const float *xh, int N0, int c, int gq, float4
   }
 }
 
+// the same with the 16 multigrid modes formed here: the V-cycle's last step,
+// the level-0 post-smoothing x = y + omega (b - A y) / d with y = omega z +
+// x_1(agg) at the column and its neighbours (the arithmetic of k_mg_up<1>),
+// for modes 4 g .. +3; the other modes from xh
+__device__ inline float4 ld4f(const float *p) { return *(const float4 *)p; }
+
+template <int MT>
+__device__ inline void load_modes_v0(const ModalDev &M, int N0, int c, int gq, float4 (&a)[MT]) {
+  load_modes<MT>(M.xh, N0, c, gq, a);
+  if (c >= N0) return;
+  const ModalLevelDev &F = M.lev[0];
+  const float om = NSX_MODAL_OMEGA;
+  const float *z = M.z[0], *b = M.b[0], *x1 = M.x[1];
+  const int m = 4 * gq;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int e = F.ptr[c]; e < F.ptr[c + 1]; ++e) {
+    const float4 zj = ld4f(z + (size_t)F.col[e] * 16 + m), xj = ld4f(x1 + (size_t)F.eagg[e] * 16 + m);
+    const float w = F.w[e];
+    acc[0] = fmaf(w, fmaf(om, zj.x, xj.x), acc[0]);
+    acc[1] = fmaf(w, fmaf(om, zj.y, xj.y), acc[1]);
+    acc[2] = fmaf(w, fmaf(om, zj.z, xj.z), acc[2]);
+    acc[3] = fmaf(w, fmaf(om, zj.w, xj.w), acc[3]);
+  }
+  const float4 zc = ld4f(z + (size_t)c * 16 + m), bc = ld4f(b + (size_t)c * 16 + m),
+               xc = ld4f(x1 + (size_t)F.agg[c] * 16 + m);
+  const float zv[4] = {zc.x, zc.y, zc.z, zc.w}, bv[4] = {bc.x, bc.y, bc.z, bc.w}, xv[4] = {xc.x, xc.y, xc.z, xc.w};
+  const float dgc = F.diag[c], arc = F.area[c];
+  float o[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float di = dgc + M.mu[m + k] * arc;
+    const float yi = fmaf(om, zv[k], xv[k]);
+    const float ay = fmaf(di, yi, -acc[k]);
+    o[k] = yi + om * (bv[k] - ay) / di;
+  }
+  a[0] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
 // MT = mp / 16.  Two blocks of 4 waves a CU; a wave with more than one tile
 // loads its next tile's operands before it forms the current one.  STAGE:
 // the wave's 16 rows of xh (one contiguous stretch) are assembled in LDS and
@@ -2186,7 +2224,7 @@ __global__ __launch_bounds__(MODAL_TB, 1) void k_modal_fwd_mfma(Geo g, ModalDev 
 // columns (one contiguous stretch of u) are assembled in LDS and leave, with
 // r read beside them for <r, u>, as aligned float4s (else each lane stores
 // its outputs where they go, in 64-byte runs that straddle two segments).
-template <int MT, bool STAGE>
+template <int MT, bool STAGE, bool V0>
 __global__ __launch_bounds__(MODAL_TB, 1) void k_modal_bwd_mfma(Geo g, ModalDev M, const float *r, float *u,
                                                                  double *P, const SolverState *st) {
   if (st->m_stop) return;
@@ -2200,9 +2238,17 @@ __global__ __launch_bounds__(MODAL_TB, 1) void k_modal_bwd_mfma(Geo g, ModalDev 
   int tile = blockIdx.x * (MODAL_TB / 64) + (threadIdx.x >> 6);
   double acc2[1] = {0.0};
   float4 a[MT], an[MT];
-  load_modes<MT>(M.xh, g.N0, tile * 16 + j, gq, a);
+  if (V0)
+    load_modes_v0<MT>(M, g.N0, tile * 16 + j, gq, a);
+  else
+    load_modes<MT>(M.xh, g.N0, tile * 16 + j, gq, a);
   for (; tile < ntile; tile += stride) {
-    if (tile + stride < ntile) load_modes<MT>(M.xh, g.N0, (tile + stride) * 16 + j, gq, an);
+    if (tile + stride < ntile) {
+      if (V0)
+        load_modes_v0<MT>(M, g.N0, (tile + stride) * 16 + j, gq, an);
+      else
+        load_modes<MT>(M.xh, g.N0, (tile + stride) * 16 + j, gq, an);
+    }
     const int c0 = tile * 16;
     f32x4 acc[MT];
 #pragma unroll
@@ -2469,6 +2515,11 @@ __global__ __launch_bounds__(NSX_BLOCK) void k_mg_up(ModalLevelDev F, const floa
 }
 
 // (ev: null, or a start/stop pair the dispatch records)
+// the backward transform forms the V-cycle's level-0 post-smoothing where it
+// runs on the matrix cores with 16 multigrid modes and the V-cycle has a
+// level below the columns
+static bool modal_fused_v0(const ModalDev &M) { return M.mt <= 8 && M.m0 == 16 && M.nlev >= 2; }
+
 // staged kernels past 64 KB of LDS: the kernel's limit raised once (false
 // when the device does not allow it)
 static bool lds_allow(const void *kern, size_t bytes, int optin) {
@@ -2494,17 +2545,30 @@ static void launch_transforms_t(const Geo &g, hipStream_t s, const ModalDev &M, 
     return;
   }
   // the staged backward transform where table and staging fit the LDS a
-  // block may have
+  // block may have; it forms the level-0 post-smoothing itself where the
+  // V-cycle left it out (modal_fused_v0)
   const size_t stage = (size_t)(MODAL_TB / 64) * 16 * g.NS * sizeof(float);
-  static size_t allowed = 65536;  // the staged kernel's LDS limit as set so far
+  const bool v0 = modal_fused_v0(M);
+  static size_t allowed = 65536;  // the staged kernels' LDS limit as set so far
   if (lds + stage > allowed &&
-      lds_allow(reinterpret_cast<const void *>(&k_modal_bwd_mfma<MT, true>), lds + stage, M.lds_optin))
+      lds_allow(reinterpret_cast<const void *>(&k_modal_bwd_mfma<MT, true, false>), lds + stage, M.lds_optin) &&
+      lds_allow(reinterpret_cast<const void *>(&k_modal_bwd_mfma<MT, true, true>), lds + stage, M.lds_optin))
     allowed = lds + stage;
-  if (lds + stage <= allowed)
-    LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r, u, P,
-                 st);
-  else
-    LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P, st);
+  if (lds + stage <= allowed) {
+    if (v0)
+      LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, true, true>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r,
+                   u, P, st);
+    else
+      LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, true, false>), dim3(M.agrid), dim3(MODAL_TB), lds + stage, s, g, M, r,
+                   u, P, st);
+  } else {
+    if (v0)
+      LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, false, true>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P,
+                   st);
+    else
+      LAUNCH_TIMED(ev, 0, (k_modal_bwd_mfma<MT, false, false>), dim3(M.agrid), dim3(MODAL_TB), lds, s, g, M, r, u, P,
+                   st);
+  }
 }
 
 // the matrix-core kernels take at most 128 interior layers (mp / 16 <= 8);
@@ -2542,7 +2606,7 @@ int launch_modal_apply(const Geo &g, hipStream_t s, const ModalDev &M, const flo
   // the coarsest solve and the last prolongation write the low modes into xh
   hipLaunchKernelGGL(k_mg_coarse, dim3(M.nc), blk, 0, s, M.nc, m0, M.cinv, M.b[L - 1], L == 1 ? M.xh : M.x[L - 1],
                      L == 1 ? M.mp : m0, st);
-  for (int l = L - 2; l >= 0; --l) {
+  for (int l = L - 2; l >= (modal_fused_v0(M) ? 1 : 0); --l) {
     float *out = l == 0 ? M.xh : M.x[l];
     const int os = l == 0 ? M.mp : m0;
     if (l == 0)

@@ -156,7 +156,11 @@ def modal_kernels(case) -> dict:
     fwd = ("k_modal_fwd_mfma<%d, %s>" % (mt, "true" if fstaged else "false")) if mt <= 8 else "k_modal_fwd_vec"
     ns = (case.NZ + 3) // 4 * 4
     staged = mt * mt * 1024 + 4 * 16 * ns * 4 <= lds_optin
-    bwd = ("k_modal_bwd_mfma<%d, %s>" % (mt, "true" if staged else "false")) if mt <= 8 else "k_modal_bwd_vec"
+    # the backward transform forms the level-0 post-smoothing where there are
+    # 16 multigrid modes and more than one level (over 400 columns)
+    v0 = mt <= 8 and nzi >= 16 and n0 > 400
+    bwd = ("k_modal_bwd_mfma<%d, %s, %s>" % (mt, "true" if staged else "false", "true" if v0 else "false")
+           if mt <= 8 else "k_modal_bwd_vec")
     return {"modal_fwd": (fwd, 4 * (2 * n + n0 * m0)), "modal_bwd": (bwd, 4 * 3 * n),
             "modal_spmv": ("k_modal_spmv", 16 * n), "modal_update": ("k_modal_update", 24 * n),
             "modal_mg": ("V-cycle (k_mg_down, k_mg_coarse, k_mg_up)", 0)}
