@@ -141,8 +141,9 @@ CGS_KERNEL = {4: "k_cgs<float, 0>", 5: "k_cgs<float, 1>", 6: "k_cgs<float, 2>"}
 def modal_kernels(case) -> dict:
     """timer -> (kernel name, algorithmic bytes per launch) of the vertical-mode
     CG: the forward transform reads r and writes the modal coefficients (and
-    z = b / d of the 16 multigrid modes); the backward one reads them and r
-    and writes u; the product reads u and p and writes p and w; the update
+    z = b / d of the 16 multigrid modes); the backward one reads them (and b
+    and z of the multigrid modes where it forms the level-0 post-smoothing)
+    and r and writes u; the product reads u and p and writes p and w; the update
     reads p, w, e, r and writes e and r (FP32, interior prisms)"""
     n0, nzi = case.N_CELL0, case.NZ - 2
     mt = (nzi + 15) // 16
@@ -161,7 +162,10 @@ def modal_kernels(case) -> dict:
     v0 = mt <= 8 and nzi >= 16 and n0 > 400
     bwd = ("k_modal_bwd_mfma<%d, %s, %s>" % (mt, "true" if staged else "false", "true" if v0 else "false")
            if mt <= 8 else "k_modal_bwd_vec")
-    return {"modal_fwd": (fwd, 4 * (2 * n + n0 * m0)), "modal_bwd": (bwd, 4 * 3 * n),
+    # (the backward transform with the level-0 post-smoothing also reads b
+    # and z of the 16 multigrid modes: the level below's x is counted as
+    # cached)
+    return {"modal_fwd": (fwd, 4 * (2 * n + n0 * m0)), "modal_bwd": (bwd, 4 * (3 * n + (2 * n0 * m0 if v0 else 0))),
             "modal_spmv": ("k_modal_spmv", 16 * n), "modal_update": ("k_modal_update", 24 * n),
             "modal_mg": ("V-cycle (k_mg_down, k_mg_coarse, k_mg_up)", 0)}
 
